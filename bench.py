@@ -1,0 +1,134 @@
+"""Motion-search throughput on one MI355X: 1080p plain search, Mpx/s.
+
+Workload (one step): a 1920x1088 clip of 16 frames (15 P-frames, 1 reference =
+the previous frame) already resident in HBM; build the F/H/V/C plane sets of the
+15 references (one launch), exhaustive +-32 integer SAD search of every 16x16
+block (one launch), cascade ring refinement to quarter pel with 4x4-Hadamard
+SATD (one launch), lambda of QP 26.  Mpx/s = 1920*1088*15 / step time.
+
+Prints one JSON line (see the field notes in DESIGN.md).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W, H, NFRAMES, R, PAD, QP, SEED = 1920, 1088, 16, 32, 32, 26, 2026
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 157.3  # published FP32 vector peak = 256 CUs x 64 lanes x 2.4 GHz x 4 ops
+
+
+def cpu_baseline(frames, lam, budget_s=12.0):
+    """The oracle (reference-semantics C restatement, -O3 -march=x86-64-v3, OpenMP)
+    on the host cores, same work per P-frame, over as many P-frames as fit the budget."""
+    from oracle.oracle import Oracle
+    cores = min(os.cpu_count() or 1, 16)
+    o = Oracle(fast=True)
+    cfg = o.cfg(W, H, PAD, 16, 16, R, 0, lam)
+    t0 = time.perf_counter()
+    done = 0
+    while done < NFRAMES - 1:
+        pl = o.planes(frames[done], W, H, PAD, fused=True)
+        o.plain_search_frame(cfg, frames[done + 1], pl, nthreads=cores)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(W * H * done / dt / 1e6, 3), "unit": "Mpx/s", "cores": cores, "kind": "port",
+            "sample": f"{done} P-frame(s) of the 1080p clip: plane build + +-32 full search + qpel refine, "
+                      f"{cores} OpenMP threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the motion search on a single MI355X "
+              "(streams/GOPs shard across GPUs with no exchange; see DESIGN.md). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from rusty_h264_amd import accel, synth
+    frames = synth.hd_clip(NFRAMES, seed=SEED, w=W, h=H)
+    lam = accel.lambda_me(QP)
+    nf = NFRAMES - 1
+    with accel.Device(0) as dev:
+        dname = dev.name()
+        clip = dev.upload(frames)
+        cur = int(clip) + W * H  # frames 1..15
+        refs = int(clip)         # frames 0..14
+        pbytes = accel.planes_bytes(W, H, PAD)
+        planes = dev.alloc(pbytes * nf)
+        p = accel.search_params(W, H, PAD, 16, 16, R, 0, lam)
+        ms = accel.MotionSearch(dev, p)
+        lib = dev.lib
+
+        def step():
+            accel.build_planes(dev, refs, nf, W, H, PAD, out=planes)
+            ms.run_plain(cur, planes, nf)
+
+        for _ in range(args.warmup):
+            step()
+        dev.sync()
+        # timed region: K steps bracketed by device syncs; HIP events on the stream
+        # the kernels run on time the whole region and, per step, the search kernels
+        lib.rh264_set_profiling(dev.ctx, args.steps)
+        dev.sync()
+        t0 = time.perf_counter()
+        dev.record(0)
+        for _ in range(args.steps):
+            step()
+        dev.record(1)
+        dev.sync()
+        wall = time.perf_counter() - t0
+        ev_ms = dev.elapsed_ms(0, 1)
+        import ctypes as C
+        fs, rf = C.c_float(), C.c_float()
+        full_ms, refine_ms = [], []
+        for i in range(lib.rh264_profile_count(dev.ctx)):
+            lib.rh264_profile_get(dev.ctx, i, C.byref(fs), C.byref(rf))
+            full_ms.append(fs.value)
+            refine_ms.append(rf.value)
+        lib.rh264_set_profiling(dev.ctx, 0)
+        res = ms.results(nf, with_evals=False)
+
+    ms_per_step = ev_ms / args.steps
+    mpx = W * H * nf / (ms_per_step / 1e3) / 1e6
+    nblk = (W // 16) * (H // 16)
+    ops = nf * nblk * (2 * R + 1) ** 2 * 256  # abs-diff-adds per full-search launch
+    full_avg = float(np.mean(full_ms))
+    achieved = ops / (full_avg / 1e3) / 1e12
+    out = {
+        "metric": "1080p motion-search Mpx/s, GPU plain vs learned, at <=1% SATD-cost regret",
+        "value": round(mpx, 2), "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "1080p 1920x1088 clip, 16 frames (15 P-frames searched), 1 ref, "
+                               "+-32 full-search SAD, 6-tap half-pel planes, 4x4-Hadamard SATD qpel ring refine, "
+                               "QP 26 lambda", "search": "plain", "block": "16x16", "range": R, "pad": PAD,
+                   "device": dname},
+        "roofline": {"bound": "valu", "kernel": "k_full_search<16,16>",
+                     "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                     "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                     "work": f"{ops} abs-diff-adds per launch ((2R+1)^2*256 per 16x16 block)",
+                     "kernel_ms": round(full_avg, 4), "refine_ms": round(float(np.mean(refine_ms)), 4)},
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "mean_cost": float(res["cost"].mean()),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(frames, lam)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

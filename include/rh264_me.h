@@ -1,0 +1,167 @@
+/*
+ * rh264_me.h -- C ABI of the MI355X (gfx950) motion-search provider for
+ * rusty_h264.
+ *
+ * This is the drop-in boundary for the encoder's motion-search hot path.  In
+ * the reference every kernel on that path sits in the one crate that may use
+ * `unsafe`, crates/rusty_h264-accel (src/lib.rs:30-58, src/x86_asm.rs:13-21);
+ * its callers are crates/rusty_h264-common/src/inter.rs and
+ * crates/rusty_h264-encoder/src/{mb16.rs,mbtree.rs}.  A Rust `extern "C"`
+ * block in that crate binds the functions below.
+ *
+ * Conventions (mirroring the accel crate's):
+ *   - return 0 = done, 1 = declined (unsupported shape/geometry: the caller runs
+ *     its own path, as the reference does when an accel kernel returns None /
+ *     false), < 0 = error (rh264_last_error() describes it).
+ *   - all frame/plane/result pointers are DEVICE pointers (use rh264_alloc /
+ *     rh264_h2d / rh264_d2h, or any HIP allocation on the context's device);
+ *     no torch types; sizes are plain integers.
+ *   - every call is asynchronous on the context's stream; rh264_sync() waits.
+ *   - a context is used by one host thread at a time; separate contexts may be
+ *     used concurrently (the reference's kernels are called from GOP-parallel
+ *     threads, crates/rusty_h264-encoder/src/lib.rs:746-800).
+ *   - motion vectors are quarter-pel int32 pairs (x, y); costs are int32.
+ *
+ * Plane-set layout: one buffer of rh264_planes_bytes(cw, ch, pad) bytes holding
+ * the four planes of inter.rs:707-722 in the order F (edge-replicated full-pel
+ * copy), H (horizontal half-pel "b"), V (vertical half-pel "h"), C (centre
+ * "j"), each pw*ph bytes with stride pw = cw + 2*pad, ph = ch + 2*pad, followed
+ * by a small zeroed tail so vector loads near the end stay inside the buffer.
+ */
+#ifndef RH264_ME_H
+#define RH264_ME_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RH264_OK 0
+#define RH264_DECLINED 1
+#define RH264_LR_N 128 /* entries of a lambda-rate table */
+
+typedef struct rh264_ctx rh264_ctx;
+
+/* ---- context and device-memory plumbing (no reference counterpart: the
+ *      reference is in-process CPU code with no device) -------------------- */
+int rh264_init(int device, rh264_ctx **out);
+int rh264_destroy(rh264_ctx *ctx);
+const char *rh264_last_error(void);
+int rh264_device_name(rh264_ctx *ctx, char *buf, int buflen);
+int rh264_alloc(rh264_ctx *ctx, size_t bytes, void **dptr);
+int rh264_free(rh264_ctx *ctx, void *dptr);
+int rh264_h2d(rh264_ctx *ctx, void *dst, const void *src, size_t bytes);
+int rh264_d2h(rh264_ctx *ctx, void *dst, const void *src, size_t bytes);
+int rh264_memset(rh264_ctx *ctx, void *dst, int value, size_t bytes);
+int rh264_sync(rh264_ctx *ctx);
+/* Device-side timing on the context's stream (HIP events). */
+int rh264_event_record(rh264_ctx *ctx, int slot);
+int rh264_event_elapsed_ms(rh264_ctx *ctx, int slot_a, int slot_b, float *ms);
+
+/* ---- block metrics, batched -----------------------------------------------
+ * Replaces the per-block accel family (crates/rusty_h264-accel/src/satd_sad.rs:
+ * 309-363 `satd_16x16|16x8|8x16|8x8|4x4`, `sad_16x16|16x8|8x16`;
+ * crates/rusty_h264-accel/src/satd_avg.rs:350-602 `satd_x4p`, `satd_x4`,
+ * `sad_x4`, `satd_avg_x4`, `satd_avg`).  A GPU cannot serve one 17 ns block
+ * per call, so each call evaluates n blocks: block i compares src + src_off[i]
+ * (stride ss) with ref + ref_off[i] (stride rs) -- or, for RH264_METRIC_SATD_AVG,
+ * with the rounded average (a+b+1)>>1 of ref + ref_off[i] and ref + ref_off2[i].
+ * kind: RH264_METRIC_SAD        sum |a-b|                    (sad_scalar)
+ *       RH264_METRIC_SATD_WELS  sum_4x4 ((sum|H d|+1)>>1)    (satd_region_scalar)
+ *       RH264_METRIC_SATD       sum_4x4 sum|H d|             (satd_x4 / satd_px domain)
+ *       RH264_METRIC_SATD_AVG   sum|H (src - avg(a,b))|      (satd_avg)
+ * Shapes: (w,h) in {16x16,16x8,8x16,8x8,8x4,4x8,4x4}; others decline. */
+#define RH264_METRIC_SAD 0
+#define RH264_METRIC_SATD_WELS 1
+#define RH264_METRIC_SATD 2
+#define RH264_METRIC_SATD_AVG 3
+int rh264_block_metric(rh264_ctx *ctx, int kind, int w, int h, int n, const uint8_t *src, int ss,
+                       const int64_t *src_off, const uint8_t *ref, int rs, const int64_t *ref_off,
+                       const int64_t *ref_off2, int32_t *out);
+
+/* ---- half-pel plane sets -------------------------------------------------
+ * Replaces build_hpel_planes (crates/rusty_h264-common/src/inter.rs:816-843:
+ * pad_plane 786-814 + the fused builder crates/rusty_h264-accel/src/hpel.rs:34-203
+ * / its twins inter.rs:868-987), for nframes frames in one launch.  Frame i
+ * (cw x ch luma, stride cw) is at frames + i*frame_stride; its plane set is
+ * written at planes + i*rh264_planes_bytes(cw, ch, pad).  cw, ch multiples of
+ * 16; pad a multiple of 4 in [4, 128] (the reference clamps RFF_HPEL_PAD to
+ * 0..128, default 32). */
+size_t rh264_planes_bytes(int cw, int ch, int pad);
+int rh264_build_planes(rh264_ctx *ctx, const uint8_t *frames, size_t frame_stride, int nframes,
+                       int cw, int ch, int pad, uint8_t *planes);
+
+/* ---- motion search --------------------------------------------------------
+ * Search parameters (the knobs the reference reads from FrameEncoder fields and
+ * RFF_* variables, made explicit).  lr_fp / lr are lambda-rate tables:
+ * lr[b] = (int64)(lambda * b) truncated exactly as mb16.rs:2563 computes
+ * `(lam_r * rate as f64) as i64` (rate = mvbits(dx) + mvbits(dy), mb16.rs:
+ * 2474-2475); lr_fp prices the SAD full-pel phase (B2, lambda * RFF_ME_SADL,
+ * mb16.rs:2582-2590, 756-763), lr the SATD phase.  Use rh264_lambda_table(). */
+typedef struct {
+    int32_t cw, ch, pad;   /* frame size (multiples of 16) and plane pad         */
+    int32_t bw, bh;        /* partition: 16x16, 16x8, 8x16, 8x8, 8x4, 4x8, 4x4   */
+    int32_t range;         /* full-search radius R in full pels, 1..64           */
+    int32_t subpel_pat;    /* 0 ring8+iterate, 1 ring4+iterate, 2 ring8 single,
+                              3 ring4 single, -1 no rings (SATD reprice only)     */
+    int32_t lr_fp[RH264_LR_N];
+    int32_t lr[RH264_LR_N];
+} rh264_search_params;
+
+void rh264_lambda_table(double lambda, int32_t *lr /* RH264_LR_N */);
+
+/* Plain exhaustive integer search (build-defined; the reference's exhaustive
+ * grids are mb16.rs:2778-2889 and 2897-2955): for every bw x bh block of each
+ * of nframes frames, cost(mv) = SAD(mv) + lr_fp[rate(mv - mvp)] over
+ * mv = centre + 4*(dx,dy), dx,dy in [-R,R], raster order (dy outer, dx inner),
+ * first minimum kept (strict <, mb16.rs:2843-2855).  Frame i: cur + i*cur_stride,
+ * plane set planes + i*planes_stride.  centres / mvps: per-block (x,y) qpel
+ * pairs (centres multiples of 4), frame-major, or NULL for (0,0).  Outputs per
+ * block: int_mv (x,y), int_cost.  Reference positions outside the padded planes
+ * read as infinite edge replication (mc_luma's clamp, inter.rs:120-124). */
+int rh264_full_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes,
+                      const uint8_t *cur, size_t cur_stride, const uint8_t *planes,
+                      size_t planes_stride, const int32_t *centres, const int32_t *mvps,
+                      int32_t *int_mv, int32_t *int_cost);
+
+/* Sub-pel refinement (mb16.rs:2956-3291, cascade ring walk, steps 2 then 1,
+ * ring8 order (s,0),(-s,0),(0,s),(0,-s),(s,s),(-s,-s),(s,-s),(-s,s)) of every
+ * block from start_mv, cost(mv) = SATD(mv) + lr[rate(mv - mvp)] with SATD the
+ * encoder-domain sum|H d| of mb16.rs:6027-6074 over the qpel prediction of
+ * inter.rs:1015-1105 (MeCtx::eval, crates/rusty_h264-accel/src/mectx.rs:112-166).
+ * Outputs per block: mv, cost, and (optional, may be NULL) evals = number of
+ * candidate evaluations. */
+int rh264_subpel_refine(rh264_ctx *ctx, const rh264_search_params *p, int nframes,
+                        const uint8_t *cur, size_t cur_stride, const uint8_t *planes,
+                        size_t planes_stride, const int32_t *start_mv, const int32_t *mvps,
+                        int32_t *mv, int32_t *cost, int32_t *evals);
+
+/* Plain search pipeline = rh264_full_search then rh264_subpel_refine from its
+ * result, two launches for the whole batch.  int_mv / int_cost may be NULL. */
+int rh264_plain_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes,
+                       const uint8_t *cur, size_t cur_stride, const uint8_t *planes,
+                       size_t planes_stride, const int32_t *centres, const int32_t *mvps,
+                       int32_t *int_mv, int32_t *int_cost, int32_t *mv, int32_t *cost);
+
+/* Stage timing of rh264_plain_search (no reference counterpart; the reference's
+ * stage profiler is crates/rusty_h264-common/src/prof.rs).  rh264_set_profiling
+ * (capacity > 0) makes the next `capacity` plain searches record HIP events around
+ * the integer-search and refinement kernels on the context's stream, without any
+ * host synchronisation; rh264_profile_get(i) returns search i's two durations.
+ * capacity 0 turns it off and resets the count. */
+int rh264_set_profiling(rh264_ctx *ctx, int capacity);
+int rh264_profile_count(rh264_ctx *ctx);
+int rh264_profile_get(rh264_ctx *ctx, int i, float *search_ms, float *refine_ms);
+
+/* Quarter-pel prediction of blocks (mc_luma, inter.rs:1193-1262, served from
+ * the plane set as hpel_block inter.rs:1107-1162 does, any MV): block i of
+ * size bw x bh at (x[i], y[i]) with mv[2i..2i+1] is written to out + i*bw*bh. */
+int rh264_predict_blocks(rh264_ctx *ctx, const uint8_t *planes, int cw, int ch, int pad, int bw,
+                         int bh, int n, const int32_t *xy, const int32_t *mv, uint8_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,832 @@
+/*
+ * me_oracle.c -- CPU restatement of rusty_h264's encoder motion-search path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker and the timed CPU baseline);
+ * the product library never links it.  See me_oracle.h for the contract.
+ * Reference citations are file:line in the reference repository.
+ */
+#include "me_oracle.h"
+
+#include <limits.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+static inline int clip_u8(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+static inline int iabs(int v) { return v < 0 ? -v : v; }
+/* Rust `>>` on i32 is arithmetic (floor); spell it out so no compiler choice leaks in. */
+static inline int asr2(int v) { return v >= 0 ? (v >> 2) : -((-v + 3) >> 2); }
+
+/* ---------------------------------------------------------------------------
+ * Block metrics
+ * ------------------------------------------------------------------------- */
+
+/* sad_scalar, crates/rusty_h264-accel/src/satd_sad.rs:74-82 */
+int32_t orc_sad(const uint8_t *a, int sa, const uint8_t *b, int sb, int w, int h)
+{
+    int32_t s = 0;
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++)
+            s += iabs((int)a[i * sa + j] - (int)b[i * sb + j]);
+    return s;
+}
+
+/* satd4x4_scalar, satd_sad.rs:39-58: (sum |H d| + 1) >> 1, row butterfly
+ * (s0+s1, s2+s3, s2-s3, s0-s1) with s0=r0+r2, s1=r1+r3, s2=r0-r2, s3=r1-r3. */
+int32_t orc_satd4x4_wels(const uint8_t *a, int sa, const uint8_t *b, int sb)
+{
+    int32_t m[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            m[i][j] = (int32_t)a[i * sa + j] - (int32_t)b[i * sb + j];
+    for (int i = 0; i < 4; i++) {
+        int32_t s0 = m[i][0] + m[i][2], s1 = m[i][1] + m[i][3];
+        int32_t s2 = m[i][0] - m[i][2], s3 = m[i][1] - m[i][3];
+        m[i][0] = s0 + s1; m[i][1] = s2 + s3; m[i][2] = s2 - s3; m[i][3] = s0 - s1;
+    }
+    int32_t sum = 0;
+    for (int j = 0; j < 4; j++) {
+        int32_t s0 = m[0][j] + m[2][j], s1 = m[1][j] + m[3][j];
+        int32_t s2 = m[0][j] - m[2][j], s3 = m[1][j] - m[3][j];
+        sum += iabs(s0 + s1) + iabs(s2 + s3) + iabs(s2 - s3) + iabs(s0 - s1);
+    }
+    return (sum + 1) >> 1;
+}
+
+/* satd_region_scalar, satd_sad.rs:60-72 (rounding per 4x4 block) */
+int32_t orc_satd_wels(const uint8_t *a, int sa, const uint8_t *b, int sb, int w, int h)
+{
+    int32_t s = 0;
+    for (int by = 0; by < h; by += 4)
+        for (int bx = 0; bx < w; bx += 4)
+            s += orc_satd4x4_wels(a + by * sa + bx, sa, b + by * sb + bx, sb);
+    return s;
+}
+
+/* hadamard_1d / hadamard_4x4, crates/rusty_h264-common/src/transform.rs:631-655 */
+void orc_hadamard_4x4(const int32_t in[16], int32_t out[16])
+{
+    int32_t m[16];
+    memcpy(m, in, sizeof(m));
+    for (int r = 0; r < 4; r++) {
+        int32_t a = m[r * 4], b = m[r * 4 + 1], c = m[r * 4 + 2], d = m[r * 4 + 3];
+        m[r * 4] = a + b + c + d; m[r * 4 + 1] = a + b - c - d;
+        m[r * 4 + 2] = a - b - c + d; m[r * 4 + 3] = a - b + c - d;
+    }
+    for (int c = 0; c < 4; c++) {
+        int32_t a = m[c], b = m[4 + c], cc = m[8 + c], d = m[12 + c];
+        m[c] = a + b + cc + d; m[4 + c] = a + b - cc - d;
+        m[8 + c] = a - b - cc + d; m[12 + c] = a - b + cc - d;
+    }
+    memcpy(out, m, sizeof(m));
+}
+
+/* satd_px scalar arm, crates/rusty_h264-encoder/src/mb16.rs:6052-6074 with
+ * satd_4x4_sum (transform.rs:701-712): sum over 4x4 blocks of sum |H d|. */
+int64_t orc_satd_px(const uint8_t *src, int ss, const uint8_t *pred, int ps, int w, int h)
+{
+    int64_t total = 0;
+    for (int by = 0; by < h / 4; by++)
+        for (int bx = 0; bx < w / 4; bx++) {
+            int32_t blk[16], co[16];
+            for (int dy = 0; dy < 4; dy++)
+                for (int dx = 0; dx < 4; dx++)
+                    blk[dy * 4 + dx] = (int32_t)src[(by * 4 + dy) * ss + bx * 4 + dx] -
+                                       (int32_t)pred[(by * 4 + dy) * ps + bx * 4 + dx];
+            orc_hadamard_4x4(blk, co);
+            for (int k = 0; k < 16; k++) total += iabs(co[k]);
+        }
+    return total;
+}
+
+/* satd_avg, crates/rusty_h264-accel/src/satd_avg.rs:579-602: sum |H (src - (a+b+1)>>1)|,
+ * unhalved (the satd_px value of the materialized average). */
+uint32_t orc_satd_avg(const uint8_t *src, int ss, const uint8_t *a, const uint8_t *b,
+                      int stride, int w, int h)
+{
+    uint8_t avg[256];
+    for (int r = 0; r < h; r++)
+        for (int i = 0; i < w; i++)
+            avg[r * w + i] = (uint8_t)(((int)a[r * stride + i] + (int)b[r * stride + i] + 1) >> 1);
+    return (uint32_t)orc_satd_px(src, ss, avg, w, w, h);
+}
+
+/* x4_shape, satd_avg.rs:342-344 */
+int orc_x4_shape(int w, int h)
+{
+    return (w == 16 && h == 16) || (w == 16 && h == 8) || (w == 8 && h == 16) || (w == 8 && h == 8);
+}
+
+/* ---------------------------------------------------------------------------
+ * Luma interpolation
+ * ------------------------------------------------------------------------- */
+
+/* at(), inter.rs:120-124 (edge clamp) */
+static inline int at(const uint8_t *ref, int cw, int ch, int x, int y)
+{
+    return ref[clampi(y, 0, ch - 1) * cw + clampi(x, 0, cw - 1)];
+}
+
+/* luma_sample, inter.rs:134-166: the spec sec. 8.4.2.2.1 per-pixel quarter-pel sample. */
+int orc_luma_sample(const uint8_t *ref, int cw, int ch, int ix, int iy, int fx, int fy)
+{
+#define G(dx, dy) at(ref, cw, ch, ix + (dx), iy + (dy))
+#define HOR6(dy) (G(-2, dy) - 5 * G(-1, dy) + 20 * G(0, dy) + 20 * G(1, dy) - 5 * G(2, dy) + G(3, dy))
+#define VER6(dx) (G(dx, -2) - 5 * G(dx, -1) + 20 * G(dx, 0) + 20 * G(dx, 1) - 5 * G(dx, 2) + G(dx, 3))
+    if (fx == 0 && fy == 0) return G(0, 0);
+    int b = clip_u8((HOR6(0) + 16) >> 5);
+    int hh = clip_u8((VER6(0) + 16) >> 5);
+    int m = clip_u8((VER6(1) + 16) >> 5);
+    int s = clip_u8((HOR6(1) + 16) >> 5);
+    int j1 = HOR6(-2) - 5 * HOR6(-1) + 20 * HOR6(0) + 20 * HOR6(1) - 5 * HOR6(2) + HOR6(3);
+    int j = clip_u8((j1 + 512) >> 10);
+#undef G
+#undef HOR6
+#undef VER6
+    switch (fx * 4 + fy) {
+    case 1 * 4 + 0: return (at(ref, cw, ch, ix, iy) + b + 1) >> 1;
+    case 2 * 4 + 0: return b;
+    case 3 * 4 + 0: return (at(ref, cw, ch, ix + 1, iy) + b + 1) >> 1;
+    case 0 * 4 + 1: return (at(ref, cw, ch, ix, iy) + hh + 1) >> 1;
+    case 1 * 4 + 1: return (b + hh + 1) >> 1;
+    case 2 * 4 + 1: return (b + j + 1) >> 1;
+    case 3 * 4 + 1: return (b + m + 1) >> 1;
+    case 0 * 4 + 2: return hh;
+    case 1 * 4 + 2: return (hh + j + 1) >> 1;
+    case 2 * 4 + 2: return j;
+    case 3 * 4 + 2: return (j + m + 1) >> 1;
+    case 0 * 4 + 3: return (at(ref, cw, ch, ix, iy + 1) + hh + 1) >> 1;
+    case 1 * 4 + 3: return (hh + s + 1) >> 1;
+    case 2 * 4 + 3: return (j + s + 1) >> 1;
+    default: return (m + s + 1) >> 1; /* (3,3) */
+    }
+}
+
+#define LUMA_TILE 21
+
+/* luma_tile_into, inter.rs:365-402: (bw+5)x(bh+5) clamped neighbourhood, block at (2,2). */
+static int luma_tile_into(uint8_t *t, const uint8_t *ref, int cw, int ch, int ix0, int iy0,
+                          int bw, int bh)
+{
+    int ts = bw + 5;
+    for (int ty = 0; ty < bh + 5; ty++) {
+        int ry = clampi(iy0 - 2 + ty, 0, ch - 1) * cw;
+        for (int tx = 0; tx < ts; tx++)
+            t[ty * ts + tx] = ref[ry + clampi(ix0 - 2 + tx, 0, cw - 1)];
+    }
+    return ts;
+}
+
+/* luma_h, inter.rs:406-431 (McHorVer20), block shifted by (dr, dc) */
+static void luma_h(const uint8_t *t, int ts, int bw, int bh, int dr, int dc, uint8_t *dst)
+{
+    for (int r = 0; r < bh; r++) {
+        const uint8_t *w = t + (2 + r + dr) * ts + 2 + dc - 2;
+        for (int c = 0; c < bw; c++) {
+            int f = w[c] - 5 * w[c + 1] + 20 * w[c + 2] + 20 * w[c + 3] - 5 * w[c + 4] + w[c + 5];
+            dst[r * bw + c] = (uint8_t)clip_u8((f + 16) >> 5);
+        }
+    }
+}
+
+/* luma_v, inter.rs:434-455 (McHorVer02) */
+static void luma_v(const uint8_t *t, int ts, int bw, int bh, int dr, int dc, uint8_t *dst)
+{
+    for (int r = 0; r < bh; r++) {
+        int base = (2 + r + dr) * ts + 2 + dc;
+        for (int c = 0; c < bw; c++) {
+            int p = base + c;
+            int f = t[p - 2 * ts] - 5 * t[p - ts] + 20 * t[p] + 20 * t[p + ts] - 5 * t[p + 2 * ts] +
+                    t[p + 3 * ts];
+            dst[r * bw + c] = (uint8_t)clip_u8((f + 16) >> 5);
+        }
+    }
+}
+
+/* luma_centre, inter.rs:459-488 (McHorVer22): vertical 6-tap (unrounded i32), then
+ * horizontal 6-tap, (+512)>>10. */
+static void luma_centre(const uint8_t *t, int ts, int bw, int bh, uint8_t *dst)
+{
+    int32_t itmp[LUMA_TILE];
+    for (int r = 0; r < bh; r++) {
+        int base = (2 + r) * ts;
+        for (int j = 0; j < bw + 5; j++) {
+            int p = base + j;
+            itmp[j] = t[p - 2 * ts] - 5 * t[p - ts] + 20 * t[p] + 20 * t[p + ts] -
+                      5 * t[p + 2 * ts] + t[p + 3 * ts];
+        }
+        for (int c = 0; c < bw; c++) {
+            const int32_t *w = itmp + c;
+            int f = w[0] - 5 * w[1] + 20 * w[2] + 20 * w[3] - 5 * w[4] + w[5];
+            dst[r * bw + c] = (uint8_t)clip_u8((f + 512) >> 10);
+        }
+    }
+}
+
+/* pixel_avg, inter.rs:491-520 */
+static void pixel_avg(const uint8_t *a, const uint8_t *b, int bw, int bh, uint8_t *dst)
+{
+    for (int i = 0; i < bw * bh; i++) dst[i] = (uint8_t)((a[i] + b[i] + 1) >> 1);
+}
+
+/* avg_full, inter.rs:523-545: full-pel block shifted by (dr, dc) averaged with a half plane */
+static void avg_full(const uint8_t *t, int ts, int bw, int bh, int dr, int dc,
+                     const uint8_t *half, uint8_t *dst)
+{
+    for (int r = 0; r < bh; r++) {
+        const uint8_t *trow = t + (2 + r + dr) * ts + 2 + dc;
+        for (int c = 0; c < bw; c++)
+            dst[r * bw + c] = (uint8_t)((trow[c] + half[r * bw + c] + 1) >> 1);
+    }
+}
+
+/* mc_luma_subpel, inter.rs:660-694, with the compose forms of qpel_one_filter
+ * (568-594), qpel_hv (598-619) and qpel_centre_adj (627-654). */
+static void mc_luma_subpel(const uint8_t *t, int ts, int bw, int bh, int fx, int fy, uint8_t *out)
+{
+    uint8_t a[256], b[256];
+    switch (fx * 4 + fy) {
+    case 2 * 4 + 0: luma_h(t, ts, bw, bh, 0, 0, out); break;
+    case 0 * 4 + 2: luma_v(t, ts, bw, bh, 0, 0, out); break;
+    case 2 * 4 + 2: luma_centre(t, ts, bw, bh, out); break;
+    case 1 * 4 + 0: luma_h(t, ts, bw, bh, 0, 0, a); avg_full(t, ts, bw, bh, 0, 0, a, out); break;
+    case 3 * 4 + 0: luma_h(t, ts, bw, bh, 0, 0, a); avg_full(t, ts, bw, bh, 0, 1, a, out); break;
+    case 0 * 4 + 1: luma_v(t, ts, bw, bh, 0, 0, a); avg_full(t, ts, bw, bh, 0, 0, a, out); break;
+    case 0 * 4 + 3: luma_v(t, ts, bw, bh, 0, 0, a); avg_full(t, ts, bw, bh, 1, 0, a, out); break;
+    case 1 * 4 + 1: luma_h(t, ts, bw, bh, 0, 0, a); luma_v(t, ts, bw, bh, 0, 0, b); pixel_avg(a, b, bw, bh, out); break;
+    case 3 * 4 + 1: luma_h(t, ts, bw, bh, 0, 0, a); luma_v(t, ts, bw, bh, 0, 1, b); pixel_avg(a, b, bw, bh, out); break;
+    case 1 * 4 + 3: luma_h(t, ts, bw, bh, 1, 0, a); luma_v(t, ts, bw, bh, 0, 0, b); pixel_avg(a, b, bw, bh, out); break;
+    case 3 * 4 + 3: luma_h(t, ts, bw, bh, 1, 0, a); luma_v(t, ts, bw, bh, 0, 1, b); pixel_avg(a, b, bw, bh, out); break;
+    case 2 * 4 + 1: luma_h(t, ts, bw, bh, 0, 0, a); luma_centre(t, ts, bw, bh, b); pixel_avg(a, b, bw, bh, out); break;
+    case 2 * 4 + 3: luma_h(t, ts, bw, bh, 1, 0, a); luma_centre(t, ts, bw, bh, b); pixel_avg(a, b, bw, bh, out); break;
+    case 1 * 4 + 2: luma_v(t, ts, bw, bh, 0, 0, a); luma_centre(t, ts, bw, bh, b); pixel_avg(a, b, bw, bh, out); break;
+    case 3 * 4 + 2: luma_v(t, ts, bw, bh, 0, 1, a); luma_centre(t, ts, bw, bh, b); pixel_avg(a, b, bw, bh, out); break;
+    default: break;
+    }
+}
+
+/* mc_luma, inter.rs:1193-1262 */
+void orc_mc_luma(const uint8_t *ref, int cw, int ch, int x0, int y0, int bw, int bh,
+                 int mvx, int mvy, uint8_t *out)
+{
+    int ix0 = x0 + asr2(mvx), iy0 = y0 + asr2(mvy);
+    int fx = mvx & 3, fy = mvy & 3;
+    if (fx == 0 && fy == 0) {
+        for (int dy = 0; dy < bh; dy++)
+            for (int dx = 0; dx < bw; dx++)
+                out[dy * bw + dx] = (uint8_t)at(ref, cw, ch, ix0 + dx, iy0 + dy);
+        return;
+    }
+    uint8_t tile[LUMA_TILE * LUMA_TILE];
+    int ts = luma_tile_into(tile, ref, cw, ch, ix0, iy0, bw, bh);
+    mc_luma_subpel(tile, ts, bw, bh, fx, fy, out);
+}
+
+/* pad_plane, inter.rs:786-814 (openh264 ExpandPicture) */
+void orc_pad_plane(const uint8_t *src, int w, int h, int pad, uint8_t *out)
+{
+    int pw = w + 2 * pad, ph = h + 2 * pad;
+    for (int y = 0; y < ph; y++) {
+        const uint8_t *row = src + clampi(y - pad, 0, h - 1) * w;
+        uint8_t *d = out + (size_t)y * pw;
+        memset(d, row[0], (size_t)pad);
+        memcpy(d + pad, row, (size_t)w);
+        memset(d + pad + w, row[w - 1], (size_t)pad);
+    }
+}
+
+/* build_hpel_tiles, inter.rs:868-897: 16x16 tile walk over the padded plane through
+ * luma_tile_into + luma_h/luma_v/luma_centre (clamped to the padded plane). */
+void orc_build_hpel_tiles(const uint8_t *f, int pw, int ph, uint8_t *h, uint8_t *v, uint8_t *c)
+{
+    uint8_t tile[LUMA_TILE * LUMA_TILE], blk[256];
+    for (int by = 0; by < ph; by += 16)
+        for (int bx = 0; bx < pw; bx += 16) {
+            int bh = ph - by < 16 ? ph - by : 16;
+            int bwid = pw - bx < 16 ? pw - bx : 16;
+            int ts = luma_tile_into(tile, f, pw, ph, bx, by, bwid, bh);
+            for (int kind = 0; kind < 3; kind++) {
+                uint8_t *plane = kind == 0 ? h : (kind == 1 ? v : c);
+                if (kind == 0) luma_h(tile, ts, bwid, bh, 0, 0, blk);
+                else if (kind == 1) luma_v(tile, ts, bwid, bh, 0, 0, blk);
+                else luma_centre(tile, ts, bwid, bh, blk);
+                for (int r = 0; r < bh; r++)
+                    memcpy(plane + (size_t)(by + r) * pw + bx, blk + r * bwid, (size_t)bwid);
+            }
+        }
+}
+
+/* build_hpel_fused, inter.rs:908-987: single pass, row-clamped vertical taps into
+ * i32 intermediates vt (column-clamped halo), then H from the clamped row, V from
+ * vt, C from the 6-tap over vt. */
+void orc_build_hpel_fused(const uint8_t *f, int pw, int ph, uint8_t *h, uint8_t *v, uint8_t *c)
+{
+    int32_t *vt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(pw + 5));
+    int32_t *hb = (int32_t *)malloc(sizeof(int32_t) * (size_t)(pw + 5));
+    for (int y = 0; y < ph; y++) {
+        const uint8_t *rows[6];
+        for (int k = 0; k < 6; k++) rows[k] = f + (size_t)clampi(y - 2 + k, 0, ph - 1) * pw;
+        for (int j = 0; j < pw + 5; j++) {
+            int x = clampi(j - 2, 0, pw - 1);
+            vt[j] = rows[0][x] - 5 * rows[1][x] + 20 * rows[2][x] + 20 * rows[3][x] - 5 * rows[4][x] +
+                    rows[5][x];
+            hb[j] = rows[2][x];
+        }
+        for (int x = 0; x < pw; x++) {
+            v[(size_t)y * pw + x] = (uint8_t)clip_u8((vt[x + 2] + 16) >> 5);
+            const int32_t *w = vt + x;
+            int s = w[0] - 5 * w[1] + 20 * w[2] + 20 * w[3] - 5 * w[4] + w[5];
+            c[(size_t)y * pw + x] = (uint8_t)clip_u8((s + 512) >> 10);
+            const int32_t *q = hb + x;
+            int t = q[0] - 5 * q[1] + 20 * q[2] + 20 * q[3] - 5 * q[4] + q[5];
+            h[(size_t)y * pw + x] = (uint8_t)clip_u8((t + 16) >> 5);
+        }
+    }
+    free(vt);
+    free(hb);
+}
+
+/* build_hpel_planes, inter.rs:816-843 */
+void orc_build_hpel_planes(const uint8_t *y, int cw, int ch, int pad, int fused, uint8_t *planes)
+{
+    int pw = cw + 2 * pad, ph = ch + 2 * pad;
+    size_t n = (size_t)pw * ph;
+    orc_pad_plane(y, cw, ch, pad, planes);
+    if (fused) orc_build_hpel_fused(planes, pw, ph, planes + n, planes + 2 * n, planes + 3 * n);
+    else orc_build_hpel_tiles(planes, pw, ph, planes + n, planes + 2 * n, planes + 3 * n);
+}
+
+/* Quarter-phase operand table shared by hpel_qpel_refs (inter.rs:1060-1105) and
+ * MeCtx::eval (crates/rusty_h264-accel/src/mectx.rs:143-156): plane indices
+ * (0=F,1=H,2=V,3=C) and (x,y) offsets of the two averaged operands. */
+static const int QTAB[16][6] = {
+    /* idx = fx*4+fy : pa, ax, ay, pb, bx, by */
+    [1 * 4 + 0] = {0, 0, 0, 1, 0, 0}, [3 * 4 + 0] = {0, 1, 0, 1, 0, 0},
+    [0 * 4 + 1] = {0, 0, 0, 2, 0, 0}, [0 * 4 + 3] = {0, 0, 1, 2, 0, 0},
+    [1 * 4 + 1] = {1, 0, 0, 2, 0, 0}, [3 * 4 + 1] = {1, 0, 0, 2, 1, 0},
+    [1 * 4 + 3] = {1, 0, 1, 2, 0, 0}, [3 * 4 + 3] = {1, 0, 1, 2, 1, 0},
+    [2 * 4 + 1] = {1, 0, 0, 3, 0, 0}, [2 * 4 + 3] = {1, 0, 1, 3, 0, 0},
+    [1 * 4 + 2] = {2, 0, 0, 3, 0, 0}, [3 * 4 + 2] = {2, 1, 0, 3, 0, 0},
+};
+
+/* hpel_block, inter.rs:1107-1162 (with hpel_qpel_refs' guard and table) */
+int orc_hpel_block(const uint8_t *planes, int cw, int ch, int pad, int x0, int y0, int bw, int bh,
+                   int mvx, int mvy, uint8_t *out)
+{
+    int pw = cw + 2 * pad, ph = ch + 2 * pad;
+    size_t n = (size_t)pw * ph;
+    int ix0 = x0 + asr2(mvx), iy0 = y0 + asr2(mvy);
+    int fx = mvx & 3, fy = mvy & 3;
+    if (fx == 0 && fy == 0) return 0;
+    int px = ix0 + pad, py = iy0 + pad;
+    if (px < 0 || py < 0 || px + bw + 1 > pw || py + bh + 1 > ph) return 0;
+    size_t base = (size_t)py * pw + px;
+    if ((fx & 1) == 0 && (fy & 1) == 0) {
+        const uint8_t *src = planes + n * (fx == 2 ? (fy == 2 ? 3 : 1) : 2);
+        for (int r = 0; r < bh; r++) memcpy(out + r * bw, src + base + (size_t)r * pw, (size_t)bw);
+        return 1;
+    }
+    if (!(bw == 16 || bw == 8 || bw == 4)) return 0;
+    const int *q = QTAB[fx * 4 + fy];
+    const uint8_t *pa = planes + n * q[0] + base + q[1] + (size_t)q[2] * pw;
+    const uint8_t *pb = planes + n * q[3] + base + q[4] + (size_t)q[5] * pw;
+    for (int r = 0; r < bh; r++)
+        for (int i = 0; i < bw; i++)
+            out[r * bw + i] = (uint8_t)((pa[(size_t)r * pw + i] + pb[(size_t)r * pw + i] + 1) >> 1);
+    return 1;
+}
+
+static inline int plane_at(const uint8_t *planes, size_t n, int pw, int ph, int k, int x, int y)
+{
+    return planes[n * k + (size_t)clampi(y, 0, ph - 1) * pw + clampi(x, 0, pw - 1)];
+}
+
+void orc_pred_block(const uint8_t *planes, int cw, int ch, int pad, int x0, int y0, int bw,
+                    int bh, int mvx, int mvy, uint8_t *out)
+{
+    int pw = cw + 2 * pad, ph = ch + 2 * pad;
+    size_t n = (size_t)pw * ph;
+    int px = x0 + asr2(mvx) + pad, py = y0 + asr2(mvy) + pad;
+    int fx = mvx & 3, fy = mvy & 3;
+    for (int r = 0; r < bh; r++)
+        for (int i = 0; i < bw; i++) {
+            int v;
+            if ((fx & 1) == 0 && (fy & 1) == 0) {
+                int k = fx == 0 ? (fy == 0 ? 0 : 2) : (fy == 0 ? 1 : 3);
+                v = plane_at(planes, n, pw, ph, k, px + i, py + r);
+            } else {
+                const int *q = QTAB[fx * 4 + fy];
+                int a = plane_at(planes, n, pw, ph, q[0], px + i + q[1], py + r + q[2]);
+                int b = plane_at(planes, n, pw, ph, q[3], px + i + q[4], py + r + q[5]);
+                v = (a + b + 1) >> 1;
+            }
+            out[r * bw + i] = (uint8_t)v;
+        }
+}
+
+/* ---------------------------------------------------------------------------
+ * Motion search
+ * ------------------------------------------------------------------------- */
+
+/* mvbits, mb16.rs:2474-2475: 1 + 2*floor(log2(codenum+1)) */
+uint32_t orc_mvbits(int32_t d)
+{
+    d = clampi(d, -(1 << 24), 1 << 24); /* keeps 2d-1 from wrapping; no coded MV is this large */
+    uint32_t codenum = d > 0 ? (uint32_t)(2 * d - 1) : (uint32_t)(-2 * d);
+    return 1u + 2u * (31u - (uint32_t)__builtin_clz(codenum + 1u));
+}
+
+/* (lambda * rate as f64) as i64, mb16.rs:2563: truncation toward zero */
+void orc_lambda_table(double lambda, int n, int64_t *lr)
+{
+    for (int b = 0; b < n; b++) lr[b] = (int64_t)(lambda * (double)b);
+}
+
+static inline int64_t rate_cost(const int64_t *lr, int mvx, int mvy, int px, int py)
+{
+    uint32_t r = orc_mvbits(mvx - px) + orc_mvbits(mvy - py);
+    return lr[r < ORC_LR_N ? r : ORC_LR_N - 1];
+}
+
+/* SAD of the block against the full-pel plane F with clamped coordinates
+ * (the mc_sad_hp values, mb16.rs:2018-2079, for full-pel candidates). */
+static int32_t sad_fullpel(const orc_search_cfg *cfg, const uint8_t *cur, const uint8_t *planes,
+                           int lx, int ly, int mvx, int mvy)
+{
+    int pw = cfg->cw + 2 * cfg->pad, ph = cfg->ch + 2 * cfg->pad;
+    int px = lx + asr2(mvx) + cfg->pad, py = ly + asr2(mvy) + cfg->pad;
+    int32_t s = 0;
+    if (px >= 0 && py >= 0 && px + cfg->bw <= pw && py + cfg->bh <= ph)
+        return orc_sad(cur + (size_t)ly * cfg->cw + lx, cfg->cw, planes + (size_t)py * pw + px, pw,
+                       cfg->bw, cfg->bh);
+    for (int r = 0; r < cfg->bh; r++)
+        for (int i = 0; i < cfg->bw; i++)
+            s += iabs((int)cur[(size_t)(ly + r) * cfg->cw + lx + i] -
+                      plane_at(planes, 0, pw, ph, 0, px + i, py + r));
+    return s;
+}
+
+int64_t orc_full_search_block(const orc_search_cfg *cfg, const uint8_t *cur,
+                              const uint8_t *planes, int lx, int ly, int cx, int cy,
+                              int mvpx, int mvpy, int32_t *mv_out)
+{
+    int R = cfg->range;
+    int64_t best = INT64_MAX;
+    int bx = cx, by = cy;
+    /* raster: dy outer, dx inner, strict < (first minimum wins), mb16.rs:2810-2856 */
+    for (int dy = -R; dy <= R; dy++)
+        for (int dx = -R; dx <= R; dx++) {
+            int mvx = cx + 4 * dx, mvy = cy + 4 * dy;
+            int64_t c = sad_fullpel(cfg, cur, planes, lx, ly, mvx, mvy) +
+                        rate_cost(cfg->lr_fp, mvx, mvy, mvpx, mvpy);
+            if (c < best) { best = c; bx = mvx; by = mvy; }
+        }
+    mv_out[0] = bx;
+    mv_out[1] = by;
+    return best;
+}
+
+static int64_t satd_dist(const orc_search_cfg *cfg, const uint8_t *cur, const uint8_t *planes,
+                         int lx, int ly, int mvx, int mvy)
+{
+    uint8_t pred[256];
+    orc_pred_block(planes, cfg->cw, cfg->ch, cfg->pad, lx, ly, cfg->bw, cfg->bh, mvx, mvy, pred);
+    return orc_satd_px(cur + (size_t)ly * cfg->cw + lx, cfg->cw, pred, cfg->bw, cfg->bw, cfg->bh);
+}
+
+int64_t orc_satd_cost(const orc_search_cfg *cfg, const uint8_t *cur, const uint8_t *planes,
+                      int lx, int ly, int mvx, int mvy, int mvpx, int mvpy)
+{
+    return satd_dist(cfg, cur, planes, lx, ly, mvx, mvy) +
+           rate_cost(cfg->lr, mvx, mvy, mvpx, mvpy);
+}
+
+/* Sub-pel ring walk, mb16.rs:3046-3046 ring8 order (s,0),(-s,0),(0,s),(0,-s),(s,s),(-s,-s),
+ * (s,-s),(-s,s); cascade with a moving centre (mb16.rs:3230-3275); steps [2,1];
+ * pattern bit0 = ring4, bit1 = single pass (mb16.rs:2976-2998). */
+int64_t orc_subpel_refine_block(const orc_search_cfg *cfg, const uint8_t *cur,
+                                const uint8_t *planes, int lx, int ly, int mvx, int mvy,
+                                int mvpx, int mvpy, int32_t *mv_out, int32_t *evals)
+{
+    int bx = mvx, by = mvy;
+    int64_t best = orc_satd_cost(cfg, cur, planes, lx, ly, bx, by, mvpx, mvpy);
+    int ne = 1;
+    int pat = cfg->subpel_pat;
+    if (pat >= 0) {
+        for (int step = 2; step >= 1; step--) {
+            const int ring[8][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step},
+                                    {step, step}, {-step, -step}, {step, -step}, {-step, step}};
+            int nring = (pat & 1) ? 4 : 8;
+            for (;;) {
+                int improved = 0;
+                for (int i = 0; i < nring; i++) {
+                    int cx = bx + ring[i][0], cy = by + ring[i][1];
+                    int64_t cc = orc_satd_cost(cfg, cur, planes, lx, ly, cx, cy, mvpx, mvpy);
+                    ne++;
+                    if (cc < best) { best = cc; bx = cx; by = cy; improved = 1; }
+                }
+                if (!improved || (pat & 2)) break;
+            }
+        }
+    }
+    mv_out[0] = bx;
+    mv_out[1] = by;
+    if (evals) *evals = ne;
+    return best;
+}
+
+void orc_plain_search_frame(const orc_search_cfg *cfg, const uint8_t *cur, const uint8_t *planes,
+                            const int32_t *centres, const int32_t *mvps, int32_t *int_mv,
+                            int64_t *int_cost, int32_t *mv, int64_t *cost, int nthreads)
+{
+    int nbx = cfg->cw / cfg->bw, nby = cfg->ch / cfg->bh;
+    int nb = nbx * nby;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads)
+#else
+    (void)nthreads;
+#endif
+    for (int b = 0; b < nb; b++) {
+        int lx = (b % nbx) * cfg->bw, ly = (b / nbx) * cfg->bh;
+        int cx = centres ? centres[2 * b] : 0, cy = centres ? centres[2 * b + 1] : 0;
+        int px = mvps ? mvps[2 * b] : 0, py = mvps ? mvps[2 * b + 1] : 0;
+        int32_t imv[2], fmv[2];
+        int64_t ic = orc_full_search_block(cfg, cur, planes, lx, ly, cx, cy, px, py, imv);
+        int64_t fc = ic;
+        fmv[0] = imv[0];
+        fmv[1] = imv[1];
+        fc = orc_subpel_refine_block(cfg, cur, planes, lx, ly, imv[0], imv[1], px, py, fmv, NULL);
+        if (int_mv) { int_mv[2 * b] = imv[0]; int_mv[2 * b + 1] = imv[1]; }
+        if (int_cost) int_cost[b] = ic;
+        if (mv) { mv[2 * b] = fmv[0]; mv[2 * b + 1] = fmv[1]; }
+        if (cost) cost[b] = fc;
+    }
+}
+
+/* SAD of any candidate through the prediction read (mc_sad_hp, mb16.rs:2018-2079) */
+static int32_t sad_any(const orc_search_cfg *cfg, const uint8_t *cur, const uint8_t *planes,
+                     int lx, int ly, int mvx, int mvy)
+{
+    if ((mvx & 3) == 0 && (mvy & 3) == 0) return sad_fullpel(cfg, cur, planes, lx, ly, mvx, mvy);
+    uint8_t pred[256];
+    orc_pred_block(planes, cfg->cw, cfg->ch, cfg->pad, lx, ly, cfg->bw, cfg->bh, mvx, mvy, pred);
+    return orc_sad(cur + (size_t)ly * cfg->cw + lx, cfg->cw, pred, cfg->bw, cfg->bw, cfg->bh);
+}
+
+/* Reference-mode search, mb16.rs:2433-3302, default knobs explicit: seed (0,0) then
+ * predictors (strict <), integer snap ((v+2) div_euclid 4)*4, diamond rungs with the
+ * cascade walk (mb16.rs:2708-2725), optional B2 SAD full-pel phase with the SATD
+ * reprice of winner and seed (mb16.rs:2739-2742), sub-pel rings, seed kept if cheaper
+ * (mb16.rs:3297-3300).  The me_wide rescue grid and the FC argmin batches are off
+ * (their defaults on the paths this restates). */
+static int div_euclid4(int v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
+
+int64_t orc_motion_search_ref(const orc_search_cfg *cfg, const orc_diamond_cfg *dc,
+                              const uint8_t *cur, const uint8_t *planes, int lx, int ly,
+                              const int32_t *preds, int npreds, int32_t *mv_out)
+{
+    int cx0 = npreds > 0 ? preds[0] : 0, cy0 = npreds > 0 ? preds[1] : 0;
+#define COST(X, Y) orc_satd_cost(cfg, cur, planes, lx, ly, (X), (Y), cx0, cy0)
+#define COST_FP(X, Y)                                                                          \
+    (dc->sadfp ? (int64_t)sad_any(cfg, cur, planes, lx, ly, (X), (Y)) +                \
+                     rate_cost(cfg->lr_fp, (X), (Y), cx0, cy0)                                 \
+               : COST(X, Y))
+    int bx = 0, by = 0;
+    int64_t bc;
+    bc = COST_FP(0, 0);
+    for (int i = 0; i < npreds; i++) {
+        int64_t pc = COST_FP(preds[2 * i], preds[2 * i + 1]);
+        if (pc < bc) { bc = pc; bx = preds[2 * i]; by = preds[2 * i + 1]; }
+    }
+    int seedx = bx, seedy = by;
+    int64_t seed_c = bc;
+    if (dc->snap && ((bx & 3) != 0 || (by & 3) != 0)) {
+        bx = div_euclid4(bx + 2) * 4;
+        by = div_euclid4(by + 2) * 4;
+        bc = COST_FP(bx, by);
+    }
+    for (int si = 0; si < dc->nsteps; si++) {
+        int step = dc->steps[si];
+        for (;;) {
+            int improved = 0;
+            const int ring[4][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step}};
+            for (int i = 0; i < 4; i++) {
+                int x = bx + ring[i][0], y = by + ring[i][1];
+                int64_t cc = COST_FP(x, y);
+                if (cc < bc) { bc = cc; bx = x; by = y; improved = 1; }
+            }
+            if (!improved) break;
+        }
+    }
+    if (dc->sadfp) {
+        bc = COST(bx, by);
+        seed_c = COST(seedx, seedy);
+    }
+    int pat = cfg->subpel_pat;
+    if (pat >= 0) {
+        for (int step = 2; step >= 1; step--) {
+            const int ring[8][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step},
+                                    {step, step}, {-step, -step}, {step, -step}, {-step, step}};
+            int nring = (pat & 1) ? 4 : 8;
+            for (;;) {
+                int improved = 0;
+                for (int i = 0; i < nring; i++) {
+                    int x = bx + ring[i][0], y = by + ring[i][1];
+                    int64_t cc = COST(x, y);
+                    if (cc < bc) { bc = cc; bx = x; by = y; improved = 1; }
+                }
+                if (!improved || (pat & 2)) break;
+            }
+        }
+    }
+    if (dc->snap && seed_c < bc) { bx = seedx; by = seedy; bc = seed_c; }
+#undef COST
+#undef COST_FP
+    mv_out[0] = bx;
+    mv_out[1] = by;
+    return bc;
+}
+
+/* ---------------------------------------------------------------------------
+ * mb-tree lookahead
+ * ------------------------------------------------------------------------- */
+
+/* downsample2x, mbtree.rs:61-74: (a+b+c+d+2)/4 */
+void orc_downsample2x(const uint8_t *y, int cw, int ch, uint8_t *out)
+{
+    int hw = cw / 2, hh = ch / 2;
+    for (int j = 0; j < hh; j++)
+        for (int i = 0; i < hw; i++) {
+            unsigned s = (unsigned)y[(size_t)2 * j * cw + 2 * i] + y[(size_t)2 * j * cw + 2 * i + 1] +
+                         y[(size_t)(2 * j + 1) * cw + 2 * i] + y[(size_t)(2 * j + 1) * cw + 2 * i + 1];
+            out[(size_t)j * hw + i] = (uint8_t)((s + 2) / 4);
+        }
+}
+
+/* intra_cost, mbtree.rs:78-93: AC SATD (coefficient 0 excluded), min i32::MAX, max 1 */
+int32_t orc_intra_cost(const uint8_t *sy, int cw, int bx0, int by0, int bs)
+{
+    int64_t s = 0;
+    for (int by = 0; by < bs / 4; by++)
+        for (int bx = 0; bx < bs / 4; bx++) {
+            int32_t blk[16], h[16];
+            for (int dy = 0; dy < 4; dy++)
+                for (int dx = 0; dx < 4; dx++)
+                    blk[dy * 4 + dx] = sy[(size_t)(by0 + by * 4 + dy) * cw + bx0 + bx * 4 + dx];
+            orc_hadamard_4x4(blk, h);
+            for (int k = 1; k < 16; k++) s += iabs(h[k]);
+        }
+    if (s > INT32_MAX) s = INT32_MAX;
+    return s < 1 ? 1 : (int32_t)s;
+}
+
+/* mc_satd, mbtree.rs:102-148: interior full-pel satd_px, else mc_luma + 4x4 SATD */
+int64_t orc_mbtree_mc_satd(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0,
+                           int by0, int bs, int mvx, int mvy)
+{
+    uint8_t pred[256];
+    orc_mc_luma(ref, cw, ch, bx0, by0, bs, bs, mvx, mvy, pred);
+    return orc_satd_px(sy + (size_t)by0 * cw + bx0, cw, pred, bs, bs, bs);
+}
+
+/* inter_cost, mbtree.rs:154-197 */
+int32_t orc_inter_cost(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0, int by0,
+                       int bs, int seedx, int seedy, int max_step, int32_t *mv_out)
+{
+    int bmx = 0, bmy = 0;
+    int64_t best = orc_mbtree_mc_satd(sy, cw, ch, ref, bx0, by0, bs, 0, 0);
+    if (best != 0) {
+        if (seedx != 0 || seedy != 0) {
+            int64_t s = orc_mbtree_mc_satd(sy, cw, ch, ref, bx0, by0, bs, seedx, seedy);
+            if (s < best) { best = s; bmx = seedx; bmy = seedy; }
+        }
+        for (int step = max_step; step >= 1; step >>= 1) {
+            for (;;) {
+                int moved = 0;
+                const int ring[4][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step}};
+                for (int i = 0; i < 4; i++) {
+                    int x = bmx + ring[i][0] * 4, y = bmy + ring[i][1] * 4;
+                    int64_t s = orc_mbtree_mc_satd(sy, cw, ch, ref, bx0, by0, bs, x, y);
+                    if (s < best) { best = s; bmx = x; bmy = y; moved = 1; }
+                }
+                if (!moved) break;
+            }
+        }
+    }
+    mv_out[0] = bmx;
+    mv_out[1] = bmy;
+    return best > INT32_MAX ? INT32_MAX : (int32_t)best;
+}
+
+/* Rust i32 `/` truncates toward zero (C99 does too). */
+void orc_frame_costs_half(const uint8_t *half, const uint8_t *ref_half, int mb_w, int mb_h,
+                          int32_t *intra, int32_t *inter, int32_t *mv)
+{
+    int cwh = mb_w * 8, chh = mb_h * 8;
+    for (int y = 0; y < mb_h; y++)
+        for (int x = 0; x < mb_w; x++) {
+            int m = y * mb_w + x;
+            int sfx = 0, sfy = 0;
+            if (x > 0) { sfx = mv[2 * (m - 1)]; sfy = mv[2 * (m - 1) + 1]; }
+            else if (y > 0) { sfx = mv[2 * (m - mb_w)]; sfy = mv[2 * (m - mb_w) + 1]; }
+            int32_t ic = orc_intra_cost(half, cwh, x * 8, y * 8, 8);
+            intra[m] = ic;
+            if (ref_half) {
+                int32_t mvp[2];
+                int32_t c = orc_inter_cost(half, cwh, chh, ref_half, x * 8, y * 8, 8, sfx / 2,
+                                           sfy / 2, 8, mvp);
+                inter[m] = c < ic ? c : ic;
+                mv[2 * m] = mvp[0] * 2;
+                mv[2 * m + 1] = mvp[1] * 2;
+            } else {
+                inter[m] = ic;
+                mv[2 * m] = 0;
+                mv[2 * m + 1] = 0;
+            }
+        }
+}
+
+/* propagate_to, mbtree.rs:268-297 */
+static void propagate_to(double *prev, int mb_w, int mb_h, int mb_x, int mb_y, int mvx, int mvy,
+                         double amount)
+{
+    if (amount <= 0.0) return;
+    int rx = clampi(mb_x * 16 + asr2(mvx), 0, (mb_w - 1) * 16);
+    int ry = clampi(mb_y * 16 + asr2(mvy), 0, (mb_h - 1) * 16);
+    int cx0 = rx / 16, cy0 = ry / 16;
+    double fx = (double)(rx % 16), fy = (double)(ry % 16);
+    double wl = 16.0 - fx, wt = 16.0 - fy;
+    const double wys[2] = {wt, fy}, wxs[2] = {wl, fx};
+    for (int dy = 0; dy < 2; dy++) {
+        if (wys[dy] <= 0.0) continue;
+        int cy = cy0 + dy < mb_h - 1 ? cy0 + dy : mb_h - 1;
+        for (int dx = 0; dx < 2; dx++) {
+            if (wxs[dx] <= 0.0) continue;
+            int cx = cx0 + dx < mb_w - 1 ? cx0 + dx : mb_w - 1;
+            prev[cy * mb_w + cx] += amount * (wxs[dx] * wys[dy]) / 256.0;
+        }
+    }
+}
+
+/* backward propagation, mbtree.rs:440-455 */
+void orc_propagate_f64(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                       const int32_t *mv, double *prop)
+{
+    int nm = mb_w * mb_h;
+    memset(prop, 0, sizeof(double) * (size_t)n * nm);
+    for (int f = n - 1; f >= 1; f--) {
+        const double *cur = prop + (size_t)f * nm;
+        double *prev = prop + (size_t)(f - 1) * nm;
+        for (int y = 0; y < mb_h; y++)
+            for (int x = 0; x < mb_w; x++) {
+                int m = y * mb_w + x;
+                size_t g = (size_t)f * nm + m;
+                double total = (double)intra[g] + cur[m];
+                double frac = (double)(intra[g] - inter[g]) / (double)intra[g];
+                propagate_to(prev, mb_w, mb_h, x, y, mv[2 * g], mv[2 * g + 1], total * frac);
+            }
+    }
+}
+
+/* Build-defined int32 propagation: same geometry as propagate_to, integer amounts.
+ * amount = ((intra + prop) * (intra - inter)) / intra (int64, floor; all terms >= 0),
+ * share = (amount * wx * wy) >> 8, accumulated in int64 and saturated to INT32_MAX. */
+void orc_propagate_i32(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                       const int32_t *mv, int32_t *prop)
+{
+    int nm = mb_w * mb_h;
+    int64_t *acc = (int64_t *)calloc((size_t)nm, sizeof(int64_t));
+    memset(prop, 0, sizeof(int32_t) * (size_t)n * nm);
+    for (int f = n - 1; f >= 1; f--) {
+        memset(acc, 0, sizeof(int64_t) * (size_t)nm);
+        for (int y = 0; y < mb_h; y++)
+            for (int x = 0; x < mb_w; x++) {
+                int m = y * mb_w + x;
+                size_t g = (size_t)f * nm + m;
+                int64_t total = (int64_t)intra[g] + prop[g];
+                int64_t amount = total * (int64_t)(intra[g] - inter[g]) / intra[g];
+                if (amount <= 0) continue;
+                int rx = clampi(x * 16 + asr2(mv[2 * g]), 0, (mb_w - 1) * 16);
+                int ry = clampi(y * 16 + asr2(mv[2 * g + 1]), 0, (mb_h - 1) * 16);
+                int cx0 = rx / 16, cy0 = ry / 16, fx = rx % 16, fy = ry % 16;
+                const int wys[2] = {16 - fy, fy}, wxs[2] = {16 - fx, fx};
+                for (int dy = 0; dy < 2; dy++) {
+                    if (wys[dy] <= 0) continue;
+                    int cy = cy0 + dy < mb_h - 1 ? cy0 + dy : mb_h - 1;
+                    for (int dx = 0; dx < 2; dx++) {
+                        if (wxs[dx] <= 0) continue;
+                        int cx = cx0 + dx < mb_w - 1 ? cx0 + dx : mb_w - 1;
+                        acc[cy * mb_w + cx] += (amount * wxs[dx] * wys[dy]) >> 8;
+                    }
+                }
+            }
+        for (int m = 0; m < nm; m++)
+            prop[(size_t)(f - 1) * nm + m] = acc[m] > INT32_MAX ? INT32_MAX : (int32_t)acc[m];
+    }
+    free(acc);
+}

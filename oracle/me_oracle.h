@@ -1,0 +1,135 @@
+/*
+ * me_oracle.h -- CPU restatement of rusty_h264's encoder motion-search path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * provider in rusty_h264_amd/ and the timed CPU baseline in bench.py.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it,
+ * and only as the checker / the baseline -- never as the thing measured or
+ * shipped.  The product library never links it.
+ *
+ * Every function restates a reference function (file:line, paths relative to
+ * the reference repository root); arithmetic and evaluation order follow the
+ * reference's scalar twins exactly.  Functions marked "build-defined" have no
+ * reference counterpart: they are the plain exhaustive search and the int32
+ * lookahead propagation, pinned by their definitions (brute force, closed
+ * forms) in tests/.
+ *
+ * Plane-set layout shared with the GPU provider: one buffer of 4*pw*ph bytes,
+ * planes in the order F (edge-replicated full-pel), H (b), V (h), C (j), each
+ * pw*ph bytes with row stride pw, pw = cw + 2*pad, ph = ch + 2*pad
+ * (crates/rusty_h264-common/src/inter.rs:707-722).
+ */
+#ifndef RH264_ME_ORACLE_H
+#define RH264_ME_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- block metrics: crates/rusty_h264-accel/src/satd_sad.rs ---------------- */
+int32_t orc_sad(const uint8_t *a, int sa, const uint8_t *b, int sb, int w, int h);
+int32_t orc_satd4x4_wels(const uint8_t *a, int sa, const uint8_t *b, int sb);
+int32_t orc_satd_wels(const uint8_t *a, int sa, const uint8_t *b, int sb, int w, int h);
+void orc_hadamard_4x4(const int32_t in[16], int32_t out[16]);
+int64_t orc_satd_px(const uint8_t *src, int ss, const uint8_t *pred, int ps, int w, int h);
+uint32_t orc_satd_avg(const uint8_t *src, int ss, const uint8_t *a, const uint8_t *b,
+                      int stride, int w, int h);
+int orc_x4_shape(int w, int h);
+
+/* ---- luma interpolation: crates/rusty_h264-common/src/inter.rs ------------- */
+int orc_luma_sample(const uint8_t *ref, int cw, int ch, int ix, int iy, int fx, int fy);
+void orc_mc_luma(const uint8_t *ref, int cw, int ch, int x0, int y0, int bw, int bh,
+                 int mvx, int mvy, uint8_t *out);
+void orc_pad_plane(const uint8_t *src, int w, int h, int pad, uint8_t *out);
+void orc_build_hpel_tiles(const uint8_t *f, int pw, int ph, uint8_t *h, uint8_t *v, uint8_t *c);
+void orc_build_hpel_fused(const uint8_t *f, int pw, int ph, uint8_t *h, uint8_t *v, uint8_t *c);
+/* planes: 4*pw*ph bytes (F,H,V,C); fused=0 uses the tile walk, 1 the fused pass */
+void orc_build_hpel_planes(const uint8_t *y, int cw, int ch, int pad, int fused, uint8_t *planes);
+/* hpel_block: 1 = accepted (out filled), 0 = declined (inter.rs:1107-1162) */
+int orc_hpel_block(const uint8_t *planes, int cw, int ch, int pad, int x0, int y0, int bw, int bh,
+                   int mvx, int mvy, uint8_t *out);
+/* prediction read from the plane set with every coordinate clamped into the
+ * padded planes; equals orc_mc_luma for any MV when pad >= 3 (build-defined
+ * read rule the GPU uses; pinned against orc_mc_luma in tests). */
+void orc_pred_block(const uint8_t *planes, int cw, int ch, int pad, int x0, int y0, int bw,
+                    int bh, int mvx, int mvy, uint8_t *out);
+
+/* ---- motion search ---------------------------------------------------------- */
+/* Exp-Golomb se(d) length, mb16.rs:2474-2475 */
+uint32_t orc_mvbits(int32_t d);
+/* lambda-rate table: lr[b] = (int64)(lambda * b), b = 0..n-1 (mb16.rs:2563) */
+void orc_lambda_table(double lambda, int n, int64_t *lr);
+
+#define ORC_LR_N 128
+
+typedef struct {
+    int cw, ch, pad;   /* frame size (MB multiples) and plane pad              */
+    int bw, bh;        /* partition shape                                      */
+    int range;         /* full-pel search radius R (full search)               */
+    int subpel_pat;    /* 0: ring8 iterate, 1: ring4 iterate, 2: ring8 single,
+                          3: ring4 single, -1: no sub-pel (mb16.rs:2956-3291)  */
+    const int64_t *lr_fp;  /* SAD-phase lambda table, ORC_LR_N entries        */
+    const int64_t *lr;     /* SATD-phase lambda table, ORC_LR_N entries       */
+} orc_search_cfg;
+
+/* Plain exhaustive integer search of one block (build-defined; raster
+ * first-min as mb16.rs:2843-2855).  centre = (cx,cy) qpel multiple of 4,
+ * mvp = rate predictor (qpel).  Returns best cost, writes mv (qpel). */
+int64_t orc_full_search_block(const orc_search_cfg *cfg, const uint8_t *cur,
+                              const uint8_t *planes, int lx, int ly, int cx, int cy,
+                              int mvpx, int mvpy, int32_t *mv_out);
+/* SATD-domain cost of one candidate (mb16.rs:2540-2565 with the sub-pel ladder
+ * values of mb16.rs:1930-2007). */
+int64_t orc_satd_cost(const orc_search_cfg *cfg, const uint8_t *cur, const uint8_t *planes,
+                      int lx, int ly, int mvx, int mvy, int mvpx, int mvpy);
+/* Sub-pel ring refinement (mb16.rs:2956-3291, cascade path). */
+int64_t orc_subpel_refine_block(const orc_search_cfg *cfg, const uint8_t *cur,
+                                const uint8_t *planes, int lx, int ly, int mvx, int mvy,
+                                int mvpx, int mvpy, int32_t *mv_out, int32_t *evals);
+
+/* Whole-frame plain search: every bw x bh block of the frame, independent.
+ * centres/mvps: per-block int32 pairs (qpel) or NULL for zero.
+ * outputs per block: int_mv[2], int_cost, mv[2], cost. nthreads>1 uses OpenMP. */
+void orc_plain_search_frame(const orc_search_cfg *cfg, const uint8_t *cur, const uint8_t *planes,
+                            const int32_t *centres, const int32_t *mvps, int32_t *int_mv,
+                            int64_t *int_cost, int32_t *mv, int64_t *cost, int nthreads);
+
+/* Reference-mode motion_search (mb16.rs:2433-3302) with the default knobs
+ * made explicit: seeds (0,0)+predictors, integer snap, diamond rungs (qpel
+ * step list, cascade walk), optional SAD full-pel phase (B2), sub-pel rings.
+ * Returns the cost, writes the MV. */
+typedef struct {
+    const int32_t *steps; int nsteps;   /* diamond rungs in qpel units, e.g. {16,8,4} */
+    int sadfp;                           /* 1: SAD full-pel phase with lr_fp          */
+    int snap;                            /* integer snap of the seed (me_snap)         */
+} orc_diamond_cfg;
+int64_t orc_motion_search_ref(const orc_search_cfg *cfg, const orc_diamond_cfg *dc,
+                              const uint8_t *cur, const uint8_t *planes, int lx, int ly,
+                              const int32_t *preds, int npreds, int32_t *mv_out);
+
+/* ---- mb-tree lookahead: crates/rusty_h264-encoder/src/mbtree.rs ------------- */
+void orc_downsample2x(const uint8_t *y, int cw, int ch, uint8_t *out);
+int32_t orc_intra_cost(const uint8_t *sy, int cw, int bx0, int by0, int bs);
+int64_t orc_mbtree_mc_satd(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0,
+                           int by0, int bs, int mvx, int mvy);
+int32_t orc_inter_cost(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0, int by0,
+                       int bs, int seedx, int seedy, int max_step, int32_t *mv_out);
+/* HalfRes frame costs: half/ref_half are (mb_w*8)x(mb_h*8); ref_half NULL = IDR.
+ * out: intra[mb], inter[mb], mv[2*mb] (full-res qpel). mbtree.rs:208-263 */
+void orc_frame_costs_half(const uint8_t *half, const uint8_t *ref_half, int mb_w, int mb_h,
+                          int32_t *intra, int32_t *inter, int32_t *mv);
+/* f64 backward propagation over n frames (mbtree.rs:268-297, 440-455).
+ * costs laid out [n][mb]; prop_out [n][mb] f64. */
+void orc_propagate_f64(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                       const int32_t *mv, double *prop_out);
+/* int32 fixed-point propagation (build-defined): amount_q = floor((intra+prop)
+ * * (intra-inter) / intra) in int64 then per-target weights (wx*wy) >> 8. */
+void orc_propagate_i32(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                       const int32_t *mv, int32_t *prop_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

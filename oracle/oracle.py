@@ -1,0 +1,211 @@
+"""ctypes binding of the CPU oracle (oracle/me_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the parity checker / the timed CPU baseline.
+The product (rusty_h264_amd) never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LR_N = 128
+
+u8p = C.POINTER(C.c_uint8)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+
+
+class SearchCfg(C.Structure):
+    _fields_ = [("cw", C.c_int), ("ch", C.c_int), ("pad", C.c_int), ("bw", C.c_int), ("bh", C.c_int),
+                ("range", C.c_int), ("subpel_pat", C.c_int), ("lr_fp", i64p), ("lr", i64p)]
+
+
+class DiamondCfg(C.Structure):
+    _fields_ = [("steps", i32p), ("nsteps", C.c_int), ("sadfp", C.c_int), ("snap", C.c_int)]
+
+
+def build(fast=False):
+    """Compile the oracle with its Makefile (gcc; seconds)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_libs = {}
+
+
+def load(fast=False):
+    name = "liboracle_v3.so" if fast else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    _libs[name] = lib
+    return lib
+
+
+def _p(a, t=u8p):
+    return a.ctypes.data_as(t)
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+class Oracle:
+    """Thin numpy-facing wrapper over the C restatement."""
+
+    def __init__(self, fast=False):
+        self.lib = load(fast)
+        L = self.lib
+        for n in ("orc_sad", "orc_satd4x4_wels", "orc_satd_wels", "orc_intra_cost"):
+            getattr(L, n).restype = C.c_int32
+        for n in ("orc_satd_px", "orc_full_search_block", "orc_satd_cost", "orc_subpel_refine_block",
+                  "orc_motion_search_ref", "orc_mbtree_mc_satd"):
+            getattr(L, n).restype = C.c_int64
+        L.orc_satd_avg.restype = C.c_uint32
+        L.orc_mvbits.restype = C.c_uint32
+        L.orc_inter_cost.restype = C.c_int32
+
+    # ---- block metrics
+    def sad(self, a, sa, b, sb, w, h):
+        return self.lib.orc_sad(_p(_u8(a)), sa, _p(_u8(b)), sb, w, h)
+
+    def satd_wels(self, a, sa, b, sb, w, h):
+        return self.lib.orc_satd_wels(_p(_u8(a)), sa, _p(_u8(b)), sb, w, h)
+
+    def satd_px(self, a, sa, b, sb, w, h):
+        return self.lib.orc_satd_px(_p(_u8(a)), sa, _p(_u8(b)), sb, w, h)
+
+    def satd_avg(self, src, ss, a, b, stride, w, h):
+        return self.lib.orc_satd_avg(_p(_u8(src)), ss, _p(_u8(a)), _p(_u8(b)), stride, w, h)
+
+    def hadamard_4x4(self, blk):
+        i = np.ascontiguousarray(blk, np.int32).reshape(16)
+        o = np.empty(16, np.int32)
+        self.lib.orc_hadamard_4x4(_p(i, i32p), _p(o, i32p))
+        return o
+
+    # ---- interpolation
+    def luma_sample(self, ref, cw, ch, ix, iy, fx, fy):
+        return self.lib.orc_luma_sample(_p(_u8(ref)), cw, ch, ix, iy, fx, fy)
+
+    def mc_luma(self, ref, cw, ch, x0, y0, bw, bh, mvx, mvy):
+        out = np.empty((bh, bw), np.uint8)
+        self.lib.orc_mc_luma(_p(_u8(ref)), cw, ch, x0, y0, bw, bh, mvx, mvy, _p(out))
+        return out
+
+    def planes(self, y, cw, ch, pad=32, fused=False):
+        pw, ph = cw + 2 * pad, ch + 2 * pad
+        out = np.empty(4 * pw * ph, np.uint8)
+        self.lib.orc_build_hpel_planes(_p(_u8(y)), cw, ch, pad, int(fused), _p(out))
+        return out.reshape(4, ph, pw)
+
+    def hpel_block(self, planes, cw, ch, pad, x0, y0, bw, bh, mvx, mvy):
+        out = np.zeros((bh, bw), np.uint8)
+        ok = self.lib.orc_hpel_block(_p(_u8(planes)), cw, ch, pad, x0, y0, bw, bh, mvx, mvy, _p(out))
+        return out if ok else None
+
+    def pred_block(self, planes, cw, ch, pad, x0, y0, bw, bh, mvx, mvy):
+        out = np.empty((bh, bw), np.uint8)
+        self.lib.orc_pred_block(_p(_u8(planes)), cw, ch, pad, x0, y0, bw, bh, mvx, mvy, _p(out))
+        return out
+
+    # ---- search
+    def mvbits(self, d):
+        return self.lib.orc_mvbits(int(d))
+
+    def lambda_table(self, lam):
+        lr = np.empty(LR_N, np.int64)
+        self.lib.orc_lambda_table(C.c_double(lam), LR_N, _p(lr, i64p))
+        return lr
+
+    def cfg(self, cw, ch, pad=32, bw=16, bh=16, search_range=32, subpel_pat=0, lam=0.0, lam_fp=None):
+        lr = self.lambda_table(lam)
+        lr_fp = self.lambda_table(lam * 0.5 if lam_fp is None else lam_fp)
+        c = SearchCfg(cw, ch, pad, bw, bh, search_range, subpel_pat, _p(lr_fp, i64p), _p(lr, i64p))
+        c._keep = (lr, lr_fp)
+        return c
+
+    def full_search_block(self, cfg, cur, planes, lx, ly, cx=0, cy=0, mvpx=0, mvpy=0):
+        mv = np.empty(2, np.int32)
+        c = self.lib.orc_full_search_block(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), lx, ly, cx, cy,
+                                           mvpx, mvpy, _p(mv, i32p))
+        return (int(mv[0]), int(mv[1])), int(c)
+
+    def satd_cost(self, cfg, cur, planes, lx, ly, mvx, mvy, mvpx=0, mvpy=0):
+        return int(self.lib.orc_satd_cost(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), lx, ly, mvx, mvy,
+                                          mvpx, mvpy))
+
+    def subpel_refine_block(self, cfg, cur, planes, lx, ly, mvx, mvy, mvpx=0, mvpy=0):
+        mv = np.empty(2, np.int32)
+        ev = C.c_int32()
+        c = self.lib.orc_subpel_refine_block(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), lx, ly, mvx, mvy,
+                                             mvpx, mvpy, _p(mv, i32p), C.byref(ev))
+        return (int(mv[0]), int(mv[1])), int(c), ev.value
+
+    def plain_search_frame(self, cfg, cur, planes, centres=None, mvps=None, nthreads=1):
+        nb = (cfg.cw // cfg.bw) * (cfg.ch // cfg.bh)
+        imv = np.empty((nb, 2), np.int32)
+        ic = np.empty(nb, np.int64)
+        mv = np.empty((nb, 2), np.int32)
+        c = np.empty(nb, np.int64)
+        ce = _p(np.ascontiguousarray(centres, np.int32), i32p) if centres is not None else None
+        mp = _p(np.ascontiguousarray(mvps, np.int32), i32p) if mvps is not None else None
+        self.lib.orc_plain_search_frame(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), ce, mp, _p(imv, i32p),
+                                        _p(ic, i64p), _p(mv, i32p), _p(c, i64p), nthreads)
+        return {"int_mv": imv, "int_cost": ic, "mv": mv, "cost": c}
+
+    def motion_search_ref(self, cfg, cur, planes, lx, ly, preds, steps=(16, 8, 4), sadfp=False, snap=True):
+        st = np.ascontiguousarray(steps, np.int32)
+        dc = DiamondCfg(_p(st, i32p), len(st), int(sadfp), int(snap))
+        pr = np.ascontiguousarray(preds, np.int32).reshape(-1)
+        mv = np.empty(2, np.int32)
+        c = self.lib.orc_motion_search_ref(C.byref(cfg), C.byref(dc), _p(_u8(cur)), _p(_u8(planes)), lx, ly,
+                                           _p(pr, i32p), len(pr) // 2, _p(mv, i32p))
+        return (int(mv[0]), int(mv[1])), int(c)
+
+    # ---- mb-tree
+    def downsample2x(self, y, cw, ch):
+        out = np.empty((ch // 2, cw // 2), np.uint8)
+        self.lib.orc_downsample2x(_p(_u8(y)), cw, ch, _p(out))
+        return out
+
+    def intra_cost(self, sy, cw, bx0, by0, bs):
+        return self.lib.orc_intra_cost(_p(_u8(sy)), cw, bx0, by0, bs)
+
+    def inter_cost(self, sy, cw, ch, ref, bx0, by0, bs, seed=(0, 0), max_step=8):
+        mv = np.empty(2, np.int32)
+        c = self.lib.orc_inter_cost(_p(_u8(sy)), cw, ch, _p(_u8(ref)), bx0, by0, bs, seed[0], seed[1], max_step,
+                                    _p(mv, i32p))
+        return int(c), (int(mv[0]), int(mv[1]))
+
+    def frame_costs_half(self, half, ref_half, mb_w, mb_h):
+        n = mb_w * mb_h
+        intra = np.empty(n, np.int32)
+        inter = np.empty(n, np.int32)
+        mv = np.zeros((n, 2), np.int32)
+        rp = _p(_u8(ref_half)) if ref_half is not None else None
+        self.lib.orc_frame_costs_half(_p(_u8(half)), rp, mb_w, mb_h, _p(intra, i32p), _p(inter, i32p),
+                                      _p(mv, i32p))
+        return intra, inter, mv
+
+    def propagate_f64(self, intra, inter, mv, mb_w, mb_h):
+        n = intra.shape[0]
+        out = np.empty((n, mb_w * mb_h), np.float64)
+        self.lib.orc_propagate_f64(n, mb_w, mb_h, _p(np.ascontiguousarray(intra, np.int32), i32p),
+                                   _p(np.ascontiguousarray(inter, np.int32), i32p),
+                                   _p(np.ascontiguousarray(mv, np.int32), i32p),
+                                   out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def propagate_i32(self, intra, inter, mv, mb_w, mb_h):
+        n = intra.shape[0]
+        out = np.empty((n, mb_w * mb_h), np.int32)
+        self.lib.orc_propagate_i32(n, mb_w, mb_h, _p(np.ascontiguousarray(intra, np.int32), i32p),
+                                   _p(np.ascontiguousarray(inter, np.int32), i32p),
+                                   _p(np.ascontiguousarray(mv, np.int32), i32p), _p(out, i32p))
+        return out

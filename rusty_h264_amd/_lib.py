@@ -1,0 +1,79 @@
+"""ctypes binding of the C ABI in include/rh264_me.h (librh264_me.so).
+
+The library is the product: every function here calls a HIP kernel on the
+GPU.  There is no CPU fallback -- if the shared library is missing or no GPU is
+present, calls raise.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librh264_me.so")
+LR_N = 128
+
+c_u8p = C.POINTER(C.c_uint8)
+c_i32p = C.POINTER(C.c_int32)
+c_i64p = C.POINTER(C.c_int64)
+c_vp = C.c_void_p
+
+
+class SearchParams(C.Structure):
+    """rh264_search_params (include/rh264_me.h)."""
+    _fields_ = [
+        ("cw", C.c_int32), ("ch", C.c_int32), ("pad", C.c_int32),
+        ("bw", C.c_int32), ("bh", C.c_int32), ("range", C.c_int32),
+        ("subpel_pat", C.c_int32),
+        ("lr_fp", C.c_int32 * LR_N), ("lr", C.c_int32 * LR_N),
+    ]
+
+
+# name -> (restype, argtypes); mirrors include/rh264_me.h one to one
+PROTOTYPES = {
+    "rh264_init": (C.c_int, [C.c_int, C.POINTER(c_vp)]),
+    "rh264_destroy": (C.c_int, [c_vp]),
+    "rh264_last_error": (C.c_char_p, []),
+    "rh264_device_name": (C.c_int, [c_vp, C.c_char_p, C.c_int]),
+    "rh264_alloc": (C.c_int, [c_vp, C.c_size_t, C.POINTER(c_vp)]),
+    "rh264_free": (C.c_int, [c_vp, c_vp]),
+    "rh264_h2d": (C.c_int, [c_vp, c_vp, c_vp, C.c_size_t]),
+    "rh264_d2h": (C.c_int, [c_vp, c_vp, c_vp, C.c_size_t]),
+    "rh264_memset": (C.c_int, [c_vp, c_vp, C.c_int, C.c_size_t]),
+    "rh264_sync": (C.c_int, [c_vp]),
+    "rh264_event_record": (C.c_int, [c_vp, C.c_int]),
+    "rh264_event_elapsed_ms": (C.c_int, [c_vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "rh264_block_metric": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.c_int, c_vp, C.c_int, c_vp,
+                                     c_vp, C.c_int, c_vp, c_vp, c_vp]),
+    "rh264_planes_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "rh264_build_planes": (C.c_int, [c_vp, c_vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, c_vp]),
+    "rh264_lambda_table": (None, [C.c_double, c_i32p]),
+    "rh264_full_search": (C.c_int, [c_vp, C.POINTER(SearchParams), C.c_int, c_vp, C.c_size_t, c_vp,
+                                    C.c_size_t, c_vp, c_vp, c_vp, c_vp]),
+    "rh264_subpel_refine": (C.c_int, [c_vp, C.POINTER(SearchParams), C.c_int, c_vp, C.c_size_t, c_vp,
+                                      C.c_size_t, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "rh264_plain_search": (C.c_int, [c_vp, C.POINTER(SearchParams), C.c_int, c_vp, C.c_size_t, c_vp,
+                                     C.c_size_t, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "rh264_set_profiling": (C.c_int, [c_vp, C.c_int]),
+    "rh264_profile_count": (C.c_int, [c_vp]),
+    "rh264_profile_get": (C.c_int, [c_vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "rh264_predict_blocks": (C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       c_vp, c_vp, c_vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load librh264_me.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                           "(the HIP provider has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

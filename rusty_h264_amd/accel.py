@@ -1,0 +1,276 @@
+"""Host-side mirror of the rusty_h264-accel motion-search interface over the
+HIP provider (include/rh264_me.h).
+
+The reference's accel crate exposes per-block kernels (crates/rusty_h264-accel/
+src/lib.rs:30-58) that the encoder calls one block at a time.  Here the same
+operations are served in batches on the GPU: `block_metric` covers the
+sad/satd/satd_avg family, `build_planes` the plane cache (`build_hpel_planes`,
+crates/rusty_h264-common/src/inter.rs:816-843), and `MotionSearch` the search
+itself (full search, sub-pel ring refinement).  Return values use the same
+domains as the reference: SAD, Wels SATD (per-4x4 rounded) or the encoder's
+unhalved sum |H d|.
+
+Declined calls (return code 1, an unsupported shape or geometry) raise
+`Declined`: the caller then runs its own path, as the reference does when an
+accel kernel returns None.  Errors raise RuntimeError.  Nothing here computes on
+the CPU.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+
+LR_N = _lib.LR_N
+METRIC_SAD, METRIC_SATD_WELS, METRIC_SATD, METRIC_SATD_AVG = 0, 1, 2, 3
+SHAPES = ((16, 16), (16, 8), (8, 16), (8, 8), (8, 4), (4, 8), (4, 4))
+HPEL_PAD_DEFAULT = 32  # inter.rs:761
+
+
+class Declined(Exception):
+    """The provider does not cover this shape/geometry (accel `None`)."""
+
+
+def _check(rc, what):
+    if rc == 0:
+        return
+    if rc == 1:
+        raise Declined(what)
+    raise RuntimeError(f"{what}: {_lib.load().rh264_last_error().decode()}")
+
+
+def lambda_me(qp, scale=1.0):
+    """Motion-search lambda: sqrt(0.85 * 2^((qp-12)/3)) * scale (mb16.rs:8642, 8838)."""
+    return math.sqrt(0.85 * 2.0 ** ((qp - 12) / 3.0)) * scale
+
+
+def lambda_table(lam):
+    """lr[b] = (int64)(lam * b): `(lam * rate as f64) as i64`, mb16.rs:2563."""
+    out = (C.c_int32 * LR_N)()
+    _lib.load().rh264_lambda_table(float(lam), out)
+    return np.frombuffer(out, dtype=np.int32).copy()
+
+
+class DevBuf:
+    """A device allocation owned by a Device."""
+
+    def __init__(self, dev, nbytes):
+        self.dev = dev
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(dev.lib.rh264_alloc(dev.ctx, self.nbytes, C.byref(p)), "rh264_alloc")
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr:
+            _check(self.dev.lib.rh264_free(self.dev.ctx, self.ptr), "rh264_free")
+            self.ptr = None
+
+    def __int__(self):
+        return self.ptr
+
+
+class Device:
+    """One GPU context (rh264_ctx): a stream, timing events and scratch."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        ctx = C.c_void_p()
+        _check(self.lib.rh264_init(device, C.byref(ctx)), "rh264_init")
+        self.ctx = ctx.value
+        self._bufs = []
+
+    def name(self):
+        buf = C.create_string_buffer(256)
+        _check(self.lib.rh264_device_name(self.ctx, buf, 256), "rh264_device_name")
+        return buf.value.decode()
+
+    def alloc(self, nbytes):
+        b = DevBuf(self, nbytes)
+        self._bufs.append(b)
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = self.alloc(arr.nbytes)
+        _check(self.lib.rh264_h2d(self.ctx, b.ptr, arr.ctypes.data, arr.nbytes), "rh264_h2d")
+        return b
+
+    def download(self, buf, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        _check(self.lib.rh264_d2h(self.ctx, out.ctypes.data, int(buf), out.nbytes), "rh264_d2h")
+        return out
+
+    def sync(self):
+        _check(self.lib.rh264_sync(self.ctx), "rh264_sync")
+
+    def record(self, slot):
+        _check(self.lib.rh264_event_record(self.ctx, slot), "rh264_event_record")
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        _check(self.lib.rh264_event_elapsed_ms(self.ctx, a, b, C.byref(ms)), "rh264_event_elapsed_ms")
+        return ms.value
+
+    def close(self):
+        for b in self._bufs:
+            b.free()
+        self._bufs = []
+        if self.ctx:
+            self.lib.rh264_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    return int(x)
+
+
+def block_metric(dev, kind, w, h, src, ss, src_off, ref, rs, ref_off, ref_off2=None):
+    """Batched accel block metrics on host arrays.
+
+    src/ref are flat uint8 host arrays; src_off/ref_off(/ref_off2) int64 offsets
+    of each block's top-left sample.  Returns int32 per block:
+    SAD (satd_sad.rs:74-82), Wels SATD (satd_sad.rs:39-72), sum|H d| (satd_x4,
+    satd_avg.rs:388-414) or the fused average form (satd_avg, satd_avg.rs:579-602).
+    """
+    n = len(src_off)
+    dsrc, dref = dev.upload(np.asarray(src, np.uint8)), dev.upload(np.asarray(ref, np.uint8))
+    dso = dev.upload(np.asarray(src_off, np.int64))
+    dro = dev.upload(np.asarray(ref_off, np.int64))
+    dro2 = dev.upload(np.asarray(ref_off2, np.int64)) if ref_off2 is not None else None
+    out = dev.alloc(4 * max(n, 1))
+    try:
+        _check(dev.lib.rh264_block_metric(dev.ctx, kind, w, h, n, dsrc.ptr, ss, dso.ptr, dref.ptr, rs,
+                                          dro.ptr, _ptr(dro2), out.ptr), "rh264_block_metric")
+        return dev.download(out, (n,), np.int32)
+    finally:
+        for b in (dsrc, dref, dso, dro, dro2, out):
+            if b is not None:
+                b.free()
+
+
+def sad_x4(dev, src, ss, base, offs, rs, w, h):
+    """sad_x4 (satd_avg.rs:462-492): one source block vs four offsets into `base`."""
+    return block_metric(dev, METRIC_SAD, w, h, src, ss, [0] * 4, base, rs, list(offs))
+
+
+def satd_x4(dev, src, ss, base, offs, rs, w, h):
+    """satd_x4 (satd_avg.rs:388-414): unhalved sum|H d| vs four offsets."""
+    return block_metric(dev, METRIC_SATD, w, h, src, ss, [0] * 4, base, rs, list(offs))
+
+
+def satd_avg_x4(dev, src, ss, plane, offs_a, offs_b, rs, w, h):
+    """satd_avg_x4 (satd_avg.rs:536-571) with all operands inside one plane buffer."""
+    return block_metric(dev, METRIC_SATD_AVG, w, h, src, ss, [0] * len(offs_a), plane, rs,
+                        list(offs_a), list(offs_b))
+
+
+def planes_bytes(cw, ch, pad=HPEL_PAD_DEFAULT):
+    return int(_lib.load().rh264_planes_bytes(cw, ch, pad))
+
+
+def build_planes(dev, frames_buf, nframes, cw, ch, pad=HPEL_PAD_DEFAULT, out=None):
+    """Plane sets (F,H,V,C) for `nframes` device frames of cw x ch -> DevBuf."""
+    pb = planes_bytes(cw, ch, pad)
+    if out is None:
+        out = dev.alloc(pb * max(nframes, 1))
+    _check(dev.lib.rh264_build_planes(dev.ctx, int(frames_buf), cw * ch, nframes, cw, ch, pad, int(out)),
+           "rh264_build_planes")
+    return out
+
+
+def search_params(cw, ch, pad=HPEL_PAD_DEFAULT, bw=16, bh=16, search_range=32, subpel_pat=0,
+                  lam=0.0, lam_fp=None):
+    """rh264_search_params; lam prices the SATD phase, lam_fp (default lam*0.5, the
+    reference's RFF_ME_SADL default, mb16.rs:756-763) the SAD full-pel phase."""
+    p = _lib.SearchParams()
+    p.cw, p.ch, p.pad, p.bw, p.bh, p.range, p.subpel_pat = cw, ch, pad, bw, bh, search_range, subpel_pat
+    lr = lambda_table(lam)
+    lr_fp = lambda_table(lam * 0.5 if lam_fp is None else lam_fp)
+    for i in range(LR_N):
+        p.lr[i] = int(lr[i])
+        p.lr_fp[i] = int(lr_fp[i])
+    return p
+
+
+class MotionSearch:
+    """The plain motion search over device-resident frames.
+
+    Results are per block, frame-major: int_mv / mv (N, 2) int32 quarter-pel,
+    int_cost / cost (N,) int32.
+    """
+
+    def __init__(self, dev, params):
+        self.dev = dev
+        self.p = params
+        self.nblk = (params.cw // params.bw) * (params.ch // params.bh)
+        self.pbytes = planes_bytes(params.cw, params.ch, params.pad)
+        self._out = {}
+
+    def _outputs(self, nframes):
+        n = self.nblk * nframes
+        if self._out.get("n", 0) < n:
+            for b in self._out.get("bufs", []):
+                b.free()
+            bufs = [self.dev.alloc(8 * n), self.dev.alloc(4 * n), self.dev.alloc(8 * n),
+                    self.dev.alloc(4 * n), self.dev.alloc(4 * n)]
+            self._out = {"n": n, "bufs": bufs}
+        return self._out["bufs"]
+
+    def run_plain(self, cur_buf, planes_buf, nframes, centres=None, mvps=None, cur_stride=None,
+                  planes_stride=None):
+        """Full search + sub-pel refine (rh264_plain_search); asynchronous."""
+        imv, icost, mv, cost, _ = self._outputs(nframes)
+        _check(self.dev.lib.rh264_plain_search(
+            self.dev.ctx, C.byref(self.p), nframes, int(cur_buf),
+            cur_stride or self.p.cw * self.p.ch, int(planes_buf), planes_stride or self.pbytes,
+            _ptr(centres), _ptr(mvps), imv.ptr, icost.ptr, mv.ptr, cost.ptr), "rh264_plain_search")
+
+    def run_full(self, cur_buf, planes_buf, nframes, centres=None, mvps=None):
+        imv, icost, _, _, _ = self._outputs(nframes)
+        _check(self.dev.lib.rh264_full_search(
+            self.dev.ctx, C.byref(self.p), nframes, int(cur_buf), self.p.cw * self.p.ch,
+            int(planes_buf), self.pbytes, _ptr(centres), _ptr(mvps), imv.ptr, icost.ptr),
+            "rh264_full_search")
+
+    def run_refine(self, cur_buf, planes_buf, nframes, start_mv, mvps=None):
+        _, _, mv, cost, evals = self._outputs(nframes)
+        _check(self.dev.lib.rh264_subpel_refine(
+            self.dev.ctx, C.byref(self.p), nframes, int(cur_buf), self.p.cw * self.p.ch,
+            int(planes_buf), self.pbytes, int(start_mv), _ptr(mvps), mv.ptr, cost.ptr, evals.ptr),
+            "rh264_subpel_refine")
+
+    def results(self, nframes, with_evals=False):
+        n = self.nblk * nframes
+        imv, icost, mv, cost, evals = self._outputs(nframes)
+        d = self.dev
+        r = {"int_mv": d.download(imv, (n, 2), np.int32), "int_cost": d.download(icost, (n,), np.int32),
+             "mv": d.download(mv, (n, 2), np.int32), "cost": d.download(cost, (n,), np.int32)}
+        if with_evals:
+            r["evals"] = d.download(evals, (n,), np.int32)
+        return r
+
+
+def predict_blocks(dev, planes_buf, cw, ch, pad, bw, bh, xy, mv):
+    """Quarter-pel prediction (mc_luma values, inter.rs:1193-1262) for n blocks."""
+    xy = np.ascontiguousarray(xy, np.int32)
+    mv = np.ascontiguousarray(mv, np.int32)
+    n = len(xy)
+    dxy, dmv = dev.upload(xy), dev.upload(mv)
+    out = dev.alloc(max(n * bw * bh, 1))
+    try:
+        _check(dev.lib.rh264_predict_blocks(dev.ctx, int(planes_buf), cw, ch, pad, bw, bh, n, dxy.ptr,
+                                            dmv.ptr, out.ptr), "rh264_predict_blocks")
+        return dev.download(out, (n, bh, bw), np.uint8)
+    finally:
+        for b in (dxy, dmv, out):
+            b.free()

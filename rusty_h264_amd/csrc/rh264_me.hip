@@ -1,0 +1,1006 @@
+// rh264_me.hip -- MI355X (gfx950) kernels and C ABI for rusty_h264's encoder
+// motion-search hot path.  Declarations and the contract: include/rh264_me.h.
+//
+// Kernels (one launch per stage per batch of frames):
+//   k_build_planes   F/H/V/C plane sets (inter.rs:786-987, hpel.rs:34-203)
+//   k_fill_u64       clears the per-block argmin keys (a kernel, not a memset,
+//                    so a captured graph replays it correctly)
+//   k_full_search    exhaustive integer SAD, quad-SAD (v_qsad_pk_u16_u8) over a
+//                    vertical ring of accumulators, segmented wave argmin +
+//                    64-bit atomicMin on (cost << 32 | raster index)
+//   k_subpel_refine  sub-pel ring walk (mb16.rs:2956-3291), one lane group per
+//                    block, 4x4 Hadamards across quads with DPP
+//   k_block_metric   the accel per-block family, batched
+//   k_predict        qpel prediction of blocks from a plane set
+//
+// All arithmetic is integer and exact.
+
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rh264_me.h"
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+static int fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+static int fail(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return -1;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail("%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+#define LAUNCH_CHECK(name)                                                                    \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess) return fail("launch of %s failed: %s", name, hipGetErrorString(e_)); \
+    } while (0)
+
+struct rh264_ctx {
+    int device;
+    hipStream_t stream;
+    hipEvent_t ev[16];
+    uint64_t *keys;      // per-block argmin keys of the full search
+    size_t keys_cap;     // in elements
+    int32_t *tmp_mv;     // integer-stage MVs when the caller does not want them
+    size_t tmp_mv_cap;   // in int32 elements
+    std::vector<hipEvent_t> prof; // 3 events per profiled plain search
+    int prof_cap, prof_n;         // capacity and count of profiled searches
+};
+
+static int ensure_keys(rh264_ctx *ctx, size_t n)
+{
+    if (ctx->keys_cap >= n) return 0;
+    if (ctx->keys) HIP_TRY(hipFree(ctx->keys));
+    ctx->keys = nullptr;
+    ctx->keys_cap = 0;
+    HIP_TRY(hipMalloc(&ctx->keys, n * sizeof(uint64_t)));
+    ctx->keys_cap = n;
+    return 0;
+}
+
+static int ensure_tmp_mv(rh264_ctx *ctx, size_t n)
+{
+    if (ctx->tmp_mv_cap >= n) return 0;
+    if (ctx->tmp_mv) HIP_TRY(hipFree(ctx->tmp_mv));
+    ctx->tmp_mv = nullptr;
+    ctx->tmp_mv_cap = 0;
+    HIP_TRY(hipMalloc(&ctx->tmp_mv, n * sizeof(int32_t)));
+    ctx->tmp_mv_cap = n;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+__device__ __forceinline__ int clip_u8(int v) { return min(max(v, 0), 255); }
+
+// Exp-Golomb se(d) length, mb16.rs:2474-2475: 1 + 2*floor(log2(codenum+1)).
+// |d| is clamped to 2^24 quarter pels (far beyond any coded MV) so 2d-1 cannot wrap.
+__device__ __forceinline__ int mvbits(int d)
+{
+    d = clampi(d, -(1 << 24), 1 << 24);
+    unsigned codenum = d > 0 ? (unsigned)(2 * d - 1) : (unsigned)(-2 * d);
+    return 1 + 2 * (31 - __clz((int)(codenum + 1u)));
+}
+
+// Quarter-phase operand table, inter.rs:1079-1092 / mectx.rs:143-156:
+// for idx = fx*4+fy, (plane_a, ax, ay, plane_b, bx, by) packed 4 bits each.
+// Planes: 0=F, 1=H, 2=V, 3=C.  Even phases use one plane (in plane_a).
+__device__ __forceinline__ uint32_t qtab(int idx)
+{
+#define Q(pa, ax, ay, pb, bx, by) \
+    ((uint32_t)(pa) | ((uint32_t)(ax) << 4) | ((uint32_t)(ay) << 8) | ((uint32_t)(pb) << 12) | \
+     ((uint32_t)(bx) << 16) | ((uint32_t)(by) << 20))
+    switch (idx) {
+    case 0 * 4 + 0: return Q(0, 0, 0, 0, 0, 0);
+    case 2 * 4 + 0: return Q(1, 0, 0, 1, 0, 0);
+    case 0 * 4 + 2: return Q(2, 0, 0, 2, 0, 0);
+    case 2 * 4 + 2: return Q(3, 0, 0, 3, 0, 0);
+    case 1 * 4 + 0: return Q(0, 0, 0, 1, 0, 0);
+    case 3 * 4 + 0: return Q(0, 1, 0, 1, 0, 0);
+    case 0 * 4 + 1: return Q(0, 0, 0, 2, 0, 0);
+    case 0 * 4 + 3: return Q(0, 0, 1, 2, 0, 0);
+    case 1 * 4 + 1: return Q(1, 0, 0, 2, 0, 0);
+    case 3 * 4 + 1: return Q(1, 0, 0, 2, 1, 0);
+    case 1 * 4 + 3: return Q(1, 0, 1, 2, 0, 0);
+    case 3 * 4 + 3: return Q(1, 0, 1, 2, 1, 0);
+    case 2 * 4 + 1: return Q(1, 0, 0, 3, 0, 0);
+    case 2 * 4 + 3: return Q(1, 0, 1, 3, 0, 0);
+    case 1 * 4 + 2: return Q(2, 0, 0, 3, 0, 0);
+    default: return Q(2, 1, 0, 3, 0, 0); // (3,2)
+    }
+#undef Q
+}
+
+// Per-byte rounded-up average (a+b+1)>>1 of four packed bytes.
+__device__ __forceinline__ uint32_t avg4(uint32_t a, uint32_t b)
+{
+    return (a | b) - (((a ^ b) >> 1) & 0x7f7f7f7fu);
+}
+
+// 4 consecutive bytes at (x, y) of a pw-stride plane, x aligned or not, all
+// bytes inside the buffer (caller guarantees): two aligned dwords + alignbyte.
+__device__ __forceinline__ uint32_t load4_fast(const uint8_t *plane, int pw, int x, int y)
+{
+    size_t a = (size_t)y * pw + x;
+    const uint32_t *p = (const uint32_t *)(plane + (a & ~(size_t)3));
+    return __builtin_amdgcn_alignbyte(p[1], p[0], (uint32_t)(a & 3));
+}
+
+// Same with every coordinate clamped into the plane (infinite edge replication).
+__device__ __forceinline__ uint32_t load4_clamped(const uint8_t *plane, int pw, int ph, int x, int y)
+{
+    const uint8_t *row = plane + (size_t)clampi(y, 0, ph - 1) * pw;
+    uint32_t v = 0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) v |= (uint32_t)row[clampi(x + m, 0, pw - 1)] << (8 * m);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// plane build: F, H, V, C over the padded plane (inter.rs:816-843 + hpel.rs /
+// build_hpel_fused inter.rs:908-987).  Every tap index is clamped, which
+// equals edge replication of the source frame (pad_plane, inter.rs:786-814).
+// Tile: 256 columns x 16 rows per 256-thread block; each thread makes a 4x4
+// patch and stores whole dwords.
+// ---------------------------------------------------------------------------
+
+constexpr int PT_W = 256;
+constexpr int PT_H = 16;
+constexpr int PT_LW = PT_W + 8; // LDS row pitch (>= PT_W + 5)
+
+__global__ __launch_bounds__(256) void k_build_planes(const uint8_t *__restrict__ frames,
+                                                      size_t frame_stride, int cw, int ch, int pad,
+                                                      uint8_t *__restrict__ planes,
+                                                      size_t planes_bytes)
+{
+    __shared__ uint8_t s_in[PT_H + 5][PT_LW];
+    __shared__ int16_t s_vt[PT_H][PT_LW];
+    const int pw = cw + 2 * pad, ph = ch + 2 * pad;
+    const size_t n = (size_t)pw * ph;
+    const uint8_t *y = frames + (size_t)blockIdx.z * frame_stride;
+    uint8_t *out = planes + (size_t)blockIdx.z * planes_bytes;
+    const int x0 = blockIdx.x * PT_W, y0 = blockIdx.y * PT_H;
+    const int tid = threadIdx.x;
+
+    // source tile: rows y0-2 .. y0+PT_H+2, columns x0-2 .. x0+PT_W+3 (padded coords)
+    for (int i = tid; i < (PT_H + 5) * (PT_W + 6); i += 256) {
+        int r = i / (PT_W + 6), c = i - r * (PT_W + 6);
+        int sy = clampi(y0 - 2 + r - pad, 0, ch - 1);
+        int sx = clampi(x0 - 2 + c - pad, 0, cw - 1);
+        s_in[r][c] = y[(size_t)sy * cw + sx];
+    }
+    __syncthreads();
+    // vertical 6-tap intermediates (exact in int16: -2550..10710)
+    for (int i = tid; i < PT_H * (PT_W + 6); i += 256) {
+        int r = i / (PT_W + 6), c = i - r * (PT_W + 6);
+        int v = (int)s_in[r][c] - 5 * (int)s_in[r + 1][c] + 20 * (int)s_in[r + 2][c] +
+                20 * (int)s_in[r + 3][c] - 5 * (int)s_in[r + 4][c] + (int)s_in[r + 5][c];
+        s_vt[r][c] = (int16_t)v;
+    }
+    __syncthreads();
+    const int cx = (tid & 63) * 4;  // tile column of the 4-wide patch
+    const int ry0 = (tid >> 6) * 4; // tile row of the 4-tall patch
+    const int gx = x0 + cx;
+    if (gx >= pw) return;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        const int r = ry0 + rr, gy = y0 + r;
+        if (gy >= ph) break;
+        uint32_t f4 = 0, h4 = 0, v4 = 0, c4 = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int c = cx + k; // output column gx+k <-> tile column c+2
+            const uint8_t *in = &s_in[r + 2][c];
+            int hs = (int)in[0] - 5 * (int)in[1] + 20 * (int)in[2] + 20 * (int)in[3] -
+                     5 * (int)in[4] + (int)in[5];
+            const int16_t *vt = &s_vt[r][c];
+            int cs = (int)vt[0] - 5 * (int)vt[1] + 20 * (int)vt[2] + 20 * (int)vt[3] -
+                     5 * (int)vt[4] + (int)vt[5];
+            f4 |= (uint32_t)in[2] << (8 * k);
+            h4 |= (uint32_t)clip_u8((hs + 16) >> 5) << (8 * k);
+            v4 |= (uint32_t)clip_u8(((int)vt[2] + 16) >> 5) << (8 * k);
+            c4 |= (uint32_t)clip_u8((cs + 512) >> 10) << (8 * k);
+        }
+        const size_t o = (size_t)gy * pw + gx;
+        *(uint32_t *)(out + o) = f4;
+        *(uint32_t *)(out + n + o) = h4;
+        *(uint32_t *)(out + 2 * n + o) = v4;
+        *(uint32_t *)(out + 3 * n + o) = c4;
+    }
+    // zero the tail slack once per plane set
+    if (blockIdx.x == 0 && blockIdx.y == 0 && tid < 64)
+        *(uint32_t *)(out + 4 * n + 4 * tid) = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// key fill
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_u64(uint64_t *p, size_t n, uint64_t v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// exhaustive integer search
+// ---------------------------------------------------------------------------
+//
+// Lane <-> (block, column group g): the lane owns the four horizontally adjacent
+// candidates dx = -R + 4g + k (k = 0..3) for every dy, walking the reference rows
+// of its window top to bottom.  Row j feeds candidate row t = j - k for source
+// row k, so a ring of BH accumulators (four packed u16 SADs each: a BWxBH block
+// SAD is at most 255*256 = 65280 < 65536) holds the candidate rows in flight;
+// after row j the candidate row j-(BH-1) is complete and is priced in raster
+// order.  v_qsad_pk_u16_u8 computes the four alignments of one source dword
+// against eight reference bytes in one instruction.
+
+template <int BW, int BH> struct FsShape {
+    static constexpr int SD = BW / 4;  // source dwords per row
+    static constexpr int RD = SD + 1;  // reference dwords per row (before realign)
+};
+
+__device__ __forceinline__ uint64_t wave_seg_min(uint64_t key, int seg, int lane)
+{
+    // segmented min over lanes with equal `seg`, lanes sorted by seg: afterwards
+    // the first lane of every segment holds that segment's minimum.
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint64_t ok = __shfl_down(key, off, 64);
+        int os = __shfl_down(seg, off, 64);
+        if (lane + off < 64 && os == seg && ok < key) key = ok;
+    }
+    return key;
+}
+
+template <int BW, int BH>
+__global__ __launch_bounds__(256) void k_full_search(rh264_search_params p,
+                                                     const uint8_t *__restrict__ cur,
+                                                     size_t cur_stride,
+                                                     const uint8_t *__restrict__ planes,
+                                                     size_t planes_stride,
+                                                     const int32_t *__restrict__ centres,
+                                                     const int32_t *__restrict__ mvps,
+                                                     uint64_t *__restrict__ keys)
+{
+    using S = FsShape<BW, BH>;
+    __shared__ int32_t s_lr[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
+    __syncthreads();
+
+    const int R = p.range, NC = 2 * R + 1, NG = (NC + 3) / 4;
+    const int nbx = p.cw / BW, nby = p.ch / BH, nblk = nbx * nby;
+    const int pw = p.cw + 2 * p.pad, ph = p.ch + 2 * p.pad;
+    const int frame = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const long gi = (long)blockIdx.x * 256 + threadIdx.x;
+    const int b = (int)(gi / NG);
+    const int g = (int)(gi - (long)b * NG);
+    const bool active = b < nblk;
+    const int bsafe = active ? b : nblk - 1;
+
+    const uint8_t *F = planes + (size_t)frame * planes_stride; // plane 0
+    const size_t boff = (size_t)frame * nblk + bsafe;
+    const int lx = (bsafe % nbx) * BW, ly = (bsafe / nbx) * BH;
+    const int cx = centres ? centres[2 * boff] : 0, cy = centres ? centres[2 * boff + 1] : 0;
+    const int mpx = mvps ? mvps[2 * boff] : 0, mpy = mvps ? mvps[2 * boff + 1] : 0;
+
+    // source block
+    uint32_t src[BH][S::SD];
+    {
+        const uint8_t *sp = cur + (size_t)frame * cur_stride + (size_t)ly * p.cw + lx;
+#pragma unroll
+        for (int k = 0; k < BH; k++)
+#pragma unroll
+            for (int i = 0; i < S::SD; i++) src[k][i] = *(const uint32_t *)(sp + (size_t)k * p.cw + 4 * i);
+    }
+
+    const int dx0 = -R + 4 * g;
+    const int px0 = lx + (cx >> 2) + p.pad + dx0; // padded column of candidate dx0
+    const int py0 = ly + (cy >> 2) + p.pad - R;   // padded row of candidate dy = -R
+    // whole window of the block inside the padded plane -> direct aligned reads
+    const int wx_lo = lx + (cx >> 2) + p.pad - R, wx_hi = lx + (cx >> 2) + p.pad + R + BW - 1;
+    const bool fast = wx_lo >= 0 && wx_hi <= pw - 1 && py0 >= 0 && py0 + 2 * R + BH - 1 <= ph - 1;
+    const int shift = px0 & 3;
+
+    int rxb[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) rxb[k] = mvbits(cx + 4 * (dx0 + k) - mpx);
+
+    uint64_t acc[BH];
+#pragma unroll
+    for (int s = 0; s < BH; s++) acc[s] = 0;
+    uint32_t best = 0xffffffffu, bidx = 0xffffffffu;
+    const int NJ = 2 * R + BH;
+
+    for (int jb = 0; jb < NJ; jb += BH) {
+#pragma unroll
+        for (int jj = 0; jj < BH; jj++) {
+            const int j = jb + jj;
+            if (j < NJ) {
+                uint32_t w[S::RD];
+                const int ry = py0 + j;
+                if (fast) {
+                    const uint32_t *rp = (const uint32_t *)(F + (size_t)ry * pw + (px0 & ~3));
+                    uint32_t a[S::RD + 1];
+#pragma unroll
+                    for (int i = 0; i < S::RD + 1; i++) a[i] = rp[i];
+#pragma unroll
+                    for (int i = 0; i < S::RD; i++) w[i] = __builtin_amdgcn_alignbyte(a[i + 1], a[i], shift);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < S::RD; i++) w[i] = load4_clamped(F, pw, ph, px0 + 4 * i, ry);
+                }
+#pragma unroll
+                for (int k = 0; k < BH; k++) {
+                    const int slot = (jj - k + BH) % BH;
+#pragma unroll
+                    for (int i = 0; i < S::SD; i++) {
+                        uint64_t ref8 = ((uint64_t)w[i + 1] << 32) | w[i];
+                        acc[slot] = __builtin_amdgcn_qsad_pk_u16_u8(ref8, src[k][i], acc[slot]);
+                    }
+                }
+                const int t = j - (BH - 1);
+                const int sd = (jj + 1) % BH;
+                if (t >= 0) {
+                    const int ry_bits = mvbits(cy + 4 * (t - R) - mpy);
+                    const uint64_t a4 = acc[sd];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int c = 4 * g + k;
+                        const uint32_t sad = (uint32_t)(a4 >> (16 * k)) & 0xffffu;
+                        const uint32_t cost = sad + (uint32_t)s_lr[rxb[k] + ry_bits];
+                        if (c < NC && cost < best) {
+                            best = cost;
+                            bidx = (uint32_t)(t * NC + c);
+                        }
+                    }
+                }
+                acc[sd] = 0;
+            }
+        }
+    }
+
+    uint64_t key = active ? (((uint64_t)best << 32) | bidx) : ~0ull;
+    const int seg = active ? b : 0x7fffffff;
+    key = wave_seg_min(key, seg, lane);
+    const int prev_seg = __shfl_up(seg, 1, 64);
+    if (active && (lane == 0 || prev_seg != seg)) atomicMin((unsigned long long *)&keys[boff], (unsigned long long)key);
+}
+
+// ---------------------------------------------------------------------------
+// sub-pel refinement
+// ---------------------------------------------------------------------------
+//
+// A group of LPB = BW*BH/4 lanes owns one block: lane (4x4 block bi, row r)
+// holds one source row of four pixels.  A candidate costs one or two 4-byte
+// plane reads per lane, the row butterfly in registers, the column butterfly
+// across the quad (DPP quad_perm), |.| and a group reduction.
+
+template <int LPB> __device__ __forceinline__ int group_sum(int v)
+{
+    v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true); // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true); // quad_perm [2,3,0,1]
+    if (LPB >= 8) v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true); // row_half_mirror
+    if (LPB >= 16) v += __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true); // row_mirror
+    if (LPB >= 32) v += __shfl_xor(v, 16, 64);
+    if (LPB >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// sum |H d| of the 4x4 block whose row r this lane holds (d = 4 diffs).
+__device__ __forceinline__ int hadamard_row_quad(int d0, int d1, int d2, int d3, int r)
+{
+    // row transform (hadamard_1d, transform.rs:631-633)
+    int e0 = d0 + d1 + d2 + d3, e1 = d0 + d1 - d2 - d3;
+    int e2 = d0 - d1 - d2 + d3, e3 = d0 - d1 + d2 - d3;
+    // column transform across the quad: stage 1 pairs rows r, r^2; stage 2 r, r^1.
+    const bool hi2 = (r & 2) != 0, odd = (r & 1) != 0;
+    int y0 = __builtin_amdgcn_mov_dpp(e0, 0x4E, 0xF, 0xF, true);
+    int y1 = __builtin_amdgcn_mov_dpp(e1, 0x4E, 0xF, 0xF, true);
+    int y2 = __builtin_amdgcn_mov_dpp(e2, 0x4E, 0xF, 0xF, true);
+    int y3 = __builtin_amdgcn_mov_dpp(e3, 0x4E, 0xF, 0xF, true);
+    int s0 = hi2 ? y0 - e0 : e0 + y0, s1 = hi2 ? y1 - e1 : e1 + y1;
+    int s2 = hi2 ? y2 - e2 : e2 + y2, s3 = hi2 ? y3 - e3 : e3 + y3;
+    int z0 = __builtin_amdgcn_mov_dpp(s0, 0xB1, 0xF, 0xF, true);
+    int z1 = __builtin_amdgcn_mov_dpp(s1, 0xB1, 0xF, 0xF, true);
+    int z2 = __builtin_amdgcn_mov_dpp(s2, 0xB1, 0xF, 0xF, true);
+    int z3 = __builtin_amdgcn_mov_dpp(s3, 0xB1, 0xF, 0xF, true);
+    int c0 = odd ? z0 - s0 : s0 + z0, c1 = odd ? z1 - s1 : s1 + z1;
+    int c2 = odd ? z2 - s2 : s2 + z2, c3 = odd ? z3 - s3 : s3 + z3;
+    return abs(c0) + abs(c1) + abs(c2) + abs(c3);
+}
+
+struct RefineGeom {
+    const uint8_t *pl; // plane set base
+    size_t n;          // bytes per plane
+    int pw, ph, pad;
+};
+
+// SATD (sum |H d|) of the block at (lx, ly) vs its prediction at (mvx, mvy).
+template <int BW, int BH, int LPB>
+__device__ __forceinline__ int eval_satd(const RefineGeom &G, uint32_t srow, int lx, int ly, int bxi,
+                                         int byi, int r, int mvx, int mvy)
+{
+    const int bpx = lx + (mvx >> 2) + G.pad, bpy = ly + (mvy >> 2) + G.pad;
+    const uint32_t q = qtab((mvx & 3) * 4 + (mvy & 3));
+    const bool two = ((mvx | mvy) & 1) != 0;
+    const int x = bpx + 4 * bxi, y = bpy + 4 * byi + r;
+    // acceptance guard of hpel_block / MeCtx (inter.rs:1030, 1075): +1 slack
+    const bool inb = bpx >= 0 && bpy >= 0 && bpx + BW + 1 <= G.pw && bpy + BH + 1 <= G.ph;
+    const uint8_t *pa = G.pl + G.n * (q & 15);
+    const int ax = (q >> 4) & 15, ay = (q >> 8) & 15;
+    uint32_t pred;
+    if (inb) {
+        pred = load4_fast(pa, G.pw, x + ax, y + ay);
+        if (two) {
+            const uint8_t *pb = G.pl + G.n * ((q >> 12) & 15);
+            pred = avg4(pred, load4_fast(pb, G.pw, x + ((q >> 16) & 15), y + ((q >> 20) & 15)));
+        }
+    } else {
+        pred = load4_clamped(pa, G.pw, G.ph, x + ax, y + ay);
+        if (two) {
+            const uint8_t *pb = G.pl + G.n * ((q >> 12) & 15);
+            pred = avg4(pred, load4_clamped(pb, G.pw, G.ph, x + ((q >> 16) & 15), y + ((q >> 20) & 15)));
+        }
+    }
+    int d0 = (int)(srow & 255) - (int)(pred & 255);
+    int d1 = (int)((srow >> 8) & 255) - (int)((pred >> 8) & 255);
+    int d2 = (int)((srow >> 16) & 255) - (int)((pred >> 16) & 255);
+    int d3 = (int)(srow >> 24) - (int)(pred >> 24);
+    return group_sum<LPB>(hadamard_row_quad(d0, d1, d2, d3, r));
+}
+
+template <int BW, int BH>
+__global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
+                                                       const uint8_t *__restrict__ cur,
+                                                       size_t cur_stride,
+                                                       const uint8_t *__restrict__ planes,
+                                                       size_t planes_stride,
+                                                       const uint64_t *__restrict__ keys,
+                                                       const int32_t *__restrict__ centres,
+                                                       const int32_t *__restrict__ start_mv,
+                                                       const int32_t *__restrict__ mvps,
+                                                       int32_t *__restrict__ int_mv,
+                                                       int32_t *__restrict__ int_cost,
+                                                       int32_t *__restrict__ out_mv,
+                                                       int32_t *__restrict__ out_cost,
+                                                       int32_t *__restrict__ out_evals)
+{
+    constexpr int LPB = BW * BH / 4;
+    constexpr int GPB = 256 / LPB; // groups per thread block
+    __shared__ int32_t s_lr[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
+    __syncthreads();
+
+    const int nbx = p.cw / BW, nby = p.ch / BH, nblk = nbx * nby;
+    const int frame = blockIdx.y;
+    const int grp = threadIdx.x / LPB, li = threadIdx.x % LPB;
+    const int b = blockIdx.x * GPB + grp;
+    if (b >= nblk) return; // whole groups exit together
+    const size_t boff = (size_t)frame * nblk + b;
+    const int lx = (b % nbx) * BW, ly = (b / nbx) * BH;
+    const int bi = li >> 2, r = li & 3;
+    const int bxi = bi % (BW / 4), byi = bi / (BW / 4);
+
+    RefineGeom G;
+    G.pw = p.cw + 2 * p.pad;
+    G.ph = p.ch + 2 * p.pad;
+    G.pad = p.pad;
+    G.n = (size_t)G.pw * G.ph;
+    G.pl = planes + (size_t)frame * planes_stride;
+    const uint32_t srow =
+        *(const uint32_t *)(cur + (size_t)frame * cur_stride + (size_t)(ly + 4 * byi + r) * p.cw + lx + 4 * bxi);
+    const int mpx = mvps ? mvps[2 * boff] : 0, mpy = mvps ? mvps[2 * boff + 1] : 0;
+
+    int bx, by;
+    if (keys) {
+        const uint64_t key = keys[boff];
+        const int NC = 2 * p.range + 1;
+        const uint32_t idx = (uint32_t)key;
+        const int t = (int)(idx / (uint32_t)NC), c = (int)(idx % (uint32_t)NC);
+        const int cx = centres ? centres[2 * boff] : 0, cy = centres ? centres[2 * boff + 1] : 0;
+        bx = cx + 4 * (c - p.range);
+        by = cy + 4 * (t - p.range);
+        if (li == 0) {
+            if (int_mv) { int_mv[2 * boff] = bx; int_mv[2 * boff + 1] = by; }
+            if (int_cost) int_cost[boff] = (int32_t)(key >> 32);
+        }
+    } else {
+        bx = start_mv[2 * boff];
+        by = start_mv[2 * boff + 1];
+    }
+
+    auto cost_of = [&](int mx, int my) -> int {
+        const int satd = eval_satd<BW, BH, LPB>(G, srow, lx, ly, bxi, byi, r, mx, my);
+        return satd + s_lr[mvbits(mx - mpx) + mvbits(my - mpy)];
+    };
+
+    int best = cost_of(bx, by);
+    int evals = 1;
+    const int pat = p.subpel_pat;
+    if (pat >= 0) {
+        const int nring = (pat & 1) ? 4 : 8;
+        for (int step = 2; step >= 1; step--) {
+            const int rx[8] = {step, -step, 0, 0, step, -step, step, -step};
+            const int ry[8] = {0, 0, step, -step, step, -step, -step, step};
+            // terminates: every extra pass strictly lowers a cost bounded below by 0
+            for (;;) {
+                bool improved = false;
+                for (int i = 0; i < nring; i++) {
+                    const int mx = bx + rx[i], my = by + ry[i];
+                    const int cc = cost_of(mx, my);
+                    evals++;
+                    if (cc < best) { best = cc; bx = mx; by = my; improved = true; }
+                }
+                if (!improved || (pat & 2)) break;
+            }
+        }
+    }
+    if (li == 0) {
+        out_mv[2 * boff] = bx;
+        out_mv[2 * boff + 1] = by;
+        out_cost[boff] = best;
+        if (out_evals) out_evals[boff] = evals;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched block metrics (the accel per-block family)
+// ---------------------------------------------------------------------------
+
+__device__ int satd4x4_sum_abs(const int d[16])
+{
+    int m[16];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        int a = d[rr * 4], b = d[rr * 4 + 1], c = d[rr * 4 + 2], e = d[rr * 4 + 3];
+        m[rr * 4] = a + b + c + e; m[rr * 4 + 1] = a + b - c - e;
+        m[rr * 4 + 2] = a - b - c + e; m[rr * 4 + 3] = a - b + c - e;
+    }
+    int s = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        int a = m[c], b = m[4 + c], cc = m[8 + c], e = m[12 + c];
+        s += abs(a + b + cc + e) + abs(a + b - cc - e) + abs(a - b - cc + e) + abs(a - b + cc - e);
+    }
+    return s;
+}
+
+__global__ void k_block_metric(int kind, int w, int h, int n, const uint8_t *__restrict__ src, int ss,
+                               const int64_t *__restrict__ src_off, const uint8_t *__restrict__ ref,
+                               int rs, const int64_t *__restrict__ ref_off,
+                               const int64_t *__restrict__ ref_off2, int32_t *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t *a = src + src_off[i];
+    const uint8_t *b = ref + ref_off[i];
+    const uint8_t *b2 = kind == RH264_METRIC_SATD_AVG ? ref + ref_off2[i] : nullptr;
+    int total = 0;
+    if (kind == RH264_METRIC_SAD) {
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) total += abs((int)a[y * ss + x] - (int)b[(size_t)y * rs + x]);
+    } else {
+        for (int by = 0; by < h; by += 4)
+            for (int bx = 0; bx < w; bx += 4) {
+                int d[16];
+                for (int y = 0; y < 4; y++)
+                    for (int x = 0; x < 4; x++) {
+                        size_t o = (size_t)(by + y) * rs + bx + x;
+                        int pv = b2 ? (((int)b[o] + (int)b2[o] + 1) >> 1) : (int)b[o];
+                        d[y * 4 + x] = (int)a[(by + y) * ss + bx + x] - pv;
+                    }
+                int s = satd4x4_sum_abs(d);
+                total += kind == RH264_METRIC_SATD_WELS ? (s + 1) >> 1 : s;
+            }
+    }
+    out[i] = total;
+}
+
+// ---------------------------------------------------------------------------
+// block prediction from a plane set
+// ---------------------------------------------------------------------------
+
+__global__ void k_predict(const uint8_t *__restrict__ planes, int pw, int ph, int pad, int bw, int bh,
+                          int n, const int32_t *__restrict__ xy, const int32_t *__restrict__ mv,
+                          uint8_t *__restrict__ out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)n * bw * bh;
+    if (t >= total) return;
+    const int i = (int)(t / (bw * bh));
+    const int pix = (int)(t - (long)i * bw * bh);
+    const int yy = pix / bw, xx = pix - yy * bw;
+    const int mvx = mv[2 * i], mvy = mv[2 * i + 1];
+    const int x = xy[2 * i] + (mvx >> 2) + pad + xx, y = xy[2 * i + 1] + (mvy >> 2) + pad + yy;
+    const size_t np = (size_t)pw * ph;
+    const uint32_t q = qtab((mvx & 3) * 4 + (mvy & 3));
+    const uint8_t *pa = planes + np * (q & 15);
+    int v = pa[(size_t)clampi(y + ((q >> 8) & 15), 0, ph - 1) * pw + clampi(x + ((q >> 4) & 15), 0, pw - 1)];
+    if ((mvx | mvy) & 1) {
+        const uint8_t *pb = planes + np * ((q >> 12) & 15);
+        int u = pb[(size_t)clampi(y + ((q >> 20) & 15), 0, ph - 1) * pw + clampi(x + ((q >> 16) & 15), 0, pw - 1)];
+        v = (v + u + 1) >> 1;
+    }
+    out[t] = (uint8_t)v;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char *rh264_last_error(void) { return g_err.c_str(); }
+
+int rh264_init(int device, rh264_ctx **out)
+{
+    if (!out) return fail("rh264_init: null out");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail("rh264_init: device %d of %d", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    rh264_ctx *c = new rh264_ctx();
+    c->device = device;
+    c->keys = nullptr;
+    c->keys_cap = 0;
+    c->tmp_mv = nullptr;
+    c->tmp_mv_cap = 0;
+    c->prof_cap = c->prof_n = 0;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail("hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (int i = 0; i < 16; i++) HIP_TRY(hipEventCreate(&c->ev[i]));
+    *out = c;
+    return 0;
+}
+
+int rh264_destroy(rh264_ctx *ctx)
+{
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->keys) (void)hipFree(ctx->keys);
+    if (ctx->tmp_mv) (void)hipFree(ctx->tmp_mv);
+    for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
+    for (hipEvent_t e : ctx->prof) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int rh264_device_name(rh264_ctx *ctx, char *buf, int buflen)
+{
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
+    snprintf(buf, (size_t)buflen, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return 0;
+}
+
+int rh264_alloc(rh264_ctx *ctx, size_t bytes, void **dptr)
+{
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMalloc(dptr, bytes ? bytes : 1));
+    return 0;
+}
+
+int rh264_free(rh264_ctx *ctx, void *dptr)
+{
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipFree(dptr));
+    return 0;
+}
+
+int rh264_h2d(rh264_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int rh264_d2h(rh264_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int rh264_memset(rh264_ctx *ctx, void *dst, int value, size_t bytes)
+{
+    HIP_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return 0;
+}
+
+int rh264_sync(rh264_ctx *ctx)
+{
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int rh264_event_record(rh264_ctx *ctx, int slot)
+{
+    if (slot < 0 || slot >= 16) return fail("event slot %d", slot);
+    HIP_TRY(hipEventRecord(ctx->ev[slot], ctx->stream));
+    return 0;
+}
+
+int rh264_event_elapsed_ms(rh264_ctx *ctx, int a, int b, float *ms)
+{
+    if (a < 0 || a >= 16 || b < 0 || b >= 16) return fail("event slot");
+    HIP_TRY(hipEventSynchronize(ctx->ev[b]));
+    HIP_TRY(hipEventElapsedTime(ms, ctx->ev[a], ctx->ev[b]));
+    return 0;
+}
+
+void rh264_lambda_table(double lambda, int32_t *lr)
+{
+    for (int b = 0; b < RH264_LR_N; b++) {
+        int64_t v = (int64_t)(lambda * (double)b); // `as i64` truncation, mb16.rs:2563
+        lr[b] = v > INT32_MAX ? INT32_MAX : (int32_t)v;
+    }
+}
+
+size_t rh264_planes_bytes(int cw, int ch, int pad)
+{
+    return (size_t)4 * (size_t)(cw + 2 * pad) * (size_t)(ch + 2 * pad) + 256;
+}
+
+static bool geom_ok(int cw, int ch, int pad)
+{
+    return cw > 0 && ch > 0 && cw % 16 == 0 && ch % 16 == 0 && pad >= 4 && pad <= 128 && pad % 4 == 0;
+}
+
+int rh264_build_planes(rh264_ctx *ctx, const uint8_t *frames, size_t frame_stride, int nframes, int cw,
+                       int ch, int pad, uint8_t *planes)
+{
+    if (!geom_ok(cw, ch, pad)) return RH264_DECLINED;
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("build_planes: nframes %d > 65535", nframes);
+    const int pw = cw + 2 * pad, ph = ch + 2 * pad;
+    dim3 grid((pw + PT_W - 1) / PT_W, (ph + PT_H - 1) / PT_H, nframes);
+    hipLaunchKernelGGL(k_build_planes, grid, dim3(256), 0, ctx->stream, frames, frame_stride, cw, ch, pad,
+                       planes, rh264_planes_bytes(cw, ch, pad));
+    LAUNCH_CHECK("k_build_planes");
+    return 0;
+}
+
+static bool shape_ok(int w, int h)
+{
+    return (w == 16 && (h == 16 || h == 8)) || (w == 8 && (h == 16 || h == 8 || h == 4)) ||
+           (w == 4 && (h == 8 || h == 4));
+}
+
+static int check_strides(size_t cur_stride, size_t planes_stride)
+{
+    if (cur_stride % 4 || planes_stride % 4) return fail("frame and plane-set strides must be multiples of 4");
+    return 0;
+}
+
+static int check_params(const rh264_search_params *p)
+{
+    if (!p) return fail("null params");
+    if (!geom_ok(p->cw, p->ch, p->pad) || !shape_ok(p->bw, p->bh)) return RH264_DECLINED;
+    if (p->range < 1 || p->range > 64) return RH264_DECLINED;
+    if (p->subpel_pat < -1 || p->subpel_pat > 3) return RH264_DECLINED;
+    for (int i = 0; i < RH264_LR_N; i++)
+        if (p->lr[i] < 0 || p->lr_fp[i] < 0 || p->lr[i] > (1 << 28) || p->lr_fp[i] > (1 << 28))
+            return fail("lambda table entry %d out of range", i);
+    return 0;
+}
+
+} // extern "C"
+
+template <int BW, int BH>
+static int launch_full(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                       size_t cur_stride, const uint8_t *planes, size_t planes_stride,
+                       const int32_t *centres, const int32_t *mvps, uint64_t *keys)
+{
+    const int NC = 2 * p->range + 1, NG = (NC + 3) / 4;
+    const long nblk = (long)(p->cw / BW) * (p->ch / BH);
+    const long lanes = nblk * NG;
+    dim3 grid((unsigned)((lanes + 255) / 256), nframes);
+    hipLaunchKernelGGL((k_full_search<BW, BH>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes,
+                       planes_stride, centres, mvps, keys);
+    LAUNCH_CHECK("k_full_search");
+    return 0;
+}
+
+template <int BW, int BH>
+static int launch_refine(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                         size_t cur_stride, const uint8_t *planes, size_t planes_stride,
+                         const uint64_t *keys, const int32_t *centres, const int32_t *start_mv,
+                         const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv,
+                         int32_t *cost, int32_t *evals)
+{
+    constexpr int LPB = BW * BH / 4, GPB = 256 / LPB;
+    const int nblk = (p->cw / BW) * (p->ch / BH);
+    dim3 grid((unsigned)((nblk + GPB - 1) / GPB), nframes);
+    hipLaunchKernelGGL((k_subpel_refine<BW, BH>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes,
+                       planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost, evals);
+    LAUNCH_CHECK("k_subpel_refine");
+    return 0;
+}
+
+extern "C" {
+
+#define SHAPE_DISPATCH(FN, ...)                                                               \
+    switch (p->bw * 100 + p->bh) {                                                            \
+    case 1616: return FN<16, 16>(__VA_ARGS__);                                                \
+    case 1608: return FN<16, 8>(__VA_ARGS__);                                                 \
+    case 816: return FN<8, 16>(__VA_ARGS__);                                                  \
+    case 808: return FN<8, 8>(__VA_ARGS__);                                                   \
+    case 804: return FN<8, 4>(__VA_ARGS__);                                                   \
+    case 408: return FN<4, 8>(__VA_ARGS__);                                                   \
+    case 404: return FN<4, 4>(__VA_ARGS__);                                                   \
+    default: return RH264_DECLINED;                                                           \
+    }
+
+static int full_search_keys(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                            size_t cur_stride, const uint8_t *planes, size_t planes_stride,
+                            const int32_t *centres, const int32_t *mvps)
+{
+    const size_t nblk = (size_t)(p->cw / p->bw) * (p->ch / p->bh);
+    const size_t nk = nblk * (size_t)nframes;
+    int rc = ensure_keys(ctx, nk);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, ctx->stream, ctx->keys, nk,
+                       ~0ull);
+    LAUNCH_CHECK("k_fill_u64");
+    SHAPE_DISPATCH(launch_full, ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps,
+                   ctx->keys);
+}
+
+static int refine_keys(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                       size_t cur_stride, const uint8_t *planes, size_t planes_stride, const int32_t *centres,
+                       const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv, int32_t *cost)
+{
+    SHAPE_DISPATCH(launch_refine, ctx, p, nframes, cur, cur_stride, planes, planes_stride, ctx->keys, centres,
+                   nullptr, mvps, int_mv, int_cost, mv, cost, nullptr);
+}
+
+int rh264_full_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                      size_t cur_stride, const uint8_t *planes, size_t planes_stride, const int32_t *centres,
+                      const int32_t *mvps, int32_t *int_mv, int32_t *int_cost)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    rc = check_strides(cur_stride, planes_stride);
+    if (rc) return rc;
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    if (!int_mv || !int_cost) return fail("full_search: null outputs");
+    rc = full_search_keys(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps);
+    if (rc) return rc;
+    // decode keys with a no-ring refine pass writing only the integer outputs
+    const size_t nblk = (size_t)(p->cw / p->bw) * (p->ch / p->bh) * nframes;
+    rc = ensure_tmp_mv(ctx, 3 * nblk);
+    if (rc) return rc;
+    rh264_search_params q = *p;
+    q.subpel_pat = -1;
+    SHAPE_DISPATCH(launch_refine, ctx, &q, nframes, cur, cur_stride, planes, planes_stride, ctx->keys, centres,
+                   nullptr, mvps, int_mv, int_cost, ctx->tmp_mv, ctx->tmp_mv + 2 * nblk, nullptr);
+}
+
+int rh264_subpel_refine(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                        size_t cur_stride, const uint8_t *planes, size_t planes_stride, const int32_t *start_mv,
+                        const int32_t *mvps, int32_t *mv, int32_t *cost, int32_t *evals)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    rc = check_strides(cur_stride, planes_stride);
+    if (rc) return rc;
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    if (!start_mv || !mv || !cost) return fail("subpel_refine: null pointers");
+    SHAPE_DISPATCH(launch_refine, ctx, p, nframes, cur, cur_stride, planes, planes_stride, nullptr, nullptr,
+                   start_mv, mvps, nullptr, nullptr, mv, cost, evals);
+}
+
+int rh264_plain_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                       size_t cur_stride, const uint8_t *planes, size_t planes_stride, const int32_t *centres,
+                       const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv, int32_t *cost)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    rc = check_strides(cur_stride, planes_stride);
+    if (rc) return rc;
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    if (!mv || !cost) return fail("plain_search: null outputs");
+    hipEvent_t *ev = nullptr;
+    if (ctx->prof_n < ctx->prof_cap) ev = &ctx->prof[3 * ctx->prof_n++];
+    if (ev) HIP_TRY(hipEventRecord(ev[0], ctx->stream));
+    rc = full_search_keys(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps);
+    if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
+    rc = refine_keys(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps, int_mv, int_cost,
+                     mv, cost);
+    if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[2], ctx->stream));
+    return 0;
+}
+
+int rh264_set_profiling(rh264_ctx *ctx, int capacity)
+{
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (hipEvent_t e : ctx->prof) HIP_TRY(hipEventDestroy(e));
+    ctx->prof.clear();
+    ctx->prof_cap = ctx->prof_n = 0;
+    if (capacity <= 0) return 0;
+    ctx->prof.resize((size_t)3 * capacity);
+    for (auto &e : ctx->prof) HIP_TRY(hipEventCreate(&e));
+    ctx->prof_cap = capacity;
+    return 0;
+}
+
+int rh264_profile_count(rh264_ctx *ctx) { return ctx->prof_n; }
+
+int rh264_profile_get(rh264_ctx *ctx, int i, float *search_ms, float *refine_ms)
+{
+    if (i < 0 || i >= ctx->prof_n) return fail("profile slot %d of %d", i, ctx->prof_n);
+    hipEvent_t *ev = &ctx->prof[3 * (size_t)i];
+    HIP_TRY(hipEventSynchronize(ev[2]));
+    HIP_TRY(hipEventElapsedTime(search_ms, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(refine_ms, ev[1], ev[2]));
+    return 0;
+}
+
+int rh264_block_metric(rh264_ctx *ctx, int kind, int w, int h, int n, const uint8_t *src, int ss,
+                       const int64_t *src_off, const uint8_t *ref, int rs, const int64_t *ref_off,
+                       const int64_t *ref_off2, int32_t *out)
+{
+    if (kind < 0 || kind > 3) return fail("block_metric: kind %d", kind);
+    if (!(shape_ok(w, h) || (w == 4 && h == 4))) return RH264_DECLINED;
+    if (kind == RH264_METRIC_SATD_AVG && !ref_off2) return fail("block_metric: SATD_AVG needs ref_off2");
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_block_metric, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, kind, w, h, n,
+                       src, ss, src_off, ref, rs, ref_off, ref_off2, out);
+    LAUNCH_CHECK("k_block_metric");
+    return 0;
+}
+
+int rh264_predict_blocks(rh264_ctx *ctx, const uint8_t *planes, int cw, int ch, int pad, int bw, int bh, int n,
+                         const int32_t *xy, const int32_t *mv, uint8_t *out)
+{
+    if (!geom_ok(cw, ch, pad)) return RH264_DECLINED;
+    if (bw < 1 || bw > 16 || bh < 1 || bh > 16) return RH264_DECLINED;
+    if (n <= 0) return 0;
+    const long total = (long)n * bw * bh;
+    hipLaunchKernelGGL(k_predict, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, planes,
+                       cw + 2 * pad, ch + 2 * pad, pad, bw, bh, n, xy, mv, out);
+    LAUNCH_CHECK("k_predict");
+    return 0;
+}
+
+} // extern "C"

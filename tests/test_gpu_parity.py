@@ -1,0 +1,203 @@
+"""GPU parity: the HIP provider (through the C ABI) vs the CPU oracle on the same
+seeded inputs.  Bit-exact for every output (all integer)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from rusty_h264_amd import accel, synth
+
+pytestmark = pytest.mark.gpu
+O = Oracle()
+NT = min(os.cpu_count() or 1, 16)
+LAM26 = accel.lambda_me(26)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = accel.Device(0)
+    print("device:", d.name(), flush=True)
+    yield d
+    d.close()
+
+
+# ------------------------------------------------------------ block metrics
+
+@pytest.mark.parametrize("kind", [accel.METRIC_SAD, accel.METRIC_SATD_WELS, accel.METRIC_SATD])
+def test_block_metrics_corpus_and_lcg(dev, kind):
+    shapes = accel.SHAPES
+    # closed-form corpus (satd_sad.rs:390-399), 96 seeds, 16-stride blocks
+    src, ref = [], []
+    for seed in range(96):
+        a, b = synth.corpus_pair(seed)
+        src.append(a.reshape(-1)); ref.append(b.reshape(-1))
+    # LCG blocks (satd_sad.rs:459-489), strides 24 / 20 re-laid at stride 16 windows
+    src = np.concatenate(src); ref = np.concatenate(ref)
+    for w, h in shapes:
+        offs = np.arange(96, dtype=np.int64) * 256
+        got = accel.block_metric(dev, kind, w, h, src, 16, offs, ref, 16, offs)
+        for i in range(96):
+            a, b = src[offs[i]:offs[i] + 256], ref[offs[i]:offs[i] + 256]
+            want = [O.sad, O.satd_wels, O.satd_px][kind](a, 16, b, 16, w, h)
+            assert got[i] == want, (w, h, i)
+    buf = synth.lcg_u8(0x777, 24 * 20 * 64)
+    so = np.arange(32, dtype=np.int64) * 24 * 20
+    ro = so + 32 * 24 * 20
+    for w, h in shapes:
+        got = accel.block_metric(dev, kind, w, h, buf, 24, so, buf, 24, ro)
+        for i in range(32):
+            want = [O.sad, O.satd_wels, O.satd_px][kind](buf[so[i]:], 24, buf[ro[i]:], 24, w, h)
+            assert got[i] == want
+
+
+def test_satd_avg_x4(dev):
+    rng = np.random.default_rng(12)
+    plane = rng.integers(0, 256, 64 * 64, dtype=np.uint8)
+    src = rng.integers(0, 256, 16 * 16, dtype=np.uint8)
+    for w, h in [(16, 16), (16, 8), (8, 16), (8, 8)]:
+        oa = rng.integers(0, 64 * 40, 4)
+        ob = rng.integers(0, 64 * 40, 4)
+        got = accel.satd_avg_x4(dev, src, 16, plane, oa, ob, 64, w, h)
+        for k in range(4):
+            assert got[k] == O.satd_avg(src, 16, plane[oa[k]:], plane[ob[k]:], 64, w, h)
+        # satd_avg(a, a) == satd_x4 (satd_avg_compare.rs:165-167)
+        assert (accel.satd_avg_x4(dev, src, 16, plane, oa, oa, 64, w, h) ==
+                accel.satd_x4(dev, src, 16, plane, oa, 64, w, h)).all()
+    # extremal planes (satd_avg_compare.rs:209-221)
+    z, f = np.zeros(256, np.uint8), np.full(256, 255, np.uint8)
+    assert accel.sad_x4(dev, z, 16, f, [0, 0, 0, 0], 16, 16, 16)[0] == 65280
+
+
+# ------------------------------------------------------------ planes
+
+@pytest.mark.parametrize("cw,ch,pad,nf", [(64, 48, 16, 1), (176, 144, 32, 3), (352, 288, 32, 2), (48, 32, 4, 2),
+                                          (1920, 1088, 32, 1), (96, 64, 128, 1)])
+def test_planes_bit_exact(dev, cw, ch, pad, nf):
+    frames = np.random.default_rng(cw + pad).integers(0, 256, (nf, ch, cw), dtype=np.uint8)
+    fb = dev.upload(frames)
+    pl = accel.build_planes(dev, fb, nf, cw, ch, pad)
+    pb = accel.planes_bytes(cw, ch, pad)
+    got = dev.download(pl, (nf, pb), np.uint8)
+    pw, ph = cw + 2 * pad, ch + 2 * pad
+    for i in range(nf):
+        want = O.planes(frames[i], cw, ch, pad, fused=True)
+        assert np.array_equal(got[i, :4 * pw * ph].reshape(4, ph, pw), want), i
+    fb.free(); pl.free()
+
+
+def test_predict_blocks_any_mv(dev):
+    """Plane reads with clamping == mc_luma (inter.rs:1193-1262) for MVs far outside."""
+    cw, ch, pad = 64, 48, 32
+    y = np.random.default_rng(13).integers(0, 256, (ch, cw), dtype=np.uint8)
+    pl = dev.upload(O.planes(y, cw, ch, pad).reshape(-1))
+    rng = np.random.default_rng(14)
+    for bw, bh in accel.SHAPES:
+        n = 300
+        xy = np.stack([rng.integers(0, cw - bw + 1, n), rng.integers(0, ch - bh + 1, n)], 1)
+        mv = rng.integers(-500, 501, (n, 2))
+        got = accel.predict_blocks(dev, pl, cw, ch, pad, bw, bh, xy, mv)
+        for i in range(n):
+            want = O.mc_luma(y, cw, ch, int(xy[i, 0]), int(xy[i, 1]), bw, bh, int(mv[i, 0]), int(mv[i, 1]))
+            assert np.array_equal(got[i], want), (bw, bh, i)
+    pl.free()
+
+
+# ------------------------------------------------------------ search
+
+def _search_case(dev, frames, cw, ch, pad, bw, bh, R, pat, lam, centres=None, mvps=None, nthreads=NT):
+    nf = frames.shape[0] - 1
+    cur = dev.upload(frames[1:])
+    refs = dev.upload(frames[:-1])
+    pl = accel.build_planes(dev, refs, nf, cw, ch, pad)
+    p = accel.search_params(cw, ch, pad, bw, bh, R, pat, lam)
+    ms = accel.MotionSearch(dev, p)
+    dc = dev.upload(centres) if centres is not None else None
+    dm = dev.upload(mvps) if mvps is not None else None
+    ms.run_plain(cur, pl, nf, dc, dm)
+    got = ms.results(nf)
+    cfg = O.cfg(cw, ch, pad, bw, bh, R, pat, lam)
+    nb = ms.nblk
+    for f in range(nf):
+        opl = O.planes(frames[f], cw, ch, pad)
+        ce = centres[f * nb:(f + 1) * nb] if centres is not None else None
+        mp = mvps[f * nb:(f + 1) * nb] if mvps is not None else None
+        want = O.plain_search_frame(cfg, frames[f + 1], opl, ce, mp, nthreads=nthreads)
+        sl = slice(f * nb, (f + 1) * nb)
+        for k in ("int_mv", "int_cost", "mv", "cost"):
+            bad = np.nonzero((got[k][sl] != want[k]).reshape(nb, -1).any(1))[0]
+            assert bad.size == 0, f"{k} frame {f}: {bad.size} blocks differ, first {bad[:5]}"
+    for b in (cur, refs, pl, dc, dm):
+        if b is not None:
+            b.free()
+    return got
+
+
+@pytest.mark.parametrize("bw,bh", accel.SHAPES)
+def test_plain_search_shapes_small(dev, bw, bh):
+    frames = synth.cif_clip(3, seed=bw * 10 + bh, w=96, h=64)
+    _search_case(dev, frames, 96, 64, 32, bw, bh, 7, 0, LAM26)
+
+
+@pytest.mark.parametrize("R,pat,lam", [(16, 0, LAM26), (16, 2, LAM26), (5, 1, 0.0), (13, 3, 9.0), (16, -1, LAM26)])
+def test_plain_search_cif(dev, R, pat, lam):
+    """Config C1 geometry: CIF 352x288, 16x16, +-16 full search, qpel SATD refine."""
+    frames = synth.cif_clip(4, seed=R + pat)
+    _search_case(dev, frames, 352, 288, 32, 16, 16, R, pat, lam)
+
+
+def test_plain_search_centres_and_predictors_outside_pad(dev):
+    """Windows that leave the padded planes (clamped reads) and non-zero rate
+    predictors; pad 8 < R so edge blocks take the clamped path."""
+    cw, ch = 128, 96
+    frames = synth.cif_clip(3, seed=77, w=cw, h=ch)
+    nb = (cw // 16) * (ch // 16) * 2
+    rng = np.random.default_rng(15)
+    centres = (4 * rng.integers(-60, 61, (nb, 2))).astype(np.int32)
+    mvps = rng.integers(-300, 301, (nb, 2)).astype(np.int32)
+    _search_case(dev, frames, cw, ch, 8, 16, 16, 12, 0, LAM26, centres, mvps)
+
+
+def test_subpel_refine_from_given_starts(dev):
+    cw, ch, pad = 176, 144, 32
+    frames = synth.cif_clip(2, seed=5, w=cw, h=ch)
+    cur, ref = dev.upload(frames[1]), dev.upload(frames[0])
+    pl = accel.build_planes(dev, ref, 1, cw, ch, pad)
+    p = accel.search_params(cw, ch, pad, 16, 16, 8, 0, LAM26)
+    ms = accel.MotionSearch(dev, p)
+    rng = np.random.default_rng(16)
+    starts = rng.integers(-200, 201, (ms.nblk, 2)).astype(np.int32)
+    ds = dev.upload(starts)
+    ms.run_refine(cur, pl, 1, ds)
+    got = ms.results(1, with_evals=True)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 8, 0, LAM26)
+    opl = O.planes(frames[0], cw, ch, pad)
+    for b in range(ms.nblk):
+        bx, by = (b % (cw // 16)) * 16, (b // (cw // 16)) * 16
+        mv, c, ev = O.subpel_refine_block(cfg, frames[1], opl, bx, by, *starts[b])
+        assert tuple(got["mv"][b]) == mv and got["cost"][b] == c and got["evals"][b] == ev, b
+
+
+def test_plain_search_1080p_frame_pair(dev):
+    """Config C2 geometry (1920x1088, +-32, 4x4-Hadamard qpel refine) on one frame pair."""
+    frames = synth.hd_clip(2, seed=3)
+    _search_case(dev, frames, 1920, 1088, 32, 16, 16, 32, 0, LAM26)
+
+
+def test_plain_search_batch_equals_single(dev):
+    """One launch over several frames == one launch per frame."""
+    cw, ch, pad = 352, 288, 32
+    frames = synth.cif_clip(5, seed=9)
+    p = accel.search_params(cw, ch, pad, 16, 16, 16, 0, LAM26)
+    cur, refs = dev.upload(frames[1:]), dev.upload(frames[:-1])
+    pl = accel.build_planes(dev, refs, 4, cw, ch, pad)
+    ms = accel.MotionSearch(dev, p)
+    ms.run_plain(cur, pl, 4)
+    batch = ms.results(4)
+    pb = accel.planes_bytes(cw, ch, pad)
+    for f in range(4):
+        ms1 = accel.MotionSearch(dev, p)
+        ms1.run_plain(int(cur) + f * cw * ch, int(pl) + f * pb, 1)
+        one = ms1.results(1)
+        sl = slice(f * ms.nblk, (f + 1) * ms.nblk)
+        assert np.array_equal(one["mv"], batch["mv"][sl]) and np.array_equal(one["cost"], batch["cost"][sl])

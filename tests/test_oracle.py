@@ -1,0 +1,347 @@
+"""Pins the CPU oracle (oracle/me_oracle.c) against closed forms, textbook
+restatements (tests/refimpl.py), brute force, invariants and the reference's own
+test-input generators.  CPU only."""
+import numpy as np
+import pytest
+
+import refimpl
+from oracle.oracle import Oracle
+from rusty_h264_amd import synth
+
+O = Oracle()
+SHAPES = [(16, 16), (16, 8), (8, 16), (8, 8), (8, 4), (4, 8), (4, 4)]
+
+
+# ---------------------------------------------------------------- block metrics
+
+def test_sad_satd_corpus_matches_matrix_forms():
+    """satd_sad.rs:401-422 corpus (96 seeds), every shape, vs matrix Hadamard."""
+    for seed in range(96):
+        a, b = synth.corpus_pair(seed)
+        for w, h in SHAPES:
+            aa, bb = a[:h, :w], b[:h, :w]
+            assert O.sad(a, 16, b, 16, w, h) == refimpl.sad(aa, bb)
+            s = refimpl.satd_sum(aa, bb)
+            assert O.satd_px(a, 16, b, 16, w, h) == s
+            assert O.satd_wels(a, 16, b, 16, w, h) == refimpl.satd_wels(aa, bb)
+
+
+def test_satd_closed_forms():
+    # constant difference c: only the DC coefficient survives, 16|c| per 4x4
+    for c in (-255, -17, -1, 1, 3, 200):
+        a = np.full((16, 16), max(c, 0), np.uint8)
+        b = np.full((16, 16), max(-c, 0), np.uint8)
+        assert O.satd_px(a, 16, b, 16, 16, 16) == 16 * 16 * abs(c)
+        assert O.satd_wels(a, 16, b, 16, 16, 16) == 16 * 8 * abs(c)
+        assert O.sad(a, 16, b, 16, 16, 16) == 256 * abs(c)
+
+
+def test_satd_parity_and_halving():
+    """sum|H d| of every 4x4 is even, so 2*wels == sum exactly (satd_asm_compare.rs:38-74)."""
+    rng = np.random.default_rng(1)
+    for _ in range(400):
+        a = rng.integers(0, 256, (4, 4), dtype=np.uint8)
+        b = rng.integers(0, 256, (4, 4), dtype=np.uint8)
+        s = O.satd_px(a, 4, b, 4, 4, 4)
+        assert s % 2 == 0
+        assert 2 * O.satd_wels(a, 4, b, 4, 4, 4) == s
+
+
+def test_hadamard_matches_matrix_and_bounds():
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        d = rng.integers(-255, 256, (4, 4))
+        assert np.array_equal(O.hadamard_4x4(d).reshape(4, 4), refimpl.hadamard(d))
+    ext = np.full((4, 4), 255)
+    assert np.abs(O.hadamard_4x4(ext)).max() == 4080
+
+
+def test_extremes_and_checkerboards():
+    """satd_sad.rs:424-442 extremal planes and alternating patterns."""
+    i, j = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
+    cb = np.where((i + j) % 2 == 0, 255, 0).astype(np.uint8)
+    rows = np.where(i % 2 == 0, 0, 255).astype(np.uint8)
+    for a, b in [(np.zeros((16, 16), np.uint8), np.full((16, 16), 255, np.uint8)), (cb, rows)]:
+        for w, h in SHAPES:
+            assert O.satd_px(a, 16, b, 16, w, h) == refimpl.satd_sum(a[:h, :w], b[:h, :w])
+            assert O.sad(a, 16, b, 16, w, h) == refimpl.sad(a[:h, :w], b[:h, :w])
+
+
+def test_lcg_strided_blocks():
+    """satd_sad.rs:459-489: LCG seed 0x777, strides 24/20."""
+    buf = synth.lcg_u8(0x777, 40 * (24 * 20 + 20 * 20))
+    off = 0
+    for _ in range(40):
+        a = buf[off:off + 24 * 20]; off += 24 * 20
+        b = buf[off:off + 20 * 20]; off += 20 * 20
+        A, B = a.reshape(20, 24), b.reshape(20, 20)
+        for w, h in SHAPES:
+            assert O.satd_px(a, 24, b, 20, w, h) == refimpl.satd_sum(A[:h, :w], B[:h, :w])
+            assert O.sad(a, 24, b, 20, w, h) == refimpl.sad(A[:h, :w], B[:h, :w])
+
+
+def test_satd_avg_equals_materialized():
+    """satd_avg_compare.rs:17-47: fused avg-SATD == SATD of the materialized average,
+    and satd_avg(a, a) == satd(a)."""
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        s = rng.integers(0, 256, (16, 16), dtype=np.uint8)
+        a = rng.integers(0, 256, (16, 16), dtype=np.uint8)
+        b = rng.integers(0, 256, (16, 16), dtype=np.uint8)
+        avg = ((a.astype(int) + b + 1) >> 1).astype(np.uint8)
+        for w, h in [(16, 16), (16, 8), (8, 16), (8, 8)]:
+            assert O.satd_avg(s, 16, a, b, 16, w, h) == refimpl.satd_sum(s[:h, :w], avg[:h, :w])
+            assert O.satd_avg(s, 16, a, a, 16, w, h) == O.satd_px(s, 16, a, 16, w, h)
+
+
+# ---------------------------------------------------------------- interpolation
+
+@pytest.mark.parametrize("cw,ch,pad,seed", [(64, 48, 16, 0), (80, 64, 12, 1), (176, 144, 32, 2), (32, 32, 4, 3)])
+def test_planes_tiles_fused_and_separable(cw, ch, pad, seed):
+    """fused == tiles (inter.rs:1823-1869 geometries) == scipy separable filter."""
+    y = np.random.default_rng(seed).integers(0, 256, (ch, cw), dtype=np.uint8)
+    t = O.planes(y, cw, ch, pad, fused=False)
+    f = O.planes(y, cw, ch, pad, fused=True)
+    assert np.array_equal(t, f)
+    assert np.array_equal(t, refimpl.planes(y, pad))
+
+
+def test_planes_flat_and_ramp():
+    flat = np.full((32, 48), 77, np.uint8)
+    assert (O.planes(flat, 48, 32, 8) == 77).all()
+    # horizontal ramp: interior H is the exact midpoint (taps symmetric, sum 32)
+    ramp = np.tile((np.arange(48) * 4).astype(np.uint8), (32, 1))
+    p = O.planes(ramp, 48, 32, 8)
+    x = np.arange(8 + 3, 8 + 48 - 4)
+    assert np.array_equal(p[1, 20, x].astype(int), (ramp[0, x - 8].astype(int) * 2 + 4) // 2)
+
+
+def test_luma_sample_matches_spec_table():
+    """luma_sample (inter.rs:134-166) == Table 8-12 equations over separable planes."""
+    cw, ch, pad = 48, 32, 8
+    y = np.random.default_rng(4).integers(0, 256, (ch, cw), dtype=np.uint8)
+    spec = refimpl.spec_samples(y, pad)
+    rng = np.random.default_rng(5)
+    for _ in range(600):
+        ix, iy = int(rng.integers(-6, cw + 6)), int(rng.integers(-6, ch + 6))
+        fx, fy = int(rng.integers(0, 4)), int(rng.integers(0, 4))
+        v = O.luma_sample(y, cw, ch, ix, iy, fx, fy)
+        assert v == spec[(fx, fy)][iy + pad, ix + pad], (ix, iy, fx, fy)
+
+
+def test_mc_luma_matches_luma_sample():
+    """inter.rs:1913+: block MC == per-pixel spec sample, all 16 phases, edges included."""
+    cw, ch = 48, 32
+    y = np.random.default_rng(6).integers(0, 256, (ch, cw), dtype=np.uint8)
+    rng = np.random.default_rng(7)
+    for _ in range(120):
+        bw, bh = SHAPES[int(rng.integers(0, len(SHAPES)))]
+        x0, y0 = int(rng.integers(0, cw - bw + 1)), int(rng.integers(0, ch - bh + 1))
+        mvx, mvy = int(rng.integers(-96, 97)), int(rng.integers(-96, 97))
+        blk = O.mc_luma(y, cw, ch, x0, y0, bw, bh, mvx, mvy)
+        ref = np.array([[O.luma_sample(y, cw, ch, x0 + (mvx >> 2) + i, y0 + (mvy >> 2) + j, mvx & 3, mvy & 3)
+                         for i in range(bw)] for j in range(bh)], np.uint8)
+        assert np.array_equal(blk, ref)
+
+
+def test_hpel_block_and_pred_block_match_mc_luma():
+    """inter.rs:1870-1912: accepted plane reads == mc_luma; the clamped plane read
+    (the GPU rule) == mc_luma for EVERY MV, far outside the pad included."""
+    cw, ch, pad = 64, 48, 32
+    y = np.random.default_rng(8).integers(0, 256, (ch, cw), dtype=np.uint8)
+    pl = O.planes(y, cw, ch, pad)
+    rng = np.random.default_rng(9)
+    accepted = 0
+    for _ in range(800):
+        bw, bh = SHAPES[int(rng.integers(0, len(SHAPES)))]
+        x0, y0 = int(rng.integers(0, cw - bw + 1)), int(rng.integers(0, ch - bh + 1))
+        mvx, mvy = int(rng.integers(-400, 401)), int(rng.integers(-400, 401))
+        ref = O.mc_luma(y, cw, ch, x0, y0, bw, bh, mvx, mvy)
+        assert np.array_equal(O.pred_block(pl, cw, ch, pad, x0, y0, bw, bh, mvx, mvy), ref)
+        hb = O.hpel_block(pl, cw, ch, pad, x0, y0, bw, bh, mvx, mvy)
+        if hb is not None:
+            accepted += 1
+            assert np.array_equal(hb, ref)
+    assert accepted > 50
+
+
+# ---------------------------------------------------------------- search
+
+def test_mvbits_closed_form():
+    for d in range(-5000, 5001):
+        assert O.mvbits(d) == refimpl.mvbits(d)
+    assert [O.mvbits(d) for d in (0, 1, -1, 2, -2, 3)] == [1, 3, 3, 5, 5, 5]
+
+
+def test_lambda_table_truncates():
+    lam = 4.6463
+    lr = O.lambda_table(lam)
+    assert list(lr[:5]) == [int(lam * b) for b in range(5)]
+
+
+def _pair(cw, ch, seed, shift=(3, -2)):
+    rng = np.random.default_rng(seed)
+    ref = rng.integers(0, 256, (ch, cw), dtype=np.uint8)
+    ref = ((ref.astype(int) + np.roll(ref, 1, 1) + np.roll(ref, 1, 0)) // 3).astype(np.uint8)
+    cur = np.roll(ref, shift, axis=(0, 1))
+    return cur, ref
+
+
+@pytest.mark.parametrize("bw,bh,R,lam", [(16, 16, 4, 0.0), (16, 16, 5, 4.65), (8, 8, 3, 2.0), (4, 8, 2, 1.0),
+                                         (16, 8, 6, 3.0)])
+def test_full_search_is_brute_force(bw, bh, R, lam):
+    cw, ch, pad = 48, 32, 8
+    cur, ref = _pair(cw, ch, 10 + bw)
+    pl = O.planes(ref, cw, ch, pad)
+    cfg = O.cfg(cw, ch, pad, bw, bh, R, 0, lam)
+    lr_fp = O.lambda_table(lam * 0.5)
+    rng = np.random.default_rng(11)
+    for by in range(0, ch, bh):
+        for bx in range(0, cw, bw):
+            cx, cy = 4 * int(rng.integers(-12, 13)), 4 * int(rng.integers(-12, 13))
+            mvp = (int(rng.integers(-40, 41)), int(rng.integers(-40, 41)))
+            got = O.full_search_block(cfg, cur, pl, bx, by, cx, cy, *mvp)
+            want = refimpl.full_search(cur, pl[0], pad, bx, by, bw, bh, R, cx, cy, mvp, lr_fp)
+            assert got == want, (bx, by)
+
+
+def test_full_search_tie_break_raster_first():
+    """A flat frame makes every candidate cost equal (lambda 0): the raster-first
+    candidate (dy=-R, dx=-R) must win, as mb16.rs:2843-2855's strict < keeps."""
+    cw, ch, pad, R = 32, 32, 8, 3
+    f = np.full((ch, cw), 90, np.uint8)
+    pl = O.planes(f, cw, ch, pad)
+    cfg = O.cfg(cw, ch, pad, 16, 16, R, 0, 0.0)
+    assert O.full_search_block(cfg, f, pl, 16, 16)[0] == (-4 * R, -4 * R)
+
+
+def _subpel_python(cost, start, pat):
+    """The ring walk of mb16.rs:3230-3275 over an arbitrary pure cost function."""
+    bx, by = start
+    best = cost(bx, by)
+    for step in (2, 1):
+        ring = [(step, 0), (-step, 0), (0, step), (0, -step), (step, step), (-step, -step), (step, -step),
+                (-step, step)][: 4 if pat & 1 else 8]
+        while True:
+            improved = False
+            for dx, dy in ring:
+                c = cost(bx + dx, by + dy)
+                if c < best:
+                    best, bx, by, improved = c, bx + dx, by + dy, True
+            if not improved or pat & 2:
+                break
+    return (bx, by), best
+
+
+@pytest.mark.parametrize("pat", [0, 1, 2, 3])
+def test_subpel_refine_trace(pat):
+    """Oracle ring walk == the walk over a cost computed by the textbook path
+    (spec sample table + matrix Hadamard + Exp-Golomb length)."""
+    cw, ch, pad = 48, 32, 16
+    cur, ref = _pair(cw, ch, 20 + pat, shift=(1, 2))
+    pl = O.planes(ref, cw, ch, pad)
+    lam = 4.65
+    cfg = O.cfg(cw, ch, pad, 16, 16, 4, pat, lam)
+    lr = O.lambda_table(lam)
+    spec = refimpl.spec_samples(ref, pad)
+    for by in range(0, ch, 16):
+        for bx in range(0, cw, 16):
+            mvp = (3, -5)
+
+            def cost(mx, my, bx=bx, by=by):
+                x = bx + (mx >> 2) + pad
+                y = by + (my >> 2) + pad
+                pred = spec[(mx & 3, my & 3)][y:y + 16, x:x + 16]
+                return refimpl.satd_sum(cur[by:by + 16, bx:bx + 16], pred) + \
+                    lr[refimpl.mvbits(mx - mvp[0]) + refimpl.mvbits(my - mvp[1])]
+
+            start = (-8, 4)
+            want = _subpel_python(cost, start, pat)
+            mv, c, ev = O.subpel_refine_block(cfg, cur, pl, bx, by, *start, *mvp)
+            assert (mv, c) == want
+            assert c <= cost(*start)
+
+
+def test_plain_search_frame_composes_blocks():
+    cw, ch, pad = 64, 48, 32
+    cur, ref = _pair(cw, ch, 30)
+    pl = O.planes(ref, cw, ch, pad)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 8, 0, 4.65)
+    r = O.plain_search_frame(cfg, cur, pl, nthreads=2)
+    for b in range(len(r["mv"])):
+        bx, by = (b % (cw // 16)) * 16, (b // (cw // 16)) * 16
+        imv, ic = O.full_search_block(cfg, cur, pl, bx, by)
+        assert tuple(r["int_mv"][b]) == imv and r["int_cost"][b] == ic
+        mv, c, _ = O.subpel_refine_block(cfg, cur, pl, bx, by, *imv)
+        assert tuple(r["mv"][b]) == mv and r["cost"][b] == c
+    # the synthetic shift is found exactly at integer pel
+    assert (r["int_mv"][len(r["mv"]) // 2] == np.array([8, -12])).all()  # cur(y,x) = ref(y-3, x+2)
+
+
+def test_motion_search_ref_invariants():
+    """mb16.rs:3297-3300: the returned cost never exceeds the seed's cost."""
+    cw, ch, pad = 64, 48, 32
+    cur, ref = _pair(cw, ch, 31)
+    pl = O.planes(ref, cw, ch, pad)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 8, 0, 4.65)
+    for by in range(0, ch, 16):
+        for bx in range(0, cw, 16):
+            preds = [(5, -3)]
+            mv, c = O.motion_search_ref(cfg, cur, pl, bx, by, preds)
+            seed_c = min(O.satd_cost(cfg, cur, pl, bx, by, 0, 0, 5, -3), O.satd_cost(cfg, cur, pl, bx, by, 5, -3, 5, -3))
+            assert c <= seed_c
+
+
+# ---------------------------------------------------------------- mb-tree
+
+def test_downsample_and_intra_cost():
+    y = np.random.default_rng(40).integers(0, 256, (32, 48), dtype=np.uint8)
+    half = O.downsample2x(y, 48, 32)
+    want = (y[0::2, 0::2].astype(int) + y[0::2, 1::2] + y[1::2, 0::2] + y[1::2, 1::2] + 2) // 4
+    assert np.array_equal(half, want)
+    for bx, by in [(0, 0), (8, 8), (16, 8)]:
+        blk = half[by:by + 8, bx:bx + 8].astype(np.int64)
+        s = 0
+        for yy in range(0, 8, 4):
+            for xx in range(0, 8, 4):
+                h = refimpl.hadamard(blk[yy:yy + 4, xx:xx + 4])
+                s += int(np.abs(h).sum() - abs(h[0, 0]))
+        assert O.intra_cost(half, 24, bx, by, 8) == max(s, 1)
+    assert O.intra_cost(np.full((16, 16), 9, np.uint8), 16, 0, 0, 8) == 1
+
+
+def test_inter_cost_invariants():
+    cw, ch = 48, 32
+    cur, ref = _pair(cw, ch, 41, shift=(2, 1))
+    assert O.inter_cost(ref, cw, ch, ref, 16, 8, 8)[0] == 0  # static -> zero cost, early out
+    c0 = O.inter_cost(cur, cw, ch, ref, 16, 8, 8, max_step=0)[0]  # no diamond: the (0,0) cost
+    c, mv = O.inter_cost(cur, cw, ch, ref, 16, 8, 8)
+    assert c <= c0
+    # cur(y,x) = ref(y-2, x-1): seeded with the true motion the seed probe finds it exactly
+    c, mv = O.inter_cost(cur, cw, ch, ref, 16, 8, 8, seed=(-4, -8))
+    assert c == 0 and mv == (-4, -8)
+
+
+def test_frame_costs_and_propagation():
+    mb_w, mb_h = 4, 3
+    frames = [np.random.default_rng(50).integers(0, 256, (48, 64), dtype=np.uint8)]
+    for t in range(3):
+        frames.append(np.roll(frames[-1], (1, 2), axis=(0, 1)))
+    halves = [O.downsample2x(f, 64, 48) for f in frames]
+    intra, inter, mv = [], [], []
+    for i, h in enumerate(halves):
+        a, b, m = O.frame_costs_half(h, halves[i - 1] if i else None, mb_w, mb_h)
+        assert (b <= a).all() and (a >= 1).all()
+        intra.append(a); inter.append(b); mv.append(m)
+    intra, inter, mv = np.stack(intra), np.stack(inter), np.stack(mv)
+    p = O.propagate_f64(intra, inter, mv, mb_w, mb_h)
+    assert (p[-1] == 0).all()
+    # conservation: what frame f sends equals what frame f-1 receives
+    for f in range(len(frames) - 1, 0, -1):
+        sent = ((intra[f] + p[f]) * (intra[f] - inter[f]) / intra[f]).sum()
+        assert abs(p[f - 1].sum() - sent) < 1e-6 * max(1.0, sent)
+    pi = O.propagate_i32(intra, inter, mv, mb_w, mb_h)
+    for f in range(len(frames) - 1, 0, -1):
+        amount = ((intra[f].astype(np.int64) + pi[f]) * (intra[f] - inter[f])) // intra[f]
+        sent = int(amount.sum())
+        assert sent - 4 * mb_w * mb_h <= int(pi[f - 1].sum()) <= sent
