@@ -23,15 +23,16 @@ def _finish(frames, rng, sigma):
 
 def cif_clip(nframes=60, seed=0, w=352, h=288):
     rng = np.random.default_rng(seed)
-    tex = rng.integers(0, 256, (64, 64)).astype(np.float64)
+    bs = min(64, w // 2, h // 2)  # 64x64 at CIF; smaller test frames get a smaller box
+    tex = rng.integers(0, 256, (bs, bs)).astype(np.float64)
     yy = np.arange(h, dtype=np.float64)[:, None]
     frames = np.empty((nframes, h, w), np.float64)
     for t in range(nframes):
         # vertical gradient scrolling 1 px/frame
         frames[t] = np.broadcast_to(((yy + t) % h) * (235.0 / h) + 16.0, (h, w))
         bx, by = 40 + 3 * t, 30 + 2 * t
-        x0, y0 = bx % (w - 64), by % (h - 64)
-        frames[t, y0:y0 + 64, x0:x0 + 64] = tex
+        x0, y0 = bx % (w - bs), by % (h - bs)
+        frames[t, y0:y0 + bs, x0:x0 + bs] = tex
     return _finish(frames, rng, 2.0)
 
 
