@@ -95,7 +95,8 @@ class Oracle:
 
     def mc_luma(self, ref, cw, ch, x0, y0, bw, bh, mvx, mvy):
         out = np.empty((bh, bw), np.uint8)
-        self.lib.orc_mc_luma(_p(_u8(ref)), cw, ch, x0, y0, bw, bh, mvx, mvy, _p(out))
+        self.lib.orc_mc_luma(_p(_u8(ref)), int(cw), int(ch), int(x0), int(y0), int(bw), int(bh), int(mvx), int(mvy),
+                             _p(out))
         return out
 
     def planes(self, y, cw, ch, pad=32, fused=False):
@@ -106,12 +107,14 @@ class Oracle:
 
     def hpel_block(self, planes, cw, ch, pad, x0, y0, bw, bh, mvx, mvy):
         out = np.zeros((bh, bw), np.uint8)
-        ok = self.lib.orc_hpel_block(_p(_u8(planes)), cw, ch, pad, x0, y0, bw, bh, mvx, mvy, _p(out))
+        ok = self.lib.orc_hpel_block(_p(_u8(planes)), int(cw), int(ch), int(pad), int(x0), int(y0), int(bw), int(bh),
+                                     int(mvx), int(mvy), _p(out))
         return out if ok else None
 
     def pred_block(self, planes, cw, ch, pad, x0, y0, bw, bh, mvx, mvy):
         out = np.empty((bh, bw), np.uint8)
-        self.lib.orc_pred_block(_p(_u8(planes)), cw, ch, pad, x0, y0, bw, bh, mvx, mvy, _p(out))
+        self.lib.orc_pred_block(_p(_u8(planes)), int(cw), int(ch), int(pad), int(x0), int(y0), int(bw), int(bh),
+                                int(mvx), int(mvy), _p(out))
         return out
 
     # ---- search
@@ -132,19 +135,19 @@ class Oracle:
 
     def full_search_block(self, cfg, cur, planes, lx, ly, cx=0, cy=0, mvpx=0, mvpy=0):
         mv = np.empty(2, np.int32)
-        c = self.lib.orc_full_search_block(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), lx, ly, cx, cy,
-                                           mvpx, mvpy, _p(mv, i32p))
+        c = self.lib.orc_full_search_block(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), int(lx), int(ly), int(cx),
+                                           int(cy), int(mvpx), int(mvpy), _p(mv, i32p))
         return (int(mv[0]), int(mv[1])), int(c)
 
     def satd_cost(self, cfg, cur, planes, lx, ly, mvx, mvy, mvpx=0, mvpy=0):
-        return int(self.lib.orc_satd_cost(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), lx, ly, mvx, mvy,
-                                          mvpx, mvpy))
+        return int(self.lib.orc_satd_cost(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), int(lx), int(ly), int(mvx),
+                                          int(mvy), int(mvpx), int(mvpy)))
 
     def subpel_refine_block(self, cfg, cur, planes, lx, ly, mvx, mvy, mvpx=0, mvpy=0):
         mv = np.empty(2, np.int32)
         ev = C.c_int32()
-        c = self.lib.orc_subpel_refine_block(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), lx, ly, mvx, mvy,
-                                             mvpx, mvpy, _p(mv, i32p), C.byref(ev))
+        c = self.lib.orc_subpel_refine_block(C.byref(cfg), _p(_u8(cur)), _p(_u8(planes)), int(lx), int(ly),
+                                             int(mvx), int(mvy), int(mvpx), int(mvpy), _p(mv, i32p), C.byref(ev))
         return (int(mv[0]), int(mv[1])), int(c), ev.value
 
     def plain_search_frame(self, cfg, cur, planes, centres=None, mvps=None, nthreads=1):

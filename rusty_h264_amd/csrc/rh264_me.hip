@@ -98,6 +98,14 @@ static int ensure_tmp_mv(rh264_ctx *ctx, size_t n)
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 __device__ __forceinline__ int clip_u8(int v) { return min(max(v, 0), 255); }
+// clip_u8((v + round) >> shift) computed as a clamp followed by a logical shift:
+// for v + round < 0 both give 0, above (256 << shift) - 1 both give 255.  Written
+// this way so the shift/saturate/pack idiom is not formed (gfx950's packed
+// shift-and-saturate form gave wrong bytes for this kernel).
+template <int SHIFT, int ROUND> __device__ __forceinline__ uint32_t round_clip_u8(int v)
+{
+    return (uint32_t)min(max(v + ROUND, 0), (256 << SHIFT) - 1) >> SHIFT;
+}
 
 // Exp-Golomb se(d) length, mb16.rs:2474-2475: 1 + 2*floor(log2(codenum+1)).
 // |d| is clamped to 2^24 quarter pels (far beyond any coded MV) so 2d-1 cannot wrap.
@@ -223,9 +231,9 @@ __global__ __launch_bounds__(256) void k_build_planes(const uint8_t *__restrict_
             int cs = (int)vt[0] - 5 * (int)vt[1] + 20 * (int)vt[2] + 20 * (int)vt[3] -
                      5 * (int)vt[4] + (int)vt[5];
             f4 |= (uint32_t)in[2] << (8 * k);
-            h4 |= (uint32_t)clip_u8((hs + 16) >> 5) << (8 * k);
-            v4 |= (uint32_t)clip_u8(((int)vt[2] + 16) >> 5) << (8 * k);
-            c4 |= (uint32_t)clip_u8((cs + 512) >> 10) << (8 * k);
+            h4 |= round_clip_u8<5, 16>(hs) << (8 * k);
+            v4 |= round_clip_u8<5, 16>((int)vt[2]) << (8 * k);
+            c4 |= round_clip_u8<10, 512>(cs) << (8 * k);
         }
         const size_t o = (size_t)gy * pw + gx;
         *(uint32_t *)(out + o) = f4;

@@ -82,7 +82,15 @@ def test_planes_bit_exact(dev, cw, ch, pad, nf):
     pw, ph = cw + 2 * pad, ch + 2 * pad
     for i in range(nf):
         want = O.planes(frames[i], cw, ch, pad, fused=True)
-        assert np.array_equal(got[i, :4 * pw * ph].reshape(4, ph, pw), want), i
+        g = got[i, :4 * pw * ph].reshape(4, ph, pw)
+        if not np.array_equal(g, want):
+            msg = []
+            for k in range(4):
+                ys, xs = np.nonzero(g[k] != want[k])
+                if ys.size:
+                    msg.append(f"plane {'FHVC'[k]}: {ys.size} px differ, rows {np.unique(ys)[:8]} "
+                               f"cols {np.unique(xs)[:8]}, got {g[k][ys[0], xs[0]]} want {want[k][ys[0], xs[0]]}")
+            raise AssertionError(f"frame {i}: " + "; ".join(msg))
     fb.free(); pl.free()
 
 
