@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "librh264_me.so")
+# RH264_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("RH264_LIB") or os.path.join(_HERE, "librh264_me.so")
 LR_N = 128
 
 c_u8p = C.POINTER(C.c_uint8)

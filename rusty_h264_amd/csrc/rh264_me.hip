@@ -119,31 +119,17 @@ __device__ __forceinline__ int mvbits(int d)
 // Quarter-phase operand table, inter.rs:1079-1092 / mectx.rs:143-156:
 // for idx = fx*4+fy, (plane_a, ax, ay, plane_b, bx, by) packed 4 bits each.
 // Planes: 0=F, 1=H, 2=V, 3=C.  Even phases use one plane (in plane_a).
-__device__ __forceinline__ uint32_t qtab(int idx)
-{
-#define Q(pa, ax, ay, pb, bx, by) \
+#define QENT(pa, ax, ay, pb, bx, by) \
     ((uint32_t)(pa) | ((uint32_t)(ax) << 4) | ((uint32_t)(ay) << 8) | ((uint32_t)(pb) << 12) | \
      ((uint32_t)(bx) << 16) | ((uint32_t)(by) << 20))
-    switch (idx) {
-    case 0 * 4 + 0: return Q(0, 0, 0, 0, 0, 0);
-    case 2 * 4 + 0: return Q(1, 0, 0, 1, 0, 0);
-    case 0 * 4 + 2: return Q(2, 0, 0, 2, 0, 0);
-    case 2 * 4 + 2: return Q(3, 0, 0, 3, 0, 0);
-    case 1 * 4 + 0: return Q(0, 0, 0, 1, 0, 0);
-    case 3 * 4 + 0: return Q(0, 1, 0, 1, 0, 0);
-    case 0 * 4 + 1: return Q(0, 0, 0, 2, 0, 0);
-    case 0 * 4 + 3: return Q(0, 0, 1, 2, 0, 0);
-    case 1 * 4 + 1: return Q(1, 0, 0, 2, 0, 0);
-    case 3 * 4 + 1: return Q(1, 0, 0, 2, 1, 0);
-    case 1 * 4 + 3: return Q(1, 0, 1, 2, 0, 0);
-    case 3 * 4 + 3: return Q(1, 0, 1, 2, 1, 0);
-    case 2 * 4 + 1: return Q(1, 0, 0, 3, 0, 0);
-    case 2 * 4 + 3: return Q(1, 0, 1, 3, 0, 0);
-    case 1 * 4 + 2: return Q(2, 0, 0, 3, 0, 0);
-    default: return Q(2, 1, 0, 3, 0, 0); // (3,2)
-    }
-#undef Q
-}
+__constant__ uint32_t c_qtab[16] = {
+    /* fx=0: fy=0..3 */ QENT(0, 0, 0, 0, 0, 0), QENT(0, 0, 0, 2, 0, 0), QENT(2, 0, 0, 2, 0, 0), QENT(0, 0, 1, 2, 0, 0),
+    /* fx=1 */          QENT(0, 0, 0, 1, 0, 0), QENT(1, 0, 0, 2, 0, 0), QENT(2, 0, 0, 3, 0, 0), QENT(1, 0, 1, 2, 0, 0),
+    /* fx=2 */          QENT(1, 0, 0, 1, 0, 0), QENT(1, 0, 0, 3, 0, 0), QENT(3, 0, 0, 3, 0, 0), QENT(1, 0, 1, 3, 0, 0),
+    /* fx=3 */          QENT(0, 1, 0, 1, 0, 0), QENT(1, 0, 0, 2, 1, 0), QENT(2, 1, 0, 3, 0, 0), QENT(1, 0, 1, 2, 1, 0),
+};
+#undef QENT
+__device__ __forceinline__ uint32_t qtab(int idx) { return c_qtab[idx]; }
 
 // Per-byte rounded-up average (a+b+1)>>1 of four packed bytes.
 __device__ __forceinline__ uint32_t avg4(uint32_t a, uint32_t b)
@@ -287,8 +273,79 @@ __device__ __forceinline__ uint64_t wave_seg_min(uint64_t key, int seg, int lane
     return key;
 }
 
+// State of one lane of the integer search.
+template <int BW, int BH> struct FsLane {
+    uint32_t src[BH][FsShape<BW, BH>::SD];
+    uint64_t acc[BH];
+    uint32_t best, bidx;
+    int rxb[4];
+};
+
+// Rows jb .. jb+BH-1 of the lane's window.  MODE 0: every (row, source row) pair;
+// MODE 1: first block of a window whose height is a multiple of BH -- only pairs
+// with candidate row t = j - k >= 0; MODE 2: last such block -- only t <= 2R.  The
+// masks are compile-time after unrolling, so no quad-SAD is spent on candidate rows
+// outside [-R, R].
+template <int BW, int BH, int MODE>
+__device__ __forceinline__ void fs_rows(FsLane<BW, BH> &L, const uint8_t *__restrict__ F, int pw, int ph,
+                                        bool fast, int px0, int py0, int shift, int jb, int NJ, int R, int NC,
+                                        int g, int cy, int mpy, const int32_t *s_lr)
+{
+    using S = FsShape<BW, BH>;
+#pragma unroll
+    for (int jj = 0; jj < BH; jj++) {
+        const int j = jb + jj;
+        if (MODE == 0 && j >= NJ) break;
+        uint32_t w[S::RD];
+        const int ry = py0 + j;
+        if (fast) {
+            const uint32_t *rp = (const uint32_t *)(F + (size_t)ry * pw + (px0 & ~3));
+            uint32_t a[S::RD + 1];
+#pragma unroll
+            for (int i = 0; i < S::RD + 1; i++) a[i] = rp[i];
+#pragma unroll
+            for (int i = 0; i < S::RD; i++) w[i] = __builtin_amdgcn_alignbyte(a[i + 1], a[i], shift);
+        } else {
+#pragma unroll
+            for (int i = 0; i < S::RD; i++) w[i] = load4_clamped(F, pw, ph, px0 + 4 * i, ry);
+        }
+#pragma unroll
+        for (int k = 0; k < BH; k++) {
+            if (MODE == 1 && k > jj) continue;
+            if (MODE == 2 && k < jj) continue;
+            const int slot = (jj - k + BH) % BH;
+#pragma unroll
+            for (int i = 0; i < S::SD; i++) {
+                uint64_t ref8 = ((uint64_t)w[i + 1] << 32) | w[i];
+                L.acc[slot] = __builtin_amdgcn_qsad_pk_u16_u8(ref8, L.src[k][i], L.acc[slot]);
+            }
+        }
+        const int t = j - (BH - 1);
+        const int sd = (jj + 1) % BH;
+        if (t >= 0) {
+            const int ry_bits = mvbits(cy + 4 * (t - R) - mpy);
+            const uint64_t a4 = L.acc[sd];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int c = 4 * g + k;
+                const uint32_t sad = (uint32_t)(a4 >> (16 * k)) & 0xffffu;
+                const uint32_t cost = sad + (uint32_t)s_lr[L.rxb[k] + ry_bits];
+                if (c < NC && cost < L.best) {
+                    L.best = cost;
+                    L.bidx = (uint32_t)(t * NC + c);
+                }
+            }
+        }
+        L.acc[sd] = 0;
+    }
+}
+
+#ifndef RH264_FS_WAVES
+#define RH264_FS_WAVES 2
+#endif
 template <int BW, int BH>
-__global__ __launch_bounds__(256) void k_full_search(rh264_search_params p,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
+void k_full_search(rh264_search_params p,
                                                      const uint8_t *__restrict__ cur,
                                                      size_t cur_stride,
                                                      const uint8_t *__restrict__ planes,
@@ -319,14 +376,13 @@ __global__ __launch_bounds__(256) void k_full_search(rh264_search_params p,
     const int cx = centres ? centres[2 * boff] : 0, cy = centres ? centres[2 * boff + 1] : 0;
     const int mpx = mvps ? mvps[2 * boff] : 0, mpy = mvps ? mvps[2 * boff + 1] : 0;
 
-    // source block
-    uint32_t src[BH][S::SD];
+    FsLane<BW, BH> L;
     {
         const uint8_t *sp = cur + (size_t)frame * cur_stride + (size_t)ly * p.cw + lx;
 #pragma unroll
         for (int k = 0; k < BH; k++)
 #pragma unroll
-            for (int i = 0; i < S::SD; i++) src[k][i] = *(const uint32_t *)(sp + (size_t)k * p.cw + 4 * i);
+            for (int i = 0; i < S::SD; i++) L.src[k][i] = *(const uint32_t *)(sp + (size_t)k * p.cw + 4 * i);
     }
 
     const int dx0 = -R + 4 * g;
@@ -337,69 +393,31 @@ __global__ __launch_bounds__(256) void k_full_search(rh264_search_params p,
     const bool fast = wx_lo >= 0 && wx_hi <= pw - 1 && py0 >= 0 && py0 + 2 * R + BH - 1 <= ph - 1;
     const int shift = px0 & 3;
 
-    int rxb[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) rxb[k] = mvbits(cx + 4 * (dx0 + k) - mpx);
-
-    uint64_t acc[BH];
+    for (int k = 0; k < 4; k++) L.rxb[k] = mvbits(cx + 4 * (dx0 + k) - mpx);
 #pragma unroll
-    for (int s = 0; s < BH; s++) acc[s] = 0;
-    uint32_t best = 0xffffffffu, bidx = 0xffffffffu;
+    for (int s = 0; s < BH; s++) L.acc[s] = 0;
+    L.best = 0xffffffffu;
+    L.bidx = 0xffffffffu;
     const int NJ = 2 * R + BH;
 
-    for (int jb = 0; jb < NJ; jb += BH) {
-#pragma unroll
-        for (int jj = 0; jj < BH; jj++) {
-            const int j = jb + jj;
-            if (j < NJ) {
-                uint32_t w[S::RD];
-                const int ry = py0 + j;
-                if (fast) {
-                    const uint32_t *rp = (const uint32_t *)(F + (size_t)ry * pw + (px0 & ~3));
-                    uint32_t a[S::RD + 1];
-#pragma unroll
-                    for (int i = 0; i < S::RD + 1; i++) a[i] = rp[i];
-#pragma unroll
-                    for (int i = 0; i < S::RD; i++) w[i] = __builtin_amdgcn_alignbyte(a[i + 1], a[i], shift);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < S::RD; i++) w[i] = load4_clamped(F, pw, ph, px0 + 4 * i, ry);
-                }
-#pragma unroll
-                for (int k = 0; k < BH; k++) {
-                    const int slot = (jj - k + BH) % BH;
-#pragma unroll
-                    for (int i = 0; i < S::SD; i++) {
-                        uint64_t ref8 = ((uint64_t)w[i + 1] << 32) | w[i];
-                        acc[slot] = __builtin_amdgcn_qsad_pk_u16_u8(ref8, src[k][i], acc[slot]);
-                    }
-                }
-                const int t = j - (BH - 1);
-                const int sd = (jj + 1) % BH;
-                if (t >= 0) {
-                    const int ry_bits = mvbits(cy + 4 * (t - R) - mpy);
-                    const uint64_t a4 = acc[sd];
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        const int c = 4 * g + k;
-                        const uint32_t sad = (uint32_t)(a4 >> (16 * k)) & 0xffffu;
-                        const uint32_t cost = sad + (uint32_t)s_lr[rxb[k] + ry_bits];
-                        if (c < NC && cost < best) {
-                            best = cost;
-                            bidx = (uint32_t)(t * NC + c);
-                        }
-                    }
-                }
-                acc[sd] = 0;
-            }
-        }
+    if ((2 * R) % BH == 0) {
+        // window height NJ is a whole number of ring blocks: peel the first and last
+        fs_rows<BW, BH, 1>(L, F, pw, ph, fast, px0, py0, shift, 0, NJ, R, NC, g, cy, mpy, s_lr);
+        for (int jb = BH; jb < NJ - BH; jb += BH)
+            fs_rows<BW, BH, 0>(L, F, pw, ph, fast, px0, py0, shift, jb, NJ, R, NC, g, cy, mpy, s_lr);
+        fs_rows<BW, BH, 2>(L, F, pw, ph, fast, px0, py0, shift, NJ - BH, NJ, R, NC, g, cy, mpy, s_lr);
+    } else {
+        for (int jb = 0; jb < NJ; jb += BH)
+            fs_rows<BW, BH, 0>(L, F, pw, ph, fast, px0, py0, shift, jb, NJ, R, NC, g, cy, mpy, s_lr);
     }
 
-    uint64_t key = active ? (((uint64_t)best << 32) | bidx) : ~0ull;
+    uint64_t key = active ? (((uint64_t)L.best << 32) | L.bidx) : ~0ull;
     const int seg = active ? b : 0x7fffffff;
     key = wave_seg_min(key, seg, lane);
     const int prev_seg = __shfl_up(seg, 1, 64);
-    if (active && (lane == 0 || prev_seg != seg)) atomicMin((unsigned long long *)&keys[boff], (unsigned long long)key);
+    if (active && (lane == 0 || prev_seg != seg))
+        atomicMin((unsigned long long *)&keys[boff], (unsigned long long)key);
 }
 
 // ---------------------------------------------------------------------------
@@ -417,8 +435,10 @@ template <int LPB> __device__ __forceinline__ int group_sum(int v)
     v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true); // quad_perm [2,3,0,1]
     if (LPB >= 8) v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true); // row_half_mirror
     if (LPB >= 16) v += __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true); // row_mirror
-    if (LPB >= 32) v += __shfl_xor(v, 16, 64);
-    if (LPB >= 64) v += __shfl_xor(v, 32, 64);
+    if (LPB == 32) v += __shfl_xor(v, 16, 64);
+    if (LPB == 64) // the group is the wave: four row sums, read into scalars
+        v = __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) +
+            __builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48);
     return v;
 }
 
@@ -485,6 +505,64 @@ __device__ __forceinline__ int eval_satd(const RefineGeom &G, uint32_t srow, int
     return group_sum<LPB>(hadamard_row_quad(d0, d1, d2, d3, r));
 }
 
+// The refine walk reads one block's neighbourhood many times.  Each group stages
+// the four planes around its start vector in LDS once: columns [ix-4, ix+BW+4),
+// rows [iy-3, iy+BH+3) of the padded planes (ix, iy = padded integer position of
+// the start).  Candidates whose integer part lies within [-3, +2] pels of the start
+// on both axes are then served from LDS; the rest read the planes in HBM.
+template <int BW, int BH> struct RefineWin {
+    static constexpr int PITCH = BW + 8;      // bytes per staged row (multiple of 4)
+    static constexpr int ROWS = BH + 6;
+    static constexpr int PLANE = PITCH * ROWS;
+    static constexpr int BYTES = 4 * PLANE;   // F, H, V, C
+};
+
+template <int BW, int BH>
+__device__ __forceinline__ void stage_window(uint8_t *win, const RefineGeom &G, int wx0, int wy0, int li, int lpb)
+{
+    using W = RefineWin<BW, BH>;
+    constexpr int DPR = W::PITCH / 4;       // dwords per staged row
+    constexpr int TOTAL = 4 * W::ROWS * DPR;
+    const bool inside = wx0 >= 0 && wy0 >= 0 && wx0 + W::PITCH <= G.pw && wy0 + W::ROWS <= G.ph;
+    for (int i = li; i < TOTAL; i += lpb) {
+        const int k = i / (W::ROWS * DPR);
+        const int rem = i - k * (W::ROWS * DPR);
+        const int row = rem / DPR, d = rem - row * DPR;
+        const uint8_t *pl = G.pl + G.n * k;
+        const int x = wx0 + 4 * d, y = wy0 + row;
+        const uint32_t v = inside ? load4_fast(pl, G.pw, x, y) : load4_clamped(pl, G.pw, G.ph, x, y);
+        *(uint32_t *)(win + k * W::PLANE + row * W::PITCH + 4 * d) = v;
+    }
+}
+
+__device__ __forceinline__ uint32_t lds_load4(const uint8_t *rowp, int x)
+{
+    const uint32_t *p = (const uint32_t *)(rowp + (x & ~3));
+    return __builtin_amdgcn_alignbyte(p[1], p[0], (uint32_t)(x & 3));
+}
+
+// SATD from the staged window; di, dj = integer offset of the candidate from the
+// start (both in [-3, 2]).
+template <int BW, int BH, int LPB>
+__device__ __forceinline__ int eval_satd_lds(const uint8_t *win, uint32_t srow, int bxi, int byi, int r, int di,
+                                             int dj, int fx, int fy)
+{
+    using W = RefineWin<BW, BH>;
+    const uint32_t q = qtab(fx * 4 + fy);
+    const int x = di + 4 + 4 * bxi, y = dj + 3 + 4 * byi + r;
+    const uint8_t *pa = win + (q & 15) * W::PLANE;
+    uint32_t pred = lds_load4(pa + (y + ((q >> 8) & 15)) * W::PITCH, x + ((q >> 4) & 15));
+    if ((fx | fy) & 1) {
+        const uint8_t *pb = win + ((q >> 12) & 15) * W::PLANE;
+        pred = avg4(pred, lds_load4(pb + (y + ((q >> 20) & 15)) * W::PITCH, x + ((q >> 16) & 15)));
+    }
+    int d0 = (int)(srow & 255) - (int)(pred & 255);
+    int d1 = (int)((srow >> 8) & 255) - (int)((pred >> 8) & 255);
+    int d2 = (int)((srow >> 16) & 255) - (int)((pred >> 16) & 255);
+    int d3 = (int)(srow >> 24) - (int)(pred >> 24);
+    return group_sum<LPB>(hadamard_row_quad(d0, d1, d2, d3, r));
+}
+
 template <int BW, int BH>
 __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
                                                        const uint8_t *__restrict__ cur,
@@ -503,15 +581,18 @@ __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
 {
     constexpr int LPB = BW * BH / 4;
     constexpr int GPB = 256 / LPB; // groups per thread block
+    using W = RefineWin<BW, BH>;
     __shared__ int32_t s_lr[RH264_LR_N];
+    __shared__ __align__(16) uint8_t s_win[GPB * W::BYTES];
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
-    __syncthreads();
 
     const int nbx = p.cw / BW, nby = p.ch / BH, nblk = nbx * nby;
     const int frame = blockIdx.y;
     const int grp = threadIdx.x / LPB, li = threadIdx.x % LPB;
-    const int b = blockIdx.x * GPB + grp;
-    if (b >= nblk) return; // whole groups exit together
+    int b0 = blockIdx.x * GPB + grp;
+    if (LPB == 64) b0 = __builtin_amdgcn_readfirstlane(b0); // the group is the wave: keep its state scalar
+    const bool active = b0 < nblk;
+    const int b = active ? b0 : nblk - 1;
     const size_t boff = (size_t)frame * nblk + b;
     const int lx = (b % nbx) * BW, ly = (b / nbx) * BH;
     const int bi = li >> 2, r = li & 3;
@@ -536,7 +617,7 @@ __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
         const int cx = centres ? centres[2 * boff] : 0, cy = centres ? centres[2 * boff + 1] : 0;
         bx = cx + 4 * (c - p.range);
         by = cy + 4 * (t - p.range);
-        if (li == 0) {
+        if (li == 0 && active) {
             if (int_mv) { int_mv[2 * boff] = bx; int_mv[2 * boff + 1] = by; }
             if (int_cost) int_cost[boff] = (int32_t)(key >> 32);
         }
@@ -545,8 +626,17 @@ __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
         by = start_mv[2 * boff + 1];
     }
 
+    uint8_t *win = s_win + grp * W::BYTES;
+    const int sxi = bx >> 2, syi = by >> 2;
+    stage_window<BW, BH>(win, G, lx + sxi + G.pad - 4, ly + syi + G.pad - 3, li, LPB);
+    __syncthreads();
+    if (!active) return; // whole groups exit together
+
     auto cost_of = [&](int mx, int my) -> int {
-        const int satd = eval_satd<BW, BH, LPB>(G, srow, lx, ly, bxi, byi, r, mx, my);
+        const int di = (mx >> 2) - sxi, dj = (my >> 2) - syi;
+        const int satd = (di >= -3 && di <= 2 && dj >= -3 && dj <= 2)
+                             ? eval_satd_lds<BW, BH, LPB>(win, srow, bxi, byi, r, di, dj, mx & 3, my & 3)
+                             : eval_satd<BW, BH, LPB>(G, srow, lx, ly, bxi, byi, r, mx, my);
         return satd + s_lr[mvbits(mx - mpx) + mvbits(my - mpy)];
     };
 
