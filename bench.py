@@ -51,6 +51,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tune", action="append", default=[], help="knob=value passed to rh264_tune")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the motion search on a single MI355X "
@@ -72,6 +73,9 @@ def main():
         p = accel.search_params(W, H, PAD, 16, 16, R, 0, lam)
         ms = accel.MotionSearch(dev, p)
         lib = dev.lib
+        for kv in args.tune:
+            k, v = kv.split("=")
+            dev.tune(k, int(v))
 
         def step():
             accel.build_planes(dev, refs, nf, W, H, PAD, out=planes)

@@ -155,6 +155,12 @@ int rh264_set_profiling(rh264_ctx *ctx, int capacity);
 int rh264_profile_count(rh264_ctx *ctx);
 int rh264_profile_get(rh264_ctx *ctx, int i, float *search_ms, float *refine_ms);
 
+/* Kernel-variant selection (no reference counterpart; set by the autotuner).
+ * Knobs: "fs_kernel" 0 = quad-SAD (v_qsad_pk_u16_u8) accumulator ring
+ * (default), 1 = v_sad_u8 ring with 2 candidates per lane, 2 = v_sad_u8 ring
+ * with 4 candidates per lane.  Every variant returns identical results. */
+int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
+
 /* Quarter-pel prediction of blocks (mc_luma, inter.rs:1193-1262, served from
  * the plane set as hpel_block inter.rs:1107-1162 does, any MV): block i of
  * size bw x bh at (x[i], y[i]) with mv[2i..2i+1] is written to out + i*bw*bh. */

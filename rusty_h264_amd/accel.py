@@ -105,6 +105,10 @@ class Device:
     def sync(self):
         _check(self.lib.rh264_sync(self.ctx), "rh264_sync")
 
+    def tune(self, knob, value):
+        """Select a kernel variant (rh264_tune); results are identical for every variant."""
+        _check(self.lib.rh264_tune(self.ctx, knob.encode(), int(value)), "rh264_tune")
+
     def record(self, slot):
         _check(self.lib.rh264_event_record(self.ctx, slot), "rh264_event_record")
 

@@ -66,6 +66,7 @@ struct rh264_ctx {
     size_t keys_cap;     // in elements
     int32_t *tmp_mv;     // integer-stage MVs when the caller does not want them
     size_t tmp_mv_cap;   // in int32 elements
+    int fs_kernel;                // integer search: 0 = quad-SAD, 1 = v_sad_u8 x2, 2 = v_sad_u8 x4
     std::vector<hipEvent_t> prof; // 3 events per profiled plain search
     int prof_cap, prof_n;         // capacity and count of profiled searches
 };
@@ -410,6 +411,159 @@ void k_full_search(rh264_search_params p,
     } else {
         for (int jb = 0; jb < NJ; jb += BH)
             fs_rows<BW, BH, 0>(L, F, pw, ph, fast, px0, py0, shift, jb, NJ, R, NC, g, cy, mpy, s_lr);
+    }
+
+    uint64_t key = active ? (((uint64_t)L.best << 32) | L.bidx) : ~0ull;
+    const int seg = active ? b : 0x7fffffff;
+    key = wave_seg_min(key, seg, lane);
+    const int prev_seg = __shfl_up(seg, 1, 64);
+    if (active && (lane == 0 || prev_seg != seg))
+        atomicMin((unsigned long long *)&keys[boff], (unsigned long long)key);
+}
+
+// The same search with v_sad_u8 (4 abs-diff-adds per lane-op at full issue rate):
+// a lane owns NQ (2 or 4) candidates at ABSOLUTE padded columns base..base+NQ-1,
+// base = (left edge of the window & ~3) + NQ*g, so each candidate's reference
+// dwords are the lane's aligned row dwords realigned by a byte shift in 0..3.
+// Accumulators are u32, one per (candidate row in flight, candidate).
+template <int BW, int BH, int NQ> struct FsLane8 {
+    uint32_t src[BH][FsShape<BW, BH>::SD];
+    uint32_t acc[BH][NQ];
+    uint32_t pre[FsShape<BW, BH>::SD + 1]; // aligned dwords of the current reference row
+    uint32_t best, bidx;
+    int rxb[NQ];
+    int c0;  // column index (0..2R) of the lane's first candidate, may be out of range
+    int sh;  // byte shift of the first candidate inside its aligned dword (0 or 2 for NQ=2)
+};
+
+template <int BW, int BH, int NQ>
+__device__ __forceinline__ void fs8_load_row(FsLane8<BW, BH, NQ> &L, const uint8_t *__restrict__ F, int pw, int ph,
+                                             bool fast, int abase, int ry)
+{
+    using S = FsShape<BW, BH>;
+    if (fast) {
+        const uint32_t *rp = (const uint32_t *)(F + (size_t)ry * pw + abase);
+#pragma unroll
+        for (int i = 0; i < S::SD + 1; i++) L.pre[i] = rp[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < S::SD + 1; i++) L.pre[i] = load4_clamped(F, pw, ph, abase + 4 * i, ry);
+    }
+}
+
+template <int BW, int BH, int NQ, int MODE>
+__device__ __forceinline__ void fs8_rows(FsLane8<BW, BH, NQ> &L, const uint8_t *__restrict__ F, int pw, int ph,
+                                         bool fast, int abase, int py0, int jb, int NJ, int R, int NC, int cy,
+                                         int mpy, const int32_t *s_lr)
+{
+    using S = FsShape<BW, BH>;
+#pragma unroll
+    for (int jj = 0; jj < BH; jj++) {
+        const int j = jb + jj;
+        if (MODE == 0 && j >= NJ) break;
+        uint32_t a[S::SD + 1];
+        fs8_load_row<BW, BH, NQ>(L, F, pw, ph, fast, abase, py0 + j);
+#pragma unroll
+        for (int i = 0; i < S::SD + 1; i++) a[i] = L.pre[i];
+        uint32_t w[NQ][S::SD];
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+#pragma unroll
+            for (int i = 0; i < S::SD; i++)
+                w[q][i] = NQ == 4 ? (q == 0 ? a[i] : __builtin_amdgcn_alignbyte(a[i + 1], a[i], q))
+                                  : __builtin_amdgcn_alignbyte(a[i + 1], a[i], L.sh + q);
+#pragma unroll
+        for (int k = 0; k < BH; k++) {
+            if (MODE == 1 && k > jj) continue;
+            if (MODE == 2 && k < jj) continue;
+            const int slot = (jj - k + BH) % BH;
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+#pragma unroll
+                for (int i = 0; i < S::SD; i++)
+                    L.acc[slot][q] = __builtin_amdgcn_sad_u8(L.src[k][i], w[q][i], L.acc[slot][q]);
+        }
+        const int t = j - (BH - 1);
+        const int sd = (jj + 1) % BH;
+        if (t >= 0) {
+            const int ry_bits = mvbits(cy + 4 * (t - R) - mpy);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int c = L.c0 + q;
+                const uint32_t cost = L.acc[sd][q] + (uint32_t)s_lr[L.rxb[q] + ry_bits];
+                if ((unsigned)c < (unsigned)NC && cost < L.best) {
+                    L.best = cost;
+                    L.bidx = (uint32_t)(t * NC + c);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; q++) L.acc[sd][q] = 0;
+    }
+}
+
+template <int NQ> __host__ __device__ constexpr int fs8_groups(int NC) { return (NC + 3 + NQ - 1) / NQ; }
+
+template <int BW, int BH, int NQ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
+void k_full_search8(rh264_search_params p, const uint8_t *__restrict__ cur, size_t cur_stride,
+                    const uint8_t *__restrict__ planes, size_t planes_stride, const int32_t *__restrict__ centres,
+                    const int32_t *__restrict__ mvps, uint64_t *__restrict__ keys)
+{
+    using S = FsShape<BW, BH>;
+    __shared__ int32_t s_lr[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
+    __syncthreads();
+
+    const int R = p.range, NC = 2 * R + 1, NG = fs8_groups<NQ>(NC);
+    const int nbx = p.cw / BW, nby = p.ch / BH, nblk = nbx * nby;
+    const int pw = p.cw + 2 * p.pad, ph = p.ch + 2 * p.pad;
+    const int frame = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const long gi = (long)blockIdx.x * 256 + threadIdx.x;
+    const int b = (int)(gi / NG);
+    const int g = (int)(gi - (long)b * NG);
+    const bool active = b < nblk;
+    const int bsafe = active ? b : nblk - 1;
+
+    const uint8_t *F = planes + (size_t)frame * planes_stride;
+    const size_t boff = (size_t)frame * nblk + bsafe;
+    const int lx = (bsafe % nbx) * BW, ly = (bsafe / nbx) * BH;
+    const int cx = centres ? centres[2 * boff] : 0, cy = centres ? centres[2 * boff + 1] : 0;
+    const int mpx = mvps ? mvps[2 * boff] : 0, mpy = mvps ? mvps[2 * boff + 1] : 0;
+
+    FsLane8<BW, BH, NQ> L;
+    {
+        const uint8_t *sp = cur + (size_t)frame * cur_stride + (size_t)ly * p.cw + lx;
+#pragma unroll
+        for (int k = 0; k < BH; k++)
+#pragma unroll
+            for (int i = 0; i < S::SD; i++) L.src[k][i] = *(const uint32_t *)(sp + (size_t)k * p.cw + 4 * i);
+    }
+    const int wx_lo = lx + (cx >> 2) + p.pad - R, wx_hi = wx_lo + 2 * R + BW - 1;
+    const int py0 = ly + (cy >> 2) + p.pad - R;
+    const int base = (wx_lo & ~3) + NQ * g; // absolute padded column of the lane's first candidate
+    const int abase = base & ~3;
+    L.sh = base & 3;
+    L.c0 = base - wx_lo;
+    const bool fast = wx_lo >= 0 && wx_hi <= pw - 1 && py0 >= 0 && py0 + 2 * R + BH - 1 <= ph - 1;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) L.rxb[q] = mvbits(cx + 4 * (L.c0 + q - R) - mpx);
+#pragma unroll
+    for (int sl = 0; sl < BH; sl++)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) L.acc[sl][q] = 0;
+    L.best = 0xffffffffu;
+    L.bidx = 0xffffffffu;
+    const int NJ = 2 * R + BH;
+    if ((2 * R) % BH == 0) {
+        fs8_rows<BW, BH, NQ, 1>(L, F, pw, ph, fast, abase, py0, 0, NJ, R, NC, cy, mpy, s_lr);
+        for (int jb = BH; jb < NJ - BH; jb += BH)
+            fs8_rows<BW, BH, NQ, 0>(L, F, pw, ph, fast, abase, py0, jb, NJ, R, NC, cy, mpy, s_lr);
+        fs8_rows<BW, BH, NQ, 2>(L, F, pw, ph, fast, abase, py0, NJ - BH, NJ, R, NC, cy, mpy, s_lr);
+    } else {
+        for (int jb = 0; jb < NJ; jb += BH)
+            fs8_rows<BW, BH, NQ, 0>(L, F, pw, ph, fast, abase, py0, jb, NJ, R, NC, cy, mpy, s_lr);
     }
 
     uint64_t key = active ? (((uint64_t)L.best << 32) | L.bidx) : ~0ull;
@@ -773,6 +927,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->tmp_mv = nullptr;
     c->tmp_mv_cap = 0;
     c->prof_cap = c->prof_n = 0;
+    c->fs_kernel = 0;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -924,12 +1079,24 @@ static int launch_full(rh264_ctx *ctx, const rh264_search_params *p, int nframes
                        size_t cur_stride, const uint8_t *planes, size_t planes_stride,
                        const int32_t *centres, const int32_t *mvps, uint64_t *keys)
 {
-    const int NC = 2 * p->range + 1, NG = (NC + 3) / 4;
+    const int NC = 2 * p->range + 1;
     const long nblk = (long)(p->cw / BW) * (p->ch / BH);
-    const long lanes = nblk * NG;
-    dim3 grid((unsigned)((lanes + 255) / 256), nframes);
-    hipLaunchKernelGGL((k_full_search<BW, BH>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes,
-                       planes_stride, centres, mvps, keys);
+    if (ctx->fs_kernel == 0) {
+        const long lanes = nblk * ((NC + 3) / 4);
+        dim3 grid((unsigned)((lanes + 255) / 256), nframes);
+        hipLaunchKernelGGL((k_full_search<BW, BH>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes,
+                           planes_stride, centres, mvps, keys);
+    } else if (ctx->fs_kernel == 2) {
+        const long lanes = nblk * fs8_groups<4>(NC);
+        dim3 grid((unsigned)((lanes + 255) / 256), nframes);
+        hipLaunchKernelGGL((k_full_search8<BW, BH, 4>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
+                           planes, planes_stride, centres, mvps, keys);
+    } else {
+        const long lanes = nblk * fs8_groups<2>(NC);
+        dim3 grid((unsigned)((lanes + 255) / 256), nframes);
+        hipLaunchKernelGGL((k_full_search8<BW, BH, 2>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
+                           planes, planes_stride, centres, mvps, keys);
+    }
     LAUNCH_CHECK("k_full_search");
     return 0;
 }
@@ -1063,6 +1230,17 @@ int rh264_set_profiling(rh264_ctx *ctx, int capacity)
 }
 
 int rh264_profile_count(rh264_ctx *ctx) { return ctx->prof_n; }
+
+int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
+{
+    if (!knob) return fail("rh264_tune: null knob");
+    if (!strcmp(knob, "fs_kernel")) {
+        if (value < 0 || value > 2) return fail("rh264_tune: fs_kernel %d", value);
+        ctx->fs_kernel = value;
+        return 0;
+    }
+    return fail("rh264_tune: unknown knob '%s'", knob);
+}
 
 int rh264_profile_get(rh264_ctx *ctx, int i, float *search_ms, float *refine_ms)
 {

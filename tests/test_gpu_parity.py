@@ -186,6 +186,23 @@ def test_subpel_refine_from_given_starts(dev):
         assert tuple(got["mv"][b]) == mv and got["cost"][b] == c and got["evals"][b] == ev, b
 
 
+@pytest.mark.parametrize("fs_kernel", [0, 1, 2])
+def test_full_search_kernel_variants(dev, fs_kernel):
+    """Every integer-search kernel variant gives the oracle's result (CIF, +-16, and a
+    window that leaves the padded planes)."""
+    dev.tune("fs_kernel", fs_kernel)
+    try:
+        frames = synth.cif_clip(3, seed=40 + fs_kernel)
+        _search_case(dev, frames, 352, 288, 32, 16, 16, 16, 0, LAM26)
+        nb = 22 * 18 * 2
+        centres = (4 * np.random.default_rng(fs_kernel).integers(-40, 41, (nb, 2))).astype(np.int32)
+        _search_case(dev, frames, 352, 288, 32, 16, 16, 16, 2, LAM26, centres)
+        for bw, bh in [(8, 8), (4, 4), (16, 8)]:
+            _search_case(dev, frames[:2], 352, 288, 32, bw, bh, 9, 0, LAM26)
+    finally:
+        dev.tune("fs_kernel", 0)
+
+
 def test_plain_search_1080p_frame_pair(dev):
     """Config C2 geometry (1920x1088, +-32, 4x4-Hadamard qpel refine) on one frame pair."""
     frames = synth.hd_clip(2, seed=3)
