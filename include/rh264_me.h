@@ -167,6 +167,30 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
 int rh264_predict_blocks(rh264_ctx *ctx, const uint8_t *planes, int cw, int ch, int pad, int bw,
                          int bh, int n, const int32_t *xy, const int32_t *mv, uint8_t *out);
 
+/* ---- mb-tree lookahead (crates/rusty_h264-encoder/src/mbtree.rs, HalfRes,
+ *      the default LookaheadMode, config.rs:568) ------------------------------
+ * rh264_mbtree_downsample: downsample2x (mbtree.rs:61-74) of nframes cw x ch
+ *   luma frames (frame i at frames + i*frame_stride) into half, frame i at
+ *   half + i*(cw/2)*(ch/2).
+ * rh264_mbtree_costs: frame_costs (mbtree.rs:208-263) for a GOP of nframes
+ *   half-res frames: frame 0 is the IDR (inter = intra, mv = 0); frame f > 0 is
+ *   searched against frame f-1 with inter_cost's seeded cascade diamond
+ *   (mbtree.rs:154-197, steps 8,4,2,1 full pel, seed = left neighbour's vector or,
+ *   in column 0, the one above, halved), 8x8 blocks; intra_cost (78-93) is the AC
+ *   SATD.  Outputs per (frame, MB): intra, inter = min(inter, intra), mv
+ *   (full-res quarter pel, x2 as in mbtree.rs:249).  mb_w x mb_h macroblocks.
+ * rh264_mbtree_propagate_i32: build-defined integer form of the backward
+ *   propagation (mbtree.rs:268-297, 440-455): prop[n-1] = 0; for f = n-1..1,
+ *   amount = ((intra + prop[f]) * (intra - inter)) / intra (int64), each of the
+ *   up to four overlapped MBs of frame f-1 receives (amount * wx * wy) >> 8;
+ *   sums are exact in int64 and saturated to INT32_MAX when stored in prop. */
+int rh264_mbtree_downsample(rh264_ctx *ctx, const uint8_t *frames, size_t frame_stride, int nframes,
+                            int cw, int ch, uint8_t *half);
+int rh264_mbtree_costs(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_w, int mb_h,
+                       int32_t *intra, int32_t *inter, int32_t *mv);
+int rh264_mbtree_propagate_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, const int32_t *intra,
+                               const int32_t *inter, const int32_t *mv, int32_t *prop);
+
 #ifdef __cplusplus
 }
 #endif

@@ -278,3 +278,41 @@ def predict_blocks(dev, planes_buf, cw, ch, pad, bw, bh, xy, mv):
     finally:
         for b in (dxy, dmv, out):
             b.free()
+
+
+class MbTree:
+    """mb-tree lookahead costs and propagation over device-resident GOP frames
+    (crates/rusty_h264-encoder/src/mbtree.rs, HalfRes mode)."""
+
+    def __init__(self, dev, cw, ch, nframes):
+        self.dev, self.cw, self.ch, self.n = dev, cw, ch, nframes
+        self.mb_w, self.mb_h = cw // 16, ch // 16
+        nm = self.mb_w * self.mb_h
+        self.half = dev.alloc(nframes * (cw // 2) * (ch // 2))
+        self.intra = dev.alloc(4 * nm * nframes)
+        self.inter = dev.alloc(4 * nm * nframes)
+        self.mv = dev.alloc(8 * nm * nframes)
+        self.prop = dev.alloc(4 * nm * nframes)
+
+    def run(self, frames_buf, frame_stride=None, propagate=True):
+        """frames_buf must hold this object's nframes frames of cw x ch luma."""
+        d = self.dev
+        if isinstance(frames_buf, DevBuf) and frames_buf.nbytes < (self.n - 1) * (frame_stride or self.cw * self.ch) \
+                + self.cw * self.ch:
+            raise ValueError("frames_buf holds fewer frames than this MbTree was built for")
+        _check(d.lib.rh264_mbtree_downsample(d.ctx, int(frames_buf), frame_stride or self.cw * self.ch, self.n,
+                                             self.cw, self.ch, self.half.ptr), "rh264_mbtree_downsample")
+        _check(d.lib.rh264_mbtree_costs(d.ctx, self.half.ptr, self.n, self.mb_w, self.mb_h, self.intra.ptr,
+                                        self.inter.ptr, self.mv.ptr), "rh264_mbtree_costs")
+        if propagate:
+            _check(d.lib.rh264_mbtree_propagate_i32(d.ctx, self.n, self.mb_w, self.mb_h, self.intra.ptr,
+                                                    self.inter.ptr, self.mv.ptr, self.prop.ptr),
+                   "rh264_mbtree_propagate_i32")
+
+    def results(self):
+        d, n, nm = self.dev, self.n, self.mb_w * self.mb_h
+        return {"half": d.download(self.half, (n, self.ch // 2, self.cw // 2), np.uint8),
+                "intra": d.download(self.intra, (n, nm), np.int32),
+                "inter": d.download(self.inter, (n, nm), np.int32),
+                "mv": d.download(self.mv, (n, nm, 2), np.int32),
+                "prop": d.download(self.prop, (n, nm), np.int32)}

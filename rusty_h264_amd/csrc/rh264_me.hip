@@ -74,6 +74,7 @@ struct rh264_ctx {
 static int ensure_keys(rh264_ctx *ctx, size_t n)
 {
     if (ctx->keys_cap >= n) return 0;
+    HIP_TRY(hipStreamSynchronize(ctx->stream)); // earlier launches may still use the old buffer
     if (ctx->keys) HIP_TRY(hipFree(ctx->keys));
     ctx->keys = nullptr;
     ctx->keys_cap = 0;
@@ -85,6 +86,7 @@ static int ensure_keys(rh264_ctx *ctx, size_t n)
 static int ensure_tmp_mv(rh264_ctx *ctx, size_t n)
 {
     if (ctx->tmp_mv_cap >= n) return 0;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->tmp_mv) HIP_TRY(hipFree(ctx->tmp_mv));
     ctx->tmp_mv = nullptr;
     ctx->tmp_mv_cap = 0;
@@ -905,6 +907,253 @@ __global__ void k_predict(const uint8_t *__restrict__ planes, int pw, int ph, in
 }
 
 // ---------------------------------------------------------------------------
+// mb-tree lookahead (crates/rusty_h264-encoder/src/mbtree.rs), HalfRes mode
+// ---------------------------------------------------------------------------
+
+// downsample2x, mbtree.rs:61-74: (a+b+c+d+2)/4, one thread per 4 output pixels.
+__global__ void k_downsample2x(const uint8_t *__restrict__ frames, size_t stride, int cw, int ch,
+                               uint8_t *__restrict__ half)
+{
+    const int hw = cw / 2, hh = ch / 2;
+    const long i4 = (long)blockIdx.x * blockDim.x + threadIdx.x; // index of a 4-pixel output group
+    const int f = blockIdx.y;
+    if (i4 * 4 >= (long)hw * hh) return;
+    const int j = (int)(i4 * 4 / hw), i = (int)(i4 * 4 - (long)j * hw);
+    const uint8_t *y = frames + (size_t)f * stride;
+    const uint8_t *r0 = y + (size_t)(2 * j) * cw + 2 * i, *r1 = r0 + cw;
+    const uint2 a = *(const uint2 *)r0, b = *(const uint2 *)r1; // 8 source bytes per row
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t aw = k < 2 ? a.x : a.y, bw = k < 2 ? b.x : b.y;
+        const int sh = 16 * (k & 1);
+        const uint32_t sum = ((aw >> sh) & 255) + ((aw >> (sh + 8)) & 255) + ((bw >> sh) & 255) +
+                             ((bw >> (sh + 8)) & 255);
+        o |= ((sum + 2) / 4) << (8 * k);
+    }
+    *(uint32_t *)(half + (size_t)f * hw * hh + (size_t)j * hw + i) = o;
+}
+
+// Per-lane rows of one 8x8 block: lane li (0..15) = 4x4 sub-block li>>2, row li&3.
+__device__ __forceinline__ uint32_t mbt_row_clamped(const uint8_t *plane, int w, int h, int x, int y)
+{
+    const uint8_t *row = plane + (size_t)clampi(y, 0, h - 1) * w;
+    uint32_t v = 0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) v |= (uint32_t)row[clampi(x + m, 0, w - 1)] << (8 * m);
+    return v;
+}
+
+__device__ __forceinline__ void diff4(uint32_t s, uint32_t p, int &d0, int &d1, int &d2, int &d3)
+{
+    d0 = (int)(s & 255) - (int)(p & 255);
+    d1 = (int)((s >> 8) & 255) - (int)((p >> 8) & 255);
+    d2 = (int)((s >> 16) & 255) - (int)((p >> 16) & 255);
+    d3 = (int)(s >> 24) - (int)(p >> 24);
+}
+
+// intra_cost, mbtree.rs:78-93, 8x8 blocks of the half-res plane: sum over the four
+// 4x4 Hadamards of |coeff| without the DC coefficient, clamped to >= 1.  Frame 0
+// (the IDR) also gets inter = intra and mv = 0 (mbtree.rs:256).
+__global__ void k_mbtree_intra(const uint8_t *__restrict__ half, int mb_w, int mb_h, int32_t *__restrict__ intra,
+                               int32_t *__restrict__ inter, int32_t *__restrict__ mv)
+{
+    const int f = blockIdx.y;
+    const int nm = mb_w * mb_h;
+    const int m0 = blockIdx.x * 16 + (threadIdx.x >> 4);
+    const bool active = m0 < nm;
+    const int m = active ? m0 : nm - 1;
+    const int li = threadIdx.x & 15, bi = li >> 2, r = li & 3;
+    const int cwh = mb_w * 8;
+    const uint8_t *pl = half + (size_t)f * cwh * (mb_h * 8);
+    const int bx0 = (m % mb_w) * 8 + 4 * (bi & 1), by0 = (m / mb_w) * 8 + 4 * (bi >> 1) + r;
+    const uint32_t s = *(const uint32_t *)(pl + (size_t)by0 * cwh + bx0);
+    int d0, d1, d2, d3;
+    diff4(s, 0u, d0, d1, d2, d3);
+    // row transform, column transform across the quad (as hadamard_row_quad) but
+    // keeping the coefficient that is the 4x4 block's DC (lane r = 0, column 0)
+    int e0 = d0 + d1 + d2 + d3, e1 = d0 + d1 - d2 - d3, e2 = d0 - d1 - d2 + d3, e3 = d0 - d1 + d2 - d3;
+    const bool hi2 = (r & 2) != 0, odd = (r & 1) != 0;
+    int y0 = __builtin_amdgcn_mov_dpp(e0, 0x4E, 0xF, 0xF, true), y1 = __builtin_amdgcn_mov_dpp(e1, 0x4E, 0xF, 0xF, true);
+    int y2 = __builtin_amdgcn_mov_dpp(e2, 0x4E, 0xF, 0xF, true), y3 = __builtin_amdgcn_mov_dpp(e3, 0x4E, 0xF, 0xF, true);
+    int s0 = hi2 ? y0 - e0 : e0 + y0, s1 = hi2 ? y1 - e1 : e1 + y1, s2 = hi2 ? y2 - e2 : e2 + y2,
+        s3 = hi2 ? y3 - e3 : e3 + y3;
+    int z0 = __builtin_amdgcn_mov_dpp(s0, 0xB1, 0xF, 0xF, true), z1 = __builtin_amdgcn_mov_dpp(s1, 0xB1, 0xF, 0xF, true);
+    int z2 = __builtin_amdgcn_mov_dpp(s2, 0xB1, 0xF, 0xF, true), z3 = __builtin_amdgcn_mov_dpp(s3, 0xB1, 0xF, 0xF, true);
+    int c0 = odd ? z0 - s0 : s0 + z0, c1 = odd ? z1 - s1 : s1 + z1, c2 = odd ? z2 - s2 : s2 + z2,
+        c3 = odd ? z3 - s3 : s3 + z3;
+    int v = (r == 0 ? 0 : abs(c0)) + abs(c1) + abs(c2) + abs(c3); // lane r=0, c0 = sum of all 16 = DC
+    v = group_sum<16>(v);
+    if (active && li == 0) {
+        const int ic = max(v, 1);
+        const size_t g = (size_t)f * nm + m;
+        intra[g] = ic;
+        if (f == 0) {
+            inter[g] = ic;
+            mv[2 * g] = 0;
+            mv[2 * g + 1] = 0;
+        }
+    }
+}
+
+// Four candidate SATDs at once (one per 16-lane slot) for the 8x8 block at
+// (bx0, by0); candidate of slot s = (mvx[s], mvy[s]), full-pel, reads clamped to
+// the plane (mc_luma's full-pel copy, inter.rs:1214-1243).  Returns the four
+// totals through scalars.
+__device__ __forceinline__ void mbt_eval4(const uint8_t *cur, const uint8_t *ref, int w, int h, int bx0, int by0,
+                                          uint32_t srow, int li, int slot, int mvx, int mvy, int out[4])
+{
+    const int bi = li >> 2, r = li & 3;
+    const int x = bx0 + (mvx >> 2) + 4 * (bi & 1), y = by0 + (mvy >> 2) + 4 * (bi >> 1) + r;
+    const uint32_t p = mbt_row_clamped(ref, w, h, x, y);
+    int d0, d1, d2, d3;
+    diff4(srow, p, d0, d1, d2, d3);
+    int v = group_sum<16>(hadamard_row_quad(d0, d1, d2, d3, r));
+    out[0] = __builtin_amdgcn_readlane(v, 0);
+    out[1] = __builtin_amdgcn_readlane(v, 16);
+    out[2] = __builtin_amdgcn_readlane(v, 32);
+    out[3] = __builtin_amdgcn_readlane(v, 48);
+}
+
+// inter_cost (mbtree.rs:154-197) for one block with one wave; the cascade diamond
+// evaluates the remaining candidates of a ring pass speculatively in parallel and
+// keeps exactly the sequential decisions (first improving candidate in ring order
+// moves the centre; later candidates of that round are discarded).
+__device__ __forceinline__ void mbt_inter(const uint8_t *cur, const uint8_t *ref, int w, int h, int bx0, int by0,
+                                          int seedx, int seedy, int max_step, int &cost, int &bmx, int &bmy)
+{
+    const int lane = threadIdx.x & 63, slot = lane >> 4, li = lane & 15;
+    const int bi = li >> 2, r = li & 3;
+    const uint32_t srow = *(const uint32_t *)(cur + (size_t)(by0 + 4 * (bi >> 1) + r) * w + bx0 + 4 * (bi & 1));
+    int c4[4];
+    // slot 0: (0,0); slot 1: the seed
+    mbt_eval4(cur, ref, w, h, bx0, by0, srow, li, slot, slot == 1 ? seedx : 0, slot == 1 ? seedy : 0, c4);
+    int best = c4[0];
+    bmx = 0;
+    bmy = 0;
+    if (best != 0) {
+        if ((seedx != 0 || seedy != 0) && c4[1] < best) {
+            best = c4[1];
+            bmx = seedx;
+            bmy = seedy;
+        }
+        const int rx[4] = {1, -1, 0, 0}, ry[4] = {0, 0, 1, -1};
+        for (int step = max_step; step >= 1; step >>= 1) {
+            for (;;) {
+                bool moved = false;
+                int i = 0;
+                while (i < 4) {
+                    const int k = min(i + slot, 3); // slots past the ring end repeat the last candidate
+                    mbt_eval4(cur, ref, w, h, bx0, by0, srow, li, slot, bmx + rx[k] * step * 4, bmy + ry[k] * step * 4,
+                              c4);
+                    int acc = -1;
+                    for (int sidx = 0; sidx < 4 - i; sidx++)
+                        if (c4[sidx] < best) { acc = sidx; break; }
+                    if (acc < 0) break;
+                    const int kk = i + acc;
+                    best = c4[acc];
+                    bmx += rx[kk] * step * 4;
+                    bmy += ry[kk] * step * 4;
+                    moved = true;
+                    i = kk + 1;
+                }
+                if (!moved) break;
+            }
+        }
+    }
+    cost = best;
+}
+
+// Column 0 of each inter frame: one wave walks rows 0..mb_h-1 (seed = the block
+// above's vector, mbtree.rs:224-231).
+__global__ __launch_bounds__(64) void k_mbtree_col0(const uint8_t *__restrict__ half, int mb_w, int mb_h,
+                                                    const int32_t *__restrict__ intra, int32_t *__restrict__ inter,
+                                                    int32_t *__restrict__ mv)
+{
+    const int f = blockIdx.x + 1;
+    const int cwh = mb_w * 8, chh = mb_h * 8, nm = mb_w * mb_h;
+    const uint8_t *cur = half + (size_t)f * cwh * chh, *ref = cur - (size_t)cwh * chh;
+    int sfx = 0, sfy = 0;
+    for (int y = 0; y < mb_h; y++) {
+        const size_t g = (size_t)f * nm + (size_t)y * mb_w;
+        int c, mx, my;
+        mbt_inter(cur, ref, cwh, chh, 0, y * 8, sfx / 2, sfy / 2, 8, c, mx, my);
+        const int ic = intra[g];
+        sfx = mx * 2;
+        sfy = my * 2;
+        if ((threadIdx.x & 63) == 0) {
+            inter[g] = min(c, ic);
+            mv[2 * g] = sfx;
+            mv[2 * g + 1] = sfy;
+        }
+    }
+}
+
+// Rows of each inter frame: one wave per (frame, row) walks x = 1..mb_w-1 from the
+// column-0 result (seed = the left neighbour's vector).
+__global__ __launch_bounds__(64) void k_mbtree_rows(const uint8_t *__restrict__ half, int mb_w, int mb_h,
+                                                    const int32_t *__restrict__ intra, int32_t *__restrict__ inter,
+                                                    int32_t *__restrict__ mv)
+{
+    const int f = blockIdx.y + 1, y = blockIdx.x;
+    const int cwh = mb_w * 8, chh = mb_h * 8, nm = mb_w * mb_h;
+    const uint8_t *cur = half + (size_t)f * cwh * chh, *ref = cur - (size_t)cwh * chh;
+    const size_t g0 = (size_t)f * nm + (size_t)y * mb_w;
+    int sfx = mv[2 * g0], sfy = mv[2 * g0 + 1];
+    for (int x = 1; x < mb_w; x++) {
+        const size_t g = g0 + x;
+        int c, mx, my;
+        mbt_inter(cur, ref, cwh, chh, x * 8, y * 8, sfx / 2, sfy / 2, 8, c, mx, my);
+        const int ic = intra[g];
+        sfx = mx * 2;
+        sfy = my * 2;
+        if ((threadIdx.x & 63) == 0) {
+            inter[g] = min(c, ic);
+            mv[2 * g] = sfx;
+            mv[2 * g + 1] = sfy;
+        }
+    }
+}
+
+// Build-defined int32 propagation step for frame f (mirrors propagate_to,
+// mbtree.rs:268-297, in integers): prop[f] = saturate(acc[f]); amount =
+// ((intra + prop) * (intra - inter)) / intra; shares (amount * wx * wy) >> 8 are
+// added into acc[f-1] with 64-bit integer atomics (order-independent).
+__global__ void k_mbtree_prop_i32(int f, int mb_w, int mb_h, const int32_t *__restrict__ intra,
+                                  const int32_t *__restrict__ inter, const int32_t *__restrict__ mv,
+                                  unsigned long long *__restrict__ acc, int32_t *__restrict__ prop)
+{
+    const int nm = mb_w * mb_h;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    const size_t g = (size_t)f * nm + m;
+    const long long a = (long long)acc[g];
+    const int32_t pf = a > 0x7fffffffLL ? 0x7fffffff : (int32_t)a;
+    prop[g] = pf;
+    if (f == 0) return;
+    const int ic = intra[g];
+    const long long amount = ((long long)ic + pf) * (long long)(ic - inter[g]) / ic;
+    if (amount <= 0) return;
+    const int x = m % mb_w, y = m / mb_w;
+    const int rx = clampi(x * 16 + (mv[2 * g] >> 2), 0, (mb_w - 1) * 16);
+    const int ry = clampi(y * 16 + (mv[2 * g + 1] >> 2), 0, (mb_h - 1) * 16);
+    const int cx0 = rx / 16, cy0 = ry / 16, fx = rx % 16, fy = ry % 16;
+    const int wys[2] = {16 - fy, fy}, wxs[2] = {16 - fx, fx};
+    unsigned long long *prev = acc + (size_t)(f - 1) * nm;
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+        if (wys[dy] <= 0) continue;
+        const int cy = min(cy0 + dy, mb_h - 1);
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+            if (wxs[dx] <= 0) continue;
+            const int cx = min(cx0 + dx, mb_w - 1);
+            atomicAdd(&prev[cy * mb_w + cx], (unsigned long long)((amount * wxs[dx] * wys[dy]) >> 8));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 
@@ -1276,6 +1525,60 @@ int rh264_predict_blocks(rh264_ctx *ctx, const uint8_t *planes, int cw, int ch, 
     hipLaunchKernelGGL(k_predict, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, planes,
                        cw + 2 * pad, ch + 2 * pad, pad, bw, bh, n, xy, mv, out);
     LAUNCH_CHECK("k_predict");
+    return 0;
+}
+
+
+int rh264_mbtree_downsample(rh264_ctx *ctx, const uint8_t *frames, size_t frame_stride, int nframes, int cw, int ch,
+                            uint8_t *half)
+{
+    if (cw <= 0 || ch <= 0 || cw % 16 || ch % 16) return RH264_DECLINED;
+    if (frame_stride % 8) return fail("mbtree_downsample: frame stride must be a multiple of 8");
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    const long groups = (long)(cw / 2) * (ch / 2) / 4;
+    hipLaunchKernelGGL(k_downsample2x, dim3((unsigned)((groups + 255) / 256), nframes), dim3(256), 0, ctx->stream,
+                       frames, frame_stride, cw, ch, half);
+    LAUNCH_CHECK("k_downsample2x");
+    return 0;
+}
+
+int rh264_mbtree_costs(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_w, int mb_h, int32_t *intra,
+                       int32_t *inter, int32_t *mv)
+{
+    if (mb_w <= 0 || mb_h <= 0) return RH264_DECLINED;
+    if (nframes <= 0) return 0;
+    if (nframes > 65535 || mb_h > 65535) return fail("mbtree_costs: nframes/mb_h too large");
+    const int nm = mb_w * mb_h;
+    hipLaunchKernelGGL(k_mbtree_intra, dim3((unsigned)((nm + 15) / 16), nframes), dim3(256), 0, ctx->stream, half,
+                       mb_w, mb_h, intra, inter, mv);
+    LAUNCH_CHECK("k_mbtree_intra");
+    if (nframes == 1) return 0;
+    hipLaunchKernelGGL(k_mbtree_col0, dim3(nframes - 1), dim3(64), 0, ctx->stream, half, mb_w, mb_h, intra, inter, mv);
+    LAUNCH_CHECK("k_mbtree_col0");
+    if (mb_w > 1) {
+        hipLaunchKernelGGL(k_mbtree_rows, dim3(mb_h, nframes - 1), dim3(64), 0, ctx->stream, half, mb_w, mb_h, intra,
+                           inter, mv);
+        LAUNCH_CHECK("k_mbtree_rows");
+    }
+    return 0;
+}
+
+int rh264_mbtree_propagate_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, const int32_t *intra,
+                               const int32_t *inter, const int32_t *mv, int32_t *prop)
+{
+    if (mb_w <= 0 || mb_h <= 0) return RH264_DECLINED;
+    if (nframes <= 0) return 0;
+    const size_t nm = (size_t)mb_w * mb_h, n = nm * nframes;
+    int rc = ensure_keys(ctx, n);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->keys, n, 0ull);
+    LAUNCH_CHECK("k_fill_u64");
+    for (int f = nframes - 1; f >= 0; f--) {
+        hipLaunchKernelGGL(k_mbtree_prop_i32, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, ctx->stream, f, mb_w,
+                           mb_h, intra, inter, mv, (unsigned long long *)ctx->keys, prop);
+        LAUNCH_CHECK("k_mbtree_prop_i32");
+    }
     return 0;
 }
 

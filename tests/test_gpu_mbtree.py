@@ -1,0 +1,68 @@
+"""GPU parity of the mb-tree lookahead (downsample, intra/inter costs with the
+seeded cascade diamond, int32 propagation) against the oracle."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from rusty_h264_amd import accel, synth
+
+pytestmark = pytest.mark.gpu
+O = Oracle()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = accel.Device(0)
+    yield d
+    d.close()
+
+
+def _oracle_gop(frames, cw, ch):
+    mb_w, mb_h = cw // 16, ch // 16
+    halves = [O.downsample2x(f, cw, ch) for f in frames]
+    intra, inter, mv = [], [], []
+    for i, h in enumerate(halves):
+        a, b, m = O.frame_costs_half(h, halves[i - 1] if i else None, mb_w, mb_h)
+        intra.append(a); inter.append(b); mv.append(m)
+    intra, inter, mv = np.stack(intra), np.stack(inter), np.stack(mv)
+    return np.stack(halves), intra, inter, mv, O.propagate_i32(intra, inter, mv, mb_w, mb_h)
+
+
+@pytest.mark.parametrize("cw,ch,n,kind", [(64, 48, 4, "cif"), (352, 288, 6, "cif"), (176, 144, 5, "noise"),
+                                          (16, 16, 3, "cif"), (640, 368, 4, "hd")])
+def test_mbtree_gop_bit_exact(dev, cw, ch, n, kind):
+    if kind == "cif":
+        frames = synth.cif_clip(n, seed=cw + n, w=cw, h=ch)
+    elif kind == "hd":
+        frames = synth.hd_clip(n, seed=5, w=cw, h=ch)
+    else:
+        frames = np.random.default_rng(cw).integers(0, 256, (n, ch, cw), dtype=np.uint8)
+    half, intra, inter, mv, prop = _oracle_gop(frames, cw, ch)
+    fb = dev.upload(frames)
+    mt = accel.MbTree(dev, cw, ch, n)
+    mt.run(fb)
+    got = mt.results()
+    assert np.array_equal(got["half"], half)
+    assert np.array_equal(got["intra"], intra)
+    for f in range(n):
+        bad = np.nonzero((got["inter"][f] != inter[f]) | (got["mv"][f] != mv[f]).any(1))[0]
+        assert bad.size == 0, f"frame {f}: {bad.size} MBs differ, first {bad[:5]}"
+    assert np.array_equal(got["prop"], prop)
+    fb.free()
+
+
+def test_mbtree_static_and_flat(dev):
+    """Static content: inter 0 everywhere after the IDR; flat frames: intra 1."""
+    f = synth.cif_clip(1, seed=3, w=128, h=64)[0]
+    frames = np.stack([f, f, f])
+    fb = dev.upload(frames)
+    mt = accel.MbTree(dev, 128, 64, 3)
+    mt.run(fb)
+    got = mt.results()
+    assert (got["inter"][1:] == 0).all() and (got["mv"][1:] == 0).all()
+    flat = np.full((2, 64, 128), 77, np.uint8)
+    fb2 = dev.upload(flat)
+    mt2 = accel.MbTree(dev, 128, 64, 2)
+    mt2.run(fb2)
+    r2 = mt2.results()
+    assert (r2["intra"] == 1).all() and (r2["inter"] == 1).all()
