@@ -167,6 +167,44 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
 int rh264_predict_blocks(rh264_ctx *ctx, const uint8_t *planes, int cw, int ch, int pad, int bw,
                          int bh, int n, const int32_t *xy, const int32_t *mv, uint8_t *out);
 
+/* ---- batched multi-stream search ------------------------------------------
+ * Many streams of uneven size, 1..4 references each, several partition shapes,
+ * searched together: for every shape, one integer-search launch over all
+ * (stream, reference) pairs (work-queue persistent kernel or one tile per block,
+ * chosen by rh264_batch_autotune), one refinement launch, and one launch that
+ * picks each block's reference as best_part does (cost + (int64)(lambda_ref *
+ * ref_bits(r, nrefs)), first minimum in reference order; mb16.rs:4921-4959,
+ * 5967-5980).  Each (stream, reference, shape) is the plain search above (zero
+ * centres).  A batch is prepared once (tables uploaded, scratch allocated), then
+ * run any number of times, directly or as a captured HIP graph.
+ *
+ * Per stream: planes = nrefs plane sets back to back (reference 0 first); mvps =
+ * per-shape block predictors in shape order (rh264_batch_mvp_len int32 pairs
+ * total) or NULL; out = per-shape records {ref, mvx, mvy, cost} int32, shapes in
+ * order (rh264_batch_out_len int32 words). */
+typedef struct {
+    const uint8_t *cur;
+    const uint8_t *planes;
+    const int32_t *mvps;
+    int32_t cw, ch, nrefs;
+    int32_t *out;
+} rh264_stream;
+typedef struct rh264_batch rh264_batch;
+size_t rh264_batch_out_len(int cw, int ch, const int32_t *shapes, int nshapes);
+size_t rh264_batch_mvp_len(int cw, int ch, const int32_t *shapes, int nshapes);
+int rh264_batch_create(rh264_ctx *ctx, const rh264_stream *streams, int nstreams, const int32_t *shapes,
+                       int nshapes, int pad, int range, int subpel_pat, const int32_t *lr_fp,
+                       const int32_t *lr, double lambda_ref, rh264_batch **out);
+int rh264_batch_run(rh264_ctx *ctx, rh264_batch *b);
+/* Times every kernel variant x scheduling choice of each shape's integer search
+ * (reps runs each) and keeps the fastest; results are identical for all. */
+int rh264_batch_autotune(rh264_ctx *ctx, rh264_batch *b, int reps);
+int rh264_batch_choice(rh264_batch *b, int shape_index, int *variant, int *persist);
+/* Capture one rh264_batch_run into a HIP graph; replay launches it. */
+int rh264_batch_capture(rh264_ctx *ctx, rh264_batch *b);
+int rh264_batch_replay(rh264_ctx *ctx, rh264_batch *b);
+int rh264_batch_destroy(rh264_batch *b);
+
 /* ---- mb-tree lookahead (crates/rusty_h264-encoder/src/mbtree.rs, HalfRes,
  *      the default LookaheadMode, config.rs:568) ------------------------------
  * rh264_mbtree_downsample: downsample2x (mbtree.rs:61-74) of nframes cw x ch

@@ -28,6 +28,12 @@ class SearchParams(C.Structure):
     ]
 
 
+class Stream(C.Structure):
+    """rh264_stream (include/rh264_me.h)."""
+    _fields_ = [("cur", C.c_void_p), ("planes", C.c_void_p), ("mvps", C.c_void_p), ("cw", C.c_int32),
+                ("ch", C.c_int32), ("nrefs", C.c_int32), ("out", C.c_void_p)]
+
+
 # name -> (restype, argtypes); mirrors include/rh264_me.h one to one
 PROTOTYPES = {
     "rh264_init": (C.c_int, [C.c_int, C.POINTER(c_vp)]),
@@ -60,6 +66,16 @@ PROTOTYPES = {
     "rh264_mbtree_downsample": (C.c_int, [c_vp, c_vp, C.c_size_t, C.c_int, C.c_int, C.c_int, c_vp]),
     "rh264_mbtree_costs": (C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp]),
     "rh264_mbtree_propagate_i32": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp]),
+    "rh264_batch_out_len": (C.c_size_t, [C.c_int, C.c_int, c_i32p, C.c_int]),
+    "rh264_batch_mvp_len": (C.c_size_t, [C.c_int, C.c_int, c_i32p, C.c_int]),
+    "rh264_batch_create": (C.c_int, [c_vp, C.POINTER(Stream), C.c_int, c_i32p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     c_i32p, c_i32p, C.c_double, C.POINTER(c_vp)]),
+    "rh264_batch_run": (C.c_int, [c_vp, c_vp]),
+    "rh264_batch_autotune": (C.c_int, [c_vp, c_vp, C.c_int]),
+    "rh264_batch_choice": (C.c_int, [c_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "rh264_batch_capture": (C.c_int, [c_vp, c_vp]),
+    "rh264_batch_replay": (C.c_int, [c_vp, c_vp]),
+    "rh264_batch_destroy": (C.c_int, [c_vp]),
     "rh264_predict_blocks": (C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        c_vp, c_vp, c_vp]),
 }

@@ -138,6 +138,12 @@ def _ptr(x):
     return int(x)
 
 
+def _need(buf, nbytes, what):
+    """Refuse a device buffer smaller than the call will read or write."""
+    if isinstance(buf, DevBuf) and buf.nbytes < nbytes:
+        raise ValueError(f"{what}: buffer of {buf.nbytes} bytes, call needs {nbytes}")
+
+
 def block_metric(dev, kind, w, h, src, ss, src_off, ref, rs, ref_off, ref_off2=None):
     """Batched accel block metrics on host arrays.
 
@@ -185,8 +191,10 @@ def planes_bytes(cw, ch, pad=HPEL_PAD_DEFAULT):
 def build_planes(dev, frames_buf, nframes, cw, ch, pad=HPEL_PAD_DEFAULT, out=None):
     """Plane sets (F,H,V,C) for `nframes` device frames of cw x ch -> DevBuf."""
     pb = planes_bytes(cw, ch, pad)
+    _need(frames_buf, nframes * cw * ch, "frames")
     if out is None:
         out = dev.alloc(pb * max(nframes, 1))
+    _need(out, pb * nframes, "plane sets")
     _check(dev.lib.rh264_build_planes(dev.ctx, int(frames_buf), cw * ch, nframes, cw, ch, pad, int(out)),
            "rh264_build_planes")
     return out
@@ -230,9 +238,17 @@ class MotionSearch:
             self._out = {"n": n, "bufs": bufs}
         return self._out["bufs"]
 
+    def _check_inputs(self, cur_buf, planes_buf, nframes, centres, mvps, cur_stride=None, planes_stride=None):
+        fb = self.p.cw * self.p.ch
+        _need(cur_buf, (nframes - 1) * (cur_stride or fb) + fb, "current frames")
+        _need(planes_buf, (nframes - 1) * (planes_stride or self.pbytes) + self.pbytes, "plane sets")
+        _need(centres, 8 * self.nblk * nframes, "centres")
+        _need(mvps, 8 * self.nblk * nframes, "mvps")
+
     def run_plain(self, cur_buf, planes_buf, nframes, centres=None, mvps=None, cur_stride=None,
                   planes_stride=None):
         """Full search + sub-pel refine (rh264_plain_search); asynchronous."""
+        self._check_inputs(cur_buf, planes_buf, nframes, centres, mvps, cur_stride, planes_stride)
         imv, icost, mv, cost, _ = self._outputs(nframes)
         _check(self.dev.lib.rh264_plain_search(
             self.dev.ctx, C.byref(self.p), nframes, int(cur_buf),
@@ -240,6 +256,7 @@ class MotionSearch:
             _ptr(centres), _ptr(mvps), imv.ptr, icost.ptr, mv.ptr, cost.ptr), "rh264_plain_search")
 
     def run_full(self, cur_buf, planes_buf, nframes, centres=None, mvps=None):
+        self._check_inputs(cur_buf, planes_buf, nframes, centres, mvps)
         imv, icost, _, _, _ = self._outputs(nframes)
         _check(self.dev.lib.rh264_full_search(
             self.dev.ctx, C.byref(self.p), nframes, int(cur_buf), self.p.cw * self.p.ch,
@@ -247,6 +264,7 @@ class MotionSearch:
             "rh264_full_search")
 
     def run_refine(self, cur_buf, planes_buf, nframes, start_mv, mvps=None):
+        self._check_inputs(cur_buf, planes_buf, nframes, start_mv, mvps)
         _, _, mv, cost, evals = self._outputs(nframes)
         _check(self.dev.lib.rh264_subpel_refine(
             self.dev.ctx, C.byref(self.p), nframes, int(cur_buf), self.p.cw * self.p.ch,
@@ -316,3 +334,92 @@ class MbTree:
                 "inter": d.download(self.inter, (n, nm), np.int32),
                 "mv": d.download(self.mv, (n, nm, 2), np.int32),
                 "prop": d.download(self.prop, (n, nm), np.int32)}
+
+
+def ref_bits(r, nrefs):
+    """ref_idx rate for the motion-search cost (mb16.rs:5967-5980)."""
+    if nrefs <= 1:
+        return 0
+    if nrefs == 2:
+        return 1
+    n, length = r + 1, 1
+    while n > 1:
+        n >>= 1
+        length += 2
+    return length
+
+
+class Batch:
+    """Batched multi-stream search (rh264_batch_*): streams of uneven size, 1..4
+    references each, several partition shapes, one launch per stage per shape.
+
+    streams: list of dicts with device buffers `cur` (cw x ch), `planes` (nrefs
+    plane sets back to back), optional `mvps`, and ints cw, ch, nrefs."""
+
+    def __init__(self, dev, streams, shapes, pad=HPEL_PAD_DEFAULT, search_range=32, subpel_pat=0, lam=0.0,
+                 lam_fp=None):
+        self.dev, self.shapes = dev, [tuple(s) for s in shapes]
+        lib = dev.lib
+        sh = np.ascontiguousarray(np.array(self.shapes, np.int32).reshape(-1))
+        self._sh = sh
+        arr = (_lib.Stream * len(streams))()
+        self.outs = []
+        for i, st in enumerate(streams):
+            cw, ch, nr = st["cw"], st["ch"], st["nrefs"]
+            _need(st["cur"], cw * ch, f"stream {i} cur")
+            _need(st["planes"], nr * planes_bytes(cw, ch, pad), f"stream {i} planes")
+            n = int(lib.rh264_batch_out_len(cw, ch, sh.ctypes.data_as(_lib.c_i32p), len(self.shapes)))
+            mvps = st.get("mvps")
+            _need(mvps, 4 * int(lib.rh264_batch_mvp_len(cw, ch, sh.ctypes.data_as(_lib.c_i32p), len(self.shapes))),
+                  f"stream {i} mvps")
+            out = dev.alloc(4 * n)
+            self.outs.append((out, cw, ch))
+            arr[i].cur, arr[i].planes = int(st["cur"]), int(st["planes"])
+            arr[i].mvps = _ptr(mvps)
+            arr[i].cw, arr[i].ch, arr[i].nrefs, arr[i].out = cw, ch, nr, out.ptr
+        lr = np.ascontiguousarray(lambda_table(lam))
+        lr_fp = np.ascontiguousarray(lambda_table(lam * 0.5 if lam_fp is None else lam_fp))
+        h = C.c_void_p()
+        _check(lib.rh264_batch_create(dev.ctx, arr, len(streams), sh.ctypes.data_as(_lib.c_i32p), len(self.shapes),
+                                      pad, search_range, subpel_pat, lr_fp.ctypes.data_as(_lib.c_i32p),
+                                      lr.ctypes.data_as(_lib.c_i32p), float(lam), C.byref(h)), "rh264_batch_create")
+        self.h = h.value
+
+    def run(self):
+        _check(self.dev.lib.rh264_batch_run(self.dev.ctx, self.h), "rh264_batch_run")
+
+    def autotune(self, reps=2):
+        _check(self.dev.lib.rh264_batch_autotune(self.dev.ctx, self.h, reps), "rh264_batch_autotune")
+        out = []
+        for i in range(len(self.shapes)):
+            v, p = C.c_int(), C.c_int()
+            _check(self.dev.lib.rh264_batch_choice(self.h, i, C.byref(v), C.byref(p)), "rh264_batch_choice")
+            out.append((v.value, p.value))
+        return out
+
+    def capture(self):
+        _check(self.dev.lib.rh264_batch_capture(self.dev.ctx, self.h), "rh264_batch_capture")
+
+    def replay(self):
+        _check(self.dev.lib.rh264_batch_replay(self.dev.ctx, self.h), "rh264_batch_replay")
+
+    def results(self):
+        """Per stream: {shape: (nblk, 4) int32 records {ref, mvx, mvy, cost}}."""
+        res = []
+        for out, cw, ch in self.outs:
+            flat = self.dev.download(out, (out.nbytes // 4,), np.int32)
+            d, off = {}, 0
+            for bw, bh in self.shapes:
+                nb = (cw // bw) * (ch // bh)
+                d[(bw, bh)] = flat[off:off + 4 * nb].reshape(nb, 4)
+                off += 4 * nb
+            res.append(d)
+        return res
+
+    def close(self):
+        if self.h:
+            self.dev.lib.rh264_batch_destroy(self.h)
+            self.h = None
+        for out, _, _ in self.outs:
+            out.free()
+        self.outs = []

@@ -346,81 +346,62 @@ __device__ __forceinline__ void fs_rows(FsLane<BW, BH> &L, const uint8_t *__rest
 #ifndef RH264_FS_WAVES
 #define RH264_FS_WAVES 2
 #endif
+// One lane's share of the integer search of one block: the block's source, its
+// reference plane F and search geometry.
+struct FsJob {
+    const uint8_t *srcp; // top-left sample of the source block (row stride cw)
+    const uint8_t *F;    // full-pel plane of the reference's plane set
+    int cw, pw, ph, pad, R;
+    int lx, ly, cx, cy, mpx, mpy, g;
+};
+
 template <int BW, int BH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
-void k_full_search(rh264_search_params p,
-                                                     const uint8_t *__restrict__ cur,
-                                                     size_t cur_stride,
-                                                     const uint8_t *__restrict__ planes,
-                                                     size_t planes_stride,
-                                                     const int32_t *__restrict__ centres,
-                                                     const int32_t *__restrict__ mvps,
-                                                     uint64_t *__restrict__ keys)
+__device__ __forceinline__ uint64_t fs_qsad_key(const FsJob &J, const int32_t *s_lr)
 {
     using S = FsShape<BW, BH>;
-    __shared__ int32_t s_lr[RH264_LR_N];
-    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
-    __syncthreads();
-
-    const int R = p.range, NC = 2 * R + 1, NG = (NC + 3) / 4;
-    const int nbx = p.cw / BW, nby = p.ch / BH, nblk = nbx * nby;
-    const int pw = p.cw + 2 * p.pad, ph = p.ch + 2 * p.pad;
-    const int frame = blockIdx.y;
-    const int lane = threadIdx.x & 63;
-    const long gi = (long)blockIdx.x * 256 + threadIdx.x;
-    const int b = (int)(gi / NG);
-    const int g = (int)(gi - (long)b * NG);
-    const bool active = b < nblk;
-    const int bsafe = active ? b : nblk - 1;
-
-    const uint8_t *F = planes + (size_t)frame * planes_stride; // plane 0
-    const size_t boff = (size_t)frame * nblk + bsafe;
-    const int lx = (bsafe % nbx) * BW, ly = (bsafe / nbx) * BH;
-    const int cx = centres ? centres[2 * boff] : 0, cy = centres ? centres[2 * boff + 1] : 0;
-    const int mpx = mvps ? mvps[2 * boff] : 0, mpy = mvps ? mvps[2 * boff + 1] : 0;
-
+    const int R = J.R, NC = 2 * R + 1, g = J.g;
     FsLane<BW, BH> L;
-    {
-        const uint8_t *sp = cur + (size_t)frame * cur_stride + (size_t)ly * p.cw + lx;
 #pragma unroll
-        for (int k = 0; k < BH; k++)
+    for (int k = 0; k < BH; k++)
 #pragma unroll
-            for (int i = 0; i < S::SD; i++) L.src[k][i] = *(const uint32_t *)(sp + (size_t)k * p.cw + 4 * i);
-    }
-
+        for (int i = 0; i < S::SD; i++) L.src[k][i] = *(const uint32_t *)(J.srcp + (size_t)k * J.cw + 4 * i);
     const int dx0 = -R + 4 * g;
-    const int px0 = lx + (cx >> 2) + p.pad + dx0; // padded column of candidate dx0
-    const int py0 = ly + (cy >> 2) + p.pad - R;   // padded row of candidate dy = -R
+    const int px0 = J.lx + (J.cx >> 2) + J.pad + dx0; // padded column of candidate dx0
+    const int py0 = J.ly + (J.cy >> 2) + J.pad - R;   // padded row of candidate dy = -R
     // whole window of the block inside the padded plane -> direct aligned reads
-    const int wx_lo = lx + (cx >> 2) + p.pad - R, wx_hi = lx + (cx >> 2) + p.pad + R + BW - 1;
-    const bool fast = wx_lo >= 0 && wx_hi <= pw - 1 && py0 >= 0 && py0 + 2 * R + BH - 1 <= ph - 1;
+    const int wx_lo = J.lx + (J.cx >> 2) + J.pad - R, wx_hi = wx_lo + 2 * R + BW - 1;
+    const bool fast = wx_lo >= 0 && wx_hi <= J.pw - 1 && py0 >= 0 && py0 + 2 * R + BH - 1 <= J.ph - 1;
     const int shift = px0 & 3;
-
 #pragma unroll
-    for (int k = 0; k < 4; k++) L.rxb[k] = mvbits(cx + 4 * (dx0 + k) - mpx);
+    for (int k = 0; k < 4; k++) L.rxb[k] = mvbits(J.cx + 4 * (dx0 + k) - J.mpx);
 #pragma unroll
-    for (int s = 0; s < BH; s++) L.acc[s] = 0;
+    for (int sl = 0; sl < BH; sl++) L.acc[sl] = 0;
     L.best = 0xffffffffu;
     L.bidx = 0xffffffffu;
     const int NJ = 2 * R + BH;
-
     if ((2 * R) % BH == 0) {
         // window height NJ is a whole number of ring blocks: peel the first and last
-        fs_rows<BW, BH, 1>(L, F, pw, ph, fast, px0, py0, shift, 0, NJ, R, NC, g, cy, mpy, s_lr);
+        fs_rows<BW, BH, 1>(L, J.F, J.pw, J.ph, fast, px0, py0, shift, 0, NJ, R, NC, g, J.cy, J.mpy, s_lr);
         for (int jb = BH; jb < NJ - BH; jb += BH)
-            fs_rows<BW, BH, 0>(L, F, pw, ph, fast, px0, py0, shift, jb, NJ, R, NC, g, cy, mpy, s_lr);
-        fs_rows<BW, BH, 2>(L, F, pw, ph, fast, px0, py0, shift, NJ - BH, NJ, R, NC, g, cy, mpy, s_lr);
+            fs_rows<BW, BH, 0>(L, J.F, J.pw, J.ph, fast, px0, py0, shift, jb, NJ, R, NC, g, J.cy, J.mpy, s_lr);
+        fs_rows<BW, BH, 2>(L, J.F, J.pw, J.ph, fast, px0, py0, shift, NJ - BH, NJ, R, NC, g, J.cy, J.mpy, s_lr);
     } else {
         for (int jb = 0; jb < NJ; jb += BH)
-            fs_rows<BW, BH, 0>(L, F, pw, ph, fast, px0, py0, shift, jb, NJ, R, NC, g, cy, mpy, s_lr);
+            fs_rows<BW, BH, 0>(L, J.F, J.pw, J.ph, fast, px0, py0, shift, jb, NJ, R, NC, g, J.cy, J.mpy, s_lr);
     }
+    return ((uint64_t)L.best << 32) | L.bidx;
+}
 
-    uint64_t key = active ? (((uint64_t)L.best << 32) | L.bidx) : ~0ull;
-    const int seg = active ? b : 0x7fffffff;
-    key = wave_seg_min(key, seg, lane);
-    const int prev_seg = __shfl_up(seg, 1, 64);
-    if (active && (lane == 0 || prev_seg != seg))
-        atomicMin((unsigned long long *)&keys[boff], (unsigned long long)key);
+// Segmented wave argmin of the lanes' keys (lanes of one block are contiguous),
+// then one 64-bit atomicMin per block segment: the global first raster minimum.
+__device__ __forceinline__ void fs_commit(uint64_t key, long seg, bool active, uint64_t *slot)
+{
+    const int lane = threadIdx.x & 63;
+    key = active ? key : ~0ull;
+    const int segi = active ? (int)(seg & 0x3fffffff) : 0x7fffffff;
+    key = wave_seg_min(key, segi, lane);
+    const int prev = __shfl_up(segi, 1, 64);
+    if (active && (lane == 0 || prev != segi)) atomicMin((unsigned long long *)slot, (unsigned long long)key);
 }
 
 // The same search with v_sad_u8 (4 abs-diff-adds per lane-op at full issue rate):
@@ -507,50 +488,24 @@ __device__ __forceinline__ void fs8_rows(FsLane8<BW, BH, NQ> &L, const uint8_t *
 template <int NQ> __host__ __device__ constexpr int fs8_groups(int NC) { return (NC + 3 + NQ - 1) / NQ; }
 
 template <int BW, int BH, int NQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
-void k_full_search8(rh264_search_params p, const uint8_t *__restrict__ cur, size_t cur_stride,
-                    const uint8_t *__restrict__ planes, size_t planes_stride, const int32_t *__restrict__ centres,
-                    const int32_t *__restrict__ mvps, uint64_t *__restrict__ keys)
+__device__ __forceinline__ uint64_t fs_sad8_key(const FsJob &J, const int32_t *s_lr)
 {
     using S = FsShape<BW, BH>;
-    __shared__ int32_t s_lr[RH264_LR_N];
-    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
-    __syncthreads();
-
-    const int R = p.range, NC = 2 * R + 1, NG = fs8_groups<NQ>(NC);
-    const int nbx = p.cw / BW, nby = p.ch / BH, nblk = nbx * nby;
-    const int pw = p.cw + 2 * p.pad, ph = p.ch + 2 * p.pad;
-    const int frame = blockIdx.y;
-    const int lane = threadIdx.x & 63;
-    const long gi = (long)blockIdx.x * 256 + threadIdx.x;
-    const int b = (int)(gi / NG);
-    const int g = (int)(gi - (long)b * NG);
-    const bool active = b < nblk;
-    const int bsafe = active ? b : nblk - 1;
-
-    const uint8_t *F = planes + (size_t)frame * planes_stride;
-    const size_t boff = (size_t)frame * nblk + bsafe;
-    const int lx = (bsafe % nbx) * BW, ly = (bsafe / nbx) * BH;
-    const int cx = centres ? centres[2 * boff] : 0, cy = centres ? centres[2 * boff + 1] : 0;
-    const int mpx = mvps ? mvps[2 * boff] : 0, mpy = mvps ? mvps[2 * boff + 1] : 0;
-
+    const int R = J.R, NC = 2 * R + 1;
     FsLane8<BW, BH, NQ> L;
-    {
-        const uint8_t *sp = cur + (size_t)frame * cur_stride + (size_t)ly * p.cw + lx;
 #pragma unroll
-        for (int k = 0; k < BH; k++)
+    for (int k = 0; k < BH; k++)
 #pragma unroll
-            for (int i = 0; i < S::SD; i++) L.src[k][i] = *(const uint32_t *)(sp + (size_t)k * p.cw + 4 * i);
-    }
-    const int wx_lo = lx + (cx >> 2) + p.pad - R, wx_hi = wx_lo + 2 * R + BW - 1;
-    const int py0 = ly + (cy >> 2) + p.pad - R;
-    const int base = (wx_lo & ~3) + NQ * g; // absolute padded column of the lane's first candidate
+        for (int i = 0; i < S::SD; i++) L.src[k][i] = *(const uint32_t *)(J.srcp + (size_t)k * J.cw + 4 * i);
+    const int wx_lo = J.lx + (J.cx >> 2) + J.pad - R, wx_hi = wx_lo + 2 * R + BW - 1;
+    const int py0 = J.ly + (J.cy >> 2) + J.pad - R;
+    const int base = (wx_lo & ~3) + NQ * J.g; // absolute padded column of the lane's first candidate
     const int abase = base & ~3;
     L.sh = base & 3;
     L.c0 = base - wx_lo;
-    const bool fast = wx_lo >= 0 && wx_hi <= pw - 1 && py0 >= 0 && py0 + 2 * R + BH - 1 <= ph - 1;
+    const bool fast = wx_lo >= 0 && wx_hi <= J.pw - 1 && py0 >= 0 && py0 + 2 * R + BH - 1 <= J.ph - 1;
 #pragma unroll
-    for (int q = 0; q < NQ; q++) L.rxb[q] = mvbits(cx + 4 * (L.c0 + q - R) - mpx);
+    for (int q = 0; q < NQ; q++) L.rxb[q] = mvbits(J.cx + 4 * (L.c0 + q - R) - J.mpx);
 #pragma unroll
     for (int sl = 0; sl < BH; sl++)
 #pragma unroll
@@ -559,21 +514,66 @@ void k_full_search8(rh264_search_params p, const uint8_t *__restrict__ cur, size
     L.bidx = 0xffffffffu;
     const int NJ = 2 * R + BH;
     if ((2 * R) % BH == 0) {
-        fs8_rows<BW, BH, NQ, 1>(L, F, pw, ph, fast, abase, py0, 0, NJ, R, NC, cy, mpy, s_lr);
+        fs8_rows<BW, BH, NQ, 1>(L, J.F, J.pw, J.ph, fast, abase, py0, 0, NJ, R, NC, J.cy, J.mpy, s_lr);
         for (int jb = BH; jb < NJ - BH; jb += BH)
-            fs8_rows<BW, BH, NQ, 0>(L, F, pw, ph, fast, abase, py0, jb, NJ, R, NC, cy, mpy, s_lr);
-        fs8_rows<BW, BH, NQ, 2>(L, F, pw, ph, fast, abase, py0, NJ - BH, NJ, R, NC, cy, mpy, s_lr);
+            fs8_rows<BW, BH, NQ, 0>(L, J.F, J.pw, J.ph, fast, abase, py0, jb, NJ, R, NC, J.cy, J.mpy, s_lr);
+        fs8_rows<BW, BH, NQ, 2>(L, J.F, J.pw, J.ph, fast, abase, py0, NJ - BH, NJ, R, NC, J.cy, J.mpy, s_lr);
     } else {
         for (int jb = 0; jb < NJ; jb += BH)
-            fs8_rows<BW, BH, NQ, 0>(L, F, pw, ph, fast, abase, py0, jb, NJ, R, NC, cy, mpy, s_lr);
+            fs8_rows<BW, BH, NQ, 0>(L, J.F, J.pw, J.ph, fast, abase, py0, jb, NJ, R, NC, J.cy, J.mpy, s_lr);
     }
+    return ((uint64_t)L.best << 32) | L.bidx;
+}
 
-    uint64_t key = active ? (((uint64_t)L.best << 32) | L.bidx) : ~0ull;
-    const int seg = active ? b : 0x7fffffff;
-    key = wave_seg_min(key, seg, lane);
-    const int prev_seg = __shfl_up(seg, 1, 64);
-    if (active && (lane == 0 || prev_seg != seg))
-        atomicMin((unsigned long long *)&keys[boff], (unsigned long long)key);
+// Lanes per block of each integer-search variant (V: 0 quad-SAD, 1 v_sad_u8 x2, 2 x4).
+template <int V> __host__ __device__ constexpr int fs_groups(int NC)
+{
+    return V == 0 ? (NC + 3) / 4 : (V == 1 ? fs8_groups<2>(NC) : fs8_groups<4>(NC));
+}
+
+template <int BW, int BH, int V>
+__device__ __forceinline__ uint64_t fs_key(const FsJob &J, const int32_t *s_lr)
+{
+    if (V == 0) return fs_qsad_key<BW, BH>(J, s_lr);
+    if (V == 1) return fs_sad8_key<BW, BH, 2>(J, s_lr);
+    return fs_sad8_key<BW, BH, 4>(J, s_lr);
+}
+
+// Single-stream integer search: nframes frames of one geometry (blockIdx.y = frame).
+template <int BW, int BH, int V>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
+void k_full_search(rh264_search_params p, const uint8_t *__restrict__ cur, size_t cur_stride,
+                   const uint8_t *__restrict__ planes, size_t planes_stride, const int32_t *__restrict__ centres,
+                   const int32_t *__restrict__ mvps, uint64_t *__restrict__ keys)
+{
+    __shared__ int32_t s_lr[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
+    __syncthreads();
+    const int NG = fs_groups<V>(2 * p.range + 1);
+    const int nbx = p.cw / BW, nblk = nbx * (p.ch / BH);
+    const int frame = blockIdx.y;
+    const long gi = (long)blockIdx.x * 256 + threadIdx.x;
+    const int b = (int)(gi / NG);
+    const bool active = b < nblk;
+    const int bs = active ? b : nblk - 1;
+    const size_t boff = (size_t)frame * nblk + bs;
+    FsJob J;
+    J.cw = p.cw;
+    J.pw = p.cw + 2 * p.pad;
+    J.ph = p.ch + 2 * p.pad;
+    J.pad = p.pad;
+    J.R = p.range;
+    J.g = (int)(gi - (long)b * NG);
+    J.lx = (bs % nbx) * BW;
+    J.ly = (bs / nbx) * BH;
+    J.cx = centres ? centres[2 * boff] : 0;
+    J.cy = centres ? centres[2 * boff + 1] : 0;
+    J.mpx = mvps ? mvps[2 * boff] : 0;
+    J.mpy = mvps ? mvps[2 * boff + 1] : 0;
+    J.F = planes + (size_t)frame * planes_stride;
+    J.srcp = cur + (size_t)frame * cur_stride + (size_t)J.ly * p.cw + J.lx;
+    const uint64_t key = fs_key<BW, BH, V>(J, s_lr);
+    fs_commit(key, b, active, keys + boff);
 }
 
 // ---------------------------------------------------------------------------
@@ -719,6 +719,41 @@ __device__ __forceinline__ int eval_satd_lds(const uint8_t *win, uint32_t srow, 
     return group_sum<LPB>(hadamard_row_quad(d0, d1, d2, d3, r));
 }
 
+// The ring walk of one block by its lane group after its window is staged.
+template <int BW, int BH, int LPB>
+__device__ __forceinline__ void refine_walk(const RefineGeom &G, const uint8_t *win, uint32_t srow, int lx, int ly,
+                                            int bxi, int byi, int r, int sxi, int syi, int &bx, int &by, int mpx,
+                                            int mpy, int pat, const int32_t *s_lr, int &best, int &evals)
+{
+    auto cost_of = [&](int mx, int my) -> int {
+        const int di = (mx >> 2) - sxi, dj = (my >> 2) - syi;
+        const int satd = (di >= -3 && di <= 2 && dj >= -3 && dj <= 2)
+                             ? eval_satd_lds<BW, BH, LPB>(win, srow, bxi, byi, r, di, dj, mx & 3, my & 3)
+                             : eval_satd<BW, BH, LPB>(G, srow, lx, ly, bxi, byi, r, mx, my);
+        return satd + s_lr[mvbits(mx - mpx) + mvbits(my - mpy)];
+    };
+    best = cost_of(bx, by);
+    evals = 1;
+    if (pat >= 0) {
+        const int nring = (pat & 1) ? 4 : 8;
+        for (int step = 2; step >= 1; step--) {
+            const int rx[8] = {step, -step, 0, 0, step, -step, step, -step};
+            const int ry[8] = {0, 0, step, -step, step, -step, -step, step};
+            // terminates: every extra pass strictly lowers a cost bounded below by 0
+            for (;;) {
+                bool improved = false;
+                for (int i = 0; i < nring; i++) {
+                    const int mx = bx + rx[i], my = by + ry[i];
+                    const int cc = cost_of(mx, my);
+                    evals++;
+                    if (cc < best) { best = cc; bx = mx; by = my; improved = true; }
+                }
+                if (!improved || (pat & 2)) break;
+            }
+        }
+    }
+}
+
 template <int BW, int BH>
 __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
                                                        const uint8_t *__restrict__ cur,
@@ -788,41 +823,206 @@ __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
     __syncthreads();
     if (!active) return; // whole groups exit together
 
-    auto cost_of = [&](int mx, int my) -> int {
-        const int di = (mx >> 2) - sxi, dj = (my >> 2) - syi;
-        const int satd = (di >= -3 && di <= 2 && dj >= -3 && dj <= 2)
-                             ? eval_satd_lds<BW, BH, LPB>(win, srow, bxi, byi, r, di, dj, mx & 3, my & 3)
-                             : eval_satd<BW, BH, LPB>(G, srow, lx, ly, bxi, byi, r, mx, my);
-        return satd + s_lr[mvbits(mx - mpx) + mvbits(my - mpy)];
-    };
-
-    int best = cost_of(bx, by);
-    int evals = 1;
-    const int pat = p.subpel_pat;
-    if (pat >= 0) {
-        const int nring = (pat & 1) ? 4 : 8;
-        for (int step = 2; step >= 1; step--) {
-            const int rx[8] = {step, -step, 0, 0, step, -step, step, -step};
-            const int ry[8] = {0, 0, step, -step, step, -step, -step, step};
-            // terminates: every extra pass strictly lowers a cost bounded below by 0
-            for (;;) {
-                bool improved = false;
-                for (int i = 0; i < nring; i++) {
-                    const int mx = bx + rx[i], my = by + ry[i];
-                    const int cc = cost_of(mx, my);
-                    evals++;
-                    if (cc < best) { best = cc; bx = mx; by = my; improved = true; }
-                }
-                if (!improved || (pat & 2)) break;
-            }
-        }
-    }
+    int best, evals;
+    refine_walk<BW, BH, LPB>(G, win, srow, lx, ly, bxi, byi, r, sxi, syi, bx, by, mpx, mpy, p.subpel_pat, s_lr, best,
+                             evals);
     if (li == 0) {
         out_mv[2 * boff] = bx;
         out_mv[2 * boff + 1] = by;
         out_cost[boff] = best;
         if (out_evals) out_evals[boff] = evals;
     }
+}
+
+// ---------------------------------------------------------------------------
+// batched multi-stream search
+// ---------------------------------------------------------------------------
+//
+// A batch is a list of segments, one per (stream, reference) for one partition
+// shape.  Segments have their own frame size and pointers; full-search lanes and
+// blocks are numbered consecutively across segments, and each lane or lane group
+// finds its segment by binary search over the segment start indices.
+
+struct SegDesc {
+    const uint8_t *cur;    // current frame of the stream (cw x ch)
+    const uint8_t *planes; // plane set of this reference
+    const int32_t *mvps;   // per-block rate predictors of this stream and shape, or null
+    int32_t cw, ch, nbx, nblk;
+    int64_t lane0;         // first integer-search lane of the segment
+    int64_t blk0;          // first block of the segment in the key / per-reference arrays
+};
+
+struct SelDesc {           // one stream's reference selection for one shape
+    int64_t ref_blk0[4];   // blk0 of the stream's segment for each reference
+    int64_t sel0;          // first block of the stream in the selection numbering
+    int32_t *out;          // {ref, mvx, mvy, cost} per block
+    int32_t nblk, nrefs;
+    int32_t lr_ref[4];     // (int64)(lambda * ref_bits(r, nrefs)), mb16.rs:4951-4956, 5967-5980
+};
+
+struct LrTab {
+    int32_t v[RH264_LR_N];
+};
+
+__device__ __forceinline__ int seg_find_lane(const SegDesc *segs, int nseg, long gi)
+{
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].lane0 <= gi) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int seg_find_blk(const SegDesc *segs, int nseg, long gb)
+{
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].blk0 <= gb) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Integer search over all segments.  persistent != 0: the grid is sized to the
+// machine and blocks take 256-lane tiles from a work counter until none is left;
+// otherwise block i does tile i.
+template <int BW, int BH, int V>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
+void k_fs_multi(const SegDesc *__restrict__ segs, int nseg, long total_lanes, int R, int pad, LrTab lr_fp,
+                uint64_t *__restrict__ keys, unsigned *__restrict__ counter, int persistent)
+{
+    __shared__ int32_t s_lr[RH264_LR_N];
+    __shared__ unsigned s_tile;
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = lr_fp.v[threadIdx.x];
+    const int NG = fs_groups<V>(2 * R + 1);
+    for (unsigned it = 0;; it++) {
+        unsigned tile;
+        if (persistent) {
+            __syncthreads(); // s_tile of the previous round has been read by every thread
+            if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+            __syncthreads();
+            tile = s_tile;
+        } else {
+            __syncthreads();
+            if (it > 0) break;
+            tile = blockIdx.x;
+        }
+        if ((long)tile * 256 >= total_lanes) break;
+        const long gi = (long)tile * 256 + threadIdx.x;
+        const bool inrange = gi < total_lanes;
+        const int si = seg_find_lane(segs, nseg, inrange ? gi : total_lanes - 1);
+        const SegDesc S = segs[si];
+        const long local = (inrange ? gi : total_lanes - 1) - S.lane0;
+        const int b = (int)(local / NG);
+        const bool active = inrange && b < S.nblk;
+        const int bs = min(b, S.nblk - 1);
+        FsJob J;
+        J.cw = S.cw;
+        J.pw = S.cw + 2 * pad;
+        J.ph = S.ch + 2 * pad;
+        J.pad = pad;
+        J.R = R;
+        J.g = (int)(local - (long)b * NG);
+        J.lx = (bs % S.nbx) * BW;
+        J.ly = (bs / S.nbx) * BH;
+        J.cx = 0;
+        J.cy = 0;
+        J.mpx = S.mvps ? S.mvps[2 * bs] : 0;
+        J.mpy = S.mvps ? S.mvps[2 * bs + 1] : 0;
+        J.F = S.planes;
+        J.srcp = S.cur + (size_t)J.ly * S.cw + J.lx;
+        const uint64_t key = fs_key<BW, BH, V>(J, s_lr);
+        fs_commit(key, S.blk0 + bs, active, keys + S.blk0 + bs);
+    }
+}
+
+// Refinement of every block of every segment from its integer result; writes
+// {mvx, mvy, cost} per block into res.
+template <int BW, int BH>
+__global__ __launch_bounds__(256) void k_refine_multi(const SegDesc *__restrict__ segs, int nseg, long total_blocks,
+                                                      int R, int pad, int pat, LrTab lr,
+                                                      const uint64_t *__restrict__ keys, int32_t *__restrict__ res)
+{
+    constexpr int LPB = BW * BH / 4;
+    constexpr int GPB = 256 / LPB;
+    using W = RefineWin<BW, BH>;
+    __shared__ int32_t s_lr[RH264_LR_N];
+    __shared__ __align__(16) uint8_t s_win[GPB * W::BYTES];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = lr.v[threadIdx.x];
+    const int grp = threadIdx.x / LPB, li = threadIdx.x % LPB;
+    long gb0 = (long)blockIdx.x * GPB + grp;
+    if (LPB == 64) gb0 = __builtin_amdgcn_readfirstlane(gb0);
+    const bool active = gb0 < total_blocks;
+    const long gb = active ? gb0 : total_blocks - 1;
+    const int si = seg_find_blk(segs, nseg, gb);
+    const SegDesc S = segs[si];
+    const int b = (int)(gb - S.blk0);
+    const int lx = (b % S.nbx) * BW, ly = (b / S.nbx) * BH;
+    const int bi = li >> 2, r = li & 3;
+    const int bxi = bi % (BW / 4), byi = bi / (BW / 4);
+    RefineGeom G;
+    G.pw = S.cw + 2 * pad;
+    G.ph = S.ch + 2 * pad;
+    G.pad = pad;
+    G.n = (size_t)G.pw * G.ph;
+    G.pl = S.planes;
+    const uint32_t srow = *(const uint32_t *)(S.cur + (size_t)(ly + 4 * byi + r) * S.cw + lx + 4 * bxi);
+    const int mpx = S.mvps ? S.mvps[2 * b] : 0, mpy = S.mvps ? S.mvps[2 * b + 1] : 0;
+    const uint64_t key = keys[gb];
+    const int NC = 2 * R + 1;
+    const uint32_t idx = (uint32_t)key;
+    int bx = 4 * ((int)(idx % (uint32_t)NC) - R), by = 4 * ((int)(idx / (uint32_t)NC) - R);
+    uint8_t *win = s_win + grp * W::BYTES;
+    const int sxi = bx >> 2, syi = by >> 2;
+    stage_window<BW, BH>(win, G, lx + sxi + pad - 4, ly + syi + pad - 3, li, LPB);
+    __syncthreads();
+    if (!active) return;
+    int best, evals;
+    refine_walk<BW, BH, LPB>(G, win, srow, lx, ly, bxi, byi, r, sxi, syi, bx, by, mpx, mpy, pat, s_lr, best, evals);
+    if (li == 0) {
+        res[3 * gb] = bx;
+        res[3 * gb + 1] = by;
+        res[3 * gb + 2] = best;
+    }
+}
+
+// best_part's reference loop (mb16.rs:4938-4957): cost + lr_ref[r], first minimum
+// over references in order.
+__global__ void k_select_ref(const SelDesc *__restrict__ sels, int nsel, long total, const int32_t *__restrict__ res)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int lo = 0, hi = nsel - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sels[mid].sel0 <= i) lo = mid; else hi = mid - 1;
+    }
+    const SelDesc &S = sels[lo];
+    const int b = (int)(i - S.sel0);
+    long long bc = 0x7fffffffffffffffLL;
+    int br = 0, bmx = 0, bmy = 0;
+    for (int rr = 0; rr < S.nrefs; rr++) {
+        const long g = S.ref_blk0[rr] + b;
+        const long long c = (long long)res[3 * g + 2] + S.lr_ref[rr];
+        if (c < bc) {
+            bc = c;
+            br = rr;
+            bmx = res[3 * g];
+            bmy = res[3 * g + 1];
+        }
+    }
+    int32_t *o = S.out + 4 * (size_t)b;
+    o[0] = br;
+    o[1] = bmx;
+    o[2] = bmy;
+    o[3] = (int32_t)bc;
+}
+
+__global__ void k_fill_u32(unsigned *p, size_t n, unsigned v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1323,29 +1523,29 @@ static int check_params(const rh264_search_params *p)
 
 } // extern "C"
 
+template <int BW, int BH, int V>
+static void launch_full_v(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                          size_t cur_stride, const uint8_t *planes, size_t planes_stride, const int32_t *centres,
+                          const int32_t *mvps, uint64_t *keys)
+{
+    const long nblk = (long)(p->cw / BW) * (p->ch / BH);
+    const long lanes = nblk * fs_groups<V>(2 * p->range + 1);
+    dim3 grid((unsigned)((lanes + 255) / 256), nframes);
+    hipLaunchKernelGGL((k_full_search<BW, BH, V>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes,
+                       planes_stride, centres, mvps, keys);
+}
+
 template <int BW, int BH>
 static int launch_full(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
                        size_t cur_stride, const uint8_t *planes, size_t planes_stride,
                        const int32_t *centres, const int32_t *mvps, uint64_t *keys)
 {
-    const int NC = 2 * p->range + 1;
-    const long nblk = (long)(p->cw / BW) * (p->ch / BH);
-    if (ctx->fs_kernel == 0) {
-        const long lanes = nblk * ((NC + 3) / 4);
-        dim3 grid((unsigned)((lanes + 255) / 256), nframes);
-        hipLaunchKernelGGL((k_full_search<BW, BH>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes,
-                           planes_stride, centres, mvps, keys);
-    } else if (ctx->fs_kernel == 2) {
-        const long lanes = nblk * fs8_groups<4>(NC);
-        dim3 grid((unsigned)((lanes + 255) / 256), nframes);
-        hipLaunchKernelGGL((k_full_search8<BW, BH, 4>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
-                           planes, planes_stride, centres, mvps, keys);
-    } else {
-        const long lanes = nblk * fs8_groups<2>(NC);
-        dim3 grid((unsigned)((lanes + 255) / 256), nframes);
-        hipLaunchKernelGGL((k_full_search8<BW, BH, 2>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
-                           planes, planes_stride, centres, mvps, keys);
-    }
+    if (ctx->fs_kernel == 1)
+        launch_full_v<BW, BH, 1>(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps, keys);
+    else if (ctx->fs_kernel == 2)
+        launch_full_v<BW, BH, 2>(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps, keys);
+    else
+        launch_full_v<BW, BH, 0>(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps, keys);
     LAUNCH_CHECK("k_full_search");
     return 0;
 }
@@ -1579,6 +1779,349 @@ int rh264_mbtree_propagate_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, 
                            mb_h, intra, inter, mv, (unsigned long long *)ctx->keys, prop);
         LAUNCH_CHECK("k_mbtree_prop_i32");
     }
+    return 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// batched multi-stream search: host side
+// ---------------------------------------------------------------------------
+
+static int ref_bits(int r, int nrefs) // mb16.rs:5967-5980
+{
+    if (nrefs <= 1) return 0;
+    if (nrefs == 2) return 1;
+    unsigned n = (unsigned)r + 1;
+    int len = 1;
+    while (n > 1) {
+        n >>= 1;
+        len += 2;
+    }
+    return len;
+}
+
+size_t rh264_batch_out_len(int cw, int ch, const int32_t *shapes, int nshapes)
+{
+    size_t n = 0;
+    for (int s = 0; s < nshapes; s++) n += (size_t)4 * (cw / shapes[2 * s]) * (ch / shapes[2 * s + 1]);
+    return n;
+}
+
+size_t rh264_batch_mvp_len(int cw, int ch, const int32_t *shapes, int nshapes)
+{
+    return rh264_batch_out_len(cw, ch, shapes, nshapes) / 2;
+}
+
+struct BatchShape {
+    int bw, bh;
+    int nseg, nsel;
+    long lanes[3];     // integer-search lanes per variant
+    long blocks;       // blocks over all segments
+    long sel_blocks;   // blocks over all streams
+    SegDesc *d_segs;
+    SelDesc *d_sels;
+    int variant, persist; // tuned kernel choice: variant 0..2, persist 0 = one tile per block, k = k*4 blocks per CU
+};
+
+struct rh264_batch {
+    int pad, range, pat, ncu;
+    LrTab lr_fp, lr;
+    std::vector<BatchShape> shapes;
+    uint64_t *keys;
+    int32_t *res;
+    unsigned *counter;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    bool has_graph;
+};
+
+int rh264_batch_destroy(rh264_batch *b)
+{
+    if (!b) return 0;
+    if (b->has_graph) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+    }
+    for (auto &S : b->shapes) {
+        if (S.d_segs) (void)hipFree(S.d_segs);
+        if (S.d_sels) (void)hipFree(S.d_sels);
+    }
+    if (b->keys) (void)hipFree(b->keys);
+    if (b->res) (void)hipFree(b->res);
+    if (b->counter) (void)hipFree(b->counter);
+    delete b;
+    return 0;
+}
+
+int rh264_batch_create(rh264_ctx *ctx, const rh264_stream *streams, int nstreams, const int32_t *shapes, int nshapes,
+                       int pad, int range, int subpel_pat, const int32_t *lr_fp, const int32_t *lr, double lambda_ref,
+                       rh264_batch **out)
+{
+    if (!out || !streams || !shapes || !lr_fp || !lr) return fail("batch_create: null argument");
+    *out = nullptr;
+    if (nstreams <= 0 || nshapes <= 0 || nshapes > 8) return fail("batch_create: %d streams, %d shapes", nstreams, nshapes);
+    if (range < 1 || range > 64 || subpel_pat < -1 || subpel_pat > 3 || pad < 4 || pad > 128 || pad % 4)
+        return RH264_DECLINED;
+    for (int s = 0; s < nshapes; s++)
+        if (!shape_ok(shapes[2 * s], shapes[2 * s + 1])) return RH264_DECLINED;
+    for (int i = 0; i < nstreams; i++) {
+        const rh264_stream &st = streams[i];
+        if (!geom_ok(st.cw, st.ch, pad) || st.nrefs < 1 || st.nrefs > 4) return RH264_DECLINED;
+        if (!st.cur || !st.planes || !st.out) return fail("batch_create: stream %d has a null pointer", i);
+    }
+    for (int i = 0; i < RH264_LR_N; i++)
+        if (lr[i] < 0 || lr_fp[i] < 0 || lr[i] > (1 << 28) || lr_fp[i] > (1 << 28))
+            return fail("lambda table entry %d out of range", i);
+    rh264_batch *b = new rh264_batch();
+    b->pad = pad;
+    b->range = range;
+    b->pat = subpel_pat;
+    b->keys = nullptr;
+    b->res = nullptr;
+    b->counter = nullptr;
+    b->has_graph = false;
+    memcpy(b->lr_fp.v, lr_fp, sizeof(b->lr_fp.v));
+    memcpy(b->lr.v, lr, sizeof(b->lr.v));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) {
+        delete b;
+        return fail("hipGetDeviceProperties failed");
+    }
+    b->ncu = prop.multiProcessorCount;
+    const int NC = 2 * range + 1;
+    long max_blocks = 1;
+    size_t mvp_off = 0, out_off = 0;
+    for (int s = 0; s < nshapes; s++) {
+        BatchShape S;
+        S.bw = shapes[2 * s];
+        S.bh = shapes[2 * s + 1];
+        S.d_segs = nullptr;
+        S.d_sels = nullptr;
+        S.variant = 0;
+        S.persist = 0;
+        std::vector<SegDesc> segs;
+        std::vector<SelDesc> sels;
+        long lane = 0, blk = 0, sel = 0;
+        const int ng0 = fs_groups<0>(NC);
+        for (int i = 0; i < nstreams; i++) {
+            const rh264_stream &st = streams[i];
+            const int nbx = st.cw / S.bw, nblk = nbx * (st.ch / S.bh);
+            // offsets of this shape's records inside the stream's out / mvp arrays
+            size_t o_off = 0, m_off = 0;
+            for (int t = 0; t < s; t++) {
+                const size_t nb = (size_t)(st.cw / shapes[2 * t]) * (st.ch / shapes[2 * t + 1]);
+                o_off += 4 * nb;
+                m_off += 2 * nb;
+            }
+            SelDesc sd;
+            memset(&sd, 0, sizeof(sd));
+            sd.sel0 = sel;
+            sd.out = st.out + o_off;
+            sd.nblk = nblk;
+            sd.nrefs = st.nrefs;
+            const size_t pbytes = rh264_planes_bytes(st.cw, st.ch, pad);
+            for (int r = 0; r < st.nrefs; r++) {
+                SegDesc g;
+                g.cur = st.cur;
+                g.planes = st.planes + (size_t)r * pbytes;
+                g.mvps = st.mvps ? st.mvps + m_off : nullptr;
+                g.cw = st.cw;
+                g.ch = st.ch;
+                g.nbx = nbx;
+                g.nblk = nblk;
+                g.lane0 = lane;
+                g.blk0 = blk;
+                sd.ref_blk0[r] = blk;
+                sd.lr_ref[r] = (int32_t)(int64_t)(lambda_ref * (double)ref_bits(r, st.nrefs));
+                segs.push_back(g);
+                lane += (long)nblk * ng0;
+                blk += nblk;
+            }
+            sels.push_back(sd);
+            sel += nblk;
+        }
+        (void)mvp_off;
+        (void)out_off;
+        S.nseg = (int)segs.size();
+        S.nsel = (int)sels.size();
+        S.blocks = blk;
+        S.sel_blocks = sel;
+        // lane numbering differs per variant (lanes per block); store each variant's table
+        // by recomputing lane0 on the device side from blocks -- keep the quad-SAD
+        // numbering here and per-variant lane counts for the launch.
+        for (int v = 0; v < 3; v++) S.lanes[v] = 0;
+        const int ngv[3] = {fs_groups<0>(NC), fs_groups<1>(NC), fs_groups<2>(NC)};
+        for (int v = 0; v < 3; v++) S.lanes[v] = blk * ngv[v];
+        // the device tables: one SegDesc array per variant (lane0 depends on the variant)
+        std::vector<SegDesc> all;
+        for (int v = 0; v < 3; v++) {
+            long l = 0;
+            for (auto g : segs) {
+                g.lane0 = l;
+                l += (long)g.nblk * ngv[v];
+                all.push_back(g);
+            }
+        }
+        hipError_t e = hipMalloc(&S.d_segs, all.size() * sizeof(SegDesc));
+        if (e == hipSuccess) e = hipMemcpy(S.d_segs, all.data(), all.size() * sizeof(SegDesc), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&S.d_sels, sels.size() * sizeof(SelDesc));
+        if (e == hipSuccess) e = hipMemcpy(S.d_sels, sels.data(), sels.size() * sizeof(SelDesc), hipMemcpyHostToDevice);
+        b->shapes.push_back(S);
+        if (e != hipSuccess) {
+            rh264_batch_destroy(b);
+            return fail("batch_create: %s", hipGetErrorString(e));
+        }
+        max_blocks = blk > max_blocks ? blk : max_blocks;
+    }
+    hipError_t e = hipMalloc(&b->keys, (size_t)max_blocks * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->res, (size_t)max_blocks * 3 * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->counter, 64);
+    if (e != hipSuccess) {
+        rh264_batch_destroy(b);
+        return fail("batch_create: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return 0;
+}
+
+} // extern "C"
+
+template <int BW, int BH, int V>
+static void launch_fs_multi_v(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S)
+{
+    const long tiles = (S.lanes[V] + 255) / 256;
+    unsigned grid = (unsigned)tiles;
+    if (S.persist > 0) {
+        const long cap = (long)b->ncu * 4 * S.persist;
+        grid = (unsigned)(tiles < cap ? tiles : cap);
+    }
+    hipLaunchKernelGGL((k_fs_multi<BW, BH, V>), dim3(grid), dim3(256), 0, ctx->stream, S.d_segs + (size_t)V * S.nseg,
+                       S.nseg, S.lanes[V], b->range, b->pad, b->lr_fp, b->keys, b->counter, S.persist);
+}
+
+template <int BW, int BH>
+static int batch_shape_run(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S, bool search_only)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((S.blocks + 255) / 256)), dim3(256), 0, ctx->stream, b->keys,
+                       (size_t)S.blocks, ~0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, b->counter, (size_t)16, 0u);
+    if (S.variant == 1) launch_fs_multi_v<BW, BH, 1>(ctx, b, S);
+    else if (S.variant == 2) launch_fs_multi_v<BW, BH, 2>(ctx, b, S);
+    else launch_fs_multi_v<BW, BH, 0>(ctx, b, S);
+    LAUNCH_CHECK("k_fs_multi");
+    if (search_only) return 0;
+    constexpr int GPB = 256 / (BW * BH / 4);
+    hipLaunchKernelGGL((k_refine_multi<BW, BH>), dim3((unsigned)((S.blocks + GPB - 1) / GPB)), dim3(256), 0,
+                       ctx->stream, S.d_segs, S.nseg, S.blocks, b->range, b->pad, b->pat, b->lr, b->keys, b->res);
+    LAUNCH_CHECK("k_refine_multi");
+    hipLaunchKernelGGL(k_select_ref, dim3((unsigned)((S.sel_blocks + 255) / 256)), dim3(256), 0, ctx->stream, S.d_sels,
+                       S.nsel, S.sel_blocks, b->res);
+    LAUNCH_CHECK("k_select_ref");
+    return 0;
+}
+
+static int batch_shape_dispatch(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S, bool search_only)
+{
+    switch (S.bw * 100 + S.bh) {
+    case 1616: return batch_shape_run<16, 16>(ctx, b, S, search_only);
+    case 1608: return batch_shape_run<16, 8>(ctx, b, S, search_only);
+    case 816: return batch_shape_run<8, 16>(ctx, b, S, search_only);
+    case 808: return batch_shape_run<8, 8>(ctx, b, S, search_only);
+    case 804: return batch_shape_run<8, 4>(ctx, b, S, search_only);
+    case 408: return batch_shape_run<4, 8>(ctx, b, S, search_only);
+    case 404: return batch_shape_run<4, 4>(ctx, b, S, search_only);
+    default: return RH264_DECLINED;
+    }
+}
+
+extern "C" {
+
+int rh264_batch_run(rh264_ctx *ctx, rh264_batch *b)
+{
+    if (!b) return fail("batch_run: null batch");
+    for (auto &S : b->shapes) {
+        int rc = batch_shape_dispatch(ctx, b, S, false);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int rh264_batch_autotune(rh264_ctx *ctx, rh264_batch *b, int reps)
+{
+    if (!b) return fail("batch_autotune: null batch");
+    if (b->has_graph) return fail("batch_autotune: the batch is already captured in a graph");
+    if (reps < 1) reps = 1;
+    const int persists[4] = {0, 1, 2, 4};
+    for (auto &S : b->shapes) {
+        float best_ms = 1e30f;
+        int bv = 0, bp = 0;
+        for (int v = 0; v < 3; v++)
+            for (int pi = 0; pi < 4; pi++) {
+                S.variant = v;
+                S.persist = persists[pi];
+                int rc = batch_shape_dispatch(ctx, b, S, true); // warm
+                if (rc) return rc;
+                HIP_TRY(hipEventRecord(ctx->ev[14], ctx->stream));
+                for (int k = 0; k < reps; k++) {
+                    rc = batch_shape_dispatch(ctx, b, S, true);
+                    if (rc) return rc;
+                }
+                HIP_TRY(hipEventRecord(ctx->ev[15], ctx->stream));
+                HIP_TRY(hipEventSynchronize(ctx->ev[15]));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[14], ctx->ev[15]));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    bv = v;
+                    bp = persists[pi];
+                }
+            }
+        S.variant = bv;
+        S.persist = bp;
+    }
+    return 0;
+}
+
+int rh264_batch_choice(rh264_batch *b, int shape_index, int *variant, int *persist)
+{
+    if (!b || shape_index < 0 || shape_index >= (int)b->shapes.size()) return fail("batch_choice: bad index");
+    *variant = b->shapes[shape_index].variant;
+    *persist = b->shapes[shape_index].persist;
+    return 0;
+}
+
+int rh264_batch_capture(rh264_ctx *ctx, rh264_batch *b)
+{
+    if (!b) return fail("batch_capture: null batch");
+    if (b->has_graph) {
+        HIP_TRY(hipGraphExecDestroy(b->exec));
+        HIP_TRY(hipGraphDestroy(b->graph));
+        b->has_graph = false;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = rh264_batch_run(ctx, b);
+    hipGraph_t g;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (e == hipSuccess) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail("hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail("hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    b->graph = g;
+    b->has_graph = true;
+    return 0;
+}
+
+int rh264_batch_replay(rh264_ctx *ctx, rh264_batch *b)
+{
+    if (!b || !b->has_graph) return fail("batch_replay: no captured graph");
+    HIP_TRY(hipGraphLaunch(b->exec, ctx->stream));
     return 0;
 }
 
