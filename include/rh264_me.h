@@ -158,7 +158,9 @@ int rh264_profile_get(rh264_ctx *ctx, int i, float *search_ms, float *refine_ms)
 /* Kernel-variant selection (no reference counterpart; set by the autotuner).
  * Knobs: "fs_kernel" 0 = quad-SAD (v_qsad_pk_u16_u8) accumulator ring
  * (default), 1 = v_sad_u8 ring with 2 candidates per lane, 2 = v_sad_u8 ring
- * with 4 candidates per lane.  Every variant returns identical results. */
+ * with 4 candidates per lane; "refine_lds" 1 (default) = the refinement of
+ * 16x16/16x8/8x16 blocks stages each block's +-3 pel plane window in LDS, 0 =
+ * it reads the plane sets in HBM.  Every variant returns identical results. */
 int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
 
 /* Quarter-pel prediction of blocks (mc_luma, inter.rs:1193-1262, served from

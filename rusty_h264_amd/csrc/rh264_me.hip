@@ -67,6 +67,7 @@ struct rh264_ctx {
     int32_t *tmp_mv;     // integer-stage MVs when the caller does not want them
     size_t tmp_mv_cap;   // in int32 elements
     int fs_kernel;                // integer search: 0 = quad-SAD, 1 = v_sad_u8 x2, 2 = v_sad_u8 x4
+    int refine_lds;               // refinement stages the +-3 pel window in LDS (blocks of >= 8 sub-blocks)
     std::vector<hipEvent_t> prof; // 3 events per profiled plain search
     int prof_cap, prof_n;         // capacity and count of profiled searches
 };
@@ -133,6 +134,15 @@ __constant__ uint32_t c_qtab[16] = {
 };
 #undef QENT
 __device__ __forceinline__ uint32_t qtab(int idx) { return c_qtab[idx]; }
+
+// Byte-wise differences of two packed rows of four pixels.
+__device__ __forceinline__ void diff4(uint32_t s, uint32_t p, int &d0, int &d1, int &d2, int &d3)
+{
+    d0 = (int)(s & 255) - (int)(p & 255);
+    d1 = (int)((s >> 8) & 255) - (int)((p >> 8) & 255);
+    d2 = (int)((s >> 16) & 255) - (int)((p >> 16) & 255);
+    d3 = (int)(s >> 24) - (int)(p >> 24);
+}
 
 // Per-byte rounded-up average (a+b+1)>>1 of four packed bytes.
 __device__ __forceinline__ uint32_t avg4(uint32_t a, uint32_t b)
@@ -627,40 +637,6 @@ struct RefineGeom {
     int pw, ph, pad;
 };
 
-// SATD (sum |H d|) of the block at (lx, ly) vs its prediction at (mvx, mvy).
-template <int BW, int BH, int LPB>
-__device__ __forceinline__ int eval_satd(const RefineGeom &G, uint32_t srow, int lx, int ly, int bxi,
-                                         int byi, int r, int mvx, int mvy)
-{
-    const int bpx = lx + (mvx >> 2) + G.pad, bpy = ly + (mvy >> 2) + G.pad;
-    const uint32_t q = qtab((mvx & 3) * 4 + (mvy & 3));
-    const bool two = ((mvx | mvy) & 1) != 0;
-    const int x = bpx + 4 * bxi, y = bpy + 4 * byi + r;
-    // acceptance guard of hpel_block / MeCtx (inter.rs:1030, 1075): +1 slack
-    const bool inb = bpx >= 0 && bpy >= 0 && bpx + BW + 1 <= G.pw && bpy + BH + 1 <= G.ph;
-    const uint8_t *pa = G.pl + G.n * (q & 15);
-    const int ax = (q >> 4) & 15, ay = (q >> 8) & 15;
-    uint32_t pred;
-    if (inb) {
-        pred = load4_fast(pa, G.pw, x + ax, y + ay);
-        if (two) {
-            const uint8_t *pb = G.pl + G.n * ((q >> 12) & 15);
-            pred = avg4(pred, load4_fast(pb, G.pw, x + ((q >> 16) & 15), y + ((q >> 20) & 15)));
-        }
-    } else {
-        pred = load4_clamped(pa, G.pw, G.ph, x + ax, y + ay);
-        if (two) {
-            const uint8_t *pb = G.pl + G.n * ((q >> 12) & 15);
-            pred = avg4(pred, load4_clamped(pb, G.pw, G.ph, x + ((q >> 16) & 15), y + ((q >> 20) & 15)));
-        }
-    }
-    int d0 = (int)(srow & 255) - (int)(pred & 255);
-    int d1 = (int)((srow >> 8) & 255) - (int)((pred >> 8) & 255);
-    int d2 = (int)((srow >> 16) & 255) - (int)((pred >> 16) & 255);
-    int d3 = (int)(srow >> 24) - (int)(pred >> 24);
-    return group_sum<LPB>(hadamard_row_quad(d0, d1, d2, d3, r));
-}
-
 // The refine walk reads one block's neighbourhood many times.  Each group stages
 // the four planes around its start vector in LDS once: columns [ix-4, ix+BW+4),
 // rows [iy-3, iy+BH+3) of the padded planes (ix, iy = padded integer position of
@@ -697,40 +673,107 @@ __device__ __forceinline__ uint32_t lds_load4(const uint8_t *rowp, int x)
     return __builtin_amdgcn_alignbyte(p[1], p[0], (uint32_t)(x & 3));
 }
 
-// SATD from the staged window; di, dj = integer offset of the candidate from the
-// start (both in [-3, 2]).
-template <int BW, int BH, int LPB>
-__device__ __forceinline__ int eval_satd_lds(const uint8_t *win, uint32_t srow, int bxi, int byi, int r, int di,
+// Refinement with one lane per 4x4 sub-block: the lane keeps its 16 source pixels
+// in four dwords, reads four prediction rows per candidate (one or two planes) and
+// does the whole 4x4 Hadamard in registers; a block's NSB = (BW/4)*(BH/4) lanes
+// then add their sums with DPP (no cross-lane butterflies).  Blocks of at least
+// 8 sub-blocks can stage their +-3 pel neighbourhood in LDS (USE_LDS).
+template <int BW, int BH> struct RefShape {
+    static constexpr int NSB = (BW / 4) * (BH / 4); // lanes per block
+    static constexpr int GPW = 256 / NSB;           // blocks per 256-thread workgroup
+};
+
+template <int LPB> __device__ __forceinline__ int group_sum_n(int v)
+{
+    if (LPB >= 2) v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+    if (LPB >= 4) v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    if (LPB >= 8) v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true); // row_half_mirror
+    if (LPB >= 16) v += __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true); // row_mirror
+    return v;
+}
+
+// sum |H d| of a 4x4 block, d = s - p, rows as packed dwords (transform.rs:637-655).
+__device__ __forceinline__ int satd4x4_regs(const uint32_t s[4], const uint32_t p[4])
+{
+    int m[16];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        int d0, d1, d2, d3;
+        diff4(s[r], p[r], d0, d1, d2, d3);
+        m[4 * r] = d0 + d1 + d2 + d3;
+        m[4 * r + 1] = d0 + d1 - d2 - d3;
+        m[4 * r + 2] = d0 - d1 - d2 + d3;
+        m[4 * r + 3] = d0 - d1 + d2 - d3;
+    }
+    int sum = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int a = m[c], b = m[4 + c], cc = m[8 + c], e = m[12 + c];
+        sum += abs(a + b + cc + e) + abs(a + b - cc - e) + abs(a - b - cc + e) + abs(a - b + cc - e);
+    }
+    return sum;
+}
+
+// Candidate read from the staged window; di, dj in [-3, 2] (integer offset from the
+// start vector), (sbx, sby) = the lane's sub-block.
+template <int BW, int BH>
+__device__ __forceinline__ int eval_lane_lds(const uint8_t *win, const uint32_t s[4], int sbx, int sby, int di,
                                              int dj, int fx, int fy)
 {
     using W = RefineWin<BW, BH>;
     const uint32_t q = qtab(fx * 4 + fy);
-    const int x = di + 4 + 4 * bxi, y = dj + 3 + 4 * byi + r;
-    const uint8_t *pa = win + (q & 15) * W::PLANE;
-    uint32_t pred = lds_load4(pa + (y + ((q >> 8) & 15)) * W::PITCH, x + ((q >> 4) & 15));
+    const int x = di + 4 + sbx, y = dj + 3 + sby;
+    const uint8_t *pa = win + (q & 15) * W::PLANE + (y + ((q >> 8) & 15)) * W::PITCH;
+    const int xa = x + ((q >> 4) & 15);
+    uint32_t p[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) p[r] = lds_load4(pa + r * W::PITCH, xa);
     if ((fx | fy) & 1) {
-        const uint8_t *pb = win + ((q >> 12) & 15) * W::PLANE;
-        pred = avg4(pred, lds_load4(pb + (y + ((q >> 20) & 15)) * W::PITCH, x + ((q >> 16) & 15)));
+        const uint8_t *pb = win + ((q >> 12) & 15) * W::PLANE + (y + ((q >> 20) & 15)) * W::PITCH;
+        const int xb = x + ((q >> 16) & 15);
+#pragma unroll
+        for (int r = 0; r < 4; r++) p[r] = avg4(p[r], lds_load4(pb + r * W::PITCH, xb));
     }
-    int d0 = (int)(srow & 255) - (int)(pred & 255);
-    int d1 = (int)((srow >> 8) & 255) - (int)((pred >> 8) & 255);
-    int d2 = (int)((srow >> 16) & 255) - (int)((pred >> 16) & 255);
-    int d3 = (int)(srow >> 24) - (int)(pred >> 24);
-    return group_sum<LPB>(hadamard_row_quad(d0, d1, d2, d3, r));
+    return satd4x4_regs(s, p);
 }
 
-// The ring walk of one block by its lane group after its window is staged.
-template <int BW, int BH, int LPB>
-__device__ __forceinline__ void refine_walk(const RefineGeom &G, const uint8_t *win, uint32_t srow, int lx, int ly,
-                                            int bxi, int byi, int r, int sxi, int syi, int &bx, int &by, int mpx,
+// Candidate read from the plane set in HBM (clamped at the plane border).
+template <int BW, int BH>
+__device__ __forceinline__ int eval_lane_global(const RefineGeom &G, const uint32_t s[4], int lx, int ly, int sbx,
+                                                int sby, int mvx, int mvy)
+{
+    const int bpx = lx + (mvx >> 2) + G.pad, bpy = ly + (mvy >> 2) + G.pad;
+    const uint32_t q = qtab((mvx & 3) * 4 + (mvy & 3));
+    const bool two = ((mvx | mvy) & 1) != 0;
+    // acceptance guard of hpel_block / MeCtx (inter.rs:1030, 1075): +1 slack
+    const bool inb = bpx >= 0 && bpy >= 0 && bpx + BW + 1 <= G.pw && bpy + BH + 1 <= G.ph;
+    const uint8_t *pa = G.pl + G.n * (q & 15), *pb = G.pl + G.n * ((q >> 12) & 15);
+    const int xa = bpx + sbx + ((q >> 4) & 15), ya = bpy + sby + ((q >> 8) & 15);
+    const int xb = bpx + sbx + ((q >> 16) & 15), yb = bpy + sby + ((q >> 20) & 15);
+    uint32_t p[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        p[r] = inb ? load4_fast(pa, G.pw, xa, ya + r) : load4_clamped(pa, G.pw, G.ph, xa, ya + r);
+        if (two) p[r] = avg4(p[r], inb ? load4_fast(pb, G.pw, xb, yb + r) : load4_clamped(pb, G.pw, G.ph, xb, yb + r));
+    }
+    return satd4x4_regs(s, p);
+}
+
+// The ring walk of one block by its lane group (mb16.rs:3230-3275).
+template <int BW, int BH, bool USE_LDS>
+__device__ __forceinline__ void refine_walk(const RefineGeom &G, const uint8_t *win, const uint32_t s[4], int lx,
+                                            int ly, int sbx, int sby, int sxi, int syi, int &bx, int &by, int mpx,
                                             int mpy, int pat, const int32_t *s_lr, int &best, int &evals)
 {
+    constexpr int NSB = RefShape<BW, BH>::NSB;
     auto cost_of = [&](int mx, int my) -> int {
         const int di = (mx >> 2) - sxi, dj = (my >> 2) - syi;
-        const int satd = (di >= -3 && di <= 2 && dj >= -3 && dj <= 2)
-                             ? eval_satd_lds<BW, BH, LPB>(win, srow, bxi, byi, r, di, dj, mx & 3, my & 3)
-                             : eval_satd<BW, BH, LPB>(G, srow, lx, ly, bxi, byi, r, mx, my);
-        return satd + s_lr[mvbits(mx - mpx) + mvbits(my - mpy)];
+        int v;
+        if (USE_LDS && di >= -3 && di <= 2 && dj >= -3 && dj <= 2)
+            v = eval_lane_lds<BW, BH>(win, s, sbx, sby, di, dj, mx & 3, my & 3);
+        else
+            v = eval_lane_global<BW, BH>(G, s, lx, ly, sbx, sby, mx, my);
+        return group_sum_n<NSB>(v) + s_lr[mvbits(mx - mpx) + mvbits(my - mpy)];
     };
     best = cost_of(bx, by);
     evals = 1;
@@ -754,7 +797,13 @@ __device__ __forceinline__ void refine_walk(const RefineGeom &G, const uint8_t *
     }
 }
 
-template <int BW, int BH>
+template <int BW, int BH> __device__ __forceinline__ void load_src4(const uint8_t *p, int stride, uint32_t s[4])
+{
+#pragma unroll
+    for (int r = 0; r < 4; r++) s[r] = *(const uint32_t *)(p + (size_t)r * stride);
+}
+
+template <int BW, int BH, bool USE_LDS>
 __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
                                                        const uint8_t *__restrict__ cur,
                                                        size_t cur_stride,
@@ -770,24 +819,21 @@ __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
                                                        int32_t *__restrict__ out_cost,
                                                        int32_t *__restrict__ out_evals)
 {
-    constexpr int LPB = BW * BH / 4;
-    constexpr int GPB = 256 / LPB; // groups per thread block
+    constexpr int NSB = RefShape<BW, BH>::NSB, GPW = RefShape<BW, BH>::GPW;
     using W = RefineWin<BW, BH>;
     __shared__ int32_t s_lr[RH264_LR_N];
-    __shared__ __align__(16) uint8_t s_win[GPB * W::BYTES];
+    __shared__ __align__(16) uint8_t s_win[USE_LDS ? GPW * W::BYTES : 16];
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
 
     const int nbx = p.cw / BW, nby = p.ch / BH, nblk = nbx * nby;
     const int frame = blockIdx.y;
-    const int grp = threadIdx.x / LPB, li = threadIdx.x % LPB;
-    int b0 = blockIdx.x * GPB + grp;
-    if (LPB == 64) b0 = __builtin_amdgcn_readfirstlane(b0); // the group is the wave: keep its state scalar
+    const int grp = threadIdx.x / NSB, li = threadIdx.x % NSB;
+    const int b0 = blockIdx.x * GPW + grp;
     const bool active = b0 < nblk;
     const int b = active ? b0 : nblk - 1;
     const size_t boff = (size_t)frame * nblk + b;
     const int lx = (b % nbx) * BW, ly = (b / nbx) * BH;
-    const int bi = li >> 2, r = li & 3;
-    const int bxi = bi % (BW / 4), byi = bi / (BW / 4);
+    const int sbx = 4 * (li % (BW / 4)), sby = 4 * (li / (BW / 4));
 
     RefineGeom G;
     G.pw = p.cw + 2 * p.pad;
@@ -795,8 +841,8 @@ __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
     G.pad = p.pad;
     G.n = (size_t)G.pw * G.ph;
     G.pl = planes + (size_t)frame * planes_stride;
-    const uint32_t srow =
-        *(const uint32_t *)(cur + (size_t)frame * cur_stride + (size_t)(ly + 4 * byi + r) * p.cw + lx + 4 * bxi);
+    uint32_t s[4];
+    load_src4<BW, BH>(cur + (size_t)frame * cur_stride + (size_t)(ly + sby) * p.cw + lx + sbx, p.cw, s);
     const int mpx = mvps ? mvps[2 * boff] : 0, mpy = mvps ? mvps[2 * boff + 1] : 0;
 
     int bx, by;
@@ -817,15 +863,15 @@ __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
         by = start_mv[2 * boff + 1];
     }
 
-    uint8_t *win = s_win + grp * W::BYTES;
+    uint8_t *win = s_win + (USE_LDS ? grp * W::BYTES : 0);
     const int sxi = bx >> 2, syi = by >> 2;
-    stage_window<BW, BH>(win, G, lx + sxi + G.pad - 4, ly + syi + G.pad - 3, li, LPB);
+    if (USE_LDS) stage_window<BW, BH>(win, G, lx + sxi + G.pad - 4, ly + syi + G.pad - 3, li, NSB);
     __syncthreads();
     if (!active) return; // whole groups exit together
 
     int best, evals;
-    refine_walk<BW, BH, LPB>(G, win, srow, lx, ly, bxi, byi, r, sxi, syi, bx, by, mpx, mpy, p.subpel_pat, s_lr, best,
-                             evals);
+    refine_walk<BW, BH, USE_LDS>(G, win, s, lx, ly, sbx, sby, sxi, syi, bx, by, mpx, mpy, p.subpel_pat, s_lr, best,
+                                 evals);
     if (li == 0) {
         out_mv[2 * boff] = bx;
         out_mv[2 * boff + 1] = by;
@@ -939,47 +985,45 @@ void k_fs_multi(const SegDesc *__restrict__ segs, int nseg, long total_lanes, in
 
 // Refinement of every block of every segment from its integer result; writes
 // {mvx, mvy, cost} per block into res.
-template <int BW, int BH>
+template <int BW, int BH, bool USE_LDS>
 __global__ __launch_bounds__(256) void k_refine_multi(const SegDesc *__restrict__ segs, int nseg, long total_blocks,
                                                       int R, int pad, int pat, LrTab lr,
                                                       const uint64_t *__restrict__ keys, int32_t *__restrict__ res)
 {
-    constexpr int LPB = BW * BH / 4;
-    constexpr int GPB = 256 / LPB;
+    constexpr int NSB = RefShape<BW, BH>::NSB, GPW = RefShape<BW, BH>::GPW;
     using W = RefineWin<BW, BH>;
     __shared__ int32_t s_lr[RH264_LR_N];
-    __shared__ __align__(16) uint8_t s_win[GPB * W::BYTES];
+    __shared__ __align__(16) uint8_t s_win[USE_LDS ? GPW * W::BYTES : 16];
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = lr.v[threadIdx.x];
-    const int grp = threadIdx.x / LPB, li = threadIdx.x % LPB;
-    long gb0 = (long)blockIdx.x * GPB + grp;
-    if (LPB == 64) gb0 = __builtin_amdgcn_readfirstlane(gb0);
+    const int grp = threadIdx.x / NSB, li = threadIdx.x % NSB;
+    const long gb0 = (long)blockIdx.x * GPW + grp;
     const bool active = gb0 < total_blocks;
     const long gb = active ? gb0 : total_blocks - 1;
     const int si = seg_find_blk(segs, nseg, gb);
     const SegDesc S = segs[si];
     const int b = (int)(gb - S.blk0);
     const int lx = (b % S.nbx) * BW, ly = (b / S.nbx) * BH;
-    const int bi = li >> 2, r = li & 3;
-    const int bxi = bi % (BW / 4), byi = bi / (BW / 4);
+    const int sbx = 4 * (li % (BW / 4)), sby = 4 * (li / (BW / 4));
     RefineGeom G;
     G.pw = S.cw + 2 * pad;
     G.ph = S.ch + 2 * pad;
     G.pad = pad;
     G.n = (size_t)G.pw * G.ph;
     G.pl = S.planes;
-    const uint32_t srow = *(const uint32_t *)(S.cur + (size_t)(ly + 4 * byi + r) * S.cw + lx + 4 * bxi);
+    uint32_t s[4];
+    load_src4<BW, BH>(S.cur + (size_t)(ly + sby) * S.cw + lx + sbx, S.cw, s);
     const int mpx = S.mvps ? S.mvps[2 * b] : 0, mpy = S.mvps ? S.mvps[2 * b + 1] : 0;
     const uint64_t key = keys[gb];
     const int NC = 2 * R + 1;
     const uint32_t idx = (uint32_t)key;
     int bx = 4 * ((int)(idx % (uint32_t)NC) - R), by = 4 * ((int)(idx / (uint32_t)NC) - R);
-    uint8_t *win = s_win + grp * W::BYTES;
+    uint8_t *win = s_win + (USE_LDS ? grp * W::BYTES : 0);
     const int sxi = bx >> 2, syi = by >> 2;
-    stage_window<BW, BH>(win, G, lx + sxi + pad - 4, ly + syi + pad - 3, li, LPB);
+    if (USE_LDS) stage_window<BW, BH>(win, G, lx + sxi + pad - 4, ly + syi + pad - 3, li, NSB);
     __syncthreads();
     if (!active) return;
     int best, evals;
-    refine_walk<BW, BH, LPB>(G, win, srow, lx, ly, bxi, byi, r, sxi, syi, bx, by, mpx, mpy, pat, s_lr, best, evals);
+    refine_walk<BW, BH, USE_LDS>(G, win, s, lx, ly, sbx, sby, sxi, syi, bx, by, mpx, mpy, pat, s_lr, best, evals);
     if (li == 0) {
         res[3 * gb] = bx;
         res[3 * gb + 1] = by;
@@ -1144,13 +1188,6 @@ __device__ __forceinline__ uint32_t mbt_row_clamped(const uint8_t *plane, int w,
     return v;
 }
 
-__device__ __forceinline__ void diff4(uint32_t s, uint32_t p, int &d0, int &d1, int &d2, int &d3)
-{
-    d0 = (int)(s & 255) - (int)(p & 255);
-    d1 = (int)((s >> 8) & 255) - (int)((p >> 8) & 255);
-    d2 = (int)((s >> 16) & 255) - (int)((p >> 16) & 255);
-    d3 = (int)(s >> 24) - (int)(p >> 24);
-}
 
 // intra_cost, mbtree.rs:78-93, 8x8 blocks of the half-res plane: sum over the four
 // 4x4 Hadamards of |coeff| without the DC coefficient, clamped to >= 1.  Frame 0
@@ -1377,6 +1414,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->tmp_mv_cap = 0;
     c->prof_cap = c->prof_n = 0;
     c->fs_kernel = 0;
+    c->refine_lds = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -1557,11 +1595,16 @@ static int launch_refine(rh264_ctx *ctx, const rh264_search_params *p, int nfram
                          const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv,
                          int32_t *cost, int32_t *evals)
 {
-    constexpr int LPB = BW * BH / 4, GPB = 256 / LPB;
+    constexpr int GPW = RefShape<BW, BH>::GPW;
     const int nblk = (p->cw / BW) * (p->ch / BH);
-    dim3 grid((unsigned)((nblk + GPB - 1) / GPB), nframes);
-    hipLaunchKernelGGL((k_subpel_refine<BW, BH>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes,
-                       planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost, evals);
+    dim3 grid((unsigned)((nblk + GPW - 1) / GPW), nframes);
+    if (RefShape<BW, BH>::NSB >= 8 && ctx->refine_lds)
+        hipLaunchKernelGGL((k_subpel_refine<BW, BH, RefShape<BW, BH>::NSB >= 8>), grid, dim3(256), 0, ctx->stream, *p,
+                           cur, cur_stride, planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv,
+                           cost, evals);
+    else
+        hipLaunchKernelGGL((k_subpel_refine<BW, BH, false>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
+                           planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost, evals);
     LAUNCH_CHECK("k_subpel_refine");
     return 0;
 }
@@ -1686,6 +1729,10 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
     if (!strcmp(knob, "fs_kernel")) {
         if (value < 0 || value > 2) return fail("rh264_tune: fs_kernel %d", value);
         ctx->fs_kernel = value;
+        return 0;
+    }
+    if (!strcmp(knob, "refine_lds")) {
+        ctx->refine_lds = value ? 1 : 0;
         return 0;
     }
     return fail("rh264_tune: unknown knob '%s'", knob);
@@ -2010,9 +2057,14 @@ static int batch_shape_run(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S, 
     else launch_fs_multi_v<BW, BH, 0>(ctx, b, S);
     LAUNCH_CHECK("k_fs_multi");
     if (search_only) return 0;
-    constexpr int GPB = 256 / (BW * BH / 4);
-    hipLaunchKernelGGL((k_refine_multi<BW, BH>), dim3((unsigned)((S.blocks + GPB - 1) / GPB)), dim3(256), 0,
-                       ctx->stream, S.d_segs, S.nseg, S.blocks, b->range, b->pad, b->pat, b->lr, b->keys, b->res);
+    constexpr int GPW = RefShape<BW, BH>::GPW;
+    if (RefShape<BW, BH>::NSB >= 8 && ctx->refine_lds)
+        hipLaunchKernelGGL((k_refine_multi<BW, BH, RefShape<BW, BH>::NSB >= 8>),
+                           dim3((unsigned)((S.blocks + GPW - 1) / GPW)), dim3(256), 0, ctx->stream, S.d_segs, S.nseg,
+                           S.blocks, b->range, b->pad, b->pat, b->lr, b->keys, b->res);
+    else
+        hipLaunchKernelGGL((k_refine_multi<BW, BH, false>), dim3((unsigned)((S.blocks + GPW - 1) / GPW)), dim3(256), 0,
+                           ctx->stream, S.d_segs, S.nseg, S.blocks, b->range, b->pad, b->pat, b->lr, b->keys, b->res);
     LAUNCH_CHECK("k_refine_multi");
     hipLaunchKernelGGL(k_select_ref, dim3((unsigned)((S.sel_blocks + 255) / 256)), dim3(256), 0, ctx->stream, S.d_sels,
                        S.nsel, S.sel_blocks, b->res);
