@@ -207,6 +207,42 @@ int rh264_batch_capture(rh264_ctx *ctx, rh264_batch *b);
 int rh264_batch_replay(rh264_ctx *ctx, rh264_batch *b);
 int rh264_batch_destroy(rh264_batch *b);
 
+/* ---- learned motion-vector proposal (no reference counterpart) -------------
+ * A small BF16 CNN (rusty_h264_amd/csrc/rh264_mvnet.hip: 1/4-res current+reference
+ * luma, conv3x3/2 2->32, 3 x conv3x3 32->32 on MFMA, 2x2 pool, 1x1 head to 289
+ * MV bins of 4 pels) proposes 8 integer-MV centres per 16x16 MB.  Parameters are
+ * FP32 (rh264_mvnet_param_count of them, layout in that file).
+ * rh264_mvnet_forward: logits [n][mb][289] (device, may be NULL) and/or top8
+ *   [n][mb][8] bin indices (device, may be NULL) for frames cur/ref (stride apart).
+ * rh264_mvnet_backward: forward + backward of the mean cross-entropy against
+ *   labels [n][mb] (bin indices); gradients stay on the device (get_grads), the
+ *   loss is returned on the host.  rh264_mvnet_adam: one Adam step. */
+typedef struct rh264_mvnet rh264_mvnet;
+size_t rh264_mvnet_param_count(void);
+int rh264_mvnet_create(rh264_ctx *ctx, int max_n, int max_w, int max_h, rh264_mvnet **out);
+int rh264_mvnet_destroy(rh264_mvnet *net);
+int rh264_mvnet_set_params(rh264_ctx *ctx, rh264_mvnet *net, const float *host, size_t count);
+int rh264_mvnet_get_params(rh264_ctx *ctx, rh264_mvnet *net, float *host, size_t count);
+int rh264_mvnet_get_grads(rh264_ctx *ctx, rh264_mvnet *net, float *host, size_t count);
+int rh264_mvnet_forward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref,
+                        size_t stride, int n, int w, int h, float *logits, int32_t *top8);
+int rh264_mvnet_backward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref,
+                         size_t stride, int n, int w, int h, const int32_t *labels, float *loss_out);
+int rh264_mvnet_adam(rh264_ctx *ctx, rh264_mvnet *net, float lr);
+
+/* Learned-proposal search of every 16x16 block of nframes frames (frame i: cur +
+ * i*frame_stride against ref + i*frame_stride, plane set planes + i*planes_stride):
+ * the network's 8 centres are re-costed exactly (the integer search above with
+ * radius local_range around each centre); the cheapest (first minimum in rank
+ * order) starts the sub-pel ring walk (p->subpel_pat); a block whose best integer
+ * cost exceeds fallback_cost is searched exhaustively with radius p->range
+ * instead (fallback[i] = 1).  Costs, rate tables and tie-breaks are the plain
+ * search's, so every evaluated candidate is priced exactly. */
+int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvnet *net, int nframes,
+                         const uint8_t *cur, const uint8_t *ref, size_t frame_stride,
+                         const uint8_t *planes, size_t planes_stride, int local_range,
+                         int32_t fallback_cost, int32_t *mv, int32_t *cost, int32_t *fallback);
+
 /* ---- mb-tree lookahead (crates/rusty_h264-encoder/src/mbtree.rs, HalfRes,
  *      the default LookaheadMode, config.rs:568) ------------------------------
  * rh264_mbtree_downsample: downsample2x (mbtree.rs:61-74) of nframes cw x ch

@@ -76,6 +76,18 @@ PROTOTYPES = {
     "rh264_batch_capture": (C.c_int, [c_vp, c_vp]),
     "rh264_batch_replay": (C.c_int, [c_vp, c_vp]),
     "rh264_batch_destroy": (C.c_int, [c_vp]),
+    "rh264_mvnet_param_count": (C.c_size_t, []),
+    "rh264_mvnet_create": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.POINTER(c_vp)]),
+    "rh264_mvnet_destroy": (C.c_int, [c_vp]),
+    "rh264_mvnet_set_params": (C.c_int, [c_vp, c_vp, C.POINTER(C.c_float), C.c_size_t]),
+    "rh264_mvnet_get_params": (C.c_int, [c_vp, c_vp, C.POINTER(C.c_float), C.c_size_t]),
+    "rh264_mvnet_get_grads": (C.c_int, [c_vp, c_vp, C.POINTER(C.c_float), C.c_size_t]),
+    "rh264_mvnet_forward": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_size_t, C.c_int, C.c_int, C.c_int, c_vp, c_vp]),
+    "rh264_mvnet_backward": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_size_t, C.c_int, C.c_int, C.c_int, c_vp,
+                                       C.POINTER(C.c_float)]),
+    "rh264_mvnet_adam": (C.c_int, [c_vp, c_vp, C.c_float]),
+    "rh264_learned_search": (C.c_int, [c_vp, C.POINTER(SearchParams), c_vp, C.c_int, c_vp, c_vp, C.c_size_t, c_vp,
+                                       C.c_size_t, C.c_int, C.c_int32, c_vp, c_vp, c_vp]),
     "rh264_predict_blocks": (C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        c_vp, c_vp, c_vp]),
 }

@@ -4,13 +4,13 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "rh264_me.hip")
+SRCS = [os.path.join(HERE, "csrc", "rh264_me.hip"), os.path.join(HERE, "csrc", "rh264_mvnet.hip")]
 OUT = os.path.join(HERE, "librh264_me.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    "-I" + os.path.join(ROOT, "include"),
+    "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "csrc"),
     # no packed-FP32 instructions anywhere in the build
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
@@ -18,7 +18,7 @@ FLAGS = [
 
 def build_library(verbose=False):
     tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp, SRC]
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + SRCS
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

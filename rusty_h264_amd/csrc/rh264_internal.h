@@ -1,0 +1,41 @@
+// rh264_internal.h -- declarations shared by the provider's HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "rh264_me.h"
+
+int rh264_fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+#define fail rh264_fail
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail("%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+#define LAUNCH_CHECK(name)                                                                    \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess) return fail("launch of %s failed: %s", name, hipGetErrorString(e_)); \
+    } while (0)
+
+struct rh264_ctx {
+    int device;
+    hipStream_t stream;
+    hipEvent_t ev[16];
+    uint64_t *keys;               // per-block argmin keys of the full search
+    size_t keys_cap;              // in elements
+    int32_t *tmp_mv;              // integer-stage MVs when the caller does not want them
+    size_t tmp_mv_cap;            // in int32 elements
+    int fs_kernel;                // integer search: 0 = quad-SAD, 1 = v_sad_u8 x2, 2 = v_sad_u8 x4
+    int refine_lds;               // refinement stages the +-3 pel window in LDS (blocks of >= 8 sub-blocks)
+    std::vector<hipEvent_t> prof; // 3 events per profiled plain search
+    int prof_cap, prof_n;         // capacity and count of profiled searches
+    void *ls_buf;                 // scratch of the learned-proposal search
+    size_t ls_cap;
+};

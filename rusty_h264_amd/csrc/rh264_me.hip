@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "rh264_me.h"
+#include "rh264_internal.h"
 
 // ---------------------------------------------------------------------------
 // error plumbing
@@ -34,8 +35,7 @@
 
 static thread_local std::string g_err;
 
-static int fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
-static int fail(const char *fmt, ...)
+int rh264_fail(const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -45,32 +45,6 @@ static int fail(const char *fmt, ...)
     g_err = buf;
     return -1;
 }
-
-#define HIP_TRY(expr)                                                                         \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess) return fail("%s failed: %s", #expr, hipGetErrorString(e_));     \
-    } while (0)
-
-#define LAUNCH_CHECK(name)                                                                    \
-    do {                                                                                      \
-        hipError_t e_ = hipGetLastError();                                                    \
-        if (e_ != hipSuccess) return fail("launch of %s failed: %s", name, hipGetErrorString(e_)); \
-    } while (0)
-
-struct rh264_ctx {
-    int device;
-    hipStream_t stream;
-    hipEvent_t ev[16];
-    uint64_t *keys;      // per-block argmin keys of the full search
-    size_t keys_cap;     // in elements
-    int32_t *tmp_mv;     // integer-stage MVs when the caller does not want them
-    size_t tmp_mv_cap;   // in int32 elements
-    int fs_kernel;                // integer search: 0 = quad-SAD, 1 = v_sad_u8 x2, 2 = v_sad_u8 x4
-    int refine_lds;               // refinement stages the +-3 pel window in LDS (blocks of >= 8 sub-blocks)
-    std::vector<hipEvent_t> prof; // 3 events per profiled plain search
-    int prof_cap, prof_n;         // capacity and count of profiled searches
-};
 
 static int ensure_keys(rh264_ctx *ctx, size_t n)
 {
@@ -1391,6 +1365,106 @@ __global__ void k_mbtree_prop_i32(int f, int mb_w, int mb_h, const int32_t *__re
 }
 
 // ---------------------------------------------------------------------------
+// learned-proposal search (the network lives in rh264_mvnet.hip)
+// ---------------------------------------------------------------------------
+
+constexpr int LS_NCAND = 8;
+constexpr int LS_BINS = 17;
+
+// Candidate centres from the network's top bins: bin c -> ((c%17)-8, (c/17)-8) * 4
+// pels, as quarter-pel MVs.  centres: [k][block][2].
+__global__ void k_cand_centres(const int32_t *__restrict__ top8, long nb, int32_t *__restrict__ centres)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb * LS_NCAND) return;
+    const long b = i / LS_NCAND;
+    const int k = (int)(i - b * LS_NCAND);
+    const int c = top8[i];
+    centres[2 * ((size_t)k * nb + b)] = ((c % LS_BINS) - LS_BINS / 2) * 16;
+    centres[2 * ((size_t)k * nb + b) + 1] = ((c / LS_BINS) - LS_BINS / 2) * 16;
+}
+
+// The exact costs of the candidates' local searches decide the start vector (first
+// minimum in candidate order); blocks whose best cost exceeds fb_cost join the
+// fall-back list for a full search.
+__global__ void k_cand_select(const uint64_t *__restrict__ keys8, const int32_t *__restrict__ centres, long nb, int Rl,
+                              int32_t fb_cost, int32_t *__restrict__ start, int32_t *__restrict__ flag,
+                              unsigned *__restrict__ count, int32_t *__restrict__ list)
+{
+    const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    uint64_t best = ~0ull;
+    int bk = 0;
+    for (int k = 0; k < LS_NCAND; k++) {
+        const uint64_t key = keys8[(size_t)k * nb + b];
+        if ((key >> 32) < (best >> 32)) {
+            best = key;
+            bk = k;
+        }
+    }
+    const int NC = 2 * Rl + 1;
+    const uint32_t idx = (uint32_t)best;
+    const int32_t *c = centres + 2 * ((size_t)bk * nb + b);
+    start[2 * b] = c[0] + 4 * ((int)(idx % (uint32_t)NC) - Rl);
+    start[2 * b + 1] = c[1] + 4 * ((int)(idx / (uint32_t)NC) - Rl);
+    const bool fb = (int64_t)(best >> 32) > (int64_t)fb_cost;
+    flag[b] = fb ? 1 : 0;
+    if (fb) list[atomicAdd(count, 1u)] = (int32_t)b;
+}
+
+// Full search of the listed blocks only (frame-major global block indices).
+template <int BW, int BH, int V>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
+void k_full_search_list(rh264_search_params p, const uint8_t *__restrict__ cur, size_t cur_stride,
+                        const uint8_t *__restrict__ planes, size_t planes_stride, const int32_t *__restrict__ list,
+                        const unsigned *__restrict__ count, const int32_t *__restrict__ mvps,
+                        uint64_t *__restrict__ keys)
+{
+    __shared__ int32_t s_lr[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
+    __syncthreads();
+    const int NG = fs_groups<V>(2 * p.range + 1);
+    const long n = (long)*count;
+    const long gi = (long)blockIdx.x * 256 + threadIdx.x;
+    if ((long)blockIdx.x * 256 >= n * NG) return; // the whole workgroup is past the list
+    const long e = gi / NG;
+    const bool active = e < n;
+    const int nbx = p.cw / BW, nblk = nbx * (p.ch / BH);
+    const long gb = list[active ? e : n - 1];
+    const int frame = (int)(gb / nblk), b = (int)(gb - (long)frame * nblk);
+    FsJob J;
+    J.cw = p.cw;
+    J.pw = p.cw + 2 * p.pad;
+    J.ph = p.ch + 2 * p.pad;
+    J.pad = p.pad;
+    J.R = p.range;
+    J.g = (int)(gi - e * NG);
+    J.lx = (b % nbx) * BW;
+    J.ly = (b / nbx) * BH;
+    J.cx = 0;
+    J.cy = 0;
+    J.mpx = mvps ? mvps[2 * gb] : 0;
+    J.mpy = mvps ? mvps[2 * gb + 1] : 0;
+    J.F = planes + (size_t)frame * planes_stride;
+    J.srcp = cur + (size_t)frame * cur_stride + (size_t)J.ly * p.cw + J.lx;
+    const uint64_t key = fs_key<BW, BH, V>(J, s_lr);
+    fs_commit(key, e, active, keys + gb);
+}
+
+// Fall-back blocks take the full search's vector as their start.
+__global__ void k_fallback_merge(const int32_t *__restrict__ list, const unsigned *__restrict__ count,
+                                 const uint64_t *__restrict__ keys, int R, int32_t *__restrict__ start)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)*count) return;
+    const long gb = list[e];
+    const uint32_t idx = (uint32_t)keys[gb];
+    const int NC = 2 * R + 1;
+    start[2 * gb] = 4 * ((int)(idx % (uint32_t)NC) - R);
+    start[2 * gb + 1] = 4 * ((int)(idx / (uint32_t)NC) - R);
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 
@@ -1413,6 +1487,8 @@ int rh264_init(int device, rh264_ctx **out)
     c->tmp_mv = nullptr;
     c->tmp_mv_cap = 0;
     c->prof_cap = c->prof_n = 0;
+    c->ls_buf = nullptr;
+    c->ls_cap = 0;
     c->fs_kernel = 0;
     c->refine_lds = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1432,6 +1508,7 @@ int rh264_destroy(rh264_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->keys) (void)hipFree(ctx->keys);
     if (ctx->tmp_mv) (void)hipFree(ctx->tmp_mv);
+    if (ctx->ls_buf) (void)hipFree(ctx->ls_buf);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
     for (hipEvent_t e : ctx->prof) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -2175,6 +2252,82 @@ int rh264_batch_replay(rh264_ctx *ctx, rh264_batch *b)
     if (!b || !b->has_graph) return fail("batch_replay: no captured graph");
     HIP_TRY(hipGraphLaunch(b->exec, ctx->stream));
     return 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// learned-proposal search: host side
+// ---------------------------------------------------------------------------
+
+struct rh264_mvnet;
+int rh264_mvnet_forward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref, size_t stride,
+                        int n, int w, int h, float *logits, int32_t *top8);
+
+static int ensure_buf(rh264_ctx *ctx, void **buf, size_t *cap, size_t bytes)
+{
+    if (*cap >= bytes) return 0;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (*buf) HIP_TRY(hipFree(*buf));
+    *buf = nullptr;
+    *cap = 0;
+    HIP_TRY(hipMalloc(buf, bytes));
+    *cap = bytes;
+    return 0;
+}
+
+int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvnet *net, int nframes,
+                         const uint8_t *cur, const uint8_t *ref, size_t frame_stride, const uint8_t *planes,
+                         size_t planes_stride, int local_range, int32_t fallback_cost, int32_t *mv, int32_t *cost,
+                         int32_t *fallback)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (p->bw != 16 || p->bh != 16 || local_range < 0 || local_range > 8) return RH264_DECLINED;
+    if (nframes <= 0) return 0;
+    if (!net || !mv || !cost || !fallback) return fail("learned_search: null argument");
+    rc = check_strides(frame_stride, planes_stride);
+    if (rc) return rc;
+    const long nb = (long)(p->cw / 16) * (p->ch / 16) * nframes;
+    // scratch: top8 | centres | start | list | count | keys8 (| keys of the fall-back search)
+    const size_t o_top = 0, o_cen = o_top + (size_t)nb * LS_NCAND * 4, o_start = o_cen + (size_t)nb * LS_NCAND * 8;
+    const size_t o_list = o_start + (size_t)nb * 8, o_count = o_list + (size_t)nb * 4 + 256;
+    const size_t o_keys = (o_count + 256 + 255) & ~(size_t)255, total = o_keys + (size_t)nb * (LS_NCAND + 1) * 8;
+    rc = ensure_buf(ctx, &ctx->ls_buf, &ctx->ls_cap, total);
+    if (rc) return rc;
+    uint8_t *base = (uint8_t *)ctx->ls_buf;
+    int32_t *top8 = (int32_t *)(base + o_top), *centres = (int32_t *)(base + o_cen), *start = (int32_t *)(base + o_start);
+    int32_t *list = (int32_t *)(base + o_list);
+    unsigned *count = (unsigned *)(base + o_count);
+    uint64_t *keys8 = (uint64_t *)(base + o_keys), *keysf = keys8 + (size_t)nb * LS_NCAND;
+
+    rc = rh264_mvnet_forward(ctx, net, cur, ref, frame_stride, nframes, p->cw, p->ch, nullptr, top8);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_cand_centres, dim3((unsigned)((nb * LS_NCAND + 255) / 256)), dim3(256), 0, ctx->stream, top8,
+                       nb, centres);
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb * (LS_NCAND + 1) + 255) / 256)), dim3(256), 0, ctx->stream,
+                       keys8, (size_t)nb * (LS_NCAND + 1), ~0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, count, (size_t)1, 0u);
+    LAUNCH_CHECK("learned_search setup");
+    rh264_search_params q = *p;
+    q.range = local_range;
+    const long nblk = nb / nframes;
+    for (int k = 0; k < LS_NCAND; k++) {
+        rc = launch_full<16, 16>(ctx, &q, nframes, cur, frame_stride, planes, planes_stride,
+                                 centres + 2 * (size_t)k * nb, nullptr, keys8 + (size_t)k * nb);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_cand_select, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keys8, centres,
+                       nb, local_range, fallback_cost, start, fallback, count, list);
+    const long lanes = nb * fs_groups<0>(2 * p->range + 1);
+    hipLaunchKernelGGL((k_full_search_list<16, 16, 0>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
+                       ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,
+                       (const int32_t *)nullptr, keysf);
+    hipLaunchKernelGGL(k_fallback_merge, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, list, count,
+                       keysf, p->range, start);
+    LAUNCH_CHECK("learned_search select");
+    (void)nblk;
+    return launch_refine<16, 16>(ctx, p, nframes, cur, frame_stride, planes, planes_stride, nullptr, nullptr, start,
+                                 nullptr, nullptr, nullptr, mv, cost, nullptr);
 }
 
 } // extern "C"

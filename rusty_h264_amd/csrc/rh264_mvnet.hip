@@ -1,0 +1,585 @@
+// rh264_mvnet.hip -- the learned motion-vector proposal network: a small BF16 CNN
+// whose 3x3 convolutions run on MFMA (v_mfma_f32_16x16x16_bf16, FP32
+// accumulation), with hand-written forward, backward and Adam kernels.
+//
+// Network (no reference counterpart; the reference has no learned component):
+//   input  : current and reference luma at 1/4 resolution ((sum of 4x4) + 8) >> 4,
+//            normalised (v - 128) / 64, 2 channels
+//   L1     : conv3x3 stride 2, 2 -> 32, bias, ReLU          (1/8 resolution)
+//   L2..L4 : conv3x3 stride 1, 32 -> 32, bias, ReLU          (1/8 resolution)
+//   pool   : 2x2 average                                     (one cell per 16x16 MB)
+//   head   : 1x1, 32 -> 289 logits = MV bins of 4 pels, dx, dy in -32..32 pels
+// Activations are NHWC BF16; parameters FP32 masters with BF16 copies.
+//
+// Flat parameter layout (FP32): W1[32][9][2], b1[32], W2[32][9][32], b2[32],
+// W3[32][9][32], b3[32], W4[32][9][32], b4[32], Wh[289][32], bh[289].
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "rh264_internal.h"
+
+namespace {
+
+constexpr int C = 32;     // channels
+constexpr int K = 289;    // MV bins (17 x 17)
+constexpr int NB = 17;    // bins per axis
+constexpr int TOPK = 8;
+
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + C * 9 * 2;
+constexpr int OFF_W2 = OFF_B1 + C, OFF_B2 = OFF_W2 + C * 9 * C;
+constexpr int OFF_W3 = OFF_B2 + C, OFF_B3 = OFF_W3 + C * 9 * C;
+constexpr int OFF_W4 = OFF_B3 + C, OFF_B4 = OFF_W4 + C * 9 * C;
+constexpr int OFF_WH = OFF_B4 + C, OFF_BH = OFF_WH + K * C;
+constexpr int NPARAM = OFF_BH + K;
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u); // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------------------
+// input preparation: 1/4-resolution current + reference, 2 channels
+// ---------------------------------------------------------------------------
+__global__ void k_prep(const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, size_t stride, int n, int w,
+                       int h, uint16_t *__restrict__ x0)
+{
+    const int w4 = w / 4, h4 = h / 4;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * w4 * h4) return;
+    const int img = (int)(i / ((long)w4 * h4));
+    const int rem = (int)(i - (long)img * w4 * h4);
+    const int y = rem / w4, x = rem - y * w4;
+    const uint8_t *pc = cur + (size_t)img * stride + (size_t)(4 * y) * w + 4 * x;
+    const uint8_t *pr = ref + (size_t)img * stride + (size_t)(4 * y) * w + 4 * x;
+    uint32_t sc = 0, sr = 0;
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            sc += pc[r * w + c];
+            sr += pr[r * w + c];
+        }
+    x0[2 * i] = f2bf(((float)((sc + 8) >> 4) - 128.0f) / 64.0f);
+    x0[2 * i + 1] = f2bf(((float)((sr + 8) >> 4) - 128.0f) / 64.0f);
+}
+
+// ---------------------------------------------------------------------------
+// L1: conv3x3 stride 2, 2 -> 32 (VALU; 18 MACs per output channel)
+// ---------------------------------------------------------------------------
+__global__ void k_conv1_fwd(const uint16_t *__restrict__ x0, const float *__restrict__ p, int n, int h4, int w4,
+                            uint16_t *__restrict__ a1)
+{
+    __shared__ float sw[C * 18 + C];
+    for (int i = threadIdx.x; i < C * 18 + C; i += blockDim.x) sw[i] = p[OFF_W1 + i];
+    __syncthreads();
+    const int h8 = h4 / 2, w8 = w4 / 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * h8 * w8) return;
+    const int img = (int)(i / ((long)h8 * w8));
+    const int rem = (int)(i - (long)img * h8 * w8);
+    const int y = rem / w8, x = rem - y * w8;
+    float in[18];
+    for (int t = 0; t < 9; t++) {
+        const int yy = 2 * y + t / 3 - 1, xx = 2 * x + t % 3 - 1;
+        const bool ok = yy >= 0 && yy < h4 && xx >= 0 && xx < w4;
+        const size_t o = (((size_t)img * h4 + (ok ? yy : 0)) * w4 + (ok ? xx : 0)) * 2;
+        in[2 * t] = ok ? bf2f(x0[o]) : 0.0f;
+        in[2 * t + 1] = ok ? bf2f(x0[o + 1]) : 0.0f;
+    }
+    uint16_t *o = a1 + (size_t)i * C;
+    for (int co = 0; co < C; co++) {
+        float acc = sw[C * 18 + co];
+        for (int k = 0; k < 18; k++) acc += sw[co * 18 + k] * in[k];
+        o[co] = f2bf(acc > 0.0f ? acc : 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// conv3x3 stride 1, 32 -> 32 on MFMA.  out[p][co] = act(sum_{tap,ci} w[tap][co][ci] *
+// in[p + tap][ci] + bias[co]); one wave makes 16 consecutive pixels of a row x 32
+// output channels: D(co x pixel) = W(co x k) * X(k x pixel), k = (tap, ci), 18
+// K-steps of 16.  Operand layout of v_mfma_f32_16x16x16_bf16: lane l supplies
+// A[l%16][4(l/16)+i], B[4(l/16)+i][l%16] and receives D[4(l/16)+i][l%16].
+// relu: clamp at 0; mask (may be null): zero where mask[p][co] == 0 (the ReLU
+// derivative of the layer that produced mask).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv3_mfma(const uint16_t *__restrict__ in, const uint16_t *__restrict__ wt,
+                                                    const float *__restrict__ bias, int relu,
+                                                    const uint16_t *__restrict__ mask, int n, int H, int W,
+                                                    uint16_t *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int wtiles = (W + 15) / 16;
+    const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave >= (long)n * H * wtiles) return;
+    const int img = (int)(wave / ((long)H * wtiles));
+    const int rem = (int)(wave - (long)img * H * wtiles);
+    const int y = rem / wtiles, x0 = (rem - y * wtiles) * 16;
+    const int col = lane & 15, kq = lane >> 4; // kq: which 4 of the 16 k of a step
+    float4_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+        const int yy = y + tap / 3 - 1, xx = x0 + col + tap % 3 - 1;
+        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const uint16_t *ip = in + (((size_t)img * H + (ok ? yy : 0)) * W + (ok ? xx : 0)) * C;
+#pragma unroll
+        for (int half = 0; half < 2; half++) {
+            const int ci = half * 16 + 4 * kq;
+            short4_t b = ok ? *(const short4_t *)(ip + ci) : short4_t{0, 0, 0, 0};
+            short4_t a0 = *(const short4_t *)(wt + ((size_t)tap * C + col) * C + ci);
+            short4_t a1 = *(const short4_t *)(wt + ((size_t)tap * C + col + 16) * C + ci);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b, acc1, 0, 0, 0);
+        }
+    }
+    const int px = x0 + col;
+    if (px >= W) return;
+    const size_t po = (((size_t)img * H + y) * W + px) * C;
+    uint16_t r0[4], r1[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int co = 4 * kq + i;
+        float v0 = acc0[i] + (bias ? bias[co] : 0.0f), v1 = acc1[i] + (bias ? bias[co + 16] : 0.0f);
+        if (relu) {
+            v0 = v0 > 0.0f ? v0 : 0.0f;
+            v1 = v1 > 0.0f ? v1 : 0.0f;
+        }
+        if (mask) {
+            if (mask[po + co] == 0) v0 = 0.0f;
+            if (mask[po + co + 16] == 0) v1 = 0.0f;
+        }
+        r0[i] = f2bf(v0);
+        r1[i] = f2bf(v1);
+    }
+    *(short4_t *)(out + po + 4 * kq) = short4_t{(short)r0[0], (short)r0[1], (short)r0[2], (short)r0[3]};
+    *(short4_t *)(out + po + 16 + 4 * kq) = short4_t{(short)r1[0], (short)r1[1], (short)r1[2], (short)r1[3]};
+}
+
+// 2x2 average pool (fp32 out) and its backward (grad / 4, masked by the ReLU of a4).
+__global__ void k_pool_fwd(const uint16_t *__restrict__ a, int n, int H, int W, float *__restrict__ f)
+{
+    const int Hm = H / 2, Wm = W / 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * Hm * Wm * C) return;
+    const int c = (int)(i % C);
+    const long cell = i / C;
+    const int img = (int)(cell / ((long)Hm * Wm));
+    const int rem = (int)(cell - (long)img * Hm * Wm);
+    const int y = rem / Wm, x = rem - y * Wm;
+    const size_t b = (((size_t)img * H + 2 * y) * W + 2 * x) * C + c;
+    f[i] = 0.25f * (bf2f(a[b]) + bf2f(a[b + C]) + bf2f(a[b + (size_t)W * C]) + bf2f(a[b + (size_t)W * C + C]));
+}
+
+__global__ void k_pool_bwd(const float *__restrict__ df, const uint16_t *__restrict__ a, int n, int H, int W,
+                           uint16_t *__restrict__ dz)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * H * W * C) return;
+    const int c = (int)(i % C);
+    const long px = i / C;
+    const int img = (int)(px / ((long)H * W));
+    const int rem = (int)(px - (long)img * H * W);
+    const int y = rem / W, x = rem - y * W;
+    const int Hm = H / 2, Wm = W / 2;
+    float g = 0.0f;
+    if (y / 2 < Hm && x / 2 < Wm) g = 0.25f * df[(((size_t)img * Hm + y / 2) * Wm + x / 2) * C + c];
+    dz[i] = f2bf(a[i] != 0 ? g : 0.0f);
+}
+
+// head: logits[cell][k] = bh[k] + sum_c Wh[k][c] f[cell][c]
+__global__ void k_head_fwd(const float *__restrict__ f, const float *__restrict__ p, long ncell,
+                           float *__restrict__ logits)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell * K) return;
+    const long cell = i / K;
+    const int k = (int)(i - cell * K);
+    const float *w = p + OFF_WH + (size_t)k * C;
+    const float *fc = f + (size_t)cell * C;
+    float acc = p[OFF_BH + k];
+    for (int c = 0; c < C; c++) acc += w[c] * fc[c];
+    logits[i] = acc;
+}
+
+// softmax cross-entropy per cell (one wave per cell): dlogits = (softmax - onehot)
+// * scale, loss += -log softmax[label] * scale.
+__global__ void k_softmax_ce(const float *__restrict__ logits, const int32_t *__restrict__ labels, long ncell,
+                             float scale, float *__restrict__ dlogits, float *__restrict__ loss)
+{
+    const long cell = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (cell >= ncell) return;
+    const float *z = logits + (size_t)cell * K;
+    float m = -1e30f;
+    for (int k = lane; k < K; k += 64) m = fmaxf(m, z[k]);
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    float se = 0.0f;
+    for (int k = lane; k < K; k += 64) se += expf(z[k] - m);
+    for (int off = 32; off > 0; off >>= 1) se += __shfl_xor(se, off, 64);
+    const int lab = labels[cell];
+    for (int k = lane; k < K; k += 64) {
+        const float pk = expf(z[k] - m) / se;
+        dlogits[(size_t)cell * K + k] = (pk - (k == lab ? 1.0f : 0.0f)) * scale;
+    }
+    if (lane == 0) atomicAdd(loss, (logf(se) + m - z[lab]) * scale);
+}
+
+// head backward: df[cell][c] = sum_k Wh[k][c] dlogits[cell][k]
+__global__ void k_head_bwd_data(const float *__restrict__ dl, const float *__restrict__ p, long ncell,
+                                float *__restrict__ df)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell * C) return;
+    const long cell = i / C;
+    const int c = (int)(i - cell * C);
+    float acc = 0.0f;
+    for (int k = 0; k < K; k++) acc += p[OFF_WH + (size_t)k * C + c] * dl[(size_t)cell * K + k];
+    df[i] = acc;
+}
+
+// head weight grads: one thread per (k, c) or bias k, summing over a chunk of cells
+__global__ void k_head_bwd_w(const float *__restrict__ dl, const float *__restrict__ f, long ncell,
+                             float *__restrict__ g)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x; // 0 .. K*C + K - 1
+    if (j >= K * C + K) return;
+    const long c0 = (long)blockIdx.y * 256, c1 = c0 + 256 < ncell ? c0 + 256 : ncell;
+    float acc = 0.0f;
+    if (j < K * C) {
+        const int k = j / C, c = j - k * C;
+        for (long cell = c0; cell < c1; cell++) acc += dl[(size_t)cell * K + k] * f[(size_t)cell * C + c];
+    } else {
+        const int k = j - K * C;
+        for (long cell = c0; cell < c1; cell++) acc += dl[(size_t)cell * K + k];
+    }
+    atomicAdd(&g[OFF_WH + j], acc);
+}
+
+// 3x3 weight gradients (VALU, FP32): dw[co][tap][ci] += sum_p dz[p][co] * x[in(p, tap)][ci],
+// db[co] += sum_p dz[p][co].  One block per chunk of output pixels stages dz and the
+// nine input taps in LDS (32 pixels per block), then each thread owns weights
+// j = tid, tid + 256, ...
+template <int CIN, int STRIDE>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dz,
+                                                    int n, int Hin, int Win, int Hout, int Wout,
+                                                    float *__restrict__ gw, float *__restrict__ gb)
+{
+    constexpr int P = 32;
+    __shared__ float s_dz[P][C];
+    __shared__ float s_x[P][9][CIN];
+    const long npx = (long)n * Hout * Wout;
+    const long p0 = (long)blockIdx.x * P;
+    for (int i = threadIdx.x; i < P * C; i += 256) {
+        const int pp = i / C, c = i - pp * C;
+        const long pi = p0 + pp;
+        s_dz[pp][c] = pi < npx ? bf2f(dz[(size_t)pi * C + c]) : 0.0f;
+    }
+    for (int i = threadIdx.x; i < P * 9 * CIN; i += 256) {
+        const int pp = i / (9 * CIN), rem = i - pp * 9 * CIN, t = rem / CIN, c = rem - t * CIN;
+        const long pi = p0 + pp;
+        float v = 0.0f;
+        if (pi < npx) {
+            const int img = (int)(pi / ((long)Hout * Wout));
+            const int r2 = (int)(pi - (long)img * Hout * Wout);
+            const int yo = r2 / Wout, xo = r2 - yo * Wout;
+            const int yy = yo * STRIDE + t / 3 - 1, xx = xo * STRIDE + t % 3 - 1;
+            if (yy >= 0 && yy < Hin && xx >= 0 && xx < Win) v = bf2f(x[(((size_t)img * Hin + yy) * Win + xx) * CIN + c]);
+        }
+        s_x[pp][t][c] = v;
+    }
+    __syncthreads();
+    const int NW = C * 9 * CIN;
+    const int woff = CIN == 2 ? OFF_W1 : 0; // caller offsets gw for the 32-channel layers
+    for (int j = threadIdx.x; j < NW + C; j += 256) {
+        float acc = 0.0f;
+        if (j < NW) {
+            const int co = j / (9 * CIN), rem = j - co * 9 * CIN, t = rem / CIN, c = rem - t * CIN;
+            for (int pp = 0; pp < P; pp++) acc += s_dz[pp][co] * s_x[pp][t][c];
+            atomicAdd(&gw[woff + j], acc);
+        } else {
+            const int co = j - NW;
+            for (int pp = 0; pp < P; pp++) acc += s_dz[pp][co];
+            atomicAdd(&gb[co], acc);
+        }
+    }
+}
+
+// Adam step over all parameters, then the BF16 copies: forward [tap][co][ci] and,
+// for the data gradient, the flipped transpose [8-tap][ci][co].
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPARAM) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void k_refresh_bf16(const float *__restrict__ p, uint16_t *__restrict__ wf, uint16_t *__restrict__ wb)
+{
+    // 3 layers x [32 co][9 tap][32 ci] -> wf[layer][tap][co][ci], wb[layer][8-tap][ci][co]
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * C * 9 * C) return;
+    const int l = i / (C * 9 * C), j = i - l * C * 9 * C;
+    const int co = j / (9 * C), rem = j - co * 9 * C, t = rem / C, ci = rem - t * C;
+    const float w = p[OFF_W2 + l * (C * 9 * C + C) + j];
+    const uint16_t b = f2bf(w);
+    wf[(size_t)l * 9 * C * C + ((size_t)t * C + co) * C + ci] = b;
+    wb[(size_t)l * 9 * C * C + ((size_t)(8 - t) * C + ci) * C + co] = b;
+}
+
+// top-8 bins per cell by repeated argmax (ties: lowest bin), one wave per cell.
+__global__ void k_topk(const float *__restrict__ logits, long ncell, int32_t *__restrict__ top)
+{
+    const long cell = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (cell >= ncell) return;
+    const float *z = logits + (size_t)cell * K;
+    float v[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        const int k = lane + 64 * j;
+        v[j] = k < K ? z[k] : -INFINITY;
+    }
+    for (int r = 0; r < TOPK; r++) {
+        float bv = -INFINITY;
+        int bk = 0x7fffffff;
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            const int k = lane + 64 * j;
+            if (k < K && (v[j] > bv || (v[j] == bv && k < bk))) {
+                bv = v[j];
+                bk = k;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const float ov = __shfl_xor(bv, off, 64);
+            const int ok = __shfl_xor(bk, off, 64);
+            if (ov > bv || (ov == bv && ok < bk)) {
+                bv = ov;
+                bk = ok;
+            }
+        }
+        if (lane == 0) top[(size_t)cell * TOPK + r] = bk;
+#pragma unroll
+        for (int j = 0; j < 5; j++)
+            if (lane + 64 * j == bk) v[j] = -INFINITY;
+    }
+}
+
+__global__ void k_fill_f32(float *p, size_t n, float v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline unsigned nblocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+} // namespace
+
+struct rh264_mvnet {
+    int max_n, max_w, max_h;
+    float *p, *g, *m, *v;   // FP32 parameters, gradients, Adam moments
+    uint16_t *wf, *wb;      // BF16 conv weights (forward / data-gradient layouts)
+    uint16_t *x0, *a1, *a2, *a3, *a4, *d4, *d3, *d2, *d1;
+    float *f, *df, *logits, *dlogits, *loss;
+    long step;
+};
+
+extern "C" {
+
+size_t rh264_mvnet_param_count(void) { return NPARAM; }
+
+int rh264_mvnet_destroy(rh264_mvnet *net)
+{
+    if (!net) return 0;
+    void *bufs[] = {net->p, net->g, net->m, net->v, net->wf, net->wb, net->x0, net->a1, net->a2, net->a3, net->a4,
+                    net->d4, net->d3, net->d2, net->d1, net->f, net->df, net->logits, net->dlogits, net->loss};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete net;
+    return 0;
+}
+
+int rh264_mvnet_create(rh264_ctx *ctx, int max_n, int max_w, int max_h, rh264_mvnet **out)
+{
+    if (!out) return fail("mvnet_create: null out");
+    *out = nullptr;
+    if (max_n < 1 || max_w < 16 || max_h < 16 || max_w % 16 || max_h % 16) return RH264_DECLINED;
+    rh264_mvnet *net = new rh264_mvnet();
+    memset(net, 0, sizeof(*net));
+    net->max_n = max_n;
+    net->max_w = max_w;
+    net->max_h = max_h;
+    const size_t n = max_n, px8 = (size_t)(max_w / 8) * (max_h / 8), px4 = (size_t)(max_w / 4) * (max_h / 4);
+    const size_t cells = (size_t)(max_w / 16) * (max_h / 16);
+    struct {
+        void **ptr;
+        size_t bytes;
+    } allocs[] = {
+        {(void **)&net->p, NPARAM * 4}, {(void **)&net->g, NPARAM * 4}, {(void **)&net->m, NPARAM * 4},
+        {(void **)&net->v, NPARAM * 4}, {(void **)&net->wf, 3 * 9 * C * C * 2}, {(void **)&net->wb, 3 * 9 * C * C * 2},
+        {(void **)&net->x0, n * px4 * 2 * 2}, {(void **)&net->a1, n * px8 * C * 2}, {(void **)&net->a2, n * px8 * C * 2},
+        {(void **)&net->a3, n * px8 * C * 2}, {(void **)&net->a4, n * px8 * C * 2}, {(void **)&net->d4, n * px8 * C * 2},
+        {(void **)&net->d3, n * px8 * C * 2}, {(void **)&net->d2, n * px8 * C * 2}, {(void **)&net->d1, n * px8 * C * 2},
+        {(void **)&net->f, n * cells * C * 4}, {(void **)&net->df, n * cells * C * 4},
+        {(void **)&net->logits, n * cells * K * 4}, {(void **)&net->dlogits, n * cells * K * 4},
+        {(void **)&net->loss, 64},
+    };
+    for (auto &a : allocs) {
+        hipError_t e = hipMalloc(a.ptr, a.bytes);
+        if (e != hipSuccess) {
+            rh264_mvnet_destroy(net);
+            return fail("mvnet_create: %s", hipGetErrorString(e));
+        }
+    }
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(NPARAM, 256)), dim3(256), 0, ctx->stream, net->m, (size_t)NPARAM, 0.0f);
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(NPARAM, 256)), dim3(256), 0, ctx->stream, net->v, (size_t)NPARAM, 0.0f);
+    *out = net;
+    return 0;
+}
+
+int rh264_mvnet_set_params(rh264_ctx *ctx, rh264_mvnet *net, const float *host, size_t count)
+{
+    if (count != (size_t)NPARAM) return fail("mvnet_set_params: %zu parameters, expected %d", count, NPARAM);
+    HIP_TRY(hipMemcpyAsync(net->p, host, NPARAM * 4, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_refresh_bf16, dim3(nblocks(3 * 9 * C * C, 256)), dim3(256), 0, ctx->stream, net->p, net->wf,
+                       net->wb);
+    LAUNCH_CHECK("k_refresh_bf16");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int rh264_mvnet_get_params(rh264_ctx *ctx, rh264_mvnet *net, float *host, size_t count)
+{
+    if (count != (size_t)NPARAM) return fail("mvnet_get_params: %zu parameters, expected %d", count, NPARAM);
+    HIP_TRY(hipMemcpyAsync(host, net->p, NPARAM * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int rh264_mvnet_get_grads(rh264_ctx *ctx, rh264_mvnet *net, float *host, size_t count)
+{
+    if (count != (size_t)NPARAM) return fail("mvnet_get_grads: %zu parameters, expected %d", count, NPARAM);
+    HIP_TRY(hipMemcpyAsync(host, net->g, NPARAM * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+static int mvnet_geom(rh264_mvnet *net, int n, int w, int h)
+{
+    if (n < 1 || n > net->max_n || w % 16 || h % 16 || w < 16 || h < 16 || (size_t)w * h > (size_t)net->max_w * net->max_h)
+        return fail("mvnet: batch %d of %dx%d exceeds the network's buffers", n, w, h);
+    return 0;
+}
+
+static int mvnet_forward_impl(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref, size_t stride,
+                              int n, int w, int h)
+{
+    const int h4 = h / 4, w4 = w / 4, h8 = h / 8, w8 = w / 8;
+    const long cells = (long)n * (h / 16) * (w / 16);
+    hipLaunchKernelGGL(k_prep, dim3(nblocks((long)n * h4 * w4, 256)), dim3(256), 0, ctx->stream, cur, ref, stride, n, w,
+                       h, net->x0);
+    hipLaunchKernelGGL(k_conv1_fwd, dim3(nblocks((long)n * h8 * w8, 256)), dim3(256), 0, ctx->stream, net->x0, net->p,
+                       n, h4, w4, net->a1);
+    const long waves = (long)n * h8 * ((w8 + 15) / 16);
+    uint16_t *acts[4] = {net->a1, net->a2, net->a3, net->a4};
+    for (int l = 0; l < 3; l++)
+        hipLaunchKernelGGL(k_conv3_mfma, dim3(nblocks(waves, 4)), dim3(256), 0, ctx->stream, acts[l],
+                           net->wf + (size_t)l * 9 * C * C, net->p + OFF_B2 + l * (C * 9 * C + C), 1,
+                           (const uint16_t *)nullptr, n, h8, w8, acts[l + 1]);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(nblocks(cells * C, 256)), dim3(256), 0, ctx->stream, net->a4, n, h8, w8, net->f);
+    hipLaunchKernelGGL(k_head_fwd, dim3(nblocks(cells * K, 256)), dim3(256), 0, ctx->stream, net->f, net->p, cells,
+                       net->logits);
+    LAUNCH_CHECK("mvnet forward");
+    return 0;
+}
+
+int rh264_mvnet_forward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref, size_t stride,
+                        int n, int w, int h, float *logits, int32_t *top8)
+{
+    int rc = mvnet_geom(net, n, w, h);
+    if (rc) return rc;
+    rc = mvnet_forward_impl(ctx, net, cur, ref, stride, n, w, h);
+    if (rc) return rc;
+    const long cells = (long)n * (h / 16) * (w / 16);
+    if (logits)
+        HIP_TRY(hipMemcpyAsync(logits, net->logits, (size_t)cells * K * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    if (top8) {
+        hipLaunchKernelGGL(k_topk, dim3(nblocks(cells, 4)), dim3(256), 0, ctx->stream, net->logits, cells, top8);
+        LAUNCH_CHECK("k_topk");
+    }
+    return 0;
+}
+
+// Forward + backward on a batch; gradients (mean cross-entropy over cells) left in
+// the network's gradient buffer; loss (mean) returned through loss_out (host).
+int rh264_mvnet_backward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref, size_t stride,
+                         int n, int w, int h, const int32_t *labels, float *loss_out)
+{
+    int rc = mvnet_geom(net, n, w, h);
+    if (rc) return rc;
+    rc = mvnet_forward_impl(ctx, net, cur, ref, stride, n, w, h);
+    if (rc) return rc;
+    const int h4 = h / 4, w4 = w / 4, h8 = h / 8, w8 = w / 8;
+    const long cells = (long)n * (h / 16) * (w / 16);
+    const float scale = 1.0f / (float)cells;
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(NPARAM, 256)), dim3(256), 0, ctx->stream, net->g, (size_t)NPARAM, 0.0f);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, net->loss, (size_t)1, 0.0f);
+    hipLaunchKernelGGL(k_softmax_ce, dim3(nblocks(cells, 4)), dim3(256), 0, ctx->stream, net->logits, labels, cells,
+                       scale, net->dlogits, net->loss);
+    hipLaunchKernelGGL(k_head_bwd_w, dim3(nblocks(K * C + K, 256), nblocks(cells, 256)), dim3(256), 0, ctx->stream,
+                       net->dlogits, net->f, cells, net->g);
+    hipLaunchKernelGGL(k_head_bwd_data, dim3(nblocks(cells * C, 256)), dim3(256), 0, ctx->stream, net->dlogits, net->p,
+                       cells, net->df);
+    const long px8 = (long)n * h8 * w8;
+    hipLaunchKernelGGL(k_pool_bwd, dim3(nblocks(px8 * C, 256)), dim3(256), 0, ctx->stream, net->df, net->a4, n, h8, w8,
+                       net->d4);
+    // layers 4, 3, 2: weight gradient from (input activation, dz); data gradient
+    // dz_prev = conv(dz, flipped transposed w) masked by ReLU'(prev activation)
+    uint16_t *acts[4] = {net->a1, net->a2, net->a3, net->a4};
+    uint16_t *dzs[4] = {net->d1, net->d2, net->d3, net->d4};
+    const long waves = (long)n * h8 * ((w8 + 15) / 16);
+    for (int l = 2; l >= 0; l--) { // conv layer l+2 maps acts[l] -> acts[l+1], dz = dzs[l+1]
+        const int off_w = OFF_W2 + l * (C * 9 * C + C), off_b = off_w + C * 9 * C;
+        hipLaunchKernelGGL((k_conv_wgrad<C, 1>), dim3(nblocks(px8, 32)), dim3(256), 0, ctx->stream, acts[l], dzs[l + 1],
+                           n, h8, w8, h8, w8, net->g + off_w, net->g + off_b);
+        hipLaunchKernelGGL(k_conv3_mfma, dim3(nblocks(waves, 4)), dim3(256), 0, ctx->stream, dzs[l + 1],
+                           net->wb + (size_t)l * 9 * C * C, (const float *)nullptr, 0, acts[l], n, h8, w8, dzs[l]);
+    }
+    hipLaunchKernelGGL((k_conv_wgrad<2, 2>), dim3(nblocks(px8, 32)), dim3(256), 0, ctx->stream, net->x0, net->d1, n, h4,
+                       w4, h8, w8, net->g, net->g + OFF_B1);
+    LAUNCH_CHECK("mvnet backward");
+    if (loss_out) {
+        HIP_TRY(hipMemcpyAsync(loss_out, net->loss, 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return 0;
+}
+
+int rh264_mvnet_adam(rh264_ctx *ctx, rh264_mvnet *net, float lr)
+{
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    net->step++;
+    const float c1 = 1.0f - powf(b1, (float)net->step), c2 = 1.0f - powf(b2, (float)net->step);
+    hipLaunchKernelGGL(k_adam, dim3(nblocks(NPARAM, 256)), dim3(256), 0, ctx->stream, net->p, net->g, net->m, net->v,
+                       lr, b1, b2, eps, c1, c2);
+    hipLaunchKernelGGL(k_refresh_bf16, dim3(nblocks(3 * 9 * C * C, 256)), dim3(256), 0, ctx->stream, net->p, net->wf,
+                       net->wb);
+    LAUNCH_CHECK("mvnet adam");
+    return 0;
+}
+
+} // extern "C"

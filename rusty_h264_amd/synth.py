@@ -76,8 +76,9 @@ def hd_clip(nframes=16, seed=0, w=1920, h=1088, nrects=8, vmax=12.0, pan_max=4.0
     margin = int(np.ceil(pan_max * nframes)) + 4
     bg = _fbm(rng, h + 2 * margin, w + 2 * margin)
     rects = []
+    hi = min(257, w // 2 + 1, h // 2 + 1)  # 32..256 px; small test frames get smaller rectangles
     for _ in range(nrects):
-        rw, rh = (int(v) for v in rng.integers(32, 257, 2))
+        rw, rh = (int(v) for v in rng.integers(min(32, hi - 1), hi, 2))
         tex = rng.integers(0, 256, (rh + 2, rw + 2)).astype(np.float64)
         x, y = rng.uniform(0, w - rw), rng.uniform(0, h - rh)
         ang, speed = rng.uniform(0, 2 * np.pi), rng.uniform(0, vmax)
