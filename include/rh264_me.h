@@ -145,22 +145,29 @@ int rh264_plain_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes
                        size_t planes_stride, const int32_t *centres, const int32_t *mvps,
                        int32_t *int_mv, int32_t *int_cost, int32_t *mv, int32_t *cost);
 
-/* Stage timing of rh264_plain_search (no reference counterpart; the reference's
- * stage profiler is crates/rusty_h264-common/src/prof.rs).  rh264_set_profiling
- * (capacity > 0) makes the next `capacity` plain searches record HIP events around
- * the integer-search and refinement kernels on the context's stream, without any
- * host synchronisation; rh264_profile_get(i) returns search i's two durations.
- * capacity 0 turns it off and resets the count. */
+/* Stage timing of rh264_plain_search and rh264_learned_search (no reference
+ * counterpart; the reference's stage profiler is
+ * crates/rusty_h264-common/src/prof.rs).  rh264_set_profiling(capacity > 0) makes
+ * the next `capacity` searches record HIP events at their phase boundaries on the
+ * context's stream, without any host synchronisation.  Phases: plain = integer
+ * search, refinement; learned = network, candidate search (local searches,
+ * selection, fall-back full search), refinement.  rh264_profile_get(i) returns
+ * search i's time before the refinement and the refinement's time;
+ * rh264_profile_phases(i) writes up to `cap` phase durations (ms) and their count.
+ * capacity 0 turns profiling off and resets the count. */
 int rh264_set_profiling(rh264_ctx *ctx, int capacity);
 int rh264_profile_count(rh264_ctx *ctx);
 int rh264_profile_get(rh264_ctx *ctx, int i, float *search_ms, float *refine_ms);
+int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases);
 
 /* Kernel-variant selection (no reference counterpart; set by the autotuner).
  * Knobs: "fs_kernel" 0 = quad-SAD (v_qsad_pk_u16_u8) accumulator ring
  * (default), 1 = v_sad_u8 ring with 2 candidates per lane, 2 = v_sad_u8 ring
  * with 4 candidates per lane; "refine_lds" 1 (default) = the refinement of
  * 16x16/16x8/8x16 blocks stages each block's +-3 pel plane window in LDS, 0 =
- * it reads the plane sets in HBM.  Every variant returns identical results. */
+ * it reads the plane sets in HBM; "mvnet_fused" 1 (default) = network inference
+ * in one fused kernel per tile, 0 = layer by layer.  Every search variant returns
+ * identical results; the two network paths agree up to FP32 summation order. */
 int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
 
 /* Quarter-pel prediction of blocks (mc_luma, inter.rs:1193-1262, served from
@@ -213,7 +220,10 @@ int rh264_batch_destroy(rh264_batch *b);
  * MV bins of 4 pels) proposes 8 integer-MV centres per 16x16 MB.  Parameters are
  * FP32 (rh264_mvnet_param_count of them, layout in that file).
  * rh264_mvnet_forward: logits [n][mb][289] (device, may be NULL) and/or top8
- *   [n][mb][8] bin indices (device, may be NULL) for frames cur/ref (stride apart).
+ *   [n][mb][8] bin indices (device, 16-byte aligned, may be NULL) for frames
+ *   cur/ref (stride apart); by default one fused kernel per tile of 8x8 MBs
+ *   (rh264_tune "mvnet_fused" 0 selects the layer-by-layer kernels of training).
+ *   The four conv layers use BF16 weights and activations, FP32 accumulation.
  * rh264_mvnet_backward: forward + backward of the mean cross-entropy against
  *   labels [n][mb] (bin indices); gradients stay on the device (get_grads), the
  *   loss is returned on the host.  rh264_mvnet_adam: one Adam step. */

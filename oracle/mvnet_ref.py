@@ -2,8 +2,8 @@
 rh264_mvnet.hip): forward, backward and the bin encoding.
 
 TEST INFRASTRUCTURE ONLY: the CPU reference the GPU kernels are checked against
-(with `bf16=True` it rounds activations, data gradients and conv weights to
-BF16 at the same points as the GPU), and in float64 (`bf16=False`) the function
+(with `bf16=True` it rounds activations, data gradients and the four conv
+layers' weights to BF16 at the same points as the GPU), and in float64 (`bf16=False`) the function
 whose gradients are checked by finite differences.  The network has no
 counterpart in the reference codec.
 """
@@ -102,7 +102,7 @@ def forward(p, cur, ref, bf=True):
     P = unpack(np.asarray(p, np.float64))
     r = bf16 if bf else (lambda a: np.asarray(a, np.float64))
     x0 = r(prep(cur, ref))
-    a1 = r(np.maximum(_conv(x0, P["W1"], P["b1"], 2), 0))
+    a1 = r(np.maximum(_conv(x0, r(P["W1"]), P["b1"], 2), 0))
     a2 = r(np.maximum(_conv(a1, r(P["W2"]), P["b2"], 1), 0))
     a3 = r(np.maximum(_conv(a2, r(P["W3"]), P["b3"], 1), 0))
     a4 = r(np.maximum(_conv(a3, r(P["W4"]), P["b4"], 1), 0))

@@ -24,6 +24,8 @@ int rh264_fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
         if (e_ != hipSuccess) return fail("launch of %s failed: %s", name, hipGetErrorString(e_)); \
     } while (0)
 
+constexpr int RH264_PROF_EV = 4; // phase boundaries per profiled search
+
 struct rh264_ctx {
     int device;
     hipStream_t stream;
@@ -34,7 +36,9 @@ struct rh264_ctx {
     size_t tmp_mv_cap;            // in int32 elements
     int fs_kernel;                // integer search: 0 = quad-SAD, 1 = v_sad_u8 x2, 2 = v_sad_u8 x4
     int refine_lds;               // refinement stages the +-3 pel window in LDS (blocks of >= 8 sub-blocks)
-    std::vector<hipEvent_t> prof; // 3 events per profiled plain search
+    int mvnet_fused;              // network inference: 1 = one fused kernel per tile, 0 = layer by layer
+    std::vector<hipEvent_t> prof; // RH264_PROF_EV events per profiled search
+    std::vector<int> prof_np;     // phases recorded in each profiled search
     int prof_cap, prof_n;         // capacity and count of profiled searches
     void *ls_buf;                 // scratch of the learned-proposal search
     size_t ls_cap;

@@ -78,8 +78,9 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 __device__ __forceinline__ int clip_u8(int v) { return min(max(v, 0), 255); }
 // clip_u8((v + round) >> shift) computed as a clamp followed by a logical shift:
 // for v + round < 0 both give 0, above (256 << shift) - 1 both give 255.  Written
-// this way so the shift/saturate/pack idiom is not formed (gfx950's packed
-// shift-and-saturate form gave wrong bytes for this kernel).
+// this way so the shift/saturate/pack idiom is not formed: when it was, the
+// compiler emitted gfx950's packed shift-and-saturate instruction for the plane
+// kernel and the H/V/C bytes it produced differed from this scalar rule.
 template <int SHIFT, int ROUND> __device__ __forceinline__ uint32_t round_clip_u8(int v)
 {
     return (uint32_t)min(max(v + ROUND, 0), (256 << SHIFT) - 1) >> SHIFT;
@@ -666,32 +667,68 @@ template <int LPB> __device__ __forceinline__ int group_sum_n(int v)
     return v;
 }
 
-// sum |H d| of a 4x4 block, d = s - p, rows as packed dwords (transform.rs:637-655).
-__device__ __forceinline__ int satd4x4_regs(const uint32_t s[4], const uint32_t p[4])
+// sum |H d| of a 4x4 block, d = s - p, rows as packed dwords (transform.rs:637-655),
+// in packed 16-bit arithmetic (every intermediate fits: |d| <= 255,
+// after the column butterflies <= 1020, after the first row stage <= 2040).  The
+// source row is held as bytes (0,2) and (1,3) in the two halves of a dword; the
+// column transform runs on both pairs at once, the row transform's first stage on
+// pairs too, and its last stage uses |a + b| + |a - b| = 2 max(|a|, |b|).
+typedef short pk16 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ pk16 as_pk(uint32_t v) { return __builtin_bit_cast(pk16, v); }
+__device__ __forceinline__ uint32_t as_u32(pk16 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t bytes02(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x0c020c00u); }
+__device__ __forceinline__ uint32_t bytes13(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x0c030c01u); }
+
+struct SrcPk {
+    uint32_t a[4], b[4]; // per row: bytes (0,2) and (1,3) as 16-bit halves
+};
+
+__device__ __forceinline__ SrcPk src_pk(const uint32_t s[4])
 {
-    int m[16];
+    SrcPk S;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        int d0, d1, d2, d3;
-        diff4(s[r], p[r], d0, d1, d2, d3);
-        m[4 * r] = d0 + d1 + d2 + d3;
-        m[4 * r + 1] = d0 + d1 - d2 - d3;
-        m[4 * r + 2] = d0 - d1 - d2 + d3;
-        m[4 * r + 3] = d0 - d1 + d2 - d3;
+        S.a[r] = bytes02(s[r]);
+        S.b[r] = bytes13(s[r]);
     }
-    int sum = 0;
+    return S;
+}
+
+__device__ __forceinline__ void hadamard4_pk(const pk16 x[4], pk16 e[4])
+{
+    const pk16 a = x[0] + x[1], b = x[2] + x[3], c = x[0] - x[1], d = x[2] - x[3];
+    e[0] = a + b;
+    e[1] = a - b;
+    e[2] = c - d;
+    e[3] = c + d;
+}
+
+__device__ __forceinline__ int satd4x4_pk(const SrcPk &S, const uint32_t p[4])
+{
+    pk16 x02[4], x13[4], P[4], Q[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int a = m[c], b = m[4 + c], cc = m[8 + c], e = m[12 + c];
-        sum += abs(a + b + cc + e) + abs(a + b - cc - e) + abs(a - b - cc + e) + abs(a - b + cc - e);
+    for (int r = 0; r < 4; r++) {
+        x02[r] = as_pk(S.a[r]) - as_pk(bytes02(p[r]));
+        x13[r] = as_pk(S.b[r]) - as_pk(bytes13(p[r]));
     }
-    return sum;
+    hadamard4_pk(x02, P); // columns 0 and 2
+    hadamard4_pk(x13, Q); // columns 1 and 3
+    pk16 acc = {0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const pk16 u = P[k] + Q[k], v = P[k] - Q[k]; // (c0+c1, c2+c3), (c0-c1, c2-c3)
+        const pk16 lo = as_pk(__builtin_amdgcn_perm(as_u32(v), as_u32(u), 0x05040100u)); // (u.x, v.x)
+        const pk16 hi = as_pk(__builtin_amdgcn_perm(as_u32(v), as_u32(u), 0x07060302u)); // (u.y, v.y)
+        const pk16 mx = __builtin_elementwise_max(lo, hi), mn = __builtin_elementwise_min(lo, hi);
+        acc += __builtin_elementwise_max(mx, -mn); // max(|lo|, |hi|)
+    }
+    return 2 * ((int)acc.x + (int)acc.y);
 }
 
 // Candidate read from the staged window; di, dj in [-3, 2] (integer offset from the
 // start vector), (sbx, sby) = the lane's sub-block.
 template <int BW, int BH>
-__device__ __forceinline__ int eval_lane_lds(const uint8_t *win, const uint32_t s[4], int sbx, int sby, int di,
+__device__ __forceinline__ int eval_lane_lds(const uint8_t *win, const SrcPk &s, int sbx, int sby, int di,
                                              int dj, int fx, int fy)
 {
     using W = RefineWin<BW, BH>;
@@ -708,12 +745,12 @@ __device__ __forceinline__ int eval_lane_lds(const uint8_t *win, const uint32_t 
 #pragma unroll
         for (int r = 0; r < 4; r++) p[r] = avg4(p[r], lds_load4(pb + r * W::PITCH, xb));
     }
-    return satd4x4_regs(s, p);
+    return satd4x4_pk(s, p);
 }
 
 // Candidate read from the plane set in HBM (clamped at the plane border).
 template <int BW, int BH>
-__device__ __forceinline__ int eval_lane_global(const RefineGeom &G, const uint32_t s[4], int lx, int ly, int sbx,
+__device__ __forceinline__ int eval_lane_global(const RefineGeom &G, const SrcPk &s, int lx, int ly, int sbx,
                                                 int sby, int mvx, int mvy)
 {
     const int bpx = lx + (mvx >> 2) + G.pad, bpy = ly + (mvy >> 2) + G.pad;
@@ -730,7 +767,7 @@ __device__ __forceinline__ int eval_lane_global(const RefineGeom &G, const uint3
         p[r] = inb ? load4_fast(pa, G.pw, xa, ya + r) : load4_clamped(pa, G.pw, G.ph, xa, ya + r);
         if (two) p[r] = avg4(p[r], inb ? load4_fast(pb, G.pw, xb, yb + r) : load4_clamped(pb, G.pw, G.ph, xb, yb + r));
     }
-    return satd4x4_regs(s, p);
+    return satd4x4_pk(s, p);
 }
 
 // The ring walk of one block by its lane group (mb16.rs:3230-3275).
@@ -740,13 +777,14 @@ __device__ __forceinline__ void refine_walk(const RefineGeom &G, const uint8_t *
                                             int mpy, int pat, const int32_t *s_lr, int &best, int &evals)
 {
     constexpr int NSB = RefShape<BW, BH>::NSB;
+    const SrcPk S = src_pk(s);
     auto cost_of = [&](int mx, int my) -> int {
         const int di = (mx >> 2) - sxi, dj = (my >> 2) - syi;
         int v;
         if (USE_LDS && di >= -3 && di <= 2 && dj >= -3 && dj <= 2)
-            v = eval_lane_lds<BW, BH>(win, s, sbx, sby, di, dj, mx & 3, my & 3);
+            v = eval_lane_lds<BW, BH>(win, S, sbx, sby, di, dj, mx & 3, my & 3);
         else
-            v = eval_lane_global<BW, BH>(G, s, lx, ly, sbx, sby, mx, my);
+            v = eval_lane_global<BW, BH>(G, S, lx, ly, sbx, sby, mx, my);
         return group_sum_n<NSB>(v) + s_lr[mvbits(mx - mpx) + mvbits(my - mpy)];
     };
     best = cost_of(bx, by);
@@ -1491,6 +1529,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->ls_cap = 0;
     c->fs_kernel = 0;
     c->refine_lds = 1;
+    c->mvnet_fused = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -1761,6 +1800,15 @@ int rh264_subpel_refine(rh264_ctx *ctx, const rh264_search_params *p, int nframe
                    start_mv, mvps, nullptr, nullptr, mv, cost, evals);
 }
 
+// The next free profiling slot (nullptr when profiling is off or full); the caller
+// records nphases + 1 events into it.
+static hipEvent_t *prof_slot(rh264_ctx *ctx, int nphases)
+{
+    if (ctx->prof_n >= ctx->prof_cap) return nullptr;
+    ctx->prof_np[(size_t)ctx->prof_n] = nphases;
+    return &ctx->prof[(size_t)RH264_PROF_EV * ctx->prof_n++];
+}
+
 int rh264_plain_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
                        size_t cur_stride, const uint8_t *planes, size_t planes_stride, const int32_t *centres,
                        const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv, int32_t *cost)
@@ -1772,8 +1820,7 @@ int rh264_plain_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes
     if (nframes <= 0) return 0;
     if (nframes > 65535) return fail("nframes %d > 65535", nframes);
     if (!mv || !cost) return fail("plain_search: null outputs");
-    hipEvent_t *ev = nullptr;
-    if (ctx->prof_n < ctx->prof_cap) ev = &ctx->prof[3 * ctx->prof_n++];
+    hipEvent_t *ev = prof_slot(ctx, 2);
     if (ev) HIP_TRY(hipEventRecord(ev[0], ctx->stream));
     rc = full_search_keys(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps);
     if (rc) return rc;
@@ -1790,11 +1837,25 @@ int rh264_set_profiling(rh264_ctx *ctx, int capacity)
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (hipEvent_t e : ctx->prof) HIP_TRY(hipEventDestroy(e));
     ctx->prof.clear();
+    ctx->prof_np.clear();
     ctx->prof_cap = ctx->prof_n = 0;
     if (capacity <= 0) return 0;
-    ctx->prof.resize((size_t)3 * capacity);
+    ctx->prof.resize((size_t)RH264_PROF_EV * capacity);
+    ctx->prof_np.assign((size_t)capacity, 0);
     for (auto &e : ctx->prof) HIP_TRY(hipEventCreate(&e));
     ctx->prof_cap = capacity;
+    return 0;
+}
+
+int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases)
+{
+    if (i < 0 || i >= ctx->prof_n) return fail("profile slot %d of %d", i, ctx->prof_n);
+    if (!ms || !nphases) return fail("rh264_profile_phases: null pointers");
+    const int np = ctx->prof_np[(size_t)i];
+    hipEvent_t *ev = &ctx->prof[(size_t)RH264_PROF_EV * i];
+    HIP_TRY(hipEventSynchronize(ev[np]));
+    for (int k = 0; k < np && k < cap; k++) HIP_TRY(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+    *nphases = np;
     return 0;
 }
 
@@ -1812,16 +1873,21 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
         ctx->refine_lds = value ? 1 : 0;
         return 0;
     }
+    if (!strcmp(knob, "mvnet_fused")) {
+        ctx->mvnet_fused = value ? 1 : 0;
+        return 0;
+    }
     return fail("rh264_tune: unknown knob '%s'", knob);
 }
 
 int rh264_profile_get(rh264_ctx *ctx, int i, float *search_ms, float *refine_ms)
 {
     if (i < 0 || i >= ctx->prof_n) return fail("profile slot %d of %d", i, ctx->prof_n);
-    hipEvent_t *ev = &ctx->prof[3 * (size_t)i];
-    HIP_TRY(hipEventSynchronize(ev[2]));
-    HIP_TRY(hipEventElapsedTime(search_ms, ev[0], ev[1]));
-    HIP_TRY(hipEventElapsedTime(refine_ms, ev[1], ev[2]));
+    const int np = ctx->prof_np[(size_t)i];
+    hipEvent_t *ev = &ctx->prof[(size_t)RH264_PROF_EV * i];
+    HIP_TRY(hipEventSynchronize(ev[np]));
+    HIP_TRY(hipEventElapsedTime(search_ms, ev[0], ev[np - 1]));
+    HIP_TRY(hipEventElapsedTime(refine_ms, ev[np - 1], ev[np]));
     return 0;
 }
 
@@ -2300,8 +2366,11 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     unsigned *count = (unsigned *)(base + o_count);
     uint64_t *keys8 = (uint64_t *)(base + o_keys), *keysf = keys8 + (size_t)nb * LS_NCAND;
 
+    hipEvent_t *ev = prof_slot(ctx, 3);
+    if (ev) HIP_TRY(hipEventRecord(ev[0], ctx->stream));
     rc = rh264_mvnet_forward(ctx, net, cur, ref, frame_stride, nframes, p->cw, p->ch, nullptr, top8);
     if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
     hipLaunchKernelGGL(k_cand_centres, dim3((unsigned)((nb * LS_NCAND + 255) / 256)), dim3(256), 0, ctx->stream, top8,
                        nb, centres);
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb * (LS_NCAND + 1) + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -2326,8 +2395,12 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
                        keysf, p->range, start);
     LAUNCH_CHECK("learned_search select");
     (void)nblk;
-    return launch_refine<16, 16>(ctx, p, nframes, cur, frame_stride, planes, planes_stride, nullptr, nullptr, start,
-                                 nullptr, nullptr, nullptr, mv, cost, nullptr);
+    if (ev) HIP_TRY(hipEventRecord(ev[2], ctx->stream));
+    rc = launch_refine<16, 16>(ctx, p, nframes, cur, frame_stride, planes, planes_stride, nullptr, nullptr, start,
+                               nullptr, nullptr, nullptr, mv, cost, nullptr);
+    if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[3], ctx->stream));
+    return 0;
 }
 
 } // extern "C"

@@ -78,7 +78,8 @@ __global__ void k_conv1_fwd(const uint16_t *__restrict__ x0, const float *__rest
                             uint16_t *__restrict__ a1)
 {
     __shared__ float sw[C * 18 + C];
-    for (int i = threadIdx.x; i < C * 18 + C; i += blockDim.x) sw[i] = p[OFF_W1 + i];
+    // BF16-rounded weights (the biases stay FP32), as in the fused kernel
+    for (int i = threadIdx.x; i < C * 18 + C; i += blockDim.x) sw[i] = i < C * 18 ? bf2f(f2bf(p[OFF_W1 + i])) : p[OFF_W1 + i];
     __syncthreads();
     const int h8 = h4 / 2, w8 = w4 / 2;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -379,6 +380,255 @@ __global__ void k_topk(const float *__restrict__ logits, long ncell, int32_t *__
     }
 }
 
+// ---------------------------------------------------------------------------
+// fused inference: one workgroup per tile of 8x8 cells (16x16 pixels at 1/8
+// resolution).  The tile's x0 window, a1..a4 (with the halo each following 3x3
+// layer needs: 22, 20, 18, 16 pixels square) and the pooled features stay in
+// LDS; the 3x3 layers run on v_mfma_f32_16x16x32_bf16 (one K step = one tap, all
+// 32 input channels), the head on v_mfma_f32_16x16x4_f32 and the top-8 selection
+// in registers.  Same arithmetic as the layer-by-layer kernels: BF16 activations
+// and conv weights, FP32 accumulation, zero padding at every layer's border.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int FT = 8;                                             // cells per tile side
+constexpr int S4 = 2 * FT, S3 = S4 + 2, S2 = S4 + 4, S1 = S4 + 6; // a4..a1 window sides
+constexpr int SX = 2 * S1 + 1;                                    // x0 window side (1/4 res)
+constexpr int PSB = 80;                                           // LDS bytes per pixel (64 + 16 pad)
+constexpr int LDS_A = S1 * S1 * PSB, LDS_B = S2 * S2 * PSB;       // a1/a3 and a2/a4
+constexpr int LDS_X = FT * FT * C * 4;                            // x0 window, then pooled features
+static_assert(SX * SX * 4 <= LDS_X, "x0 window fits the feature buffer");
+
+__device__ __forceinline__ float4_t mfma_bf16_k32(u32x4 a, u32x4 b, float4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
+}
+
+// Epilogue of one 16-pixel tile of a conv layer: bias, ReLU, zero outside the
+// image, BF16, four channels (8 bytes) per lane and half.
+__device__ __forceinline__ void conv_store(uint8_t *out, int pc, bool valid, bool inside, const float4_t acc[2],
+                                           const float *bias, int q)
+{
+    if (!valid) return;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        uint32_t w[2];
+#pragma unroll
+        for (int i = 0; i < 4; i += 2) {
+            float v0 = acc[h][i] + bias[16 * h + 4 * q + i], v1 = acc[h][i + 1] + bias[16 * h + 4 * q + i + 1];
+            v0 = inside && v0 > 0.0f ? v0 : 0.0f;
+            v1 = inside && v1 > 0.0f ? v1 : 0.0f;
+            w[i / 2] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+        }
+        *(uint2 *)(out + pc * PSB + (16 * h + 4 * q) * 2) = make_uint2(w[0], w[1]);
+    }
+}
+
+// 3x3 conv, 32 -> 32, from window `in` (side SIN) to window `out` (side SOUT =
+// SIN - 2) whose origin lies `m` pixels up-left of the tile origin (oy, ox).
+template <int SIN>
+__device__ __forceinline__ void conv32_window(const uint8_t *in, uint8_t *out, const uint16_t *__restrict__ wt,
+                                              const float *__restrict__ bias, int oy, int ox, int m, int h8, int w8,
+                                              int lane, int wave)
+{
+    constexpr int SOUT = SIN - 2, NPX = SOUT * SOUT, NT = (NPX + 15) / 16;
+    const int q = lane >> 4, col = lane & 15;
+    u32x4 wa[9][2];
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) wa[t][h] = *(const u32x4 *)(wt + ((size_t)t * C + 16 * h + col) * C + 8 * q);
+    for (int tile = wave; tile < NT; tile += 4) {
+        const int px = tile * 16 + col;
+        const bool valid = px < NPX;
+        const int pc = valid ? px : NPX - 1;
+        const int y = pc / SOUT, x = pc - y * SOUT;
+        float4_t acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            const u32x4 b = *(const u32x4 *)(in + ((y + t / 3) * SIN + x + t % 3) * PSB + 16 * q);
+            acc[0] = mfma_bf16_k32(wa[t][0], b, acc[0]);
+            acc[1] = mfma_bf16_k32(wa[t][1], b, acc[1]);
+        }
+        const int gy = oy - m + y, gx = ox - m + x;
+        conv_store(out, pc, valid, gy >= 0 && gy < h8 && gx >= 0 && gx < w8, acc, bias, q);
+    }
+}
+
+// (v, k) ordering of the top-8: larger logit first, ties to the lower bin.
+__device__ __forceinline__ bool topk_before(float va, int ka, float vb, int kb)
+{
+    return va > vb || (va == vb && ka < kb);
+}
+
+__device__ __forceinline__ void topk_cas(float &va, int &ka, float &vb, int &kb)
+{
+    if (topk_before(vb, kb, va, ka)) {
+        const float tv = va;
+        const int tk = ka;
+        va = vb;
+        ka = kb;
+        vb = tv;
+        kb = tk;
+    }
+}
+
+// Merge this lane's sorted top-8 with the one of lane ^ xr: the larger of L[i]
+// and P[7-i] form a bitonic sequence holding the top 8, sorted by a bitonic merge.
+__device__ __forceinline__ void topk_merge(float v[8], int k[8], int xr)
+{
+    float pv[8];
+    int pk[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        pv[j] = __shfl_xor(v[j], xr, 64);
+        pk[j] = __shfl_xor(k[j], xr, 64);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        if (topk_before(pv[7 - j], pk[7 - j], v[j], k[j])) {
+            v[j] = pv[7 - j];
+            k[j] = pk[7 - j];
+        }
+#pragma unroll
+    for (int d = 4; d >= 1; d >>= 1)
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if ((j & d) == 0) topk_cas(v[j], k[j], v[j + d], k[j + d]);
+}
+
+__global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ x0, const float *__restrict__ p,
+                                                   const uint16_t *__restrict__ wf, int h4, int w4, int tiles_x,
+                                                   int tiles_y, float *__restrict__ logits,
+                                                   int32_t *__restrict__ top8)
+{
+    __shared__ __align__(16) uint8_t s_a[LDS_A];
+    __shared__ __align__(16) uint8_t s_b[LDS_B];
+    __shared__ __align__(16) uint8_t s_x[LDS_X];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int q = lane >> 4, col = lane & 15;
+    const int tpi = tiles_x * tiles_y;
+    const int img = blockIdx.x / tpi, tr = blockIdx.x - img * tpi;
+    const int ty = tr / tiles_x, tx = tr - ty * tiles_x;
+    const int h8 = h4 / 2, w8 = w4 / 2, hc = h8 / 2, wc = w8 / 2;
+    const int oy = ty * S4, ox = tx * S4; // tile origin at 1/8 resolution
+
+    // x0 window: 1/4-res rows/cols 2(o - 3) - 1 .. + SX - 1, zero outside the image
+    {
+        const uint32_t *xw = (const uint32_t *)x0 + (size_t)img * h4 * w4;
+        const int y0 = 2 * (oy - 3) - 1, x0c = 2 * (ox - 3) - 1;
+        for (int i = tid; i < SX * SX; i += 256) {
+            const int r = i / SX, c = i - r * SX, gy = y0 + r, gx = x0c + c;
+            ((uint32_t *)s_x)[i] = gy >= 0 && gy < h4 && gx >= 0 && gx < w4 ? xw[(size_t)gy * w4 + gx] : 0u;
+        }
+    }
+    __syncthreads();
+
+    // L1: 3x3 stride 2, 2 -> 32; K = 9 taps x 2 channels = 18 of the 32 (rest zero)
+    {
+        u32x4 wa[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int k = 8 * q + 2 * j;
+                const float *w = p + OFF_W1 + (16 * h + col) * 18 + k;
+                wa[h][j] = k < 18 ? (uint32_t)f2bf(w[0]) | ((uint32_t)f2bf(w[1]) << 16) : 0u;
+            }
+        constexpr int NPX = S1 * S1, NT = (NPX + 15) / 16;
+        for (int tile = wave; tile < NT; tile += 4) {
+            const int px = tile * 16 + col;
+            const bool valid = px < NPX;
+            const int pc = valid ? px : NPX - 1;
+            const int y = pc / S1, x = pc - y * S1;
+            u32x4 b;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int t = 4 * q + j; // tap of k = 8q + 2j (both channels in one dword)
+                b[j] = t < 9 ? ((const uint32_t *)s_x)[(2 * y + t / 3) * SX + 2 * x + t % 3] : 0u;
+            }
+            float4_t acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+            acc[0] = mfma_bf16_k32(wa[0], b, acc[0]);
+            acc[1] = mfma_bf16_k32(wa[1], b, acc[1]);
+            const int gy = oy - 3 + y, gx = ox - 3 + x;
+            conv_store(s_a, pc, valid, gy >= 0 && gy < h8 && gx >= 0 && gx < w8, acc, p + OFF_B1, q);
+        }
+    }
+    __syncthreads();
+    conv32_window<S1>(s_a, s_b, wf, p + OFF_B2, oy, ox, 2, h8, w8, lane, wave);
+    __syncthreads();
+    conv32_window<S2>(s_b, s_a, wf + 9 * C * C, p + OFF_B3, oy, ox, 1, h8, w8, lane, wave);
+    __syncthreads();
+    conv32_window<S3>(s_a, s_b, wf + 2 * 9 * C * C, p + OFF_B4, oy, ox, 0, h8, w8, lane, wave);
+    __syncthreads();
+
+    // 2x2 average pool -> f[cell][c] (FP32, in the x0 buffer)
+    float *s_f = (float *)s_x;
+    for (int i = tid; i < FT * FT * C; i += 256) {
+        const int cell = i / C, c = i - cell * C, cy = cell / FT, cx = cell - cy * FT;
+        const uint8_t *a = s_b + ((2 * cy) * S4 + 2 * cx) * PSB + 2 * c;
+        const float v00 = bf2f(*(const uint16_t *)a), v01 = bf2f(*(const uint16_t *)(a + PSB));
+        const float v10 = bf2f(*(const uint16_t *)(a + S4 * PSB)), v11 = bf2f(*(const uint16_t *)(a + S4 * PSB + PSB));
+        s_f[i] = 0.25f * (v00 + v01 + v10 + v11);
+    }
+    __syncthreads();
+
+    // head + top-8: wave w takes cells 16w .. 16w+15 of the tile (lane column = cell)
+    const int cell = 16 * wave + col, cy = cell / FT, cx = cell - cy * FT;
+    const int gcy = ty * FT + cy, gcx = tx * FT + cx;
+    const bool cvalid = gcy < hc && gcx < wc;
+    const size_t gcell = ((size_t)img * hc + (cvalid ? gcy : 0)) * wc + (cvalid ? gcx : 0);
+    float fb[8];
+#pragma unroll
+    for (int s8 = 0; s8 < 8; s8++) fb[s8] = s_f[cell * C + 4 * s8 + q];
+    float tv[8];
+    int tk[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        tv[j] = -INFINITY;
+        tk[j] = 0x7fffffff;
+    }
+    constexpr int NBT = (K + 15) / 16;
+    for (int bt = 0; bt < NBT; bt++) {
+        const int arow = 16 * bt + col;
+        float4_t acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int s8 = 0; s8 < 8; s8++) {
+            const float a = arow < K ? p[OFF_WH + arow * C + 4 * s8 + q] : 0.0f;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, fb[s8], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int bin = 16 * bt + 4 * q + i;
+            if (bin >= K) continue;
+            const float z = acc[i] + p[OFF_BH + bin];
+            if (logits && cvalid) logits[gcell * K + bin] = z;
+            if (z > tv[7]) { // bins arrive in increasing order: ties keep the earlier bin
+                tv[7] = z;
+                tk[7] = bin;
+#pragma unroll
+                for (int j = 7; j > 0; j--)
+                    if (tv[j] > tv[j - 1]) {
+                        const float t0 = tv[j];
+                        const int t1 = tk[j];
+                        tv[j] = tv[j - 1];
+                        tk[j] = tk[j - 1];
+                        tv[j - 1] = t0;
+                        tk[j - 1] = t1;
+                    }
+            }
+        }
+    }
+    topk_merge(tv, tk, 16);
+    topk_merge(tv, tk, 32);
+    if (top8 && cvalid && q == 0) {
+        *(int4 *)(top8 + gcell * TOPK) = make_int4(tk[0], tk[1], tk[2], tk[3]);
+        *(int4 *)(top8 + gcell * TOPK + 4) = make_int4(tk[4], tk[5], tk[6], tk[7]);
+    }
+}
+
 __global__ void k_fill_f32(float *p, size_t n, float v)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -512,9 +762,20 @@ int rh264_mvnet_forward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, co
 {
     int rc = mvnet_geom(net, n, w, h);
     if (rc) return rc;
+    if ((uintptr_t)top8 & 15) return fail("mvnet_forward: top8 must be 16-byte aligned");
+    const long cells = (long)n * (h / 16) * (w / 16);
+    if (ctx->mvnet_fused) {
+        const int h4 = h / 4, w4 = w / 4;
+        const int tiles_x = (w / 16 + FT - 1) / FT, tiles_y = (h / 16 + FT - 1) / FT;
+        hipLaunchKernelGGL(k_prep, dim3(nblocks((long)n * h4 * w4, 256)), dim3(256), 0, ctx->stream, cur, ref, stride,
+                           n, w, h, net->x0);
+        hipLaunchKernelGGL(k_fwd_fused, dim3((unsigned)((long)n * tiles_x * tiles_y)), dim3(256), 0, ctx->stream,
+                           net->x0, net->p, net->wf, h4, w4, tiles_x, tiles_y, logits, top8);
+        LAUNCH_CHECK("k_fwd_fused");
+        return 0;
+    }
     rc = mvnet_forward_impl(ctx, net, cur, ref, stride, n, w, h);
     if (rc) return rc;
-    const long cells = (long)n * (h / 16) * (w / 16);
     if (logits)
         HIP_TRY(hipMemcpyAsync(logits, net->logits, (size_t)cells * K * 4, hipMemcpyDeviceToDevice, ctx->stream));
     if (top8) {

@@ -15,6 +15,8 @@ from . import _lib
 from .accel import DevBuf, _check, _need, _ptr
 
 NB, K, CH = 17, 289, 32
+LOCAL_RANGE = 2       # integer search of +-2 pels around each proposed centre
+FALLBACK_COST = 6000  # best candidate's integer cost above which a block is fully searched
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "weights", "mvnet.npz")
 
@@ -98,8 +100,8 @@ class MVNet:
             self.h = None
 
 
-def learned_search(dev, net, params, nframes, cur, ref, frame_stride, planes, planes_stride, local_range=2,
-                   fallback_cost=1 << 30, out=None):
+def learned_search(dev, net, params, nframes, cur, ref, frame_stride, planes, planes_stride, local_range=LOCAL_RANGE,
+                   fallback_cost=FALLBACK_COST, out=None):
     """rh264_learned_search; returns (mv, cost, fallback) device buffers (asynchronous)."""
     nb = (params.cw // 16) * (params.ch // 16) * nframes
     if out is None:

@@ -38,26 +38,35 @@ def _params(seed):
     return R.pack(P).astype(np.float32)
 
 
-def test_forward_logits_and_top8(dev):
-    n, w, h = 2, 128, 96
+@pytest.mark.parametrize("w,h", [(128, 96), (208, 160)])
+@pytest.mark.parametrize("fused", [1, 0])
+def test_forward_logits_and_top8(dev, fused, w, h):
+    """Both inference paths: the fused per-tile kernel (tiles of 8x8 cells; 208x160
+    leaves partial tiles on both axes) and the layer-by-layer kernels the training
+    step uses."""
+    n = 2
     cur, ref = _pair_batch(n, w, h, 10)
     p = _params(1)
-    net = mvnet.MVNet(dev, n, w, h, p)
-    both = dev.upload(np.stack([cur, ref]))       # [cur frames..., ref frames...]
-    ncell = n * (h // 16) * (w // 16)
-    lg = dev.alloc(4 * ncell * R.K)
-    t8 = dev.alloc(4 * ncell * 8)
-    net.forward(int(both), int(both) + n * w * h, w * h, n, w, h, lg, t8)
-    got = dev.download(lg, (ncell, R.K), np.float32)
+    dev.tune("mvnet_fused", fused)
+    try:
+        net = mvnet.MVNet(dev, n, w, h, p)
+        both = dev.upload(np.stack([cur, ref]))       # [cur frames..., ref frames...]
+        ncell = n * (h // 16) * (w // 16)
+        lg = dev.alloc(4 * ncell * R.K)
+        t8 = dev.alloc(4 * ncell * 8)
+        net.forward(int(both), int(both) + n * w * h, w * h, n, w, h, lg, t8)
+        got = dev.download(lg, (ncell, R.K), np.float32)
+        top = dev.download(t8, (ncell, 8), np.int32)
+        net.close()
+    finally:
+        dev.tune("mvnet_fused", 1)
     want = R.forward(p, cur, ref, bf=True)["logits"].reshape(ncell, R.K)
     # BF16 activations: summation order can flip a rounding; |err| <= 0.02 + 2% of |logit|
     err = np.abs(got - want)
     assert (err <= 0.02 + 0.02 * np.abs(want)).all(), float(err.max())
     # top-8 = eight largest logits in order, ties to the lower bin (checked on the GPU's own logits)
-    top = dev.download(t8, (ncell, 8), np.int32)
     order = np.lexsort((np.arange(R.K)[None, :].repeat(ncell, 0), -got), axis=1)[:, :8]
     assert np.array_equal(top, order)
-    net.close()
 
 
 def test_backward_gradients(dev):
