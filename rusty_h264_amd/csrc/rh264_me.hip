@@ -1450,6 +1450,112 @@ __global__ void k_cand_select(const uint64_t *__restrict__ keys8, const int32_t 
     if (fb) list[atomicAdd(count, 1u)] = (int32_t)b;
 }
 
+// All candidates' local searches in one pass.  Lane (block, candidate k, window
+// column dx) keeps the block's 16 source rows in registers, realigns each of the
+// 16 + 2RL reference rows of its column once and adds it into the SADs of the
+// 2RL + 1 window rows that contain it.  A block's key is the minimum over its
+// lanes of (cost << 32 | k << 8 | raster index): the lowest cost, then the
+// earliest candidate, then the first raster position of that candidate's window
+// -- the order of the per-candidate searches followed by k_cand_select.
+template <int RL, bool FAST>
+__device__ __forceinline__ void cand_rows(const uint32_t (&s)[16][4], const uint8_t *__restrict__ F, int pw, int ph,
+                                          int x, int y0, uint32_t (&acc)[2 * RL + 1])
+{
+    constexpr int NP = 2 * RL + 1, NJ = 16 + 2 * RL;
+    const int sh = x & 3;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        uint32_t w[4];
+        if (FAST) {
+            const uint32_t *rp = (const uint32_t *)(F + (size_t)(y0 + j) * pw + (x & ~3));
+            uint32_t a[5];
+#pragma unroll
+            for (int i = 0; i < 5; i++) a[i] = rp[i];
+#pragma unroll
+            for (int i = 0; i < 4; i++) w[i] = __builtin_amdgcn_alignbyte(a[i + 1], a[i], (uint32_t)sh);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) w[i] = load4_clamped(F, pw, ph, x + 4 * i, y0 + j);
+        }
+#pragma unroll
+        for (int t = 0; t < NP; t++) {
+            const int r = j - t;
+            if (r < 0 || r >= 16) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) acc[t] = __builtin_amdgcn_sad_u8(s[r][i], w[i], acc[t]);
+        }
+    }
+}
+
+template <int RL>
+__global__ __launch_bounds__(256) void k_cand_search(rh264_search_params p, const uint8_t *__restrict__ cur,
+                                                     size_t cur_stride, const uint8_t *__restrict__ planes,
+                                                     size_t planes_stride, const int32_t *__restrict__ top8,
+                                                     const int32_t *__restrict__ mvps, long nb, int32_t fb_cost,
+                                                     int32_t *__restrict__ start, int32_t *__restrict__ flag,
+                                                     unsigned *__restrict__ count, int32_t *__restrict__ list)
+{
+    constexpr int NP = 2 * RL + 1, LPB = LS_NCAND * NP, BPW = 256 / LPB, NJ = 16 + 2 * RL;
+    __shared__ int32_t s_lr[RH264_LR_N];
+    __shared__ unsigned long long s_key[BPW];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
+    if (threadIdx.x < BPW) s_key[threadIdx.x] = ~0ull;
+    __syncthreads();
+    const int slot = threadIdx.x / LPB, li = threadIdx.x - slot * LPB;
+    const long gb0 = (long)blockIdx.x * BPW + slot;
+    const bool active = slot < BPW && gb0 < nb;
+    if (active) {
+        const long gb = gb0;
+        const int k = li / NP, dxi = li - k * NP;
+        const int nbx = p.cw / 16, nblk = nbx * (p.ch / 16);
+        const int frame = (int)(gb / nblk), b = (int)(gb - (long)frame * nblk);
+        const int lx = (b % nbx) * 16, ly = (b / nbx) * 16;
+        const int c = top8[gb * LS_NCAND + k];
+        const int cx = ((c % LS_BINS) - LS_BINS / 2) * 16, cy = ((c / LS_BINS) - LS_BINS / 2) * 16;
+        const int mpx = mvps ? mvps[2 * gb] : 0, mpy = mvps ? mvps[2 * gb + 1] : 0;
+        const int pw = p.cw + 2 * p.pad, ph = p.ch + 2 * p.pad;
+        const uint8_t *F = planes + (size_t)frame * planes_stride;
+        const uint8_t *src = cur + (size_t)frame * cur_stride + (size_t)ly * p.cw + lx;
+        uint32_t s[16][4];
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) s[r][i] = *(const uint32_t *)(src + (size_t)r * p.cw + 4 * i);
+        const int x = lx + (cx >> 2) + p.pad + dxi - RL; // padded column of the lane's candidates
+        const int y0 = ly + (cy >> 2) + p.pad - RL;      // padded row of the window's first row
+        uint32_t acc[NP];
+#pragma unroll
+        for (int t = 0; t < NP; t++) acc[t] = 0;
+        if (x >= 0 && (x & ~3) + 20 <= pw && y0 >= 0 && y0 + NJ <= ph)
+            cand_rows<RL, true>(s, F, pw, ph, x, y0, acc);
+        else
+            cand_rows<RL, false>(s, F, pw, ph, x, y0, acc);
+        const int rx = mvbits(cx + 4 * (dxi - RL) - mpx);
+        uint64_t best = ~0ull;
+#pragma unroll
+        for (int t = 0; t < NP; t++) {
+            const uint32_t cost = acc[t] + (uint32_t)s_lr[rx + mvbits(cy + 4 * (t - RL) - mpy)];
+            const uint64_t key = ((uint64_t)cost << 32) | ((uint64_t)k << 8) | (uint64_t)(t * NP + dxi);
+            best = key < best ? key : best;
+        }
+        atomicMin(&s_key[slot], (unsigned long long)best);
+    }
+    __syncthreads();
+    if (threadIdx.x < BPW) {
+        const long g = (long)blockIdx.x * BPW + threadIdx.x;
+        if (g < nb) {
+            const uint64_t key = s_key[threadIdx.x];
+            const int kk = (int)((key >> 8) & 255), idx = (int)(key & 255);
+            const int ck = top8[g * LS_NCAND + kk];
+            start[2 * g] = ((ck % LS_BINS) - LS_BINS / 2) * 16 + 4 * (idx % NP - RL);
+            start[2 * g + 1] = ((ck / LS_BINS) - LS_BINS / 2) * 16 + 4 * (idx / NP - RL);
+            const bool fb = (int64_t)(key >> 32) > (int64_t)fb_cost;
+            flag[g] = fb ? 1 : 0;
+            if (fb) list[atomicAdd(count, 1u)] = (int32_t)g;
+        }
+    }
+}
+
 // Full search of the listed blocks only (frame-major global block indices).
 template <int BW, int BH, int V>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
@@ -2371,22 +2477,47 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     rc = rh264_mvnet_forward(ctx, net, cur, ref, frame_stride, nframes, p->cw, p->ch, nullptr, top8);
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
-    hipLaunchKernelGGL(k_cand_centres, dim3((unsigned)((nb * LS_NCAND + 255) / 256)), dim3(256), 0, ctx->stream, top8,
-                       nb, centres);
-    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb * (LS_NCAND + 1) + 255) / 256)), dim3(256), 0, ctx->stream,
-                       keys8, (size_t)nb * (LS_NCAND + 1), ~0ull);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, count, (size_t)1, 0u);
-    LAUNCH_CHECK("learned_search setup");
-    rh264_search_params q = *p;
-    q.range = local_range;
-    const long nblk = nb / nframes;
-    for (int k = 0; k < LS_NCAND; k++) {
-        rc = launch_full<16, 16>(ctx, &q, nframes, cur, frame_stride, planes, planes_stride,
-                                 centres + 2 * (size_t)k * nb, nullptr, keys8 + (size_t)k * nb);
-        if (rc) return rc;
+    if (local_range <= 4) {
+        // one pass over all candidates (the per-candidate searches below give the same result)
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keysf,
+                           (size_t)nb, ~0ull);
+        const int lpb = LS_NCAND * (2 * local_range + 1), bpw = 256 / lpb;
+        const dim3 grid((unsigned)((nb + bpw - 1) / bpw));
+        switch (local_range) {
+        case 0: hipLaunchKernelGGL(k_cand_search<0>, grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride, planes,
+                                   planes_stride, top8, (const int32_t *)nullptr, nb, fallback_cost, start, fallback,
+                                   count, list); break;
+        case 1: hipLaunchKernelGGL(k_cand_search<1>, grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride, planes,
+                                   planes_stride, top8, (const int32_t *)nullptr, nb, fallback_cost, start, fallback,
+                                   count, list); break;
+        case 2: hipLaunchKernelGGL(k_cand_search<2>, grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride, planes,
+                                   planes_stride, top8, (const int32_t *)nullptr, nb, fallback_cost, start, fallback,
+                                   count, list); break;
+        case 3: hipLaunchKernelGGL(k_cand_search<3>, grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride, planes,
+                                   planes_stride, top8, (const int32_t *)nullptr, nb, fallback_cost, start, fallback,
+                                   count, list); break;
+        default: hipLaunchKernelGGL(k_cand_search<4>, grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride, planes,
+                                    planes_stride, top8, (const int32_t *)nullptr, nb, fallback_cost, start, fallback,
+                                    count, list); break;
+        }
+        LAUNCH_CHECK("k_cand_search");
+    } else {
+        hipLaunchKernelGGL(k_cand_centres, dim3((unsigned)((nb * LS_NCAND + 255) / 256)), dim3(256), 0, ctx->stream,
+                           top8, nb, centres);
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb * (LS_NCAND + 1) + 255) / 256)), dim3(256), 0,
+                           ctx->stream, keys8, (size_t)nb * (LS_NCAND + 1), ~0ull);
+        LAUNCH_CHECK("learned_search setup");
+        rh264_search_params q = *p;
+        q.range = local_range;
+        for (int k = 0; k < LS_NCAND; k++) {
+            rc = launch_full<16, 16>(ctx, &q, nframes, cur, frame_stride, planes, planes_stride,
+                                     centres + 2 * (size_t)k * nb, nullptr, keys8 + (size_t)k * nb);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_cand_select, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keys8,
+                           centres, nb, local_range, fallback_cost, start, fallback, count, list);
     }
-    hipLaunchKernelGGL(k_cand_select, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keys8, centres,
-                       nb, local_range, fallback_cost, start, fallback, count, list);
     const long lanes = nb * fs_groups<0>(2 * p->range + 1);
     hipLaunchKernelGGL((k_full_search_list<16, 16, 0>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
                        ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,
@@ -2394,7 +2525,6 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     hipLaunchKernelGGL(k_fallback_merge, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, list, count,
                        keysf, p->range, start);
     LAUNCH_CHECK("learned_search select");
-    (void)nblk;
     if (ev) HIP_TRY(hipEventRecord(ev[2], ctx->stream));
     rc = launch_refine<16, 16>(ctx, p, nframes, cur, frame_stride, planes, planes_stride, nullptr, nullptr, start,
                                nullptr, nullptr, nullptr, mv, cost, nullptr);

@@ -195,7 +195,7 @@ __global__ void k_pool_bwd(const float *__restrict__ df, const uint16_t *__restr
     dz[i] = f2bf(a[i] != 0 ? g : 0.0f);
 }
 
-// head: logits[cell][k] = bh[k] + sum_c Wh[k][c] f[cell][c]
+// head: logits[cell][k] = sum_c bf16(Wh[k][c]) bf16(f[cell][c]) + bh[k]
 __global__ void k_head_fwd(const float *__restrict__ f, const float *__restrict__ p, long ncell,
                            float *__restrict__ logits)
 {
@@ -205,9 +205,9 @@ __global__ void k_head_fwd(const float *__restrict__ f, const float *__restrict_
     const int k = (int)(i - cell * K);
     const float *w = p + OFF_WH + (size_t)k * C;
     const float *fc = f + (size_t)cell * C;
-    float acc = p[OFF_BH + k];
-    for (int c = 0; c < C; c++) acc += w[c] * fc[c];
-    logits[i] = acc;
+    float acc = 0.0f;
+    for (int c = 0; c < C; c++) acc += bf2f(f2bf(w[c])) * bf2f(f2bf(fc[c])); // BF16 operands, as on MFMA
+    logits[i] = acc + p[OFF_BH + k];
 }
 
 // softmax cross-entropy per cell (one wave per cell): dlogits = (softmax - onehot)
@@ -328,11 +328,19 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+constexpr int WF_HEAD = 3 * 9 * C * C; // BF16 head weights [K][C] follow the conv layers in wf
+constexpr int WF_COUNT = WF_HEAD + K * C;
+
 __global__ void k_refresh_bf16(const float *__restrict__ p, uint16_t *__restrict__ wf, uint16_t *__restrict__ wb)
 {
-    // 3 layers x [32 co][9 tap][32 ci] -> wf[layer][tap][co][ci], wb[layer][8-tap][ci][co]
+    // 3 layers x [32 co][9 tap][32 ci] -> wf[layer][tap][co][ci], wb[layer][8-tap][ci][co];
+    // then the head [289][32] -> wf[WF_HEAD + k * 32 + c]
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 3 * C * 9 * C) return;
+    if (i >= WF_COUNT) return;
+    if (i >= WF_HEAD) {
+        wf[i] = f2bf(p[OFF_WH + (i - WF_HEAD)]);
+        return;
+    }
     const int l = i / (C * 9 * C), j = i - l * C * 9 * C;
     const int co = j / (9 * C), rem = j - co * 9 * C, t = rem / C, ci = rem - t * C;
     const float w = p[OFF_W2 + l * (C * 9 * C + C) + j];
@@ -341,42 +349,38 @@ __global__ void k_refresh_bf16(const float *__restrict__ p, uint16_t *__restrict
     wb[(size_t)l * 9 * C * C + ((size_t)(8 - t) * C + ci) * C + co] = b;
 }
 
-// top-8 bins per cell by repeated argmax (ties: lowest bin), one wave per cell.
+// Ranking of the proposals: each logit z of bin b gets the 32-bit key
+// (order(z) & ~511) | (511 - b), order() = the order-preserving unsigned image of
+// the FP32 bits.  The top-8 are the 8 largest keys, largest first: by logit with
+// its 9 lowest order bits dropped, ties to the lower bin.  One wave per cell.
+__device__ __forceinline__ uint32_t rank_key(float z, int bin)
+{
+    uint32_t u = __float_as_uint(z);
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return (u & ~511u) | (uint32_t)(511 - bin);
+}
+
 __global__ void k_topk(const float *__restrict__ logits, long ncell, int32_t *__restrict__ top)
 {
     const long cell = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (cell >= ncell) return;
     const float *z = logits + (size_t)cell * K;
-    float v[5];
+    uint32_t v[5];
 #pragma unroll
     for (int j = 0; j < 5; j++) {
         const int k = lane + 64 * j;
-        v[j] = k < K ? z[k] : -INFINITY;
+        v[j] = k < K ? rank_key(z[k], k) : 0u;
     }
     for (int r = 0; r < TOPK; r++) {
-        float bv = -INFINITY;
-        int bk = 0x7fffffff;
+        uint32_t m = v[0];
 #pragma unroll
-        for (int j = 0; j < 5; j++) {
-            const int k = lane + 64 * j;
-            if (k < K && (v[j] > bv || (v[j] == bv && k < bk))) {
-                bv = v[j];
-                bk = k;
-            }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            const float ov = __shfl_xor(bv, off, 64);
-            const int ok = __shfl_xor(bk, off, 64);
-            if (ov > bv || (ov == bv && ok < bk)) {
-                bv = ov;
-                bk = ok;
-            }
-        }
-        if (lane == 0) top[(size_t)cell * TOPK + r] = bk;
+        for (int j = 1; j < 5; j++) m = max(m, v[j]);
+        for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+        if (lane == 0) top[(size_t)cell * TOPK + r] = 511 - (int)(m & 511u);
 #pragma unroll
         for (int j = 0; j < 5; j++)
-            if (lane + 64 * j == bk) v[j] = -INFINITY;
+            if (v[j] == m) v[j] = 0u;
     }
 }
 
@@ -384,10 +388,10 @@ __global__ void k_topk(const float *__restrict__ logits, long ncell, int32_t *__
 // fused inference: one workgroup per tile of 8x8 cells (16x16 pixels at 1/8
 // resolution).  The tile's x0 window, a1..a4 (with the halo each following 3x3
 // layer needs: 22, 20, 18, 16 pixels square) and the pooled features stay in
-// LDS; the 3x3 layers run on v_mfma_f32_16x16x32_bf16 (one K step = one tap, all
-// 32 input channels), the head on v_mfma_f32_16x16x4_f32 and the top-8 selection
-// in registers.  Same arithmetic as the layer-by-layer kernels: BF16 activations
-// and conv weights, FP32 accumulation, zero padding at every layer's border.
+// LDS; the convolutions and the head run on v_mfma_f32_16x16x32_bf16 (for a 3x3
+// layer one K step = one tap, all 32 input channels), the top-8 selection in
+// registers.  Same arithmetic as the layer-by-layer kernels: BF16 activations,
+// weights and features, FP32 accumulation, zero padding at every layer's border.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
@@ -397,8 +401,8 @@ constexpr int S4 = 2 * FT, S3 = S4 + 2, S2 = S4 + 4, S1 = S4 + 6; // a4..a1 wind
 constexpr int SX = 2 * S1 + 1;                                    // x0 window side (1/4 res)
 constexpr int PSB = 80;                                           // LDS bytes per pixel (64 + 16 pad)
 constexpr int LDS_A = S1 * S1 * PSB, LDS_B = S2 * S2 * PSB;       // a1/a3 and a2/a4
-constexpr int LDS_X = FT * FT * C * 4;                            // x0 window, then pooled features
-static_assert(SX * SX * 4 <= LDS_X, "x0 window fits the feature buffer");
+constexpr int LDS_X = FT * FT * PSB;                              // pooled features (BF16, PSB per cell)
+static_assert(SX * SX * 4 <= LDS_B, "the x0 window fits the a2/a4 buffer");
 
 __device__ __forceinline__ float4_t mfma_bf16_k32(u32x4 a, u32x4 b, float4_t c)
 {
@@ -409,7 +413,7 @@ __device__ __forceinline__ float4_t mfma_bf16_k32(u32x4 a, u32x4 b, float4_t c)
 // Epilogue of one 16-pixel tile of a conv layer: bias, ReLU, zero outside the
 // image, BF16, four channels (8 bytes) per lane and half.
 __device__ __forceinline__ void conv_store(uint8_t *out, int pc, bool valid, bool inside, const float4_t acc[2],
-                                           const float *bias, int q)
+                                           const float bias[2][4], int q)
 {
     if (!valid) return;
 #pragma unroll
@@ -417,7 +421,7 @@ __device__ __forceinline__ void conv_store(uint8_t *out, int pc, bool valid, boo
         uint32_t w[2];
 #pragma unroll
         for (int i = 0; i < 4; i += 2) {
-            float v0 = acc[h][i] + bias[16 * h + 4 * q + i], v1 = acc[h][i + 1] + bias[16 * h + 4 * q + i + 1];
+            float v0 = acc[h][i] + bias[h][i], v1 = acc[h][i + 1] + bias[h][i + 1];
             v0 = inside && v0 > 0.0f ? v0 : 0.0f;
             v1 = inside && v1 > 0.0f ? v1 : 0.0f;
             w[i / 2] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
@@ -426,8 +430,17 @@ __device__ __forceinline__ void conv_store(uint8_t *out, int pc, bool valid, boo
     }
 }
 
+__device__ __forceinline__ void load_bias(const float *__restrict__ bias, int q, float b[2][4])
+{
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) b[h][i] = bias[16 * h + 4 * q + i];
+}
+
 // 3x3 conv, 32 -> 32, from window `in` (side SIN) to window `out` (side SOUT =
-// SIN - 2) whose origin lies `m` pixels up-left of the tile origin (oy, ox).
+// SIN - 2) whose origin lies `m` pixels up-left of the tile origin (oy, ox).  A
+// wave takes two 16-pixel tiles at a time (four independent accumulators).
 template <int SIN>
 __device__ __forceinline__ void conv32_window(const uint8_t *in, uint8_t *out, const uint16_t *__restrict__ wt,
                                               const float *__restrict__ bias, int oy, int ox, int m, int h8, int w8,
@@ -440,63 +453,63 @@ __device__ __forceinline__ void conv32_window(const uint8_t *in, uint8_t *out, c
     for (int t = 0; t < 9; t++)
 #pragma unroll
         for (int h = 0; h < 2; h++) wa[t][h] = *(const u32x4 *)(wt + ((size_t)t * C + 16 * h + col) * C + 8 * q);
-    for (int tile = wave; tile < NT; tile += 4) {
-        const int px = tile * 16 + col;
-        const bool valid = px < NPX;
-        const int pc = valid ? px : NPX - 1;
-        const int y = pc / SOUT, x = pc - y * SOUT;
-        float4_t acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    float b[2][4];
+    load_bias(bias, q, b);
+    for (int tile = wave; tile < NT; tile += 8) {
+        int pc[2], y[2], x[2];
+        bool valid[2];
 #pragma unroll
-        for (int t = 0; t < 9; t++) {
-            const u32x4 b = *(const u32x4 *)(in + ((y + t / 3) * SIN + x + t % 3) * PSB + 16 * q);
-            acc[0] = mfma_bf16_k32(wa[t][0], b, acc[0]);
-            acc[1] = mfma_bf16_k32(wa[t][1], b, acc[1]);
+        for (int u = 0; u < 2; u++) {
+            const int px = (tile + 4 * u) * 16 + col;
+            valid[u] = px < NPX;
+            pc[u] = valid[u] ? px : NPX - 1;
+            y[u] = pc[u] / SOUT;
+            x[u] = pc[u] - y[u] * SOUT;
         }
-        const int gy = oy - m + y, gx = ox - m + x;
-        conv_store(out, pc, valid, gy >= 0 && gy < h8 && gx >= 0 && gx < w8, acc, bias, q);
-    }
-}
-
-// (v, k) ordering of the top-8: larger logit first, ties to the lower bin.
-__device__ __forceinline__ bool topk_before(float va, int ka, float vb, int kb)
-{
-    return va > vb || (va == vb && ka < kb);
-}
-
-__device__ __forceinline__ void topk_cas(float &va, int &ka, float &vb, int &kb)
-{
-    if (topk_before(vb, kb, va, ka)) {
-        const float tv = va;
-        const int tk = ka;
-        va = vb;
-        ka = kb;
-        vb = tv;
-        kb = tk;
-    }
-}
-
-// Merge this lane's sorted top-8 with the one of lane ^ xr: the larger of L[i]
-// and P[7-i] form a bitonic sequence holding the top 8, sorted by a bitonic merge.
-__device__ __forceinline__ void topk_merge(float v[8], int k[8], int xr)
-{
-    float pv[8];
-    int pk[8];
+        float4_t acc[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        pv[j] = __shfl_xor(v[j], xr, 64);
-        pk[j] = __shfl_xor(k[j], xr, 64);
-    }
+        for (int t = 0; t < 9; t++)
 #pragma unroll
-    for (int j = 0; j < 8; j++)
-        if (topk_before(pv[7 - j], pk[7 - j], v[j], k[j])) {
-            v[j] = pv[7 - j];
-            k[j] = pk[7 - j];
+            for (int u = 0; u < 2; u++) {
+                const u32x4 bv = *(const u32x4 *)(in + ((y[u] + t / 3) * SIN + x[u] + t % 3) * PSB + 16 * q);
+                acc[u][0] = mfma_bf16_k32(wa[t][0], bv, acc[u][0]);
+                acc[u][1] = mfma_bf16_k32(wa[t][1], bv, acc[u][1]);
+            }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int gy = oy - m + y[u], gx = ox - m + x[u];
+            conv_store(out, pc[u], valid[u], gy >= 0 && gy < h8 && gx >= 0 && gx < w8, acc[u], b, q);
         }
+    }
+}
+
+// Sorted (descending) ranking keys: compare-and-swap, and the merge of this lane's
+// top-8 with lane ^ xr's (the larger of A[i] and B[7-i] is a bitonic sequence
+// holding the top 8, sorted by a bitonic merge).
+__device__ __forceinline__ void key_cas(uint32_t &a, uint32_t &b)
+{
+    const uint32_t hi = max(a, b), lo = min(a, b);
+    a = hi;
+    b = lo;
+}
+
+__device__ __forceinline__ void bitonic8_desc(uint32_t t[8])
+{
 #pragma unroll
     for (int d = 4; d >= 1; d >>= 1)
 #pragma unroll
         for (int j = 0; j < 8; j++)
-            if ((j & d) == 0) topk_cas(v[j], k[j], v[j + d], k[j + d]);
+            if ((j & d) == 0) key_cas(t[j], t[j + d]);
+}
+
+__device__ __forceinline__ void topk_merge_lanes(uint32_t t[8], int xr)
+{
+    uint32_t o[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] = (uint32_t)__shfl_xor((int)t[j], xr, 64);
+#pragma unroll
+    for (int j = 0; j < 8; j++) t[j] = max(t[j], o[7 - j]);
+    bitonic8_desc(t);
 }
 
 __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ x0, const float *__restrict__ p,
@@ -506,7 +519,7 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
 {
     __shared__ __align__(16) uint8_t s_a[LDS_A];
     __shared__ __align__(16) uint8_t s_b[LDS_B];
-    __shared__ __align__(16) uint8_t s_x[LDS_X];
+    __shared__ __align__(16) uint8_t s_f[LDS_X];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int q = lane >> 4, col = lane & 15;
     const int tpi = tiles_x * tiles_y;
@@ -515,13 +528,15 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     const int h8 = h4 / 2, w8 = w4 / 2, hc = h8 / 2, wc = w8 / 2;
     const int oy = ty * S4, ox = tx * S4; // tile origin at 1/8 resolution
 
-    // x0 window: 1/4-res rows/cols 2(o - 3) - 1 .. + SX - 1, zero outside the image
+    // x0 window (in the a2/a4 buffer): 1/4-res rows/cols 2(o - 3) - 1 .. + SX - 1,
+    // zero outside the image
+    uint32_t *s_x = (uint32_t *)s_b;
     {
         const uint32_t *xw = (const uint32_t *)x0 + (size_t)img * h4 * w4;
         const int y0 = 2 * (oy - 3) - 1, x0c = 2 * (ox - 3) - 1;
         for (int i = tid; i < SX * SX; i += 256) {
             const int r = i / SX, c = i - r * SX, gy = y0 + r, gx = x0c + c;
-            ((uint32_t *)s_x)[i] = gy >= 0 && gy < h4 && gx >= 0 && gx < w4 ? xw[(size_t)gy * w4 + gx] : 0u;
+            s_x[i] = gy >= 0 && gy < h4 && gx >= 0 && gx < w4 ? xw[(size_t)gy * w4 + gx] : 0u;
         }
     }
     __syncthreads();
@@ -537,23 +552,25 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
                 const float *w = p + OFF_W1 + (16 * h + col) * 18 + k;
                 wa[h][j] = k < 18 ? (uint32_t)f2bf(w[0]) | ((uint32_t)f2bf(w[1]) << 16) : 0u;
             }
+        float b[2][4];
+        load_bias(p + OFF_B1, q, b);
         constexpr int NPX = S1 * S1, NT = (NPX + 15) / 16;
         for (int tile = wave; tile < NT; tile += 4) {
             const int px = tile * 16 + col;
             const bool valid = px < NPX;
             const int pc = valid ? px : NPX - 1;
             const int y = pc / S1, x = pc - y * S1;
-            u32x4 b;
+            u32x4 bv;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int t = 4 * q + j; // tap of k = 8q + 2j (both channels in one dword)
-                b[j] = t < 9 ? ((const uint32_t *)s_x)[(2 * y + t / 3) * SX + 2 * x + t % 3] : 0u;
+                bv[j] = t < 9 ? s_x[(2 * y + t / 3) * SX + 2 * x + t % 3] : 0u;
             }
             float4_t acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-            acc[0] = mfma_bf16_k32(wa[0], b, acc[0]);
-            acc[1] = mfma_bf16_k32(wa[1], b, acc[1]);
+            acc[0] = mfma_bf16_k32(wa[0], bv, acc[0]);
+            acc[1] = mfma_bf16_k32(wa[1], bv, acc[1]);
             const int gy = oy - 3 + y, gx = ox - 3 + x;
-            conv_store(s_a, pc, valid, gy >= 0 && gy < h8 && gx >= 0 && gx < w8, acc, p + OFF_B1, q);
+            conv_store(s_a, pc, valid, gy >= 0 && gy < h8 && gx >= 0 && gx < w8, acc, b, q);
         }
     }
     __syncthreads();
@@ -564,68 +581,66 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     conv32_window<S3>(s_a, s_b, wf + 2 * 9 * C * C, p + OFF_B4, oy, ox, 0, h8, w8, lane, wave);
     __syncthreads();
 
-    // 2x2 average pool -> f[cell][c] (FP32, in the x0 buffer)
-    float *s_f = (float *)s_x;
-    for (int i = tid; i < FT * FT * C; i += 256) {
-        const int cell = i / C, c = i - cell * C, cy = cell / FT, cx = cell - cy * FT;
+    // 2x2 average pool -> f[cell][c] as BF16 (PSB bytes per cell); two channels per thread
+    for (int i = tid; i < FT * FT * C / 2; i += 256) {
+        const int cell = i / (C / 2), c = 2 * (i - cell * (C / 2)), cy = cell / FT, cx = cell - cy * FT;
         const uint8_t *a = s_b + ((2 * cy) * S4 + 2 * cx) * PSB + 2 * c;
-        const float v00 = bf2f(*(const uint16_t *)a), v01 = bf2f(*(const uint16_t *)(a + PSB));
-        const float v10 = bf2f(*(const uint16_t *)(a + S4 * PSB)), v11 = bf2f(*(const uint16_t *)(a + S4 * PSB + PSB));
-        s_f[i] = 0.25f * (v00 + v01 + v10 + v11);
+        uint32_t packed = 0;
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const float v00 = bf2f(*(const uint16_t *)(a + 2 * e)), v01 = bf2f(*(const uint16_t *)(a + PSB + 2 * e));
+            const float v10 = bf2f(*(const uint16_t *)(a + S4 * PSB + 2 * e));
+            const float v11 = bf2f(*(const uint16_t *)(a + S4 * PSB + PSB + 2 * e));
+            packed |= (uint32_t)f2bf(0.25f * (v00 + v01 + v10 + v11)) << (16 * e);
+        }
+        *(uint32_t *)(s_f + cell * PSB + 2 * c) = packed;
     }
     __syncthreads();
 
-    // head + top-8: wave w takes cells 16w .. 16w+15 of the tile (lane column = cell)
+    // head + top-8: wave w takes cells 16w .. 16w+15 of the tile (MFMA column = cell)
     const int cell = 16 * wave + col, cy = cell / FT, cx = cell - cy * FT;
     const int gcy = ty * FT + cy, gcx = tx * FT + cx;
     const bool cvalid = gcy < hc && gcx < wc;
     const size_t gcell = ((size_t)img * hc + (cvalid ? gcy : 0)) * wc + (cvalid ? gcx : 0);
-    float fb[8];
+    const u32x4 fv = *(const u32x4 *)(s_f + cell * PSB + 16 * q); // B[k = 8q + i][cell]
+    const uint16_t *whb = wf + WF_HEAD;
+    uint32_t top[8];
 #pragma unroll
-    for (int s8 = 0; s8 < 8; s8++) fb[s8] = s_f[cell * C + 4 * s8 + q];
-    float tv[8];
-    int tk[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        tv[j] = -INFINITY;
-        tk[j] = 0x7fffffff;
-    }
+    for (int j = 0; j < 8; j++) top[j] = 0u;
     constexpr int NBT = (K + 15) / 16;
+#pragma unroll 2
     for (int bt = 0; bt < NBT; bt++) {
         const int arow = 16 * bt + col;
+        const u32x4 wv = arow < K ? *(const u32x4 *)(whb + (size_t)arow * C + 8 * q) : u32x4{0, 0, 0, 0};
         float4_t acc = {0, 0, 0, 0};
-#pragma unroll
-        for (int s8 = 0; s8 < 8; s8++) {
-            const float a = arow < K ? p[OFF_WH + arow * C + 4 * s8 + q] : 0.0f;
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, fb[s8], acc, 0, 0, 0);
-        }
+        acc = mfma_bf16_k32(wv, fv, acc);
+        uint32_t nk[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int bin = 16 * bt + 4 * q + i;
-            if (bin >= K) continue;
-            const float z = acc[i] + p[OFF_BH + bin];
-            if (logits && cvalid) logits[gcell * K + bin] = z;
-            if (z > tv[7]) { // bins arrive in increasing order: ties keep the earlier bin
-                tv[7] = z;
-                tk[7] = bin;
-#pragma unroll
-                for (int j = 7; j > 0; j--)
-                    if (tv[j] > tv[j - 1]) {
-                        const float t0 = tv[j];
-                        const int t1 = tk[j];
-                        tv[j] = tv[j - 1];
-                        tk[j] = tk[j - 1];
-                        tv[j - 1] = t0;
-                        tk[j - 1] = t1;
-                    }
-            }
+            const bool ok = bin < K;
+            const float z = acc[i] + (ok ? p[OFF_BH + bin] : 0.0f);
+            if (logits && cvalid && ok) logits[gcell * K + bin] = z;
+            nk[i] = ok ? rank_key(z, bin) : 0u;
         }
+        // sort the four new keys, keep the top 8 of the union
+        key_cas(nk[0], nk[1]);
+        key_cas(nk[2], nk[3]);
+        key_cas(nk[0], nk[2]);
+        key_cas(nk[1], nk[3]);
+        key_cas(nk[1], nk[2]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) top[4 + i] = max(top[4 + i], nk[3 - i]);
+        bitonic8_desc(top);
     }
-    topk_merge(tv, tk, 16);
-    topk_merge(tv, tk, 32);
+    topk_merge_lanes(top, 16);
+    topk_merge_lanes(top, 32);
     if (top8 && cvalid && q == 0) {
-        *(int4 *)(top8 + gcell * TOPK) = make_int4(tk[0], tk[1], tk[2], tk[3]);
-        *(int4 *)(top8 + gcell * TOPK + 4) = make_int4(tk[4], tk[5], tk[6], tk[7]);
+        int b8[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) b8[j] = 511 - (int)(top[j] & 511u);
+        *(int4 *)(top8 + gcell * TOPK) = make_int4(b8[0], b8[1], b8[2], b8[3]);
+        *(int4 *)(top8 + gcell * TOPK + 4) = make_int4(b8[4], b8[5], b8[6], b8[7]);
     }
 }
 
@@ -680,7 +695,7 @@ int rh264_mvnet_create(rh264_ctx *ctx, int max_n, int max_w, int max_h, rh264_mv
         size_t bytes;
     } allocs[] = {
         {(void **)&net->p, NPARAM * 4}, {(void **)&net->g, NPARAM * 4}, {(void **)&net->m, NPARAM * 4},
-        {(void **)&net->v, NPARAM * 4}, {(void **)&net->wf, 3 * 9 * C * C * 2}, {(void **)&net->wb, 3 * 9 * C * C * 2},
+        {(void **)&net->v, NPARAM * 4}, {(void **)&net->wf, WF_COUNT * 2}, {(void **)&net->wb, 3 * 9 * C * C * 2},
         {(void **)&net->x0, n * px4 * 2 * 2}, {(void **)&net->a1, n * px8 * C * 2}, {(void **)&net->a2, n * px8 * C * 2},
         {(void **)&net->a3, n * px8 * C * 2}, {(void **)&net->a4, n * px8 * C * 2}, {(void **)&net->d4, n * px8 * C * 2},
         {(void **)&net->d3, n * px8 * C * 2}, {(void **)&net->d2, n * px8 * C * 2}, {(void **)&net->d1, n * px8 * C * 2},
@@ -705,7 +720,7 @@ int rh264_mvnet_set_params(rh264_ctx *ctx, rh264_mvnet *net, const float *host, 
 {
     if (count != (size_t)NPARAM) return fail("mvnet_set_params: %zu parameters, expected %d", count, NPARAM);
     HIP_TRY(hipMemcpyAsync(net->p, host, NPARAM * 4, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_refresh_bf16, dim3(nblocks(3 * 9 * C * C, 256)), dim3(256), 0, ctx->stream, net->p, net->wf,
+    hipLaunchKernelGGL(k_refresh_bf16, dim3(nblocks(WF_COUNT, 256)), dim3(256), 0, ctx->stream, net->p, net->wf,
                        net->wb);
     LAUNCH_CHECK("k_refresh_bf16");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -837,7 +852,7 @@ int rh264_mvnet_adam(rh264_ctx *ctx, rh264_mvnet *net, float lr)
     const float c1 = 1.0f - powf(b1, (float)net->step), c2 = 1.0f - powf(b2, (float)net->step);
     hipLaunchKernelGGL(k_adam, dim3(nblocks(NPARAM, 256)), dim3(256), 0, ctx->stream, net->p, net->g, net->m, net->v,
                        lr, b1, b2, eps, c1, c2);
-    hipLaunchKernelGGL(k_refresh_bf16, dim3(nblocks(3 * 9 * C * C, 256)), dim3(256), 0, ctx->stream, net->p, net->wf,
+    hipLaunchKernelGGL(k_refresh_bf16, dim3(nblocks(WF_COUNT, 256)), dim3(256), 0, ctx->stream, net->p, net->wf,
                        net->wb);
     LAUNCH_CHECK("mvnet adam");
     return 0;
