@@ -35,16 +35,15 @@ def cpu_baseline(frames, lam, budget_s=12.0):
     cfg = o.cfg(W, H, PAD, 16, 16, R, 0, lam)
     t0 = time.perf_counter()
     done = 0
-    while done < NFRAMES - 1:
-        pl = o.planes(frames[done], W, H, PAD, fused=True)
-        o.plain_search_frame(cfg, frames[done + 1], pl, nthreads=cores)
+    while time.perf_counter() - t0 < budget_s:
+        f = done % (NFRAMES - 1)  # cycles over the clip's 15 P-frames
+        pl = o.planes(frames[f], W, H, PAD, fused=True)
+        o.plain_search_frame(cfg, frames[f + 1], pl, nthreads=cores)
         done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
     dt = time.perf_counter() - t0
     return {"value": round(W * H * done / dt / 1e6, 3), "unit": "Mpx/s", "cores": cores, "kind": "port",
-            "sample": f"{done} P-frame(s) of the 1080p clip: plane build + +-32 full search + qpel refine, "
-                      f"{cores} OpenMP threads, {dt:.1f} s"}
+            "sample": f"{done} P-frame searches over the 1080p clip's 15 P-frames: plane build + +-32 full "
+                      f"search + qpel refine, {cores} OpenMP threads, {dt:.1f} s"}
 
 
 MAX_REGRET = 0.01  # the learned path counts only at <= 1% mean SATD-cost regret
