@@ -893,6 +893,300 @@ __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
 }
 
 // ---------------------------------------------------------------------------
+// sub-pel refinement of 16x16 blocks on the matrix cores
+// ---------------------------------------------------------------------------
+//
+// One wave per block; the ring walk's control flow is wave-uniform (scalar).  The
+// sixteen 4x4 Hadamards of a candidate are two 16x16x16 f16 matrix products with
+// FP32 accumulation:  X = P * Hb (row transforms), Y = Hb * X (column transforms),
+// Hb = diag(H4, H4, H4, H4), H4 the symmetric Sylvester matrix (a row order of the
+// reference's hadamard_1d, transform.rs:631-655; sum |.| does not depend on it).
+// Every value is an integer that fp16 / fp32 hold exactly: P enters as p + 1024
+// (fp16 spacing is 1 on [1024, 2048)), the accumulator starts at -4096 on the DC
+// columns so X = P * Hb with |X| <= 1020 < 2048, and the second product starts at
+// -Hb S Hb (the source block's transform, computed once), so Y = Hb (P - S) Hb with
+// |Y| <= 8160.  sum |Y| is the encoder-domain SATD (mb16.rs:6027-6074).
+//
+// Lane l holds pixel row l%16, columns 4(l/16)..+3 of the block (the A operand of
+// the first product); the first product's result is already in the layout of the
+// second one's B operand.  Each wave stages the four planes around its start
+// vector in LDS (integer offsets -4..+3 of the start on both axes, i.e. quarter-pel
+// offsets -16..+15); farther candidates read the planes in HBM.  Per-candidate
+// tables live in the lanes of a VGPR and are read with v_readlane: the operand
+// offsets of the 16 quarter phases, the mv-bit counts of the offsets -32..+31
+// from the start on each axis, and the 128-entry lambda-rate table.
+
+typedef _Float16 mx_h4 __attribute__((ext_vector_type(4)));
+typedef float mx_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t mx_u2 __attribute__((ext_vector_type(2)));
+
+constexpr int MX_WL = 4;                 // window reach in full pels
+constexpr int MX_ROWS = 2 * MX_WL + 16;  // staged rows per plane
+constexpr int MX_DW = 7;                 // staged dwords per row (28 bytes)
+constexpr int MX_PITCH = 40;             // row pitch in bytes (10 dwords: conflict-free reads)
+constexpr int MX_PLANE = MX_ROWS * MX_PITCH;
+constexpr int MX_WIN = 4 * MX_PLANE;     // bytes per wave
+
+// The quarter-phase operand table (qtab) as byte offsets inside a staged window:
+// plane * MX_PLANE + row offset * MX_PITCH + column offset, operand a in the low
+// half, operand b in the high half.
+#define MXOFF(pa, ax, ay, pb, bx, by) \
+    ((uint32_t)((pa) * MX_PLANE + (ay) * MX_PITCH + (ax)) | ((uint32_t)((pb) * MX_PLANE + (by) * MX_PITCH + (bx)) << 16))
+__constant__ uint32_t c_mxoff[16] = {
+    /* fx=0: fy=0..3 */ MXOFF(0, 0, 0, 0, 0, 0), MXOFF(0, 0, 0, 2, 0, 0), MXOFF(2, 0, 0, 2, 0, 0), MXOFF(0, 0, 1, 2, 0, 0),
+    /* fx=1 */          MXOFF(0, 0, 0, 1, 0, 0), MXOFF(1, 0, 0, 2, 0, 0), MXOFF(2, 0, 0, 3, 0, 0), MXOFF(1, 0, 1, 2, 0, 0),
+    /* fx=2 */          MXOFF(1, 0, 0, 1, 0, 0), MXOFF(1, 0, 0, 3, 0, 0), MXOFF(3, 0, 0, 3, 0, 0), MXOFF(1, 0, 1, 3, 0, 0),
+    /* fx=3 */          MXOFF(0, 1, 0, 1, 0, 0), MXOFF(1, 0, 0, 2, 1, 0), MXOFF(2, 1, 0, 3, 0, 0), MXOFF(1, 0, 1, 2, 1, 0),
+};
+#undef MXOFF
+
+struct MxConst {
+    mx_h4 hb; // Hb entries of this lane: B operand of the first product, A operand of the second
+    mx_f4 c1; // accumulator start of the first product
+};
+
+__device__ __forceinline__ MxConst mx_const(int lane)
+{
+    MxConst K;
+    const int q = lane >> 4, n = lane & 15;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        // Hb[4q + i][n]: nonzero inside the 4x4 diagonal blocks, H4[i][n & 3] = (-1)^popc(i & n)
+        const float h = (q == (n >> 2)) ? ((__popc(i & n & 3) & 1) ? -1.0f : 1.0f) : 0.0f;
+        K.hb[i] = (_Float16)h;
+        K.c1[i] = (n & 3) == 0 ? -4096.0f : 0.0f;
+    }
+    return K;
+}
+
+// Sum over the wave, returned as a scalar: row sums with DPP, then rows 0+1 and
+// 2+3 with row_bcast:15, the total into lane 63 with row_bcast:31.
+__device__ __forceinline__ int mx_wave_sum(int v)
+{
+    v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true); // row_half_mirror
+    v += __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true); // row_mirror
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false); // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false); // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Hb (P - S) Hb, P = the lane's 4 predicted pixels, c2 = -(Hb S Hb) in accumulator
+// layout (zero to transform S itself).  Returns the lane's four coefficients.
+__device__ __forceinline__ mx_f4 mx_transform(uint32_t p, const MxConst &K, mx_f4 c2)
+{
+    mx_u2 a2;
+    a2[0] = __builtin_amdgcn_perm(0x64646464u, p, 0x04010400u); // (1024 + p0, 1024 + p1) as fp16
+    a2[1] = __builtin_amdgcn_perm(0x64646464u, p, 0x04030402u); // (1024 + p2, 1024 + p3)
+    const mx_h4 a = __builtin_bit_cast(mx_h4, a2);
+    const mx_f4 x = __builtin_amdgcn_mfma_f32_16x16x16f16(a, K.hb, K.c1, 0, 0, 0);
+    mx_u2 b2;
+    b2[0] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x[0], x[1]));
+    b2[1] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x[2], x[3]));
+    const mx_h4 b = __builtin_bit_cast(mx_h4, b2);
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(K.hb, b, c2, 0, 0, 0);
+}
+
+__device__ __forceinline__ int mx_satd(uint32_t p, const MxConst &K, mx_f4 c2)
+{
+    const mx_f4 y = mx_transform(p, K, c2);
+    const float s = (fabsf(y[0]) + fabsf(y[1])) + (fabsf(y[2]) + fabsf(y[3]));
+    return mx_wave_sum((int)s);
+}
+
+// 4 bytes at a dword-aligned LDS address plus a byte shift of 0..3.
+__device__ __forceinline__ uint32_t mx_lds4(const uint8_t *p4, int shift)
+{
+    const uint32_t *p = (const uint32_t *)p4;
+    return __builtin_amdgcn_alignbyte(p[1], p[0], (uint32_t)shift);
+}
+
+// Wave-uniform state of one block's refinement.
+struct MxBlock {
+    const uint8_t *pl;  // plane set
+    int pw, ph;
+    size_t pn;          // bytes per plane
+    int ix, iy;         // padded integer position of the start vector
+    int sx, sy;         // start vector (quarter pel, multiple of 4)
+    int mpx, mpy;       // rate predictor
+    int xoff;           // byte offset of column ix - MX_WL inside the staged rows
+};
+
+// Cost of a candidate outside the staged window: the plane set in HBM, clamped at
+// its border, and the rate computed directly.  Kept out of line so that its table
+// reads and address arithmetic stay off the common path.
+__device__ __attribute__((noinline)) int mx_cost_far(const MxBlock &B, const MxConst &K, mx_f4 c2,
+                                                      const int32_t *s_lr, int lane, int rx, int ry)
+{
+    const int r = lane & 15, qd = lane >> 4;
+    const int mx = B.sx + rx, my = B.sy + ry;
+    const int fx = mx & 3, fy = my & 3;
+    const uint32_t q = qtab(fx * 4 + fy);
+    const int bpx = B.ix + (rx >> 2), bpy = B.iy + (ry >> 2);
+    const bool inb = bpx >= 0 && bpy >= 0 && bpx + 17 <= B.pw && bpy + 17 <= B.ph;
+    const uint8_t *plA = B.pl + B.pn * (q & 15);
+    const int xa = bpx + ((q >> 4) & 15) + 4 * qd, ya = bpy + ((q >> 8) & 15) + r;
+    uint32_t v = inb ? load4_fast(plA, B.pw, xa, ya) : load4_clamped(plA, B.pw, B.ph, xa, ya);
+    if ((fx | fy) & 1) {
+        const uint8_t *plB = B.pl + B.pn * ((q >> 12) & 15);
+        const int xb = bpx + ((q >> 16) & 15) + 4 * qd, yb = bpy + ((q >> 20) & 15) + r;
+        const uint32_t w = inb ? load4_fast(plB, B.pw, xb, yb) : load4_clamped(plB, B.pw, B.ph, xb, yb);
+        v = __builtin_amdgcn_lerp(v, w, 0x01010101u);
+    }
+    const int s = mx_satd(v, K, c2);
+    return __builtin_amdgcn_readfirstlane(s + s_lr[mvbits(mx - B.mpx) + mvbits(my - B.mpy)]);
+}
+
+template <int PAT>
+__global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, const uint8_t *__restrict__ cur,
+                                                     size_t cur_stride, const uint8_t *__restrict__ planes,
+                                                     size_t planes_stride, const uint64_t *__restrict__ keys,
+                                                     const int32_t *__restrict__ centres,
+                                                     const int32_t *__restrict__ start_mv,
+                                                     const int32_t *__restrict__ mvps, int32_t *__restrict__ int_mv,
+                                                     int32_t *__restrict__ int_cost, int32_t *__restrict__ out_mv,
+                                                     int32_t *__restrict__ out_cost, int32_t *__restrict__ out_evals)
+{
+    __shared__ int32_t s_lr[RH264_LR_N];
+    __shared__ __align__(16) uint8_t s_win[4 * MX_WIN];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
+    __syncthreads();
+
+    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + wave; // wave-uniform
+    if (b >= nblk) return;
+    const int frame = blockIdx.y;
+    const size_t boff = (size_t)frame * nblk + b;
+    const int lx = (b % nbx) << 4, ly = (b / nbx) << 4;
+    const int r = lane & 15, qd = lane >> 4; // pixel row, column quad of this lane
+
+    MxBlock B;
+    B.pw = p.cw + 2 * p.pad;
+    B.ph = p.ch + 2 * p.pad;
+    B.pn = (size_t)B.pw * B.ph;
+    B.pl = planes + (size_t)frame * planes_stride;
+    B.mpx = __builtin_amdgcn_readfirstlane(mvps ? mvps[2 * boff] : 0);
+    B.mpy = __builtin_amdgcn_readfirstlane(mvps ? mvps[2 * boff + 1] : 0);
+    const uint32_t src = *(const uint32_t *)(cur + (size_t)frame * cur_stride + (size_t)(ly + r) * p.cw + lx + 4 * qd);
+
+    int bx, by;
+    if (keys) {
+        const uint64_t key = keys[boff];
+        const int NC = 2 * p.range + 1;
+        const uint32_t idx = __builtin_amdgcn_readfirstlane((uint32_t)key);
+        const int t = (int)(idx / (uint32_t)NC), c = (int)(idx % (uint32_t)NC);
+        const int cx = __builtin_amdgcn_readfirstlane(centres ? centres[2 * boff] : 0);
+        const int cy = __builtin_amdgcn_readfirstlane(centres ? centres[2 * boff + 1] : 0);
+        bx = cx + 4 * (c - p.range);
+        by = cy + 4 * (t - p.range);
+        if (lane == 0) {
+            if (int_mv) { int_mv[2 * boff] = bx; int_mv[2 * boff + 1] = by; }
+            if (int_cost) int_cost[boff] = (int32_t)(key >> 32);
+        }
+    } else {
+        bx = __builtin_amdgcn_readfirstlane(start_mv[2 * boff]);
+        by = __builtin_amdgcn_readfirstlane(start_mv[2 * boff + 1]);
+    }
+    // the walk runs on offsets (rx, ry) from S = the start's integer position, so
+    // the phase of a candidate is (rx & 3, ry & 3) and its integer offset (rx >> 2, ry >> 2)
+    const int fx0 = bx & 3, fy0 = by & 3;
+    B.sx = bx - fx0;
+    B.sy = by - fy0;
+    B.ix = lx + (B.sx >> 2) + p.pad;
+    B.iy = ly + (B.sy >> 2) + p.pad;
+
+    // stage the window: planes F, H, V, C, rows iy-4 .. iy+19, columns from wxa
+    // (ix-4 rounded down to a dword) for 28 bytes
+    const int wx0 = B.ix - MX_WL, wy0 = B.iy - MX_WL;
+    const int wxa = wx0 & ~3;
+    B.xoff = wx0 - wxa;
+    uint8_t *win = s_win + wave * MX_WIN;
+    {
+        // lane -> (row in a group of 9, dword of the row); lane 63 idles
+        const int srow = lane / MX_DW, sd = lane - srow * MX_DW;
+        const bool inside = wxa >= 0 && wy0 >= 0 && wxa + 4 * MX_DW <= B.pw && wy0 + MX_ROWS <= B.ph;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint8_t *plk = B.pl + B.pn * k;
+#pragma unroll
+            for (int r0 = 0; r0 < MX_ROWS; r0 += 64 / MX_DW) {
+                const int row = r0 + srow;
+                if (srow < 64 / MX_DW && row < MX_ROWS) {
+                    const int x = wxa + 4 * sd, y = wy0 + row;
+                    const uint32_t v = inside ? *(const uint32_t *)(plk + (size_t)y * B.pw + x)
+                                              : load4_clamped(plk, B.pw, B.ph, x, y);
+                    *(uint32_t *)(win + k * MX_PLANE + row * MX_PITCH + 4 * sd) = v;
+                }
+            }
+        }
+    }
+    // lane tables: operand offsets per phase, mv bits per offset -32..31, lambda-rate
+    const uint32_t t_off = c_mxoff[lane & 15];
+    const int t_bx = mvbits(B.sx + lane - 32 - B.mpx), t_by = mvbits(B.sy + lane - 32 - B.mpy);
+    const int t_lr = s_lr[lane];
+    // the rate table covers every in-window candidate when the largest bit count
+    // there (at an end of each axis' offset range, mvbits is monotone in |d|) is < 64
+    const bool rate_fast = max(mvbits(B.sx - 16 - B.mpx), mvbits(B.sx + 15 - B.mpx)) +
+                               max(mvbits(B.sy - 16 - B.mpy), mvbits(B.sy + 15 - B.mpy)) < 64;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const uint8_t *lwin = win + r * MX_PITCH + 4 * qd; // this lane's row and column quad
+    const MxConst K = mx_const(lane);
+    mx_f4 c2 = -mx_transform(src, K, mx_f4{0.0f, 0.0f, 0.0f, 0.0f});
+
+    auto cost_of = [&](int rx, int ry) -> int {
+        if (!rate_fast || (unsigned)(rx + 4 * MX_WL) >= (unsigned)(8 * MX_WL) ||
+            (unsigned)(ry + 4 * MX_WL) >= (unsigned)(8 * MX_WL))
+            return __builtin_amdgcn_readfirstlane(mx_cost_far(B, K, c2, s_lr, lane, rx, ry));
+        const int ph = ((rx & 3) << 2) | (ry & 3);
+        const uint32_t qo = __builtin_amdgcn_readlane(t_off, ph);
+        // byte offsets (wave-uniform) of the operands' top-left samples in the staged
+        // window (one-plane phases list the same operand twice: lerp(a, a) = a); the
+        // lane adds its own row and column quad
+        const int o0 = ((ry >> 2) + MX_WL) * MX_PITCH + B.xoff + MX_WL + (rx >> 2);
+        const int oa = o0 + (int)(qo & 0xffffu), ob = o0 + (int)(qo >> 16);
+        const uint32_t v = __builtin_amdgcn_lerp(mx_lds4(lwin + (oa & ~3), oa & 3), mx_lds4(lwin + (ob & ~3), ob & 3),
+                                                 0x01010101u);
+        const int bits = __builtin_amdgcn_readlane(t_bx, rx + 32) + __builtin_amdgcn_readlane(t_by, ry + 32);
+        return mx_satd(v, K, c2) + __builtin_amdgcn_readlane(t_lr, bits);
+    };
+
+    int cx = fx0, cy = fy0; // current best, as an offset from S
+    int best = cost_of(cx, cy);
+    int evals = 1;
+    if (PAT >= 0) {
+        constexpr int NRING = (PAT & 1) ? 4 : 8;
+        constexpr int RX[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+        constexpr int RY[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+#pragma unroll
+        for (int step = 2; step >= 1; step--) {
+            // terminates: every extra pass strictly lowers a cost bounded below by 0
+            for (;;) {
+                bool improved = false;
+#pragma unroll
+                for (int i = 0; i < NRING; i++) {
+                    const int mx = cx + step * RX[i], my = cy + step * RY[i];
+                    const int cc = cost_of(mx, my);
+                    evals++;
+                    if (cc < best) { best = cc; cx = mx; cy = my; improved = true; }
+                }
+                if (!improved || (PAT & 2)) break;
+            }
+        }
+    }
+    if (lane == 0) {
+        out_mv[2 * boff] = B.sx + cx;
+        out_mv[2 * boff + 1] = B.sy + cy;
+        out_cost[boff] = best;
+        if (out_evals) out_evals[boff] = evals;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // batched multi-stream search
 // ---------------------------------------------------------------------------
 //
@@ -1635,6 +1929,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->ls_cap = 0;
     c->fs_kernel = 0;
     c->refine_lds = 1;
+    c->refine_mfma = 1;
     c->mvnet_fused = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -1819,6 +2114,24 @@ static int launch_refine(rh264_ctx *ctx, const rh264_search_params *p, int nfram
 {
     constexpr int GPW = RefShape<BW, BH>::GPW;
     const int nblk = (p->cw / BW) * (p->ch / BH);
+    if constexpr (BW == 16 && BH == 16) {
+        if (ctx->refine_mfma) {
+            const dim3 grid((unsigned)((nblk + 3) / 4), nframes);
+#define MX_LAUNCH(PT)                                                                                        \
+    hipLaunchKernelGGL(k_refine_mx16<PT>, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes, planes_stride, \
+                       keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost, evals)
+            switch (p->subpel_pat) {
+            case 0: MX_LAUNCH(0); break;
+            case 1: MX_LAUNCH(1); break;
+            case 2: MX_LAUNCH(2); break;
+            case 3: MX_LAUNCH(3); break;
+            default: MX_LAUNCH(-1); break;
+            }
+#undef MX_LAUNCH
+            LAUNCH_CHECK("k_refine_mx16");
+            return 0;
+        }
+    }
     dim3 grid((unsigned)((nblk + GPW - 1) / GPW), nframes);
     if (RefShape<BW, BH>::NSB >= 8 && ctx->refine_lds)
         hipLaunchKernelGGL((k_subpel_refine<BW, BH, RefShape<BW, BH>::NSB >= 8>), grid, dim3(256), 0, ctx->stream, *p,
@@ -1977,6 +2290,10 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
     }
     if (!strcmp(knob, "refine_lds")) {
         ctx->refine_lds = value ? 1 : 0;
+        return 0;
+    }
+    if (!strcmp(knob, "refine_mfma")) {
+        ctx->refine_mfma = value ? 1 : 0;
         return 0;
     }
     if (!strcmp(knob, "mvnet_fused")) {

@@ -186,6 +186,46 @@ def test_subpel_refine_from_given_starts(dev):
         assert tuple(got["mv"][b]) == mv and got["cost"][b] == c and got["evals"][b] == ev, b
 
 
+def _smooth_pair(cw, ch, dx, dy):
+    """A smooth reference (long-wavelength sinusoid) and a current frame that is the
+    reference moved by (dx, dy) pels: cur(x, y) = ref(x + dx, y + dy)."""
+    yy, xx = np.mgrid[0:ch + 64, 0:cw + 64]
+    big = np.rint(128 + 90 * np.sin(2 * np.pi * xx / 97.0) * np.cos(2 * np.pi * yy / 83.0)).astype(np.uint8)
+    ref = np.ascontiguousarray(big[16:16 + ch, 16:16 + cw])
+    cur = np.ascontiguousarray(big[16 + dy:16 + dy + ch, 16 + dx:16 + dx + cw])
+    return np.stack([ref, cur])
+
+
+@pytest.mark.parametrize("pat,refine_mfma", [(0, 1), (1, 1), (0, 0)])
+def test_subpel_refine_long_walks(dev, pat, refine_mfma):
+    """Ring walks that travel far from their start (smooth content, lambda 0, start at
+    zero, true motion (+7, +3) pels): candidates beyond the window staged around the
+    start read the planes in HBM.  Bit-exact with the oracle, evaluation counts included."""
+    cw, ch, pad = 176, 144, 32
+    frames = _smooth_pair(cw, ch, 7, 3)
+    dev.tune("refine_mfma", refine_mfma)
+    try:
+        cur, ref = dev.upload(frames[1]), dev.upload(frames[0])
+        pl = accel.build_planes(dev, ref, 1, cw, ch, pad)
+        p = accel.search_params(cw, ch, pad, 16, 16, 8, pat, 0.0)
+        ms = accel.MotionSearch(dev, p)
+        starts = np.zeros((ms.nblk, 2), np.int32)
+        ds = dev.upload(starts)
+        ms.run_refine(cur, pl, 1, ds)
+        got = ms.results(1, with_evals=True)
+    finally:
+        dev.tune("refine_mfma", 1)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 8, pat, 0.0)
+    opl = O.planes(frames[0], cw, ch, pad)
+    far = 0
+    for b in range(ms.nblk):
+        bx, by = (b % (cw // 16)) * 16, (b // (cw // 16)) * 16
+        mv, c, ev = O.subpel_refine_block(cfg, frames[1], opl, bx, by, 0, 0)
+        assert tuple(got["mv"][b]) == mv and got["cost"][b] == c and got["evals"][b] == ev, b
+        far += max(abs(mv[0]), abs(mv[1])) > 16
+    assert far > ms.nblk // 2  # most walks leave the +-4 pel window
+
+
 @pytest.mark.parametrize("fs_kernel", [0, 1, 2])
 def test_full_search_kernel_variants(dev, fs_kernel):
     """Every integer-search kernel variant gives the oracle's result (CIF, +-16, and a
