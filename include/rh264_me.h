@@ -165,7 +165,12 @@ int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases
  * (default), 1 = v_sad_u8 ring with 2 candidates per lane, 2 = v_sad_u8 ring
  * with 4 candidates per lane; "refine_lds" 1 (default) = the refinement of
  * 16x16/16x8/8x16 blocks stages each block's +-3 pel plane window in LDS, 0 =
- * it reads the plane sets in HBM; "mvnet_fused" 1 (default) = network inference
+ * it reads the plane sets in HBM; "refine_mfma" 1 (default) = the refinement of
+ * 16x16 blocks computes its Hadamards on the matrix cores (one wave per block),
+ * 0 = the lane-per-4x4 kernel above; "cand_quad" 1 (default) = the learned search's
+ * candidate windows (local range <= 3) use quad-SADs, 0 = v_sad_u8 lanes;
+ * "fallback_v" 1 (default) = its fall-back full search uses the v_sad_u8 x2
+ * variant, 0 = quad-SADs; "mvnet_fused" 1 (default) = network inference
  * in one fused kernel per tile, 0 = layer by layer.  Every search variant returns
  * identical results; the two network paths agree up to FP32 summation order. */
 int rh264_tune(rh264_ctx *ctx, const char *knob, int value);

@@ -37,6 +37,8 @@ struct rh264_ctx {
     int fs_kernel;                // integer search: 0 = quad-SAD, 1 = v_sad_u8 x2, 2 = v_sad_u8 x4
     int refine_lds;               // refinement stages the +-3 pel window in LDS (blocks of >= 8 sub-blocks)
     int refine_mfma;              // 16x16 refinement: Hadamards as matrix-core products, one wave per block
+    int cand_quad;                // learned search: candidate windows with quad-SADs (local range <= 3)
+    int fallback_v;               // learned search: fall-back full search, 0 = quad-SAD, 1 = v_sad_u8 x2
     int mvnet_fused;              // network inference: 1 = one fused kernel per tile, 0 = layer by layer
     std::vector<hipEvent_t> prof; // RH264_PROF_EV events per profiled search
     std::vector<int> prof_np;     // phases recorded in each profiled search

@@ -340,8 +340,10 @@ struct FsJob {
     int lx, ly, cx, cy, mpx, mpy, g;
 };
 
-template <int BW, int BH>
-__device__ __forceinline__ uint64_t fs_qsad_key(const FsJob &J, const int32_t *s_lr)
+// LSRC: the source block is staged in LDS at lsrc (BH rows of SD dwords) instead of
+// read from J.srcp.
+template <int BW, int BH, bool LSRC = false>
+__device__ __forceinline__ uint64_t fs_qsad_key(const FsJob &J, const int32_t *s_lr, const uint32_t *lsrc = nullptr)
 {
     using S = FsShape<BW, BH>;
     const int R = J.R, NC = 2 * R + 1, g = J.g;
@@ -349,7 +351,12 @@ __device__ __forceinline__ uint64_t fs_qsad_key(const FsJob &J, const int32_t *s
 #pragma unroll
     for (int k = 0; k < BH; k++)
 #pragma unroll
-        for (int i = 0; i < S::SD; i++) L.src[k][i] = *(const uint32_t *)(J.srcp + (size_t)k * J.cw + 4 * i);
+        for (int i = 0; i < S::SD; i++) {
+            if constexpr (LSRC)
+                L.src[k][i] = lsrc[k * S::SD + i];
+            else
+                L.src[k][i] = *(const uint32_t *)(J.srcp + (size_t)k * J.cw + 4 * i);
+        }
     const int dx0 = -R + 4 * g;
     const int px0 = J.lx + (J.cx >> 2) + J.pad + dx0; // padded column of candidate dx0
     const int py0 = J.ly + (J.cy >> 2) + J.pad - R;   // padded row of candidate dy = -R
@@ -1850,6 +1857,128 @@ __global__ __launch_bounds__(256) void k_cand_search(rh264_search_params p, cons
     }
 }
 
+// The same search with quad-SADs: lane (block slot, candidate k, column group g)
+// owns window columns 4g..4g+3 (four packed u16 SADs per window row, one
+// v_qsad_pk_u16_u8 per source dword and reference row); the blocks of a workgroup
+// stage their sources in LDS.  Keys and tie-breaks are those of k_cand_search.
+template <int RL, bool FAST>
+__device__ __forceinline__ void cand_rows_q(const uint32_t (&s)[16][4], const uint8_t *__restrict__ F, int pw, int ph,
+                                            int x, int y0, uint64_t (&acc)[2 * RL + 1])
+{
+    constexpr int NP = 2 * RL + 1, NJ = 16 + 2 * RL;
+    const int sh = x & 3;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        uint32_t w[5];
+        if (FAST) {
+            const uint32_t *rp = (const uint32_t *)(F + (size_t)(y0 + j) * pw + (x & ~3));
+            uint32_t a[6];
+#pragma unroll
+            for (int i = 0; i < 6; i++) a[i] = rp[i];
+#pragma unroll
+            for (int i = 0; i < 5; i++) w[i] = __builtin_amdgcn_alignbyte(a[i + 1], a[i], (uint32_t)sh);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 5; i++) w[i] = load4_clamped(F, pw, ph, x + 4 * i, y0 + j);
+        }
+#pragma unroll
+        for (int t = 0; t < NP; t++) {
+            const int r = j - t;
+            if (r < 0 || r >= 16) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                acc[t] = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)w[i + 1] << 32) | w[i], s[r][i], acc[t]);
+        }
+    }
+}
+
+template <int RL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_cand_search_q(rh264_search_params p, const uint8_t *__restrict__ cur, size_t cur_stride,
+                     const uint8_t *__restrict__ planes, size_t planes_stride, const int32_t *__restrict__ top8,
+                     const int32_t *__restrict__ mvps, long nb, int32_t fb_cost, int32_t *__restrict__ start,
+                     int32_t *__restrict__ flag, unsigned *__restrict__ count, int32_t *__restrict__ list)
+{
+    constexpr int NC = 2 * RL + 1, NG = (NC + 3) / 4, LPB = LS_NCAND * NG, BPW = 256 / LPB;
+    static_assert(BPW * LPB == 256 && BPW % 4 == 0, "whole blocks per workgroup");
+    __shared__ int32_t s_lr[RH264_LR_N];
+    __shared__ unsigned long long s_key[BPW];
+    __shared__ uint32_t s_src[BPW][64];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
+    if (threadIdx.x < BPW) s_key[threadIdx.x] = ~0ull;
+    const int nbx = p.cw / 16, nblk = nbx * (p.ch / 16);
+    // stage the sources: 64 threads per block, one dword each (row t/4, dword t%4)
+#pragma unroll
+    for (int s0 = 0; s0 < BPW; s0 += 4) {
+        const int sl = s0 + (threadIdx.x >> 6), t = threadIdx.x & 63;
+        const long g = (long)blockIdx.x * BPW + sl;
+        if (g < nb) {
+            const int frame = (int)(g / nblk), b = (int)(g - (long)frame * nblk);
+            const uint8_t *src = cur + (size_t)frame * cur_stride + (size_t)((b / nbx) * 16 + (t >> 2)) * p.cw +
+                                 (b % nbx) * 16 + 4 * (t & 3);
+            s_src[sl][t] = *(const uint32_t *)src;
+        }
+    }
+    __syncthreads();
+    const int slot = threadIdx.x / LPB, li = threadIdx.x - slot * LPB;
+    const long gb = (long)blockIdx.x * BPW + slot;
+    if (gb < nb) {
+        const int k = li / NG, g = li - k * NG;
+        const int frame = (int)(gb / nblk), b = (int)(gb - (long)frame * nblk);
+        const int lx = (b % nbx) * 16, ly = (b / nbx) * 16;
+        const int c = top8[gb * LS_NCAND + k];
+        const int cx = ((c % LS_BINS) - LS_BINS / 2) * 16, cy = ((c / LS_BINS) - LS_BINS / 2) * 16;
+        const int mpx = mvps ? mvps[2 * gb] : 0, mpy = mvps ? mvps[2 * gb + 1] : 0;
+        const int pw = p.cw + 2 * p.pad, ph = p.ch + 2 * p.pad;
+        const uint8_t *F = planes + (size_t)frame * planes_stride;
+        uint32_t sr[16][4];
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) sr[r][i] = s_src[slot][4 * r + i];
+        const int x = lx + (cx >> 2) + p.pad + 4 * g - RL; // padded column of the lane's first candidate
+        const int y0 = ly + (cy >> 2) + p.pad - RL;        // padded row of the window's first row
+        uint64_t acc[NC];
+#pragma unroll
+        for (int t = 0; t < NC; t++) acc[t] = 0;
+        if (x >= 0 && (x & ~3) + 24 <= pw && y0 >= 0 && y0 + 16 + 2 * RL <= ph)
+            cand_rows_q<RL, true>(sr, F, pw, ph, x, y0, acc);
+        else
+            cand_rows_q<RL, false>(sr, F, pw, ph, x, y0, acc);
+        int rxb[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) rxb[q] = mvbits(cx + 4 * (4 * g + q - RL) - mpx);
+        uint64_t best = ~0ull;
+#pragma unroll
+        for (int t = 0; t < NC; t++) {
+            const int ryb = mvbits(cy + 4 * (t - RL) - mpy);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int col = 4 * g + q;
+                const uint32_t cost = (uint32_t)(acc[t] >> (16 * q)) & 0xffffu;
+                const uint64_t key = ((uint64_t)(cost + (uint32_t)s_lr[rxb[q] + ryb]) << 32) | ((uint64_t)k << 8) |
+                                     (uint64_t)(t * NC + col);
+                if (col < NC && key < best) best = key;
+            }
+        }
+        atomicMin(&s_key[slot], (unsigned long long)best);
+    }
+    __syncthreads();
+    if (threadIdx.x < BPW) {
+        const long g = (long)blockIdx.x * BPW + threadIdx.x;
+        if (g < nb) {
+            const uint64_t key = s_key[threadIdx.x];
+            const int kk = (int)((key >> 8) & 255), idx = (int)(key & 255);
+            const int ck = top8[g * LS_NCAND + kk];
+            start[2 * g] = ((ck % LS_BINS) - LS_BINS / 2) * 16 + 4 * (idx % NC - RL);
+            start[2 * g + 1] = ((ck / LS_BINS) - LS_BINS / 2) * 16 + 4 * (idx / NC - RL);
+            const bool fb = (int64_t)(key >> 32) > (int64_t)fb_cost;
+            flag[g] = fb ? 1 : 0;
+            if (fb) list[atomicAdd(count, 1u)] = (int32_t)g;
+        }
+    }
+}
+
 // Full search of the listed blocks only (frame-major global block indices).
 template <int BW, int BH, int V>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
@@ -1930,6 +2059,8 @@ int rh264_init(int device, rh264_ctx **out)
     c->fs_kernel = 0;
     c->refine_lds = 1;
     c->refine_mfma = 1;
+    c->cand_quad = 1;
+    c->fallback_v = 1;
     c->mvnet_fused = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -2290,6 +2421,14 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
     }
     if (!strcmp(knob, "refine_lds")) {
         ctx->refine_lds = value ? 1 : 0;
+        return 0;
+    }
+    if (!strcmp(knob, "fallback_v")) {
+        ctx->fallback_v = value ? 1 : 0;
+        return 0;
+    }
+    if (!strcmp(knob, "cand_quad")) {
+        ctx->cand_quad = value ? 1 : 0;
         return 0;
     }
     if (!strcmp(knob, "refine_mfma")) {
@@ -2795,7 +2934,24 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, count, (size_t)1, 0u);
-    if (local_range <= 4) {
+    if (local_range <= 3 && ctx->cand_quad) {
+        // one pass over all candidates with quad-SADs (same result as the kernels below)
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keysf,
+                           (size_t)nb, ~0ull);
+#define CQ_LAUNCH(RL)                                                                                             \
+    hipLaunchKernelGGL(k_cand_search_q<RL>, dim3((unsigned)((nb + 256 / (LS_NCAND * ((2 * RL + 4) / 4)) - 1) /        \
+                                                          (256 / (LS_NCAND * ((2 * RL + 4) / 4))))),                  \
+                       dim3(256), 0, ctx->stream, *p, cur, frame_stride, planes, planes_stride, top8,                 \
+                       (const int32_t *)nullptr, nb, fallback_cost, start, fallback, count, list)
+        switch (local_range) {
+        case 0: CQ_LAUNCH(0); break;
+        case 1: CQ_LAUNCH(1); break;
+        case 2: CQ_LAUNCH(2); break;
+        default: CQ_LAUNCH(3); break;
+        }
+#undef CQ_LAUNCH
+        LAUNCH_CHECK("k_cand_search_q");
+    } else if (local_range <= 4) {
         // one pass over all candidates (the per-candidate searches below give the same result)
         hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keysf,
                            (size_t)nb, ~0ull);
@@ -2835,10 +2991,19 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
         hipLaunchKernelGGL(k_cand_select, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keys8,
                            centres, nb, local_range, fallback_cost, start, fallback, count, list);
     }
-    const long lanes = nb * fs_groups<0>(2 * p->range + 1);
-    hipLaunchKernelGGL((k_full_search_list<16, 16, 0>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
-                       ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,
-                       (const int32_t *)nullptr, keysf);
+    // the fall-back list is short (a few % of the blocks): the v_sad_u8 variant with two
+    // candidates per lane gives it twice the lanes of the quad-SAD one
+    if (ctx->fallback_v == 1) {
+        const long lanes = nb * fs_groups<1>(2 * p->range + 1);
+        hipLaunchKernelGGL((k_full_search_list<16, 16, 1>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
+                           ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,
+                           (const int32_t *)nullptr, keysf);
+    } else {
+        const long lanes = nb * fs_groups<0>(2 * p->range + 1);
+        hipLaunchKernelGGL((k_full_search_list<16, 16, 0>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
+                           ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,
+                           (const int32_t *)nullptr, keysf);
+    }
     hipLaunchKernelGGL(k_fallback_merge, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, list, count,
                        keysf, p->range, start);
     LAUNCH_CHECK("learned_search select");

@@ -95,13 +95,25 @@ def test_backward_gradients(dev):
     net.close()
 
 
-@pytest.mark.parametrize("Rl", [2, 1, 5])
-def test_learned_search_recosting_is_exact(dev, Rl):
+@pytest.mark.parametrize("Rl,cand_quad,fallback_v", [(2, 1, 1), (1, 1, 1), (3, 1, 0), (4, 1, 1), (5, 1, 1),
+                                                      (2, 0, 0)])
+def test_learned_search_recosting_is_exact(dev, Rl, cand_quad, fallback_v):
     """Every step after the network is exact: oracle local searches around the GPU's
     eight centres, first minimum, fall-back full search, ring refine.  Run once
     without fall-back and once with the threshold at the median candidate cost so
-    both paths are exercised.  Local ranges up to 4 take the one-pass candidate
-    kernel, larger ones the per-candidate integer searches."""
+    both paths are exercised.  Local ranges up to 3 take the quad-SAD candidate
+    kernel (cand_quad 1) or the v_sad_u8 one, 4 the latter, larger ones the
+    per-candidate integer searches; both fall-back search variants are covered."""
+    dev.tune("cand_quad", cand_quad)
+    dev.tune("fallback_v", fallback_v)
+    try:
+        _learned_recost_case(dev, Rl)
+    finally:
+        dev.tune("cand_quad", 1)
+        dev.tune("fallback_v", 1)
+
+
+def _learned_recost_case(dev, Rl):
     cw, ch, pad, Rf = 176, 144, 32, 16
     frames = synth.hd_clip(3, seed=31, w=cw, h=ch, nrects=3)
     nf = 2
