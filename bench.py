@@ -23,7 +23,13 @@ sys.path.insert(0, ROOT)
 
 W, H, NFRAMES, R, PAD, QP, SEED = 1920, 1088, 16, 32, 32, 26, 2026
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 157.3  # published FP32 vector peak = 256 CUs x 64 lanes x 2.4 GHz x 4 ops
+# byte abs-diff-adds: v_sad_u8 does 4 per lane per cycle at full rate, v_qsad_pk_u16_u8 16 at
+# quarter rate: 64 per SIMD per cycle x 1024 SIMDs x 2.4 GHz
+VALU_PEAK_TOPS = 157.3
+# HBM bytes of one k_full_search<16,16> launch of this workload, from rocprofv3 --pmc
+# FETCH_SIZE (54,345 KiB, x2: a plain 512 MiB copy kernel reads 262,160 KiB) and
+# WRITE_SIZE (1,912 KiB, x1)
+FULL_SEARCH_TRAFFIC = (54345 * 2 + 1912) * 1024
 
 
 def cpu_baseline(frames, lam, budget_s=12.0):
@@ -193,7 +199,8 @@ def main():
                    "device": dname},
         "roofline": {"bound": "valu", "kernel": "k_full_search<16,16>",
                      "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                     "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                     "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": FULL_SEARCH_TRAFFIC,
+                     "traffic_unit": "bytes per launch", "algorithmic_bytes": nf * (W * H + (W + 2 * PAD) * (H + 2 * PAD)),
                      "work": f"{ops} abs-diff-adds per launch ((2R+1)^2*256 per 16x16 block)",
                      "kernel_ms": round(full_avg, 4), "refine_ms": round(float(np.mean(refine_ms)), 4)},
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
