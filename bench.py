@@ -286,7 +286,91 @@ def copy_probe(mib):
 
 
 def bench_lookahead4k(args):
-    raise SystemExit("lookahead4k workload not built yet")
+    """4K mb-tree lookahead over a 40-frame window: 3840x2160 frames downsampled to
+    half resolution (1920x1080, 8x8 blocks = one per 16x16 MB), intra AC-SATD and the
+    seeded cascade diamond against the previous frame (the reference's HalfRes mode,
+    mbtree.rs:61-297), then int32 backward propagation over the 40 frames.  One step
+    = the whole window.  Mpx/s = 3840*2160*40 / step time.  The first three frames'
+    costs and the whole window's propagation are checked against the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle.oracle import Oracle
+    from rusty_h264_amd import accel, synth
+    w, h, n = 3840, 2160, 40
+    mb_w, mb_h = w // 16, h // 16
+    t_gen = time.perf_counter()
+    frames = synth.hd_clip(n, seed=SEED + 5, w=w, h=h)
+    t_gen = time.perf_counter() - t_gen
+    with accel.Device(0) as dev:
+        dname = dev.name()
+        fb = dev.upload(frames)
+        mt = accel.MbTree(dev, w, h, n)
+        lib = dev.lib
+
+        def step(ev=None):
+            if ev is not None:
+                dev.record(ev)
+            accel._check(lib.rh264_mbtree_downsample(dev.ctx, int(fb), w * h, n, w, h, mt.half.ptr), "downsample")
+            if ev is not None:
+                dev.record(ev + 1)
+            accel._check(lib.rh264_mbtree_costs(dev.ctx, mt.half.ptr, n, mb_w, mb_h, mt.intra.ptr, mt.inter.ptr,
+                                                mt.mv.ptr), "costs")
+            if ev is not None:
+                dev.record(ev + 2)
+            accel._check(lib.rh264_mbtree_propagate_i32(dev.ctx, n, mb_w, mb_h, mt.intra.ptr, mt.inter.ptr,
+                                                        mt.mv.ptr, mt.prop.ptr), "propagate")
+            if ev is not None:
+                dev.record(ev + 3)
+
+        for _ in range(args.warmup):
+            step()
+        dev.sync()
+        t0 = time.perf_counter()
+        dev.record(0)
+        for _ in range(args.steps):
+            step()
+        dev.record(1)
+        dev.sync()
+        wall = time.perf_counter() - t0
+        ms = dev.elapsed_ms(0, 1) / args.steps
+        step(ev=2)  # one more, with phase events
+        dev.sync()
+        phases = {"downsample": dev.elapsed_ms(2, 3), "costs": dev.elapsed_ms(3, 4), "propagate": dev.elapsed_ms(4, 5)}
+        res = mt.results()
+    # parity spot check: costs of frames 0..2 and the propagation from the GPU's own costs
+    o = Oracle(fast=True)
+    halves = [o.downsample2x(frames[f], w, h) for f in range(3)]
+    ok = all(np.array_equal(res["half"][f], halves[f]) for f in range(3))
+    for f in range(3):
+        a, b, m = o.frame_costs_half(halves[f], halves[f - 1] if f else None, mb_w, mb_h)
+        ok = ok and np.array_equal(res["intra"][f], a) and np.array_equal(res["inter"][f], b) and \
+            np.array_equal(res["mv"][f], m)
+    ok = ok and np.array_equal(res["prop"], o.propagate_i32(res["intra"], res["inter"], res["mv"], mb_w, mb_h))
+    out = {"metric": "4K mb-tree lookahead Mpx/s (40-frame window, half-res 8x8 costs, int32 propagation)",
+           "value": round(w * h * n / (ms / 1e3) / 1e6, 2), "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+           "config": {"workload": "3840x2160, 40 frames, HalfRes mb-tree (1920x1080, 8x8 blocks, 1 ref = previous "
+                                  "frame, seeded 4-point cascade diamond steps 8/4/2/1), int32 backward propagation",
+                      "device": dname},
+           "phase_ms": {k: round(v, 4) for k, v in phases.items()},
+           "parity_vs_oracle": "bit-exact" if ok else "MISMATCH",
+           "wall_ms_per_step": round(wall * 1e3 / args.steps, 4), "generate_s": round(t_gen, 1)}
+    if not args.no_cpu:
+        # the same window on the host: the oracle's downsample and frame costs (one C
+        # call per frame, frames spread over a thread pool), then its propagation
+        cores = min(os.cpu_count() or 1, 16)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(cores) as ex:
+            hv = list(ex.map(lambda f: o.downsample2x(frames[f], w, h), range(n)))
+            cs = list(ex.map(lambda f: o.frame_costs_half(hv[f], hv[f - 1] if f else None, mb_w, mb_h), range(n)))
+        o.propagate_i32(np.stack([c[0] for c in cs]), np.stack([c[1] for c in cs]), np.stack([c[2] for c in cs]),
+                        mb_w, mb_h)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(w * h * n / dt / 1e6, 3), "unit": "Mpx/s", "cores": cores,
+                               "kind": "port", "sample": f"the whole 40-frame window once (downsample, frame costs, "
+                                                         f"int32 propagation), {cores} threads, {dt:.1f} s"}
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -42,9 +42,17 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float bf2f(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u); // round to nearest even
-    return (uint16_t)(u >> 16);
+    // round to nearest even (v_cvt_pk_bf16_f32)
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
+
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// Two floats as packed BF16 (lo, hi), one conversion instruction.
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi)
+{
+    const bf16x2_t v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -424,7 +432,7 @@ __device__ __forceinline__ void conv_store(uint8_t *out, int pc, bool valid, boo
             float v0 = acc[h][i] + bias[h][i], v1 = acc[h][i + 1] + bias[h][i + 1];
             v0 = inside && v0 > 0.0f ? v0 : 0.0f;
             v1 = inside && v1 > 0.0f ? v1 : 0.0f;
-            w[i / 2] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+            w[i / 2] = f2bf2(v0, v1);
         }
         *(uint2 *)(out + pc * PSB + (16 * h + 4 * q) * 2) = make_uint2(w[0], w[1]);
     }
@@ -467,14 +475,41 @@ __device__ __forceinline__ void conv32_window(const uint8_t *in, uint8_t *out, c
             x[u] = pc[u] - y[u] * SOUT;
         }
         float4_t acc[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+        // B operands row by row of the 3x3 kernel: the next kernel row's six loads are
+        // in flight while the current row's twelve products run
+        const uint8_t *base[2];
 #pragma unroll
-        for (int t = 0; t < 9; t++)
+        for (int u = 0; u < 2; u++) base[u] = in + (y[u] * SIN + x[u]) * PSB + 16 * q;
+        u32x4 bv[2][3][2];
 #pragma unroll
-            for (int u = 0; u < 2; u++) {
-                const u32x4 bv = *(const u32x4 *)(in + ((y[u] + t / 3) * SIN + x[u] + t % 3) * PSB + 16 * q);
-                acc[u][0] = mfma_bf16_k32(wa[t][0], bv, acc[u][0]);
-                acc[u][1] = mfma_bf16_k32(wa[t][1], bv, acc[u][1]);
+        for (int tx = 0; tx < 3; tx++)
+#pragma unroll
+            for (int u = 0; u < 2; u++) bv[0][tx][u] = *(const u32x4 *)(base[u] + tx * PSB);
+#pragma unroll
+        for (int ty = 0; ty < 3; ty++) {
+            if (ty < 2) {
+#pragma unroll
+                for (int tx = 0; tx < 3; tx++)
+#pragma unroll
+                    for (int u = 0; u < 2; u++)
+                        bv[(ty + 1) & 1][tx][u] = *(const u32x4 *)(base[u] + ((ty + 1) * SIN + tx) * PSB);
             }
+#pragma unroll
+            for (int tx = 0; tx < 3; tx++)
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    acc[u][0] = mfma_bf16_k32(wa[3 * ty + tx][0], bv[ty & 1][tx][u], acc[u][0]);
+                    acc[u][1] = mfma_bf16_k32(wa[3 * ty + tx][1], bv[ty & 1][tx][u], acc[u][1]);
+                }
+        }
+        // keep that order: loads of rows 0 and 1, products of row 0, loads of row 2,
+        // products of rows 1 and 2 (mask 0x100 = LDS reads, 0x008 = MFMA)
+        __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);
 #pragma unroll
         for (int u = 0; u < 2; u++) {
             const int gy = oy - m + y[u], gx = ox - m + x[u];

@@ -29,7 +29,7 @@ def _oracle_gop(frames, cw, ch):
 
 
 @pytest.mark.parametrize("cw,ch,n,kind", [(64, 48, 4, "cif"), (352, 288, 6, "cif"), (176, 144, 5, "noise"),
-                                          (16, 16, 3, "cif"), (640, 368, 4, "hd")])
+                                          (16, 16, 3, "cif"), (640, 368, 4, "hd"), (3840, 2160, 3, "hd")])
 def test_mbtree_gop_bit_exact(dev, cw, ch, n, kind):
     if kind == "cif":
         frames = synth.cif_clip(n, seed=cw + n, w=cw, h=ch)
