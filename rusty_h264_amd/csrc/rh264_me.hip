@@ -1162,6 +1162,45 @@ __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, cons
         return mx_satd(v, K, c2) + __builtin_amdgcn_readlane(t_lr, bits);
     };
 
+    // Ring tables: lane i < 8 holds candidate i of the ring around the current centre
+    // -- its two operands' dword offsets in the window, their byte shifts and its
+    // lambda-rate cost, or -1 when the candidate lies outside the window -- built in
+    // a few vector instructions per ring (and again after a move), so a candidate
+    // costs four v_readlane instead of scalar address and rate arithmetic.
+    const int li8 = lane & 7;
+    const int lrx = (li8 < 2 || li8 >= 4) ? 1 - 2 * (li8 & 1) : 0;                      // RX[i]
+    const int lry = li8 < 2 ? 0 : li8 < 4 ? 1 - 2 * (li8 & 1) : ((li8 & 1) == ((li8 >> 1) & 1) ? 1 : -1); // RY[i]
+    struct Tab {
+        uint32_t ta, tb, sh;
+        int lr;
+    };
+    auto make_tab = [&](int ccx, int ccy, int step) -> Tab {
+        const int rx = ccx + step * lrx, ry = ccy + step * lry;
+        const bool inw = rate_fast && (unsigned)(rx + 4 * MX_WL) < (unsigned)(8 * MX_WL) &&
+                         (unsigned)(ry + 4 * MX_WL) < (unsigned)(8 * MX_WL);
+        const uint32_t qo = (uint32_t)__builtin_amdgcn_ds_bpermute((((rx & 3) << 2) | (ry & 3)) << 2, (int)t_off);
+        const int o0 = ((ry >> 2) + MX_WL) * MX_PITCH + B.xoff + MX_WL + (rx >> 2);
+        const int oa = o0 + (int)(qo & 0xffffu), ob = o0 + (int)(qo >> 16);
+        const int bits = __builtin_amdgcn_ds_bpermute((rx + 32) << 2, t_bx) +
+                         __builtin_amdgcn_ds_bpermute((ry + 32) << 2, t_by);
+        const int lrc = __builtin_amdgcn_ds_bpermute(bits << 2, t_lr);
+        Tab T;
+        T.ta = (uint32_t)(oa & ~3);
+        T.tb = (uint32_t)(ob & ~3);
+        T.sh = (uint32_t)((oa & 3) | ((ob & 3) << 8));
+        T.lr = inw ? lrc : -1;
+        return T;
+    };
+    auto eval_tab = [&](const Tab &T, int i, int rx, int ry) -> int {
+        const int lrc = __builtin_amdgcn_readlane(T.lr, i);
+        if (lrc < 0) return __builtin_amdgcn_readfirstlane(mx_cost_far(B, K, c2, s_lr, lane, rx, ry));
+        const uint32_t ta = __builtin_amdgcn_readlane(T.ta, i), tb = __builtin_amdgcn_readlane(T.tb, i);
+        const uint32_t sh = __builtin_amdgcn_readlane(T.sh, i);
+        const uint32_t v = __builtin_amdgcn_lerp(mx_lds4(lwin + ta, sh & 3), mx_lds4(lwin + tb, (sh >> 8) & 3),
+                                                 0x01010101u);
+        return mx_satd(v, K, c2) + lrc;
+    };
+
     int cx = fx0, cy = fy0; // current best, as an offset from S
     int best = cost_of(cx, cy);
     int evals = 1;
@@ -1174,12 +1213,19 @@ __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, cons
             // terminates: every extra pass strictly lowers a cost bounded below by 0
             for (;;) {
                 bool improved = false;
+                Tab T = make_tab(cx, cy, step);
 #pragma unroll
                 for (int i = 0; i < NRING; i++) {
                     const int mx = cx + step * RX[i], my = cy + step * RY[i];
-                    const int cc = cost_of(mx, my);
+                    const int cc = eval_tab(T, i, mx, my);
                     evals++;
-                    if (cc < best) { best = cc; cx = mx; cy = my; improved = true; }
+                    if (cc < best) {
+                        best = cc;
+                        cx = mx;
+                        cy = my;
+                        improved = true;
+                        if (i + 1 < NRING) T = make_tab(cx, cy, step); // the rest of the pass moves with the centre
+                    }
                 }
                 if (!improved || (PAT & 2)) break;
             }
