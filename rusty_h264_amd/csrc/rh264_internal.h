@@ -39,6 +39,7 @@ struct rh264_ctx {
     int refine_mfma;              // 16x16 refinement: Hadamards as matrix-core products, one wave per block
     int cand_quad;                // learned search: candidate windows with quad-SADs (local range <= 3)
     int fallback_v;               // learned search: fall-back full search, 0 = quad-SAD, 1 = v_sad_u8 x2
+    int planes_rw;                // plane build: 1 = register-window kernel, 0 = LDS tile kernel
     int mvnet_fused;              // network inference: 1 = one fused kernel per tile, 0 = layer by layer
     std::vector<hipEvent_t> prof; // RH264_PROF_EV events per profiled search
     std::vector<int> prof_np;     // phases recorded in each profiled search

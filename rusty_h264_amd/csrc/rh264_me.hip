@@ -230,6 +230,129 @@ __global__ void k_fill_u64(uint64_t *p, size_t n, uint64_t v)
     if (i < n) p[i] = v;
 }
 
+// Plane build, register-window form: a 256 x 64 output tile per 256-thread block.
+// The clamped source rows (y0-2 .. y0+66, columns x0-4 .. x0+263) are staged in LDS
+// as dwords; thread (c, g) makes columns x0+4c .. +3 of rows y0+16g .. +15, sliding
+// a window of six source rows held as 16-bit pairs down its strip.  Per row:
+//   vt (vertical 6-tap, exact in int16: -2550..10710) at columns x-2 .. x+7,
+//   V = clip((vt + 16) >> 5) at x .. x+3,
+//   C = clip((6-tap of vt + 512) >> 10) with v_dot2_i32_i16 (32-bit sums),
+//   H = clip((horizontal 6-tap + 16) >> 5) on the centre row in int16 pairs,
+//   F = the centre row's source dword.
+// Every staged dword lies wholly inside the frame or wholly in the replicated
+// border (x0, pad and cw are multiples of 4), so interior dwords are plain loads.
+constexpr int PB_W = 256, PB_H = 64, PB_DW = PB_W / 4 + 4, PB_ROWS = PB_H + 5;
+
+typedef short pks2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ pks2 pk_of(uint32_t v) { return __builtin_bit_cast(pks2, v); }
+__device__ __forceinline__ uint32_t u_of(pks2 v) { return __builtin_bit_cast(uint32_t, v); }
+// (b[k], b[k+1]) of a little-endian dword as two 16-bit lanes, k = 0 or 2
+__device__ __forceinline__ pks2 bytes_lo(uint32_t v) { return pk_of(__builtin_amdgcn_perm(0u, v, 0x0c010c00u)); }
+__device__ __forceinline__ pks2 bytes_hi(uint32_t v) { return pk_of(__builtin_amdgcn_perm(0u, v, 0x0c030c02u)); }
+// pair (a.hi, b.lo): the 16-bit pair that starts one lane to the right of a
+__device__ __forceinline__ pks2 pk_shift(pks2 a, pks2 b) { return pk_of(__builtin_amdgcn_alignbit(u_of(b), u_of(a), 16)); }
+// 6-tap (1, -5, 20, 20, -5, 1) over six pairs, lane-wise, exact in int16 for byte inputs
+__device__ __forceinline__ pks2 tap6(pks2 a, pks2 b, pks2 c, pks2 d, pks2 e, pks2 f)
+{
+    const pks2 k20 = {20, 20}, k5 = {5, 5};
+    return (a + f) + k20 * (c + d) - k5 * (b + e);
+}
+// clip((v + 16) >> 5) of two int16 lanes, as bytes 0 and 1 of the result
+__device__ __forceinline__ uint32_t round5_pair(pks2 v)
+{
+    const pks2 r = {16, 16}, z = {0, 0}, m = {255, 255};
+    pks2 t = (v + r) >> 5;
+    t = __builtin_elementwise_min(__builtin_elementwise_max(t, z), m);
+    return u_of(t);
+}
+
+__global__ __launch_bounds__(256) void k_build_planes_rw(const uint8_t *__restrict__ frames, size_t frame_stride,
+                                                         int cw, int ch, int pad, uint8_t *__restrict__ planes,
+                                                         size_t planes_bytes)
+{
+    __shared__ uint32_t s_src[PB_ROWS * PB_DW];
+    const int pw = cw + 2 * pad, ph = ch + 2 * pad;
+    const size_t n = (size_t)pw * ph;
+    const uint8_t *fr = frames + (size_t)blockIdx.z * frame_stride;
+    uint8_t *out = planes + (size_t)blockIdx.z * planes_bytes;
+    const int x0 = blockIdx.x * PB_W, y0 = blockIdx.y * PB_H;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < PB_ROWS * PB_DW; i += 256) {
+        const int r = i / PB_DW, d = i - r * PB_DW;
+        const uint8_t *row = fr + (size_t)clampi(y0 - 2 + r - pad, 0, ch - 1) * cw;
+        const int sx = x0 - 4 + 4 * d - pad;
+        uint32_t v;
+        if (sx < 0) v = 0x01010101u * row[0];
+        else if (sx >= cw) v = 0x01010101u * row[cw - 1];
+        else v = *(const uint32_t *)(row + sx);
+        s_src[i] = v;
+    }
+    __syncthreads();
+    const int c = tid & 63, g = tid >> 6;
+    const int x = x0 + 4 * c;
+    if (x >= pw) return;
+    // window: source rows (LDS rows 16g + k .. +5), each as 3 raw dwords and 6 pairs
+    pks2 P[6][6];
+    uint32_t raw[6];
+    auto load_row = [&](int w, int lr) {
+        const uint32_t *rp = s_src + lr * PB_DW + c;
+        const uint32_t d0 = rp[0], d1 = rp[1], d2 = rp[2];
+        raw[w] = d1;
+        P[w][0] = bytes_lo(d0); P[w][1] = bytes_hi(d0);
+        P[w][2] = bytes_lo(d1); P[w][3] = bytes_hi(d1);
+        P[w][4] = bytes_lo(d2); P[w][5] = bytes_hi(d2);
+    };
+#pragma unroll
+    for (int w = 0; w < 5; w++) load_row(w, 16 * g + w);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int w5 = (k + 5) % 6; // ring slot of the new row
+        load_row(w5, 16 * g + k + 5);
+        const int oy = y0 + 16 * g + k;
+        if (oy >= ph) break;
+        // ring slots of window rows 0..5 (source rows oy-2 .. oy+3)
+        int sl[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) sl[j] = (k + j) % 6;
+        // vertical 6-tap at column pairs 1..5 (columns x-2 .. x+7)
+        pks2 vt[6];
+#pragma unroll
+        for (int m = 1; m < 6; m++)
+            vt[m] = tap6(P[sl[0]][m], P[sl[1]][m], P[sl[2]][m], P[sl[3]][m], P[sl[4]][m], P[sl[5]][m]);
+        // V at columns x .. x+3
+        const uint32_t v01 = round5_pair(vt[2]), v23 = round5_pair(vt[3]);
+        const uint32_t vq = __builtin_amdgcn_perm(v23, v01, 0x06040200u);
+        // C: 6-tap of vt in 32 bits, outputs x+0..x+3
+        const pks2 w15 = {1, -5}, w20 = {20, 20}, w51 = {-5, 1};
+        const pks2 s1 = pk_shift(vt[1], vt[2]), s2 = pk_shift(vt[2], vt[3]), s3 = pk_shift(vt[3], vt[4]),
+                   s4 = pk_shift(vt[4], vt[5]);
+        int cs[4];
+        cs[0] = __builtin_amdgcn_sdot2(vt[1], w15, __builtin_amdgcn_sdot2(vt[2], w20, __builtin_amdgcn_sdot2(vt[3], w51, 512, false), false), false);
+        cs[1] = __builtin_amdgcn_sdot2(s1, w15, __builtin_amdgcn_sdot2(s2, w20, __builtin_amdgcn_sdot2(s3, w51, 512, false), false), false);
+        cs[2] = __builtin_amdgcn_sdot2(vt[2], w15, __builtin_amdgcn_sdot2(vt[3], w20, __builtin_amdgcn_sdot2(vt[4], w51, 512, false), false), false);
+        cs[3] = __builtin_amdgcn_sdot2(s2, w15, __builtin_amdgcn_sdot2(s3, w20, __builtin_amdgcn_sdot2(s4, w51, 512, false), false), false);
+        // clip((c + 512) >> 10) as clamp-then-logical-shift (the +512 is in the sums):
+        // this spelling keeps the compiler from forming gfx950's packed
+        // shift-and-saturate instruction, whose bytes differ from the rule
+        uint32_t cq = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) cq |= ((uint32_t)min(max(cs[q], 0), (256 << 10) - 1) >> 10) << (8 * q);
+        // H on the centre row (window row 2)
+        const pks2 *R = P[sl[2]];
+        const pks2 r1 = pk_shift(R[1], R[2]), r2 = pk_shift(R[2], R[3]), r3 = pk_shift(R[3], R[4]),
+                   r4 = pk_shift(R[4], R[5]);
+        const uint32_t h01 = round5_pair(tap6(R[1], r1, R[2], r2, R[3], r3));
+        const uint32_t h23 = round5_pair(tap6(R[2], r2, R[3], r3, R[4], r4));
+        const uint32_t hq = __builtin_amdgcn_perm(h23, h01, 0x06040200u);
+        const size_t o = (size_t)oy * pw + x;
+        *(uint32_t *)(out + o) = raw[sl[2]];
+        *(uint32_t *)(out + n + o) = hq;
+        *(uint32_t *)(out + 2 * n + o) = vq;
+        *(uint32_t *)(out + 3 * n + o) = cq;
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && tid < 64) *(uint32_t *)(out + 4 * n + 4 * tid) = 0u;
+}
+
 // ---------------------------------------------------------------------------
 // exhaustive integer search
 // ---------------------------------------------------------------------------
@@ -2107,6 +2230,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->refine_mfma = 1;
     c->cand_quad = 1;
     c->fallback_v = 1;
+    c->planes_rw = 1;
     c->mvnet_fused = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -2222,6 +2346,13 @@ int rh264_build_planes(rh264_ctx *ctx, const uint8_t *frames, size_t frame_strid
     if (nframes <= 0) return 0;
     if (nframes > 65535) return fail("build_planes: nframes %d > 65535", nframes);
     const int pw = cw + 2 * pad, ph = ch + 2 * pad;
+    if (ctx->planes_rw) {
+        dim3 grid((pw + PB_W - 1) / PB_W, (ph + PB_H - 1) / PB_H, nframes);
+        hipLaunchKernelGGL(k_build_planes_rw, grid, dim3(256), 0, ctx->stream, frames, frame_stride, cw, ch, pad,
+                           planes, rh264_planes_bytes(cw, ch, pad));
+        LAUNCH_CHECK("k_build_planes_rw");
+        return 0;
+    }
     dim3 grid((pw + PT_W - 1) / PT_W, (ph + PT_H - 1) / PT_H, nframes);
     hipLaunchKernelGGL(k_build_planes, grid, dim3(256), 0, ctx->stream, frames, frame_stride, cw, ch, pad,
                        planes, rh264_planes_bytes(cw, ch, pad));
@@ -2467,6 +2598,10 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
     }
     if (!strcmp(knob, "refine_lds")) {
         ctx->refine_lds = value ? 1 : 0;
+        return 0;
+    }
+    if (!strcmp(knob, "planes_rw")) {
+        ctx->planes_rw = value ? 1 : 0;
         return 0;
     }
     if (!strcmp(knob, "fallback_v")) {

@@ -71,12 +71,29 @@ def test_satd_avg_x4(dev):
 
 # ------------------------------------------------------------ planes
 
-@pytest.mark.parametrize("cw,ch,pad,nf", [(64, 48, 16, 1), (176, 144, 32, 3), (352, 288, 32, 2), (48, 32, 4, 2),
-                                          (1920, 1088, 32, 1), (96, 64, 128, 1)])
-def test_planes_bit_exact(dev, cw, ch, pad, nf):
-    frames = np.random.default_rng(cw + pad).integers(0, 256, (nf, ch, cw), dtype=np.uint8)
+def _plane_frames(kind, nf, ch, cw, seed):
+    if kind == "noise":
+        return np.random.default_rng(seed).integers(0, 256, (nf, ch, cw), dtype=np.uint8)
+    # extremes: 0/255 checkerboards and stripes drive every filter to its range ends
+    yy, xx = np.mgrid[0:ch, 0:cw]
+    pats = [((xx + yy) & 1) * 255, (xx & 1) * 255, ((yy >> 1) & 1) * 255, ((xx // 3 + yy // 2) & 1) * 255]
+    return np.stack([pats[i % 4] for i in range(nf)]).astype(np.uint8)
+
+
+@pytest.mark.parametrize("planes_rw", [1, 0])
+@pytest.mark.parametrize("cw,ch,pad,nf,kind", [(64, 48, 16, 1, "noise"), (176, 144, 32, 3, "noise"),
+                                               (352, 288, 32, 2, "noise"), (48, 32, 4, 2, "noise"),
+                                               (1920, 1088, 32, 1, "noise"), (96, 64, 128, 1, "noise"),
+                                               (272, 80, 12, 4, "extreme"), (16, 16, 4, 1, "noise")])
+def test_planes_bit_exact(dev, cw, ch, pad, nf, kind, planes_rw):
+    """Both plane kernels: the register-window one (default) and the LDS-tile one."""
+    frames = _plane_frames(kind, nf, ch, cw, cw + pad)
     fb = dev.upload(frames)
-    pl = accel.build_planes(dev, fb, nf, cw, ch, pad)
+    dev.tune("planes_rw", planes_rw)
+    try:
+        pl = accel.build_planes(dev, fb, nf, cw, ch, pad)
+    finally:
+        dev.tune("planes_rw", 1)
     pb = accel.planes_bytes(cw, ch, pad)
     got = dev.download(pl, (nf, pb), np.uint8)
     pw, ph = cw + 2 * pad, ch + 2 * pad
