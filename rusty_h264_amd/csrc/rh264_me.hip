@@ -230,9 +230,9 @@ __global__ void k_fill_u64(uint64_t *p, size_t n, uint64_t v)
     if (i < n) p[i] = v;
 }
 
-// Plane build, register-window form: a 256 x 64 output tile per 256-thread block.
-// The clamped source rows (y0-2 .. y0+66, columns x0-4 .. x0+263) are staged in LDS
-// as dwords; thread (c, g) makes columns x0+4c .. +3 of rows y0+16g .. +15, sliding
+// Plane build, register-window form: a 256 x PB_H output tile per 256-thread block.
+// The clamped source rows (y0-2 .. y0+PB_H+2, columns x0-4 .. x0+263) are staged in
+// LDS as dwords; thread (c, g) makes columns x0+4c .. +3 of PB_H/4 rows, sliding
 // a window of six source rows held as 16-bit pairs down its strip.  Per row:
 //   vt (vertical 6-tap, exact in int16: -2550..10710) at columns x-2 .. x+7,
 //   V = clip((vt + 16) >> 5) at x .. x+3,
@@ -241,7 +241,10 @@ __global__ void k_fill_u64(uint64_t *p, size_t n, uint64_t v)
 //   F = the centre row's source dword.
 // Every staged dword lies wholly inside the frame or wholly in the replicated
 // border (x0, pad and cw are multiples of 4), so interior dwords are plain loads.
-constexpr int PB_W = 256, PB_H = 64, PB_DW = PB_W / 4 + 4, PB_ROWS = PB_H + 5;
+#ifndef RH264_PB_H
+#define RH264_PB_H 32
+#endif
+constexpr int PB_W = 256, PB_H = RH264_PB_H, PB_RPT = PB_H / 4, PB_DW = PB_W / 4 + 4, PB_ROWS = PB_H + 5;
 
 typedef short pks2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ pks2 pk_of(uint32_t v) { return __builtin_bit_cast(pks2, v); }
@@ -303,12 +306,12 @@ __global__ __launch_bounds__(256) void k_build_planes_rw(const uint8_t *__restri
         P[w][4] = bytes_lo(d2); P[w][5] = bytes_hi(d2);
     };
 #pragma unroll
-    for (int w = 0; w < 5; w++) load_row(w, 16 * g + w);
+    for (int w = 0; w < 5; w++) load_row(w, PB_RPT * g + w);
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
+    for (int k = 0; k < PB_RPT; k++) {
         const int w5 = (k + 5) % 6; // ring slot of the new row
-        load_row(w5, 16 * g + k + 5);
-        const int oy = y0 + 16 * g + k;
+        load_row(w5, PB_RPT * g + k + 5);
+        const int oy = y0 + PB_RPT * g + k;
         if (oy >= ph) break;
         // ring slots of window rows 0..5 (source rows oy-2 .. oy+3)
         int sl[6];
