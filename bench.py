@@ -57,7 +57,8 @@ MAX_REGRET = 0.01  # the learned path counts only at <= 1% mean SATD-cost regret
 
 def run_learned(dev, args, cur, refs, planes, p, plain_cost):
     """The learned-proposal path on the same clip: plane build + rh264_learned_search
-    (network, 8 local searches, selection, fall-back full search, qpel refine).
+    (network, 8 local searches, selection, fall-back full search, then the +-2
+    quarter-pel window around the best integer vector: the 25 positions, exact SATD).
     Regret = sum of final costs / sum of the plain search's final costs - 1."""
     from rusty_h264_amd import accel, mvnet
     if not os.path.exists(mvnet.WEIGHTS):
@@ -67,12 +68,14 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost):
     pbytes = accel.planes_bytes(W, H, PAD)
     fb_cost = mvnet.FALLBACK_COST if args.fallback_cost is None else args.fallback_cost
     net = mvnet.MVNet(dev, nf, W, H, mvnet.load_weights())
+    # the proposal's refinement: the +-2 quarter-pel window around the best integer vector
+    pl = accel.search_params(W, H, PAD, 16, 16, R, mvnet.REFINE_PAT, accel.lambda_me(QP))
     outs = (dev.alloc(8 * nb), dev.alloc(4 * nb), dev.alloc(4 * nb))
     lib = dev.lib
 
     def step():
         accel.build_planes(dev, refs, nf, W, H, PAD, out=planes)
-        mvnet.learned_search(dev, net, p, nf, cur, refs, W * H, planes, pbytes, mvnet.LOCAL_RANGE, fb_cost, out=outs)
+        mvnet.learned_search(dev, net, pl, nf, cur, refs, W * H, planes, pbytes, mvnet.LOCAL_RANGE, fb_cost, out=outs)
 
     for _ in range(args.warmup):
         step()
@@ -101,6 +104,7 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost):
             "regret": round(float(cost.sum() / pc.sum() - 1.0), 6),
             "blocks_worse": round(float((cost > pc).mean()), 5), "blocks_better": round(float((cost < pc).mean()), 5),
             "fallback_rate": round(float(fb.mean()), 5), "local_range": mvnet.LOCAL_RANGE, "fallback_cost": fb_cost,
+            "refine": "+-2 quarter-pel window (subpel_pat 4)",
             "phase_ms": {"network": round(float(ph_mean[0]), 4), "candidates": round(float(ph_mean[1]), 4),
                          "refine": round(float(ph_mean[2]), 4)}}
 

@@ -105,7 +105,12 @@ typedef struct {
     int32_t bw, bh;        /* partition: 16x16, 16x8, 8x16, 8x8, 8x4, 4x8, 4x4   */
     int32_t range;         /* full-search radius R in full pels, 1..64           */
     int32_t subpel_pat;    /* 0 ring8+iterate, 1 ring4+iterate, 2 ring8 single,
-                              3 ring4 single, -1 no rings (SATD reprice only)     */
+                              3 ring4 single, -1 no rings (SATD reprice only),
+                              4 quarter-pel window: the 25 positions within +-2
+                              quarter pels of the start, cheapest wins, ties to
+                              the start, then raster order (dy, then dx); no
+                              reference counterpart, the learned search's
+                              refinement                                          */
     int32_t lr_fp[RH264_LR_N];
     int32_t lr[RH264_LR_N];
 } rh264_search_params;
@@ -170,7 +175,9 @@ int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases
  * 0 = the lane-per-4x4 kernel above; "cand_quad" 1 (default) = the learned search's
  * candidate windows (local range <= 3) use quad-SADs, 0 = v_sad_u8 lanes;
  * "fallback_v" 1 (default) = its fall-back full search uses the v_sad_u8 x2
- * variant, 0 = quad-SADs; "mvnet_fused" 1 (default) = network inference
+ * variant, 0 = quad-SADs; "planes_rw" 1 (default) = the plane build slides a
+ * register window down 256x32 tiles, 0 = the LDS-tile kernel; "mvnet_fused" 1
+ * (default) = network inference
  * in one fused kernel per tile, 0 = layer by layer.  Every search variant returns
  * identical results; the two network paths agree up to FP32 summation order. */
 int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
@@ -249,7 +256,8 @@ int rh264_mvnet_adam(rh264_ctx *ctx, rh264_mvnet *net, float lr);
  * i*frame_stride against ref + i*frame_stride, plane set planes + i*planes_stride):
  * the network's 8 centres are re-costed exactly (the integer search above with
  * radius local_range around each centre); the cheapest (first minimum in rank
- * order) starts the sub-pel ring walk (p->subpel_pat); a block whose best integer
+ * order) starts the sub-pel refinement (p->subpel_pat: a ring walk, or 4, the
+ * +-2 quarter-pel window, which is what the proposal pairs with); a block whose best integer
  * cost exceeds fallback_cost is searched exhaustively with radius p->range
  * instead (fallback[i] = 1).  Costs, rate tables and tie-breaks are the plain
  * search's, so every evaluated candidate is priced exactly. */

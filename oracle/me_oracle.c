@@ -505,6 +505,25 @@ int64_t orc_satd_cost(const orc_search_cfg *cfg, const uint8_t *cur, const uint8
            rate_cost(cfg->lr, mvx, mvy, mvpx, mvpy);
 }
 
+/* Quarter-pel window (build-defined, subpel_pat 4): the 24 positions around the
+ * start (*bx, *by) within +-2 quarter pels, in raster order (dy, then dx), each
+ * replacing the best on a strictly lower cost; *best holds the start's cost on
+ * entry.  Returns the number of positions priced (24). */
+static int qpel_window(const orc_search_cfg *cfg, const uint8_t *cur, const uint8_t *planes, int lx, int ly,
+                       int mvpx, int mvpy, int *bx, int *by, int64_t *best)
+{
+    const int sx = *bx, sy = *by;
+    int n = 0;
+    for (int dy = -2; dy <= 2; dy++)
+        for (int dx = -2; dx <= 2; dx++) {
+            if (dx == 0 && dy == 0) continue;
+            int64_t cc = orc_satd_cost(cfg, cur, planes, lx, ly, sx + dx, sy + dy, mvpx, mvpy);
+            n++;
+            if (cc < *best) { *best = cc; *bx = sx + dx; *by = sy + dy; }
+        }
+    return n;
+}
+
 /* Sub-pel ring walk, mb16.rs:3046-3046 ring8 order (s,0),(-s,0),(0,s),(0,-s),(s,s),(-s,-s),
  * (s,-s),(-s,s); cascade with a moving centre (mb16.rs:3230-3275); steps [2,1];
  * pattern bit0 = ring4, bit1 = single pass (mb16.rs:2976-2998). */
@@ -516,7 +535,9 @@ int64_t orc_subpel_refine_block(const orc_search_cfg *cfg, const uint8_t *cur,
     int64_t best = orc_satd_cost(cfg, cur, planes, lx, ly, bx, by, mvpx, mvpy);
     int ne = 1;
     int pat = cfg->subpel_pat;
-    if (pat >= 0) {
+    if (pat == 4) {
+        ne += qpel_window(cfg, cur, planes, lx, ly, mvpx, mvpy, &bx, &by, &best);
+    } else if (pat >= 0) {
         for (int step = 2; step >= 1; step--) {
             const int ring[8][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step},
                                     {step, step}, {-step, -step}, {step, -step}, {-step, step}};
@@ -628,7 +649,9 @@ int64_t orc_motion_search_ref(const orc_search_cfg *cfg, const orc_diamond_cfg *
         seed_c = COST(seedx, seedy);
     }
     int pat = cfg->subpel_pat;
-    if (pat >= 0) {
+    if (pat == 4) {
+        qpel_window(cfg, cur, planes, lx, ly, cx0, cy0, &bx, &by, &bc);
+    } else if (pat >= 0) {
         for (int step = 2; step >= 1; step--) {
             const int ring[8][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step},
                                     {step, step}, {-step, -step}, {step, -step}, {-step, step}};

@@ -69,7 +69,8 @@ typedef struct {
     int bw, bh;        /* partition shape                                      */
     int range;         /* full-pel search radius R (full search)               */
     int subpel_pat;    /* 0: ring8 iterate, 1: ring4 iterate, 2: ring8 single,
-                          3: ring4 single, -1: no sub-pel (mb16.rs:2956-3291)  */
+                          3: ring4 single, -1: no sub-pel (mb16.rs:2956-3291);
+                          4: quarter-pel window (build-defined, below)          */
     const int64_t *lr_fp;  /* SAD-phase lambda table, ORC_LR_N entries        */
     const int64_t *lr;     /* SATD-phase lambda table, ORC_LR_N entries       */
 } orc_search_cfg;
@@ -84,7 +85,11 @@ int64_t orc_full_search_block(const orc_search_cfg *cfg, const uint8_t *cur,
  * values of mb16.rs:1930-2007). */
 int64_t orc_satd_cost(const orc_search_cfg *cfg, const uint8_t *cur, const uint8_t *planes,
                       int lx, int ly, int mvx, int mvy, int mvpx, int mvpy);
-/* Sub-pel ring refinement (mb16.rs:2956-3291, cascade path). */
+/* Sub-pel ring refinement (mb16.rs:2956-3291, cascade path).  Pattern 4 is the
+ * build-defined quarter-pel window that refines the learned proposal: the 25
+ * positions start + (dx, dy), dx, dy in -2..2 quarter pels, priced like the ring's
+ * candidates; the cheapest wins, ties going to the start, then to the earlier
+ * position in raster order (dy, then dx).  evals = 25. */
 int64_t orc_subpel_refine_block(const orc_search_cfg *cfg, const uint8_t *cur,
                                 const uint8_t *planes, int lx, int ly, int mvx, int mvy,
                                 int mvpx, int mvpy, int32_t *mv_out, int32_t *evals);

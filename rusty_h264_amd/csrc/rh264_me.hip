@@ -23,6 +23,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -95,19 +96,30 @@ __device__ __forceinline__ int mvbits(int d)
     return 1 + 2 * (31 - __clz((int)(codenum + 1u)));
 }
 
-// Quarter-phase operand table, inter.rs:1079-1092 / mectx.rs:143-156:
-// for idx = fx*4+fy, (plane_a, ax, ay, plane_b, bx, by) packed 4 bits each.
-// Planes: 0=F, 1=H, 2=V, 3=C.  Even phases use one plane (in plane_a).
-#define QENT(pa, ax, ay, pb, bx, by) \
-    ((uint32_t)(pa) | ((uint32_t)(ax) << 4) | ((uint32_t)(ay) << 8) | ((uint32_t)(pb) << 12) | \
-     ((uint32_t)(bx) << 16) | ((uint32_t)(by) << 20))
-__constant__ uint32_t c_qtab[16] = {
-    /* fx=0: fy=0..3 */ QENT(0, 0, 0, 0, 0, 0), QENT(0, 0, 0, 2, 0, 0), QENT(2, 0, 0, 2, 0, 0), QENT(0, 0, 1, 2, 0, 0),
-    /* fx=1 */          QENT(0, 0, 0, 1, 0, 0), QENT(1, 0, 0, 2, 0, 0), QENT(2, 0, 0, 3, 0, 0), QENT(1, 0, 1, 2, 0, 0),
-    /* fx=2 */          QENT(1, 0, 0, 1, 0, 0), QENT(1, 0, 0, 3, 0, 0), QENT(3, 0, 0, 3, 0, 0), QENT(1, 0, 1, 3, 0, 0),
-    /* fx=3 */          QENT(0, 1, 0, 1, 0, 0), QENT(1, 0, 0, 2, 1, 0), QENT(2, 1, 0, 3, 0, 0), QENT(1, 0, 1, 2, 1, 0),
+// Quarter-phase operand table, inter.rs:1079-1092 / mectx.rs:143-156: for phase
+// fx*4+fy, the plane (0=F, 1=H, 2=V, 3=C) and integer offset (x, y) of operand a
+// and of operand b; one-plane phases name the same operand twice.
+struct QPhase {
+    int pa, ax, ay, pb, bx, by;
 };
-#undef QENT
+constexpr QPhase k_qphase[16] = {
+    /* fx=0: fy=0..3 */ {0, 0, 0, 0, 0, 0}, {0, 0, 0, 2, 0, 0}, {2, 0, 0, 2, 0, 0}, {0, 0, 1, 2, 0, 0},
+    /* fx=1 */          {0, 0, 0, 1, 0, 0}, {1, 0, 0, 2, 0, 0}, {2, 0, 0, 3, 0, 0}, {1, 0, 1, 2, 0, 0},
+    /* fx=2 */          {1, 0, 0, 1, 0, 0}, {1, 0, 0, 3, 0, 0}, {3, 0, 0, 3, 0, 0}, {1, 0, 1, 3, 0, 0},
+    /* fx=3 */          {0, 1, 0, 1, 0, 0}, {1, 0, 0, 2, 1, 0}, {2, 1, 0, 3, 0, 0}, {1, 0, 1, 2, 1, 0},
+};
+// the same, packed 4 bits a field (plane_a, ax, ay, plane_b, bx, by from bit 0)
+constexpr uint32_t qent(const QPhase &q)
+{
+    return (uint32_t)q.pa | ((uint32_t)q.ax << 4) | ((uint32_t)q.ay << 8) | ((uint32_t)q.pb << 12) |
+           ((uint32_t)q.bx << 16) | ((uint32_t)q.by << 20);
+}
+__constant__ uint32_t c_qtab[16] = {
+    qent(k_qphase[0]),  qent(k_qphase[1]),  qent(k_qphase[2]),  qent(k_qphase[3]),
+    qent(k_qphase[4]),  qent(k_qphase[5]),  qent(k_qphase[6]),  qent(k_qphase[7]),
+    qent(k_qphase[8]),  qent(k_qphase[9]),  qent(k_qphase[10]), qent(k_qphase[11]),
+    qent(k_qphase[12]), qent(k_qphase[13]), qent(k_qphase[14]), qent(k_qphase[15]),
+};
 __device__ __forceinline__ uint32_t qtab(int idx) { return c_qtab[idx]; }
 
 // Byte-wise differences of two packed rows of four pixels.
@@ -922,7 +934,17 @@ __device__ __forceinline__ void refine_walk(const RefineGeom &G, const uint8_t *
     };
     best = cost_of(bx, by);
     evals = 1;
-    if (pat >= 0) {
+    if (pat == 4) {
+        // quarter-pel window: the start first, then raster order
+        const int sx = bx, sy = by;
+        for (int dy = -2; dy <= 2; dy++)
+            for (int dx = -2; dx <= 2; dx++) {
+                if (dx == 0 && dy == 0) continue;
+                const int cc = cost_of(sx + dx, sy + dy);
+                evals++;
+                if (cc < best) { best = cc; bx = sx + dx; by = sy + dy; }
+            }
+    } else if (pat >= 0) {
         const int nring = (pat & 1) ? 4 : 8;
         for (int step = 2; step >= 1; step--) {
             const int rx[8] = {step, -step, 0, 0, step, -step, step, -step};
@@ -1063,15 +1085,16 @@ constexpr int MX_WIN = 4 * MX_PLANE;     // bytes per wave
 // The quarter-phase operand table (qtab) as byte offsets inside a staged window:
 // plane * MX_PLANE + row offset * MX_PITCH + column offset, operand a in the low
 // half, operand b in the high half.
-#define MXOFF(pa, ax, ay, pb, bx, by) \
-    ((uint32_t)((pa) * MX_PLANE + (ay) * MX_PITCH + (ax)) | ((uint32_t)((pb) * MX_PLANE + (by) * MX_PITCH + (bx)) << 16))
+constexpr uint32_t mxoff(const QPhase &q)
+{
+    return (uint32_t)(q.pa * MX_PLANE + q.ay * MX_PITCH + q.ax) | ((uint32_t)(q.pb * MX_PLANE + q.by * MX_PITCH + q.bx) << 16);
+}
 __constant__ uint32_t c_mxoff[16] = {
-    /* fx=0: fy=0..3 */ MXOFF(0, 0, 0, 0, 0, 0), MXOFF(0, 0, 0, 2, 0, 0), MXOFF(2, 0, 0, 2, 0, 0), MXOFF(0, 0, 1, 2, 0, 0),
-    /* fx=1 */          MXOFF(0, 0, 0, 1, 0, 0), MXOFF(1, 0, 0, 2, 0, 0), MXOFF(2, 0, 0, 3, 0, 0), MXOFF(1, 0, 1, 2, 0, 0),
-    /* fx=2 */          MXOFF(1, 0, 0, 1, 0, 0), MXOFF(1, 0, 0, 3, 0, 0), MXOFF(3, 0, 0, 3, 0, 0), MXOFF(1, 0, 1, 3, 0, 0),
-    /* fx=3 */          MXOFF(0, 1, 0, 1, 0, 0), MXOFF(1, 0, 0, 2, 1, 0), MXOFF(2, 1, 0, 3, 0, 0), MXOFF(1, 0, 1, 2, 1, 0),
+    mxoff(k_qphase[0]),  mxoff(k_qphase[1]),  mxoff(k_qphase[2]),  mxoff(k_qphase[3]),
+    mxoff(k_qphase[4]),  mxoff(k_qphase[5]),  mxoff(k_qphase[6]),  mxoff(k_qphase[7]),
+    mxoff(k_qphase[8]),  mxoff(k_qphase[9]),  mxoff(k_qphase[10]), mxoff(k_qphase[11]),
+    mxoff(k_qphase[12]), mxoff(k_qphase[13]), mxoff(k_qphase[14]), mxoff(k_qphase[15]),
 };
-#undef MXOFF
 
 struct MxConst {
     mx_h4 hb; // Hb entries of this lane: B operand of the first product, A operand of the second
@@ -1146,15 +1169,12 @@ struct MxBlock {
     int xoff;           // byte offset of column ix - MX_WL inside the staged rows
 };
 
-// Cost of a candidate outside the staged window: the plane set in HBM, clamped at
-// its border, and the rate computed directly.  Kept out of line so that its table
-// reads and address arithmetic stay off the common path.
-__device__ __attribute__((noinline)) int mx_cost_far(const MxBlock &B, const MxConst &K, mx_f4 c2,
-                                                      const int32_t *s_lr, int lane, int rx, int ry)
+// The lane's four predicted pixels of the candidate at offset (rx, ry) from the
+// start's integer position, read from the plane set in HBM, clamped at its border.
+__device__ __forceinline__ uint32_t mx_operand_far(const MxBlock &B, int lane, int rx, int ry)
 {
     const int r = lane & 15, qd = lane >> 4;
-    const int mx = B.sx + rx, my = B.sy + ry;
-    const int fx = mx & 3, fy = my & 3;
+    const int fx = rx & 3, fy = ry & 3;
     const uint32_t q = qtab(fx * 4 + fy);
     const int bpx = B.ix + (rx >> 2), bpy = B.iy + (ry >> 2);
     const bool inb = bpx >= 0 && bpy >= 0 && bpx + 17 <= B.pw && bpy + 17 <= B.ph;
@@ -1167,7 +1187,17 @@ __device__ __attribute__((noinline)) int mx_cost_far(const MxBlock &B, const MxC
         const uint32_t w = inb ? load4_fast(plB, B.pw, xb, yb) : load4_clamped(plB, B.pw, B.ph, xb, yb);
         v = __builtin_amdgcn_lerp(v, w, 0x01010101u);
     }
-    const int s = mx_satd(v, K, c2);
+    return v;
+}
+
+// Cost of a candidate outside the staged window: the plane set in HBM, clamped at
+// its border, and the rate computed directly.  Kept out of line so that its table
+// reads and address arithmetic stay off the common path.
+__device__ __attribute__((noinline)) int mx_cost_far(const MxBlock &B, const MxConst &K, mx_f4 c2,
+                                                      const int32_t *s_lr, int lane, int rx, int ry)
+{
+    const int mx = B.sx + rx, my = B.sy + ry;
+    const int s = mx_satd(mx_operand_far(B, lane, rx, ry), K, c2);
     return __builtin_amdgcn_readfirstlane(s + s_lr[mvbits(mx - B.mpx) + mvbits(my - B.mpy)]);
 }
 
@@ -1362,6 +1392,225 @@ __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, cons
         out_mv[2 * boff + 1] = B.sy + cy;
         out_cost[boff] = best;
         if (out_evals) out_evals[boff] = evals;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Quarter-pel window refinement of 16x16 blocks (subpel_pat 4), one wave per
+// block: the 25 positions start + (dx, dy), dx, dy in -2..2 quarter pels, each
+// priced as the ring walk prices a candidate (SATD of the prediction + lambda
+// rate); the cheapest wins, ties going to the start, then to the earlier position
+// in raster order (dy, then dx).  The positions do not depend on each other, so:
+//  * for an integer-pel start every position's phase and integer offset are
+//    compile-time constants; its prediction is one of 16 operands, or the
+//    rounded average of two (planes F, H, V, C at integer offsets -1 and 0 on
+//    each axis), which each lane reads once with one 8-byte load per operand pair;
+//  * each position's Hadamards are k_refine_mx16's two MFMAs, after which each
+//    lane holds its partial sum of |coefficients| (exact in fp32);
+//  * one butterfly over the 25 partials (two permlane swaps, four DPP stages)
+//    leaves each lane with one position's whole SATD, so the rates and the
+//    minimum are computed once, across the lanes.
+// A fractional start (not produced by the integer search) prices the positions
+// one by one from the plane set in HBM.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float wn_f(uint32_t v) { return __builtin_bit_cast(float, v); }
+__device__ __forceinline__ uint32_t wn_u(float v) { return __builtin_bit_cast(uint32_t, v); }
+
+// Butterfly stages.  Each takes a pair (a, b) of per-lane partials and returns a
+// on the lanes where the stage's lane bit is clear, b where it is set, summed over
+// the lane and its partner (the lane with that bit flipped; for the stage on bit
+// 2, the partner l ^ 7 of row_half_mirror, which keeps bits 3..5).
+__device__ __forceinline__ float wn_swap32(float a, float b) // lane bit 5
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(wn_u(a), wn_u(b), false, false);
+    return wn_f(r[0]) + wn_f(r[1]);
+}
+__device__ __forceinline__ float wn_swap16(float a, float b) // lane bit 4
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(wn_u(a), wn_u(b), false, false);
+    return wn_f(r[0]) + wn_f(r[1]);
+}
+template <int CTRL> __device__ __forceinline__ float wn_dpp(float a, float b, bool hi)
+{
+    const float keep = hi ? b : a, send = hi ? a : b;
+    return keep + wn_f((uint32_t)__builtin_amdgcn_mov_dpp((int)wn_u(send), CTRL, 0xF, 0xF, true));
+}
+
+// Whole-wave sums of the partials t[0..24]: lane l ends with the sum of position
+// wn_lane_pos(l) (bits 5, 4, 2, 0, 1 of l as bits 0..4 of the position; lanes
+// whose position is >= 25 hold 0).  The partner sets of the six stages are the
+// lane xors 32, 16, 7, 1, 2 and 8, which span all 64 lanes.
+__device__ __forceinline__ float wn_sum25(const float (&t)[25], int lane)
+{
+    float u[13], w[7], x[4];
+#pragma unroll
+    for (int k = 0; k < 13; k++) u[k] = wn_swap32(t[2 * k], 2 * k + 1 < 25 ? t[2 * k + 1] : 0.0f);
+#pragma unroll
+    for (int k = 0; k < 7; k++) w[k] = wn_swap16(u[2 * k], 2 * k + 1 < 13 ? u[2 * k + 1] : 0.0f);
+    const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = wn_dpp<0x141>(w[2 * k], 2 * k + 1 < 7 ? w[2 * k + 1] : 0.0f, b2); // row_half_mirror
+    const float y0 = wn_dpp<0xB1>(x[0], x[1], b0), y1 = wn_dpp<0xB1>(x[2], x[3], b0); // quad_perm [1,0,3,2]
+    const float z = wn_dpp<0x4E>(y0, y1, b1);                                          // quad_perm [2,3,0,1]
+    return z + wn_f((uint32_t)__builtin_amdgcn_mov_dpp((int)wn_u(z), 0x128, 0xF, 0xF, true)); // row_ror:8
+}
+__device__ __forceinline__ int wn_lane_pos(int lane)
+{
+    return ((lane >> 5) & 1) | ((lane >> 3) & 2) | (lane & 4) | ((lane & 1) << 3) | ((lane & 2) << 3);
+}
+
+// Prediction of position C (dx = C % 5 - 2, dy = C / 5 - 2) around an integer
+// start from the operands o[plane][y offset + 1][x offset + 1].
+template <int C> __device__ __forceinline__ uint32_t wn_operand(const uint32_t (&o)[4][2][2])
+{
+    constexpr int dx = C % 5 - 2, dy = C / 5 - 2;
+    constexpr QPhase q = k_qphase[(dx & 3) * 4 + (dy & 3)];
+    constexpr int ox = dx < 0 ? 0 : 1, oy = dy < 0 ? 0 : 1; // integer offset -1 or 0, plus 1
+    static_assert(ox + q.ax <= 1 && oy + q.ay <= 1 && ox + q.bx <= 1 && oy + q.by <= 1, "operand not loaded");
+    const uint32_t a = o[q.pa][oy + q.ay][ox + q.ax];
+    if constexpr (q.pa == q.pb && q.ax == q.bx && q.ay == q.by)
+        return a;
+    else
+        return __builtin_amdgcn_lerp(a, o[q.pb][oy + q.by][ox + q.bx], 0x01010101u);
+}
+template <typename F, int... C>
+__device__ __forceinline__ void wn_partials(float (&t)[25], const uint32_t (&o)[4][2][2], F &&partial,
+                                            std::integer_sequence<int, C...>)
+{
+    ((t[C] = partial(wn_operand<C>(o))), ...);
+}
+
+__global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, const uint8_t *__restrict__ cur,
+                                                     size_t cur_stride, const uint8_t *__restrict__ planes,
+                                                     size_t planes_stride, const uint64_t *__restrict__ keys,
+                                                     const int32_t *__restrict__ centres,
+                                                     const int32_t *__restrict__ start_mv,
+                                                     const int32_t *__restrict__ mvps, int32_t *__restrict__ int_mv,
+                                                     int32_t *__restrict__ int_cost, int32_t *__restrict__ out_mv,
+                                                     int32_t *__restrict__ out_cost, int32_t *__restrict__ out_evals)
+{
+    __shared__ int32_t s_lr[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
+    __syncthreads();
+
+    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + wave; // wave-uniform
+    if (b >= nblk) return;
+    const int frame = blockIdx.y;
+    const size_t boff = (size_t)frame * nblk + b;
+    const int lx = (b % nbx) << 4, ly = (b / nbx) << 4;
+    const int r = lane & 15, qd = lane >> 4; // pixel row, column quad of this lane
+
+    MxBlock B;
+    B.pw = p.cw + 2 * p.pad;
+    B.ph = p.ch + 2 * p.pad;
+    B.pn = (size_t)B.pw * B.ph;
+    B.pl = planes + (size_t)frame * planes_stride;
+    B.mpx = __builtin_amdgcn_readfirstlane(mvps ? mvps[2 * boff] : 0);
+    B.mpy = __builtin_amdgcn_readfirstlane(mvps ? mvps[2 * boff + 1] : 0);
+    const uint32_t src = *(const uint32_t *)(cur + (size_t)frame * cur_stride + (size_t)(ly + r) * p.cw + lx + 4 * qd);
+
+    int bx, by;
+    if (keys) {
+        const uint64_t key = keys[boff];
+        const int NC = 2 * p.range + 1;
+        const uint32_t idx = __builtin_amdgcn_readfirstlane((uint32_t)key);
+        const int t = (int)(idx / (uint32_t)NC), c = (int)(idx % (uint32_t)NC);
+        const int cx = __builtin_amdgcn_readfirstlane(centres ? centres[2 * boff] : 0);
+        const int cy = __builtin_amdgcn_readfirstlane(centres ? centres[2 * boff + 1] : 0);
+        bx = cx + 4 * (c - p.range);
+        by = cy + 4 * (t - p.range);
+        if (lane == 0) {
+            if (int_mv) { int_mv[2 * boff] = bx; int_mv[2 * boff + 1] = by; }
+            if (int_cost) int_cost[boff] = (int32_t)(key >> 32);
+        }
+    } else {
+        bx = __builtin_amdgcn_readfirstlane(start_mv[2 * boff]);
+        by = __builtin_amdgcn_readfirstlane(start_mv[2 * boff + 1]);
+    }
+    const int fx0 = bx & 3, fy0 = by & 3;
+    B.sx = bx - fx0;
+    B.sy = by - fy0;
+    B.ix = lx + (B.sx >> 2) + p.pad;
+    B.iy = ly + (B.sy >> 2) + p.pad;
+    B.xoff = 0;
+
+    const MxConst K = mx_const(lane);
+    const mx_f4 c2 = -mx_transform(src, K, mx_f4{0.0f, 0.0f, 0.0f, 0.0f});
+    int best, best_pos; // wave-uniform
+    if ((fx0 | fy0) == 0) {
+        // o[k][j][i]: plane k, row iy - 1 + j + r, columns ix - 1 + i + 4 qd .. +3
+        uint32_t o[4][2][2];
+        if (B.ix >= 1 && B.iy >= 1 && B.ix + 16 <= B.pw && B.iy + 16 <= B.ph) {
+            // every used byte is inside the plane; the 8-byte load at the dword
+            // below column ix - 1 + 4 qd ends at most 3 bytes past column ix + 15,
+            // inside the buffer (the next row, the next plane or the tail)
+            const int col = B.ix - 1 + 4 * qd;
+            const uint32_t sh = (uint32_t)(B.ix - 1) & 3u; // the plane stride is a multiple of 4
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const uint8_t *row = B.pl + B.pn * k + (size_t)(B.iy - 1 + j + r) * B.pw;
+                    const uint32_t *d = (const uint32_t *)(row + (col & ~3));
+                    const uint32_t d0 = d[0], d1 = d[1];
+                    o[k][j][0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
+                    o[k][j][1] = sh == 3u ? d1 : __builtin_amdgcn_alignbyte(d1, d0, sh + 1u);
+                }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+                        o[k][j][i] = load4_clamped(B.pl + B.pn * k, B.pw, B.ph, B.ix - 1 + i + 4 * qd, B.iy - 1 + j + r);
+        }
+        float t[25];
+        wn_partials(t, o,
+                    [&](uint32_t v) -> float {
+                        const mx_f4 y = mx_transform(v, K, c2);
+                        return (fabsf(y[0]) + fabsf(y[1])) + (fabsf(y[2]) + fabsf(y[3]));
+                    },
+                    std::make_integer_sequence<int, 25>{});
+        const int satd = (int)wn_sum25(t, lane);
+        const int pos = wn_lane_pos(lane);
+        const int dx = pos % 5 - 2, dy = pos / 5 - 2;
+        const bool valid = pos < 25;
+        const int rate = s_lr[valid ? mvbits(bx + dx - B.mpx) + mvbits(by + dy - B.mpy) : 0];
+        // (cost, order): the start first, then raster order
+        uint64_t key = valid ? ((uint64_t)(uint32_t)(satd + rate) << 5) | (uint32_t)(pos == 12 ? 0 : pos + 1) : ~0ull;
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const uint64_t o2 = __shfl_xor(key, m);
+            key = o2 < key ? o2 : key;
+        }
+        key = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(key >> 32)) << 32) |
+              __builtin_amdgcn_readfirstlane((uint32_t)key);
+        best = (int)(key >> 5);
+        const int ord = (int)(key & 31);
+        best_pos = ord == 0 ? 12 : ord - 1;
+    } else {
+        auto cost_at = [&](int c) -> int {
+            const int dx = c % 5 - 2, dy = c / 5 - 2;
+            const int s = mx_satd(mx_operand_far(B, lane, fx0 + dx, fy0 + dy), K, c2);
+            return __builtin_amdgcn_readfirstlane(s + s_lr[mvbits(bx + dx - B.mpx) + mvbits(by + dy - B.mpy)]);
+        };
+        best = cost_at(12);
+        best_pos = 12;
+        for (int c = 0; c < 25; c++) {
+            if (c == 12) continue;
+            const int cc = cost_at(c);
+            if (cc < best) { best = cc; best_pos = c; }
+        }
+    }
+    if (lane == 0) {
+        out_mv[2 * boff] = bx + best_pos % 5 - 2;
+        out_mv[2 * boff + 1] = by + best_pos / 5 - 2;
+        out_cost[boff] = best;
+        if (out_evals) out_evals[boff] = 25;
     }
 }
 
@@ -2380,7 +2629,7 @@ static int check_params(const rh264_search_params *p)
     if (!p) return fail("null params");
     if (!geom_ok(p->cw, p->ch, p->pad) || !shape_ok(p->bw, p->bh)) return RH264_DECLINED;
     if (p->range < 1 || p->range > 64) return RH264_DECLINED;
-    if (p->subpel_pat < -1 || p->subpel_pat > 3) return RH264_DECLINED;
+    if (p->subpel_pat < -1 || p->subpel_pat > 4) return RH264_DECLINED;
     for (int i = 0; i < RH264_LR_N; i++)
         if (p->lr[i] < 0 || p->lr_fp[i] < 0 || p->lr[i] > (1 << 28) || p->lr_fp[i] > (1 << 28))
             return fail("lambda table entry %d out of range", i);
@@ -2431,6 +2680,12 @@ static int launch_refine(rh264_ctx *ctx, const rh264_search_params *p, int nfram
 #define MX_LAUNCH(PT)                                                                                        \
     hipLaunchKernelGGL(k_refine_mx16<PT>, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes, planes_stride, \
                        keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost, evals)
+            if (p->subpel_pat == 4) {
+                hipLaunchKernelGGL(k_refine_wn16, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes,
+                                   planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost, evals);
+                LAUNCH_CHECK("k_refine_wn16");
+                return 0;
+            }
             switch (p->subpel_pat) {
             case 0: MX_LAUNCH(0); break;
             case 1: MX_LAUNCH(1); break;
@@ -2796,7 +3051,7 @@ int rh264_batch_create(rh264_ctx *ctx, const rh264_stream *streams, int nstreams
     if (!out || !streams || !shapes || !lr_fp || !lr) return fail("batch_create: null argument");
     *out = nullptr;
     if (nstreams <= 0 || nshapes <= 0 || nshapes > 8) return fail("batch_create: %d streams, %d shapes", nstreams, nshapes);
-    if (range < 1 || range > 64 || subpel_pat < -1 || subpel_pat > 3 || pad < 4 || pad > 128 || pad % 4)
+    if (range < 1 || range > 64 || subpel_pat < -1 || subpel_pat > 4 || pad < 4 || pad > 128 || pad % 4)
         return RH264_DECLINED;
     for (int s = 0; s < nshapes; s++)
         if (!shape_ok(shapes[2 * s], shapes[2 * s + 1])) return RH264_DECLINED;

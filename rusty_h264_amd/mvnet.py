@@ -17,6 +17,7 @@ from .accel import DevBuf, _check, _need, _ptr
 NB, K, CH = 17, 289, 32
 LOCAL_RANGE = 2       # integer search of +-2 pels around each proposed centre
 FALLBACK_COST = 6000  # best candidate's integer cost above which a block is fully searched
+REFINE_PAT = 4        # sub-pel refinement of the proposal: the +-2 quarter-pel window (subpel_pat 4)
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "weights", "mvnet.npz")
 

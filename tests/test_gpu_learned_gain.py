@@ -31,7 +31,8 @@ def case(dev):
     ms = accel.MotionSearch(dev, p)
     ms.run_plain(cur, planes, NF)
     res = ms.results(NF)
-    return {"cur": cur, "refs": refs, "planes": planes, "p": p, "plain": res}
+    pl = accel.search_params(W, H, PAD, 16, 16, 32, mvnet.REFINE_PAT, LAM)
+    return {"cur": cur, "refs": refs, "planes": planes, "p": pl, "plain": res}
 
 
 def _learned(dev, case, params, fb_cost):

@@ -95,38 +95,39 @@ def test_backward_gradients(dev):
     net.close()
 
 
-@pytest.mark.parametrize("Rl,cand_quad,fallback_v", [(2, 1, 1), (1, 1, 1), (3, 1, 0), (4, 1, 1), (5, 1, 1),
-                                                      (2, 0, 0)])
-def test_learned_search_recosting_is_exact(dev, Rl, cand_quad, fallback_v):
+@pytest.mark.parametrize("Rl,cand_quad,fallback_v,pat", [(2, 1, 1, 4), (2, 1, 1, 2), (1, 1, 1, 4), (3, 1, 0, 2),
+                                                          (4, 1, 1, 4), (5, 1, 1, 2), (2, 0, 0, 4)])
+def test_learned_search_recosting_is_exact(dev, Rl, cand_quad, fallback_v, pat):
     """Every step after the network is exact: oracle local searches around the GPU's
-    eight centres, first minimum, fall-back full search, ring refine.  Run once
-    without fall-back and once with the threshold at the median candidate cost so
-    both paths are exercised.  Local ranges up to 3 take the quad-SAD candidate
-    kernel (cand_quad 1) or the v_sad_u8 one, 4 the latter, larger ones the
-    per-candidate integer searches; both fall-back search variants are covered."""
+    eight centres, first minimum, fall-back full search, then the refinement (the
+    +-2 quarter-pel window, pattern 4, or a ring walk).  Run once without fall-back
+    and once with the threshold at the median candidate cost so both paths are
+    exercised.  Local ranges up to 3 take the quad-SAD candidate kernel (cand_quad
+    1) or the v_sad_u8 one, 4 the latter, larger ones the per-candidate integer
+    searches; both fall-back search variants are covered."""
     dev.tune("cand_quad", cand_quad)
     dev.tune("fallback_v", fallback_v)
     try:
-        _learned_recost_case(dev, Rl)
+        _learned_recost_case(dev, Rl, pat)
     finally:
         dev.tune("cand_quad", 1)
         dev.tune("fallback_v", 1)
 
 
-def _learned_recost_case(dev, Rl):
+def _learned_recost_case(dev, Rl, pat):
     cw, ch, pad, Rf = 176, 144, 32, 16
     frames = synth.hd_clip(3, seed=31, w=cw, h=ch, nrects=3)
     nf = 2
     clip = dev.upload(frames)
     planes = accel.build_planes(dev, clip, nf, cw, ch, pad)
-    p = accel.search_params(cw, ch, pad, 16, 16, Rf, 2, LAM)
+    p = accel.search_params(cw, ch, pad, 16, 16, Rf, pat, LAM)
     net = mvnet.MVNet(dev, nf, cw, ch, _params(4))
     nblk = (cw // 16) * (ch // 16)
     t8 = dev.alloc(4 * nf * nblk * 8)
     net.forward(int(clip) + cw * ch, int(clip), cw * ch, nf, cw, ch, None, t8)
     top = dev.download(t8, (nf, nblk, 8), np.int32)
-    loc = O.cfg(cw, ch, pad, 16, 16, Rl, 2, LAM)
-    full = O.cfg(cw, ch, pad, 16, 16, Rf, 2, LAM)
+    loc = O.cfg(cw, ch, pad, 16, 16, Rl, pat, LAM)
+    full = O.cfg(cw, ch, pad, 16, 16, Rf, pat, LAM)
     opls = [O.planes(frames[f], cw, ch, pad) for f in range(nf)]
     cand = {}
     for f in range(nf):

@@ -164,7 +164,8 @@ def test_plain_search_shapes_small(dev, bw, bh):
     _search_case(dev, frames, 96, 64, 32, bw, bh, 7, 0, LAM26)
 
 
-@pytest.mark.parametrize("R,pat,lam", [(16, 0, LAM26), (16, 2, LAM26), (5, 1, 0.0), (13, 3, 9.0), (16, -1, LAM26)])
+@pytest.mark.parametrize("R,pat,lam", [(16, 0, LAM26), (16, 2, LAM26), (5, 1, 0.0), (13, 3, 9.0), (16, -1, LAM26),
+                                       (16, 4, LAM26)])
 def test_plain_search_cif(dev, R, pat, lam):
     """Config C1 geometry: CIF 352x288, 16x16, +-16 full search, qpel SATD refine."""
     frames = synth.cif_clip(4, seed=R + pat)
@@ -241,6 +242,65 @@ def test_subpel_refine_long_walks(dev, pat, refine_mfma):
         assert tuple(got["mv"][b]) == mv and got["cost"][b] == c and got["evals"][b] == ev, b
         far += max(abs(mv[0]), abs(mv[1])) > 16
     assert far > ms.nblk // 2  # most walks leave the +-4 pel window
+
+
+@pytest.mark.parametrize("refine_mfma", [1, 0])
+@pytest.mark.parametrize("bw,bh", [(16, 16), (16, 8), (8, 8), (4, 4)])
+def test_qpel_window_refine(dev, bw, bh, refine_mfma):
+    """subpel_pat 4 (the +-2 quarter-pel window) from given starts: integer-pel
+    starts (the window kernel's compile-time path), fractional starts, and starts
+    far outside the padded planes (clamped reads), with rate predictors.  mv, cost
+    and evaluation count bit-exact with the oracle."""
+    cw, ch, pad = 176, 144, 32
+    frames = synth.cif_clip(2, seed=bw + bh + refine_mfma, w=cw, h=ch)
+    dev.tune("refine_mfma", refine_mfma)
+    try:
+        cur, ref = dev.upload(frames[1]), dev.upload(frames[0])
+        pl = accel.build_planes(dev, ref, 1, cw, ch, pad)
+        p = accel.search_params(cw, ch, pad, bw, bh, 8, 4, LAM26)
+        ms = accel.MotionSearch(dev, p)
+        rng = np.random.default_rng(bw * bh)
+        nb = ms.nblk
+        starts = 4 * rng.integers(-12, 13, (nb, 2))
+        kind = rng.integers(0, 4, nb)
+        starts[kind == 1] += rng.integers(-3, 4, (int((kind == 1).sum()), 2))  # fractional
+        starts[kind == 2] = 4 * rng.integers(-70, 71, (int((kind == 2).sum()), 2))  # far outside the pad
+        starts = starts.astype(np.int32)
+        mvps = rng.integers(-60, 61, (nb, 2)).astype(np.int32)
+        ds, dm = dev.upload(starts), dev.upload(mvps)
+        ms.run_refine(cur, pl, 1, ds, dm)
+        got = ms.results(1, with_evals=True)
+    finally:
+        dev.tune("refine_mfma", 1)
+    cfg = O.cfg(cw, ch, pad, bw, bh, 8, 4, LAM26)
+    opl = O.planes(frames[0], cw, ch, pad)
+    nbx = cw // bw
+    for b in range(nb):
+        bx, by = (b % nbx) * bw, (b // nbx) * bh
+        mv, c, ev = O.subpel_refine_block(cfg, frames[1], opl, bx, by, *starts[b], *mvps[b])
+        assert tuple(got["mv"][b]) == mv and got["cost"][b] == c and got["evals"][b] == ev == 25, (b, starts[b])
+    for x in (cur, ref, pl, ds, dm):
+        x.free()
+
+
+def test_qpel_window_ties(dev):
+    """Ramp ref = 4x, cur = 4x + 1, lambda 0: the five dx = +1 positions tie at cost 0
+    and the raster-first (dy = -2) wins; a flat frame keeps the start."""
+    cw, ch, pad = 64, 48, 16
+    ramp = np.tile((4 * np.arange(cw) % 256).astype(np.uint8), (ch, 1))
+    for ref, cur_f, want in ((ramp, ramp + 1, (1, -2)), (np.full((ch, cw), 77, np.uint8),) * 2 + ((0, 0),)):
+        cur, refd = dev.upload(cur_f), dev.upload(ref)
+        pl = accel.build_planes(dev, refd, 1, cw, ch, pad)
+        p = accel.search_params(cw, ch, pad, 16, 16, 4, 4, 0.0)
+        ms = accel.MotionSearch(dev, p)
+        ds = dev.upload(np.zeros((ms.nblk, 2), np.int32))
+        ms.run_refine(cur, pl, 1, ds)
+        got = ms.results(1, with_evals=True)
+        # the interior column of blocks (x = 16, 32) sees an unwrapped ramp
+        for b in (1, 2, 5, 6):
+            assert tuple(got["mv"][b]) == want and got["cost"][b] == 0, b
+        for x in (cur, refd, pl, ds):
+            x.free()
 
 
 @pytest.mark.parametrize("fs_kernel", [0, 1, 2])

@@ -262,6 +262,67 @@ def test_subpel_refine_trace(pat):
             assert c <= cost(*start)
 
 
+def _window_python(cost, start):
+    """The +-2 quarter-pel window (subpel_pat 4): start first, then raster order."""
+    (sx, sy), (bx, by) = start, start
+    best = cost(sx, sy)
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            if (dx, dy) != (0, 0) and cost(sx + dx, sy + dy) < best:
+                best, bx, by = cost(sx + dx, sy + dy), sx + dx, sy + dy
+    return (bx, by), best
+
+
+def test_qpel_window_trace_and_ring_containment():
+    """Pattern 4 == the window over the textbook cost (spec samples + matrix
+    Hadamard + Exp-Golomb rate), integer and fractional starts; and wherever the
+    ring walk (pattern 0) ends inside the window, the window's cost is <= the
+    walk's (it prices a superset of those positions)."""
+    cw, ch, pad = 48, 32, 16
+    cur, ref = _pair(cw, ch, 27, shift=(1, 2))
+    pl = O.planes(ref, cw, ch, pad)
+    lam = 4.65
+    cfg = O.cfg(cw, ch, pad, 16, 16, 4, 4, lam)
+    ring = O.cfg(cw, ch, pad, 16, 16, 4, 0, lam)
+    lr = O.lambda_table(lam)
+    spec = refimpl.spec_samples(ref, pad)
+    contained = 0
+    for by in range(0, ch, 16):
+        for bx in range(0, cw, 16):
+            mvp = (3, -5)
+
+            def cost(mx, my, bx=bx, by=by):
+                x = bx + (mx >> 2) + pad
+                y = by + (my >> 2) + pad
+                pred = spec[(mx & 3, my & 3)][y:y + 16, x:x + 16]
+                return refimpl.satd_sum(cur[by:by + 16, bx:bx + 16], pred) + \
+                    lr[refimpl.mvbits(mx - mvp[0]) + refimpl.mvbits(my - mvp[1])]
+
+            for start in ((-8, 4), (-8, -4), (-7, -3), (-9, -6), (1, -1)):
+                mv, c, ev = O.subpel_refine_block(cfg, cur, pl, bx, by, *start, *mvp)
+                assert (mv, c) == _window_python(cost, start) and ev == 25
+                rmv, rc, _ = O.subpel_refine_block(ring, cur, pl, bx, by, *start, *mvp)
+                if abs(rmv[0] - start[0]) <= 2 and abs(rmv[1] - start[1]) <= 2:
+                    contained += 1
+                    assert c <= rc
+    assert contained >= 3
+
+
+def test_qpel_window_ties():
+    """Ties: a flat frame at lambda 0 prices every position alike and the start
+    wins; a horizontal ramp ref = 4x against cur = 4x + 1 is matched exactly by
+    the quarter-pel sample (4x + (4x + 2) + 1) >> 1 at every dy, so the five
+    (dx = +1) positions tie at cost 0 and the raster-first one, dy = -2, wins."""
+    cw, ch, pad = 48, 48, 16
+    flat = np.full((ch, cw), 77, np.uint8)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 4, 4, 0.0)
+    assert O.subpel_refine_block(cfg, flat, O.planes(flat, cw, ch, pad), 16, 16, 8, -4)[:2] == ((8, -4), 0)
+    ramp = np.tile((4 * np.arange(cw)).astype(np.uint8), (ch, 1))
+    cur = ramp + 1
+    mv, c, _ = O.subpel_refine_block(cfg, cur, O.planes(ramp, cw, ch, pad), 16, 16, 0, 0)
+    assert (mv, c) == ((1, -2), 0)
+
+
 def test_plain_search_frame_composes_blocks():
     cw, ch, pad = 64, 48, 32
     cur, ref = _pair(cw, ch, 30)
