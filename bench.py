@@ -55,7 +55,7 @@ def cpu_baseline(frames, lam, budget_s=12.0):
 MAX_REGRET = 0.01  # the learned path counts only at <= 1% mean SATD-cost regret
 
 
-def run_learned(dev, args, cur, refs, planes, p, plain_cost):
+def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
     """The learned-proposal path on the same clip: plane build + rh264_learned_search
     (network, 8 local searches, selection, fall-back full search, then the +-2
     quarter-pel window around the best integer vector: the 25 positions, exact SATD).
@@ -68,8 +68,10 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost):
     pbytes = accel.planes_bytes(W, H, PAD)
     fb_cost = mvnet.FALLBACK_COST if args.fallback_cost is None else args.fallback_cost
     net = mvnet.MVNet(dev, nf, W, H, mvnet.load_weights())
-    # the proposal's refinement: the +-2 quarter-pel window around the best integer vector
-    pl = accel.search_params(W, H, PAD, 16, 16, R, mvnet.REFINE_PAT, accel.lambda_me(QP))
+    # the proposal's refinement: the +-2 quarter-pel window around the best integer
+    # vector (or, for comparison, the plain search's ring walk, pattern 0)
+    pat = mvnet.REFINE_PAT if pat is None else pat
+    pl = accel.search_params(W, H, PAD, 16, 16, R, pat, accel.lambda_me(QP))
     outs = (dev.alloc(8 * nb), dev.alloc(4 * nb), dev.alloc(4 * nb))
     lib = dev.lib
 
@@ -104,7 +106,7 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost):
             "regret": round(float(cost.sum() / pc.sum() - 1.0), 6),
             "blocks_worse": round(float((cost > pc).mean()), 5), "blocks_better": round(float((cost < pc).mean()), 5),
             "fallback_rate": round(float(fb.mean()), 5), "local_range": mvnet.LOCAL_RANGE, "fallback_cost": fb_cost,
-            "refine": "+-2 quarter-pel window (subpel_pat 4)",
+            "refine": "+-2 quarter-pel window (subpel_pat 4)" if pat == 4 else f"ring walk (subpel_pat {pat})",
             "phase_ms": {"network": round(float(ph_mean[0]), 4), "candidates": round(float(ph_mean[1]), 4),
                          "refine": round(float(ph_mean[2]), 4)}}
 
@@ -182,9 +184,11 @@ def main():
             refine_ms.append(rf.value)
         lib.rh264_set_profiling(dev.ctx, 0)
         res = ms.results(nf, with_evals=False)
-        learned = None
+        learned = learned_ring = None
         if not args.no_learned:
             learned = run_learned(dev, args, cur, refs, planes, p, res["cost"])
+            # the same proposal refined by the plain search's ring walk, for comparison
+            learned_ring = run_learned(dev, args, cur, refs, planes, p, res["cost"], pat=0)
 
     ms_per_step = ev_ms / args.steps
     mpx = W * H * nf / (ms_per_step / 1e3) / 1e6
@@ -210,7 +214,7 @@ def main():
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
         "mean_cost": float(res["cost"].mean()),
         "paths": {"plain": {"mpx_s": round(mpx, 2), "ms_per_step": round(ms_per_step, 4), "regret": 0.0},
-                  "learned": learned},
+                  "learned": learned, "learned_ring": learned_ring},
     }
     if learned is not None and learned["regret"] <= MAX_REGRET and learned["mpx_s"] > mpx:
         # the headline is the faster path that meets the regret bar
@@ -293,9 +297,11 @@ def bench_lookahead4k(args):
     """4K mb-tree lookahead over a 40-frame window: 3840x2160 frames downsampled to
     half resolution (1920x1080, 8x8 blocks = one per 16x16 MB), intra AC-SATD and the
     seeded cascade diamond against the previous frame (the reference's HalfRes mode,
-    mbtree.rs:61-297), then int32 backward propagation over the 40 frames.  One step
-    = the whole window.  Mpx/s = 3840*2160*40 / step time.  The first three frames'
-    costs and the whole window's propagation are checked against the oracle."""
+    mbtree.rs:61-297), then int32 backward propagation over the 40 frames, and the
+    rate-control complexity of every full-resolution frame (lookahead.rs:38-101; frame
+    0 as the IDR, each other against its predecessor).  One step = the whole window.
+    Mpx/s = 3840*2160*40 / step time.  The first three frames' costs and complexities
+    and the whole window's propagation are checked against the oracle."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle.oracle import Oracle
@@ -309,6 +315,7 @@ def bench_lookahead4k(args):
         dname = dev.name()
         fb = dev.upload(frames)
         mt = accel.MbTree(dev, w, h, n)
+        cx = dev.alloc(8 * n)
         lib = dev.lib
 
         def step(ev=None):
@@ -325,6 +332,11 @@ def bench_lookahead4k(args):
                                                         mt.mv.ptr, mt.prop.ptr), "propagate")
             if ev is not None:
                 dev.record(ev + 3)
+            accel._check(lib.rh264_rc_complexity(dev.ctx, int(fb), None, w * h, 1, w, h, cx.ptr), "complexity")
+            accel._check(lib.rh264_rc_complexity(dev.ctx, int(fb) + w * h, int(fb), w * h, n - 1, w, h, cx.ptr + 8),
+                         "complexity")
+            if ev is not None:
+                dev.record(ev + 4)
 
         for _ in range(args.warmup):
             step()
@@ -339,8 +351,10 @@ def bench_lookahead4k(args):
         ms = dev.elapsed_ms(0, 1) / args.steps
         step(ev=2)  # one more, with phase events
         dev.sync()
-        phases = {"downsample": dev.elapsed_ms(2, 3), "costs": dev.elapsed_ms(3, 4), "propagate": dev.elapsed_ms(4, 5)}
+        phases = {"downsample": dev.elapsed_ms(2, 3), "costs": dev.elapsed_ms(3, 4), "propagate": dev.elapsed_ms(4, 5),
+                  "rc_complexity": dev.elapsed_ms(5, 6)}
         res = mt.results()
+        cplx = dev.download(cx, (n,), np.float64)
     # parity spot check: costs of frames 0..2 and the propagation from the GPU's own costs
     o = Oracle(fast=True)
     halves = [o.downsample2x(frames[f], w, h) for f in range(3)]
@@ -350,12 +364,15 @@ def bench_lookahead4k(args):
         ok = ok and np.array_equal(res["intra"][f], a) and np.array_equal(res["inter"][f], b) and \
             np.array_equal(res["mv"][f], m)
     ok = ok and np.array_equal(res["prop"], o.propagate_i32(res["intra"], res["inter"], res["mv"], mb_w, mb_h))
-    out = {"metric": "4K mb-tree lookahead Mpx/s (40-frame window, half-res 8x8 costs, int32 propagation)",
+    ok = ok and all(cplx[f] == o.rc_complexity(frames[f], frames[f - 1] if f else None, w, h) for f in range(3))
+    out = {"metric": "4K lookahead Mpx/s (40-frame window: half-res mb-tree costs, int32 propagation, "
+                     "RC complexity)",
            "value": round(w * h * n / (ms / 1e3) / 1e6, 2), "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
            "config": {"workload": "3840x2160, 40 frames, HalfRes mb-tree (1920x1080, 8x8 blocks, 1 ref = previous "
-                                  "frame, seeded 4-point cascade diamond steps 8/4/2/1), int32 backward propagation",
+                                  "frame, seeded 4-point cascade diamond steps 8/4/2/1), int32 backward propagation, "
+                                  "full-res RC complexity (9 full-pel candidates per MB)",
                       "device": dname},
            "phase_ms": {k: round(v, 4) for k, v in phases.items()},
            "parity_vs_oracle": "bit-exact" if ok else "MISMATCH",
@@ -368,12 +385,14 @@ def bench_lookahead4k(args):
         with ThreadPoolExecutor(cores) as ex:
             hv = list(ex.map(lambda f: o.downsample2x(frames[f], w, h), range(n)))
             cs = list(ex.map(lambda f: o.frame_costs_half(hv[f], hv[f - 1] if f else None, mb_w, mb_h), range(n)))
+            list(ex.map(lambda f: o.rc_complexity(frames[f], frames[f - 1] if f else None, w, h), range(n)))
         o.propagate_i32(np.stack([c[0] for c in cs]), np.stack([c[1] for c in cs]), np.stack([c[2] for c in cs]),
                         mb_w, mb_h)
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(w * h * n / dt / 1e6, 3), "unit": "Mpx/s", "cores": cores,
                                "kind": "port", "sample": f"the whole 40-frame window once (downsample, frame costs, "
-                                                         f"int32 propagation), {cores} threads, {dt:.1f} s"}
+                                                         f"int32 propagation, RC complexity), {cores} threads, "
+                                                         f"{dt:.1f} s"}
     print(json.dumps(out), flush=True)
 
 

@@ -290,6 +290,20 @@ int rh264_mbtree_costs(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_
 int rh264_mbtree_propagate_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, const int32_t *intra,
                                const int32_t *inter, const int32_t *mv, int32_t *prop);
 
+/* Rate-control frame complexity, lookahead::complexity (lookahead.rs:38-101), of
+ * nframes coded-size cw x ch luma frames (frame i at cur + i*frame_stride; cw, ch
+ * multiples of 16): out[i] = max(total, 1) as a double, where total sums over
+ * the macroblocks
+ *  - ref == NULL (IDR): intra_activity (55-72), the AC part (coefficient 0
+ *    excluded) of every 4x4 Hadamard of the pixels;
+ *  - otherwise: inter_activity (76-101), the minimum over the full-pel candidates
+ *    (0,0), (+-1,0), (0,+-1), (+-2,0), (0,+-2) of the residual's sum |H d| against
+ *    frame i of ref (ref + i*frame_stride), read with coordinates clamped into the
+ *    frame (mc_luma, inter.rs:120-124).
+ * out: nframes doubles in device memory. */
+int rh264_rc_complexity(rh264_ctx *ctx, const uint8_t *cur, const uint8_t *ref, size_t frame_stride, int nframes,
+                        int cw, int ch, double *out);
+
 #ifdef __cplusplus
 }
 #endif

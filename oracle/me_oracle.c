@@ -746,6 +746,43 @@ int32_t orc_inter_cost(const uint8_t *sy, int cw, int ch, const uint8_t *ref, in
     return best > INT32_MAX ? INT32_MAX : (int32_t)best;
 }
 
+/* lookahead::complexity, lookahead.rs:38-101: the rate-control score of a coded-size
+ * frame.  IDR (ref NULL): sum over macroblocks of intra_activity (55-72), the per-4x4
+ * AC SATD (coefficient 0 excluded).  P: sum over macroblocks of inter_activity
+ * (76-101), the minimum over nine full-pel candidates of the residual's 4x4 SATD
+ * against mc_luma of the reference.  Returned as max(total, 1) in f64. */
+double orc_rc_complexity(const uint8_t *sy, const uint8_t *ref, int cw, int ch)
+{
+    static const int cands[9][2] = {{0, 0}, {4, 0}, {-4, 0}, {0, 4}, {0, -4}, {8, 0}, {-8, 0}, {0, 8}, {0, -8}};
+    int64_t total = 0;
+    for (int my = 0; my < ch / 16; my++)
+        for (int mx = 0; mx < cw / 16; mx++) {
+            if (!ref) {
+                int64_t s = 0;
+                for (int by = 0; by < 4; by++)
+                    for (int bx = 0; bx < 4; bx++) {
+                        int32_t blk[16], h[16];
+                        for (int dy = 0; dy < 4; dy++)
+                            for (int dx = 0; dx < 4; dx++)
+                                blk[dy * 4 + dx] = sy[(size_t)(my * 16 + by * 4 + dy) * cw + mx * 16 + bx * 4 + dx];
+                        orc_hadamard_4x4(blk, h);
+                        for (int k = 1; k < 16; k++) s += iabs(h[k]);
+                    }
+                total += s;
+            } else {
+                int64_t best = INT64_MAX;
+                for (int c = 0; c < 9; c++) {
+                    uint8_t pred[256];
+                    orc_mc_luma(ref, cw, ch, mx * 16, my * 16, 16, 16, cands[c][0], cands[c][1], pred);
+                    int64_t s = orc_satd_px(sy + (size_t)my * 16 * cw + mx * 16, cw, pred, 16, 16, 16);
+                    if (s < best) best = s;
+                }
+                total += best;
+            }
+        }
+    return total < 1 ? 1.0 : (double)total;
+}
+
 /* Rust i32 `/` truncates toward zero (C99 does too). */
 void orc_frame_costs_half(const uint8_t *half, const uint8_t *ref_half, int mb_w, int mb_h,
                           int32_t *intra, int32_t *inter, int32_t *mv)

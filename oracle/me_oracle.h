@@ -134,6 +134,10 @@ void orc_propagate_f64(int n, int mb_w, int mb_h, const int32_t *intra, const in
 void orc_propagate_i32(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
                        const int32_t *mv, int32_t *prop_out);
 
+/* lookahead::complexity (lookahead.rs:38-101): rate-control score of one coded-size
+ * frame, IDR when ref is NULL; max(total, 1). */
+double orc_rc_complexity(const uint8_t *sy, const uint8_t *ref, int cw, int ch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,7 @@ class Oracle:
         L.orc_satd_avg.restype = C.c_uint32
         L.orc_mvbits.restype = C.c_uint32
         L.orc_inter_cost.restype = C.c_int32
+        L.orc_rc_complexity.restype = C.c_double
 
     # ---- block metrics
     def sad(self, a, sa, b, sb, w, h):
@@ -185,6 +186,11 @@ class Oracle:
         c = self.lib.orc_inter_cost(_p(_u8(sy)), cw, ch, _p(_u8(ref)), bx0, by0, bs, seed[0], seed[1], max_step,
                                     _p(mv, i32p))
         return int(c), (int(mv[0]), int(mv[1]))
+
+    def rc_complexity(self, y, ref, cw, ch):
+        """lookahead::complexity of one coded-size frame (ref None = IDR)."""
+        rp = _p(_u8(ref)) if ref is not None else None
+        return float(self.lib.orc_rc_complexity(_p(_u8(y)), rp, cw, ch))
 
     def frame_costs_half(self, half, ref_half, mb_w, mb_h):
         n = mb_w * mb_h

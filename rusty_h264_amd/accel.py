@@ -336,6 +336,16 @@ class MbTree:
                 "prop": d.download(self.prop, (n, nm), np.int32)}
 
 
+def rc_complexity(dev, cur, ref, nframes, cw, ch, frame_stride=None, out=None):
+    """lookahead::complexity of nframes device-resident frames (ref None = IDR); returns
+    the nframes scores as float64 (lookahead.rs:38-101)."""
+    if out is None:
+        out = dev.alloc(8 * nframes)
+    _check(dev.lib.rh264_rc_complexity(dev.ctx, int(cur), int(ref) if ref is not None else None,
+                                       frame_stride or cw * ch, nframes, cw, ch, out.ptr), "rh264_rc_complexity")
+    return dev.download(out, (nframes,), np.float64)
+
+
 def ref_bits(r, nrefs):
     """ref_idx rate for the motion-search cost (mb16.rs:5967-5980)."""
     if nrefs <= 1:

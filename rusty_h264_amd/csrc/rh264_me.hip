@@ -1615,6 +1615,87 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
 }
 
 // ---------------------------------------------------------------------------
+// Rate-control frame complexity (lookahead::complexity, lookahead.rs:38-101), one
+// wave per macroblock, with the refinement's matrix-core 4x4 Hadamards:
+//  * IDR: the AC part of the sixteen transforms of the pixels (intra_activity,
+//    55-72); the DC coefficient of each 4x4 is element 0 of lanes l % 4 == 0;
+//  * P: the minimum over the nine full-pel candidates (0,0), (+-1,0), (0,+-1),
+//    (+-2,0), (0,+-2) of the residual's sum |H d| (inter_activity, 76-101), the
+//    prediction read from the reference frame with coordinates clamped into it.
+// Each wave scores RC_MB_PER_WAVE macroblocks (strided over the frame), the four
+// waves of a workgroup add their sums in LDS, and one atomic add per workgroup
+// accumulates the frame's 64-bit total (integer, so the order of the adds does not
+// matter); k_rc_finalize writes max(total, 1).
+// ---------------------------------------------------------------------------
+
+constexpr int RC_MB_PER_WAVE = 16;
+
+__device__ __forceinline__ int rc_mb_score(const uint8_t *__restrict__ cf, const uint8_t *__restrict__ rf, int cw,
+                                           int ch, int x0, int y0, int lane, const MxConst &K)
+{
+    const int r = lane & 15, qd = lane >> 4;
+    const uint32_t src = *(const uint32_t *)(cf + (size_t)(y0 + r) * cw + x0 + 4 * qd);
+    int score;
+    if (!rf) {
+        const mx_f4 y = mx_transform(src, K, mx_f4{0.0f, 0.0f, 0.0f, 0.0f});
+        const float s = ((lane & 3) ? fabsf(y[0]) : 0.0f) + fabsf(y[1]) + (fabsf(y[2]) + fabsf(y[3]));
+        score = mx_wave_sum((int)s);
+    } else {
+        const mx_f4 c2 = -mx_transform(src, K, mx_f4{0.0f, 0.0f, 0.0f, 0.0f});
+        // v[0..4]: dy = 0 with dx = 0, +1, -1, +2, -2; v[5..8]: dx = 0 with dy = +1, -1, +2, -2
+        uint32_t v[9];
+        const int col = x0 + 4 * qd;
+        if (x0 >= 4 && y0 >= 2 && x0 + 20 <= cw && y0 + 18 <= ch) {
+            // aligned dwords: every byte read lies inside the frame
+            const uint32_t *row = (const uint32_t *)(rf + (size_t)(y0 + r) * cw + col);
+            const uint32_t d0 = row[-1], d1 = row[0], d2 = row[1];
+            v[0] = d1;
+            v[1] = __builtin_amdgcn_alignbyte(d2, d1, 1u);
+            v[2] = __builtin_amdgcn_alignbyte(d1, d0, 3u);
+            v[3] = __builtin_amdgcn_alignbyte(d2, d1, 2u);
+            v[4] = __builtin_amdgcn_alignbyte(d1, d0, 2u);
+            v[5] = *(const uint32_t *)(rf + (size_t)(y0 + r + 1) * cw + col);
+            v[6] = *(const uint32_t *)(rf + (size_t)(y0 + r - 1) * cw + col);
+            v[7] = *(const uint32_t *)(rf + (size_t)(y0 + r + 2) * cw + col);
+            v[8] = *(const uint32_t *)(rf + (size_t)(y0 + r - 2) * cw + col);
+        } else {
+            constexpr int DX[9] = {0, 1, -1, 2, -2, 0, 0, 0, 0}, DY[9] = {0, 0, 0, 0, 0, 1, -1, 2, -2};
+#pragma unroll
+            for (int c = 0; c < 9; c++) v[c] = load4_clamped(rf, cw, ch, col + DX[c], y0 + r + DY[c]);
+        }
+        score = mx_satd(v[0], K, c2);
+#pragma unroll
+        for (int c = 1; c < 9; c++) score = min(score, mx_satd(v[c], K, c2));
+    }
+    return score;
+}
+
+__global__ __launch_bounds__(256) void k_rc_complexity(const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref,
+                                                       size_t frame_stride, int cw, int ch,
+                                                       unsigned long long *__restrict__ total)
+{
+    __shared__ unsigned long long s_sum[4];
+    const int mbw = cw >> 4, nmb = mbw * (ch >> 4);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int frame = blockIdx.y;
+    const uint8_t *cf = cur + (size_t)frame * frame_stride;
+    const uint8_t *rf = ref ? ref + (size_t)frame * frame_stride : nullptr;
+    const MxConst K = mx_const(lane);
+    unsigned long long sum = 0; // wave-uniform
+    for (int m = blockIdx.x * 4 + wave; m < nmb; m += gridDim.x * 4)
+        sum += (unsigned)rc_mb_score(cf, rf, cw, ch, (m % mbw) << 4, (m / mbw) << 4, lane, K);
+    if (lane == 0) s_sum[wave] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(total + frame, s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3]);
+}
+
+__global__ void k_rc_finalize(const unsigned long long *__restrict__ total, int n, double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = total[i] < 1ull ? 1.0 : (double)total[i];
+}
+
+// ---------------------------------------------------------------------------
 // batched multi-stream search
 // ---------------------------------------------------------------------------
 //
@@ -2952,6 +3033,31 @@ int rh264_mbtree_costs(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_
                            inter, mv);
         LAUNCH_CHECK("k_mbtree_rows");
     }
+    return 0;
+}
+
+int rh264_rc_complexity(rh264_ctx *ctx, const uint8_t *cur, const uint8_t *ref, size_t frame_stride, int nframes,
+                        int cw, int ch, double *out)
+{
+    if (cw <= 0 || ch <= 0 || cw % 16 || ch % 16) return RH264_DECLINED;
+    if (frame_stride % 4 || frame_stride < (size_t)cw * ch)
+        return fail("rc_complexity: frame stride must be a multiple of 4 and at least cw*ch");
+    if (!cur || !out) return fail("rc_complexity: null frame or output");
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    int rc = ensure_keys(ctx, (size_t)nframes);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nframes + 255) / 256)), dim3(256), 0, ctx->stream, ctx->keys,
+                       (size_t)nframes, 0ull);
+    LAUNCH_CHECK("k_fill_u64");
+    const int nmb = (cw / 16) * (ch / 16);
+    const int groups = (nmb + 4 * RC_MB_PER_WAVE - 1) / (4 * RC_MB_PER_WAVE);
+    hipLaunchKernelGGL(k_rc_complexity, dim3((unsigned)groups, nframes), dim3(256), 0, ctx->stream, cur, ref,
+                       frame_stride, cw, ch, (unsigned long long *)ctx->keys);
+    LAUNCH_CHECK("k_rc_complexity");
+    hipLaunchKernelGGL(k_rc_finalize, dim3((unsigned)((nframes + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const unsigned long long *)ctx->keys, nframes, out);
+    LAUNCH_CHECK("k_rc_finalize");
     return 0;
 }
 

@@ -67,3 +67,34 @@ def test_mbtree_static_and_flat(dev):
     mt2.run(fb2)
     r2 = mt2.results()
     assert (r2["intra"] == 1).all() and (r2["inter"][0] == 1).all() and (r2["inter"][1] == 0).all()
+
+
+@pytest.mark.parametrize("cw,ch,n,kind", [(16, 16, 3, "noise"), (64, 48, 4, "noise"), (352, 288, 4, "cif"),
+                                          (1920, 1088, 3, "hd")])
+def test_rc_complexity_bit_exact(dev, cw, ch, n, kind):
+    """lookahead::complexity: IDR and P scores of every frame equal the oracle's
+    exactly (frames small enough that every macroblock takes the clamped path,
+    and large ones where most take the aligned path)."""
+    if kind == "cif":
+        frames = synth.cif_clip(n + 1, seed=cw, w=cw, h=ch)
+    elif kind == "hd":
+        frames = synth.hd_clip(n + 1, seed=9, w=cw, h=ch)
+    else:
+        frames = np.random.default_rng(cw * ch).integers(0, 256, (n + 1, ch, cw), dtype=np.uint8)
+    fb = dev.upload(frames)
+    cur, ref = int(fb) + cw * ch, int(fb)
+    got_p = accel.rc_complexity(dev, cur, ref, n, cw, ch)
+    got_i = accel.rc_complexity(dev, cur, None, n, cw, ch)
+    for i in range(n):
+        assert got_p[i] == O.rc_complexity(frames[i + 1], frames[i], cw, ch), i
+        assert got_i[i] == O.rc_complexity(frames[i + 1], None, cw, ch), i
+    fb.free()
+
+
+def test_rc_complexity_floor(dev):
+    """Flat IDR frames and P frames equal to their reference score the floor 1."""
+    f = np.full((2, 48, 64), 123, np.uint8)
+    fb = dev.upload(f)
+    assert (accel.rc_complexity(dev, int(fb), None, 2, 64, 48) == 1.0).all()
+    assert (accel.rc_complexity(dev, int(fb) + 64 * 48, int(fb), 1, 64, 48) == 1.0).all()
+    fb.free()

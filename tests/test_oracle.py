@@ -353,6 +353,61 @@ def test_motion_search_ref_invariants():
             assert c <= seed_c
 
 
+# ---------------------------------------------------------------- RC complexity
+
+def _rc_textbook(y, ref):
+    """lookahead.rs:38-101 by matrix Hadamards and numpy edge-clamped indexing."""
+    ch, cw = y.shape
+    yi = y.astype(np.int64)
+    total = 0
+    for my in range(0, ch, 16):
+        for mx in range(0, cw, 16):
+            blk = yi[my:my + 16, mx:mx + 16]
+            if ref is None:
+                for by in range(0, 16, 4):
+                    for bx in range(0, 16, 4):
+                        h = np.abs(refimpl.hadamard(blk[by:by + 4, bx:bx + 4]))
+                        total += int(h.sum() - h[0, 0])
+            else:
+                best = None
+                for dx, dy in ((0, 0), (1, 0), (-1, 0), (0, 1), (0, -1), (2, 0), (-2, 0), (0, 2), (0, -2)):
+                    rows = np.clip(np.arange(my, my + 16) + dy, 0, ch - 1)
+                    cols = np.clip(np.arange(mx, mx + 16) + dx, 0, cw - 1)
+                    pred = ref[np.ix_(rows, cols)]
+                    sv = refimpl.satd_sum(y[my:my + 16, mx:mx + 16], pred)
+                    best = sv if best is None else min(best, sv)
+                total += best
+    return float(max(total, 1))
+
+
+def test_rc_complexity_textbook_and_closed_forms():
+    """IDR and P scores == the textbook restatement (random, edge and clip content);
+    a flat IDR frame and a P frame equal to its reference score 1 (the floor); a frame
+    moved one pel right against its reference scores at most the residual of its
+    first pixel column (every other pixel has an exact candidate)."""
+    rng = np.random.default_rng(51)
+    for cw, ch in ((48, 32), (64, 48)):
+        y = rng.integers(0, 256, (ch, cw), dtype=np.uint8)
+        ref = rng.integers(0, 256, (ch, cw), dtype=np.uint8)
+        assert O.rc_complexity(y, None, cw, ch) == _rc_textbook(y, None)
+        assert O.rc_complexity(y, ref, cw, ch) == _rc_textbook(y, ref)
+    cur, ref = _pair(64, 48, 52, shift=(0, 1))
+    assert O.rc_complexity(cur, ref, 64, 48) == _rc_textbook(cur, ref)
+    flat = np.full((32, 48), 200, np.uint8)
+    assert O.rc_complexity(flat, None, 48, 32) == 1.0
+    assert O.rc_complexity(ref, ref, 64, 48) == 1.0
+    moved = np.empty_like(ref)
+    moved[:, 1:] = ref[:, :-1]
+    moved[:, 0] = 255 - ref[:, 0]  # not the edge-clamped sample, so column 0 has a residual
+    got = O.rc_complexity(moved, ref, 64, 48)
+    # the left macroblocks' best candidate is dx = -1 (exact except pixel column 0)
+    left = 0
+    for my in range(0, 48, 16):
+        d = moved[my:my + 16, :16].astype(np.int64) - ref[my:my + 16, np.clip(np.arange(16) - 1, 0, 63)]
+        left += sum(int(np.abs(refimpl.hadamard(d[y:y + 4, 0:4])).sum()) for y in range(0, 16, 4))
+    assert got == _rc_textbook(moved, ref) and 1 < got <= left
+
+
 # ---------------------------------------------------------------- mb-tree
 
 def test_downsample_and_intra_cost():
