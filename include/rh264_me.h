@@ -72,11 +72,16 @@ int rh264_event_elapsed_ms(rh264_ctx *ctx, int slot_a, int slot_b, float *ms);
  *       RH264_METRIC_SATD_WELS  sum_4x4 ((sum|H d|+1)>>1)    (satd_region_scalar)
  *       RH264_METRIC_SATD       sum_4x4 sum|H d|             (satd_x4 / satd_px domain)
  *       RH264_METRIC_SATD_AVG   sum|H (src - avg(a,b))|      (satd_avg)
+ *       RH264_METRIC_SATD8      sum_8x8 sum|H8 d H8| / 2     (no reference
+ *         counterpart: the 8x8 Hadamard, H8 the order-8 Sylvester matrix; every
+ *         coefficient has the parity of sum d, so the halving is exact; w, h
+ *         multiples of 8)
  * Shapes: (w,h) in {16x16,16x8,8x16,8x8,8x4,4x8,4x4}; others decline. */
 #define RH264_METRIC_SAD 0
 #define RH264_METRIC_SATD_WELS 1
 #define RH264_METRIC_SATD 2
 #define RH264_METRIC_SATD_AVG 3
+#define RH264_METRIC_SATD8 4
 int rh264_block_metric(rh264_ctx *ctx, int kind, int w, int h, int n, const uint8_t *src, int ss,
                        const int64_t *src_off, const uint8_t *ref, int rs, const int64_t *ref_off,
                        const int64_t *ref_off2, int32_t *out);
@@ -113,6 +118,11 @@ typedef struct {
                               refinement                                          */
     int32_t lr_fp[RH264_LR_N];
     int32_t lr[RH264_LR_N];
+    int32_t hadamard;      /* SATD transform of the sub-pel stage: 4 (or 0) =
+                              the reference's 4x4 sums; 8 = the 8x8 Hadamard,
+                              sum over 8x8 blocks of sum|H8 d H8| / 2 (no
+                              reference counterpart; 16x16 blocks only, other
+                              shapes decline)                                     */
 } rh264_search_params;
 
 void rh264_lambda_table(double lambda, int32_t *lr /* RH264_LR_N */);

@@ -105,6 +105,43 @@ int64_t orc_satd_px(const uint8_t *src, int ss, const uint8_t *pred, int ps, int
 
 /* satd_avg, crates/rusty_h264-accel/src/satd_avg.rs:579-602: sum |H (src - (a+b+1)>>1)|,
  * unhalved (the satd_px value of the materialized average). */
+/* 8-point Sylvester Hadamard of 8 values (stride st), in place: three butterfly
+ * stages over index bits 2, 1, 0; output k is sum_j (-1)^popcount(j & k) x_j. */
+static void had8(int32_t *x, int st)
+{
+    for (int half = 4; half >= 1; half >>= 1)
+        for (int i = 0; i < 8; i++)
+            if (!(i & half)) {
+                int32_t a = x[i * st], b = x[(i + half) * st];
+                x[i * st] = a + b;
+                x[(i + half) * st] = a - b;
+            }
+}
+
+void orc_hadamard_8x8(const int32_t in[64], int32_t out[64])
+{
+    memcpy(out, in, 64 * sizeof(int32_t));
+    for (int r = 0; r < 8; r++) had8(out + 8 * r, 1);
+    for (int c = 0; c < 8; c++) had8(out + c, 8);
+}
+
+int64_t orc_satd8_px(const uint8_t *src, int ss, const uint8_t *pred, int ps, int w, int h)
+{
+    int64_t s = 0;
+    for (int by = 0; by < h; by += 8)
+        for (int bx = 0; bx < w; bx += 8) {
+            int32_t d[64], co[64];
+            for (int y = 0; y < 8; y++)
+                for (int x = 0; x < 8; x++)
+                    d[y * 8 + x] = (int)src[(by + y) * ss + bx + x] - (int)pred[(by + y) * ps + bx + x];
+            orc_hadamard_8x8(d, co);
+            int64_t t = 0;
+            for (int k = 0; k < 64; k++) t += iabs(co[k]);
+            s += t / 2;
+        }
+    return s;
+}
+
 uint32_t orc_satd_avg(const uint8_t *src, int ss, const uint8_t *a, const uint8_t *b,
                       int stride, int w, int h)
 {
@@ -495,6 +532,8 @@ static int64_t satd_dist(const orc_search_cfg *cfg, const uint8_t *cur, const ui
 {
     uint8_t pred[256];
     orc_pred_block(planes, cfg->cw, cfg->ch, cfg->pad, lx, ly, cfg->bw, cfg->bh, mvx, mvy, pred);
+    if (cfg->hadamard == 8)
+        return orc_satd8_px(cur + (size_t)ly * cfg->cw + lx, cfg->cw, pred, cfg->bw, cfg->bw, cfg->bh);
     return orc_satd_px(cur + (size_t)ly * cfg->cw + lx, cfg->cw, pred, cfg->bw, cfg->bw, cfg->bh);
 }
 

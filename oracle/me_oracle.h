@@ -73,6 +73,7 @@ typedef struct {
                           4: quarter-pel window (build-defined, below)          */
     const int64_t *lr_fp;  /* SAD-phase lambda table, ORC_LR_N entries        */
     const int64_t *lr;     /* SATD-phase lambda table, ORC_LR_N entries       */
+    int hadamard;          /* sub-pel SATD transform: 4 (or 0) 4x4, 8 = 8x8     */
 } orc_search_cfg;
 
 /* Plain exhaustive integer search of one block (build-defined; raster
@@ -133,6 +134,13 @@ void orc_propagate_f64(int n, int mb_w, int mb_h, const int32_t *intra, const in
  * * (intra-inter) / intra) in int64 then per-target weights (wx*wy) >> 8. */
 void orc_propagate_i32(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
                        const int32_t *mv, int32_t *prop_out);
+
+/* Build-defined 8x8 Hadamard SATD (no reference counterpart): sum over the 8x8
+ * blocks of a w x h region (w, h multiples of 8) of sum |H8 d H8^T| / 2, H8 the
+ * order-8 Sylvester matrix (every coefficient has the parity of sum d, so the sum
+ * is even and the halving exact). */
+void orc_hadamard_8x8(const int32_t in[64], int32_t out[64]);
+int64_t orc_satd8_px(const uint8_t *src, int ss, const uint8_t *pred, int ps, int w, int h);
 
 /* lookahead::complexity (lookahead.rs:38-101): rate-control score of one coded-size
  * frame, IDR when ref is NULL; max(total, 1). */

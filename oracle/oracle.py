@@ -20,7 +20,8 @@ i64p = C.POINTER(C.c_int64)
 
 class SearchCfg(C.Structure):
     _fields_ = [("cw", C.c_int), ("ch", C.c_int), ("pad", C.c_int), ("bw", C.c_int), ("bh", C.c_int),
-                ("range", C.c_int), ("subpel_pat", C.c_int), ("lr_fp", i64p), ("lr", i64p)]
+                ("range", C.c_int), ("subpel_pat", C.c_int), ("lr_fp", i64p), ("lr", i64p),
+                ("hadamard", C.c_int)]
 
 
 class DiamondCfg(C.Structure):
@@ -64,7 +65,7 @@ class Oracle:
         for n in ("orc_sad", "orc_satd4x4_wels", "orc_satd_wels", "orc_intra_cost"):
             getattr(L, n).restype = C.c_int32
         for n in ("orc_satd_px", "orc_full_search_block", "orc_satd_cost", "orc_subpel_refine_block",
-                  "orc_motion_search_ref", "orc_mbtree_mc_satd"):
+                  "orc_motion_search_ref", "orc_mbtree_mc_satd", "orc_satd8_px"):
             getattr(L, n).restype = C.c_int64
         L.orc_satd_avg.restype = C.c_uint32
         L.orc_mvbits.restype = C.c_uint32
@@ -80,6 +81,9 @@ class Oracle:
 
     def satd_px(self, a, sa, b, sb, w, h):
         return self.lib.orc_satd_px(_p(_u8(a)), sa, _p(_u8(b)), sb, w, h)
+
+    def satd8_px(self, src, ss, pred, ps, w, h):
+        return int(self.lib.orc_satd8_px(_p(_u8(src)), ss, _p(_u8(pred)), ps, w, h))
 
     def satd_avg(self, src, ss, a, b, stride, w, h):
         return self.lib.orc_satd_avg(_p(_u8(src)), ss, _p(_u8(a)), _p(_u8(b)), stride, w, h)
@@ -127,10 +131,10 @@ class Oracle:
         self.lib.orc_lambda_table(C.c_double(lam), LR_N, _p(lr, i64p))
         return lr
 
-    def cfg(self, cw, ch, pad=32, bw=16, bh=16, search_range=32, subpel_pat=0, lam=0.0, lam_fp=None):
+    def cfg(self, cw, ch, pad=32, bw=16, bh=16, search_range=32, subpel_pat=0, lam=0.0, lam_fp=None, hadamard=4):
         lr = self.lambda_table(lam)
         lr_fp = self.lambda_table(lam * 0.5 if lam_fp is None else lam_fp)
-        c = SearchCfg(cw, ch, pad, bw, bh, search_range, subpel_pat, _p(lr_fp, i64p), _p(lr, i64p))
+        c = SearchCfg(cw, ch, pad, bw, bh, search_range, subpel_pat, _p(lr_fp, i64p), _p(lr, i64p), hadamard)
         c._keep = (lr, lr_fp)
         return c
 

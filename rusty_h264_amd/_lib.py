@@ -24,7 +24,7 @@ class SearchParams(C.Structure):
         ("cw", C.c_int32), ("ch", C.c_int32), ("pad", C.c_int32),
         ("bw", C.c_int32), ("bh", C.c_int32), ("range", C.c_int32),
         ("subpel_pat", C.c_int32),
-        ("lr_fp", C.c_int32 * LR_N), ("lr", C.c_int32 * LR_N),
+        ("lr_fp", C.c_int32 * LR_N), ("lr", C.c_int32 * LR_N), ("hadamard", C.c_int32),
     ]
 
 

@@ -23,7 +23,7 @@ import numpy as np
 from . import _lib
 
 LR_N = _lib.LR_N
-METRIC_SAD, METRIC_SATD_WELS, METRIC_SATD, METRIC_SATD_AVG = 0, 1, 2, 3
+METRIC_SAD, METRIC_SATD_WELS, METRIC_SATD, METRIC_SATD_AVG, METRIC_SATD8 = 0, 1, 2, 3, 4
 SHAPES = ((16, 16), (16, 8), (8, 16), (8, 8), (8, 4), (4, 8), (4, 4))
 HPEL_PAD_DEFAULT = 32  # inter.rs:761
 
@@ -201,11 +201,13 @@ def build_planes(dev, frames_buf, nframes, cw, ch, pad=HPEL_PAD_DEFAULT, out=Non
 
 
 def search_params(cw, ch, pad=HPEL_PAD_DEFAULT, bw=16, bh=16, search_range=32, subpel_pat=0,
-                  lam=0.0, lam_fp=None):
+                  lam=0.0, lam_fp=None, hadamard=4):
     """rh264_search_params; lam prices the SATD phase, lam_fp (default lam*0.5, the
-    reference's RFF_ME_SADL default, mb16.rs:756-763) the SAD full-pel phase."""
+    reference's RFF_ME_SADL default, mb16.rs:756-763) the SAD full-pel phase;
+    hadamard 8 selects the 8x8 transform for the sub-pel stage (16x16 blocks)."""
     p = _lib.SearchParams()
     p.cw, p.ch, p.pad, p.bw, p.bh, p.range, p.subpel_pat = cw, ch, pad, bw, bh, search_range, subpel_pat
+    p.hadamard = hadamard
     lr = lambda_table(lam)
     lr_fp = lambda_table(lam * 0.5 if lam_fp is None else lam_fp)
     for i in range(LR_N):

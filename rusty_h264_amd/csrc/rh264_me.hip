@@ -1101,16 +1101,20 @@ struct MxConst {
     mx_f4 c1; // accumulator start of the first product
 };
 
-__device__ __forceinline__ MxConst mx_const(int lane)
+// T8: Hb = diag(H8, H8) (the 8x8 transform, build-defined): |X| <= 2040 < 2048 and
+// |Y| <= 16320, still exact; the DC columns of the first product start at -8192.
+template <bool T8 = false> __device__ __forceinline__ MxConst mx_const(int lane)
 {
     MxConst K;
     const int q = lane >> 4, n = lane & 15;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        // Hb[4q + i][n]: nonzero inside the 4x4 diagonal blocks, H4[i][n & 3] = (-1)^popc(i & n)
-        const float h = (q == (n >> 2)) ? ((__popc(i & n & 3) & 1) ? -1.0f : 1.0f) : 0.0f;
+        const int k = 4 * q + i; // row of Hb
+        // Hb[k][n]: nonzero inside the diagonal blocks, H[k][n] = (-1)^popc(k & n) within a block
+        const float h = T8 ? (((k >> 3) == (n >> 3)) ? ((__popc(k & n & 7) & 1) ? -1.0f : 1.0f) : 0.0f)
+                           : ((q == (n >> 2)) ? ((__popc(i & n & 3) & 1) ? -1.0f : 1.0f) : 0.0f);
         K.hb[i] = (_Float16)h;
-        K.c1[i] = (n & 3) == 0 ? -4096.0f : 0.0f;
+        K.c1[i] = T8 ? ((n & 7) == 0 ? -8192.0f : 0.0f) : ((n & 3) == 0 ? -4096.0f : 0.0f);
     }
     return K;
 }
@@ -1144,11 +1148,14 @@ __device__ __forceinline__ mx_f4 mx_transform(uint32_t p, const MxConst &K, mx_f
     return __builtin_amdgcn_mfma_f32_16x16x16f16(K.hb, b, c2, 0, 0, 0);
 }
 
-__device__ __forceinline__ int mx_satd(uint32_t p, const MxConst &K, mx_f4 c2)
+// SATD of the candidate: the wave's sum |Y|, halved for the 8x8 transform (exact:
+// the sum over each 8x8 block is even)
+template <bool T8 = false> __device__ __forceinline__ int mx_satd(uint32_t p, const MxConst &K, mx_f4 c2)
 {
     const mx_f4 y = mx_transform(p, K, c2);
     const float s = (fabsf(y[0]) + fabsf(y[1])) + (fabsf(y[2]) + fabsf(y[3]));
-    return mx_wave_sum((int)s);
+    const int t = mx_wave_sum((int)s);
+    return T8 ? t >> 1 : t;
 }
 
 // 4 bytes at a dword-aligned LDS address plus a byte shift of 0..3.
@@ -1193,15 +1200,16 @@ __device__ __forceinline__ uint32_t mx_operand_far(const MxBlock &B, int lane, i
 // Cost of a candidate outside the staged window: the plane set in HBM, clamped at
 // its border, and the rate computed directly.  Kept out of line so that its table
 // reads and address arithmetic stay off the common path.
+template <bool T8>
 __device__ __attribute__((noinline)) int mx_cost_far(const MxBlock &B, const MxConst &K, mx_f4 c2,
                                                       const int32_t *s_lr, int lane, int rx, int ry)
 {
     const int mx = B.sx + rx, my = B.sy + ry;
-    const int s = mx_satd(mx_operand_far(B, lane, rx, ry), K, c2);
+    const int s = mx_satd<T8>(mx_operand_far(B, lane, rx, ry), K, c2);
     return __builtin_amdgcn_readfirstlane(s + s_lr[mvbits(mx - B.mpx) + mvbits(my - B.mpy)]);
 }
 
-template <int PAT>
+template <int PAT, bool T8>
 __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, const uint8_t *__restrict__ cur,
                                                      size_t cur_stride, const uint8_t *__restrict__ planes,
                                                      size_t planes_stride, const uint64_t *__restrict__ keys,
@@ -1298,13 +1306,13 @@ __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, cons
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     const uint8_t *lwin = win + r * MX_PITCH + 4 * qd; // this lane's row and column quad
-    const MxConst K = mx_const(lane);
+    const MxConst K = mx_const<T8>(lane);
     mx_f4 c2 = -mx_transform(src, K, mx_f4{0.0f, 0.0f, 0.0f, 0.0f});
 
     auto cost_of = [&](int rx, int ry) -> int {
         if (!rate_fast || (unsigned)(rx + 4 * MX_WL) >= (unsigned)(8 * MX_WL) ||
             (unsigned)(ry + 4 * MX_WL) >= (unsigned)(8 * MX_WL))
-            return __builtin_amdgcn_readfirstlane(mx_cost_far(B, K, c2, s_lr, lane, rx, ry));
+            return __builtin_amdgcn_readfirstlane(mx_cost_far<T8>(B, K, c2, s_lr, lane, rx, ry));
         const int ph = ((rx & 3) << 2) | (ry & 3);
         const uint32_t qo = __builtin_amdgcn_readlane(t_off, ph);
         // byte offsets (wave-uniform) of the operands' top-left samples in the staged
@@ -1315,7 +1323,7 @@ __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, cons
         const uint32_t v = __builtin_amdgcn_lerp(mx_lds4(lwin + (oa & ~3), oa & 3), mx_lds4(lwin + (ob & ~3), ob & 3),
                                                  0x01010101u);
         const int bits = __builtin_amdgcn_readlane(t_bx, rx + 32) + __builtin_amdgcn_readlane(t_by, ry + 32);
-        return mx_satd(v, K, c2) + __builtin_amdgcn_readlane(t_lr, bits);
+        return mx_satd<T8>(v, K, c2) + __builtin_amdgcn_readlane(t_lr, bits);
     };
 
     // Ring tables: lane i < 8 holds candidate i of the ring around the current centre
@@ -1349,12 +1357,12 @@ __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, cons
     };
     auto eval_tab = [&](const Tab &T, int i, int rx, int ry) -> int {
         const int lrc = __builtin_amdgcn_readlane(T.lr, i);
-        if (lrc < 0) return __builtin_amdgcn_readfirstlane(mx_cost_far(B, K, c2, s_lr, lane, rx, ry));
+        if (lrc < 0) return __builtin_amdgcn_readfirstlane(mx_cost_far<T8>(B, K, c2, s_lr, lane, rx, ry));
         const uint32_t ta = __builtin_amdgcn_readlane(T.ta, i), tb = __builtin_amdgcn_readlane(T.tb, i);
         const uint32_t sh = __builtin_amdgcn_readlane(T.sh, i);
         const uint32_t v = __builtin_amdgcn_lerp(mx_lds4(lwin + ta, sh & 3), mx_lds4(lwin + tb, (sh >> 8) & 3),
                                                  0x01010101u);
-        return mx_satd(v, K, c2) + lrc;
+        return mx_satd<T8>(v, K, c2) + lrc;
     };
 
     int cx = fx0, cy = fy0; // current best, as an offset from S
@@ -1481,6 +1489,7 @@ __device__ __forceinline__ void wn_partials(float (&t)[25], const uint32_t (&o)[
     ((t[C] = partial(wn_operand<C>(o))), ...);
 }
 
+template <bool T8>
 __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, const uint8_t *__restrict__ cur,
                                                      size_t cur_stride, const uint8_t *__restrict__ planes,
                                                      size_t planes_stride, const uint64_t *__restrict__ keys,
@@ -1537,7 +1546,7 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
     B.iy = ly + (B.sy >> 2) + p.pad;
     B.xoff = 0;
 
-    const MxConst K = mx_const(lane);
+    const MxConst K = mx_const<T8>(lane);
     const mx_f4 c2 = -mx_transform(src, K, mx_f4{0.0f, 0.0f, 0.0f, 0.0f});
     int best, best_pos; // wave-uniform
     if ((fx0 | fy0) == 0) {
@@ -1575,7 +1584,7 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
                         return (fabsf(y[0]) + fabsf(y[1])) + (fabsf(y[2]) + fabsf(y[3]));
                     },
                     std::make_integer_sequence<int, 25>{});
-        const int satd = (int)wn_sum25(t, lane);
+        const int satd = T8 ? (int)wn_sum25(t, lane) >> 1 : (int)wn_sum25(t, lane);
         const int pos = wn_lane_pos(lane);
         const int dx = pos % 5 - 2, dy = pos / 5 - 2;
         const bool valid = pos < 25;
@@ -1595,7 +1604,7 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
     } else {
         auto cost_at = [&](int c) -> int {
             const int dx = c % 5 - 2, dy = c / 5 - 2;
-            const int s = mx_satd(mx_operand_far(B, lane, fx0 + dx, fy0 + dy), K, c2);
+            const int s = mx_satd<T8>(mx_operand_far(B, lane, fx0 + dx, fy0 + dy), K, c2);
             return __builtin_amdgcn_readfirstlane(s + s_lr[mvbits(bx + dx - B.mpx) + mvbits(by + dy - B.mpy)]);
         };
         best = cost_at(12);
@@ -1906,6 +1915,32 @@ __device__ int satd4x4_sum_abs(const int d[16])
     return s;
 }
 
+// sum |H8 d H8^T| of an 8x8 difference, H8 the order-8 Sylvester Hadamard: three
+// butterfly stages on the rows, three on the columns (even: every coefficient has
+// the parity of sum d)
+__device__ int satd8x8_sum_abs(int d[64])
+{
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+        const int st = pass ? 8 : 1, ln = pass ? 1 : 8; // element stride, line stride
+#pragma unroll
+        for (int l = 0; l < 8; l++)
+#pragma unroll
+            for (int half = 4; half >= 1; half >>= 1)
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    if (!(i & half)) {
+                        const int a = d[l * ln + i * st], b = d[l * ln + (i + half) * st];
+                        d[l * ln + i * st] = a + b;
+                        d[l * ln + (i + half) * st] = a - b;
+                    }
+    }
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < 64; k++) s += abs(d[k]);
+    return s;
+}
+
 __global__ void k_block_metric(int kind, int w, int h, int n, const uint8_t *__restrict__ src, int ss,
                                const int64_t *__restrict__ src_off, const uint8_t *__restrict__ ref,
                                int rs, const int64_t *__restrict__ ref_off,
@@ -1920,6 +1955,15 @@ __global__ void k_block_metric(int kind, int w, int h, int n, const uint8_t *__r
     if (kind == RH264_METRIC_SAD) {
         for (int y = 0; y < h; y++)
             for (int x = 0; x < w; x++) total += abs((int)a[y * ss + x] - (int)b[(size_t)y * rs + x]);
+    } else if (kind == RH264_METRIC_SATD8) {
+        for (int by = 0; by < h; by += 8)
+            for (int bx = 0; bx < w; bx += 8) {
+                int d[64];
+                for (int y = 0; y < 8; y++)
+                    for (int x = 0; x < 8; x++)
+                        d[y * 8 + x] = (int)a[(by + y) * ss + bx + x] - (int)b[(size_t)(by + y) * rs + bx + x];
+                total += satd8x8_sum_abs(d) >> 1;
+            }
     } else {
         for (int by = 0; by < h; by += 4)
             for (int bx = 0; bx < w; bx += 4) {
@@ -2711,6 +2755,8 @@ static int check_params(const rh264_search_params *p)
     if (!geom_ok(p->cw, p->ch, p->pad) || !shape_ok(p->bw, p->bh)) return RH264_DECLINED;
     if (p->range < 1 || p->range > 64) return RH264_DECLINED;
     if (p->subpel_pat < -1 || p->subpel_pat > 4) return RH264_DECLINED;
+    if (p->hadamard != 0 && p->hadamard != 4 && p->hadamard != 8) return fail("hadamard %d: 4 or 8", p->hadamard);
+    if (p->hadamard == 8 && (p->bw != 16 || p->bh != 16)) return RH264_DECLINED;
     for (int i = 0; i < RH264_LR_N; i++)
         if (p->lr[i] < 0 || p->lr_fp[i] < 0 || p->lr[i] > (1 << 28) || p->lr_fp[i] > (1 << 28))
             return fail("lambda table entry %d out of range", i);
@@ -2756,14 +2802,30 @@ static int launch_refine(rh264_ctx *ctx, const rh264_search_params *p, int nfram
     constexpr int GPW = RefShape<BW, BH>::GPW;
     const int nblk = (p->cw / BW) * (p->ch / BH);
     if constexpr (BW == 16 && BH == 16) {
-        if (ctx->refine_mfma) {
+        // the 8x8 transform exists only in the matrix-core kernels
+        const bool t8 = p->hadamard == 8;
+        if (ctx->refine_mfma || t8) {
             const dim3 grid((unsigned)((nblk + 3) / 4), nframes);
-#define MX_LAUNCH(PT)                                                                                        \
-    hipLaunchKernelGGL(k_refine_mx16<PT>, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes, planes_stride, \
-                       keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost, evals)
+#define MX_LAUNCH(PT)                                                                                                \
+    do {                                                                                                             \
+        if (t8)                                                                                                      \
+            hipLaunchKernelGGL((k_refine_mx16<PT, true>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,      \
+                               planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost,     \
+                               evals);                                                                               \
+        else                                                                                                         \
+            hipLaunchKernelGGL((k_refine_mx16<PT, false>), grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,     \
+                               planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost,     \
+                               evals);                                                                               \
+    } while (0)
             if (p->subpel_pat == 4) {
-                hipLaunchKernelGGL(k_refine_wn16, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride, planes,
-                                   planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv, cost, evals);
+                if (t8)
+                    hipLaunchKernelGGL(k_refine_wn16<true>, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
+                                       planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv,
+                                       cost, evals);
+                else
+                    hipLaunchKernelGGL(k_refine_wn16<false>, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
+                                       planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv,
+                                       cost, evals);
                 LAUNCH_CHECK("k_refine_wn16");
                 return 0;
             }
@@ -2977,8 +3039,9 @@ int rh264_block_metric(rh264_ctx *ctx, int kind, int w, int h, int n, const uint
                        const int64_t *src_off, const uint8_t *ref, int rs, const int64_t *ref_off,
                        const int64_t *ref_off2, int32_t *out)
 {
-    if (kind < 0 || kind > 3) return fail("block_metric: kind %d", kind);
+    if (kind < 0 || kind > 4) return fail("block_metric: kind %d", kind);
     if (!(shape_ok(w, h) || (w == 4 && h == 4))) return RH264_DECLINED;
+    if (kind == RH264_METRIC_SATD8 && (w % 8 || h % 8)) return RH264_DECLINED;
     if (kind == RH264_METRIC_SATD_AVG && !ref_off2) return fail("block_metric: SATD_AVG needs ref_off2");
     if (n <= 0) return 0;
     hipLaunchKernelGGL(k_block_metric, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, kind, w, h, n,

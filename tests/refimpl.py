@@ -14,6 +14,23 @@ TAPS = np.array([1, -5, 20, 20, -5, 1], dtype=np.float64)
 H4 = np.array([[1, 1, 1, 1], [1, 1, -1, -1], [1, -1, -1, 1], [1, -1, 1, -1]], dtype=np.int64)
 
 
+# order-8 Sylvester Hadamard, H2 (x) H2 (x) H2 (the build-defined 8x8 SATD)
+H8 = np.kron(np.kron([[1, 1], [1, -1]], [[1, 1], [1, -1]]), [[1, 1], [1, -1]]).astype(np.int64)
+
+
+def satd8_sum(a, b):
+    """Sum over 8x8 blocks of sum |H8 d H8^T| / 2 (matrix form)."""
+    d = a.astype(np.int64) - b.astype(np.int64)
+    h, w = d.shape
+    s = 0
+    for y in range(0, h, 8):
+        for x in range(0, w, 8):
+            t = int(np.abs(H8 @ d[y:y + 8, x:x + 8] @ H8.T).sum())
+            assert t % 2 == 0
+            s += t // 2
+    return s
+
+
 def sad(a, b):
     return int(np.abs(a.astype(np.int64) - b.astype(np.int64)).sum())
 

@@ -24,9 +24,11 @@ def dev():
 
 # ------------------------------------------------------------ block metrics
 
-@pytest.mark.parametrize("kind", [accel.METRIC_SAD, accel.METRIC_SATD_WELS, accel.METRIC_SATD])
+@pytest.mark.parametrize("kind", [accel.METRIC_SAD, accel.METRIC_SATD_WELS, accel.METRIC_SATD, accel.METRIC_SATD8])
 def test_block_metrics_corpus_and_lcg(dev, kind):
-    shapes = accel.SHAPES
+    shapes = accel.SHAPES if kind != accel.METRIC_SATD8 else [s for s in accel.SHAPES if s[0] % 8 == 0 and s[1] % 8 == 0]
+    fn = {accel.METRIC_SAD: O.sad, accel.METRIC_SATD_WELS: O.satd_wels, accel.METRIC_SATD: O.satd_px,
+          accel.METRIC_SATD8: O.satd8_px}[kind]
     # closed-form corpus (satd_sad.rs:390-399), 96 seeds, 16-stride blocks
     src, ref = [], []
     for seed in range(96):
@@ -39,7 +41,7 @@ def test_block_metrics_corpus_and_lcg(dev, kind):
         got = accel.block_metric(dev, kind, w, h, src, 16, offs, ref, 16, offs)
         for i in range(96):
             a, b = src[offs[i]:offs[i] + 256], ref[offs[i]:offs[i] + 256]
-            want = [O.sad, O.satd_wels, O.satd_px][kind](a, 16, b, 16, w, h)
+            want = fn(a, 16, b, 16, w, h)
             assert got[i] == want, (w, h, i)
     buf = synth.lcg_u8(0x777, 24 * 20 * 64)
     so = np.arange(32, dtype=np.int64) * 24 * 20
@@ -47,7 +49,7 @@ def test_block_metrics_corpus_and_lcg(dev, kind):
     for w, h in shapes:
         got = accel.block_metric(dev, kind, w, h, buf, 24, so, buf, 24, ro)
         for i in range(32):
-            want = [O.sad, O.satd_wels, O.satd_px][kind](buf[so[i]:], 24, buf[ro[i]:], 24, w, h)
+            want = fn(buf[so[i]:], 24, buf[ro[i]:], 24, w, h)
             assert got[i] == want
 
 
@@ -130,18 +132,18 @@ def test_predict_blocks_any_mv(dev):
 
 # ------------------------------------------------------------ search
 
-def _search_case(dev, frames, cw, ch, pad, bw, bh, R, pat, lam, centres=None, mvps=None, nthreads=NT):
+def _search_case(dev, frames, cw, ch, pad, bw, bh, R, pat, lam, centres=None, mvps=None, nthreads=NT, hadamard=4):
     nf = frames.shape[0] - 1
     cur = dev.upload(frames[1:])
     refs = dev.upload(frames[:-1])
     pl = accel.build_planes(dev, refs, nf, cw, ch, pad)
-    p = accel.search_params(cw, ch, pad, bw, bh, R, pat, lam)
+    p = accel.search_params(cw, ch, pad, bw, bh, R, pat, lam, hadamard=hadamard)
     ms = accel.MotionSearch(dev, p)
     dc = dev.upload(centres) if centres is not None else None
     dm = dev.upload(mvps) if mvps is not None else None
     ms.run_plain(cur, pl, nf, dc, dm)
     got = ms.results(nf)
-    cfg = O.cfg(cw, ch, pad, bw, bh, R, pat, lam)
+    cfg = O.cfg(cw, ch, pad, bw, bh, R, pat, lam, hadamard=hadamard)
     nb = ms.nblk
     for f in range(nf):
         opl = O.planes(frames[f], cw, ch, pad)
@@ -164,12 +166,14 @@ def test_plain_search_shapes_small(dev, bw, bh):
     _search_case(dev, frames, 96, 64, 32, bw, bh, 7, 0, LAM26)
 
 
-@pytest.mark.parametrize("R,pat,lam", [(16, 0, LAM26), (16, 2, LAM26), (5, 1, 0.0), (13, 3, 9.0), (16, -1, LAM26),
-                                       (16, 4, LAM26)])
-def test_plain_search_cif(dev, R, pat, lam):
-    """Config C1 geometry: CIF 352x288, 16x16, +-16 full search, qpel SATD refine."""
-    frames = synth.cif_clip(4, seed=R + pat)
-    _search_case(dev, frames, 352, 288, 32, 16, 16, R, pat, lam)
+@pytest.mark.parametrize("R,pat,lam,had", [(16, 0, LAM26, 4), (16, 2, LAM26, 4), (5, 1, 0.0, 4), (13, 3, 9.0, 4),
+                                           (16, -1, LAM26, 4), (16, 4, LAM26, 4), (16, 0, LAM26, 8), (16, 4, LAM26, 8),
+                                           (9, 3, 0.0, 8)])
+def test_plain_search_cif(dev, R, pat, lam, had):
+    """Config C1 geometry: CIF 352x288, 16x16, +-16 full search, qpel SATD refine
+    (the 4x4 transform, or the 8x8 one)."""
+    frames = synth.cif_clip(4, seed=R + pat + had)
+    _search_case(dev, frames, 352, 288, 32, 16, 16, R, pat, lam, hadamard=had)
 
 
 def test_plain_search_centres_and_predictors_outside_pad(dev):
@@ -214,8 +218,8 @@ def _smooth_pair(cw, ch, dx, dy):
     return np.stack([ref, cur])
 
 
-@pytest.mark.parametrize("pat,refine_mfma", [(0, 1), (1, 1), (0, 0)])
-def test_subpel_refine_long_walks(dev, pat, refine_mfma):
+@pytest.mark.parametrize("pat,refine_mfma,had", [(0, 1, 4), (1, 1, 4), (0, 0, 4), (0, 1, 8)])
+def test_subpel_refine_long_walks(dev, pat, refine_mfma, had):
     """Ring walks that travel far from their start (smooth content, lambda 0, start at
     zero, true motion (+7, +3) pels): candidates beyond the window staged around the
     start read the planes in HBM.  Bit-exact with the oracle, evaluation counts included."""
@@ -225,7 +229,7 @@ def test_subpel_refine_long_walks(dev, pat, refine_mfma):
     try:
         cur, ref = dev.upload(frames[1]), dev.upload(frames[0])
         pl = accel.build_planes(dev, ref, 1, cw, ch, pad)
-        p = accel.search_params(cw, ch, pad, 16, 16, 8, pat, 0.0)
+        p = accel.search_params(cw, ch, pad, 16, 16, 8, pat, 0.0, hadamard=had)
         ms = accel.MotionSearch(dev, p)
         starts = np.zeros((ms.nblk, 2), np.int32)
         ds = dev.upload(starts)
@@ -233,7 +237,7 @@ def test_subpel_refine_long_walks(dev, pat, refine_mfma):
         got = ms.results(1, with_evals=True)
     finally:
         dev.tune("refine_mfma", 1)
-    cfg = O.cfg(cw, ch, pad, 16, 16, 8, pat, 0.0)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 8, pat, 0.0, hadamard=had)
     opl = O.planes(frames[0], cw, ch, pad)
     far = 0
     for b in range(ms.nblk):
@@ -244,20 +248,22 @@ def test_subpel_refine_long_walks(dev, pat, refine_mfma):
     assert far > ms.nblk // 2  # most walks leave the +-4 pel window
 
 
-@pytest.mark.parametrize("refine_mfma", [1, 0])
+@pytest.mark.parametrize("refine_mfma,had", [(1, 4), (0, 4), (1, 8)])
 @pytest.mark.parametrize("bw,bh", [(16, 16), (16, 8), (8, 8), (4, 4)])
-def test_qpel_window_refine(dev, bw, bh, refine_mfma):
+def test_qpel_window_refine(dev, bw, bh, refine_mfma, had):
     """subpel_pat 4 (the +-2 quarter-pel window) from given starts: integer-pel
     starts (the window kernel's compile-time path), fractional starts, and starts
     far outside the padded planes (clamped reads), with rate predictors.  mv, cost
     and evaluation count bit-exact with the oracle."""
     cw, ch, pad = 176, 144, 32
+    if had == 8 and (bw, bh) != (16, 16):
+        pytest.skip("the 8x8 transform refines 16x16 blocks only (other shapes decline)")
     frames = synth.cif_clip(2, seed=bw + bh + refine_mfma, w=cw, h=ch)
     dev.tune("refine_mfma", refine_mfma)
     try:
         cur, ref = dev.upload(frames[1]), dev.upload(frames[0])
         pl = accel.build_planes(dev, ref, 1, cw, ch, pad)
-        p = accel.search_params(cw, ch, pad, bw, bh, 8, 4, LAM26)
+        p = accel.search_params(cw, ch, pad, bw, bh, 8, 4, LAM26, hadamard=had)
         ms = accel.MotionSearch(dev, p)
         rng = np.random.default_rng(bw * bh)
         nb = ms.nblk
@@ -272,7 +278,7 @@ def test_qpel_window_refine(dev, bw, bh, refine_mfma):
         got = ms.results(1, with_evals=True)
     finally:
         dev.tune("refine_mfma", 1)
-    cfg = O.cfg(cw, ch, pad, bw, bh, 8, 4, LAM26)
+    cfg = O.cfg(cw, ch, pad, bw, bh, 8, 4, LAM26, hadamard=had)
     opl = O.planes(frames[0], cw, ch, pad)
     nbx = cw // bw
     for b in range(nb):
@@ -280,6 +286,25 @@ def test_qpel_window_refine(dev, bw, bh, refine_mfma):
         mv, c, ev = O.subpel_refine_block(cfg, frames[1], opl, bx, by, *starts[b], *mvps[b])
         assert tuple(got["mv"][b]) == mv and got["cost"][b] == c and got["evals"][b] == ev == 25, (b, starts[b])
     for x in (cur, ref, pl, ds, dm):
+        x.free()
+
+
+def test_hadamard8_declines_other_shapes(dev):
+    """The 8x8 transform refines 16x16 blocks only: other shapes decline (status 1,
+    the caller's CPU path), and a transform other than 4 or 8 is an error."""
+    cw, ch, pad = 64, 48, 16
+    f = synth.cif_clip(2, seed=1, w=cw, h=ch)
+    cur, ref = dev.upload(f[1]), dev.upload(f[0])
+    pl = accel.build_planes(dev, ref, 1, cw, ch, pad)
+    p = accel.search_params(cw, ch, pad, 8, 8, 4, 0, LAM26, hadamard=8)
+    ms = accel.MotionSearch(dev, p)
+    with pytest.raises(accel.Declined):
+        ms.run_plain(cur, pl, 1)
+    p = accel.search_params(cw, ch, pad, 16, 16, 4, 0, LAM26, hadamard=6)
+    ms = accel.MotionSearch(dev, p)
+    with pytest.raises(RuntimeError):
+        ms.run_plain(cur, pl, 1)
+    for x in (cur, ref, pl):
         x.free()
 
 

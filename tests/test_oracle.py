@@ -262,6 +262,56 @@ def test_subpel_refine_trace(pat):
             assert c <= cost(*start)
 
 
+def test_satd8_matrix_form_and_closed_forms():
+    """8x8 Hadamard SATD (build-defined): == the Kronecker-product matrix form on
+    random and extreme blocks of every 8-multiple shape; a constant difference c
+    puts 64c in the DC alone (32|c| per block after the halving); a difference equal
+    to a*row_i(H8)^T row_j(H8) puts 64a in one coefficient (32|a|)."""
+    rng = np.random.default_rng(61)
+    for w, h in ((8, 8), (16, 16), (16, 8), (8, 16)):
+        for _ in range(40):
+            a = rng.integers(0, 256, (h, w), dtype=np.uint8)
+            b = rng.integers(0, 256, (h, w), dtype=np.uint8)
+            assert O.satd8_px(a, w, b, w, w, h) == refimpl.satd8_sum(a, b)
+        ext = np.where(rng.integers(0, 2, (h, w)) == 1, 255, 0).astype(np.uint8)
+        assert O.satd8_px(ext, w, 255 - ext, w, w, h) == refimpl.satd8_sum(ext, 255 - ext)
+    base = np.full((8, 8), 100, np.uint8)
+    for c in (-7, 1, 30):
+        assert O.satd8_px((base.astype(int) + c).astype(np.uint8), 8, base, 8, 8, 8) == 32 * abs(c)
+    for i, j, amp in ((3, 5, 2), (7, 0, -3), (0, 0, 4)):
+        pat = amp * np.outer(refimpl.H8[i], refimpl.H8[j])
+        a = (100 + pat).astype(np.uint8)
+        assert O.satd8_px(a, 8, base, 8, 8, 8) == 32 * abs(amp)
+
+
+@pytest.mark.parametrize("pat", [0, 4])
+def test_subpel_refine_hadamard8_trace(pat):
+    """With the 8x8 transform, the ring walk and the quarter-pel window follow the
+    textbook cost (spec samples + Kronecker H8 + Exp-Golomb rate)."""
+    cw, ch, pad = 48, 32, 16
+    cur, ref = _pair(cw, ch, 63 + pat, shift=(1, 2))
+    pl = O.planes(ref, cw, ch, pad)
+    lam = 4.65
+    cfg = O.cfg(cw, ch, pad, 16, 16, 4, pat, lam, hadamard=8)
+    lr = O.lambda_table(lam)
+    spec = refimpl.spec_samples(ref, pad)
+    for by in range(0, ch, 16):
+        for bx in range(0, cw, 16):
+            mvp = (3, -5)
+
+            def cost(mx, my, bx=bx, by=by):
+                x = bx + (mx >> 2) + pad
+                y = by + (my >> 2) + pad
+                pred = spec[(mx & 3, my & 3)][y:y + 16, x:x + 16]
+                return refimpl.satd8_sum(cur[by:by + 16, bx:bx + 16], pred) + \
+                    lr[refimpl.mvbits(mx - mvp[0]) + refimpl.mvbits(my - mvp[1])]
+
+            for start in ((-8, -4), (-7, 2)):
+                want = _subpel_python(cost, start, pat) if pat != 4 else _window_python(cost, start)
+                mv, c, ev = O.subpel_refine_block(cfg, cur, pl, bx, by, *start, *mvp)
+                assert (mv, c) == want
+
+
 def _window_python(cost, start):
     """The +-2 quarter-pel window (subpel_pat 4): start first, then raster order."""
     (sx, sy), (bx, by) = start, start
