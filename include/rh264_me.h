@@ -300,6 +300,27 @@ int rh264_mbtree_costs(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_
 int rh264_mbtree_propagate_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, const int32_t *intra,
                                const int32_t *inter, const int32_t *mv, int32_t *prop);
 
+/* Multi-reference mb-tree costs (build-defined, no reference counterpart: the 4K
+ * lookahead configuration's "3 references, +-64" form; the reference's inter_cost,
+ * mbtree.rs:154-197, walks a seeded diamond against one reference).  half as for
+ * rh264_mbtree_costs: nframes frames of mb_w*8 x mb_h*8.  8x8 blocks, one per MB:
+ *   intra = max(1, (sum of the |AC coefficients| of H8 x H8) >> 1);
+ *   frame 0: inter = intra, mv = (0,0), ref = 0;
+ *   frame f > 0, for each reference k = 1..min(nrefs, f) (frame f-k): the
+ *     exhaustive full-pel SAD search over +-range (reads clamped into the frame,
+ *     raster first minimum, dy outer), then the 8x8 SATD sum|H8 d H8|/2 at its
+ *     winner; the cheapest k wins (ties to the smaller): inter = min(its SATD,
+ *     intra), mv = 8 * winner (full-resolution quarter pel), ref = k.
+ * nrefs 1..4, range 1..64; other values decline.  Scratch: padded copies of the
+ * frames, held by the context. */
+int rh264_mbtree_costs_multi(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_w, int mb_h, int nrefs,
+                             int range, int32_t *intra, int32_t *inter, int32_t *mv, int32_t *ref);
+/* rh264_mbtree_propagate_i32 with frame f's amount going to frame f - ref[f][mb]
+ * (the ref output of rh264_mbtree_costs_multi); ref all 1 is the single-reference
+ * propagation. */
+int rh264_mbtree_propagate_multi_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, const int32_t *intra,
+                                     const int32_t *inter, const int32_t *mv, const int32_t *ref, int32_t *prop);
+
 /* Rate-control frame complexity, lookahead::complexity (lookahead.rs:38-101), of
  * nframes coded-size cw x ch luma frames (frame i at cur + i*frame_stride; cw, ch
  * multiples of 16): out[i] = max(total, 1) as a double, where total sums over

@@ -822,6 +822,127 @@ double orc_rc_complexity(const uint8_t *sy, const uint8_t *ref, int cw, int ch)
     return total < 1 ? 1.0 : (double)total;
 }
 
+/* Build-defined multi-reference lookahead (the 4K configuration's 3 refs, +-64 form;
+ * no reference counterpart, whose inter_cost, mbtree.rs:154-197, walks a seeded
+ * diamond against one reference).  One 8x8 block of frame f at (bx0, by0):
+ * for k = 1..min(nrefs, f): exhaustive full-pel SAD over +-range against frame f-k
+ * (reads clamped into the frame; raster first minimum, dy outer), then the 8x8 SATD
+ * at the winner; the cheapest reference wins, ties to the smaller k. */
+static int32_t la_intra8(const uint8_t *y, int cw, int bx0, int by0)
+{
+    int32_t d[64], co[64];
+    for (int j = 0; j < 8; j++)
+        for (int i = 0; i < 8; i++) d[j * 8 + i] = y[(size_t)(by0 + j) * cw + bx0 + i];
+    orc_hadamard_8x8(d, co);
+    int64_t s = 0;
+    for (int k = 1; k < 64; k++) s += iabs(co[k]);
+    s >>= 1;
+    return s < 1 ? 1 : (int32_t)s;
+}
+
+int64_t orc_la_block_multi(const uint8_t *half, int cw, int ch, int f, int nrefs, int range, int bx0, int by0,
+                           int32_t *mv_out, int32_t *ref_out)
+{
+    const size_t fs = (size_t)cw * ch;
+    const uint8_t *cur = half + (size_t)f * fs;
+    int64_t best = INT64_MAX;
+    int bmx = 0, bmy = 0, bk = 0;
+    for (int k = 1; k <= nrefs && k <= f; k++) {
+        const uint8_t *ref = half + (size_t)(f - k) * fs;
+        int64_t bs = INT64_MAX;
+        int wx = 0, wy = 0;
+        for (int dy = -range; dy <= range; dy++)
+            for (int dx = -range; dx <= range; dx++) {
+                int64_t sad = 0;
+                for (int j = 0; j < 8; j++)
+                    for (int i = 0; i < 8; i++)
+                        sad += iabs((int)cur[(size_t)(by0 + j) * cw + bx0 + i] -
+                                    at(ref, cw, ch, bx0 + i + dx, by0 + j + dy));
+                if (sad < bs) { bs = sad; wx = dx; wy = dy; }
+            }
+        uint8_t pred[64];
+        for (int j = 0; j < 8; j++)
+            for (int i = 0; i < 8; i++) pred[j * 8 + i] = (uint8_t)at(ref, cw, ch, bx0 + i + wx, by0 + j + wy);
+        int64_t c = orc_satd8_px(cur + (size_t)by0 * cw + bx0, cw, pred, 8, 8, 8);
+        if (c < best) { best = c; bmx = wx; bmy = wy; bk = k; }
+    }
+    mv_out[0] = 8 * bmx;
+    mv_out[1] = 8 * bmy;
+    *ref_out = bk;
+    return best;
+}
+
+void orc_mbtree_costs_multi(const uint8_t *half, int nframes, int mb_w, int mb_h, int nrefs, int range,
+                            int32_t *intra, int32_t *inter, int32_t *mv, int32_t *ref, int nthreads)
+{
+    const int cw = mb_w * 8, ch = mb_h * 8, nm = mb_w * mb_h;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) collapse(2)
+#else
+    (void)nthreads;
+#endif
+    for (int f = 0; f < nframes; f++)
+        for (int m = 0; m < nm; m++) {
+            const size_t g = (size_t)f * nm + m;
+            const int bx0 = (m % mb_w) * 8, by0 = (m / mb_w) * 8;
+            const int32_t ic = la_intra8(half + (size_t)f * cw * ch, cw, bx0, by0);
+            intra[g] = ic;
+            if (f == 0) {
+                inter[g] = ic;
+                mv[2 * g] = mv[2 * g + 1] = 0;
+                ref[g] = 0;
+                continue;
+            }
+            int32_t m2[2], k;
+            int64_t c = orc_la_block_multi(half, cw, ch, f, nrefs, range, bx0, by0, m2, &k);
+            inter[g] = c < ic ? (int32_t)c : ic;
+            mv[2 * g] = m2[0];
+            mv[2 * g + 1] = m2[1];
+            ref[g] = k;
+        }
+}
+
+/* Multi-reference int32 propagation (build-defined): as orc_propagate_i32, with
+ * frame f's amount going to frame f - ref[f][mb]; a frame's accumulator is final
+ * once every later frame has been processed. */
+void orc_propagate_multi_i32(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                             const int32_t *mv, const int32_t *ref, int32_t *prop)
+{
+    int nm = mb_w * mb_h;
+    int64_t *acc = (int64_t *)calloc((size_t)n * nm, sizeof(int64_t));
+    for (int f = n - 1; f >= 0; f--) {
+        for (int m = 0; m < nm; m++) {
+            int64_t a = acc[(size_t)f * nm + m];
+            prop[(size_t)f * nm + m] = a > INT32_MAX ? INT32_MAX : (int32_t)a;
+        }
+        if (f == 0) break;
+        for (int y = 0; y < mb_h; y++)
+            for (int x = 0; x < mb_w; x++) {
+                int m = y * mb_w + x;
+                size_t g = (size_t)f * nm + m;
+                int64_t total = (int64_t)intra[g] + prop[g];
+                int64_t amount = total * (int64_t)(intra[g] - inter[g]) / intra[g];
+                if (amount <= 0) continue;
+                int t = f - (ref ? ref[g] : 1);
+                int rx = clampi(x * 16 + asr2(mv[2 * g]), 0, (mb_w - 1) * 16);
+                int ry = clampi(y * 16 + asr2(mv[2 * g + 1]), 0, (mb_h - 1) * 16);
+                int cx0 = rx / 16, cy0 = ry / 16, fx = rx % 16, fy = ry % 16;
+                const int wys[2] = {16 - fy, fy}, wxs[2] = {16 - fx, fx};
+                for (int dy = 0; dy < 2; dy++) {
+                    if (wys[dy] <= 0) continue;
+                    int cy = cy0 + dy < mb_h - 1 ? cy0 + dy : mb_h - 1;
+                    for (int dx = 0; dx < 2; dx++) {
+                        if (wxs[dx] <= 0) continue;
+                        int cx = cx0 + dx < mb_w - 1 ? cx0 + dx : mb_w - 1;
+                        acc[(size_t)t * nm + cy * mb_w + cx] += (amount * wxs[dx] * wys[dy]) >> 8;
+                    }
+                }
+            }
+    }
+    free(acc);
+}
+
 /* Rust i32 `/` truncates toward zero (C99 does too). */
 void orc_frame_costs_half(const uint8_t *half, const uint8_t *ref_half, int mb_w, int mb_h,
                           int32_t *intra, int32_t *inter, int32_t *mv)

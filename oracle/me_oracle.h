@@ -146,6 +146,21 @@ int64_t orc_satd8_px(const uint8_t *src, int ss, const uint8_t *pred, int ps, in
  * frame, IDR when ref is NULL; max(total, 1). */
 double orc_rc_complexity(const uint8_t *sy, const uint8_t *ref, int cw, int ch);
 
+/* Build-defined multi-reference lookahead (the 4K configuration's 3 refs +-64):
+ * half-res frames mb_w*8 x mb_h*8, 8x8 blocks.  intra = max(1, AC sum|H8 x H8| >> 1);
+ * frame 0: inter = intra, mv 0, ref 0; frame f > 0: per reference k = 1..min(nrefs,
+ * f) the exhaustive full-pel SAD search over +-range (clamped reads, raster first
+ * minimum), the 8x8 SATD at its winner, the cheapest k (ties to the smaller);
+ * inter = min(that, intra), mv = 8 * winner (full-res quarter pel), ref = k. */
+int64_t orc_la_block_multi(const uint8_t *half, int cw, int ch, int f, int nrefs, int range, int bx0, int by0,
+                           int32_t *mv_out, int32_t *ref_out);
+void orc_mbtree_costs_multi(const uint8_t *half, int nframes, int mb_w, int mb_h, int nrefs, int range,
+                            int32_t *intra, int32_t *inter, int32_t *mv, int32_t *ref, int nthreads);
+/* int32 propagation with frame f's amount going to frame f - ref[f][mb]
+ * (ref NULL: every block references f - 1, which is orc_propagate_i32). */
+void orc_propagate_multi_i32(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                             const int32_t *mv, const int32_t *ref, int32_t *prop);
+
 #ifdef __cplusplus
 }
 #endif

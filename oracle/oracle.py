@@ -65,7 +65,7 @@ class Oracle:
         for n in ("orc_sad", "orc_satd4x4_wels", "orc_satd_wels", "orc_intra_cost"):
             getattr(L, n).restype = C.c_int32
         for n in ("orc_satd_px", "orc_full_search_block", "orc_satd_cost", "orc_subpel_refine_block",
-                  "orc_motion_search_ref", "orc_mbtree_mc_satd", "orc_satd8_px"):
+                  "orc_motion_search_ref", "orc_mbtree_mc_satd", "orc_satd8_px", "orc_la_block_multi"):
             getattr(L, n).restype = C.c_int64
         L.orc_satd_avg.restype = C.c_uint32
         L.orc_mvbits.restype = C.c_uint32
@@ -205,6 +205,34 @@ class Oracle:
         self.lib.orc_frame_costs_half(_p(_u8(half)), rp, mb_w, mb_h, _p(intra, i32p), _p(inter, i32p),
                                       _p(mv, i32p))
         return intra, inter, mv
+
+    def mbtree_costs_multi(self, halves, mb_w, mb_h, nrefs=3, search_range=64, nthreads=1):
+        """Build-defined multi-reference costs of a stack of half-res frames."""
+        n = halves.shape[0]
+        nm = mb_w * mb_h
+        intra = np.empty((n, nm), np.int32)
+        inter = np.empty((n, nm), np.int32)
+        mv = np.empty((n, nm, 2), np.int32)
+        ref = np.empty((n, nm), np.int32)
+        self.lib.orc_mbtree_costs_multi(_p(_u8(halves)), n, mb_w, mb_h, nrefs, search_range, _p(intra, i32p),
+                                        _p(inter, i32p), _p(mv, i32p), _p(ref, i32p), nthreads)
+        return intra, inter, mv, ref
+
+    def la_block_multi(self, halves, cw, ch, f, nrefs, search_range, bx0, by0):
+        mv = np.empty(2, np.int32)
+        k = C.c_int32()
+        c = self.lib.orc_la_block_multi(_p(_u8(halves)), cw, ch, f, nrefs, search_range, bx0, by0, _p(mv, i32p),
+                                        C.byref(k))
+        return int(c), (int(mv[0]), int(mv[1])), k.value
+
+    def propagate_multi_i32(self, intra, inter, mv, ref, mb_w, mb_h):
+        n = intra.shape[0]
+        out = np.empty((n, mb_w * mb_h), np.int32)
+        rp = _p(np.ascontiguousarray(ref, np.int32), i32p) if ref is not None else None
+        self.lib.orc_propagate_multi_i32(n, mb_w, mb_h, _p(np.ascontiguousarray(intra, np.int32), i32p),
+                                         _p(np.ascontiguousarray(inter, np.int32), i32p),
+                                         _p(np.ascontiguousarray(mv, np.int32), i32p), rp, _p(out, i32p))
+        return out
 
     def propagate_f64(self, intra, inter, mv, mb_w, mb_h):
         n = intra.shape[0]

@@ -329,13 +329,34 @@ class MbTree:
                                                     self.inter.ptr, self.mv.ptr, self.prop.ptr),
                    "rh264_mbtree_propagate_i32")
 
+    def run_multi(self, frames_buf, nrefs=3, search_range=64, frame_stride=None, propagate=True):
+        """The build-defined multi-reference form (rh264_mbtree_costs_multi +
+        rh264_mbtree_propagate_multi_i32): exhaustive +-search_range per reference,
+        8x8 Hadamard costs, the cheapest of nrefs previous frames."""
+        d = self.dev
+        nm = self.mb_w * self.mb_h
+        if not hasattr(self, "ref"):
+            self.ref = d.alloc(4 * nm * self.n)
+        _check(d.lib.rh264_mbtree_downsample(d.ctx, int(frames_buf), frame_stride or self.cw * self.ch, self.n,
+                                             self.cw, self.ch, self.half.ptr), "rh264_mbtree_downsample")
+        _check(d.lib.rh264_mbtree_costs_multi(d.ctx, self.half.ptr, self.n, self.mb_w, self.mb_h, nrefs, search_range,
+                                              self.intra.ptr, self.inter.ptr, self.mv.ptr, self.ref.ptr),
+               "rh264_mbtree_costs_multi")
+        if propagate:
+            _check(d.lib.rh264_mbtree_propagate_multi_i32(d.ctx, self.n, self.mb_w, self.mb_h, self.intra.ptr,
+                                                          self.inter.ptr, self.mv.ptr, self.ref.ptr, self.prop.ptr),
+                   "rh264_mbtree_propagate_multi_i32")
+
     def results(self):
         d, n, nm = self.dev, self.n, self.mb_w * self.mb_h
-        return {"half": d.download(self.half, (n, self.ch // 2, self.cw // 2), np.uint8),
-                "intra": d.download(self.intra, (n, nm), np.int32),
-                "inter": d.download(self.inter, (n, nm), np.int32),
-                "mv": d.download(self.mv, (n, nm, 2), np.int32),
-                "prop": d.download(self.prop, (n, nm), np.int32)}
+        r = {"half": d.download(self.half, (n, self.ch // 2, self.cw // 2), np.uint8),
+             "intra": d.download(self.intra, (n, nm), np.int32),
+             "inter": d.download(self.inter, (n, nm), np.int32),
+             "mv": d.download(self.mv, (n, nm, 2), np.int32),
+             "prop": d.download(self.prop, (n, nm), np.int32)}
+        if hasattr(self, "ref"):
+            r["ref"] = d.download(self.ref, (n, nm), np.int32)
+        return r
 
 
 def rc_complexity(dev, cur, ref, nframes, cw, ch, frame_stride=None, out=None):

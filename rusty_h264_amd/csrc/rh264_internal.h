@@ -46,4 +46,6 @@ struct rh264_ctx {
     int prof_cap, prof_n;         // capacity and count of profiled searches
     void *ls_buf;                 // scratch of the learned-proposal search
     size_t ls_cap;
+    void *la_buf;                 // padded half-res planes of the multi-reference lookahead
+    size_t la_cap;
 };

@@ -1918,7 +1918,7 @@ __device__ int satd4x4_sum_abs(const int d[16])
 // sum |H8 d H8^T| of an 8x8 difference, H8 the order-8 Sylvester Hadamard: three
 // butterfly stages on the rows, three on the columns (even: every coefficient has
 // the parity of sum d)
-__device__ int satd8x8_sum_abs(int d[64])
+__device__ __forceinline__ int satd8x8_sum_abs(int d[64])
 {
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
@@ -2214,10 +2214,12 @@ __global__ __launch_bounds__(64) void k_mbtree_rows(const uint8_t *__restrict__ 
 // Build-defined int32 propagation step for frame f (mirrors propagate_to,
 // mbtree.rs:268-297, in integers): prop[f] = saturate(acc[f]); amount =
 // ((intra + prop) * (intra - inter)) / intra; shares (amount * wx * wy) >> 8 are
-// added into acc[f-1] with 64-bit integer atomics (order-independent).
+// added into acc[f-1] -- or acc[f - refs[f][mb]] for the multi-reference costs --
+// with 64-bit integer atomics (order-independent).
 __global__ void k_mbtree_prop_i32(int f, int mb_w, int mb_h, const int32_t *__restrict__ intra,
                                   const int32_t *__restrict__ inter, const int32_t *__restrict__ mv,
-                                  unsigned long long *__restrict__ acc, int32_t *__restrict__ prop)
+                                  const int32_t *__restrict__ refs, unsigned long long *__restrict__ acc,
+                                  int32_t *__restrict__ prop)
 {
     const int nm = mb_w * mb_h;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2235,7 +2237,7 @@ __global__ void k_mbtree_prop_i32(int f, int mb_w, int mb_h, const int32_t *__re
     const int ry = clampi(y * 16 + (mv[2 * g + 1] >> 2), 0, (mb_h - 1) * 16);
     const int cx0 = rx / 16, cy0 = ry / 16, fx = rx % 16, fy = ry % 16;
     const int wys[2] = {16 - fy, fy}, wxs[2] = {16 - fx, fx};
-    unsigned long long *prev = acc + (size_t)(f - 1) * nm;
+    unsigned long long *prev = acc + (size_t)(f - (refs ? refs[g] : 1)) * nm;
 #pragma unroll
     for (int dy = 0; dy < 2; dy++) {
         if (wys[dy] <= 0) continue;
@@ -2247,6 +2249,119 @@ __global__ void k_mbtree_prop_i32(int f, int mb_w, int mb_h, const int32_t *__re
             atomicAdd(&prev[cy * mb_w + cx], (unsigned long long)((amount * wxs[dx] * wys[dy]) >> 8));
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-reference lookahead (build-defined: the 4K configuration's 3 refs, +-64;
+// the reference's inter_cost walks a seeded diamond against one reference).
+// The half-res frames are copied into edge-replicated planes with a pad of
+// range + 8 (rounded up to 4), so every search window and every winner's 8x8
+// block lies inside a plane and the clamped reads of the definition are plain
+// reads: the integer search is the full search's lane ring (fs_key) at zero rate,
+// one workgroup row per (frame, reference); k_la_cost then prices each
+// reference's winner with the 8x8 Hadamard and keeps the cheapest.
+// ---------------------------------------------------------------------------
+
+__global__ void k_la_pad(const uint8_t *__restrict__ half, int hw, int hh, int P, uint8_t *__restrict__ planes,
+                         size_t pstride)
+{
+    const int pw = hw + 2 * P, ph = hh + 2 * P;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x; // dword of the padded plane
+    if (i >= (long)pw * ph / 4) return;
+    const int y = (int)(i / (pw / 4)), x = (int)(i - (long)y * (pw / 4)) * 4;
+    const uint8_t *row = half + (size_t)blockIdx.y * hw * hh + (size_t)clampi(y - P, 0, hh - 1) * hw;
+    uint32_t v = 0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) v |= (uint32_t)row[clampi(x + m - P, 0, hw - 1)] << (8 * m);
+    *(uint32_t *)(planes + (size_t)blockIdx.y * pstride + (size_t)y * pw + x) = v;
+}
+
+template <int V>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
+void k_la_search(const uint8_t *__restrict__ half, int mb_w, int mb_h, int nrefs, int R, int P,
+                 const uint8_t *__restrict__ planes, size_t pstride, uint64_t *__restrict__ keys)
+{
+    __shared__ int32_t s_lr[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = 0; // zero rate: the raster-first SAD minimum
+    __syncthreads();
+    const int f = 1 + (int)blockIdx.y / nrefs, k = 1 + (int)blockIdx.y % nrefs;
+    if (k > f) return; // no such reference (the whole workgroup)
+    const int hw = mb_w * 8, hh = mb_h * 8, nblk = mb_w * mb_h;
+    const int NG = fs_groups<V>(2 * R + 1);
+    const long gi = (long)blockIdx.x * 256 + threadIdx.x;
+    const int b = (int)(gi / NG);
+    const bool active = b < nblk;
+    const int bs = active ? b : nblk - 1;
+    FsJob J;
+    J.cw = hw;
+    J.pw = hw + 2 * P;
+    J.ph = hh + 2 * P;
+    J.pad = P;
+    J.R = R;
+    J.g = (int)(gi - (long)b * NG);
+    J.lx = (bs % mb_w) * 8;
+    J.ly = (bs / mb_w) * 8;
+    J.cx = J.cy = J.mpx = J.mpy = 0;
+    J.F = planes + (size_t)(f - k) * pstride;
+    J.srcp = half + (size_t)f * hw * hh + (size_t)J.ly * hw + J.lx;
+    const uint64_t key = fs_key<8, 8, V>(J, s_lr);
+    fs_commit(key, b, active, keys + ((size_t)(f - 1) * nrefs + (k - 1)) * nblk + bs);
+}
+
+// One thread per (frame, block): intra = max(1, AC sum|H8 x H8| >> 1), and for
+// frames > 0 the cheapest reference's 8x8 SATD (ties to the nearest reference).
+__global__ __launch_bounds__(256) void k_la_cost(const uint8_t *__restrict__ half, int mb_w, int mb_h, int nframes,
+                                                 int nrefs, int R, int P, const uint8_t *__restrict__ planes,
+                                                 size_t pstride, const uint64_t *__restrict__ keys,
+                                                 int32_t *__restrict__ intra, int32_t *__restrict__ inter,
+                                                 int32_t *__restrict__ mv, int32_t *__restrict__ ref)
+{
+    const int nm = mb_w * mb_h;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)nframes * nm) return;
+    const int f = (int)(i / nm), m = (int)(i - (long)f * nm);
+    const int hw = mb_w * 8, hh = mb_h * 8, pw = hw + 2 * P;
+    const int bx0 = (m % mb_w) * 8, by0 = (m / mb_w) * 8;
+    const uint8_t *cur = half + (size_t)f * hw * hh + (size_t)by0 * hw + bx0;
+    uint32_t src[16]; // the block's rows as packed dwords
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        src[2 * j] = *(const uint32_t *)(cur + (size_t)j * hw);
+        src[2 * j + 1] = *(const uint32_t *)(cur + (size_t)j * hw + 4);
+    }
+    int d[64];
+    int dc = 0;
+#pragma unroll
+    for (int k = 0; k < 64; k++) {
+        d[k] = (src[k >> 2] >> (8 * (k & 3))) & 255;
+        dc += d[k];
+    }
+    // the DC coefficient of H8 x H8 is the block sum; the AC part is the rest
+    const int ic = max(1, (satd8x8_sum_abs(d) - dc) >> 1);
+    int best = 0x7fffffff, bdx = 0, bdy = 0, bk = 0;
+    const int NC = 2 * R + 1;
+    for (int k = 1; k <= nrefs && k <= f; k++) {
+        const uint32_t idx = (uint32_t)keys[((size_t)(f - 1) * nrefs + (k - 1)) * nm + m];
+        const int dy = (int)(idx / (uint32_t)NC) - R, dx = (int)(idx % (uint32_t)NC) - R;
+        const uint8_t *pl = planes + (size_t)(f - k) * pstride + (size_t)(P + by0 + dy) * pw + P + bx0 + dx;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+                d[j * 8 + b] = (int)((src[(j * 8 + b) >> 2] >> (8 * (b & 3))) & 255) - (int)pl[(size_t)j * pw + b];
+        const int c = satd8x8_sum_abs(d) >> 1;
+        if (c < best) {
+            best = c;
+            bdx = dx;
+            bdy = dy;
+            bk = k;
+        }
+    }
+    intra[i] = ic;
+    inter[i] = f == 0 ? ic : min(best, ic);
+    mv[2 * i] = 8 * bdx;
+    mv[2 * i + 1] = 8 * bdy;
+    ref[i] = bk;
 }
 
 // ---------------------------------------------------------------------------
@@ -2602,6 +2717,8 @@ int rh264_init(int device, rh264_ctx **out)
     c->prof_cap = c->prof_n = 0;
     c->ls_buf = nullptr;
     c->ls_cap = 0;
+    c->la_buf = nullptr;
+    c->la_cap = 0;
     c->fs_kernel = 0;
     c->refine_lds = 1;
     c->refine_mfma = 1;
@@ -2627,6 +2744,7 @@ int rh264_destroy(rh264_ctx *ctx)
     if (ctx->keys) (void)hipFree(ctx->keys);
     if (ctx->tmp_mv) (void)hipFree(ctx->tmp_mv);
     if (ctx->ls_buf) (void)hipFree(ctx->ls_buf);
+    if (ctx->la_buf) (void)hipFree(ctx->la_buf);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
     for (hipEvent_t e : ctx->prof) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -3124,8 +3242,8 @@ int rh264_rc_complexity(rh264_ctx *ctx, const uint8_t *cur, const uint8_t *ref, 
     return 0;
 }
 
-int rh264_mbtree_propagate_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, const int32_t *intra,
-                               const int32_t *inter, const int32_t *mv, int32_t *prop)
+static int mbtree_propagate(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, const int32_t *intra,
+                            const int32_t *inter, const int32_t *mv, const int32_t *refs, int32_t *prop)
 {
     if (mb_w <= 0 || mb_h <= 0) return RH264_DECLINED;
     if (nframes <= 0) return 0;
@@ -3136,9 +3254,67 @@ int rh264_mbtree_propagate_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, 
     LAUNCH_CHECK("k_fill_u64");
     for (int f = nframes - 1; f >= 0; f--) {
         hipLaunchKernelGGL(k_mbtree_prop_i32, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, ctx->stream, f, mb_w,
-                           mb_h, intra, inter, mv, (unsigned long long *)ctx->keys, prop);
+                           mb_h, intra, inter, mv, refs, (unsigned long long *)ctx->keys, prop);
         LAUNCH_CHECK("k_mbtree_prop_i32");
     }
+    return 0;
+}
+
+int rh264_mbtree_propagate_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, const int32_t *intra,
+                               const int32_t *inter, const int32_t *mv, int32_t *prop)
+{
+    return mbtree_propagate(ctx, nframes, mb_w, mb_h, intra, inter, mv, nullptr, prop);
+}
+
+int rh264_mbtree_propagate_multi_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, const int32_t *intra,
+                                     const int32_t *inter, const int32_t *mv, const int32_t *ref, int32_t *prop)
+{
+    if (!ref) return fail("mbtree_propagate_multi_i32: null ref");
+    return mbtree_propagate(ctx, nframes, mb_w, mb_h, intra, inter, mv, ref, prop);
+}
+
+static int ensure_buf(rh264_ctx *ctx, void **buf, size_t *cap, size_t bytes);
+
+int rh264_mbtree_costs_multi(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_w, int mb_h, int nrefs,
+                             int range, int32_t *intra, int32_t *inter, int32_t *mv, int32_t *ref)
+{
+    if (mb_w <= 0 || mb_h <= 0) return RH264_DECLINED;
+    if (nrefs < 1 || nrefs > 4 || range < 1 || range > 64) return RH264_DECLINED;
+    if (nframes <= 0) return 0;
+    if (nframes > 16384) return fail("mbtree_costs_multi: nframes %d > 16384", nframes);
+    const int hw = mb_w * 8, hh = mb_h * 8, nm = mb_w * mb_h;
+    const int P = (range + 8 + 3) & ~3; // every window and every winner's block inside the padded plane
+    const int pw = hw + 2 * P, ph = hh + 2 * P;
+    const size_t pstride = (size_t)pw * ph;
+    int rc = ensure_buf(ctx, &ctx->la_buf, &ctx->la_cap, pstride * nframes + 256);
+    if (rc) return rc;
+    uint8_t *planes = (uint8_t *)ctx->la_buf;
+    hipLaunchKernelGGL(k_la_pad, dim3((unsigned)((pstride / 4 + 255) / 256), nframes), dim3(256), 0, ctx->stream, half,
+                       hw, hh, P, planes, pstride);
+    LAUNCH_CHECK("k_la_pad");
+    if (nframes > 1) {
+        const size_t nk = (size_t)(nframes - 1) * nrefs * nm;
+        rc = ensure_keys(ctx, nk);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, ctx->stream, ctx->keys, nk,
+                           ~0ull);
+        LAUNCH_CHECK("k_fill_u64");
+        const dim3 grid_y(1, (unsigned)((nframes - 1) * nrefs));
+        if (ctx->fs_kernel == 1) {
+            const long lanes = (long)nm * fs_groups<1>(2 * range + 1);
+            hipLaunchKernelGGL(k_la_search<1>, dim3((unsigned)((lanes + 255) / 256), grid_y.y), dim3(256), 0,
+                               ctx->stream, half, mb_w, mb_h, nrefs, range, P, planes, pstride, ctx->keys);
+        } else {
+            const long lanes = (long)nm * fs_groups<0>(2 * range + 1);
+            hipLaunchKernelGGL(k_la_search<0>, dim3((unsigned)((lanes + 255) / 256), grid_y.y), dim3(256), 0,
+                               ctx->stream, half, mb_w, mb_h, nrefs, range, P, planes, pstride, ctx->keys);
+        }
+        LAUNCH_CHECK("k_la_search");
+    }
+    const long nb = (long)nframes * nm;
+    hipLaunchKernelGGL(k_la_cost, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, half, mb_w, mb_h,
+                       nframes, nrefs, range, P, planes, pstride, ctx->keys, intra, inter, mv, ref);
+    LAUNCH_CHECK("k_la_cost");
     return 0;
 }
 

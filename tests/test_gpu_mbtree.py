@@ -98,3 +98,31 @@ def test_rc_complexity_floor(dev):
     assert (accel.rc_complexity(dev, int(fb), None, 2, 64, 48) == 1.0).all()
     assert (accel.rc_complexity(dev, int(fb) + 64 * 48, int(fb), 1, 64, 48) == 1.0).all()
     fb.free()
+
+
+@pytest.mark.parametrize("cw,ch,n,nrefs,R,kind", [(64, 48, 5, 3, 8, "noise"), (176, 144, 4, 3, 16, "cif"),
+                                                  (352, 288, 4, 3, 64, "cif"), (16, 16, 6, 4, 1, "noise"),
+                                                  (640, 368, 3, 2, 24, "hd")])
+def test_mbtree_multi_reference_bit_exact(dev, cw, ch, n, nrefs, R, kind):
+    """The build-defined multi-reference form (exhaustive +-R per reference, 8x8
+    Hadamard costs, cheapest reference) and its propagation == the oracle."""
+    if kind == "cif":
+        frames = synth.cif_clip(n, seed=cw + R, w=cw, h=ch)
+    elif kind == "hd":
+        frames = synth.hd_clip(n, seed=11, w=cw, h=ch)
+    else:
+        frames = np.random.default_rng(cw + n).integers(0, 256, (n, ch, cw), dtype=np.uint8)
+    mb_w, mb_h = cw // 16, ch // 16
+    halves = np.stack([O.downsample2x(f, cw, ch) for f in frames])
+    intra, inter, mv, ref = O.mbtree_costs_multi(halves, mb_w, mb_h, nrefs, R, nthreads=16)
+    prop = O.propagate_multi_i32(intra, inter, mv, ref, mb_w, mb_h)
+    fb = dev.upload(frames)
+    mt = accel.MbTree(dev, cw, ch, n)
+    mt.run_multi(fb, nrefs=nrefs, search_range=R)
+    got = mt.results()
+    assert np.array_equal(got["intra"], intra)
+    for f in range(n):
+        bad = np.nonzero((got["inter"][f] != inter[f]) | (got["mv"][f] != mv[f]).any(1) | (got["ref"][f] != ref[f]))[0]
+        assert bad.size == 0, f"frame {f}: {bad.size} MBs differ, first {bad[:5]}"
+    assert np.array_equal(got["prop"], prop)
+    fb.free()

@@ -511,3 +511,85 @@ def test_frame_costs_and_propagation():
         amount = ((intra[f].astype(np.int64) + pi[f]) * (intra[f] - inter[f])) // intra[f]
         sent = int(amount.sum())
         assert sent - 4 * mb_w * mb_h <= int(pi[f - 1].sum()) <= sent
+
+
+# ------------------------------------------------ multi-reference lookahead
+
+def _la_textbook(halves, f, nrefs, R, bx0, by0):
+    """Exhaustive SAD over +-R per reference with numpy edge-clamped indexing (raster
+    first minimum), the Kronecker-form 8x8 SATD at the winner, the cheapest reference
+    (ties to the nearest)."""
+    n, ch, cw = halves.shape
+    cur = halves[f, by0:by0 + 8, bx0:bx0 + 8].astype(np.int64)
+    best = None
+    for k in range(1, min(nrefs, f) + 1):
+        ref = halves[f - k]
+        win = None
+        for dy in range(-R, R + 1):
+            for dx in range(-R, R + 1):
+                rows = np.clip(np.arange(by0, by0 + 8) + dy, 0, ch - 1)
+                cols = np.clip(np.arange(bx0, bx0 + 8) + dx, 0, cw - 1)
+                sad = int(np.abs(cur - ref[np.ix_(rows, cols)]).sum())
+                if win is None or sad < win[0]:
+                    win = (sad, dx, dy)
+        _, dx, dy = win
+        rows = np.clip(np.arange(by0, by0 + 8) + dy, 0, ch - 1)
+        cols = np.clip(np.arange(bx0, bx0 + 8) + dx, 0, cw - 1)
+        c = refimpl.satd8_sum(halves[f, by0:by0 + 8, bx0:bx0 + 8], ref[np.ix_(rows, cols)])
+        if best is None or c < best[0]:
+            best = (c, (8 * dx, 8 * dy), k)
+    return best
+
+
+def test_lookahead_multi_block_brute_force():
+    """Each block's (cost, mv, ref) == the numpy brute force, interior and edge
+    blocks, 1..3 references, ranges that leave the frame."""
+    rng = np.random.default_rng(70)
+    base = rng.integers(0, 256, (40, 48), dtype=np.uint8)
+    halves = np.stack([np.roll(base, (t, -2 * t), axis=(0, 1)) for t in range(4)])
+    halves[2] = rng.integers(0, 256, (40, 48), dtype=np.uint8)  # a frame unrelated to the others
+    ch, cw = 40, 48
+    for f, nrefs, R in ((1, 1, 3), (3, 3, 4), (3, 2, 2), (2, 3, 5)):
+        for bx0, by0 in ((0, 0), (16, 8), (40, 32), (24, 16)):
+            got = O.la_block_multi(halves, cw, ch, f, nrefs, R, bx0, by0)
+            assert got == _la_textbook(halves, f, nrefs, R, bx0, by0), (f, nrefs, R, bx0, by0)
+
+
+def test_lookahead_multi_reference_choice_and_floor():
+    """A frame that repeats the frame three back (and nothing nearer) picks ref 3 with
+    mv 0 and cost 0; a static clip picks the nearest reference (ties to the smaller
+    k); a flat frame's intra is the floor 1."""
+    rng = np.random.default_rng(71)
+    fr = rng.integers(0, 256, (4, 32, 48), dtype=np.uint8)
+    fr[3] = fr[0]
+    intra, inter, mv, ref = O.mbtree_costs_multi(fr, 6, 4, nrefs=3, search_range=2)
+    assert (ref[3] == 3).all() and (inter[3] == 0).all() and (mv[3] == 0).all()
+    assert (ref[0] == 0).all() and (inter[0] == intra[0]).all()
+    static = np.repeat(fr[:1], 4, axis=0)
+    _, inter_s, _, ref_s = O.mbtree_costs_multi(static, 6, 4, nrefs=3, search_range=2)
+    assert (inter_s[1:] == 0).all() and (ref_s[1:] == 1).all()
+    flat = np.full((2, 16, 16), 9, np.uint8)
+    assert (O.mbtree_costs_multi(flat, 2, 2, nrefs=1, search_range=1)[0] == 1).all()
+
+
+def test_propagate_multi_reduces_to_single_and_conserves():
+    """With every block referencing the previous frame the multi-reference
+    propagation is the single-reference one; with mixed references, what each frame
+    sends arrives (up to the per-share truncation) in the frames it references."""
+    mb_w, mb_h, n = 4, 3, 6
+    rng = np.random.default_rng(72)
+    nm = mb_w * mb_h
+    intra = rng.integers(100, 5000, (n, nm)).astype(np.int32)
+    inter = (intra * rng.uniform(0.1, 1.0, (n, nm))).astype(np.int32)
+    mv = rng.integers(-40, 41, (n, nm, 2)).astype(np.int32)
+    single = O.propagate_i32(intra, inter, mv, mb_w, mb_h)
+    assert np.array_equal(O.propagate_multi_i32(intra, inter, mv, None, mb_w, mb_h), single)
+    assert np.array_equal(O.propagate_multi_i32(intra, inter, mv, np.ones((n, nm), np.int32), mb_w, mb_h), single)
+    ref = np.minimum(rng.integers(1, 4, (n, nm)), np.arange(n)[:, None]).astype(np.int32)
+    p = O.propagate_multi_i32(intra, inter, mv, ref, mb_w, mb_h)
+    sent = np.zeros(n, np.int64)
+    for f in range(n - 1, 0, -1):
+        amount = ((intra[f].astype(np.int64) + p[f]) * (intra[f] - inter[f])) // intra[f]
+        np.add.at(sent, f - ref[f], np.maximum(amount, 0))
+    recv = p.astype(np.int64).sum(1)
+    assert (recv <= sent).all() and (sent - recv <= 4 * nm * 3).all() and sent[:-1].sum() > 0
