@@ -294,105 +294,165 @@ def copy_probe(mib):
 
 
 def bench_lookahead4k(args):
-    """4K mb-tree lookahead over a 40-frame window: 3840x2160 frames downsampled to
-    half resolution (1920x1080, 8x8 blocks = one per 16x16 MB), intra AC-SATD and the
-    seeded cascade diamond against the previous frame (the reference's HalfRes mode,
-    mbtree.rs:61-297), then int32 backward propagation over the 40 frames, and the
-    rate-control complexity of every full-resolution frame (lookahead.rs:38-101; frame
-    0 as the IDR, each other against its predecessor).  One step = the whole window.
-    Mpx/s = 3840*2160*40 / step time.  The first three frames' costs and complexities
-    and the whole window's propagation are checked against the oracle."""
+    """4K lookahead over a 40-frame window of 3840x2160 frames.  One step = the
+    whole window:
+      * downsample to half resolution (1920x1080; 8x8 blocks, one per 16x16 MB);
+      * costs, in the configuration's form (build-defined, the headline): against
+        each of the 3 previous frames an exhaustive +-64 full-pel search, the 8x8
+        Hadamard cost of each winner, the cheapest reference; or in the reference's
+        form (reported beside it): the seeded cascade diamond against the previous
+        frame with 4x4 Hadamard costs (mbtree.rs:61-297);
+      * int32 backward propagation (to each block's chosen reference);
+      * the rate-control complexity of every full-resolution frame
+        (lookahead.rs:38-101; frame 0 as the IDR, the others against their
+        predecessors).
+    Mpx/s = 3840*2160*40 / step time.  Parity: sampled blocks of frames 1..3 and
+    whole frames 0..2 against the oracle, and each form's propagation from the
+    GPU's own costs."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle.oracle import Oracle
     from rusty_h264_amd import accel, synth
     w, h, n = 3840, 2160, 40
+    nrefs, rng_la = 3, 64
     mb_w, mb_h = w // 16, h // 16
+    nm = mb_w * mb_h
     t_gen = time.perf_counter()
     frames = synth.hd_clip(n, seed=SEED + 5, w=w, h=h)
     t_gen = time.perf_counter() - t_gen
     with accel.Device(0) as dev:
         dname = dev.name()
         fb = dev.upload(frames)
+        for kv in args.tune:
+            k, v = kv.split("=")
+            dev.tune(k, int(v))
         mt = accel.MbTree(dev, w, h, n)
+        mref = dev.alloc(4 * nm * n)
         cx = dev.alloc(8 * n)
         lib = dev.lib
 
-        def step(ev=None):
-            if ev is not None:
-                dev.record(ev)
+        def step(multi, ev=None):
+            def mark(k):
+                if ev is not None:
+                    dev.record(ev + k)
+            mark(0)
             accel._check(lib.rh264_mbtree_downsample(dev.ctx, int(fb), w * h, n, w, h, mt.half.ptr), "downsample")
-            if ev is not None:
-                dev.record(ev + 1)
-            accel._check(lib.rh264_mbtree_costs(dev.ctx, mt.half.ptr, n, mb_w, mb_h, mt.intra.ptr, mt.inter.ptr,
-                                                mt.mv.ptr), "costs")
-            if ev is not None:
-                dev.record(ev + 2)
-            accel._check(lib.rh264_mbtree_propagate_i32(dev.ctx, n, mb_w, mb_h, mt.intra.ptr, mt.inter.ptr,
-                                                        mt.mv.ptr, mt.prop.ptr), "propagate")
-            if ev is not None:
-                dev.record(ev + 3)
+            mark(1)
+            if multi:
+                accel._check(lib.rh264_mbtree_costs_multi(dev.ctx, mt.half.ptr, n, mb_w, mb_h, nrefs, rng_la,
+                                                          mt.intra.ptr, mt.inter.ptr, mt.mv.ptr, mref.ptr), "costs")
+            else:
+                accel._check(lib.rh264_mbtree_costs(dev.ctx, mt.half.ptr, n, mb_w, mb_h, mt.intra.ptr, mt.inter.ptr,
+                                                    mt.mv.ptr), "costs")
+            mark(2)
+            if multi:
+                accel._check(lib.rh264_mbtree_propagate_multi_i32(dev.ctx, n, mb_w, mb_h, mt.intra.ptr, mt.inter.ptr,
+                                                                  mt.mv.ptr, mref.ptr, mt.prop.ptr), "propagate")
+            else:
+                accel._check(lib.rh264_mbtree_propagate_i32(dev.ctx, n, mb_w, mb_h, mt.intra.ptr, mt.inter.ptr,
+                                                            mt.mv.ptr, mt.prop.ptr), "propagate")
+            mark(3)
             accel._check(lib.rh264_rc_complexity(dev.ctx, int(fb), None, w * h, 1, w, h, cx.ptr), "complexity")
             accel._check(lib.rh264_rc_complexity(dev.ctx, int(fb) + w * h, int(fb), w * h, n - 1, w, h, cx.ptr + 8),
                          "complexity")
-            if ev is not None:
-                dev.record(ev + 4)
+            mark(4)
 
-        for _ in range(args.warmup):
-            step()
-        dev.sync()
-        t0 = time.perf_counter()
-        dev.record(0)
-        for _ in range(args.steps):
-            step()
-        dev.record(1)
-        dev.sync()
-        wall = time.perf_counter() - t0
-        ms = dev.elapsed_ms(0, 1) / args.steps
-        step(ev=2)  # one more, with phase events
-        dev.sync()
-        phases = {"downsample": dev.elapsed_ms(2, 3), "costs": dev.elapsed_ms(3, 4), "propagate": dev.elapsed_ms(4, 5),
-                  "rc_complexity": dev.elapsed_ms(5, 6)}
-        res = mt.results()
-        cplx = dev.download(cx, (n,), np.float64)
-    # parity spot check: costs of frames 0..2 and the propagation from the GPU's own costs
+        def timed(multi):
+            for _ in range(args.warmup):
+                step(multi)
+            dev.sync()
+            t0 = time.perf_counter()
+            dev.record(0)
+            for _ in range(args.steps):
+                step(multi)
+            dev.record(1)
+            dev.sync()
+            wall = time.perf_counter() - t0
+            ms = dev.elapsed_ms(0, 1) / args.steps
+            step(multi, ev=2)  # one more, with phase events
+            dev.sync()
+            names = ("downsample", "costs", "propagate", "rc_complexity")
+            phases = {k: round(dev.elapsed_ms(2 + i, 3 + i), 4) for i, k in enumerate(names)}
+            res = mt.results()
+            if multi:
+                res["ref"] = dev.download(mref, (n, nm), np.int32)
+            res["cplx"] = dev.download(cx, (n,), np.float64)
+            return ms, wall, phases, res
+
+        ms_r, wall_r, ph_r, res_r = timed(False)
+        ms_m, wall_m, ph_m, res_m = timed(True)
     o = Oracle(fast=True)
-    halves = [o.downsample2x(frames[f], w, h) for f in range(3)]
-    ok = all(np.array_equal(res["half"][f], halves[f]) for f in range(3))
+    halves = [o.downsample2x(frames[f], w, h) for f in range(4)]
+    # reference form: whole frames 0..2
+    ok_r = all(np.array_equal(res_r["half"][f], halves[f]) for f in range(3))
     for f in range(3):
         a, b, m = o.frame_costs_half(halves[f], halves[f - 1] if f else None, mb_w, mb_h)
-        ok = ok and np.array_equal(res["intra"][f], a) and np.array_equal(res["inter"][f], b) and \
-            np.array_equal(res["mv"][f], m)
-    ok = ok and np.array_equal(res["prop"], o.propagate_i32(res["intra"], res["inter"], res["mv"], mb_w, mb_h))
-    ok = ok and all(cplx[f] == o.rc_complexity(frames[f], frames[f - 1] if f else None, w, h) for f in range(3))
+        ok_r = ok_r and np.array_equal(res_r["intra"][f], a) and np.array_equal(res_r["inter"][f], b) and \
+            np.array_equal(res_r["mv"][f], m)
+    ok_r = ok_r and np.array_equal(res_r["prop"], o.propagate_i32(res_r["intra"], res_r["inter"], res_r["mv"],
+                                                                  mb_w, mb_h))
+    ok_c = all(res_m["cplx"][f] == o.rc_complexity(frames[f], frames[f - 1] if f else None, w, h) for f in range(3))
+    # multi-reference form: 64 sampled blocks of frames 1..3 (the exhaustive oracle is
+    # slow at 4K), frame 0 whole, and the propagation
+    hv = np.stack(halves)
+    smp = np.random.default_rng(7).integers(0, nm, (3, 64))
+    ok_m = True
+    i0, _, _, _ = o.mbtree_costs_multi(hv[:1], mb_w, mb_h, nrefs, rng_la)
+    ok_m = ok_m and np.array_equal(res_m["intra"][0], i0[0]) and np.array_equal(res_m["inter"][0], i0[0])
+    for f in (1, 2, 3):
+        for mbi in smp[f - 1]:
+            c, mvv, k = o.la_block_multi(hv, w // 2, h // 2, f, nrefs, rng_la, (mbi % mb_w) * 8, (mbi // mb_w) * 8)
+            ic = int(res_m["intra"][f, mbi])
+            ok_m = ok_m and int(res_m["inter"][f, mbi]) == min(c, ic) and tuple(res_m["mv"][f, mbi]) == mvv and \
+                int(res_m["ref"][f, mbi]) == k
+    ok_m = ok_m and np.array_equal(res_m["prop"], o.propagate_multi_i32(res_m["intra"], res_m["inter"], res_m["mv"],
+                                                                        res_m["ref"], mb_w, mb_h))
+    mpx = lambda ms: round(w * h * n / (ms / 1e3) / 1e6, 2)
     out = {"metric": "4K lookahead Mpx/s (40-frame window: half-res mb-tree costs, int32 propagation, "
                      "RC complexity)",
-           "value": round(w * h * n / (ms / 1e3) / 1e6, 2), "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+           "value": mpx(ms_m), "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_m, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
-           "config": {"workload": "3840x2160, 40 frames, HalfRes mb-tree (1920x1080, 8x8 blocks, 1 ref = previous "
-                                  "frame, seeded 4-point cascade diamond steps 8/4/2/1), int32 backward propagation, "
-                                  "full-res RC complexity (9 full-pel candidates per MB)",
+           "config": {"workload": "3840x2160, 40 frames; half-res (1920x1080) 8x8 blocks, 3 references, exhaustive "
+                                  "+-64 full-pel search per reference, 8x8 Hadamard costs, cheapest reference; int32 "
+                                  "propagation to the chosen references; full-res RC complexity (9 full-pel "
+                                  "candidates per MB)",
                       "device": dname},
-           "phase_ms": {k: round(v, 4) for k, v in phases.items()},
-           "parity_vs_oracle": "bit-exact" if ok else "MISMATCH",
-           "wall_ms_per_step": round(wall * 1e3 / args.steps, 4), "generate_s": round(t_gen, 1)}
+           "phase_ms": ph_m,
+           "paths": {"multi_ref": {"mpx_s": mpx(ms_m), "ms_per_step": round(ms_m, 4), "phase_ms": ph_m,
+                                   "parity_vs_oracle": "bit-exact" if ok_m and ok_c else "MISMATCH"},
+                     "reference_form": {"mpx_s": mpx(ms_r), "ms_per_step": round(ms_r, 4), "phase_ms": ph_r,
+                                        "search": "seeded 4-point cascade diamond (steps 8/4/2/1) against the "
+                                                  "previous frame, 4x4 Hadamard costs (mbtree.rs:154-263)",
+                                        "parity_vs_oracle": "bit-exact" if ok_r and ok_c else "MISMATCH"}},
+           "parity_vs_oracle": "bit-exact" if ok_m and ok_r and ok_c else "MISMATCH",
+           "wall_ms_per_step": round(wall_m * 1e3 / args.steps, 4), "generate_s": round(t_gen, 1)}
     if not args.no_cpu:
-        # the same window on the host: the oracle's downsample and frame costs (one C
-        # call per frame, frames spread over a thread pool), then its propagation
         cores = min(os.cpu_count() or 1, 16)
+        # reference form: the whole window on the host (downsample, frame costs, int32
+        # propagation, RC complexity), frames spread over a thread pool
         t0 = time.perf_counter()
         with ThreadPoolExecutor(cores) as ex:
-            hv = list(ex.map(lambda f: o.downsample2x(frames[f], w, h), range(n)))
-            cs = list(ex.map(lambda f: o.frame_costs_half(hv[f], hv[f - 1] if f else None, mb_w, mb_h), range(n)))
+            hvs = list(ex.map(lambda f: o.downsample2x(frames[f], w, h), range(n)))
+            cs = list(ex.map(lambda f: o.frame_costs_half(hvs[f], hvs[f - 1] if f else None, mb_w, mb_h), range(n)))
             list(ex.map(lambda f: o.rc_complexity(frames[f], frames[f - 1] if f else None, w, h), range(n)))
         o.propagate_i32(np.stack([c[0] for c in cs]), np.stack([c[1] for c in cs]), np.stack([c[2] for c in cs]),
                         mb_w, mb_h)
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(w * h * n / dt / 1e6, 3), "unit": "Mpx/s", "cores": cores,
-                               "kind": "port", "sample": f"the whole 40-frame window once (downsample, frame costs, "
-                                                         f"int32 propagation, RC complexity), {cores} threads, "
-                                                         f"{dt:.1f} s"}
+        dt_r = time.perf_counter() - t0
+        # multi-reference form: its exhaustive search dominates; time 2 block rows of
+        # frame 3 (3 references each) and scale to the window's 39 x 32,400 blocks
+        blocks = [(x * 8, y * 8) for y in (40, 41) for x in range(mb_w)]
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(cores) as ex:
+            list(ex.map(lambda b: o.la_block_multi(hv, w // 2, h // 2, 3, nrefs, rng_la, b[0], b[1]), blocks))
+        dt_s = time.perf_counter() - t0
+        dt_m = dt_s * (n - 1) * nm / len(blocks)
+        out["cpu_baseline"] = {"value": round(w * h * n / dt_m / 1e6, 3), "unit": "Mpx/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"multi-reference search of {len(blocks)} blocks (2 block rows of frame 3, "
+                                         f"3 references, +-64) in {dt_s:.1f} s on {cores} threads, scaled to the "
+                                         f"window's {(n - 1) * nm} blocks; the reference form's whole window took "
+                                         f"{dt_r:.1f} s ({w * h * n / dt_r / 1e6:.0f} Mpx/s)"}
     print(json.dumps(out), flush=True)
 
 

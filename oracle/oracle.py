@@ -221,8 +221,8 @@ class Oracle:
     def la_block_multi(self, halves, cw, ch, f, nrefs, search_range, bx0, by0):
         mv = np.empty(2, np.int32)
         k = C.c_int32()
-        c = self.lib.orc_la_block_multi(_p(_u8(halves)), cw, ch, f, nrefs, search_range, bx0, by0, _p(mv, i32p),
-                                        C.byref(k))
+        c = self.lib.orc_la_block_multi(_p(_u8(halves)), int(cw), int(ch), int(f), int(nrefs), int(search_range),
+                                        int(bx0), int(by0), _p(mv, i32p), C.byref(k))
         return int(c), (int(mv[0]), int(mv[1])), k.value
 
     def propagate_multi_i32(self, intra, inter, mv, ref, mb_w, mb_h):

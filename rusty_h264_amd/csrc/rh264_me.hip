@@ -3304,6 +3304,10 @@ int rh264_mbtree_costs_multi(rh264_ctx *ctx, const uint8_t *half, int nframes, i
             const long lanes = (long)nm * fs_groups<1>(2 * range + 1);
             hipLaunchKernelGGL(k_la_search<1>, dim3((unsigned)((lanes + 255) / 256), grid_y.y), dim3(256), 0,
                                ctx->stream, half, mb_w, mb_h, nrefs, range, P, planes, pstride, ctx->keys);
+        } else if (ctx->fs_kernel == 2) {
+            const long lanes = (long)nm * fs_groups<2>(2 * range + 1);
+            hipLaunchKernelGGL(k_la_search<2>, dim3((unsigned)((lanes + 255) / 256), grid_y.y), dim3(256), 0,
+                               ctx->stream, half, mb_w, mb_h, nrefs, range, P, planes, pstride, ctx->keys);
         } else {
             const long lanes = (long)nm * fs_groups<0>(2 * range + 1);
             hipLaunchKernelGGL(k_la_search<0>, dim3((unsigned)((lanes + 255) / 256), grid_y.y), dim3(256), 0,
