@@ -182,8 +182,10 @@ int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases
  * 16x16/16x8/8x16 blocks stages each block's +-3 pel plane window in LDS, 0 =
  * it reads the plane sets in HBM; "refine_mfma" 1 (default) = the refinement of
  * 16x16 blocks computes its Hadamards on the matrix cores (one wave per block),
- * 0 = the lane-per-4x4 kernel above; "cand_quad" 1 (default) = the learned search's
- * candidate windows (local range <= 3) use quad-SADs, 0 = v_sad_u8 lanes;
+ * 0 = the lane-per-4x4 kernel above; "cand_quad" 2 (default) = the learned search's
+ * candidate windows (local range 1..4) take one lane per candidate, quad-SADs for
+ * column groups of four and v_sad_u8 for the rest, 1 = quad-SAD column groups
+ * (local range <= 3), 0 = v_sad_u8 lanes;
  * "fallback_v" 1 (default) = its fall-back full search uses the v_sad_u8 x2
  * variant, 0 = quad-SADs; "planes_rw" 1 (default) = the plane build slides a
  * register window down 256x32 tiles, 0 = the LDS-tile kernel; "mvnet_fused" 1

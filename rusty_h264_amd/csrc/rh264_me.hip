@@ -669,6 +669,29 @@ __device__ __forceinline__ uint64_t fs_key(const FsJob &J, const int32_t *s_lr)
     return fs_sad8_key<BW, BH, 4>(J, s_lr);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear workgroup id, and
+// each XCD has its own L2.  The search kernels renumber their workgroups so that
+// XCD x runs one contiguous share of the logical ones: neighbouring blocks, whose
+// reference windows overlap, then meet in one L2 instead of being fetched by all
+// eight.
+#ifndef RH264_XCD_MAP
+#define RH264_XCD_MAP 1
+#endif
+constexpr unsigned RH264_XCDS = 8;
+__device__ __forceinline__ unsigned xcd_linear(unsigned id, unsigned n)
+{
+    if (!RH264_XCD_MAP) return id;
+    const unsigned x = id % RH264_XCDS, i = id / RH264_XCDS, q = n / RH264_XCDS, r = n % RH264_XCDS;
+    return x * q + min(x, r) + i;
+}
+// logical (x, y) of this workgroup in a 2-D grid
+__device__ __forceinline__ void xcd_grid2(unsigned &bx, unsigned &by)
+{
+    const unsigned l = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    bx = l % gridDim.x;
+    by = l / gridDim.x;
+}
+
 // Single-stream integer search: nframes frames of one geometry (blockIdx.y = frame).
 template <int BW, int BH, int V>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
@@ -681,8 +704,10 @@ void k_full_search(rh264_search_params p, const uint8_t *__restrict__ cur, size_
     __syncthreads();
     const int NG = fs_groups<V>(2 * p.range + 1);
     const int nbx = p.cw / BW, nblk = nbx * (p.ch / BH);
-    const int frame = blockIdx.y;
-    const long gi = (long)blockIdx.x * 256 + threadIdx.x;
+    unsigned wgx, wgy;
+    xcd_grid2(wgx, wgy);
+    const int frame = (int)wgy;
+    const long gi = (long)wgx * 256 + threadIdx.x;
     const int b = (int)(gi / NG);
     const bool active = b < nblk;
     const int bs = active ? b : nblk - 1;
@@ -1226,9 +1251,11 @@ __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, cons
 
     const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int b = blockIdx.x * 4 + wave; // wave-uniform
+    unsigned wgx, wgy;
+    xcd_grid2(wgx, wgy);
+    const int b = (int)wgx * 4 + wave; // wave-uniform
     if (b >= nblk) return;
-    const int frame = blockIdx.y;
+    const int frame = (int)wgy;
     const size_t boff = (size_t)frame * nblk + b;
     const int lx = (b % nbx) << 4, ly = (b / nbx) << 4;
     const int r = lane & 15, qd = lane >> 4; // pixel row, column quad of this lane
@@ -1505,9 +1532,11 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
 
     const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int b = blockIdx.x * 4 + wave; // wave-uniform
+    unsigned wgx, wgy;
+    xcd_grid2(wgx, wgy);
+    const int b = (int)wgx * 4 + wave; // wave-uniform
     if (b >= nblk) return;
-    const int frame = blockIdx.y;
+    const int frame = (int)wgy;
     const size_t boff = (size_t)frame * nblk + b;
     const int lx = (b % nbx) << 4, ly = (b / nbx) << 4;
     const int r = lane & 15, qd = lane >> 4; // pixel row, column quad of this lane
@@ -2567,12 +2596,13 @@ void k_cand_search_q(rh264_search_params p, const uint8_t *__restrict__ cur, siz
     __shared__ uint32_t s_src[BPW][64];
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
     if (threadIdx.x < BPW) s_key[threadIdx.x] = ~0ull;
+    const unsigned wg = xcd_linear(blockIdx.x, gridDim.x);
     const int nbx = p.cw / 16, nblk = nbx * (p.ch / 16);
     // stage the sources: 64 threads per block, one dword each (row t/4, dword t%4)
 #pragma unroll
     for (int s0 = 0; s0 < BPW; s0 += 4) {
         const int sl = s0 + (threadIdx.x >> 6), t = threadIdx.x & 63;
-        const long g = (long)blockIdx.x * BPW + sl;
+        const long g = (long)wg * BPW + sl;
         if (g < nb) {
             const int frame = (int)(g / nblk), b = (int)(g - (long)frame * nblk);
             const uint8_t *src = cur + (size_t)frame * cur_stride + (size_t)((b / nbx) * 16 + (t >> 2)) * p.cw +
@@ -2582,7 +2612,7 @@ void k_cand_search_q(rh264_search_params p, const uint8_t *__restrict__ cur, siz
     }
     __syncthreads();
     const int slot = threadIdx.x / LPB, li = threadIdx.x - slot * LPB;
-    const long gb = (long)blockIdx.x * BPW + slot;
+    const long gb = (long)wg * BPW + slot;
     if (gb < nb) {
         const int k = li / NG, g = li - k * NG;
         const int frame = (int)(gb / nblk), b = (int)(gb - (long)frame * nblk);
@@ -2626,7 +2656,154 @@ void k_cand_search_q(rh264_search_params p, const uint8_t *__restrict__ cur, siz
     }
     __syncthreads();
     if (threadIdx.x < BPW) {
-        const long g = (long)blockIdx.x * BPW + threadIdx.x;
+        const long g = (long)wg * BPW + threadIdx.x;
+        if (g < nb) {
+            const uint64_t key = s_key[threadIdx.x];
+            const int kk = (int)((key >> 8) & 255), idx = (int)(key & 255);
+            const int ck = top8[g * LS_NCAND + kk];
+            start[2 * g] = ((ck % LS_BINS) - LS_BINS / 2) * 16 + 4 * (idx % NC - RL);
+            start[2 * g + 1] = ((ck / LS_BINS) - LS_BINS / 2) * 16 + 4 * (idx / NC - RL);
+            const bool fb = (int64_t)(key >> 32) > (int64_t)fb_cost;
+            flag[g] = fb ? 1 : 0;
+            if (fb) list[atomicAdd(count, 1u)] = (int32_t)g;
+        }
+    }
+}
+
+template <int RL, bool FAST>
+__device__ __forceinline__ void cand_rows_m(const uint32_t (&s)[16][4], const uint8_t *__restrict__ F, int pw, int ph,
+                                            int x, int y0, uint64_t (&aq)[(2 * RL + 1) / 4 > 0 ? (2 * RL + 1) / 4 : 1][2 * RL + 1],
+                                            uint32_t (&ar)[(2 * RL + 1) % 4 > 0 ? (2 * RL + 1) % 4 : 1][2 * RL + 1])
+{
+    constexpr int NC = 2 * RL + 1, NQ = NC / 4, REM = NC - 4 * NQ, NJ = 16 + 2 * RL;
+    constexpr int NW = (NC + 15 + 3) / 4; // realigned window dwords per row (bytes 0 .. NC + 14)
+    const int sh = x & 3;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        uint32_t w[NW];
+        if (FAST) {
+            const uint32_t *rp = (const uint32_t *)(F + (size_t)(y0 + j) * pw + (x & ~3));
+            uint32_t a[NW + 1];
+#pragma unroll
+            for (int i = 0; i < NW + 1; i++) a[i] = rp[i];
+#pragma unroll
+            for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(a[i + 1], a[i], (uint32_t)sh);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NW; i++) w[i] = load4_clamped(F, pw, ph, x + 4 * i, y0 + j);
+        }
+#pragma unroll
+        for (int t = 0; t < NC; t++) {
+            const int r = j - t;
+            if (r < 0 || r >= 16) continue;
+#pragma unroll
+            for (int g = 0; g < NQ; g++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    aq[g][t] = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)w[g + i + 1] << 32) | w[g + i], s[r][i],
+                                                               aq[g][t]);
+#pragma unroll
+            for (int q = 0; q < REM; q++) {
+                const int col = 4 * NQ + q; // window bytes col + 4i .. col + 4i + 3
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t ref4 = (col & 3) == 0 ? w[(col >> 2) + i]
+                                                         : __builtin_amdgcn_alignbyte(w[(col >> 2) + i + 1],
+                                                                                      w[(col >> 2) + i],
+                                                                                      (uint32_t)(col & 3));
+                    ar[q][t] = __builtin_amdgcn_sad_u8(s[r][i], ref4, ar[q][t]);
+                }
+            }
+        }
+    }
+}
+
+// The same search with one lane per (block, candidate): the lane reads each of its
+// window's 16 + 2RL reference rows once (one realigned span of 16 + 2RL bytes),
+// prices the window's columns in groups of four with quad-SADs and the 2RL + 1 - 4NQ
+// remaining columns with v_sad_u8 (no column beyond the window is computed), so a
+// block costs half the loads of k_cand_search_q and no wasted quad-SAD columns.
+// Keys and tie-breaks are those of k_cand_search.
+template <int RL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RL <= 2 ? 3 : 2)))
+void k_cand_search_m(rh264_search_params p, const uint8_t *__restrict__ cur, size_t cur_stride,
+                     const uint8_t *__restrict__ planes, size_t planes_stride, const int32_t *__restrict__ top8,
+                     const int32_t *__restrict__ mvps, long nb, int32_t fb_cost, int32_t *__restrict__ start,
+                     int32_t *__restrict__ flag, unsigned *__restrict__ count, int32_t *__restrict__ list)
+{
+    constexpr int NC = 2 * RL + 1, NQ = NC / 4, REM = NC - 4 * NQ, BPW = 256 / LS_NCAND, NJ = 16 + 2 * RL;
+    constexpr int NW = (NC + 15 + 3) / 4; // realigned window dwords per row (bytes 0 .. NC + 14)
+    __shared__ int32_t s_lr[RH264_LR_N];
+    __shared__ unsigned long long s_key[BPW];
+    __shared__ uint32_t s_src[BPW][64];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
+    if (threadIdx.x < BPW) s_key[threadIdx.x] = ~0ull;
+    const unsigned wg = xcd_linear(blockIdx.x, gridDim.x);
+    const int nbx = p.cw / 16, nblk = nbx * (p.ch / 16);
+    // stage the sources: 64 threads per block, one dword each (row t/4, dword t%4)
+#pragma unroll
+    for (int s0 = 0; s0 < BPW; s0 += 4) {
+        const int sl = s0 + (threadIdx.x >> 6), t = threadIdx.x & 63;
+        const long g = (long)wg * BPW + sl;
+        if (g < nb) {
+            const int frame = (int)(g / nblk), b = (int)(g - (long)frame * nblk);
+            const uint8_t *src = cur + (size_t)frame * cur_stride + (size_t)((b / nbx) * 16 + (t >> 2)) * p.cw +
+                                 (b % nbx) * 16 + 4 * (t & 3);
+            s_src[sl][t] = *(const uint32_t *)src;
+        }
+    }
+    __syncthreads();
+    const int slot = threadIdx.x / LS_NCAND, k = threadIdx.x - slot * LS_NCAND;
+    const long gb = (long)wg * BPW + slot;
+    if (gb < nb) {
+        const int frame = (int)(gb / nblk), b = (int)(gb - (long)frame * nblk);
+        const int lx = (b % nbx) * 16, ly = (b / nbx) * 16;
+        const int c = top8[gb * LS_NCAND + k];
+        const int cx = ((c % LS_BINS) - LS_BINS / 2) * 16, cy = ((c / LS_BINS) - LS_BINS / 2) * 16;
+        const int mpx = mvps ? mvps[2 * gb] : 0, mpy = mvps ? mvps[2 * gb + 1] : 0;
+        const int pw = p.cw + 2 * p.pad, ph = p.ch + 2 * p.pad;
+        const uint8_t *F = planes + (size_t)frame * planes_stride;
+        uint32_t sr[16][4];
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) sr[r][i] = s_src[slot][4 * r + i];
+        const int x = lx + (cx >> 2) + p.pad - RL; // padded column of the window's first candidate
+        const int y0 = ly + (cy >> 2) + p.pad - RL; // padded row of the window's first row
+        const bool fast = x >= 0 && (x & ~3) + 4 * (NW + 1) <= pw && y0 >= 0 && y0 + NJ <= ph;
+        const int sh = x & 3;
+        uint64_t aq[NQ > 0 ? NQ : 1][NC];
+        uint32_t ar[REM > 0 ? REM : 1][NC];
+#pragma unroll
+        for (int t = 0; t < NC; t++) {
+#pragma unroll
+            for (int g = 0; g < NQ; g++) aq[g][t] = 0;
+#pragma unroll
+            for (int q = 0; q < REM; q++) ar[q][t] = 0;
+        }
+        if (fast)
+            cand_rows_m<RL, true>(sr, F, pw, ph, x, y0, aq, ar);
+        else
+            cand_rows_m<RL, false>(sr, F, pw, ph, x, y0, aq, ar);
+        uint64_t best = ~0ull;
+#pragma unroll
+        for (int t = 0; t < NC; t++) {
+            const int ryb = mvbits(cy + 4 * (t - RL) - mpy);
+#pragma unroll
+            for (int col = 0; col < NC; col++) {
+                const uint32_t sad = col < 4 * NQ ? (uint32_t)(aq[col >> 2][t] >> (16 * (col & 3))) & 0xffffu
+                                                  : ar[col - 4 * NQ][t];
+                const int rxb = mvbits(cx + 4 * (col - RL) - mpx);
+                const uint64_t key = ((uint64_t)(sad + (uint32_t)s_lr[rxb + ryb]) << 32) | ((uint64_t)k << 8) |
+                                     (uint64_t)(t * NC + col);
+                best = key < best ? key : best;
+            }
+        }
+        atomicMin(&s_key[slot], (unsigned long long)best);
+    }
+    __syncthreads();
+    if (threadIdx.x < BPW) {
+        const long g = (long)wg * BPW + threadIdx.x;
         if (g < nb) {
             const uint64_t key = s_key[threadIdx.x];
             const int kk = (int)((key >> 8) & 255), idx = (int)(key & 255);
@@ -2722,7 +2899,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->fs_kernel = 0;
     c->refine_lds = 1;
     c->refine_mfma = 1;
-    c->cand_quad = 1;
+    c->cand_quad = 2;
     c->fallback_v = 1;
     c->planes_rw = 1;
     c->mvnet_fused = 1;
@@ -3128,7 +3305,8 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
         return 0;
     }
     if (!strcmp(knob, "cand_quad")) {
-        ctx->cand_quad = value ? 1 : 0;
+        if (value < 0 || value > 2) return fail("cand_quad %d: 0, 1 or 2", value);
+        ctx->cand_quad = value;
         return 0;
     }
     if (!strcmp(knob, "refine_mfma")) {
@@ -3722,7 +3900,23 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, count, (size_t)1, 0u);
-    if (local_range <= 3 && ctx->cand_quad) {
+    if (local_range >= 1 && local_range <= 4 && ctx->cand_quad == 2) {
+        // one lane per candidate: quad-SAD column groups plus v_sad_u8 for the rest
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keysf,
+                           (size_t)nb, ~0ull);
+#define CM_LAUNCH(RL)                                                                                              \
+    hipLaunchKernelGGL(k_cand_search_m<RL>, dim3((unsigned)((nb + 256 / LS_NCAND - 1) / (256 / LS_NCAND))), dim3(256), \
+                       0, ctx->stream, *p, cur, frame_stride, planes, planes_stride, top8, (const int32_t *)nullptr,  \
+                       nb, fallback_cost, start, fallback, count, list)
+        switch (local_range) {
+        case 1: CM_LAUNCH(1); break;
+        case 2: CM_LAUNCH(2); break;
+        case 3: CM_LAUNCH(3); break;
+        default: CM_LAUNCH(4); break;
+        }
+#undef CM_LAUNCH
+        LAUNCH_CHECK("k_cand_search_m");
+    } else if (local_range <= 3 && ctx->cand_quad) {
         // one pass over all candidates with quad-SADs (same result as the kernels below)
         hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keysf,
                            (size_t)nb, ~0ull);

@@ -96,7 +96,8 @@ def test_backward_gradients(dev):
 
 
 @pytest.mark.parametrize("Rl,cand_quad,fallback_v,pat", [(2, 1, 1, 4), (2, 1, 1, 2), (1, 1, 1, 4), (3, 1, 0, 2),
-                                                          (4, 1, 1, 4), (5, 1, 1, 2), (2, 0, 0, 4)])
+                                                          (4, 1, 1, 4), (5, 1, 1, 2), (2, 0, 0, 4), (2, 2, 1, 4),
+                                                          (1, 2, 1, 2), (3, 2, 1, 4), (4, 2, 0, 4)])
 def test_learned_search_recosting_is_exact(dev, Rl, cand_quad, fallback_v, pat):
     """Every step after the network is exact: oracle local searches around the GPU's
     eight centres, first minimum, fall-back full search, then the refinement (the
@@ -104,13 +105,14 @@ def test_learned_search_recosting_is_exact(dev, Rl, cand_quad, fallback_v, pat):
     and once with the threshold at the median candidate cost so both paths are
     exercised.  Local ranges up to 3 take the quad-SAD candidate kernel (cand_quad
     1) or the v_sad_u8 one, 4 the latter, larger ones the per-candidate integer
-    searches; both fall-back search variants are covered."""
+    searches; cand_quad 2 is the lane-per-candidate kernel (ranges 1..4); both
+    fall-back search variants are covered."""
     dev.tune("cand_quad", cand_quad)
     dev.tune("fallback_v", fallback_v)
     try:
         _learned_recost_case(dev, Rl, pat)
     finally:
-        dev.tune("cand_quad", 1)
+        dev.tune("cand_quad", 2)
         dev.tune("fallback_v", 1)
 
 
