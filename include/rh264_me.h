@@ -187,7 +187,7 @@ int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases
  * column groups of four and v_sad_u8 for the rest, 1 = quad-SAD column groups
  * (local range <= 3), 0 = v_sad_u8 lanes;
  * "fallback_v" 1 (default) = its fall-back full search uses the v_sad_u8 x2
- * variant, 0 = quad-SADs; "planes_rw" 1 (default) = the plane build slides a
+ * variant, 2 = v_sad_u8 with one candidate per lane, 0 = quad-SADs; "planes_rw" 1 (default) = the plane build slides a
  * register window down 256x32 tiles, 0 = the LDS-tile kernel; "mvnet_fused" 1
  * (default) = network inference
  * in one fused kernel per tile, 0 = layer by layer.  Every search variant returns

@@ -535,7 +535,7 @@ __device__ __forceinline__ void fs_commit(uint64_t key, long seg, bool active, u
 }
 
 // The same search with v_sad_u8 (4 abs-diff-adds per lane-op at full issue rate):
-// a lane owns NQ (2 or 4) candidates at ABSOLUTE padded columns base..base+NQ-1,
+// a lane owns NQ (1, 2 or 4) candidates at ABSOLUTE padded columns base..base+NQ-1,
 // base = (left edge of the window & ~3) + NQ*g, so each candidate's reference
 // dwords are the lane's aligned row dwords realigned by a byte shift in 0..3.
 // Accumulators are u32, one per (candidate row in flight, candidate).
@@ -655,10 +655,11 @@ __device__ __forceinline__ uint64_t fs_sad8_key(const FsJob &J, const int32_t *s
     return ((uint64_t)L.best << 32) | L.bidx;
 }
 
-// Lanes per block of each integer-search variant (V: 0 quad-SAD, 1 v_sad_u8 x2, 2 x4).
+// Lanes per block of each integer-search variant (V: 0 quad-SAD, 1 v_sad_u8 x2, 2 x4,
+// 3 v_sad_u8 with one candidate per lane: the most lanes, for short block lists).
 template <int V> __host__ __device__ constexpr int fs_groups(int NC)
 {
-    return V == 0 ? (NC + 3) / 4 : (V == 1 ? fs8_groups<2>(NC) : fs8_groups<4>(NC));
+    return V == 0 ? (NC + 3) / 4 : (V == 1 ? fs8_groups<2>(NC) : (V == 2 ? fs8_groups<4>(NC) : fs8_groups<1>(NC)));
 }
 
 template <int BW, int BH, int V>
@@ -666,7 +667,8 @@ __device__ __forceinline__ uint64_t fs_key(const FsJob &J, const int32_t *s_lr)
 {
     if (V == 0) return fs_qsad_key<BW, BH>(J, s_lr);
     if (V == 1) return fs_sad8_key<BW, BH, 2>(J, s_lr);
-    return fs_sad8_key<BW, BH, 4>(J, s_lr);
+    if (V == 2) return fs_sad8_key<BW, BH, 4>(J, s_lr);
+    return fs_sad8_key<BW, BH, 1>(J, s_lr);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin by linear workgroup id, and
@@ -3301,7 +3303,8 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
         return 0;
     }
     if (!strcmp(knob, "fallback_v")) {
-        ctx->fallback_v = value ? 1 : 0;
+        if (value < 0 || value > 2) return fail("fallback_v %d: 0, 1 or 2", value);
+        ctx->fallback_v = value;
         return 0;
     }
     if (!strcmp(knob, "cand_quad")) {
@@ -3975,7 +3978,12 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     }
     // the fall-back list is short (a few % of the blocks): the v_sad_u8 variant with two
     // candidates per lane gives it twice the lanes of the quad-SAD one
-    if (ctx->fallback_v == 1) {
+    if (ctx->fallback_v == 2) {
+        const long lanes = nb * fs_groups<3>(2 * p->range + 1);
+        hipLaunchKernelGGL((k_full_search_list<16, 16, 3>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
+                           ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,
+                           (const int32_t *)nullptr, keysf);
+    } else if (ctx->fallback_v == 1) {
         const long lanes = nb * fs_groups<1>(2 * p->range + 1);
         hipLaunchKernelGGL((k_full_search_list<16, 16, 1>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
                            ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,

@@ -97,7 +97,7 @@ def test_backward_gradients(dev):
 
 @pytest.mark.parametrize("Rl,cand_quad,fallback_v,pat", [(2, 1, 1, 4), (2, 1, 1, 2), (1, 1, 1, 4), (3, 1, 0, 2),
                                                           (4, 1, 1, 4), (5, 1, 1, 2), (2, 0, 0, 4), (2, 2, 1, 4),
-                                                          (1, 2, 1, 2), (3, 2, 1, 4), (4, 2, 0, 4)])
+                                                          (1, 2, 1, 2), (3, 2, 1, 4), (4, 2, 0, 4), (2, 2, 2, 4)])
 def test_learned_search_recosting_is_exact(dev, Rl, cand_quad, fallback_v, pat):
     """Every step after the network is exact: oracle local searches around the GPU's
     eight centres, first minimum, fall-back full search, then the refinement (the
