@@ -410,6 +410,8 @@ constexpr int SX = 2 * S1 + 1;                                    // x0 window s
 constexpr int PSB = 80;                                           // LDS bytes per pixel (64 + 16 pad)
 constexpr int LDS_A = S1 * S1 * PSB, LDS_B = S2 * S2 * PSB;       // a1/a3 and a2/a4
 constexpr int LDS_X = FT * FT * PSB;                              // pooled features (BF16, PSB per cell)
+constexpr int HSB = 80;                                           // LDS bytes per head-weight row (64 + 16 pad)
+static_assert(K * HSB + K * 4 <= LDS_A, "the head's weights and bias fit the a1/a3 buffer");
 static_assert(SX * SX * 4 <= LDS_B, "the x0 window fits the a2/a4 buffer");
 
 __device__ __forceinline__ float4_t mfma_bf16_k32(u32x4 a, u32x4 b, float4_t c)
@@ -616,6 +618,18 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     conv32_window<S3>(s_a, s_b, wf + 2 * 9 * C * C, p + OFF_B4, oy, ox, 0, h8, w8, lane, wave);
     __syncthreads();
 
+    // s_a is free once every wave has finished L4: stage the head's BF16 weights there
+    // (rows of HSB bytes, padded so the 16 rows one MFMA reads fall in distinct banks)
+    // with its FP32 bias, so the head loop reads LDS instead of waiting on global loads
+    {
+        const u32x4 *src = (const u32x4 *)(wf + WF_HEAD);
+        for (int i = tid; i < K * 4; i += 256) {
+            const int row = i >> 2, part = i & 3;
+            *(u32x4 *)(s_a + row * HSB + 16 * part) = src[i];
+        }
+        float *s_bh = (float *)(s_a + K * HSB);
+        for (int i = tid; i < K; i += 256) s_bh[i] = p[OFF_BH + i];
+    }
     // 2x2 average pool -> f[cell][c] as BF16 (PSB bytes per cell); two channels per thread
     for (int i = tid; i < FT * FT * C / 2; i += 256) {
         const int cell = i / (C / 2), c = 2 * (i - cell * (C / 2)), cy = cell / FT, cx = cell - cy * FT;
@@ -638,7 +652,7 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     const bool cvalid = gcy < hc && gcx < wc;
     const size_t gcell = ((size_t)img * hc + (cvalid ? gcy : 0)) * wc + (cvalid ? gcx : 0);
     const u32x4 fv = *(const u32x4 *)(s_f + cell * PSB + 16 * q); // B[k = 8q + i][cell]
-    const uint16_t *whb = wf + WF_HEAD;
+    const float *s_bh = (const float *)(s_a + K * HSB);
     uint32_t top[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) top[j] = 0u;
@@ -646,7 +660,7 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
 #pragma unroll 2
     for (int bt = 0; bt < NBT; bt++) {
         const int arow = 16 * bt + col;
-        const u32x4 wv = arow < K ? *(const u32x4 *)(whb + (size_t)arow * C + 8 * q) : u32x4{0, 0, 0, 0};
+        const u32x4 wv = arow < K ? *(const u32x4 *)(s_a + arow * HSB + 16 * q) : u32x4{0, 0, 0, 0};
         float4_t acc = {0, 0, 0, 0};
         acc = mfma_bf16_k32(wv, fv, acc);
         uint32_t nk[4];
@@ -654,7 +668,7 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
         for (int i = 0; i < 4; i++) {
             const int bin = 16 * bt + 4 * q + i;
             const bool ok = bin < K;
-            const float z = acc[i] + (ok ? p[OFF_BH + bin] : 0.0f);
+            const float z = acc[i] + (ok ? s_bh[bin] : 0.0f);
             if (logits && cvalid && ok) logits[gcell * K + bin] = z;
             nk[i] = ok ? rank_key(z, bin) : 0u;
         }
