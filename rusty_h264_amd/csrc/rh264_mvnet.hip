@@ -451,69 +451,80 @@ __device__ __forceinline__ void load_bias(const float *__restrict__ bias, int q,
 // 3x3 conv, 32 -> 32, from window `in` (side SIN) to window `out` (side SOUT =
 // SIN - 2) whose origin lies `m` pixels up-left of the tile origin (oy, ox).  A
 // wave takes two 16-pixel tiles at a time (four independent accumulators).
-template <int SIN>
-__device__ __forceinline__ void conv32_window(const uint8_t *in, uint8_t *out, const uint16_t *__restrict__ wt,
-                                              const float *__restrict__ bias, int oy, int ox, int m, int h8, int w8,
-                                              int lane, int wave)
+// The A operands (weights) of one 3x3 32 -> 32 layer for this lane.
+__device__ __forceinline__ void load_w32(const uint16_t *__restrict__ wt, int lane, u32x4 (&wa)[9][2])
 {
-    constexpr int SOUT = SIN - 2, NPX = SOUT * SOUT, NT = (NPX + 15) / 16;
     const int q = lane >> 4, col = lane & 15;
-    u32x4 wa[9][2];
 #pragma unroll
     for (int t = 0; t < 9; t++)
 #pragma unroll
         for (int h = 0; h < 2; h++) wa[t][h] = *(const u32x4 *)(wt + ((size_t)t * C + 16 * h + col) * C + 8 * q);
-    float b[2][4];
-    load_bias(bias, q, b);
-    for (int tile = wave; tile < NT; tile += 8) {
-        int pc[2], y[2], x[2];
-        bool valid[2];
+}
+
+// wa, b: the layer's weights and bias, loaded by the caller ahead of the barrier
+// before the layer so that their latency overlaps the other waves' work.
+#ifndef RH264_CONV_U
+#define RH264_CONV_U 2
+#endif
+template <int SIN, int U = RH264_CONV_U>
+__device__ __forceinline__ void conv32_window(const uint8_t *in, uint8_t *out, const u32x4 (&wa)[9][2],
+                                              const float (&b)[2][4], int oy, int ox, int m, int h8, int w8,
+                                              int lane, int wave)
+{
+    // a wave takes U 16-pixel tiles at a time (2U independent accumulators)
+    constexpr int SOUT = SIN - 2, NPX = SOUT * SOUT, NT = (NPX + 15) / 16;
+    const int q = lane >> 4, col = lane & 15;
+    for (int tile = wave; tile < NT; tile += 4 * U) {
+        int pc[U], y[U], x[U];
+        bool valid[U];
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < U; u++) {
             const int px = (tile + 4 * u) * 16 + col;
             valid[u] = px < NPX;
             pc[u] = valid[u] ? px : NPX - 1;
             y[u] = pc[u] / SOUT;
             x[u] = pc[u] - y[u] * SOUT;
         }
-        float4_t acc[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
-        // B operands row by row of the 3x3 kernel: the next kernel row's six loads are
-        // in flight while the current row's twelve products run
-        const uint8_t *base[2];
+        float4_t acc[U][2];
 #pragma unroll
-        for (int u = 0; u < 2; u++) base[u] = in + (y[u] * SIN + x[u]) * PSB + 16 * q;
-        u32x4 bv[2][3][2];
+        for (int u = 0; u < U; u++) acc[u][0] = acc[u][1] = float4_t{0, 0, 0, 0};
+        // B operands row by row of the 3x3 kernel: the next kernel row's loads are in
+        // flight while the current row's products run
+        const uint8_t *base[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) base[u] = in + (y[u] * SIN + x[u]) * PSB + 16 * q;
+        u32x4 bv[2][3][U];
 #pragma unroll
         for (int tx = 0; tx < 3; tx++)
 #pragma unroll
-            for (int u = 0; u < 2; u++) bv[0][tx][u] = *(const u32x4 *)(base[u] + tx * PSB);
+            for (int u = 0; u < U; u++) bv[0][tx][u] = *(const u32x4 *)(base[u] + tx * PSB);
 #pragma unroll
         for (int ty = 0; ty < 3; ty++) {
             if (ty < 2) {
 #pragma unroll
                 for (int tx = 0; tx < 3; tx++)
 #pragma unroll
-                    for (int u = 0; u < 2; u++)
+                    for (int u = 0; u < U; u++)
                         bv[(ty + 1) & 1][tx][u] = *(const u32x4 *)(base[u] + ((ty + 1) * SIN + tx) * PSB);
             }
 #pragma unroll
             for (int tx = 0; tx < 3; tx++)
 #pragma unroll
-                for (int u = 0; u < 2; u++) {
+                for (int u = 0; u < U; u++) {
                     acc[u][0] = mfma_bf16_k32(wa[3 * ty + tx][0], bv[ty & 1][tx][u], acc[u][0]);
                     acc[u][1] = mfma_bf16_k32(wa[3 * ty + tx][1], bv[ty & 1][tx][u], acc[u][1]);
                 }
         }
         // keep that order: loads of rows 0 and 1, products of row 0, loads of row 2,
         // products of rows 1 and 2 (mask 0x100 = LDS reads, 0x008 = MFMA)
-        __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 3 * U, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 3 * U, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 6 * U, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 3 * U, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 6 * U, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 6 * U, 0);
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < U; u++) {
             const int gy = oy - m + y[u], gx = ox - m + x[u];
             conv_store(out, pc[u], valid[u], gy >= 0 && gy < h8 && gx >= 0 && gx < w8, acc[u], b, q);
         }
@@ -565,6 +576,13 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     const int h8 = h4 / 2, w8 = w4 / 2, hc = h8 / 2, wc = w8 / 2;
     const int oy = ty * S4, ox = tx * S4; // tile origin at 1/8 resolution
 
+    // each 32 -> 32 layer's weights are loaded ahead of the barrier before the layer
+    // (the first one's here, under the window staging and layer 1)
+    u32x4 wa[9][2];
+    float bl[2][4];
+    load_w32(wf, lane, wa);
+    load_bias(p + OFF_B2, q, bl);
+
     // x0 window (in the a2/a4 buffer): 1/4-res rows/cols 2(o - 3) - 1 .. + SX - 1,
     // zero outside the image
     uint32_t *s_x = (uint32_t *)s_b;
@@ -611,11 +629,15 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
         }
     }
     __syncthreads();
-    conv32_window<S1>(s_a, s_b, wf, p + OFF_B2, oy, ox, 2, h8, w8, lane, wave);
+    conv32_window<S1>(s_a, s_b, wa, bl, oy, ox, 2, h8, w8, lane, wave);
+    load_w32(wf + 9 * C * C, lane, wa);
+    load_bias(p + OFF_B3, q, bl);
     __syncthreads();
-    conv32_window<S2>(s_b, s_a, wf + 9 * C * C, p + OFF_B3, oy, ox, 1, h8, w8, lane, wave);
+    conv32_window<S2>(s_b, s_a, wa, bl, oy, ox, 1, h8, w8, lane, wave);
+    load_w32(wf + 2 * 9 * C * C, lane, wa);
+    load_bias(p + OFF_B4, q, bl);
     __syncthreads();
-    conv32_window<S3>(s_a, s_b, wf + 2 * 9 * C * C, p + OFF_B4, oy, ox, 0, h8, w8, lane, wave);
+    conv32_window<S3>(s_a, s_b, wa, bl, oy, ox, 0, h8, w8, lane, wave);
     __syncthreads();
 
     // s_a is free once every wave has finished L4: stage the head's BF16 weights there
