@@ -27,9 +27,9 @@ HBM_PEAK_GBS = 8000.0
 # quarter rate: 64 per SIMD per cycle x 1024 SIMDs x 2.4 GHz
 VALU_PEAK_TOPS = 157.3
 # HBM bytes of one k_full_search<16,16> launch of this workload, from rocprofv3 --pmc
-# FETCH_SIZE (54,345 KiB, x2: a plain 512 MiB copy kernel reads 262,160 KiB) and
+# FETCH_SIZE (33,046 KiB, x2: a plain 512 MiB copy kernel reads 262,160 KiB) and
 # WRITE_SIZE (1,912 KiB, x1)
-FULL_SEARCH_TRAFFIC = (54345 * 2 + 1912) * 1024
+FULL_SEARCH_TRAFFIC = (33046 * 2 + 1912) * 1024
 
 
 def cpu_baseline(frames, lam, budget_s=12.0):
