@@ -2731,7 +2731,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RL <= 2 ? 3
 void k_cand_search_m(rh264_search_params p, const uint8_t *__restrict__ cur, size_t cur_stride,
                      const uint8_t *__restrict__ planes, size_t planes_stride, const int32_t *__restrict__ top8,
                      const int32_t *__restrict__ mvps, long nb, int32_t fb_cost, int32_t *__restrict__ start,
-                     int32_t *__restrict__ flag, unsigned *__restrict__ count, int32_t *__restrict__ list)
+                     int32_t *__restrict__ flag, unsigned *__restrict__ count, int32_t *__restrict__ list,
+                     uint64_t *__restrict__ fkeys)
 {
     constexpr int NC = 2 * RL + 1, NQ = NC / 4, REM = NC - 4 * NQ, BPW = 256 / LS_NCAND, NJ = 16 + 2 * RL;
     constexpr int NW = (NC + 15 + 3) / 4; // realigned window dwords per row (bytes 0 .. NC + 14)
@@ -2814,7 +2815,10 @@ void k_cand_search_m(rh264_search_params p, const uint8_t *__restrict__ cur, siz
             start[2 * g + 1] = ((ck / LS_BINS) - LS_BINS / 2) * 16 + 4 * (idx / NC - RL);
             const bool fb = (int64_t)(key >> 32) > (int64_t)fb_cost;
             flag[g] = fb ? 1 : 0;
-            if (fb) list[atomicAdd(count, 1u)] = (int32_t)g;
+            if (fb) {
+                fkeys[g] = ~0ull; // the fall-back search's argmin slot
+                list[atomicAdd(count, 1u)] = (int32_t)g;
+            }
         }
     }
 }
@@ -2832,30 +2836,34 @@ void k_full_search_list(rh264_search_params p, const uint8_t *__restrict__ cur, 
     __syncthreads();
     const int NG = fs_groups<V>(2 * p.range + 1);
     const long n = (long)*count;
-    const long gi = (long)blockIdx.x * 256 + threadIdx.x;
-    if ((long)blockIdx.x * 256 >= n * NG) return; // the whole workgroup is past the list
-    const long e = gi / NG;
-    const bool active = e < n;
     const int nbx = p.cw / BW, nblk = nbx * (p.ch / BH);
-    const long gb = list[active ? e : n - 1];
-    const int frame = (int)(gb / nblk), b = (int)(gb - (long)frame * nblk);
-    FsJob J;
-    J.cw = p.cw;
-    J.pw = p.cw + 2 * p.pad;
-    J.ph = p.ch + 2 * p.pad;
-    J.pad = p.pad;
-    J.R = p.range;
-    J.g = (int)(gi - e * NG);
-    J.lx = (b % nbx) * BW;
-    J.ly = (b / nbx) * BH;
-    J.cx = 0;
-    J.cy = 0;
-    J.mpx = mvps ? mvps[2 * gb] : 0;
-    J.mpy = mvps ? mvps[2 * gb + 1] : 0;
-    J.F = planes + (size_t)frame * planes_stride;
-    J.srcp = cur + (size_t)frame * cur_stride + (size_t)J.ly * p.cw + J.lx;
-    const uint64_t key = fs_key<BW, BH, V>(J, s_lr);
-    fs_commit(key, e, active, keys + gb);
+    // the grid covers every block; workgroups past the list's end (known only here) leave
+    {
+        const long chunk = blockIdx.x;
+        if (chunk * 256 >= n * NG) return;
+        const long gi = chunk * 256 + threadIdx.x;
+        const long e = gi / NG;
+        const bool active = e < n;
+        const long gb = list[active ? e : n - 1];
+        const int frame = (int)(gb / nblk), b = (int)(gb - (long)frame * nblk);
+        FsJob J;
+        J.cw = p.cw;
+        J.pw = p.cw + 2 * p.pad;
+        J.ph = p.ch + 2 * p.pad;
+        J.pad = p.pad;
+        J.R = p.range;
+        J.g = (int)(gi - e * NG);
+        J.lx = (b % nbx) * BW;
+        J.ly = (b / nbx) * BH;
+        J.cx = 0;
+        J.cy = 0;
+        J.mpx = mvps ? mvps[2 * gb] : 0;
+        J.mpy = mvps ? mvps[2 * gb + 1] : 0;
+        J.F = planes + (size_t)frame * planes_stride;
+        J.srcp = cur + (size_t)frame * cur_stride + (size_t)J.ly * p.cw + J.lx;
+        const uint64_t key = fs_key<BW, BH, V>(J, s_lr);
+        fs_commit(key, e, active, keys + gb);
+    }
 }
 
 // Fall-back blocks take the full search's vector as their start.
@@ -3904,13 +3912,12 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     if (ev) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, count, (size_t)1, 0u);
     if (local_range >= 1 && local_range <= 4 && ctx->cand_quad == 2) {
-        // one lane per candidate: quad-SAD column groups plus v_sad_u8 for the rest
-        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keysf,
-                           (size_t)nb, ~0ull);
+        // one lane per candidate: quad-SAD column groups plus v_sad_u8 for the rest; the
+        // kernel clears the fall-back blocks' argmin slots itself
 #define CM_LAUNCH(RL)                                                                                              \
     hipLaunchKernelGGL(k_cand_search_m<RL>, dim3((unsigned)((nb + 256 / LS_NCAND - 1) / (256 / LS_NCAND))), dim3(256), \
                        0, ctx->stream, *p, cur, frame_stride, planes, planes_stride, top8, (const int32_t *)nullptr,  \
-                       nb, fallback_cost, start, fallback, count, list)
+                       nb, fallback_cost, start, fallback, count, list, keysf)
         switch (local_range) {
         case 1: CM_LAUNCH(1); break;
         case 2: CM_LAUNCH(2); break;
@@ -3978,21 +3985,20 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     }
     // the fall-back list is short (a few % of the blocks): the v_sad_u8 variant with two
     // candidates per lane gives it twice the lanes of the quad-SAD one
-    if (ctx->fallback_v == 2) {
-        const long lanes = nb * fs_groups<3>(2 * p->range + 1);
-        hipLaunchKernelGGL((k_full_search_list<16, 16, 3>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
-                           ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,
-                           (const int32_t *)nullptr, keysf);
-    } else if (ctx->fallback_v == 1) {
-        const long lanes = nb * fs_groups<1>(2 * p->range + 1);
-        hipLaunchKernelGGL((k_full_search_list<16, 16, 1>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
-                           ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,
-                           (const int32_t *)nullptr, keysf);
-    } else {
-        const long lanes = nb * fs_groups<0>(2 * p->range + 1);
-        hipLaunchKernelGGL((k_full_search_list<16, 16, 0>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
-                           ctx->stream, *p, cur, frame_stride, planes, planes_stride, list, count,
-                           (const int32_t *)nullptr, keysf);
+    {
+        const int V = ctx->fallback_v == 2 ? 3 : ctx->fallback_v;
+        const long lanes = nb * (V == 3 ? fs_groups<3>(2 * p->range + 1)
+                                        : V == 1 ? fs_groups<1>(2 * p->range + 1) : fs_groups<0>(2 * p->range + 1));
+        const dim3 grid((unsigned)((lanes + 255) / 256));
+        if (V == 3)
+            hipLaunchKernelGGL((k_full_search_list<16, 16, 3>), grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride,
+                               planes, planes_stride, list, count, (const int32_t *)nullptr, keysf);
+        else if (V == 1)
+            hipLaunchKernelGGL((k_full_search_list<16, 16, 1>), grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride,
+                               planes, planes_stride, list, count, (const int32_t *)nullptr, keysf);
+        else
+            hipLaunchKernelGGL((k_full_search_list<16, 16, 0>), grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride,
+                               planes, planes_stride, list, count, (const int32_t *)nullptr, keysf);
     }
     hipLaunchKernelGGL(k_fallback_merge, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, list, count,
                        keysf, p->range, start);
