@@ -317,6 +317,20 @@ int rh264_mbtree_propagate_i32(rh264_ctx *ctx, int nframes, int mb_w, int mb_h, 
  * frames, held by the context. */
 int rh264_mbtree_costs_multi(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_w, int mb_h, int nrefs,
                              int range, int32_t *intra, int32_t *inter, int32_t *mv, int32_t *ref);
+/* The learned-seeded form of the above (build-defined; the 4K configuration's
+ * "learned proposal seeding the inter search").  frames: the nframes full-resolution
+ * cw x ch frames (stride frame_stride) that half was downsampled from.  For every
+ * reference k the network (net, created for at least nframes - 1 pairs of cw x ch)
+ * proposes 8 vectors per 16x16 MB of the pair (frame f, frame f - k), i.e. per 8x8
+ * half-res block; each is searched exhaustively over +-local_range half-res pels
+ * (1..4), the first minimum in proposal order (then raster order) winning; a best
+ * SAD above fallback_sad falls back to the exhaustive +-range search.  Costs,
+ * reference choice and outputs as rh264_mbtree_costs_multi; fallback[f][mb] = bit
+ * k-1 set when reference k fell back (device, may not be NULL). */
+int rh264_mbtree_costs_multi_learned(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *frames, size_t frame_stride,
+                                     const uint8_t *half, int nframes, int cw, int ch, int nrefs, int range,
+                                     int local_range, int32_t fallback_sad, int32_t *intra, int32_t *inter,
+                                     int32_t *mv, int32_t *ref, int32_t *fallback);
 /* rh264_mbtree_propagate_i32 with frame f's amount going to frame f - ref[f][mb]
  * (the ref output of rh264_mbtree_costs_multi); ref all 1 is the single-reference
  * propagation. */

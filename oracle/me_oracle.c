@@ -840,6 +840,70 @@ static int32_t la_intra8(const uint8_t *y, int cw, int bx0, int by0)
     return s < 1 ? 1 : (int32_t)s;
 }
 
+/* SAD of the 8x8 block at (bx0, by0) of cur against ref displaced by (dx, dy), reads
+ * clamped into the frame */
+static int64_t la_sad8(const uint8_t *cur, const uint8_t *ref, int cw, int ch, int bx0, int by0, int dx, int dy)
+{
+    int64_t sad = 0;
+    for (int j = 0; j < 8; j++)
+        for (int i = 0; i < 8; i++)
+            sad += iabs((int)cur[(size_t)(by0 + j) * cw + bx0 + i] - at(ref, cw, ch, bx0 + i + dx, by0 + j + dy));
+    return sad;
+}
+
+/* Exhaustive full-pel search of +-r around (cx, cy), raster first minimum */
+static int64_t la_window8(const uint8_t *cur, const uint8_t *ref, int cw, int ch, int bx0, int by0, int cx, int cy,
+                          int r, int *wx, int *wy)
+{
+    int64_t bs = INT64_MAX;
+    for (int dy = -r; dy <= r; dy++)
+        for (int dx = -r; dx <= r; dx++) {
+            int64_t sad = la_sad8(cur, ref, cw, ch, bx0, by0, cx + dx, cy + dy);
+            if (sad < bs) { bs = sad; *wx = cx + dx; *wy = cy + dy; }
+        }
+    return bs;
+}
+
+/* The learned-seeded form (build-defined): per reference k the 8 centres
+ * cen[(k-1)*16 + 2c .. +1] (half-res full pels, the network's proposals in rank order)
+ * are each searched over +-local_range; the first minimum in candidate order (then in
+ * raster order) wins; a best SAD above fb_sad falls back to the exhaustive +-range
+ * search (bit k-1 of *fb_mask).  Then as orc_la_block_multi. */
+int64_t orc_la_block_learned(const uint8_t *half, int cw, int ch, int f, int nrefs, int range, int local_range,
+                             int fb_sad, const int32_t *cen, int bx0, int by0, int32_t *mv_out, int32_t *ref_out,
+                             int32_t *fb_mask)
+{
+    const size_t fs = (size_t)cw * ch;
+    const uint8_t *cur = half + (size_t)f * fs;
+    int64_t best = INT64_MAX;
+    int bmx = 0, bmy = 0, bk = 0;
+    *fb_mask = 0;
+    for (int k = 1; k <= nrefs && k <= f; k++) {
+        const uint8_t *ref = half + (size_t)(f - k) * fs;
+        int64_t bs = INT64_MAX;
+        int wx = 0, wy = 0;
+        for (int c = 0; c < 8; c++) {
+            int cx2 = 0, cy2 = 0;
+            int64_t s2 = la_window8(cur, ref, cw, ch, bx0, by0, cen[(k - 1) * 16 + 2 * c], cen[(k - 1) * 16 + 2 * c + 1],
+                                    local_range, &cx2, &cy2);
+            if (s2 < bs) { bs = s2; wx = cx2; wy = cy2; }
+        }
+        if (bs > fb_sad) {
+            la_window8(cur, ref, cw, ch, bx0, by0, 0, 0, range, &wx, &wy);
+            *fb_mask |= 1 << (k - 1);
+        }
+        uint8_t pred[64];
+        for (int j = 0; j < 8; j++)
+            for (int i = 0; i < 8; i++) pred[j * 8 + i] = (uint8_t)at(ref, cw, ch, bx0 + i + wx, by0 + j + wy);
+        int64_t c = orc_satd8_px(cur + (size_t)by0 * cw + bx0, cw, pred, 8, 8, 8);
+        if (c < best) { best = c; bmx = wx; bmy = wy; bk = k; }
+    }
+    mv_out[0] = 8 * bmx;
+    mv_out[1] = 8 * bmy;
+    *ref_out = bk;
+    return best;
+}
+
 int64_t orc_la_block_multi(const uint8_t *half, int cw, int ch, int f, int nrefs, int range, int bx0, int by0,
                            int32_t *mv_out, int32_t *ref_out)
 {
@@ -849,17 +913,8 @@ int64_t orc_la_block_multi(const uint8_t *half, int cw, int ch, int f, int nrefs
     int bmx = 0, bmy = 0, bk = 0;
     for (int k = 1; k <= nrefs && k <= f; k++) {
         const uint8_t *ref = half + (size_t)(f - k) * fs;
-        int64_t bs = INT64_MAX;
         int wx = 0, wy = 0;
-        for (int dy = -range; dy <= range; dy++)
-            for (int dx = -range; dx <= range; dx++) {
-                int64_t sad = 0;
-                for (int j = 0; j < 8; j++)
-                    for (int i = 0; i < 8; i++)
-                        sad += iabs((int)cur[(size_t)(by0 + j) * cw + bx0 + i] -
-                                    at(ref, cw, ch, bx0 + i + dx, by0 + j + dy));
-                if (sad < bs) { bs = sad; wx = dx; wy = dy; }
-            }
+        la_window8(cur, ref, cw, ch, bx0, by0, 0, 0, range, &wx, &wy);
         uint8_t pred[64];
         for (int j = 0; j < 8; j++)
             for (int i = 0; i < 8; i++) pred[j * 8 + i] = (uint8_t)at(ref, cw, ch, bx0 + i + wx, by0 + j + wy);

@@ -154,6 +154,13 @@ double orc_rc_complexity(const uint8_t *sy, const uint8_t *ref, int cw, int ch);
  * inter = min(that, intra), mv = 8 * winner (full-res quarter pel), ref = k. */
 int64_t orc_la_block_multi(const uint8_t *half, int cw, int ch, int f, int nrefs, int range, int bx0, int by0,
                            int32_t *mv_out, int32_t *ref_out);
+/* Learned-seeded form: per reference k the 8 centres cen[(k-1)*16 + 2c .. +1] (half-res
+ * full pels) are searched over +-local_range (first minimum in candidate order, then
+ * raster order); a best SAD above fb_sad falls back to the exhaustive +-range search
+ * (bit k-1 of *fb_mask); then the 8x8 SATD and reference choice as above. */
+int64_t orc_la_block_learned(const uint8_t *half, int cw, int ch, int f, int nrefs, int range, int local_range,
+                             int fb_sad, const int32_t *cen, int bx0, int by0, int32_t *mv_out, int32_t *ref_out,
+                             int32_t *fb_mask);
 void orc_mbtree_costs_multi(const uint8_t *half, int nframes, int mb_w, int mb_h, int nrefs, int range,
                             int32_t *intra, int32_t *inter, int32_t *mv, int32_t *ref, int nthreads);
 /* int32 propagation with frame f's amount going to frame f - ref[f][mb]

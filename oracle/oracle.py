@@ -65,7 +65,8 @@ class Oracle:
         for n in ("orc_sad", "orc_satd4x4_wels", "orc_satd_wels", "orc_intra_cost"):
             getattr(L, n).restype = C.c_int32
         for n in ("orc_satd_px", "orc_full_search_block", "orc_satd_cost", "orc_subpel_refine_block",
-                  "orc_motion_search_ref", "orc_mbtree_mc_satd", "orc_satd8_px", "orc_la_block_multi"):
+                  "orc_motion_search_ref", "orc_mbtree_mc_satd", "orc_satd8_px", "orc_la_block_multi",
+                  "orc_la_block_learned"):
             getattr(L, n).restype = C.c_int64
         L.orc_satd_avg.restype = C.c_uint32
         L.orc_mvbits.restype = C.c_uint32
@@ -224,6 +225,16 @@ class Oracle:
         c = self.lib.orc_la_block_multi(_p(_u8(halves)), int(cw), int(ch), int(f), int(nrefs), int(search_range),
                                         int(bx0), int(by0), _p(mv, i32p), C.byref(k))
         return int(c), (int(mv[0]), int(mv[1])), k.value
+
+    def la_block_learned(self, halves, cw, ch, f, nrefs, search_range, local_range, fb_sad, centres, bx0, by0):
+        """centres: [nrefs][8][2] half-res full-pel proposals in rank order."""
+        mv = np.empty(2, np.int32)
+        k, fb = C.c_int32(), C.c_int32()
+        cen = np.ascontiguousarray(centres, np.int32).reshape(-1)
+        c = self.lib.orc_la_block_learned(_p(_u8(halves)), int(cw), int(ch), int(f), int(nrefs), int(search_range),
+                                          int(local_range), int(fb_sad), _p(cen, i32p), int(bx0), int(by0),
+                                          _p(mv, i32p), C.byref(k), C.byref(fb))
+        return int(c), (int(mv[0]), int(mv[1])), k.value, fb.value
 
     def propagate_multi_i32(self, intra, inter, mv, ref, mb_w, mb_h):
         n = intra.shape[0]

@@ -69,6 +69,9 @@ PROTOTYPES = {
     "rh264_mbtree_propagate_i32": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp]),
     "rh264_mbtree_costs_multi": (C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp,
                                            c_vp]),
+    "rh264_mbtree_costs_multi_learned": (C.c_int, [c_vp, c_vp, c_vp, C.c_size_t, c_vp, C.c_int, C.c_int, C.c_int,
+                                                   C.c_int, C.c_int, C.c_int, C.c_int32, c_vp, c_vp, c_vp, c_vp,
+                                                   c_vp]),
     "rh264_mbtree_propagate_multi_i32": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "rh264_rc_complexity": (C.c_int, [c_vp, c_vp, c_vp, C.c_size_t, C.c_int, C.c_int, C.c_int, c_vp]),
     "rh264_batch_out_len": (C.c_size_t, [C.c_int, C.c_int, c_i32p, C.c_int]),

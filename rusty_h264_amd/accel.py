@@ -347,6 +347,28 @@ class MbTree:
                                                           self.inter.ptr, self.mv.ptr, self.ref.ptr, self.prop.ptr),
                    "rh264_mbtree_propagate_multi_i32")
 
+    def run_multi_learned(self, frames_buf, net, nrefs=3, search_range=64, local_range=2, fallback_sad=512,
+                          frame_stride=None, propagate=True):
+        """The learned-seeded multi-reference form (rh264_mbtree_costs_multi_learned):
+        net is an mvnet.MVNet created for at least nframes - 1 pairs of cw x ch."""
+        d = self.dev
+        nm = self.mb_w * self.mb_h
+        if not hasattr(self, "ref"):
+            self.ref = d.alloc(4 * nm * self.n)
+        if not hasattr(self, "fb"):
+            self.fb = d.alloc(4 * nm * self.n)
+        stride = frame_stride or self.cw * self.ch
+        _check(d.lib.rh264_mbtree_downsample(d.ctx, int(frames_buf), stride, self.n, self.cw, self.ch, self.half.ptr),
+               "rh264_mbtree_downsample")
+        _check(d.lib.rh264_mbtree_costs_multi_learned(d.ctx, net.h, int(frames_buf), stride, self.half.ptr, self.n,
+                                                      self.cw, self.ch, nrefs, search_range, local_range,
+                                                      int(fallback_sad), self.intra.ptr, self.inter.ptr, self.mv.ptr,
+                                                      self.ref.ptr, self.fb.ptr), "rh264_mbtree_costs_multi_learned")
+        if propagate:
+            _check(d.lib.rh264_mbtree_propagate_multi_i32(d.ctx, self.n, self.mb_w, self.mb_h, self.intra.ptr,
+                                                          self.inter.ptr, self.mv.ptr, self.ref.ptr, self.prop.ptr),
+                   "rh264_mbtree_propagate_multi_i32")
+
     def results(self):
         d, n, nm = self.dev, self.n, self.mb_w * self.mb_h
         r = {"half": d.download(self.half, (n, self.ch // 2, self.cw // 2), np.uint8),
@@ -356,6 +378,8 @@ class MbTree:
              "prop": d.download(self.prop, (n, nm), np.int32)}
         if hasattr(self, "ref"):
             r["ref"] = d.download(self.ref, (n, nm), np.int32)
+        if hasattr(self, "fb"):
+            r["fb"] = d.download(self.fb, (n, nm), np.int32)
         return r
 
 

@@ -48,4 +48,6 @@ struct rh264_ctx {
     size_t ls_cap;
     void *la_buf;                 // padded half-res planes of the multi-reference lookahead
     size_t la_cap;
+    void *lal_buf;                // proposals, vectors and fall-back list of its learned-seeded form
+    size_t lal_cap;
 };

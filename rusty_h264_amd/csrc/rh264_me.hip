@@ -22,6 +22,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <utility>
 #include <string>
@@ -2282,6 +2283,10 @@ __global__ void k_mbtree_prop_i32(int f, int mb_w, int mb_h, const int32_t *__re
     }
 }
 
+// The network's proposals: 8 per block, bins of a 17 x 17 grid of 4-pel steps
+constexpr int LS_NCAND = 8;
+constexpr int LS_BINS = 17;
+
 // ---------------------------------------------------------------------------
 // Multi-reference lookahead (build-defined: the 4K configuration's 3 refs, +-64;
 // the reference's inter_cost walks a seeded diamond against one reference).
@@ -2341,11 +2346,15 @@ void k_la_search(const uint8_t *__restrict__ half, int mb_w, int mb_h, int nrefs
 
 // One thread per (frame, block): intra = max(1, AC sum|H8 x H8| >> 1), and for
 // frames > 0 the cheapest reference's 8x8 SATD (ties to the nearest reference).
+// The winners come from the exhaustive search's keys, or (learned-seeded form) as
+// vectors wmv[((f-1)*nrefs + k-1)*nm + m] with fall-back flags wfb at the same index.
 __global__ __launch_bounds__(256) void k_la_cost(const uint8_t *__restrict__ half, int mb_w, int mb_h, int nframes,
                                                  int nrefs, int R, int P, const uint8_t *__restrict__ planes,
                                                  size_t pstride, const uint64_t *__restrict__ keys,
+                                                 const int32_t *__restrict__ wmv, const int32_t *__restrict__ wfb,
                                                  int32_t *__restrict__ intra, int32_t *__restrict__ inter,
-                                                 int32_t *__restrict__ mv, int32_t *__restrict__ ref)
+                                                 int32_t *__restrict__ mv, int32_t *__restrict__ ref,
+                                                 int32_t *__restrict__ fbmask)
 {
     const int nm = mb_w * mb_h;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2369,11 +2378,20 @@ __global__ __launch_bounds__(256) void k_la_cost(const uint8_t *__restrict__ hal
     }
     // the DC coefficient of H8 x H8 is the block sum; the AC part is the rest
     const int ic = max(1, (satd8x8_sum_abs(d) - dc) >> 1);
-    int best = 0x7fffffff, bdx = 0, bdy = 0, bk = 0;
+    int best = 0x7fffffff, bdx = 0, bdy = 0, bk = 0, fbm = 0;
     const int NC = 2 * R + 1;
     for (int k = 1; k <= nrefs && k <= f; k++) {
-        const uint32_t idx = (uint32_t)keys[((size_t)(f - 1) * nrefs + (k - 1)) * nm + m];
-        const int dy = (int)(idx / (uint32_t)NC) - R, dx = (int)(idx % (uint32_t)NC) - R;
+        const size_t w = ((size_t)(f - 1) * nrefs + (k - 1)) * nm + m;
+        int dx, dy;
+        if (wmv) {
+            dx = wmv[2 * w];
+            dy = wmv[2 * w + 1];
+            fbm |= wfb[w] << (k - 1);
+        } else {
+            const uint32_t idx = (uint32_t)keys[w];
+            dy = (int)(idx / (uint32_t)NC) - R;
+            dx = (int)(idx % (uint32_t)NC) - R;
+        }
         const uint8_t *pl = planes + (size_t)(f - k) * pstride + (size_t)(P + by0 + dy) * pw + P + bx0 + dx;
 #pragma unroll
         for (int j = 0; j < 8; j++)
@@ -2393,14 +2411,181 @@ __global__ __launch_bounds__(256) void k_la_cost(const uint8_t *__restrict__ hal
     mv[2 * i] = 8 * bdx;
     mv[2 * i + 1] = 8 * bdy;
     ref[i] = bk;
+    if (fbmask) fbmask[i] = fbm;
+}
+
+// Learned-seeded lookahead: one lane per (block, proposal) searches the proposal's
+// +-RL window of one reference exhaustively (8x8 SADs: a quad-SAD group for columns
+// 0..3, v_sad_u8 for the rest, as k_cand_search_m); the block's winner is the first
+// minimum in proposal order, then raster order.  A block whose best SAD exceeds
+// fb_sad is appended to the fall-back list (entry = its (frame, reference, block)
+// index) and its argmin slot cleared.  grid.y = (frame - 1) * nrefs + reference - 1.
+template <int RL>
+__global__ __launch_bounds__(256) void k_la_cand(const uint8_t *__restrict__ half, int mb_w, int mb_h, int nframes,
+                                                 int nrefs, int P, const uint8_t *__restrict__ planes, size_t pstride,
+                                                 const int32_t *__restrict__ top8, int fb_sad,
+                                                 int32_t *__restrict__ wmv, int32_t *__restrict__ wfb,
+                                                 unsigned *__restrict__ count, int32_t *__restrict__ list,
+                                                 uint64_t *__restrict__ lkeys)
+{
+    constexpr int NC = 2 * RL + 1, NQ = NC / 4, REM = NC - 4 * NQ, BPW = 256 / LS_NCAND, NJ = 8 + 2 * RL;
+    constexpr int NW = (NC + 7 + 3) / 4; // realigned window dwords per row (bytes 0 .. NC + 6)
+    __shared__ unsigned long long s_key[BPW];
+    if (threadIdx.x < BPW) s_key[threadIdx.x] = ~0ull;
+    __syncthreads();
+    unsigned wgx, wgy;
+    xcd_grid2(wgx, wgy);
+    const int f = 1 + (int)wgy / nrefs, k = 1 + (int)wgy % nrefs;
+    if (k > f) return; // no such reference (the whole workgroup)
+    const int hw = mb_w * 8, hh = mb_h * 8, nm = mb_w * mb_h, pw = hw + 2 * P;
+    const int slot = threadIdx.x / LS_NCAND, c = threadIdx.x - slot * LS_NCAND;
+    const int m = (int)wgx * BPW + slot;
+    // proposals of (frame f, reference f - k): the network's run over the pairs (j + k, j)
+    const int32_t *t8 = top8 + (((size_t)(k - 1) * nframes + (f - k)) * nm) * LS_NCAND;
+    if (m < nm) {
+        const int bx0 = (m % mb_w) * 8, by0 = (m / mb_w) * 8;
+        const int bin = t8[(size_t)m * LS_NCAND + c];
+        // bin -> full-resolution vector ((bin % 17) - 8, (bin / 17) - 8) * 4 pels = *2 half-res pels
+        const int cx = ((bin % LS_BINS) - LS_BINS / 2) * 2, cy = ((bin / LS_BINS) - LS_BINS / 2) * 2;
+        const uint8_t *src = half + (size_t)f * hw * hh + (size_t)by0 * hw + bx0;
+        uint32_t sr[8][2];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            sr[r][0] = *(const uint32_t *)(src + (size_t)r * hw);
+            sr[r][1] = *(const uint32_t *)(src + (size_t)r * hw + 4);
+        }
+        const uint8_t *F = planes + (size_t)(f - k) * pstride;
+        // |cx|, |cy| <= 16 and P >= 16 + RL + 8: the window lies inside the padded plane
+        const int x = P + bx0 + cx - RL, y0 = P + by0 + cy - RL;
+        const int sh = x & 3;
+        uint64_t aq[NQ > 0 ? NQ : 1][NC];
+        uint32_t ar[REM > 0 ? REM : 1][NC];
+#pragma unroll
+        for (int t = 0; t < NC; t++) {
+#pragma unroll
+            for (int g = 0; g < NQ; g++) aq[g][t] = 0;
+#pragma unroll
+            for (int q = 0; q < REM; q++) ar[q][t] = 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const uint32_t *rp = (const uint32_t *)(F + (size_t)(y0 + j) * pw + (x & ~3));
+            uint32_t a[NW + 1], w[NW];
+#pragma unroll
+            for (int i = 0; i < NW + 1; i++) a[i] = rp[i];
+#pragma unroll
+            for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(a[i + 1], a[i], (uint32_t)sh);
+#pragma unroll
+            for (int t = 0; t < NC; t++) {
+                const int r = j - t;
+                if (r < 0 || r >= 8) continue;
+#pragma unroll
+                for (int g = 0; g < NQ; g++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+                        aq[g][t] = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)w[g + i + 1] << 32) | w[g + i], sr[r][i],
+                                                                   aq[g][t]);
+#pragma unroll
+                for (int q = 0; q < REM; q++) {
+                    const int col = 4 * NQ + q;
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        const uint32_t ref4 = (col & 3) == 0 ? w[(col >> 2) + i]
+                                                             : __builtin_amdgcn_alignbyte(w[(col >> 2) + i + 1],
+                                                                                          w[(col >> 2) + i],
+                                                                                          (uint32_t)(col & 3));
+                        ar[q][t] = __builtin_amdgcn_sad_u8(sr[r][i], ref4, ar[q][t]);
+                    }
+                }
+            }
+        }
+        uint64_t best = ~0ull;
+#pragma unroll
+        for (int t = 0; t < NC; t++)
+#pragma unroll
+            for (int col = 0; col < NC; col++) {
+                const uint32_t sad = col < 4 * NQ ? (uint32_t)(aq[col >> 2][t] >> (16 * (col & 3))) & 0xffffu
+                                                  : ar[col - 4 * NQ][t];
+                const uint64_t key = ((uint64_t)sad << 32) | ((uint64_t)c << 8) | (uint64_t)(t * NC + col);
+                best = key < best ? key : best;
+            }
+        atomicMin(&s_key[slot], (unsigned long long)best);
+    }
+    __syncthreads();
+    if (threadIdx.x < BPW) {
+        const int mm = (int)wgx * BPW + threadIdx.x;
+        if (mm < nm) {
+            const uint64_t key = s_key[threadIdx.x];
+            const int kk = (int)((key >> 8) & 255), idx = (int)(key & 255);
+            const int bin = t8[(size_t)mm * LS_NCAND + kk];
+            const size_t w = ((size_t)(f - 1) * nrefs + (k - 1)) * nm + mm;
+            wmv[2 * w] = ((bin % LS_BINS) - LS_BINS / 2) * 2 + idx % NC - RL;
+            wmv[2 * w + 1] = ((bin / LS_BINS) - LS_BINS / 2) * 2 + idx / NC - RL;
+            const bool fb = (int64_t)(key >> 32) > (int64_t)fb_sad;
+            wfb[w] = fb ? 1 : 0;
+            if (fb) {
+                const unsigned e = atomicAdd(count, 1u);
+                list[e] = (int32_t)w;
+                lkeys[e] = ~0ull;
+            }
+        }
+    }
+}
+
+// Exhaustive +-R search of the fall-back entries (w = ((f-1)*nrefs + k-1)*nm + m): the
+// full search's lane ring at zero rate; the grid strides over the list, whose length
+// only the device knows.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FS_WAVES)))
+void k_la_search_list(const uint8_t *__restrict__ half, int mb_w, int mb_h, int nrefs, int R, int P,
+                      const uint8_t *__restrict__ planes, size_t pstride, const int32_t *__restrict__ list,
+                      const unsigned *__restrict__ count, uint64_t *__restrict__ lkeys)
+{
+    __shared__ int32_t s_lr[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = 0;
+    __syncthreads();
+    const int hw = mb_w * 8, hh = mb_h * 8, nm = mb_w * mb_h;
+    const int NG = fs_groups<0>(2 * R + 1);
+    const long n = (long)*count;
+    for (long chunk = blockIdx.x; chunk * 256 < n * NG; chunk += gridDim.x) {
+        const long gi = chunk * 256 + threadIdx.x;
+        const long e = gi / NG;
+        const bool active = e < n;
+        const long w = list[active ? e : n - 1];
+        const int m = (int)(w % nm), fk = (int)(w / nm), f = 1 + fk / nrefs, k = 1 + fk % nrefs;
+        FsJob J;
+        J.cw = hw;
+        J.pw = hw + 2 * P;
+        J.ph = hh + 2 * P;
+        J.pad = P;
+        J.R = R;
+        J.g = (int)(gi - e * NG);
+        J.lx = (m % mb_w) * 8;
+        J.ly = (m / mb_w) * 8;
+        J.cx = J.cy = J.mpx = J.mpy = 0;
+        J.F = planes + (size_t)(f - k) * pstride;
+        J.srcp = half + (size_t)f * hw * hh + (size_t)J.ly * hw + J.lx;
+        const uint64_t key = fs_key<8, 8, 0>(J, s_lr);
+        fs_commit(key, e, active, lkeys + (active ? e : n - 1));
+    }
+}
+
+// The fall-back entries take the exhaustive search's winner.
+__global__ void k_la_fb_merge(const int32_t *__restrict__ list, const unsigned *__restrict__ count,
+                              const uint64_t *__restrict__ lkeys, int R, int32_t *__restrict__ wmv)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)*count) return;
+    const long w = list[e];
+    const uint32_t idx = (uint32_t)lkeys[e];
+    const int NC = 2 * R + 1;
+    wmv[2 * w] = (int)(idx % (uint32_t)NC) - R;
+    wmv[2 * w + 1] = (int)(idx / (uint32_t)NC) - R;
 }
 
 // ---------------------------------------------------------------------------
 // learned-proposal search (the network lives in rh264_mvnet.hip)
 // ---------------------------------------------------------------------------
 
-constexpr int LS_NCAND = 8;
-constexpr int LS_BINS = 17;
 
 // Candidate centres from the network's top bins: bin c -> ((c%17)-8, (c/17)-8) * 4
 // pels, as quarter-pel MVs.  centres: [k][block][2].
@@ -2906,6 +3091,8 @@ int rh264_init(int device, rh264_ctx **out)
     c->ls_cap = 0;
     c->la_buf = nullptr;
     c->la_cap = 0;
+    c->lal_buf = nullptr;
+    c->lal_cap = 0;
     c->fs_kernel = 0;
     c->refine_lds = 1;
     c->refine_mfma = 1;
@@ -2932,6 +3119,7 @@ int rh264_destroy(rh264_ctx *ctx)
     if (ctx->tmp_mv) (void)hipFree(ctx->tmp_mv);
     if (ctx->ls_buf) (void)hipFree(ctx->ls_buf);
     if (ctx->la_buf) (void)hipFree(ctx->la_buf);
+    if (ctx->lal_buf) (void)hipFree(ctx->lal_buf);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
     for (hipEvent_t e : ctx->prof) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -3506,7 +3694,78 @@ int rh264_mbtree_costs_multi(rh264_ctx *ctx, const uint8_t *half, int nframes, i
     }
     const long nb = (long)nframes * nm;
     hipLaunchKernelGGL(k_la_cost, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, half, mb_w, mb_h,
-                       nframes, nrefs, range, P, planes, pstride, ctx->keys, intra, inter, mv, ref);
+                       nframes, nrefs, range, P, planes, pstride, ctx->keys, (const int32_t *)nullptr,
+                       (const int32_t *)nullptr, intra, inter, mv, ref, (int32_t *)nullptr);
+    LAUNCH_CHECK("k_la_cost");
+    return 0;
+}
+
+int rh264_mbtree_costs_multi_learned(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *frames, size_t frame_stride,
+                                     const uint8_t *half, int nframes, int cw, int ch, int nrefs, int range,
+                                     int local_range, int32_t fallback_sad, int32_t *intra, int32_t *inter,
+                                     int32_t *mv, int32_t *ref, int32_t *fallback)
+{
+    if (!net) return fail("mbtree_costs_multi_learned: null network");
+    if (cw <= 0 || ch <= 0 || cw % 16 || ch % 16) return RH264_DECLINED;
+    if (nrefs < 1 || nrefs > 4 || range < 1 || range > 64 || local_range < 1 || local_range > 4)
+        return RH264_DECLINED;
+    if (nframes <= 0) return 0;
+    if (nframes > 16384) return fail("mbtree_costs_multi_learned: nframes %d > 16384", nframes);
+    const int mb_w = cw / 16, mb_h = ch / 16, hw = mb_w * 8, hh = mb_h * 8, nm = mb_w * mb_h;
+    // pad: the exhaustive window, and every proposal (|v| <= 16 half-res pels) with its
+    // +-local_range window
+    const int P = (std::max(range, 16 + local_range) + 8 + 3) & ~3;
+    const int pw = hw + 2 * P, ph = hh + 2 * P;
+    const size_t pstride = (size_t)pw * ph;
+    int rc = ensure_buf(ctx, &ctx->la_buf, &ctx->la_cap, pstride * nframes + 256);
+    if (rc) return rc;
+    uint8_t *planes = (uint8_t *)ctx->la_buf;
+    hipLaunchKernelGGL(k_la_pad, dim3((unsigned)((pstride / 4 + 255) / 256), nframes), dim3(256), 0, ctx->stream, half,
+                       hw, hh, P, planes, pstride);
+    LAUNCH_CHECK("k_la_pad");
+    const size_t nw = (size_t)std::max(nframes - 1, 1) * nrefs * nm; // (frame, reference, block) entries
+    const size_t o_t8 = 0, o_mv = o_t8 + (size_t)nrefs * nframes * nm * LS_NCAND * 4;
+    const size_t o_fb = o_mv + nw * 8, o_list = o_fb + nw * 4, o_keys = (o_list + nw * 4 + 15) & ~(size_t)15;
+    const size_t o_cnt = o_keys + nw * 8, total = o_cnt + 256;
+    rc = ensure_buf(ctx, &ctx->lal_buf, &ctx->lal_cap, total);
+    if (rc) return rc;
+    uint8_t *base = (uint8_t *)ctx->lal_buf;
+    int32_t *top8 = (int32_t *)(base + o_t8), *wmv = (int32_t *)(base + o_mv), *wfb = (int32_t *)(base + o_fb);
+    int32_t *list = (int32_t *)(base + o_list);
+    uint64_t *lkeys = (uint64_t *)(base + o_keys);
+    unsigned *count = (unsigned *)(base + o_cnt);
+    if (nframes > 1) {
+        // proposals for every (frame, reference) pair: the network over the pairs (j + k, j)
+        for (int k = 1; k <= nrefs && k < nframes; k++) {
+            rc = rh264_mvnet_forward(ctx, net, frames + (size_t)k * frame_stride, frames, frame_stride, nframes - k,
+                                     cw, ch, nullptr, top8 + (size_t)(k - 1) * nframes * nm * LS_NCAND);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, count, (size_t)1, 0u);
+        const dim3 grid((unsigned)((nm + 256 / LS_NCAND - 1) / (256 / LS_NCAND)), (unsigned)((nframes - 1) * nrefs));
+#define LC_LAUNCH(RL)                                                                                             \
+    hipLaunchKernelGGL(k_la_cand<RL>, grid, dim3(256), 0, ctx->stream, half, mb_w, mb_h, nframes, nrefs, P, planes, \
+                       pstride, top8, (int)fallback_sad, wmv, wfb, count, list, lkeys)
+        switch (local_range) {
+        case 1: LC_LAUNCH(1); break;
+        case 2: LC_LAUNCH(2); break;
+        case 3: LC_LAUNCH(3); break;
+        default: LC_LAUNCH(4); break;
+        }
+#undef LC_LAUNCH
+        LAUNCH_CHECK("k_la_cand");
+        const long lanes = (long)nw * fs_groups<0>(2 * range + 1);
+        hipLaunchKernelGGL(k_la_search_list, dim3((unsigned)std::min<long>((lanes + 255) / 256, 2048)), dim3(256), 0,
+                           ctx->stream, half, mb_w, mb_h, nrefs, range, P, planes, pstride, list, count, lkeys);
+        LAUNCH_CHECK("k_la_search_list");
+        hipLaunchKernelGGL(k_la_fb_merge, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, ctx->stream, list, count,
+                           lkeys, range, wmv);
+        LAUNCH_CHECK("k_la_fb_merge");
+    }
+    const long nb = (long)nframes * nm;
+    hipLaunchKernelGGL(k_la_cost, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, half, mb_w, mb_h,
+                       nframes, nrefs, range, P, planes, pstride, (const uint64_t *)nullptr, wmv, wfb, intra, inter,
+                       mv, ref, fallback);
     LAUNCH_CHECK("k_la_cost");
     return 0;
 }

@@ -126,3 +126,51 @@ def test_mbtree_multi_reference_bit_exact(dev, cw, ch, n, nrefs, R, kind):
         assert bad.size == 0, f"frame {f}: {bad.size} MBs differ, first {bad[:5]}"
     assert np.array_equal(got["prop"], prop)
     fb.free()
+
+
+@pytest.mark.parametrize("cw,ch,n,nrefs,R,Rl,fb", [(128, 96, 5, 3, 16, 2, 300), (176, 144, 4, 2, 24, 1, 150),
+                                                   (64, 48, 4, 3, 8, 4, -1), (352, 288, 3, 3, 64, 2, 400)])
+def test_mbtree_multi_learned_bit_exact(dev, cw, ch, n, nrefs, R, Rl, fb):
+    """The learned-seeded multi-reference form: the network's proposals for every
+    (frame, reference) pair (taken from the GPU network itself), the oracle's proposal
+    windows, fall-back and reference choice; costs, vectors, references, fall-back
+    masks and the propagation bit-exact.  fb = -1 sends every block to the fall-back."""
+    from rusty_h264_amd import mvnet
+    frames = synth.hd_clip(n, seed=cw + n, w=cw, h=ch, nrects=3)
+    mb_w, mb_h = cw // 16, ch // 16
+    nm = mb_w * mb_h
+    fbuf = dev.upload(frames)
+    net = mvnet.MVNet(dev, n - 1, cw, ch, mvnet.load_weights())
+    try:
+        mt = accel.MbTree(dev, cw, ch, n)
+        mt.run_multi_learned(fbuf, net, nrefs=nrefs, search_range=R, local_range=Rl, fallback_sad=fb)
+        got = mt.results()
+        # proposals per (frame f, reference k): the network over the pairs (j + k, j)
+        cen = np.zeros((n, nrefs, nm, 8, 2), np.int32)
+        for k in range(1, min(nrefs, n - 1) + 1):
+            t8 = dev.alloc(4 * (n - k) * nm * 8)
+            net.forward(int(fbuf) + k * cw * ch, int(fbuf), cw * ch, n - k, cw, ch, None, t8)
+            top = dev.download(t8, (n - k, nm, 8), np.int32)
+            t8.free()
+            cen[k:, k - 1, :, :, 0] = (top % 17 - 8) * 2
+            cen[k:, k - 1, :, :, 1] = (top // 17 - 8) * 2
+    finally:
+        net.close()
+    halves = got["half"]
+    intra0, _, _, _ = O.mbtree_costs_multi(halves[:1], mb_w, mb_h, nrefs, R)
+    assert np.array_equal(got["intra"][0], intra0[0]) and np.array_equal(got["inter"][0], intra0[0])
+    fbs = 0
+    for f in range(1, n):
+        for m in range(nm):
+            c, mv, k, mask = O.la_block_learned(halves, cw // 2, ch // 2, f, nrefs, R, Rl, fb, cen[f, :, m],
+                                                (m % mb_w) * 8, (m // mb_w) * 8)
+            ic = int(got["intra"][f, m])
+            want = (min(c, ic), mv, k, mask)
+            have = (int(got["inter"][f, m]), tuple(got["mv"][f, m]), int(got["ref"][f, m]), int(got["fb"][f, m]))
+            assert have == want, (f, m, have, want)
+            fbs += mask != 0
+    if fb < 0:
+        assert fbs == (n - 1) * nm
+    assert np.array_equal(got["prop"], O.propagate_multi_i32(got["intra"], got["inter"], got["mv"], got["ref"],
+                                                             mb_w, mb_h))
+    fbuf.free()

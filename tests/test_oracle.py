@@ -593,3 +593,63 @@ def test_propagate_multi_reduces_to_single_and_conserves():
         np.add.at(sent, f - ref[f], np.maximum(amount, 0))
     recv = p.astype(np.int64).sum(1)
     assert (recv <= sent).all() and (sent - recv <= 4 * nm * 3).all() and sent[:-1].sum() > 0
+
+
+def _la_learned_textbook(halves, f, nrefs, R, Rl, fb, cen, bx0, by0):
+    """Candidate windows by numpy brute force (first minimum in candidate order, then
+    raster), exhaustive fall-back above fb, Kronecker H8 cost, nearest reference on ties."""
+    n, ch, cw = halves.shape
+    cur = halves[f, by0:by0 + 8, bx0:bx0 + 8].astype(np.int64)
+
+    def sad(ref, dx, dy):
+        rows = np.clip(np.arange(by0, by0 + 8) + dy, 0, ch - 1)
+        cols = np.clip(np.arange(bx0, bx0 + 8) + dx, 0, cw - 1)
+        return int(np.abs(cur - ref[np.ix_(rows, cols)]).sum()), ref[np.ix_(rows, cols)]
+
+    best, mask = None, 0
+    for k in range(1, min(nrefs, f) + 1):
+        ref = halves[f - k]
+        win = None
+        for c in range(8):
+            cx, cy = cen[k - 1][c]
+            for dy in range(-Rl, Rl + 1):
+                for dx in range(-Rl, Rl + 1):
+                    s = sad(ref, cx + dx, cy + dy)[0]
+                    if win is None or s < win[0]:
+                        win = (s, cx + dx, cy + dy)
+        if win[0] > fb:
+            mask |= 1 << (k - 1)
+            win = None
+            for dy in range(-R, R + 1):
+                for dx in range(-R, R + 1):
+                    s = sad(ref, dx, dy)[0]
+                    if win is None or s < win[0]:
+                        win = (s, dx, dy)
+        c = refimpl.satd8_sum(halves[f, by0:by0 + 8, bx0:bx0 + 8], sad(ref, win[1], win[2])[1])
+        if best is None or c < best[0]:
+            best = (c, (8 * win[1], 8 * win[2]), k)
+    return best + (mask,)
+
+
+def test_lookahead_learned_block_brute_force_and_reductions():
+    """The learned-seeded block search == numpy brute force (proposals near and far from
+    the true motion, fall-back taken and not); with the threshold below every SAD it is
+    the exhaustive form exactly; proposals at the true motion find it without fall-back."""
+    rng = np.random.default_rng(73)
+    base = rng.integers(0, 256, (40, 48), dtype=np.uint8)
+    halves = np.stack([np.roll(base, (2 * t, -3 * t), axis=(0, 1)) for t in range(4)])
+    ch, cw = 40, 48
+    for f, nrefs, R, Rl, fb in ((3, 3, 6, 2, 300), (2, 2, 4, 1, 50), (3, 1, 5, 2, 1 << 30)):
+        cen = rng.integers(-7, 8, (nrefs, 8, 2))
+        cen[:, 3] = [(3 * k, -2 * k) for k in range(1, nrefs + 1)][:nrefs]  # the true motion, rank 4
+        for bx0, by0 in ((16, 16), (0, 8), (40, 32)):
+            got = O.la_block_learned(halves, cw, ch, f, nrefs, R, Rl, fb, cen, bx0, by0)
+            assert got == _la_learned_textbook(halves, f, nrefs, R, Rl, fb, cen, bx0, by0), (f, bx0, by0)
+    for bx0, by0 in ((8, 8), (32, 24)):
+        ex = O.la_block_multi(halves, cw, ch, 3, 3, 5, bx0, by0)
+        got = O.la_block_learned(halves, cw, ch, 3, 3, 5, 2, -1, np.zeros((3, 8, 2), np.int32), bx0, by0)
+        assert got[:3] == ex and got[3] == 0b111
+    cen = np.zeros((1, 8, 2), np.int32)
+    cen[0, 0] = (3, -2)  # frame 1 is frame 0 moved by (-3, +2): its block matches 3 right, 2 up
+    c, mv, k, mask = O.la_block_learned(halves, cw, ch, 1, 1, 8, 1, 0, cen, 16, 16)
+    assert (c, mv, k, mask) == (0, (24, -16), 1, 0)
