@@ -200,6 +200,28 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
 int rh264_predict_blocks(rh264_ctx *ctx, const uint8_t *planes, int cw, int ch, int pad, int bw,
                          int bh, int n, const int32_t *xy, const int32_t *mv, uint8_t *out);
 
+/* Reference-mode motion search of every 16x16 block (motion_search, mb16.rs:
+ * 2433-3302, with the default knobs explicit, as the oracle's
+ * orc_motion_search_ref): the cost is cost(mv) = SATD (p->hadamard) + lr[rate] with
+ * rate = mvbits(mv - preds[0]); seeds (0,0) then the block's predictors
+ * preds[(frame*nblk + b)*npreds + i] (quarter pel pairs; strict <); the integer snap
+ * ((v + 2) div_euclid 4) * 4 of a fractional winner; the diamond rungs dc->steps
+ * (quarter pels, 4-point rings, cascade with a moving centre, each rung repeated
+ * until it no longer improves); with dc->sadfp the seed and diamond phases price
+ * SAD + lr_fp[rate] and the winner and seed are repriced in SATD (mb16.rs:2739-2742);
+ * then the sub-pel stage p->subpel_pat; with dc->snap the seed is kept if it is
+ * cheaper (mb16.rs:3297-3300).  npreds 0..8, nsteps 0..8; 16x16 only (other shapes
+ * decline). */
+typedef struct {
+    int32_t steps[8]; /* diamond rungs in quarter pels, e.g. 16, 8, 4 */
+    int32_t nsteps;
+    int32_t sadfp;    /* 1: SAD + lr_fp for the seed and diamond phases */
+    int32_t snap;     /* 1: integer snap of a fractional seed, seed kept if cheaper */
+} rh264_diamond;
+int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc, int nframes,
+                        const uint8_t *cur, size_t cur_stride, const uint8_t *planes, size_t planes_stride,
+                        const int32_t *preds, int npreds, int32_t *mv, int32_t *cost);
+
 /* ---- batched multi-stream search ------------------------------------------
  * Many streams of uneven size, 1..4 references each, several partition shapes,
  * searched together: for every shape, one integer-search launch over all

@@ -28,6 +28,11 @@ class SearchParams(C.Structure):
     ]
 
 
+class Diamond(C.Structure):
+    """rh264_diamond (include/rh264_me.h)."""
+    _fields_ = [("steps", C.c_int32 * 8), ("nsteps", C.c_int32), ("sadfp", C.c_int32), ("snap", C.c_int32)]
+
+
 class Stream(C.Structure):
     """rh264_stream (include/rh264_me.h)."""
     _fields_ = [("cur", C.c_void_p), ("planes", C.c_void_p), ("mvps", C.c_void_p), ("cw", C.c_int32),
@@ -73,6 +78,8 @@ PROTOTYPES = {
                                                    C.c_int, C.c_int, C.c_int, C.c_int32, c_vp, c_vp, c_vp, c_vp,
                                                    c_vp]),
     "rh264_mbtree_propagate_multi_i32": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "rh264_motion_search": (C.c_int, [c_vp, C.POINTER(SearchParams), C.POINTER(Diamond), C.c_int, c_vp, C.c_size_t,
+                                      c_vp, C.c_size_t, c_vp, C.c_int, c_vp, c_vp]),
     "rh264_rc_complexity": (C.c_int, [c_vp, c_vp, c_vp, C.c_size_t, C.c_int, C.c_int, C.c_int, c_vp]),
     "rh264_batch_out_len": (C.c_size_t, [C.c_int, C.c_int, c_i32p, C.c_int]),
     "rh264_batch_mvp_len": (C.c_size_t, [C.c_int, C.c_int, c_i32p, C.c_int]),

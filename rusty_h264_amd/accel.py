@@ -383,6 +383,28 @@ class MbTree:
         return r
 
 
+def motion_search_ref(dev, params, nframes, cur, planes, preds=None, npreds=0, steps=(16, 8, 4), sadfp=False,
+                      snap=True, cur_stride=None, planes_stride=None):
+    """The reference-mode motion search (rh264_motion_search, mb16.rs:2433-3302) of every
+    16x16 block; preds: device buffer [nframes*nblk][npreds][2] (quarter pel), the first
+    predictor pricing the rate.  Returns (mv [n][2], cost [n]) as host arrays."""
+    dc = _lib.Diamond()
+    for i, st in enumerate(steps):
+        dc.steps[i] = int(st)
+    dc.nsteps, dc.sadfp, dc.snap = len(steps), int(sadfp), int(snap)
+    n = (params.cw // 16) * (params.ch // 16) * nframes
+    mv, cost = dev.alloc(8 * n), dev.alloc(4 * n)
+    try:
+        _check(dev.lib.rh264_motion_search(dev.ctx, C.byref(params), C.byref(dc), nframes, int(cur),
+                                           cur_stride or params.cw * params.ch, int(planes),
+                                           planes_stride or planes_bytes(params.cw, params.ch, params.pad),
+                                           _ptr(preds), npreds, mv.ptr, cost.ptr), "rh264_motion_search")
+        return dev.download(mv, (n, 2), np.int32), dev.download(cost, (n,), np.int32)
+    finally:
+        mv.free()
+        cost.free()
+
+
 def rc_complexity(dev, cur, ref, nframes, cw, ch, frame_stride=None, out=None):
     """lookahead::complexity of nframes device-resident frames (ref None = IDR); returns
     the nframes scores as float64 (lookahead.rs:38-101)."""

@@ -1656,6 +1656,153 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
 }
 
 // ---------------------------------------------------------------------------
+// Reference-mode motion search (motion_search, mb16.rs:2433-3302, the default knobs
+// explicit), one wave per 16x16 block.  Every candidate is read from the plane set in
+// HBM (mx_operand_far: any vector, clamped at the plane's border) and priced with
+// the matrix-core SATD of the refinement kernels, or with v_sad_u8 and a wave sum in
+// the SAD phase of the B2 mode; the walk itself is wave-uniform scalar code.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int ms_div_euclid4(int v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
+
+template <bool T8>
+__global__ __launch_bounds__(256) void k_motion_search(rh264_search_params p, rh264_diamond dc,
+                                                       const uint8_t *__restrict__ cur, size_t cur_stride,
+                                                       const uint8_t *__restrict__ planes, size_t planes_stride,
+                                                       const int32_t *__restrict__ preds, int npreds,
+                                                       int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost)
+{
+    __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) {
+        s_lr[threadIdx.x] = p.lr[threadIdx.x];
+        s_lrfp[threadIdx.x] = p.lr_fp[threadIdx.x];
+    }
+    __syncthreads();
+    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    unsigned wgx, wgy;
+    xcd_grid2(wgx, wgy);
+    const int b = (int)wgx * 4 + wave; // wave-uniform
+    if (b >= nblk) return;
+    const int frame = (int)wgy;
+    const size_t boff = (size_t)frame * nblk + b;
+    const int lx = (b % nbx) << 4, ly = (b / nbx) << 4;
+    const int r = lane & 15, qd = lane >> 4;
+    MxBlock B;
+    B.pw = p.cw + 2 * p.pad;
+    B.ph = p.ch + 2 * p.pad;
+    B.pn = (size_t)B.pw * B.ph;
+    B.pl = planes + (size_t)frame * planes_stride;
+    B.sx = B.sy = 0; // candidates are addressed by their vectors
+    B.ix = lx + p.pad;
+    B.iy = ly + p.pad;
+    B.xoff = 0;
+    const int32_t *pr = preds + boff * (size_t)(npreds > 0 ? npreds : 1) * 2;
+    B.mpx = npreds > 0 ? __builtin_amdgcn_readfirstlane(pr[0]) : 0; // the rate predictor
+    B.mpy = npreds > 0 ? __builtin_amdgcn_readfirstlane(pr[1]) : 0;
+    const uint32_t src = *(const uint32_t *)(cur + (size_t)frame * cur_stride + (size_t)(ly + r) * p.cw + lx + 4 * qd);
+    const MxConst K = mx_const<T8>(lane);
+    const mx_f4 c2 = -mx_transform(src, K, mx_f4{0.0f, 0.0f, 0.0f, 0.0f});
+    auto rate = [&](const int32_t *lr, int mx, int my) { return lr[mvbits(mx - B.mpx) + mvbits(my - B.mpy)]; };
+    auto cost = [&](int mx, int my) -> int { // SATD phase (mc_satd_hp, mb16.rs:1930-2007)
+        const int s = mx_satd<T8>(mx_operand_far(B, lane, mx, my), K, c2);
+        return __builtin_amdgcn_readfirstlane(s + rate(s_lr, mx, my));
+    };
+    auto cost_fp = [&](int mx, int my) -> int { // seed and diamond phases
+        if (!dc.sadfp) return cost(mx, my);
+        const int s = mx_wave_sum((int)__builtin_amdgcn_sad_u8(src, mx_operand_far(B, lane, mx, my), 0u));
+        return __builtin_amdgcn_readfirstlane(s + rate(s_lrfp, mx, my));
+    };
+    // seeds: (0, 0), then the predictors (strict <)
+    int bx = 0, by = 0;
+    int bc = cost_fp(0, 0);
+    for (int i = 0; i < npreds; i++) {
+        const int px = __builtin_amdgcn_readfirstlane(pr[2 * i]), py = __builtin_amdgcn_readfirstlane(pr[2 * i + 1]);
+        const int pc = cost_fp(px, py);
+        if (pc < bc) {
+            bc = pc;
+            bx = px;
+            by = py;
+        }
+    }
+    const int seedx = bx, seedy = by;
+    int seed_c = bc;
+    if (dc.snap && ((bx & 3) | (by & 3))) {
+        bx = ms_div_euclid4(bx + 2) * 4;
+        by = ms_div_euclid4(by + 2) * 4;
+        bc = cost_fp(bx, by);
+    }
+    // diamond rungs, each a 4-point ring with a moving centre, repeated while it improves
+    for (int si = 0; si < dc.nsteps; si++) {
+        const int step = dc.steps[si];
+        for (;;) {
+            bool improved = false;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = bx + (i == 0 ? step : i == 1 ? -step : 0), y = by + (i == 2 ? step : i == 3 ? -step : 0);
+                const int cc = cost_fp(x, y);
+                if (cc < bc) {
+                    bc = cc;
+                    bx = x;
+                    by = y;
+                    improved = true;
+                }
+            }
+            if (!improved) break;
+        }
+    }
+    if (dc.sadfp) {
+        bc = cost(bx, by);
+        seed_c = cost(seedx, seedy);
+    }
+    // the sub-pel stage
+    const int pat = p.subpel_pat;
+    if (pat == 4) {
+        const int sx = bx, sy = by;
+        for (int dy = -2; dy <= 2; dy++)
+            for (int dx = -2; dx <= 2; dx++) {
+                if (dx == 0 && dy == 0) continue;
+                const int cc = cost(sx + dx, sy + dy);
+                if (cc < bc) {
+                    bc = cc;
+                    bx = sx + dx;
+                    by = sy + dy;
+                }
+            }
+    } else if (pat >= 0) {
+        const int nring = (pat & 1) ? 4 : 8;
+        for (int step = 2; step >= 1; step--) {
+            for (;;) {
+                bool improved = false;
+                for (int i = 0; i < nring; i++) {
+                    // ring8 order (s,0),(-s,0),(0,s),(0,-s),(s,s),(-s,-s),(s,-s),(-s,s)
+                    const int rx = (i < 2 || i >= 4) ? ((i & 1) ? -step : step) : 0;
+                    const int ry = i < 2 ? 0 : i < 4 ? ((i & 1) ? -step : step) : ((i == 4 || i == 7) ? step : -step);
+                    const int cc = cost(bx + rx, by + ry);
+                    if (cc < bc) {
+                        bc = cc;
+                        bx += rx;
+                        by += ry;
+                        improved = true;
+                    }
+                }
+                if (!improved || (pat & 2)) break;
+            }
+        }
+    }
+    if (dc.snap && seed_c < bc) {
+        bx = seedx;
+        by = seedy;
+        bc = seed_c;
+    }
+    if (lane == 0) {
+        out_mv[2 * boff] = bx;
+        out_mv[2 * boff + 1] = by;
+        out_cost[boff] = bc;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Rate-control frame complexity (lookahead::complexity, lookahead.rs:38-101), one
 // wave per macroblock, with the refinement's matrix-core 4x4 Hadamards:
 //  * IDR: the AC part of the sixteen transforms of the pixels (intra_activity,
@@ -3591,6 +3738,34 @@ int rh264_mbtree_costs(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_
                            inter, mv);
         LAUNCH_CHECK("k_mbtree_rows");
     }
+    return 0;
+}
+
+int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc, int nframes,
+                        const uint8_t *cur, size_t cur_stride, const uint8_t *planes, size_t planes_stride,
+                        const int32_t *preds, int npreds, int32_t *mv, int32_t *cost)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (!dc) return fail("motion_search: null diamond");
+    if (p->bw != 16 || p->bh != 16) return RH264_DECLINED;
+    if (npreds < 0 || npreds > 8 || dc->nsteps < 0 || dc->nsteps > 8) return RH264_DECLINED;
+    if (npreds > 0 && !preds) return fail("motion_search: null predictors");
+    for (int i = 0; i < dc->nsteps; i++)
+        if (dc->steps[i] < 1 || dc->steps[i] > 1024) return fail("motion_search: diamond step %d", dc->steps[i]);
+    rc = check_strides(cur_stride, planes_stride);
+    if (rc) return rc;
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    const int nblk = (p->cw / 16) * (p->ch / 16);
+    const dim3 grid((unsigned)((nblk + 3) / 4), nframes);
+    if (p->hadamard == 8)
+        hipLaunchKernelGGL(k_motion_search<true>, grid, dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes,
+                           planes_stride, preds, npreds, mv, cost);
+    else
+        hipLaunchKernelGGL(k_motion_search<false>, grid, dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes,
+                           planes_stride, preds, npreds, mv, cost);
+    LAUNCH_CHECK("k_motion_search");
     return 0;
 }
 

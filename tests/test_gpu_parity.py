@@ -368,3 +368,34 @@ def test_plain_search_batch_equals_single(dev):
         one = ms1.results(1)
         sl = slice(f * ms.nblk, (f + 1) * ms.nblk)
         assert np.array_equal(one["mv"], batch["mv"][sl]) and np.array_equal(one["cost"], batch["cost"][sl])
+
+
+@pytest.mark.parametrize("npreds,steps,sadfp,snap,pat,had", [(1, (16, 8, 4), False, True, 0, 4),
+                                                             (3, (16, 8, 4), True, True, 0, 4),
+                                                             (0, (64, 32, 16, 8, 4), False, False, 2, 4),
+                                                             (2, (8, 4), True, False, 1, 4),
+                                                             (2, (16, 8, 4), False, True, 4, 4),
+                                                             (1, (16, 8, 4), True, True, -1, 4),
+                                                             (2, (16, 8, 4), False, True, 0, 8)])
+def test_motion_search_reference_mode(dev, npreds, steps, sadfp, snap, pat, had):
+    """The reference-mode motion search (seeds, integer snap, diamond rungs, the B2
+    SAD phase, sub-pel stage, seed kept if cheaper) == the oracle's restatement for
+    every block, with fractional and far predictors."""
+    cw, ch, pad = 176, 144, 32
+    frames = synth.cif_clip(2, seed=npreds + len(steps) + pat, w=cw, h=ch)
+    cur, ref = dev.upload(frames[1]), dev.upload(frames[0])
+    pl = accel.build_planes(dev, ref, 1, cw, ch, pad)
+    p = accel.search_params(cw, ch, pad, 16, 16, 16, pat, LAM26, hadamard=had)
+    nb = (cw // 16) * (ch // 16)
+    rng = np.random.default_rng(npreds * 7 + pat)
+    preds = rng.integers(-90, 91, (nb, max(npreds, 1), 2)).astype(np.int32)
+    dp = dev.upload(preds)
+    mv, cost = accel.motion_search_ref(dev, p, 1, cur, pl, dp, npreds, steps=steps, sadfp=sadfp, snap=snap)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 16, pat, LAM26, hadamard=had)
+    opl = O.planes(frames[0], cw, ch, pad)
+    for b in range(nb):
+        bx, by = (b % (cw // 16)) * 16, (b // (cw // 16)) * 16
+        want = O.motion_search_ref(cfg, frames[1], opl, bx, by, preds[b, :npreds], steps=steps, sadfp=sadfp, snap=snap)
+        assert (tuple(mv[b]), int(cost[b])) == want, (b, preds[b, :npreds])
+    for x in (cur, ref, pl, dp):
+        x.free()
