@@ -222,6 +222,24 @@ int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh26
                         const uint8_t *cur, size_t cur_stride, const uint8_t *planes, size_t planes_stride,
                         const int32_t *preds, int npreds, int32_t *mv, int32_t *cost);
 
+/* The encoder's per-macroblock 16x16 decision in macroblock order, for P frames
+ * whose every macroblock takes its P_L0_16x16 best_part result (mb16.rs:4921-4959
+ * with no extra seeds, as called at mb16.rs:5491): for each reference r < nrefs,
+ * the search of rh264_motion_search with the single predictor predict_mv
+ * (inter.rs:36-73) against r of the left, above and above-right (else above-left)
+ * neighbours' decided vectors and references (mv_neighbors, mb16.rs:1813-1836);
+ * cost + (int64)(lambda_ref * ref_bits(r, nrefs)) (mb16.rs:5967-5980), the first
+ * minimum in reference order.  planes holds nframes + nrefs - 1 plane sets of
+ * consecutive frames, planes_stride bytes apart; frame f's reference r (0 = the
+ * most recent) is plane set f + nrefs - 1 - r.  The frames are decided as a
+ * wavefront: one launch per anti-diagonal x + 2y = t, all frames at once.
+ * Outputs per macroblock: ref, mv (pair) and cost, int32.  16x16 only; nrefs
+ * 1..4. */
+int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc,
+                                  int nframes, int nrefs, double lambda_ref, const uint8_t *cur, size_t cur_stride,
+                                  const uint8_t *planes, size_t planes_stride, int32_t *ref, int32_t *mv,
+                                  int32_t *cost);
+
 /* ---- batched multi-stream search ------------------------------------------
  * Many streams of uneven size, 1..4 references each, several partition shapes,
  * searched together: for every shape, one integer-search launch over all

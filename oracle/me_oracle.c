@@ -714,6 +714,89 @@ int64_t orc_motion_search_ref(const orc_search_cfg *cfg, const orc_diamond_cfg *
     return bc;
 }
 
+/* median3 (inter.rs:9-11) and predict_mv (inter.rs:36-73). */
+static int median3(int a, int b, int c)
+{
+    int mn = a < b ? a : b, mx = a > b ? a : b;
+    mn = c < mn ? c : mn;
+    mx = c > mx ? c : mx;
+    return a + b + c - mn - mx;
+}
+
+void orc_predict_mv(const int32_t nb[12], int cur_ref, int32_t out[2])
+{
+    int mv[3][2], ref[3];
+    for (int k = 0; k < 3; k++) {
+        const int32_t *n = nb + 4 * k;
+        int usable = n[0] && n[3] >= 0; /* an intra or unavailable neighbour: (0,0), ref -1 */
+        mv[k][0] = usable ? n[1] : 0;
+        mv[k][1] = usable ? n[2] : 0;
+        ref[k] = usable ? n[3] : -1;
+    }
+    if (!nb[4] && !nb[8] && nb[0]) { /* B and C unavailable, A available: both take A */
+        for (int k = 1; k < 3; k++) {
+            mv[k][0] = mv[0][0];
+            mv[k][1] = mv[0][1];
+            ref[k] = ref[0];
+        }
+    }
+    int matches = (ref[0] == cur_ref) + (ref[1] == cur_ref) + (ref[2] == cur_ref);
+    if (matches == 1) {
+        int k = ref[0] == cur_ref ? 0 : ref[1] == cur_ref ? 1 : 2;
+        out[0] = mv[k][0];
+        out[1] = mv[k][1];
+    } else {
+        out[0] = median3(mv[0][0], mv[1][0], mv[2][0]);
+        out[1] = median3(mv[0][1], mv[1][1], mv[2][1]);
+    }
+}
+
+static int ref_bits_orc(int r, int nrefs) /* mb16.rs:5967-5980 */
+{
+    if (nrefs <= 1) return 0;
+    if (nrefs == 2) return 1;
+    unsigned n = (unsigned)r + 1;
+    int len = 1;
+    while (n > 1) {
+        n >>= 1;
+        len += 2;
+    }
+    return len;
+}
+
+void orc_motion_search_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *dc, const uint8_t *cur,
+                             const uint8_t *const *planes, int nrefs, double lambda_ref, int32_t *ref,
+                             int32_t *mv, int64_t *cost)
+{
+    const int nbx = cfg->cw / 16, nby = cfg->ch / 16;
+    /* raster macroblock order, as the encoder's frame loop; each macroblock
+     * takes its 16x16 best_part result (mb16.rs:4921-4959) */
+    for (int y = 0; y < nby; y++)
+        for (int x = 0; x < nbx; x++) {
+            int32_t nb[12] = {0, 0, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1};
+            int ia = y * nbx + x - 1, ib = (y - 1) * nbx + x, ic, cav;
+            if (x > 0) { nb[0] = 1; nb[1] = mv[2 * ia]; nb[2] = mv[2 * ia + 1]; nb[3] = ref[ia]; }
+            if (y > 0) { nb[4] = 1; nb[5] = mv[2 * ib]; nb[6] = mv[2 * ib + 1]; nb[7] = ref[ib]; }
+            if (y > 0 && x + 1 < nbx) { ic = (y - 1) * nbx + x + 1; cav = 1; }  /* C = above-right */
+            else { ic = (y - 1) * nbx + x - 1; cav = x > 0 && y > 0; }          /* else D = above-left */
+            if (cav) { nb[8] = 1; nb[9] = mv[2 * ic]; nb[10] = mv[2 * ic + 1]; nb[11] = ref[ic]; }
+            const int i = y * nbx + x;
+            int64_t bc = INT64_MAX;
+            int br = 0, bmx = 0, bmy = 0;
+            for (int r = 0; r < nrefs; r++) {
+                int32_t pr[2], m[2];
+                orc_predict_mv(nb, r, pr);
+                int64_t c = orc_motion_search_ref(cfg, dc, cur, planes[r], 16 * x, 16 * y, pr, 1, m);
+                c += (int64_t)(lambda_ref * (double)ref_bits_orc(r, nrefs));
+                if (c < bc) { bc = c; br = r; bmx = m[0]; bmy = m[1]; }
+            }
+            ref[i] = br;
+            mv[2 * i] = bmx;
+            mv[2 * i + 1] = bmy;
+            cost[i] = bc;
+        }
+}
+
 /* ---------------------------------------------------------------------------
  * mb-tree lookahead
  * ------------------------------------------------------------------------- */

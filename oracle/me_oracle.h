@@ -115,6 +115,19 @@ int64_t orc_motion_search_ref(const orc_search_cfg *cfg, const orc_diamond_cfg *
                               const uint8_t *cur, const uint8_t *planes, int lx, int ly,
                               const int32_t *preds, int npreds, int32_t *mv_out);
 
+/* predict_mv (inter.rs:36-73): nb = neighbours A (left), B (above), C (above-right
+ * or above-left) as {available, mvx, mvy, ref_idx}; out = the predictor. */
+void orc_predict_mv(const int32_t nb[12], int cur_ref, int32_t out[2]);
+/* A P frame decided macroblock by macroblock in raster order, every macroblock
+ * taking its 16x16 best_part result (mb16.rs:4921-4959, no extra seeds): for each
+ * reference r (planes[r], r = 0 the most recent) orc_motion_search_ref with the
+ * single predictor predict_mv(.., r) of the decided neighbours (mv_neighbors,
+ * mb16.rs:1813-1836), cost + (int64)(lambda_ref * ref_bits(r, nrefs)), the first
+ * minimum in reference order.  ref, cost: one per macroblock; mv: two. */
+void orc_motion_search_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *dc, const uint8_t *cur,
+                             const uint8_t *const *planes, int nrefs, double lambda_ref, int32_t *ref,
+                             int32_t *mv, int64_t *cost);
+
 /* ---- mb-tree lookahead: crates/rusty_h264-encoder/src/mbtree.rs ------------- */
 void orc_downsample2x(const uint8_t *y, int cw, int ch, uint8_t *out);
 int32_t orc_intra_cost(const uint8_t *sy, int cw, int bx0, int by0, int bs);

@@ -177,6 +177,32 @@ class Oracle:
                                            _p(pr, i32p), len(pr) // 2, _p(mv, i32p))
         return (int(mv[0]), int(mv[1])), int(c)
 
+    def predict_mv(self, a, b, c, cur_ref):
+        """predict_mv (inter.rs:36-73); a, b, c = (available, (mvx, mvy), ref_idx) or None."""
+        nb = np.zeros(12, np.int32)
+        for k, n in enumerate((a, b, c)):
+            nb[4 * k:4 * k + 4] = (0, 0, 0, -1) if n is None else (int(n[0]), n[1][0], n[1][1], n[2])
+        out = np.empty(2, np.int32)
+        self.lib.orc_predict_mv(_p(nb, i32p), int(cur_ref), _p(out, i32p))
+        return int(out[0]), int(out[1])
+
+    def motion_search_frame(self, cfg, cur, planes, lambda_ref=0.0, steps=(16, 8, 4), sadfp=False, snap=True):
+        """A P frame's 16x16 best_part decisions in raster order with predict_mv
+        predictors; planes: one plane set (reference 0) or a list, most recent first.
+        Returns (ref (nblk,) int32, mv (nblk, 2) int32, cost (nblk,) int64)."""
+        st = np.ascontiguousarray(steps, np.int32)
+        dc = DiamondCfg(_p(st, i32p), len(st), int(sadfp), int(snap))
+        pls = [planes] if isinstance(planes, np.ndarray) else list(planes)
+        pls = [_u8(x) for x in pls]
+        arr = (C.c_void_p * len(pls))(*[x.ctypes.data for x in pls])
+        nb = (cfg.cw // 16) * (cfg.ch // 16)
+        ref = np.empty(nb, np.int32)
+        mv = np.empty((nb, 2), np.int32)
+        c = np.empty(nb, np.int64)
+        self.lib.orc_motion_search_frame(C.byref(cfg), C.byref(dc), _p(_u8(cur)), arr, len(pls),
+                                         C.c_double(lambda_ref), _p(ref, i32p), _p(mv, i32p), _p(c, i64p))
+        return ref, mv, c
+
     # ---- mb-tree
     def downsample2x(self, y, cw, ch):
         out = np.empty((ch // 2, cw // 2), np.uint8)

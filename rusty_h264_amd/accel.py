@@ -405,6 +405,33 @@ def motion_search_ref(dev, params, nframes, cur, planes, preds=None, npreds=0, s
         cost.free()
 
 
+def motion_search_wavefront(dev, params, nframes, cur, planes, nrefs=1, lambda_ref=0.0, steps=(16, 8, 4),
+                            sadfp=False, snap=True, cur_stride=None, planes_stride=None, out=None):
+    """Every macroblock of nframes P frames decided in macroblock order with its 16x16
+    best_part result over nrefs references (rh264_motion_search_wavefront;
+    mb16.rs:4921-4959, predictor predict_mv of the decided neighbours, inter.rs:36-73).
+    planes: nframes + nrefs - 1 consecutive plane sets, frame f's reference r being
+    set f + nrefs - 1 - r.  out: (ref, mv, cost) device buffers or None; returns them
+    (asynchronous)."""
+    dc = _lib.Diamond()
+    for i, st in enumerate(steps):
+        dc.steps[i] = int(st)
+    dc.nsteps, dc.sadfp, dc.snap = len(steps), int(sadfp), int(snap)
+    n = (params.cw // 16) * (params.ch // 16) * nframes
+    if out is None:
+        out = (dev.alloc(4 * n), dev.alloc(8 * n), dev.alloc(4 * n))
+    ref, mv, cost = out
+    _need(ref, 4 * n, "ref")
+    _need(mv, 8 * n, "mv")
+    _need(cost, 4 * n, "cost")
+    _check(dev.lib.rh264_motion_search_wavefront(dev.ctx, C.byref(params), C.byref(dc), nframes, nrefs,
+                                                 float(lambda_ref), int(cur), cur_stride or params.cw * params.ch,
+                                                 int(planes),
+                                                 planes_stride or planes_bytes(params.cw, params.ch, params.pad),
+                                                 ref.ptr, mv.ptr, cost.ptr), "rh264_motion_search_wavefront")
+    return out
+
+
 def rc_complexity(dev, cur, ref, nframes, cw, ch, frame_stride=None, out=None):
     """lookahead::complexity of nframes device-resident frames (ref None = IDR); returns
     the nframes scores as float64 (lookahead.rs:38-101)."""

@@ -1665,42 +1665,29 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
 
 __device__ __forceinline__ int ms_div_euclid4(int v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
 
+// One 16x16 block's reference-mode search (motion_search, mb16.rs:2433-3302, the
+// default knobs explicit) by one wave: the block at (lx, ly) of the frame at
+// frame_px against plane set pl; seeds (0, 0) then pr[0..npreds) (wave-uniform
+// quarter-pel pairs), pr[0] pricing the rate; returns the vector and its cost.
 template <bool T8>
-__global__ __launch_bounds__(256) void k_motion_search(rh264_search_params p, rh264_diamond dc,
-                                                       const uint8_t *__restrict__ cur, size_t cur_stride,
-                                                       const uint8_t *__restrict__ planes, size_t planes_stride,
-                                                       const int32_t *__restrict__ preds, int npreds,
-                                                       int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost)
+__device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh264_diamond &dc,
+                                          const int32_t *s_lr, const int32_t *s_lrfp, const uint8_t *frame_px,
+                                          const uint8_t *pl, int lx, int ly, const int32_t *pr, int npreds, int lane,
+                                          int &obx, int &oby, int &obc)
 {
-    __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
-    if (threadIdx.x < RH264_LR_N) {
-        s_lr[threadIdx.x] = p.lr[threadIdx.x];
-        s_lrfp[threadIdx.x] = p.lr_fp[threadIdx.x];
-    }
-    __syncthreads();
-    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    unsigned wgx, wgy;
-    xcd_grid2(wgx, wgy);
-    const int b = (int)wgx * 4 + wave; // wave-uniform
-    if (b >= nblk) return;
-    const int frame = (int)wgy;
-    const size_t boff = (size_t)frame * nblk + b;
-    const int lx = (b % nbx) << 4, ly = (b / nbx) << 4;
-    const int r = lane & 15, qd = lane >> 4;
     MxBlock B;
     B.pw = p.cw + 2 * p.pad;
     B.ph = p.ch + 2 * p.pad;
     B.pn = (size_t)B.pw * B.ph;
-    B.pl = planes + (size_t)frame * planes_stride;
+    B.pl = pl;
     B.sx = B.sy = 0; // candidates are addressed by their vectors
     B.ix = lx + p.pad;
     B.iy = ly + p.pad;
     B.xoff = 0;
-    const int32_t *pr = preds + boff * (size_t)(npreds > 0 ? npreds : 1) * 2;
     B.mpx = npreds > 0 ? __builtin_amdgcn_readfirstlane(pr[0]) : 0; // the rate predictor
     B.mpy = npreds > 0 ? __builtin_amdgcn_readfirstlane(pr[1]) : 0;
-    const uint32_t src = *(const uint32_t *)(cur + (size_t)frame * cur_stride + (size_t)(ly + r) * p.cw + lx + 4 * qd);
+    const int r = lane & 15, qd = lane >> 4;
+    const uint32_t src = *(const uint32_t *)(frame_px + (size_t)(ly + r) * p.cw + lx + 4 * qd);
     const MxConst K = mx_const<T8>(lane);
     const mx_f4 c2 = -mx_transform(src, K, mx_f4{0.0f, 0.0f, 0.0f, 0.0f});
     auto rate = [&](const int32_t *lr, int mx, int my) { return lr[mvbits(mx - B.mpx) + mvbits(my - B.mpy)]; };
@@ -1795,10 +1782,147 @@ __global__ __launch_bounds__(256) void k_motion_search(rh264_search_params p, rh
         by = seedy;
         bc = seed_c;
     }
+    obx = bx;
+    oby = by;
+    obc = bc;
+}
+
+template <bool T8>
+__global__ __launch_bounds__(256) void k_motion_search(rh264_search_params p, rh264_diamond dc,
+                                                       const uint8_t *__restrict__ cur, size_t cur_stride,
+                                                       const uint8_t *__restrict__ planes, size_t planes_stride,
+                                                       const int32_t *__restrict__ preds, int npreds,
+                                                       int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost)
+{
+    __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) {
+        s_lr[threadIdx.x] = p.lr[threadIdx.x];
+        s_lrfp[threadIdx.x] = p.lr_fp[threadIdx.x];
+    }
+    __syncthreads();
+    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    unsigned wgx, wgy;
+    xcd_grid2(wgx, wgy);
+    const int b = (int)wgx * 4 + wave; // wave-uniform
+    if (b >= nblk) return;
+    const int frame = (int)wgy;
+    const size_t boff = (size_t)frame * nblk + b;
+    int bx, by, bc;
+    ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride, planes + (size_t)frame * planes_stride,
+                  (b % nbx) << 4, (b / nbx) << 4, preds + boff * (size_t)(npreds > 0 ? npreds : 1) * 2, npreds,
+                  lane, bx, by, bc);
     if (lane == 0) {
         out_mv[2 * boff] = bx;
         out_mv[2 * boff + 1] = by;
         out_cost[boff] = bc;
+    }
+}
+
+// median3 (inter.rs:9-11)
+__device__ __forceinline__ int ms_median3(int a, int b, int c)
+{
+    return a + b + c - min(min(a, b), c) - max(max(a, b), c);
+}
+
+// predict_mv (inter.rs:36-73) against reference cur_ref for macroblock (x, y), from
+// the decided vectors and references of its neighbours A (left), B (above) and C
+// (above-right, else D = above-left; mv_neighbors, mb16.rs:1813-1836).  Every
+// decided macroblock is an inter block, so an available neighbour is usable.
+__device__ __forceinline__ void ms_predict_mv(const int32_t *mv, const int32_t *ref, int nbx, int x, int y,
+                                              int cur_ref, int &px, int &py)
+{
+    const bool av_a = x > 0, av_b = y > 0;
+    const bool c_is_c = y > 0 && x + 1 < nbx;
+    const bool av_c = c_is_c || (x > 0 && y > 0);
+    const int ia = y * nbx + x - 1, ib = (y - 1) * nbx + x, ic = c_is_c ? (y - 1) * nbx + x + 1 : (y - 1) * nbx + x - 1;
+    int ax = 0, ay = 0, ra = -1, bx = 0, by = 0, rb = -1, cx = 0, cy = 0, rc = -1;
+    if (av_a) { ax = mv[2 * ia]; ay = mv[2 * ia + 1]; ra = ref[ia]; }
+    if (av_b) { bx = mv[2 * ib]; by = mv[2 * ib + 1]; rb = ref[ib]; }
+    if (av_c) { cx = mv[2 * ic]; cy = mv[2 * ic + 1]; rc = ref[ic]; }
+    if (!av_b && !av_c && av_a) { // B and C take A
+        bx = cx = ax;
+        by = cy = ay;
+        rb = rc = ra;
+    }
+    const int matches = (ra == cur_ref) + (rb == cur_ref) + (rc == cur_ref);
+    if (matches == 1) {
+        px = ra == cur_ref ? ax : rb == cur_ref ? bx : cx;
+        py = ra == cur_ref ? ay : rb == cur_ref ? by : cy;
+    } else {
+        px = ms_median3(ax, bx, cx);
+        py = ms_median3(ay, by, cy);
+    }
+}
+
+// The encoder's 16x16 decision (best_part, mb16.rs:4921-4959, no extra seeds) as a
+// wavefront: launch wf_t covers the macroblocks (x, y) with x + 2y = wf_t of every
+// frame, whose left, above and above-right (or above-left) neighbours were decided
+// by earlier launches.  RW waves share a macroblock, wave r searching reference r
+// (plane set frame + nrefs - 1 - r) from its single seed predict_mv(.., r); the
+// cheapest cost + refc[r] wins, the first in reference order on ties.
+struct MsRefCost {
+    int32_t c[4];
+};
+template <bool T8, int RW>
+__global__ __launch_bounds__(256) void k_ms_wavefront(rh264_search_params p, rh264_diamond dc,
+                                                      const uint8_t *__restrict__ cur, size_t cur_stride,
+                                                      const uint8_t *__restrict__ planes, size_t planes_stride,
+                                                      int nrefs, MsRefCost refc, int wf_t, int32_t *__restrict__ out_ref,
+                                                      int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost)
+{
+    __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
+    __shared__ int32_t s_res[4][3];
+    if (threadIdx.x < RH264_LR_N) {
+        s_lr[threadIdx.x] = p.lr[threadIdx.x];
+        s_lrfp[threadIdx.x] = p.lr_fp[threadIdx.x];
+    }
+    __syncthreads();
+    const int nbx = p.cw >> 4, nby = p.ch >> 4, nblk = nbx * nby;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    // the anti-diagonal's macroblocks: y from y_lo while x = wf_t - 2y stays >= 0
+    const int y_lo = max(0, (wf_t - (nbx - 1) + 1) / 2), y_hi = min(nby - 1, wf_t / 2);
+    const int n_t = y_hi - y_lo + 1;
+    const int j = (int)blockIdx.x * (4 / RW) + wave / RW, rf = wave % RW; // wave-uniform
+    const int frame = (int)blockIdx.y;
+    const bool mb_ok = j < n_t;
+    const int y = y_lo + j, x = wf_t - 2 * y;
+    const size_t boff = (size_t)frame * nblk + (size_t)y * nbx + x;
+    if (mb_ok && rf < nrefs) {
+        int32_t pr[2];
+        int px, py;
+        ms_predict_mv(out_mv + (size_t)frame * nblk * 2, out_ref + (size_t)frame * nblk, nbx, x, y, rf, px, py);
+        pr[0] = __builtin_amdgcn_readfirstlane(px);
+        pr[1] = __builtin_amdgcn_readfirstlane(py);
+        int bx, by, bc;
+        ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
+                      planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, x << 4, y << 4, pr, 1, lane, bx, by,
+                      bc);
+        bc += refc.c[rf];
+        if constexpr (RW == 1) {
+            if (lane == 0) {
+                out_ref[boff] = 0;
+                out_mv[2 * boff] = bx;
+                out_mv[2 * boff + 1] = by;
+                out_cost[boff] = bc;
+            }
+        } else if (lane == 0) {
+            s_res[wave][0] = bx;
+            s_res[wave][1] = by;
+            s_res[wave][2] = bc;
+        }
+    }
+    if constexpr (RW > 1) {
+        __syncthreads();
+        if (mb_ok && rf == 0 && lane == 0) {
+            int br = 0;
+            for (int q = 1; q < nrefs; q++)
+                if (s_res[wave + q][2] < s_res[wave + br][2]) br = q;
+            out_ref[boff] = br;
+            out_mv[2 * boff] = s_res[wave + br][0];
+            out_mv[2 * boff + 1] = s_res[wave + br][1];
+            out_cost[boff] = s_res[wave + br][2];
+        }
     }
 }
 
@@ -3766,6 +3890,54 @@ int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh26
         hipLaunchKernelGGL(k_motion_search<false>, grid, dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes,
                            planes_stride, preds, npreds, mv, cost);
     LAUNCH_CHECK("k_motion_search");
+    return 0;
+}
+
+static int ref_bits(int r, int nrefs);
+
+#define MS_WF_LAUNCH(T8, RW)                                                                                  \
+    hipLaunchKernelGGL((k_ms_wavefront<T8, RW>), grid, dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes, \
+                       planes_stride, nrefs, refc, t, ref, mv, cost)
+
+int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc,
+                                  int nframes, int nrefs, double lambda_ref, const uint8_t *cur, size_t cur_stride,
+                                  const uint8_t *planes, size_t planes_stride, int32_t *ref, int32_t *mv,
+                                  int32_t *cost)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (!dc) return fail("motion_search_wavefront: null diamond");
+    if (p->bw != 16 || p->bh != 16) return RH264_DECLINED;
+    if (nrefs < 1 || nrefs > 4 || dc->nsteps < 0 || dc->nsteps > 8) return RH264_DECLINED;
+    for (int i = 0; i < dc->nsteps; i++)
+        if (dc->steps[i] < 1 || dc->steps[i] > 1024) return fail("motion_search: diamond step %d", dc->steps[i]);
+    if (!(lambda_ref >= 0.0 && lambda_ref < 1e8)) return fail("motion_search_wavefront: lambda_ref %g", lambda_ref);
+    if (!ref || !mv || !cost) return fail("motion_search_wavefront: null output");
+    rc = check_strides(cur_stride, planes_stride);
+    if (rc) return rc;
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    MsRefCost refc{};
+    for (int r = 0; r < nrefs; r++) refc.c[r] = (int32_t)(int64_t)(lambda_ref * (double)ref_bits(r, nrefs));
+    const int rw = nrefs == 1 ? 1 : nrefs == 2 ? 2 : 4, per_wg = 4 / rw;
+    const int nbx = p->cw / 16, nby = p->ch / 16;
+    // anti-diagonals x + 2y = t: each macroblock's left, above and above-right
+    // (or above-left) neighbours lie on earlier ones
+    for (int t = 0; t <= (nbx - 1) + 2 * (nby - 1); t++) {
+        const int y_lo = std::max(0, (t - (nbx - 1) + 1) / 2), y_hi = std::min(nby - 1, t / 2);
+        const int n_t = y_hi - y_lo + 1;
+        if (n_t <= 0) continue;
+        const dim3 grid((unsigned)((n_t + per_wg - 1) / per_wg), nframes);
+        const bool t8 = p->hadamard == 8;
+        if (rw == 1) {
+            if (t8) MS_WF_LAUNCH(true, 1); else MS_WF_LAUNCH(false, 1);
+        } else if (rw == 2) {
+            if (t8) MS_WF_LAUNCH(true, 2); else MS_WF_LAUNCH(false, 2);
+        } else {
+            if (t8) MS_WF_LAUNCH(true, 4); else MS_WF_LAUNCH(false, 4);
+        }
+        LAUNCH_CHECK("k_ms_wavefront");
+    }
     return 0;
 }
 

@@ -399,3 +399,43 @@ def test_motion_search_reference_mode(dev, npreds, steps, sadfp, snap, pat, had)
         assert (tuple(mv[b]), int(cost[b])) == want, (b, preds[b, :npreds])
     for x in (cur, ref, pl, dp):
         x.free()
+
+
+@pytest.mark.parametrize("cw,ch,nf,nrefs,lam_ref,steps,sadfp,snap,pat,had", [
+    (176, 144, 3, 1, 0.0, (16, 8, 4), False, True, 0, 4),
+    (352, 288, 1, 1, 0.0, (16, 8, 4), True, True, 4, 4),
+    (16, 96, 2, 1, 0.0, (64, 32, 16, 8, 4), False, False, 2, 4),
+    (320, 16, 2, 1, 0.0, (8, 4), True, False, 1, 4),
+    (176, 144, 2, 1, 0.0, (16, 8, 4), False, True, 0, 8),
+    (176, 144, 2, 2, 6.4, (16, 8, 4), False, True, 0, 4),
+    (176, 144, 2, 3, 3.2, (16, 8, 4), True, True, 4, 4),
+    (64, 80, 3, 4, 0.0, (16, 8, 4), False, True, 0, 8)])
+def test_motion_search_wavefront(dev, cw, ch, nf, nrefs, lam_ref, steps, sadfp, snap, pat, had):
+    """Every macroblock decided in macroblock order with its best_part result over
+    nrefs references, predict_mv of its decided neighbours per reference (one launch
+    per anti-diagonal, all frames at once) == the oracle's raster-order frame
+    decision, one-column and one-row frames included."""
+    pad = 32
+    clip = synth.cif_clip(nf + nrefs, seed=cw + nf + pat + nrefs, w=cw, h=ch)
+    cur, refs = dev.upload(clip[nrefs:]), dev.upload(clip[:-1])
+    pl = accel.build_planes(dev, refs, nf + nrefs - 1, cw, ch, pad)
+    p = accel.search_params(cw, ch, pad, 16, 16, 16, pat, LAM26, hadamard=had)
+    nb = (cw // 16) * (ch // 16)
+    print(f"wavefront {cw}x{ch} frames {nf} refs {nrefs} steps {steps} pat {pat} had {had}", flush=True)
+    out = accel.motion_search_wavefront(dev, p, nf, cur, pl, nrefs=nrefs, lambda_ref=lam_ref, steps=steps,
+                                        sadfp=sadfp, snap=snap)
+    ref = dev.download(out[0], (nf * nb,), np.int32)
+    mv = dev.download(out[1], (nf * nb, 2), np.int32)
+    cost = dev.download(out[2], (nf * nb,), np.int32)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 16, pat, LAM26, hadamard=had)
+    opl = [O.planes(clip[k], cw, ch, pad) for k in range(nf + nrefs - 1)]
+    for f in range(nf):
+        w_ref, w_mv, w_c = O.motion_search_frame(cfg, clip[f + nrefs], [opl[f + nrefs - 1 - r] for r in range(nrefs)],
+                                                 lambda_ref=lam_ref, steps=steps, sadfp=sadfp, snap=snap)
+        sl = slice(f * nb, (f + 1) * nb)
+        bad = np.nonzero((ref[sl] != w_ref) | (mv[sl] != w_mv).any(1) | (cost[sl] != w_c))[0]
+        assert bad.size == 0, (f, bad[:8], ref[sl][bad[:4]], w_ref[bad[:4]], mv[sl][bad[:4]], w_mv[bad[:4]])
+        if nrefs > 1 and f == 0:
+            assert len(set(w_ref.tolist())) > 1
+    for x in (cur, refs, pl) + tuple(out):
+        x.free()

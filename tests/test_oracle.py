@@ -403,6 +403,81 @@ def test_motion_search_ref_invariants():
             assert c <= seed_c
 
 
+def _inter(mv, r=0):
+    return (True, mv, r)
+
+
+def test_predict_mv_known_answers():
+    """The reference's own known answers (inter.rs:1993-2034) and the median cases."""
+    assert O.predict_mv(_inter((8, -4)), None, None, 0) == (8, -4)
+    assert O.predict_mv(_inter((4, 0)), _inter((8, 0)), _inter((12, 0)), 0) == (8, 0)
+    intra = (True, (0, 0), -1)
+    assert O.predict_mv(intra, _inter((5, 7)), intra, 0) == (5, 7)
+    a, b, c = _inter((4, 4), 1), _inter((5, 7), 0), intra
+    assert O.predict_mv(a, b, c, 0) == (5, 7) and O.predict_mv(a, b, c, 1) == (4, 4)
+    # median3 (inter.rs:1993-1999) through three matching neighbours
+    for x3, want in (((1, 2, 3), 2), ((3, 1, 2), 2), ((-5, 0, 5), 0), ((7, 7, 2), 7)):
+        got = O.predict_mv(*[_inter((v, -v)) for v in x3], 0)
+        assert got == (want, -want)
+
+
+def test_predict_mv_spec_cases():
+    """Spec 8.4.1.3.1 cases the known answers leave open."""
+    assert O.predict_mv(None, None, None, 0) == (0, 0)
+    # B and C unavailable: both take A, even when A's reference does not match
+    assert O.predict_mv(_inter((9, 3), 1), None, None, 0) == (9, 3)
+    # an intra neighbour enters the median as (0, 0)
+    assert O.predict_mv((True, (40, 40), -1), _inter((8, 8)), _inter((4, 2)), 0) == (4, 2)
+    # only C unavailable: no copy from A; the unavailable C is (0, 0) with no match
+    assert O.predict_mv(_inter((8, 8)), _inter((4, -6)), None, 0) == (4, 0)
+    # two matches of three: the median, the other reference's vector included
+    assert O.predict_mv(_inter((8, 8)), _inter((-4, 6), 2), _inter((2, 2)), 0) == (2, 6)
+    # a permutation of three matching neighbours leaves the median unchanged
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        v = [tuple(int(t) for t in rng.integers(-64, 65, 2)) for _ in range(3)]
+        want = tuple(int(np.median([m[k] for m in v])) for k in range(2))
+        for perm in ((0, 1, 2), (2, 0, 1), (1, 2, 0)):
+            assert O.predict_mv(*[_inter(v[i]) for i in perm], 0) == want
+
+
+@pytest.mark.parametrize("cw,ch,nrefs,lam_ref", [(64, 48, 1, 0.0), (16, 48, 1, 0.0), (48, 16, 1, 0.0),
+                                                  (64, 48, 2, 7.3), (64, 64, 3, 2.9), (48, 48, 4, 0.0)])
+def test_motion_search_frame_composes_best_part(cw, ch, nrefs, lam_ref):
+    """The raster-order frame decision == best_part (mb16.rs:4921-4959) of each
+    macroblock: per reference motion_search_ref with predict_mv(.., r) of its decided
+    left, above and above-right (else above-left) neighbours (mb16.rs:1813-1836),
+    plus (int)(lambda_ref * ref_bits(r)), first minimum in reference order; the
+    frame's last column, one-macroblock-wide and one-row frames included."""
+    pad = 32
+    rng = np.random.default_rng(cw + nrefs)
+    cur, ref0 = _pair(cw, ch, 33)
+    refs = [ref0] + [np.clip(ref0.astype(np.int32) + rng.integers(-6, 7, ref0.shape), 0, 255).astype(np.uint8)
+                     for _ in range(nrefs - 1)]
+    pls = [O.planes(r, cw, ch, pad) for r in refs]
+    cfg = O.cfg(cw, ch, pad, 16, 16, 8, 0, 4.65)
+    ref, mv, cost = O.motion_search_frame(cfg, cur, pls, lambda_ref=lam_ref)
+    nbx = cw // 16
+    rb = [0 if nrefs == 1 else 1 if nrefs == 2 else 2 * int(np.log2(r + 1)) + 1 for r in range(nrefs)]
+    for b in range(len(mv)):
+        x, y = b % nbx, b // nbx
+
+        def nbr(av, xx, yy):
+            i = yy * nbx + xx
+            return _inter(tuple(int(t) for t in mv[i]), int(ref[i])) if av else None
+        c = nbr(True, x + 1, y - 1) if (y > 0 and x + 1 < nbx) else nbr(x > 0 and y > 0, x - 1, y - 1)
+        a, bb = nbr(x > 0, x - 1, y), nbr(y > 0, x, y - 1)
+        best = None
+        for r in range(nrefs):
+            m, cr = O.motion_search_ref(cfg, cur, pls[r], 16 * x, 16 * y, [O.predict_mv(a, bb, c, r)])
+            cr += int(lam_ref * rb[r])
+            if best is None or cr < best[2]:
+                best = (r, m, cr)
+        assert (int(ref[b]), tuple(mv[b]), int(cost[b])) == best, b
+    if nrefs > 1:
+        assert len(set(ref.tolist())) > 1  # the case exercises the reference choice
+
+
 # ---------------------------------------------------------------- RC complexity
 
 def _rc_textbook(y, ref):
