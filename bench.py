@@ -114,6 +114,31 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
                          "refine": round(float(ph_mean[2]), 4)}}
 
 
+def run_decision(dev, args, cur, planes, lam, nf):
+    """The encoder-order 16x16 decision of the 15 P-frames (rh264_motion_search_wavefront:
+    one reference, each macroblock's predictor predict_mv of its decided neighbours,
+    diamond rungs 16/8/4 and the iterated ring-8 sub-pel walk), timed with HIP events."""
+    from rusty_h264_amd import accel
+    p = accel.search_params(W, H, PAD, 16, 16, R, 0, lam)
+    out = accel.motion_search_wavefront(dev, p, nf, cur, planes)
+    for _ in range(max(1, args.warmup)):
+        accel.motion_search_wavefront(dev, p, nf, cur, planes, out=out)
+    steps = max(1, min(args.steps, 10))
+    dev.sync()
+    dev.record(0)
+    for _ in range(steps):
+        accel.motion_search_wavefront(dev, p, nf, cur, planes, out=out)
+    dev.record(1)
+    dev.sync()
+    ms = dev.elapsed_ms(0, 1) / steps
+    cost = dev.download(out[2], (nf * (W // 16) * (H // 16),), np.int32)
+    for x in out:
+        x.free()
+    return {"mpx_s": round(W * H * nf / (ms / 1e3) / 1e6, 2), "ms_per_step": round(ms, 4), "steps": steps,
+            "mean_cost": float(cost.mean()),
+            "what": "encoder-order decision: 16x16, 1 ref, predict_mv predictors, diamond 16/8/4, ring-8 walk"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +146,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal path")
+    ap.add_argument("--no-decision", action="store_true", help="skip the encoder-order decision path")
     ap.add_argument("--fallback-cost", type=int, default=None, help="learned path: fall-back threshold")
     ap.add_argument("--la-fb", type=int, default=LA_FALLBACK_SAD,
                     help="lookahead4k: 8x8 SAD above which a learned-seeded block falls back to the full search")
@@ -194,6 +220,7 @@ def main():
             learned = run_learned(dev, args, cur, refs, planes, p, res["cost"])
             # the same proposal refined by the plain search's ring walk, for comparison
             learned_ring = run_learned(dev, args, cur, refs, planes, p, res["cost"], pat=0)
+        decision = None if args.no_decision else run_decision(dev, args, cur, planes, lam, nf)
 
     ms_per_step = ev_ms / args.steps
     mpx = W * H * nf / (ms_per_step / 1e3) / 1e6
@@ -219,7 +246,7 @@ def main():
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
         "mean_cost": float(res["cost"].mean()),
         "paths": {"plain": {"mpx_s": round(mpx, 2), "ms_per_step": round(ms_per_step, 4), "regret": 0.0},
-                  "learned": learned, "learned_ring": learned_ring},
+                  "learned": learned, "learned_ring": learned_ring, "decision": decision},
     }
     if learned is not None and learned["regret"] <= MAX_REGRET and learned["mpx_s"] > mpx:
         # the headline is the faster path that meets the regret bar

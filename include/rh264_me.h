@@ -190,8 +190,12 @@ int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases
  * variant, 2 = v_sad_u8 with one candidate per lane, 0 = quad-SADs; "planes_rw" 1 (default) = the plane build slides a
  * register window down 256x32 tiles, 0 = the LDS-tile kernel; "mvnet_fused" 1
  * (default) = network inference
- * in one fused kernel per tile, 0 = layer by layer.  Every search variant returns
- * identical results; the two network paths agree up to FP32 summation order. */
+ * in one fused kernel per tile, 0 = layer by layer; "decide_mode" of
+ * rh264_motion_search_wavefront: 0 = one launch per anti-diagonal, 1 = fixed-point
+ * passes (each re-searches the macroblocks whose predictors changed; the stream
+ * is synchronised every 8 passes to test for the fixed point), 2 (default) = 1
+ * for one or two references, else 0.  Every search variant returns identical
+ * results; the two network paths agree up to FP32 summation order. */
 int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
 
 /* Quarter-pel prediction of blocks (mc_luma, inter.rs:1193-1262, served from
@@ -231,8 +235,12 @@ int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh26
  * cost + (int64)(lambda_ref * ref_bits(r, nrefs)) (mb16.rs:5967-5980), the first
  * minimum in reference order.  planes holds nframes + nrefs - 1 plane sets of
  * consecutive frames, planes_stride bytes apart; frame f's reference r (0 = the
- * most recent) is plane set f + nrefs - 1 - r.  The frames are decided as a
- * wavefront: one launch per anti-diagonal x + 2y = t, all frames at once.
+ * most recent) is plane set f + nrefs - 1 - r.  All frames are decided at once,
+ * by one of two exact schedules (rh264_tune "decide_mode"): one launch per
+ * anti-diagonal x + 2y = t, whose macroblocks' neighbours lie on earlier ones; or
+ * fixed-point passes that re-search the macroblocks whose predictors changed
+ * until none does (the encoder-order result is the only fixed point; this
+ * schedule synchronises the stream).
  * Outputs per macroblock: ref, mv (pair) and cost, int32.  16x16 only; nrefs
  * 1..4. */
 int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc,

@@ -50,4 +50,8 @@ struct rh264_ctx {
     size_t la_cap;
     void *lal_buf;                // proposals, vectors and fall-back list of its learned-seeded form
     size_t lal_cap;
+    void *ms_buf;                 // predictors, work list and counters of the fixed-point decision
+    size_t ms_cap;
+    int decide_mode;              // encoder-order decision: 0 = one launch per anti-diagonal, 1 = fixed-point
+                                  // passes, 2 = the fixed point for 1-2 references, else the anti-diagonals
 };

@@ -1926,6 +1926,112 @@ __global__ __launch_bounds__(256) void k_ms_wavefront(rh264_search_params p, rh2
     }
 }
 
+// The same decision as a fixed point.  S = the (ref, mv) of every macroblock.  A
+// pass re-searches exactly the macroblocks whose predictors, computed from S,
+// differ from the predictors their current result was searched with.  The
+// encoder-order decision is the only S with no such macroblock (by induction over
+// the anti-diagonals), and after pass k every macroblock on the first k
+// anti-diagonals holds its final result, so at most (number of anti-diagonals + 1)
+// passes reach it; vectors that settle early end it much sooner.
+//
+// Check: one thread per macroblock of every frame, predictors from S for each
+// reference; a changed macroblock stores its new predictors in used[i][r] and
+// joins the work list (one atomic per wave).
+__global__ __launch_bounds__(256) void k_ms_check(int nbx, int nblk, int ntot, int nrefs,
+                                                  const int32_t *__restrict__ ref, const int32_t *__restrict__ mv,
+                                                  int32_t *__restrict__ used, int32_t *__restrict__ list,
+                                                  uint32_t *__restrict__ count)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    bool dirty = false;
+    if (i < ntot) {
+        const int frame = i / nblk, b = i - frame * nblk;
+        for (int r = 0; r < nrefs; r++) {
+            int px, py;
+            ms_predict_mv(mv + (size_t)frame * nblk * 2, ref + (size_t)frame * nblk, nbx, b % nbx, b / nbx, r, px, py);
+            int32_t *u = used + (size_t)i * 8 + 2 * r;
+            if (u[0] != px || u[1] != py) {
+                u[0] = px;
+                u[1] = py;
+                dirty = true;
+            }
+        }
+    }
+    const uint64_t m = __ballot(dirty);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (dirty) list[base + below] = i;
+}
+
+// Search of the listed macroblocks with their stored predictors: RW waves per
+// macroblock, wave r searching reference r, the workgroups striding over the list
+// (count entries, written by the check just before).  Block 0 clears the other
+// counter for the next check.
+template <bool T8, int RW>
+__global__ __launch_bounds__(256) void k_ms_listed(rh264_search_params p, rh264_diamond dc,
+                                                   const uint8_t *__restrict__ cur, size_t cur_stride,
+                                                   const uint8_t *__restrict__ planes, size_t planes_stride,
+                                                   int nrefs, MsRefCost refc, const int32_t *__restrict__ used,
+                                                   const int32_t *__restrict__ list, const uint32_t *__restrict__ count,
+                                                   uint32_t *__restrict__ next_count, int32_t *__restrict__ out_ref,
+                                                   int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost)
+{
+    __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
+    __shared__ int32_t s_res[4][3];
+    if (threadIdx.x < RH264_LR_N) {
+        s_lr[threadIdx.x] = p.lr[threadIdx.x];
+        s_lrfp[threadIdx.x] = p.lr_fp[threadIdx.x];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+    __syncthreads();
+    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int per_wg = 4 / RW, slot = wave / RW, rf = wave % RW;
+    const int n = (int)__builtin_amdgcn_readfirstlane(*count);
+    for (int e0 = (int)blockIdx.x * per_wg; e0 < n; e0 += (int)gridDim.x * per_wg) { // workgroup-uniform
+        const int e = e0 + slot;
+        const bool ok = e < n;
+        const int i = ok ? __builtin_amdgcn_readfirstlane(list[e]) : 0;
+        const int frame = i / nblk, b = i - frame * nblk;
+        if (ok && rf < nrefs) {
+            int bx, by, bc;
+            ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
+                          planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, (b % nbx) << 4, (b / nbx) << 4,
+                          used + (size_t)i * 8 + 2 * rf, 1, lane, bx, by, bc);
+            bc += refc.c[rf];
+            if constexpr (RW == 1) {
+                if (lane == 0) {
+                    out_ref[i] = 0;
+                    out_mv[2 * i] = bx;
+                    out_mv[2 * i + 1] = by;
+                    out_cost[i] = bc;
+                }
+            } else if (lane == 0) {
+                s_res[wave][0] = bx;
+                s_res[wave][1] = by;
+                s_res[wave][2] = bc;
+            }
+        }
+        if constexpr (RW > 1) {
+            __syncthreads();
+            if (ok && rf == 0 && lane == 0) {
+                int br = 0;
+                for (int q = 1; q < nrefs; q++)
+                    if (s_res[wave + q][2] < s_res[wave + br][2]) br = q;
+                out_ref[i] = br;
+                out_mv[2 * i] = s_res[wave + br][0];
+                out_mv[2 * i + 1] = s_res[wave + br][1];
+                out_cost[i] = s_res[wave + br][2];
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Rate-control frame complexity (lookahead::complexity, lookahead.rs:38-101), one
 // wave per macroblock, with the refinement's matrix-core 4x4 Hadamards:
@@ -3364,6 +3470,9 @@ int rh264_init(int device, rh264_ctx **out)
     c->la_cap = 0;
     c->lal_buf = nullptr;
     c->lal_cap = 0;
+    c->ms_buf = nullptr;
+    c->ms_cap = 0;
+    c->decide_mode = 2;
     c->fs_kernel = 0;
     c->refine_lds = 1;
     c->refine_mfma = 1;
@@ -3391,6 +3500,7 @@ int rh264_destroy(rh264_ctx *ctx)
     if (ctx->ls_buf) (void)hipFree(ctx->ls_buf);
     if (ctx->la_buf) (void)hipFree(ctx->la_buf);
     if (ctx->lal_buf) (void)hipFree(ctx->lal_buf);
+    if (ctx->ms_buf) (void)hipFree(ctx->ms_buf);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
     for (hipEvent_t e : ctx->prof) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -3769,6 +3879,11 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
         ctx->planes_rw = value ? 1 : 0;
         return 0;
     }
+    if (!strcmp(knob, "decide_mode")) {
+        if (value < 0 || value > 2) return fail("decide_mode %d: 0, 1 or 2", value);
+        ctx->decide_mode = value;
+        return 0;
+    }
     if (!strcmp(knob, "fallback_v")) {
         if (value < 0 || value > 2) return fail("fallback_v %d: 0, 1 or 2", value);
         ctx->fallback_v = value;
@@ -3894,6 +4009,7 @@ int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh26
 }
 
 static int ref_bits(int r, int nrefs);
+static int ensure_buf(rh264_ctx *ctx, void **buf, size_t *cap, size_t bytes);
 
 #define MS_WF_LAUNCH(T8, RW)                                                                                  \
     hipLaunchKernelGGL((k_ms_wavefront<T8, RW>), grid, dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes, \
@@ -3920,15 +4036,63 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
     MsRefCost refc{};
     for (int r = 0; r < nrefs; r++) refc.c[r] = (int32_t)(int64_t)(lambda_ref * (double)ref_bits(r, nrefs));
     const int rw = nrefs == 1 ? 1 : nrefs == 2 ? 2 : 4, per_wg = 4 / rw;
-    const int nbx = p->cw / 16, nby = p->ch / 16;
+    const int nbx = p->cw / 16, nby = p->ch / 16, ndiag = (nbx - 1) + 2 * (nby - 1) + 1;
+    const bool t8 = p->hadamard == 8;
+    // mode 2 picks by measurement at 1080p: the fixed point for one or two
+    // references, where few passes settle it, the anti-diagonals for more
+    if (ctx->decide_mode == 1 || (ctx->decide_mode == 2 && nrefs <= 2)) {
+        // fixed-point passes: S starts at (ref 0, mv 0) everywhere and no predictor
+        // matches the sentinel, so the first pass searches every macroblock
+        const size_t ntot = (size_t)nframes * nbx * nby;
+        if (ntot > (size_t)INT32_MAX / 8) return fail("motion_search_wavefront: %zu macroblocks", ntot);
+        rc = ensure_buf(ctx, &ctx->ms_buf, &ctx->ms_cap, ntot * 9 * 4 + 256);
+        if (rc) return rc;
+        int32_t *used = (int32_t *)ctx->ms_buf, *list = used + ntot * 8;
+        uint32_t *counts = (uint32_t *)(list + ntot); // [2], alternating between passes
+        const unsigned fg = (unsigned)((ntot * 8 + 255) / 256), ng = (unsigned)((ntot + 255) / 256);
+        hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, ctx->stream, (unsigned *)used, ntot * 8, 0x80000000u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(ng), dim3(256), 0, ctx->stream, (unsigned *)ref, ntot, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(2 * ng), dim3(256), 0, ctx->stream, (unsigned *)mv, ntot * 2, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts, (size_t)2, 0u);
+        LAUNCH_CHECK("k_fill_u32");
+        // enough workgroups to fill the chip on the first (full) pass
+        const unsigned sg = (unsigned)std::min<size_t>((ntot + per_wg - 1) / per_wg, 256 * 16);
+        const int chunk = 8;
+        for (int k = 0; k <= ndiag;) {
+            for (int c = 0; c < chunk && k <= ndiag; c++, k++) {
+                uint32_t *cnt = counts + (k & 1), *nxt = counts + ((k + 1) & 1);
+                hipLaunchKernelGGL(k_ms_check, dim3(ng), dim3(256), 0, ctx->stream, nbx, nbx * nby, (int)ntot, nrefs,
+                                   (const int32_t *)ref, (const int32_t *)mv, used, list, cnt);
+                LAUNCH_CHECK("k_ms_check");
+#define MS_FP_LAUNCH(T8, RW)                                                                                          \
+    hipLaunchKernelGGL((k_ms_listed<T8, RW>), dim3(sg), dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes,   \
+                       planes_stride, nrefs, refc, (const int32_t *)used, (const int32_t *)list,                        \
+                       (const uint32_t *)cnt, nxt, ref, mv, cost)
+                if (rw == 1) {
+                    if (t8) MS_FP_LAUNCH(true, 1); else MS_FP_LAUNCH(false, 1);
+                } else if (rw == 2) {
+                    if (t8) MS_FP_LAUNCH(true, 2); else MS_FP_LAUNCH(false, 2);
+                } else {
+                    if (t8) MS_FP_LAUNCH(true, 4); else MS_FP_LAUNCH(false, 4);
+                }
+#undef MS_FP_LAUNCH
+                LAUNCH_CHECK("k_ms_listed");
+            }
+            // the last check's count (the search clears only the other counter)
+            uint32_t last = 1;
+            HIP_TRY(hipMemcpyAsync(&last, counts + ((k - 1) & 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            if (last == 0) return 0;
+        }
+        return fail("motion_search_wavefront: no fixed point after %d passes", ndiag + 1);
+    }
     // anti-diagonals x + 2y = t: each macroblock's left, above and above-right
     // (or above-left) neighbours lie on earlier ones
-    for (int t = 0; t <= (nbx - 1) + 2 * (nby - 1); t++) {
+    for (int t = 0; t < ndiag; t++) {
         const int y_lo = std::max(0, (t - (nbx - 1) + 1) / 2), y_hi = std::min(nby - 1, t / 2);
         const int n_t = y_hi - y_lo + 1;
         if (n_t <= 0) continue;
         const dim3 grid((unsigned)((n_t + per_wg - 1) / per_wg), nframes);
-        const bool t8 = p->hadamard == 8;
         if (rw == 1) {
             if (t8) MS_WF_LAUNCH(true, 1); else MS_WF_LAUNCH(false, 1);
         } else if (rw == 2) {

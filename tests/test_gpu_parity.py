@@ -421,21 +421,31 @@ def test_motion_search_wavefront(dev, cw, ch, nf, nrefs, lam_ref, steps, sadfp, 
     pl = accel.build_planes(dev, refs, nf + nrefs - 1, cw, ch, pad)
     p = accel.search_params(cw, ch, pad, 16, 16, 16, pat, LAM26, hadamard=had)
     nb = (cw // 16) * (ch // 16)
-    print(f"wavefront {cw}x{ch} frames {nf} refs {nrefs} steps {steps} pat {pat} had {had}", flush=True)
-    out = accel.motion_search_wavefront(dev, p, nf, cur, pl, nrefs=nrefs, lambda_ref=lam_ref, steps=steps,
-                                        sadfp=sadfp, snap=snap)
-    ref = dev.download(out[0], (nf * nb,), np.int32)
-    mv = dev.download(out[1], (nf * nb, 2), np.int32)
-    cost = dev.download(out[2], (nf * nb,), np.int32)
     cfg = O.cfg(cw, ch, pad, 16, 16, 16, pat, LAM26, hadamard=had)
     opl = [O.planes(clip[k], cw, ch, pad) for k in range(nf + nrefs - 1)]
-    for f in range(nf):
-        w_ref, w_mv, w_c = O.motion_search_frame(cfg, clip[f + nrefs], [opl[f + nrefs - 1 - r] for r in range(nrefs)],
-                                                 lambda_ref=lam_ref, steps=steps, sadfp=sadfp, snap=snap)
-        sl = slice(f * nb, (f + 1) * nb)
-        bad = np.nonzero((ref[sl] != w_ref) | (mv[sl] != w_mv).any(1) | (cost[sl] != w_c))[0]
-        assert bad.size == 0, (f, bad[:8], ref[sl][bad[:4]], w_ref[bad[:4]], mv[sl][bad[:4]], w_mv[bad[:4]])
-        if nrefs > 1 and f == 0:
-            assert len(set(w_ref.tolist())) > 1
-    for x in (cur, refs, pl) + tuple(out):
+    want = [O.motion_search_frame(cfg, clip[f + nrefs], [opl[f + nrefs - 1 - r] for r in range(nrefs)],
+                                  lambda_ref=lam_ref, steps=steps, sadfp=sadfp, snap=snap) for f in range(nf)]
+    if nrefs > 1:
+        assert len(set(want[0][0].tolist())) > 1  # the case exercises the reference choice
+    try:
+        for mode in (0, 1):  # one launch per anti-diagonal; fixed-point passes
+            dev.tune("decide_mode", mode)
+            print(f"wavefront {cw}x{ch} frames {nf} refs {nrefs} steps {steps} pat {pat} had {had} mode {mode}",
+                  flush=True)
+            out = accel.motion_search_wavefront(dev, p, nf, cur, pl, nrefs=nrefs, lambda_ref=lam_ref, steps=steps,
+                                                sadfp=sadfp, snap=snap)
+            ref = dev.download(out[0], (nf * nb,), np.int32)
+            mv = dev.download(out[1], (nf * nb, 2), np.int32)
+            cost = dev.download(out[2], (nf * nb,), np.int32)
+            for x in out:
+                x.free()
+            for f in range(nf):
+                w_ref, w_mv, w_c = want[f]
+                sl = slice(f * nb, (f + 1) * nb)
+                bad = np.nonzero((ref[sl] != w_ref) | (mv[sl] != w_mv).any(1) | (cost[sl] != w_c))[0]
+                assert bad.size == 0, (mode, f, bad[:8], ref[sl][bad[:4]], w_ref[bad[:4]], mv[sl][bad[:4]],
+                                       w_mv[bad[:4]])
+    finally:
+        dev.tune("decide_mode", 2)
+    for x in (cur, refs, pl):
         x.free()
