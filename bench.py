@@ -132,11 +132,29 @@ def run_decision(dev, args, cur, planes, lam, nf):
     dev.sync()
     ms = dev.elapsed_ms(0, 1) / steps
     cost = dev.download(out[2], (nf * (W // 16) * (H // 16),), np.int32)
-    for x in out:
+    # the step after: P_Skip vectors and both luma predictions of every macroblock
+    nblk = nf * (W // 16) * (H // 16)
+    smv, pred, spred = dev.alloc(8 * nblk), dev.alloc(W * H * nf), dev.alloc(W * H * nf)
+    pstride = accel.planes_bytes(W, H, PAD)
+
+    def predict():
+        rc = dev.lib.rh264_decision_predict(dev.ctx, W, H, PAD, nf, 1, int(planes), pstride, out[0].ptr, out[1].ptr,
+                                            smv.ptr, pred.ptr, spred.ptr)
+        assert rc == 0, dev.lib.rh264_last_error()
+    predict()
+    dev.sync()
+    dev.record(0)
+    for _ in range(steps):
+        predict()
+    dev.record(1)
+    dev.sync()
+    ms_pred = dev.elapsed_ms(0, 1) / steps
+    for x in tuple(out) + (smv, pred, spred):
         x.free()
     return {"mpx_s": round(W * H * nf / (ms / 1e3) / 1e6, 2), "ms_per_step": round(ms, 4), "steps": steps,
-            "mean_cost": float(cost.mean()),
-            "what": "encoder-order decision: 16x16, 1 ref, predict_mv predictors, diamond 16/8/4, ring-8 walk"}
+            "mean_cost": float(cost.mean()), "predict_ms": round(ms_pred, 4),
+            "what": "encoder-order decision: 16x16, 1 ref, predict_mv predictors, diamond 16/8/4, ring-8 walk; "
+                    "predict_ms: P_Skip vectors + MC and P_Skip luma predictions of the decided frames"}
 
 
 def main():

@@ -248,6 +248,20 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
                                   const uint8_t *planes, size_t planes_stride, int32_t *ref, int32_t *mv,
                                   int32_t *cost);
 
+/* The luma predictions of decided P frames (the step after the decision): per
+ * macroblock the P_Skip vector skip_mv (skip_mv, mb16.rs:1841-1853: (0,0) when the
+ * left or above neighbour is unavailable or is a zero vector of reference 0, else
+ * predict_mv(A, B, C, 0), inter.rs:36-73), pred = the motion-compensated 16x16
+ * prediction at the decided (ref, mv), and skip_pred = the P_Skip prediction from
+ * reference 0 at skip_mv (mc_luma, inter.rs:1193-1262, as mc_luma_cached serves it
+ * from the plane sets, mb16.rs:4514-4541; any vector).  ref/mv as
+ * rh264_motion_search_wavefront writes them (ref values outside 0..nrefs-1 are
+ * clamped into it), planes in its layout.  pred and skip_pred: nframes frames of
+ * cw x ch bytes, cw*ch apart; skip_mv: int32 pairs per macroblock. */
+int rh264_decision_predict(rh264_ctx *ctx, int cw, int ch, int pad, int nframes, int nrefs, const uint8_t *planes,
+                           size_t planes_stride, const int32_t *ref, const int32_t *mv, int32_t *skip_mv,
+                           uint8_t *pred, uint8_t *skip_pred);
+
 /* ---- batched multi-stream search ------------------------------------------
  * Many streams of uneven size, 1..4 references each, several partition shapes,
  * searched together: for every shape, one integer-search launch over all

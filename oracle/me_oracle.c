@@ -797,6 +797,41 @@ void orc_motion_search_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *d
         }
 }
 
+/* skip_mv (mb16.rs:1841-1853) from the neighbours in predict_mv's layout. */
+void orc_skip_mv(const int32_t nb[12], int32_t out[2])
+{
+    const int32_t *a = nb, *b = nb + 4;
+    if (!a[0] || !b[0] || (a[3] == 0 && a[1] == 0 && a[2] == 0) || (b[3] == 0 && b[1] == 0 && b[2] == 0)) {
+        out[0] = out[1] = 0;
+        return;
+    }
+    orc_predict_mv(nb, 0, out);
+}
+
+void orc_decision_predict(int cw, int ch, int nrefs, const uint8_t *const *refs, const int32_t *ref,
+                          const int32_t *mv, int32_t *skip_mv, uint8_t *pred, uint8_t *skip_pred)
+{
+    const int nbx = cw / 16, nby = ch / 16;
+    uint8_t blk[256];
+    for (int y = 0; y < nby; y++)
+        for (int x = 0; x < nbx; x++) {
+            const int i = y * nbx + x;
+            int32_t nb[12] = {0, 0, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1};
+            int ia = i - 1, ib = i - nbx, ic, cav;
+            if (x > 0) { nb[0] = 1; nb[1] = mv[2 * ia]; nb[2] = mv[2 * ia + 1]; nb[3] = ref[ia]; }
+            if (y > 0) { nb[4] = 1; nb[5] = mv[2 * ib]; nb[6] = mv[2 * ib + 1]; nb[7] = ref[ib]; }
+            if (y > 0 && x + 1 < nbx) { ic = ib + 1; cav = 1; }
+            else { ic = ib - 1; cav = x > 0 && y > 0; }
+            if (cav) { nb[8] = 1; nb[9] = mv[2 * ic]; nb[10] = mv[2 * ic + 1]; nb[11] = ref[ic]; }
+            orc_skip_mv(nb, skip_mv + 2 * i);
+            int r = ref[i] < 0 ? 0 : ref[i] >= nrefs ? nrefs - 1 : ref[i];
+            orc_mc_luma(refs[r], cw, ch, 16 * x, 16 * y, 16, 16, mv[2 * i], mv[2 * i + 1], blk);
+            for (int k = 0; k < 16; k++) memcpy(pred + (size_t)(16 * y + k) * cw + 16 * x, blk + 16 * k, 16);
+            orc_mc_luma(refs[0], cw, ch, 16 * x, 16 * y, 16, 16, skip_mv[2 * i], skip_mv[2 * i + 1], blk);
+            for (int k = 0; k < 16; k++) memcpy(skip_pred + (size_t)(16 * y + k) * cw + 16 * x, blk + 16 * k, 16);
+        }
+}
+
 /* ---------------------------------------------------------------------------
  * mb-tree lookahead
  * ------------------------------------------------------------------------- */

@@ -128,6 +128,16 @@ void orc_motion_search_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *d
                              const uint8_t *const *planes, int nrefs, double lambda_ref, int32_t *ref,
                              int32_t *mv, int64_t *cost);
 
+/* skip_mv (mb16.rs:1841-1853): (0,0) when A or B is unavailable or a zero vector of
+ * reference 0, else predict_mv(A, B, C, 0); nb as orc_predict_mv's. */
+void orc_skip_mv(const int32_t nb[12], int32_t out[2]);
+/* The luma predictions of a decided P frame: per macroblock the P_Skip vector of the
+ * decided neighbours, mc_luma (inter.rs:1193-1262) of the frame refs[ref] at mv into
+ * pred and of refs[0] at the skip vector into skip_pred (cw x ch each). refs: the
+ * unpadded reference frames, most recent first. */
+void orc_decision_predict(int cw, int ch, int nrefs, const uint8_t *const *refs, const int32_t *ref,
+                          const int32_t *mv, int32_t *skip_mv, uint8_t *pred, uint8_t *skip_pred);
+
 /* ---- mb-tree lookahead: crates/rusty_h264-encoder/src/mbtree.rs ------------- */
 void orc_downsample2x(const uint8_t *y, int cw, int ch, uint8_t *out);
 int32_t orc_intra_cost(const uint8_t *sy, int cw, int bx0, int by0, int bs);

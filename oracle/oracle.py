@@ -203,6 +203,28 @@ class Oracle:
                                          C.c_double(lambda_ref), _p(ref, i32p), _p(mv, i32p), _p(c, i64p))
         return ref, mv, c
 
+    def skip_mv(self, a, b, c):
+        """skip_mv (mb16.rs:1841-1853); neighbours as predict_mv's."""
+        nb = np.zeros(12, np.int32)
+        for k, n in enumerate((a, b, c)):
+            nb[4 * k:4 * k + 4] = (0, 0, 0, -1) if n is None else (int(n[0]), n[1][0], n[1][1], n[2])
+        out = np.empty(2, np.int32)
+        self.lib.orc_skip_mv(_p(nb, i32p), _p(out, i32p))
+        return int(out[0]), int(out[1])
+
+    def decision_predict(self, cw, ch, refs, ref, mv):
+        """(skip_mv (nblk, 2), pred (ch, cw), skip_pred (ch, cw)) of one decided frame;
+        refs: unpadded reference frames, most recent first."""
+        rs = [_u8(x) for x in refs]
+        arr = (C.c_void_p * len(rs))(*[x.ctypes.data for x in rs])
+        nb = (cw // 16) * (ch // 16)
+        smv = np.empty((nb, 2), np.int32)
+        pred = np.empty((ch, cw), np.uint8)
+        sp = np.empty((ch, cw), np.uint8)
+        self.lib.orc_decision_predict(cw, ch, len(rs), arr, _p(np.ascontiguousarray(ref, np.int32), i32p),
+                                      _p(np.ascontiguousarray(mv, np.int32), i32p), _p(smv, i32p), _p(pred), _p(sp))
+        return smv, pred, sp
+
     # ---- mb-tree
     def downsample2x(self, y, cw, ch):
         out = np.empty((ch // 2, cw // 2), np.uint8)

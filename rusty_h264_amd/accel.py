@@ -432,6 +432,24 @@ def motion_search_wavefront(dev, params, nframes, cur, planes, nrefs=1, lambda_r
     return out
 
 
+def decision_predict(dev, cw, ch, pad, nframes, nrefs, planes, ref, mv, planes_stride=None):
+    """rh264_decision_predict over decided frames (ref, mv device buffers as
+    motion_search_wavefront returns them): (skip_mv [n][2], pred [f][ch][cw],
+    skip_pred [f][ch][cw]) as host arrays (P_Skip vector mb16.rs:1841-1853, luma MC
+    inter.rs:1193-1262)."""
+    n = (cw // 16) * (ch // 16) * nframes
+    smv, pr, sp = dev.alloc(8 * n), dev.alloc(cw * ch * nframes), dev.alloc(cw * ch * nframes)
+    try:
+        _check(dev.lib.rh264_decision_predict(dev.ctx, cw, ch, pad, nframes, nrefs, int(planes),
+                                              planes_stride or planes_bytes(cw, ch, pad), ref.ptr, mv.ptr, smv.ptr,
+                                              pr.ptr, sp.ptr), "rh264_decision_predict")
+        return (dev.download(smv, (n, 2), np.int32), dev.download(pr, (nframes, ch, cw), np.uint8),
+                dev.download(sp, (nframes, ch, cw), np.uint8))
+    finally:
+        for x in (smv, pr, sp):
+            x.free()
+
+
 def rc_complexity(dev, cur, ref, nframes, cw, ch, frame_stride=None, out=None):
     """lookahead::complexity of nframes device-resident frames (ref None = IDR); returns
     the nframes scores as float64 (lookahead.rs:38-101)."""

@@ -2032,6 +2032,67 @@ __global__ __launch_bounds__(256) void k_ms_listed(rh264_search_params p, rh264_
     }
 }
 
+// The luma predictions of decided P frames, one thread per 4 pixels of a row:
+// the motion-compensated 16x16 prediction of each macroblock at its decided (ref,
+// mv), and its P_Skip prediction from reference 0 at skip_mv (mb16.rs:1841-1853):
+// (0, 0) when A or B is unavailable or is a zero vector of reference 0, else
+// predict_mv(A, B, C, 0).  Samples are read from the plane sets with the operand
+// table and coordinates clamped into the padded plane, which equals mc_luma
+// (inter.rs:1193-1262) for any vector (mc_luma_cached, mb16.rs:4514-4541).
+__device__ __forceinline__ uint32_t dp_pred4(const uint8_t *pl, int pw, int ph, int x, int y, int mvx, int mvy)
+{
+    const size_t np = (size_t)pw * ph;
+    const uint32_t q = qtab((mvx & 3) * 4 + (mvy & 3));
+    const uint8_t *pa = pl + np * (q & 15), *pb = pl + np * ((q >> 12) & 15);
+    const int px = x + (mvx >> 2), py = y + (mvy >> 2);
+    const uint8_t *ra = pa + (size_t)clampi(py + ((q >> 8) & 15), 0, ph - 1) * pw;
+    const uint8_t *rb = pb + (size_t)clampi(py + ((q >> 20) & 15), 0, ph - 1) * pw;
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int v = ra[clampi(px + k + ((q >> 4) & 15), 0, pw - 1)];
+        if ((mvx | mvy) & 1) v = (v + rb[clampi(px + k + ((q >> 16) & 15), 0, pw - 1)] + 1) >> 1;
+        o |= (uint32_t)v << (8 * k);
+    }
+    return o;
+}
+
+__global__ __launch_bounds__(256) void k_decision_predict(int cw, int ch, int pad, int nrefs,
+                                                          const uint8_t *__restrict__ planes, size_t planes_stride,
+                                                          const int32_t *__restrict__ ref,
+                                                          const int32_t *__restrict__ mv,
+                                                          int32_t *__restrict__ skip_mv, uint8_t *__restrict__ pred,
+                                                          uint8_t *__restrict__ skip_pred)
+{
+    const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x); // 4-pixel group of the frame
+    const int frame = (int)blockIdx.y;
+    if (g >= cw * ch / 4) return;
+    const int y = (g * 4) / cw, x = g * 4 - y * cw;
+    const int nbx = cw >> 4, nblk = nbx * (ch >> 4), mx = x >> 4, my = y >> 4, b = my * nbx + mx;
+    const int32_t *fmv = mv + (size_t)frame * nblk * 2, *fref = ref + (size_t)frame * nblk;
+    // P_Skip vector from the decided neighbours
+    int sx = 0, sy = 0;
+    const bool av_a = mx > 0, av_b = my > 0;
+    if (av_a && av_b) {
+        const int ia = b - 1, ib = b - nbx;
+        const bool zero_a = fref[ia] == 0 && fmv[2 * ia] == 0 && fmv[2 * ia + 1] == 0;
+        const bool zero_b = fref[ib] == 0 && fmv[2 * ib] == 0 && fmv[2 * ib + 1] == 0;
+        if (!zero_a && !zero_b) ms_predict_mv(fmv, fref, nbx, mx, my, 0, sx, sy);
+    }
+    const int pw = cw + 2 * pad, ph = ch + 2 * pad;
+    const size_t boff = (size_t)frame * nblk + b;
+    if ((x & 15) == 0 && (y & 15) == 0) {
+        skip_mv[2 * boff] = sx;
+        skip_mv[2 * boff + 1] = sy;
+    }
+    const int r = clampi(fref[b], 0, nrefs - 1);
+    const uint8_t *pl_r = planes + (size_t)(frame + nrefs - 1 - r) * planes_stride;
+    const uint8_t *pl_0 = planes + (size_t)(frame + nrefs - 1) * planes_stride;
+    const size_t o = (size_t)frame * cw * ch + (size_t)y * cw + x;
+    *(uint32_t *)(pred + o) = dp_pred4(pl_r, pw, ph, x + pad, y + pad, fmv[2 * b], fmv[2 * b + 1]);
+    *(uint32_t *)(skip_pred + o) = dp_pred4(pl_0, pw, ph, x + pad, y + pad, sx, sy);
+}
+
 // ---------------------------------------------------------------------------
 // Rate-control frame complexity (lookahead::complexity, lookahead.rs:38-101), one
 // wave per macroblock, with the refinement's matrix-core 4x4 Hadamards:
@@ -4102,6 +4163,25 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
         }
         LAUNCH_CHECK("k_ms_wavefront");
     }
+    return 0;
+}
+
+int rh264_decision_predict(rh264_ctx *ctx, int cw, int ch, int pad, int nframes, int nrefs, const uint8_t *planes,
+                           size_t planes_stride, const int32_t *ref, const int32_t *mv, int32_t *skip_mv,
+                           uint8_t *pred, uint8_t *skip_pred)
+{
+    if (cw <= 0 || ch <= 0 || cw % 16 || ch % 16) return RH264_DECLINED;
+    if (pad < 4 || pad > 128 || pad % 4) return RH264_DECLINED;
+    if (nrefs < 1 || nrefs > 16) return RH264_DECLINED;
+    if (!planes || !ref || !mv || !skip_mv || !pred || !skip_pred) return fail("decision_predict: null pointer");
+    if (planes_stride < rh264_planes_bytes(cw, ch, pad)) return fail("decision_predict: planes stride too small");
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    if ((size_t)cw * ch / 4 > (size_t)INT32_MAX) return fail("decision_predict: frame too large");
+    const unsigned groups = (unsigned)((size_t)cw * ch / 4);
+    hipLaunchKernelGGL(k_decision_predict, dim3((groups + 255) / 256, nframes), dim3(256), 0, ctx->stream, cw, ch,
+                       pad, nrefs, planes, planes_stride, ref, mv, skip_mv, pred, skip_pred);
+    LAUNCH_CHECK("k_decision_predict");
     return 0;
 }
 

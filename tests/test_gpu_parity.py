@@ -449,3 +449,39 @@ def test_motion_search_wavefront(dev, cw, ch, nf, nrefs, lam_ref, steps, sadfp, 
         dev.tune("decide_mode", 2)
     for x in (cur, refs, pl):
         x.free()
+
+
+@pytest.mark.parametrize("source", ["decision", "random"])
+def test_decision_predict(dev, source):
+    """P_Skip vectors and the luma predictions of decided frames (the decision's own
+    output, or arbitrary references and vectors up to 150 px outside the frame) ==
+    the oracle's mc_luma on the unpadded reference frames."""
+    cw, ch, pad, nf, nrefs = 176, 144, 32, 2, 3
+    clip = synth.cif_clip(nf + nrefs, seed=77, w=cw, h=ch)
+    cur, refs = dev.upload(clip[nrefs:]), dev.upload(clip[:-1])
+    pl = accel.build_planes(dev, refs, nf + nrefs - 1, cw, ch, pad)
+    nb = (cw // 16) * (ch // 16)
+    if source == "decision":
+        p = accel.search_params(cw, ch, pad, 16, 16, 16, 0, LAM26)
+        ref_d, mv_d, cost_d = accel.motion_search_wavefront(dev, p, nf, cur, pl, nrefs=nrefs, lambda_ref=3.0)
+        cost_d.free()
+        ref = dev.download(ref_d, (nf * nb,), np.int32)
+        mv = dev.download(mv_d, (nf * nb, 2), np.int32)
+    else:
+        rng = np.random.default_rng(5)
+        ref = rng.integers(0, nrefs, nf * nb).astype(np.int32)
+        mv = rng.integers(-600, 601, (nf * nb, 2)).astype(np.int32)
+        zero = rng.random(nf * nb) < 0.2
+        mv[zero], ref[zero] = 0, 0
+        ref_d, mv_d = dev.upload(ref), dev.upload(mv)
+    print(f"decision_predict {source} {cw}x{ch} frames {nf} refs {nrefs}", flush=True)
+    smv, pred, sp = accel.decision_predict(dev, cw, ch, pad, nf, nrefs, pl, ref_d, mv_d)
+    for f in range(nf):
+        sl = slice(f * nb, (f + 1) * nb)
+        w_smv, w_pred, w_sp = O.decision_predict(cw, ch, [clip[f + nrefs - 1 - r] for r in range(nrefs)], ref[sl],
+                                                 mv[sl])
+        assert np.array_equal(smv[sl], w_smv), f
+        assert np.array_equal(pred[f], w_pred), (f, np.argwhere(pred[f] != w_pred)[:4])
+        assert np.array_equal(sp[f], w_sp), (f, np.argwhere(sp[f] != w_sp)[:4])
+    for x in (cur, refs, pl, ref_d, mv_d):
+        x.free()

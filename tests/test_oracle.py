@@ -478,6 +478,48 @@ def test_motion_search_frame_composes_best_part(cw, ch, nrefs, lam_ref):
         assert len(set(ref.tolist())) > 1  # the case exercises the reference choice
 
 
+def test_skip_mv_spec_cases():
+    """P_Skip vector (mb16.rs:1841-1853, spec 8.4.1.1)."""
+    z, m = _inter((0, 0)), _inter((8, -4))
+    assert O.skip_mv(None, m, m) == (0, 0)                 # A unavailable
+    assert O.skip_mv(m, None, m) == (0, 0)                 # B unavailable
+    assert O.skip_mv(z, m, m) == (0, 0)                    # A a zero vector of reference 0
+    assert O.skip_mv(m, z, m) == (0, 0)                    # B likewise
+    # a zero vector of another reference, or an intra neighbour, is not the zero rule
+    assert O.skip_mv(_inter((0, 0), 1), _inter((4, 4)), None, ) == (4, 4)
+    assert O.skip_mv((True, (0, 0), -1), _inter((4, 4)), _inter((12, 8))) == (4, 4)
+    # otherwise predict_mv(A, B, C, 0): the median of three matching neighbours
+    assert O.skip_mv(_inter((4, 0)), _inter((8, 2)), _inter((12, -6))) == (8, 0)
+
+
+def test_decision_predict_composes_mc_luma():
+    """The decided frame's predictions == mc_luma of each macroblock at its (ref, mv)
+    and of reference 0 at its P_Skip vector, with vectors far outside the frame."""
+    cw, ch = 64, 48
+    rng = np.random.default_rng(41)
+    refs = [rng.integers(0, 256, (ch, cw)).astype(np.uint8) for _ in range(2)]
+    nb = (cw // 16) * (ch // 16)
+    ref = rng.integers(0, 2, nb).astype(np.int32)
+    mv = rng.integers(-500, 501, (nb, 2)).astype(np.int32)
+    mv[[1, 5]] = 0
+    ref[[1, 5]] = 0  # zero vectors of reference 0 trigger the skip rule
+    smv, pred, sp = O.decision_predict(cw, ch, refs, ref, mv)
+    nbx = cw // 16
+    for b in range(nb):
+        x, y = b % nbx, b // nbx
+
+        def nbr(av, xx, yy):
+            i = yy * nbx + xx
+            return _inter(tuple(int(t) for t in mv[i]), int(ref[i])) if av else None
+        c = nbr(True, x + 1, y - 1) if (y > 0 and x + 1 < nbx) else nbr(x > 0 and y > 0, x - 1, y - 1)
+        want = O.skip_mv(nbr(x > 0, x - 1, y), nbr(y > 0, x, y - 1), c)
+        assert tuple(smv[b]) == want
+        blk = (slice(16 * y, 16 * y + 16), slice(16 * x, 16 * x + 16))
+        assert np.array_equal(pred[blk], O.mc_luma(refs[ref[b]], cw, ch, 16 * x, 16 * y, 16, 16, *mv[b]))
+        assert np.array_equal(sp[blk], O.mc_luma(refs[0], cw, ch, 16 * x, 16 * y, 16, 16, *want))
+    assert (smv == 0).all(1).sum() < nb  # not every skip vector is zero
+
+
 # ---------------------------------------------------------------- RC complexity
 
 def _rc_textbook(y, ref):
