@@ -262,6 +262,23 @@ int rh264_decision_predict(rh264_ctx *ctx, int cw, int ch, int pad, int nframes,
                            size_t planes_stride, const int32_t *ref, const int32_t *mv, int32_t *skip_mv,
                            uint8_t *pred, uint8_t *skip_pred);
 
+/* The mb-tree QP-offset policy of a GOP (gop_qp_offsets, crates/rusty_h264-encoder/
+ * src/mbtree.rs:369-593) from its frame costs, on the HOST (host pointers; the f64
+ * propagation is order-dependent and stays in the reference's order): f64 backward
+ * propagation (frames descending, raster order, mbtree.rs:268-297, 440-455); the
+ * predictability back-off (all zero when 1 - mean((intra-inter)/intra) over frames
+ * 1.. is below res_min > 0); offsets -strength*log2((intra + propagated)/intra)
+ * centred on the GOP mean; the differentiation latch (all zero when the RMS of the
+ * centred offsets / strength is below sd_min); round half away from zero, clamp to
+ * +-6.  intra/inter [nframes][mb_w*mb_h] with 1 <= intra, 0 <= inter <= intra (as
+ * rh264_mbtree_costs writes them), mv pairs alike; offsets the same shape; stats
+ * (may be NULL) = {residual fraction, spread, effective strength}.  The reference's
+ * defaults: strength 0.9, res_min 0.03 (RFF_MBTREE_RESMIN), sd_min = the
+ * configuration's mbtree_spread_min, 0.0 by default (config.rs:566). */
+int rh264_mbtree_qp_offsets(int nframes, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                            const int32_t *mv, double strength, double res_min, double sd_min, int32_t *offsets,
+                            double *stats);
+
 /* ---- batched multi-stream search ------------------------------------------
  * Many streams of uneven size, 1..4 references each, several partition shapes,
  * searched together: for every shape, one integer-search launch over all

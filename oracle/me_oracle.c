@@ -8,6 +8,7 @@
 #include "me_oracle.h"
 
 #include <limits.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1184,6 +1185,49 @@ void orc_propagate_f64(int n, int mb_w, int mb_h, const int32_t *intra, const in
                 propagate_to(prev, mb_w, mb_h, x, y, mv[2 * g], mv[2 * g + 1], total * frac);
             }
     }
+}
+
+/* gop_qp_offsets' policy (mbtree.rs:369-593) over given frame costs: f64
+ * propagation, back-off latch, centred -s*log2 offsets, spread latch, round/clamp. */
+void orc_mbtree_qp_offsets(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                           const int32_t *mv, double strength, double res_min, double sd_min, int32_t *off,
+                           double *stats)
+{
+    size_t nm = (size_t)mb_w * mb_h, N = (size_t)n * nm;
+    double resid = 0.0, sd = 0.0, eff = 0.0;
+    memset(off, 0, sizeof(int32_t) * N);
+    if (strength > 0.0 && N > 0) {
+        double *prop = (double *)malloc(sizeof(double) * N), *o = (double *)malloc(sizeof(double) * N);
+        orc_propagate_f64(n, mb_w, mb_h, intra, inter, mv, prop);
+        double fsum = 0.0, fc = 0.0;
+        for (int f = 1; f < n; f++)
+            for (size_t m = 0; m < nm; m++) {
+                size_t g = (size_t)f * nm + m;
+                fsum += (double)(intra[g] - inter[g]) / (double)intra[g];
+                fc += 1.0;
+            }
+        resid = 1.0 - (fc > 0.0 ? fsum / fc : 0.0);
+        eff = (res_min > 0.0 && resid < res_min) ? 0.0 : strength;
+        if (eff != 0.0) {
+            double sum = 0.0, sq = 0.0;
+            for (size_t g = 0; g < N; g++) {
+                o[g] = -eff * log2(((double)intra[g] + prop[g]) / (double)intra[g]);
+                sum += o[g];
+            }
+            double mean = sum / (double)N;
+            for (size_t g = 0; g < N; g++) o[g] -= mean;
+            for (size_t g = 0; g < N; g++) sq += o[g] * o[g];
+            sd = sqrt(sq / (double)N) / (eff > 1e-9 ? eff : 1e-9);
+            if (!(sd < sd_min))
+                for (size_t g = 0; g < N; g++) {
+                    double r = round(o[g]);
+                    off[g] = (int32_t)(r < -6.0 ? -6.0 : r > 6.0 ? 6.0 : r);
+                }
+        }
+        free(prop);
+        free(o);
+    }
+    if (stats) { stats[0] = resid; stats[1] = sd; stats[2] = eff; }
 }
 
 /* Build-defined int32 propagation: same geometry as propagate_to, integer amounts.

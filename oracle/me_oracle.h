@@ -153,6 +153,13 @@ void orc_frame_costs_half(const uint8_t *half, const uint8_t *ref_half, int mb_w
  * costs laid out [n][mb]; prop_out [n][mb] f64. */
 void orc_propagate_f64(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
                        const int32_t *mv, double *prop_out);
+/* gop_qp_offsets' policy (mbtree.rs:369-593) over frame costs [n][mb]: the f64
+ * propagation above, the back-off latch (res_min), -strength*log2(total/intra)
+ * centred on the GOP mean, the spread latch (sd_min), round half away from zero,
+ * clamp +-6.  stats (may be NULL): residual fraction, spread, effective strength. */
+void orc_mbtree_qp_offsets(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                           const int32_t *mv, double strength, double res_min, double sd_min, int32_t *off,
+                           double *stats);
 /* int32 fixed-point propagation (build-defined): amount_q = floor((intra+prop)
  * * (intra-inter) / intra) in int64 then per-target weights (wx*wy) >> 8. */
 void orc_propagate_i32(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,

@@ -226,6 +226,19 @@ class Oracle:
         return smv, pred, sp
 
     # ---- mb-tree
+    def mbtree_qp_offsets(self, intra, inter, mv, strength=0.9, res_min=0.03, sd_min=0.0):
+        """gop_qp_offsets' policy on frame costs [n][mb_h][mb_w]: (offsets, stats)."""
+        intra = np.ascontiguousarray(intra, np.int32)
+        inter = np.ascontiguousarray(inter, np.int32)
+        mv = np.ascontiguousarray(mv, np.int32)
+        n, mb_h, mb_w = intra.shape
+        off = np.empty_like(intra)
+        st = np.zeros(3, np.float64)
+        self.lib.orc_mbtree_qp_offsets(n, mb_w, mb_h, _p(intra, i32p), _p(inter, i32p), _p(mv, i32p),
+                                       C.c_double(strength), C.c_double(res_min), C.c_double(sd_min), _p(off, i32p),
+                                       st.ctypes.data_as(C.POINTER(C.c_double)))
+        return off, {"residual_frac": float(st[0]), "spread": float(st[1]), "eff_strength": float(st[2])}
+
     def downsample2x(self, y, cw, ch):
         out = np.empty((ch // 2, cw // 2), np.uint8)
         self.lib.orc_downsample2x(_p(_u8(y)), cw, ch, _p(out))

@@ -450,6 +450,23 @@ def decision_predict(dev, cw, ch, pad, nframes, nrefs, planes, ref, mv, planes_s
             x.free()
 
 
+def mbtree_qp_offsets(lib, intra, inter, mv, strength=0.9, res_min=0.03, sd_min=0.0):
+    """gop_qp_offsets' policy (mbtree.rs:369-593) on host arrays of frame costs
+    ([n][mb_h][mb_w] intra/inter, [n][mb_h][mb_w][2] mv), through the library's host
+    function rh264_mbtree_qp_offsets.  Returns (offsets int32 like intra, stats
+    {residual_frac, spread, eff_strength})."""
+    intra = np.ascontiguousarray(intra, np.int32)
+    inter = np.ascontiguousarray(inter, np.int32)
+    mv = np.ascontiguousarray(mv, np.int32)
+    n, mb_h, mb_w = intra.shape
+    off = np.empty_like(intra)
+    st = np.zeros(3, np.float64)
+    _check(lib.rh264_mbtree_qp_offsets(n, mb_w, mb_h, intra.ctypes.data, inter.ctypes.data, mv.ctypes.data,
+                                       float(strength), float(res_min), float(sd_min), off.ctypes.data,
+                                       st.ctypes.data), "rh264_mbtree_qp_offsets")
+    return off, {"residual_frac": float(st[0]), "spread": float(st[1]), "eff_strength": float(st[2])}
+
+
 def rc_complexity(dev, cur, ref, nframes, cw, ch, frame_stride=None, out=None):
     """lookahead::complexity of nframes device-resident frames (ref None = IDR); returns
     the nframes scores as float64 (lookahead.rs:38-101)."""

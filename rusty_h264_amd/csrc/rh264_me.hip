@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <cmath>
 #include <utility>
 #include <string>
 #include <vector>
@@ -4183,6 +4184,91 @@ int rh264_decision_predict(rh264_ctx *ctx, int cw, int ch, int pad, int nframes,
                        pad, nrefs, planes, planes_stride, ref, mv, skip_mv, pred, skip_pred);
     LAUNCH_CHECK("k_decision_predict");
     return 0;
+}
+
+// ---- mb-tree QP-offset policy (gop_qp_offsets, mbtree.rs:369-593), host f64 ----
+// The f64 propagation is order-dependent (frames descending, macroblocks in raster
+// order, four targets per block), so it runs on the host in exactly that order.
+static void mbt_propagate_to(double *prev, int mb_w, int mb_h, int mb_x, int mb_y, int mvx, int mvy, double amount)
+{
+#pragma clang fp contract(off) // every product and sum rounded on its own, as the reference's
+    if (amount <= 0.0) return; // mbtree.rs:268-297
+    const int rx = std::min(std::max(mb_x * 16 + (mvx >> 2), 0), (mb_w - 1) * 16);
+    const int ry = std::min(std::max(mb_y * 16 + (mvy >> 2), 0), (mb_h - 1) * 16);
+    const int cx0 = rx / 16, cy0 = ry / 16;
+    const double fx = (double)(rx % 16), fy = (double)(ry % 16);
+    const double wys[2] = {16.0 - fy, fy}, wxs[2] = {16.0 - fx, fx};
+    for (int dy = 0; dy < 2; dy++) {
+        if (wys[dy] <= 0.0) continue;
+        const int cy = std::min(cy0 + dy, mb_h - 1);
+        for (int dx = 0; dx < 2; dx++) {
+            if (wxs[dx] <= 0.0) continue;
+            const int cx = std::min(cx0 + dx, mb_w - 1);
+            prev[cy * mb_w + cx] += amount * (wxs[dx] * wys[dy]) / 256.0;
+        }
+    }
+}
+
+int rh264_mbtree_qp_offsets(int nframes, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
+                            const int32_t *mv, double strength, double res_min, double sd_min, int32_t *offsets,
+                            double *stats)
+{
+#pragma clang fp contract(off)
+    if (nframes < 0 || mb_w < 0 || mb_h < 0) return fail("mbtree_qp_offsets: negative size");
+    const size_t nm = (size_t)mb_w * mb_h, n = (size_t)nframes;
+    if (n == 0 || nm == 0) return 0;
+    if (!intra || !inter || !mv || !offsets) return fail("mbtree_qp_offsets: null pointer");
+    for (size_t i = 0; i < n * nm; i++)
+        if (intra[i] < 1 || inter[i] < 0 || inter[i] > intra[i])
+            return fail("mbtree_qp_offsets: costs need 1 <= intra and 0 <= inter <= intra (block %zu)", i);
+    std::fill(offsets, offsets + n * nm, 0);
+    double st[3] = {0.0, 0.0, 0.0}; // residual fraction, spread, effective strength
+    auto done = [&]() {
+        if (stats) std::copy(st, st + 3, stats);
+        return 0;
+    };
+    if (strength <= 0.0) return done();
+    // backward propagation (mbtree.rs:440-455)
+    std::vector<double> prop(n * nm, 0.0);
+    for (size_t f = n - 1; f >= 1; f--) {
+        const double *cur = prop.data() + f * nm;
+        double *prev = prop.data() + (f - 1) * nm;
+        for (int y = 0; y < mb_h; y++)
+            for (int x = 0; x < mb_w; x++) {
+                const size_t m = (size_t)y * mb_w + x, g = f * nm + m;
+                const double total = (double)intra[g] + cur[m];
+                const double frac = (double)(intra[g] - inter[g]) / (double)intra[g];
+                mbt_propagate_to(prev, mb_w, mb_h, x, y, mv[2 * g], mv[2 * g + 1], total * frac);
+            }
+    }
+    // predictability back-off: off below res_min (mbtree.rs:478-489)
+    double fsum = 0.0, fc = 0.0;
+    for (size_t g = nm; g < n * nm; g++) {
+        fsum += (double)(intra[g] - inter[g]) / (double)intra[g];
+        fc += 1.0;
+    }
+    st[0] = 1.0 - (fc > 0.0 ? fsum / fc : 0.0);
+    st[2] = (res_min > 0.0 && st[0] < res_min) ? 0.0 : strength;
+    if (st[2] == 0.0) return done();
+    // offsets -s log2(total / intra), centred on the GOP mean (mbtree.rs:506-527)
+    std::vector<double> offs(n * nm);
+    for (size_t g = 0; g < n * nm; g++) {
+        const double in = (double)intra[g];
+        offs[g] = -st[2] * std::log2((in + prop[g]) / in);
+    }
+    const double cnt = (double)(n * nm);
+    double sum = 0.0;
+    for (size_t g = 0; g < n * nm; g++) sum += offs[g];
+    const double mean = sum / cnt;
+    for (size_t g = 0; g < n * nm; g++) offs[g] -= mean;
+    // differentiation latch on the strength-invariant spread (mbtree.rs:528-583)
+    double sq = 0.0;
+    for (size_t g = 0; g < n * nm; g++) sq += offs[g] * offs[g];
+    st[1] = std::sqrt(sq / cnt) / std::max(st[2], 1e-9);
+    if (st[1] < sd_min) return done();
+    for (size_t g = 0; g < n * nm; g++) // round half away from zero, clamp to +-6 (mbtree.rs:584-592)
+        offsets[g] = (int32_t)std::min(std::max(std::round(offs[g]), -6.0), 6.0);
+    return done();
 }
 
 int rh264_rc_complexity(rh264_ctx *ctx, const uint8_t *cur, const uint8_t *ref, size_t frame_stride, int nframes,

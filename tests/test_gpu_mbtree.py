@@ -174,3 +174,25 @@ def test_mbtree_multi_learned_bit_exact(dev, cw, ch, n, nrefs, R, Rl, fb):
     assert np.array_equal(got["prop"], O.propagate_multi_i32(got["intra"], got["inter"], got["mv"], got["ref"],
                                                              mb_w, mb_h))
     fbuf.free()
+
+
+@pytest.mark.parametrize("kind,strength,res_min", [("cif", 0.9, 0.03), ("hd", 0.9, 0.03), ("cif", 2.0, 0.0)])
+def test_mbtree_policy_on_gpu_costs(dev, kind, strength, res_min):
+    """The QP-offset policy (rh264_mbtree_qp_offsets, host f64) applied to a GOP's
+    costs from the GPU == the oracle's policy on the oracle's costs."""
+    cw, ch, n = (352, 288, 8) if kind == "cif" else (1920, 1088, 4)
+    frames = synth.cif_clip(n, seed=21, w=cw, h=ch) if kind == "cif" else synth.hd_clip(n, seed=6, w=cw, h=ch)
+    _, intra, inter, mv, _ = _oracle_gop(frames, cw, ch)
+    fb = dev.upload(frames)
+    mt = accel.MbTree(dev, cw, ch, n)
+    mt.run(fb)
+    got = mt.results()
+    fb.free()
+    mb_w, mb_h = cw // 16, ch // 16
+    shp = (n, mb_h, mb_w)
+    off, st = accel.mbtree_qp_offsets(dev.lib, got["intra"].reshape(shp), got["inter"].reshape(shp),
+                                      got["mv"].reshape(shp + (2,)), strength, res_min)
+    want, wst = O.mbtree_qp_offsets(np.reshape(intra, shp), np.reshape(inter, shp), np.reshape(mv, shp + (2,)),
+                                    strength, res_min)
+    assert np.array_equal(off, want) and st == wst
+    assert off.any()  # the GOP's offsets are not latched off
