@@ -26,6 +26,9 @@ HBM_PEAK_GBS = 8000.0
 # byte abs-diff-adds: v_sad_u8 does 4 per lane per cycle at full rate, v_qsad_pk_u16_u8 16 at
 # quarter rate: 64 per SIMD per cycle x 1024 SIMDs x 2.4 GHz
 VALU_PEAK_TOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0  # dense matrix-core peaks
+FP16_PEAK_TFLOPS = 2500.0
+C_NET, K_NET = 32, 289     # the proposal network's channels and MV bins
 # HBM bytes of one k_full_search<16,16> launch of this workload, from rocprofv3 --pmc
 # FETCH_SIZE (33,046 KiB, x2: a plain 512 MiB copy kernel reads 262,160 KiB) and
 # WRITE_SIZE (1,912 KiB, x1)
@@ -105,7 +108,27 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
     net.close()
     pc = plain_cost.astype(np.int64)
     ph_mean = np.mean(np.array(phases), axis=0) if phases else [0.0, 0.0, 0.0]
+    # the learned step's phases against their rooflines (algorithmic work per step
+    # over the phase's event-timed duration on the library's stream)
+    nblk1 = (W // 16) * (H // 16)
+    w4, h4 = W // 4, H // 4
+    net_flop = nf * 2 * ((w4 // 2) * (h4 // 2) * C_NET * (9 * 2 + 3 * 9 * C_NET) + nblk1 * K_NET * C_NET)
+    n_fb = float(fb.sum())
+    cand_ops = nb * 8 * (2 * mvnet.LOCAL_RANGE + 1) ** 2 * 256 + n_fb * (2 * R + 1) ** 2 * 256
+    win_flop = nb * 25 * 2 * (16 * 16 * 16 * 2)  # two 16x16x16 products per window position
+    roof = []
+    for name, work, ms, peak, unit, bound in (
+            ("network (k_prep + k_fwd_fused)", net_flop, ph_mean[0], BF16_PEAK_TFLOPS, "TFLOP/s", "mfma"),
+            ("candidates (k_cand_search_m + fall-back k_full_search_list)", cand_ops, ph_mean[1], VALU_PEAK_TOPS,
+             "TOP/s", "valu"),
+            ("refine (k_refine_wn16; VALU-bound, matrix-core work shown)", win_flop, ph_mean[2], FP16_PEAK_TFLOPS,
+             "TFLOP/s", "valu")):
+        if ms > 0:
+            a = work / (ms / 1e3) / 1e12
+            roof.append({"phase": name, "bound": bound, "work_per_step": float(work), "ms": round(float(ms), 4),
+                         "achieved": round(a, 2), "peak": peak, "unit": unit, "frac": round(a / peak, 4)})
     return {"mpx_s": round(W * H * nf / (ms_step / 1e3) / 1e6, 2), "ms_per_step": round(ms_step, 4),
+            "roofline": roof,
             "regret": round(float(cost.sum() / pc.sum() - 1.0), 6),
             "blocks_worse": round(float((cost > pc).mean()), 5), "blocks_better": round(float((cost < pc).mean()), 5),
             "fallback_rate": round(float(fb.mean()), 5), "local_range": mvnet.LOCAL_RANGE, "fallback_cost": fb_cost,
@@ -256,6 +279,8 @@ def main():
                                "QP 26 lambda", "search": "plain", "block": "16x16", "range": R, "pad": PAD,
                    "device": dname},
         "roofline": {"bound": "valu", "kernel": "k_full_search<16,16>",
+                     "note": "the exhaustive search kernel (the plain path; the learned path's fall-back); the "
+                             "learned step's phases are in paths.learned.roofline",
                      "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
                      "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": FULL_SEARCH_TRAFFIC,
                      "traffic_unit": "bytes per launch", "algorithmic_bytes": nf * (W * H + (W + 2 * PAD) * (H + 2 * PAD)),
