@@ -402,9 +402,11 @@ def bench_lookahead4k(args):
     with accel.Device(0) as dev:
         dname = dev.name()
         fb = dev.upload(frames)
+        la_scaled = 1  # the library's default: distance-1 proposals scaled by the distance
         for kv in args.tune:
             k, v = kv.split("=")
             dev.tune(k, int(v))
+            la_scaled = int(v) if k == "la_scaled" else la_scaled
         mt = accel.MbTree(dev, w, h, n)
         mref = dev.alloc(4 * nm * n)
         mfb = dev.alloc(4 * nm * n)
@@ -472,15 +474,21 @@ def bench_lookahead4k(args):
         res_l = None
         if net is not None:
             ms_l, wall_l, ph_l, res_l = timed(2)
-            # the proposals of frames 1..3, for the parity spot check: the pairs (j + k, j)
+            # the proposals of frames 1..3, for the parity spot check: the pairs (j + k, j),
+            # or with la_scaled the pairs (j + 1, j) times the distance k, clamped to the
+            # library's bound P - 8 - local range
             cen = np.zeros((4, nrefs, nm, 8, 2), np.int32)
-            for k in range(1, nrefs + 1):
+            cmax = ((max(rng_la, 16 + LA_LOCAL_RANGE) + 8 + 3) & ~3) - 8 - LA_LOCAL_RANGE
+            for k in range(1, (1 if la_scaled else nrefs) + 1):
                 t8 = dev.alloc(4 * (4 - k) * nm * 8)
                 net.forward(int(fb) + k * w * h, int(fb), w * h, 4 - k, w, h, None, t8)
                 top = dev.download(t8, (4 - k, nm, 8), np.int32)
                 t8.free()
-                cen[k:, k - 1, :, :, 0] = (top % 17 - 8) * 2
-                cen[k:, k - 1, :, :, 1] = (top // 17 - 8) * 2
+                for kk in ((1, 2, 3)[:nrefs] if la_scaled else (k,)):
+                    cen[kk:, kk - 1, :, :, 0] = np.clip((top[kk - k:] % 17 - 8) * 2 * (kk if la_scaled else 1),
+                                                        -cmax, cmax)
+                    cen[kk:, kk - 1, :, :, 1] = np.clip((top[kk - k:] // 17 - 8) * 2 * (kk if la_scaled else 1),
+                                                        -cmax, cmax)
             net.close()
     o = Oracle(fast=True)
     halves = [o.downsample2x(frames[f], w, h) for f in range(4)]
@@ -528,8 +536,10 @@ def bench_lookahead4k(args):
                    "regret": round(float(inter_l.sum() / inter_ex.sum() - 1.0), 6),
                    "blocks_worse": round(float((inter_l > inter_ex).mean()), 5),
                    "fallback_rate": round(fbits / pairs, 5), "local_range": LA_LOCAL_RANGE, "fallback_sad": args.la_fb,
-                   "search": "network proposals (8 per block and reference) searched over +-2 half-res pels, "
-                             "exhaustive +-64 fall-back above the SAD threshold",
+                   "search": "network proposals (8 per block; for reference k the distance-1 pair's "
+                             "proposals times k when la_scaled, else the pair (f, f-k)'s) searched over +-2 "
+                             "half-res pels, exhaustive +-64 fall-back above the SAD threshold",
+                   "la_scaled": la_scaled,
                    "parity_vs_oracle": "bit-exact" if ok_l and ok_c else "MISMATCH"}
     mpx = lambda ms: round(w * h * n / (ms / 1e3) / 1e6, 2)
     out = {"metric": "4K lookahead Mpx/s (40-frame window: half-res mb-tree costs, int32 propagation, "

@@ -194,8 +194,10 @@ int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases
  * rh264_motion_search_wavefront: 0 = one launch per anti-diagonal, 1 = fixed-point
  * passes (each re-searches the macroblocks whose predictors changed; the stream
  * is synchronised every 8 passes to test for the fixed point), 2 (default) = 1
- * for one or two references, else 0.  Every search variant returns identical
- * results; the two network paths agree up to FP32 summation order. */
+ * for one or two references, else 0; "la_scaled" of
+ * rh264_mbtree_costs_multi_learned (see there; it changes the proposals, so the
+ * results).  Every other search variant returns identical results; the two
+ * network paths agree up to FP32 summation order. */
 int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
 
 /* Quarter-pel prediction of blocks (mc_luma, inter.rs:1193-1262, served from
@@ -398,12 +400,17 @@ int rh264_mbtree_costs_multi(rh264_ctx *ctx, const uint8_t *half, int nframes, i
                              int range, int32_t *intra, int32_t *inter, int32_t *mv, int32_t *ref);
 /* The learned-seeded form of the above (build-defined; the 4K configuration's
  * "learned proposal seeding the inter search").  frames: the nframes full-resolution
- * cw x ch frames (stride frame_stride) that half was downsampled from.  For every
- * reference k the network (net, created for at least nframes - 1 pairs of cw x ch)
- * proposes 8 vectors per 16x16 MB of the pair (frame f, frame f - k), i.e. per 8x8
- * half-res block; each is searched exhaustively over +-local_range half-res pels
- * (1..4), the first minimum in proposal order (then raster order) winning; a best
- * SAD above fallback_sad falls back to the exhaustive +-range search.  Costs,
+ * cw x ch frames (stride frame_stride) that half was downsampled from.  The network
+ * (net, created for at least nframes - 1 pairs of cw x ch) proposes 8 vectors per
+ * 16x16 MB, i.e. per 8x8 half-res block, for reference k of frame f: with
+ * rh264_tune "la_scaled" 1 (the default) its proposals for the pair (frame f,
+ * frame f - 1) times k, one network run in all; with 0 its proposals for the pair
+ * (frame f, frame f - k), one run per distance.  A proposal's centre (2 half-res
+ * pels per bin, times k when scaled) is clamped to +-(P - 8 - local_range), P =
+ * (max(range, 16 + local_range) + 11) & ~3.  Each is searched exhaustively over
+ * +-local_range half-res pels (1..4), the first minimum in proposal order (then
+ * raster order) winning; a best SAD above fallback_sad falls back to the exhaustive
+ * +-range search.  Costs,
  * reference choice and outputs as rh264_mbtree_costs_multi; fallback[f][mb] = bit
  * k-1 set when reference k fell back (device, may not be NULL). */
 int rh264_mbtree_costs_multi_learned(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *frames, size_t frame_stride,

@@ -52,6 +52,8 @@ struct rh264_ctx {
     size_t lal_cap;
     void *ms_buf;                 // predictors, work list and counters of the fixed-point decision
     size_t ms_cap;
+    int la_scaled;                // learned-seeded lookahead: 1 = distance-1 proposals scaled by the distance,
+                                  // 0 = the network run for every reference distance
     int decide_mode;              // encoder-order decision: 0 = one launch per anti-diagonal, 1 = fixed-point
                                   // passes, 2 = the fixed point for 1-2 references, else the anti-diagonals
 };

@@ -2862,7 +2862,7 @@ __global__ __launch_bounds__(256) void k_la_cost(const uint8_t *__restrict__ hal
 template <int RL>
 __global__ __launch_bounds__(256) void k_la_cand(const uint8_t *__restrict__ half, int mb_w, int mb_h, int nframes,
                                                  int nrefs, int P, const uint8_t *__restrict__ planes, size_t pstride,
-                                                 const int32_t *__restrict__ top8, int fb_sad,
+                                                 const int32_t *__restrict__ top8, int scaled, int cmax, int fb_sad,
                                                  int32_t *__restrict__ wmv, int32_t *__restrict__ wfb,
                                                  unsigned *__restrict__ count, int32_t *__restrict__ list,
                                                  uint64_t *__restrict__ lkeys)
@@ -2879,13 +2879,17 @@ __global__ __launch_bounds__(256) void k_la_cand(const uint8_t *__restrict__ hal
     const int hw = mb_w * 8, hh = mb_h * 8, nm = mb_w * mb_h, pw = hw + 2 * P;
     const int slot = threadIdx.x / LS_NCAND, c = threadIdx.x - slot * LS_NCAND;
     const int m = (int)wgx * BPW + slot;
-    // proposals of (frame f, reference f - k): the network's run over the pairs (j + k, j)
-    const int32_t *t8 = top8 + (((size_t)(k - 1) * nframes + (f - k)) * nm) * LS_NCAND;
+    // proposals of (frame f, reference f - k): the network's run over the pairs (j + k, j),
+    // or (scaled) its run over the pairs (j + 1, j) for frame f, times the distance k
+    const int32_t *t8 = scaled ? top8 + ((size_t)(f - 1) * nm) * LS_NCAND
+                               : top8 + (((size_t)(k - 1) * nframes + (f - k)) * nm) * LS_NCAND;
     if (m < nm) {
         const int bx0 = (m % mb_w) * 8, by0 = (m / mb_w) * 8;
         const int bin = t8[(size_t)m * LS_NCAND + c];
         // bin -> full-resolution vector ((bin % 17) - 8, (bin / 17) - 8) * 4 pels = *2 half-res pels
-        const int cx = ((bin % LS_BINS) - LS_BINS / 2) * 2, cy = ((bin / LS_BINS) - LS_BINS / 2) * 2;
+        const int sk = scaled ? k : 1;
+        const int cx = clampi(((bin % LS_BINS) - LS_BINS / 2) * 2 * sk, -cmax, cmax);
+        const int cy = clampi(((bin / LS_BINS) - LS_BINS / 2) * 2 * sk, -cmax, cmax);
         const uint8_t *src = half + (size_t)f * hw * hh + (size_t)by0 * hw + bx0;
         uint32_t sr[8][2];
 #pragma unroll
@@ -2894,7 +2898,7 @@ __global__ __launch_bounds__(256) void k_la_cand(const uint8_t *__restrict__ hal
             sr[r][1] = *(const uint32_t *)(src + (size_t)r * hw + 4);
         }
         const uint8_t *F = planes + (size_t)(f - k) * pstride;
-        // |cx|, |cy| <= 16 and P >= 16 + RL + 8: the window lies inside the padded plane
+        // |cx|, |cy| <= cmax = P - 8 - RL: the window lies inside the padded plane
         const int x = P + bx0 + cx - RL, y0 = P + by0 + cy - RL;
         const int sh = x & 3;
         uint64_t aq[NQ > 0 ? NQ : 1][NC];
@@ -2958,8 +2962,9 @@ __global__ __launch_bounds__(256) void k_la_cand(const uint8_t *__restrict__ hal
             const int kk = (int)((key >> 8) & 255), idx = (int)(key & 255);
             const int bin = t8[(size_t)mm * LS_NCAND + kk];
             const size_t w = ((size_t)(f - 1) * nrefs + (k - 1)) * nm + mm;
-            wmv[2 * w] = ((bin % LS_BINS) - LS_BINS / 2) * 2 + idx % NC - RL;
-            wmv[2 * w + 1] = ((bin / LS_BINS) - LS_BINS / 2) * 2 + idx / NC - RL;
+            const int sk = scaled ? k : 1; // the winning proposal's centre, as the search placed it
+            wmv[2 * w] = clampi(((bin % LS_BINS) - LS_BINS / 2) * 2 * sk, -cmax, cmax) + idx % NC - RL;
+            wmv[2 * w + 1] = clampi(((bin / LS_BINS) - LS_BINS / 2) * 2 * sk, -cmax, cmax) + idx / NC - RL;
             const bool fb = (int64_t)(key >> 32) > (int64_t)fb_sad;
             wfb[w] = fb ? 1 : 0;
             if (fb) {
@@ -3535,6 +3540,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->ms_buf = nullptr;
     c->ms_cap = 0;
     c->decide_mode = 2;
+    c->la_scaled = 1;
     c->fs_kernel = 0;
     c->refine_lds = 1;
     c->refine_mfma = 1;
@@ -3939,6 +3945,11 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
     }
     if (!strcmp(knob, "planes_rw")) {
         ctx->planes_rw = value ? 1 : 0;
+        return 0;
+    }
+    if (!strcmp(knob, "la_scaled")) {
+        if (value < 0 || value > 1) return fail("la_scaled %d: 0 or 1", value);
+        ctx->la_scaled = value;
         return 0;
     }
     if (!strcmp(knob, "decide_mode")) {
@@ -4389,8 +4400,8 @@ int rh264_mbtree_costs_multi_learned(rh264_ctx *ctx, rh264_mvnet *net, const uin
     if (nframes <= 0) return 0;
     if (nframes > 16384) return fail("mbtree_costs_multi_learned: nframes %d > 16384", nframes);
     const int mb_w = cw / 16, mb_h = ch / 16, hw = mb_w * 8, hh = mb_h * 8, nm = mb_w * mb_h;
-    // pad: the exhaustive window, and every proposal (|v| <= 16 half-res pels) with its
-    // +-local_range window
+    // pad: the exhaustive window, and every proposal (|v| <= 16 half-res pels; scaled
+    // ones are clamped to P - 8 - local_range) with its +-local_range window
     const int P = (std::max(range, 16 + local_range) + 8 + 3) & ~3;
     const int pw = hw + 2 * P, ph = hh + 2 * P;
     const size_t pstride = (size_t)pw * ph;
@@ -4413,7 +4424,9 @@ int rh264_mbtree_costs_multi_learned(rh264_ctx *ctx, rh264_mvnet *net, const uin
     unsigned *count = (unsigned *)(base + o_cnt);
     if (nframes > 1) {
         // proposals for every (frame, reference) pair: the network over the pairs (j + k, j)
-        for (int k = 1; k <= nrefs && k < nframes; k++) {
+        // for each distance k, or (la_scaled) over the pairs (j + 1, j) only
+        const int scaled = ctx->la_scaled, cmax = P - 8 - local_range;
+        for (int k = 1; k <= (scaled ? 1 : nrefs) && k < nframes; k++) {
             rc = rh264_mvnet_forward(ctx, net, frames + (size_t)k * frame_stride, frames, frame_stride, nframes - k,
                                      cw, ch, nullptr, top8 + (size_t)(k - 1) * nframes * nm * LS_NCAND);
             if (rc) return rc;
@@ -4422,7 +4435,7 @@ int rh264_mbtree_costs_multi_learned(rh264_ctx *ctx, rh264_mvnet *net, const uin
         const dim3 grid((unsigned)((nm + 256 / LS_NCAND - 1) / (256 / LS_NCAND)), (unsigned)((nframes - 1) * nrefs));
 #define LC_LAUNCH(RL)                                                                                             \
     hipLaunchKernelGGL(k_la_cand<RL>, grid, dim3(256), 0, ctx->stream, half, mb_w, mb_h, nframes, nrefs, P, planes, \
-                       pstride, top8, (int)fallback_sad, wmv, wfb, count, list, lkeys)
+                       pstride, top8, scaled, cmax, (int)fallback_sad, wmv, wfb, count, list, lkeys)
         switch (local_range) {
         case 1: LC_LAUNCH(1); break;
         case 2: LC_LAUNCH(2); break;

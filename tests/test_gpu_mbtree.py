@@ -128,34 +128,46 @@ def test_mbtree_multi_reference_bit_exact(dev, cw, ch, n, nrefs, R, kind):
     fb.free()
 
 
-@pytest.mark.parametrize("cw,ch,n,nrefs,R,Rl,fb", [(128, 96, 5, 3, 16, 2, 300), (176, 144, 4, 2, 24, 1, 150),
-                                                   (64, 48, 4, 3, 8, 4, -1), (352, 288, 3, 3, 64, 2, 400)])
-def test_mbtree_multi_learned_bit_exact(dev, cw, ch, n, nrefs, R, Rl, fb):
+@pytest.mark.parametrize("cw,ch,n,nrefs,R,Rl,fb,scaled", [
+    (128, 96, 5, 3, 16, 2, 300, 0), (176, 144, 4, 2, 24, 1, 150, 0), (64, 48, 4, 3, 8, 4, -1, 0),
+    (352, 288, 3, 3, 64, 2, 400, 0), (128, 96, 5, 3, 16, 2, 300, 1), (352, 288, 4, 3, 64, 2, 400, 1),
+    (176, 144, 4, 4, 8, 3, 150, 1)])
+def test_mbtree_multi_learned_bit_exact(dev, cw, ch, n, nrefs, R, Rl, fb, scaled):
     """The learned-seeded multi-reference form: the network's proposals for every
     (frame, reference) pair (taken from the GPU network itself), the oracle's proposal
     windows, fall-back and reference choice; costs, vectors, references, fall-back
-    masks and the propagation bit-exact.  fb = -1 sends every block to the fall-back."""
+    masks and the propagation bit-exact.  fb = -1 sends every block to the fall-back.
+    scaled: the distance-1 proposals times the distance k, clamped to the padded
+    plane's bound P - 8 - Rl (la_scaled 1)."""
     from rusty_h264_amd import mvnet
     frames = synth.hd_clip(n, seed=cw + n, w=cw, h=ch, nrects=3)
     mb_w, mb_h = cw // 16, ch // 16
     nm = mb_w * mb_h
     fbuf = dev.upload(frames)
     net = mvnet.MVNet(dev, n - 1, cw, ch, mvnet.load_weights())
+    dev.tune("la_scaled", scaled)
     try:
         mt = accel.MbTree(dev, cw, ch, n)
         mt.run_multi_learned(fbuf, net, nrefs=nrefs, search_range=R, local_range=Rl, fallback_sad=fb)
         got = mt.results()
-        # proposals per (frame f, reference k): the network over the pairs (j + k, j)
+        # proposals per (frame f, reference k): the network over the pairs (j + k, j), or
+        # (scaled) over the pairs (j + 1, j) times k
         cen = np.zeros((n, nrefs, nm, 8, 2), np.int32)
-        for k in range(1, min(nrefs, n - 1) + 1):
+        cmax = ((max(R, 16 + Rl) + 8 + 3) & ~3) - 8 - Rl
+        for k in range(1, min(1 if scaled else nrefs, n - 1) + 1):
             t8 = dev.alloc(4 * (n - k) * nm * 8)
             net.forward(int(fbuf) + k * cw * ch, int(fbuf), cw * ch, n - k, cw, ch, None, t8)
             top = dev.download(t8, (n - k, nm, 8), np.int32)
             t8.free()
-            cen[k:, k - 1, :, :, 0] = (top % 17 - 8) * 2
-            cen[k:, k - 1, :, :, 1] = (top // 17 - 8) * 2
+            for kk in (range(1, min(nrefs, n - 1) + 1) if scaled else (k,)):
+                s = kk if scaled else 1
+                cen[kk:, kk - 1, :, :, 0] = np.clip((top[kk - k:] % 17 - 8) * 2 * s, -cmax, cmax)
+                cen[kk:, kk - 1, :, :, 1] = np.clip((top[kk - k:] // 17 - 8) * 2 * s, -cmax, cmax)
+        if scaled:
+            assert np.abs(cen).max() > 16  # scaled proposals beyond the network's own range
     finally:
         net.close()
+        dev.tune("la_scaled", 1)
     halves = got["half"]
     intra0, _, _, _ = O.mbtree_costs_multi(halves[:1], mb_w, mb_h, nrefs, R)
     assert np.array_equal(got["intra"][0], intra0[0]) and np.array_equal(got["inter"][0], intra0[0])
