@@ -2157,12 +2157,16 @@ __global__ __launch_bounds__(256) void k_rc_complexity(const uint8_t *__restrict
     __shared__ unsigned long long s_sum[4];
     const int mbw = cw >> 4, nmb = mbw * (ch >> 4);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int frame = blockIdx.y;
+    // logical workgroup ids renumbered per XCD, so that neighbouring macroblocks, whose
+    // reference rows share cache lines, meet in one L2
+    unsigned wgx, wgy;
+    xcd_grid2(wgx, wgy);
+    const int frame = (int)wgy;
     const uint8_t *cf = cur + (size_t)frame * frame_stride;
     const uint8_t *rf = ref ? ref + (size_t)frame * frame_stride : nullptr;
     const MxConst K = mx_const(lane);
     unsigned long long sum = 0; // wave-uniform
-    for (int m = blockIdx.x * 4 + wave; m < nmb; m += gridDim.x * 4)
+    for (int m = (int)wgx * 4 + wave; m < nmb; m += gridDim.x * 4)
         sum += (unsigned)rc_mb_score(cf, rf, cw, ch, (m % mbw) << 4, (m / mbw) << 4, lane, K);
     if (lane == 0) s_sum[wave] = sum;
     __syncthreads();
