@@ -3371,7 +3371,9 @@ void k_cand_search_m(rh264_search_params p, const uint8_t *__restrict__ cur, siz
     constexpr int NW = (NC + 15 + 3) / 4; // realigned window dwords per row (bytes 0 .. NC + 14)
     __shared__ int32_t s_lr[RH264_LR_N];
     __shared__ unsigned long long s_key[BPW];
-    __shared__ uint32_t s_src[BPW][64];
+    // 68 dwords a block: the 8 blocks one wave reads start 4 banks apart (a pitch of 64
+    // would put them all on the same banks)
+    __shared__ uint32_t s_src[BPW][68];
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
     if (threadIdx.x < BPW) s_key[threadIdx.x] = ~0ull;
     const unsigned wg = xcd_linear(blockIdx.x, gridDim.x);
