@@ -485,3 +485,34 @@ def test_decision_predict(dev, source):
         assert np.array_equal(sp[f], w_sp), (f, np.argwhere(sp[f] != w_sp)[:4])
     for x in (cur, refs, pl, ref_d, mv_d):
         x.free()
+
+
+def test_decision_entry_points_decline_and_reject(dev):
+    """Shapes and sizes outside the encoder-order decision's and the predictions' scope
+    decline (the caller keeps its own path); malformed arguments are errors; nothing is
+    launched in either case."""
+    cw, ch, pad = 64, 48, 32
+    clip = synth.cif_clip(2, seed=1, w=cw, h=ch)
+    cur, refs = dev.upload(clip[1:]), dev.upload(clip[:1])
+    pl = accel.build_planes(dev, refs, 1, cw, ch, pad)
+    p16 = accel.search_params(cw, ch, pad, 16, 16, 16, 0, LAM26)
+    p8 = accel.search_params(cw, ch, pad, 8, 8, 16, 0, LAM26)
+    with pytest.raises(accel.Declined):
+        accel.motion_search_wavefront(dev, p8, 1, cur, pl)
+    with pytest.raises(accel.Declined):
+        accel.motion_search_wavefront(dev, p16, 1, cur, pl, nrefs=5)
+    with pytest.raises(RuntimeError):
+        accel.motion_search_wavefront(dev, p16, 1, cur, pl, steps=(0,))
+    with pytest.raises(RuntimeError):
+        accel.motion_search_wavefront(dev, p16, 1, cur, pl, lambda_ref=-1.0)
+    ref_d, mv_d, cost_d = accel.motion_search_wavefront(dev, p16, 1, cur, pl)
+    with pytest.raises(accel.Declined):
+        accel.decision_predict(dev, cw + 8, ch, pad, 1, 1, pl, ref_d, mv_d)
+    with pytest.raises(accel.Declined):
+        accel.decision_predict(dev, cw, ch, 3, 1, 1, pl, ref_d, mv_d)
+    with pytest.raises(accel.Declined):
+        accel.decision_predict(dev, cw, ch, pad, 1, 0, pl, ref_d, mv_d)
+    with pytest.raises(RuntimeError):
+        accel.decision_predict(dev, cw, ch, pad, 1, 1, pl, ref_d, mv_d, planes_stride=16)
+    for x in (cur, refs, pl, ref_d, mv_d, cost_d):
+        x.free()
