@@ -1228,18 +1228,33 @@ __device__ __forceinline__ uint32_t mx_operand_far(const MxBlock &B, int lane, i
 
 // Cost of a candidate outside the staged window: the plane set in HBM, clamped at
 // its border, and the rate computed directly.  Kept out of line so that its table
-// reads and address arithmetic stay off the common path.
+// reads and address arithmetic stay off the common path.  Its arguments are plain
+// values (the block's fields, the transform constants rebuilt from the lane), so the
+// caller keeps nothing in scratch memory for it.
 template <bool T8>
-__device__ __attribute__((noinline)) int mx_cost_far(const MxBlock &B, const MxConst &K, mx_f4 c2,
-                                                      const int32_t *s_lr, int lane, int rx, int ry)
+__device__ __attribute__((noinline)) int mx_cost_far(const uint8_t *pl, int pw, int ph, int ix, int iy, int sx, int sy,
+                                                      int mpx, int mpy, mx_f4 c2, const int32_t *s_lr, int lane,
+                                                      int rx, int ry)
 {
-    const int mx = B.sx + rx, my = B.sy + ry;
+    MxBlock B;
+    B.pl = pl;
+    B.pw = pw;
+    B.ph = ph;
+    B.pn = (size_t)pw * ph;
+    B.ix = ix;
+    B.iy = iy;
+    B.sx = sx;
+    B.sy = sy;
+    B.mpx = mpx;
+    B.mpy = mpy;
+    B.xoff = 0;
+    const MxConst K = mx_const<T8>(lane);
     const int s = mx_satd<T8>(mx_operand_far(B, lane, rx, ry), K, c2);
-    return __builtin_amdgcn_readfirstlane(s + s_lr[mvbits(mx - B.mpx) + mvbits(my - B.mpy)]);
+    return __builtin_amdgcn_readfirstlane(s + s_lr[mvbits(sx + rx - mpx) + mvbits(sy + ry - mpy)]);
 }
 
 template <int PAT, bool T8>
-__global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, const uint8_t *__restrict__ cur,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_refine_mx16(rh264_search_params p, const uint8_t *__restrict__ cur,
                                                      size_t cur_stride, const uint8_t *__restrict__ planes,
                                                      size_t planes_stride, const uint64_t *__restrict__ keys,
                                                      const int32_t *__restrict__ centres,
@@ -1343,7 +1358,8 @@ __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, cons
     auto cost_of = [&](int rx, int ry) -> int {
         if (!rate_fast || (unsigned)(rx + 4 * MX_WL) >= (unsigned)(8 * MX_WL) ||
             (unsigned)(ry + 4 * MX_WL) >= (unsigned)(8 * MX_WL))
-            return __builtin_amdgcn_readfirstlane(mx_cost_far<T8>(B, K, c2, s_lr, lane, rx, ry));
+            return __builtin_amdgcn_readfirstlane(
+                mx_cost_far<T8>(B.pl, B.pw, B.ph, B.ix, B.iy, B.sx, B.sy, B.mpx, B.mpy, c2, s_lr, lane, rx, ry));
         const int ph = ((rx & 3) << 2) | (ry & 3);
         const uint32_t qo = __builtin_amdgcn_readlane(t_off, ph);
         // byte offsets (wave-uniform) of the operands' top-left samples in the staged
@@ -1388,7 +1404,9 @@ __global__ __launch_bounds__(256) void k_refine_mx16(rh264_search_params p, cons
     };
     auto eval_tab = [&](const Tab &T, int i, int rx, int ry) -> int {
         const int lrc = __builtin_amdgcn_readlane(T.lr, i);
-        if (lrc < 0) return __builtin_amdgcn_readfirstlane(mx_cost_far<T8>(B, K, c2, s_lr, lane, rx, ry));
+        if (lrc < 0)
+            return __builtin_amdgcn_readfirstlane(
+                mx_cost_far<T8>(B.pl, B.pw, B.ph, B.ix, B.iy, B.sx, B.sy, B.mpx, B.mpy, c2, s_lr, lane, rx, ry));
         const uint32_t ta = __builtin_amdgcn_readlane(T.ta, i), tb = __builtin_amdgcn_readlane(T.tb, i);
         const uint32_t sh = __builtin_amdgcn_readlane(T.sh, i);
         const uint32_t v = __builtin_amdgcn_lerp(mx_lds4(lwin + ta, sh & 3), mx_lds4(lwin + tb, (sh >> 8) & 3),
