@@ -310,7 +310,7 @@ def bench_batched(args):
     shapes = [(16, 16), (16, 8), (8, 16), (8, 8), (4, 4)]
     lam = accel.lambda_me(QP)
     with accel.Device(0) as dev:
-        streams, planes_jobs, px = [], [], 0
+        streams, planes_jobs, px, ref_px = [], [], 0, 0
         for i, (w, h) in enumerate(sizes):
             nrefs = 1 + i % 4
             gen = synth.cif_clip if w == 352 else synth.hd_clip
@@ -322,6 +322,7 @@ def bench_batched(args):
             streams.append({"cur": frames, "planes": planes, "cw": w, "ch": h, "nrefs": nrefs})
             planes_jobs.append((int(frames) + w * h, int(planes), w, h))
             px += w * h
+            ref_px += nrefs * w * h
         bt = accel.Batch(dev, streams, shapes, PAD, R, 0, lam)
         choices = bt.autotune(reps=1)
 
@@ -350,6 +351,14 @@ def bench_batched(args):
                                   "16x16/16x8/8x16/8x8/4x4, +-32 full search + qpel refine + ref choice, "
                                   "graph replay", "pixels_per_step": px, "autotune": choices},
            "wall_ms_per_step": round(wall * 1e3 / args.steps, 4)}
+    # the integer searches' algorithmic work over the whole step: every pixel of every
+    # (stream, reference) once per shape per candidate; the step also builds planes and
+    # refines, so this is a lower bound on the search kernels' own fraction
+    ops = ref_px * len(shapes) * (2 * R + 1) ** 2
+    ach = ops / (ms / 1e3) / 1e12
+    out["roofline"] = {"bound": "valu", "scope": "whole step (integer searches' abs-diff-adds / step time)",
+                       "work": float(ops), "achieved": round(ach, 2), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                       "frac": round(ach / VALU_PEAK_TOPS, 4)}
     print(json.dumps(out), flush=True)
 
 
