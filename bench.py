@@ -194,12 +194,7 @@ def main():
     ap.add_argument("--tune", action="append", default=[], help="knob=value passed to rh264_tune")
     ap.add_argument("--workload", default="1080p", choices=["1080p", "batched", "lookahead4k"],
                     help="1080p (the headline metric) or a secondary workload")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only copy MIB MiB device-to-device 3 times (a plain copy kernel of known size, "
-                         "the reference point for HBM byte counters)")
     args = ap.parse_args()
-    if args.copy_probe:
-        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the motion search on a single MI355X "
               "(streams and GOPs shard across GPUs with no exchange between them). Use --gpus 1.",
@@ -360,21 +355,6 @@ def bench_batched(args):
                        "work": float(ops), "achieved": round(ach, 2), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
                        "frac": round(ach / VALU_PEAK_TOPS, 4)}
     print(json.dumps(out), flush=True)
-
-
-def copy_probe(mib):
-    """Device-to-device copy of `mib` MiB, three times: under rocprofv3 --pmc its
-    FETCH_SIZE / WRITE_SIZE per dispatch give the counters' scale (a plain copy reads
-    and writes exactly mib MiB)."""
-    import torch
-    n = mib << 20
-    src = torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    ok = bool(torch.equal(dst, src))
-    print(json.dumps({"copy_probe_bytes": n, "copies": 3, "equal": ok}), flush=True)
 
 
 def bench_lookahead4k(args):
