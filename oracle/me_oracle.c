@@ -639,80 +639,254 @@ static int32_t sad_any(const orc_search_cfg *cfg, const uint8_t *cur, const uint
     return orc_sad(cur + (size_t)ly * cfg->cw + lx, cfg->cw, pred, cfg->bw, cfg->bw, cfg->bh);
 }
 
-/* Reference-mode search, mb16.rs:2433-3302, default knobs explicit: seed (0,0) then
- * predictors (strict <), integer snap ((v+2) div_euclid 4)*4, diamond rungs with the
- * cascade walk (mb16.rs:2708-2725), optional B2 SAD full-pel phase with the SATD
- * reprice of winner and seed (mb16.rs:2739-2742), sub-pel rings, seed kept if cheaper
- * (mb16.rs:3297-3300).  The me_wide rescue grid and the FC argmin batches are off
- * (their defaults on the paths this restates). */
+/* ---- reference-mode motion_search (mb16.rs:2433-3302) ------------------------
+ *
+ * The search restated step by step, every knob explicit (orc_diamond_cfg):
+ *   rate  mvbits(d) = Exp-Golomb se(d) length (mb16.rs:2474-2475), or with a table
+ *         (mv_cost_kind 1/2, mb16.rs:2459-2471) tab[min(|d|, 4095)];
+ *   cost  the closure of mb16.rs:2540-2565: dist + (int64)(lam_r * rate), lam_r =
+ *         lambda * 0.25 with a table, else lambda; dist = the SATD of the prediction
+ *         (encoder domain) or, in the Fast preset (sad_only), its SAD;
+ *   cost_fp  B2 (mb16.rs:2571-2578): SAD + (int64)(lambda_fp * rate), or cost;
+ *   1. seeds (0,0) then the predictors, strict < (mb16.rs:2580-2594);
+ *   2. snap a fractional winner to ((v+2) div_euclid 4)*4 (mb16.rs:2602-2607);
+ *   3. diamond rungs, each walked until it stops improving (mb16.rs:2632-2727):
+ *      with fc, on a full-pel centre whose +-step box lies inside the frame and a
+ *      16x16/16x8/8x16/8x8 block, the fixed-centre argmin of the 4-point ring
+ *      (first minimum in ring order, strict < against the centre, mb16.rs:2646-
+ *      2706), 16-wide blocks priced as the x4 batch does, dist + (int64)(lam_fp *
+ *      rate) with lam_fp = lambda_fp under B2 else lambda (mb16.rs:2680-2683);
+ *      otherwise one cascade pass (first improver moves the centre, mb16.rs:2708-
+ *      2725);
+ *   4. B2 reprice of the winner and the seed in the cost domain (mb16.rs:2739-2742);
+ *   5. the me_wide rescue (mb16.rs:2778-2889): when enabled, me_fast or the flat
+ *      test passes, and (best - (int64)(lambda * rate(best))) / (w*h) exceeds the
+ *      threshold: a step-2 grid over +-range around the snapped winner (strict <;
+ *      a 16x16 block whose grid lies inside the frame prices dist + (int64)(lambda
+ *      * rate), mb16.rs:2822-2836), then the 3x3 +-1 pel walk with a moving centre;
+ *      events report that it ran and whether it cut the cost by >= 1/16;
+ *   6. the sub-pel rings (steps 2, 1; ring8 or ring4; walked or single pass) or the
+ *      build-defined quarter-pel window (subpel_pat 4), priced by cost;
+ *   7. with the snap, the pre-snap seed is kept when cheaper (mb16.rs:3297-3300).
+ * The sub-pel memo (mb16.rs:3006-3029) is value-transparent and not restated; the
+ * sub-pel FC batches (RFF_SP_FC), the pattern dispatch and the iteration cap are off
+ * by default (mb16.rs:663-676, 940-958, 740-750) and not restated.
+ * Trace (optional): every priced candidate as {phase, mvx, mvy, cost}, int64 each;
+ * phases 0 seed, 1 snap, 2 cascade, 3 fixed-centre, 4 reprice, 5 rescue grid,
+ * 6 rescue walk, 7 sub-pel. */
 static int div_euclid4(int v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
 
-int64_t orc_motion_search_ref(const orc_search_cfg *cfg, const orc_diamond_cfg *dc,
-                              const uint8_t *cur, const uint8_t *planes, int lx, int ly,
-                              const int32_t *preds, int npreds, int32_t *mv_out)
+typedef struct {
+    const orc_search_cfg *cfg;
+    const orc_diamond_cfg *dc;
+    const uint8_t *cur, *planes;
+    int lx, ly, cx0, cy0;
+    int64_t *trace;
+    int cap, n;
+} ms_state;
+
+uint32_t orc_mv_rate(const uint16_t *tab, int32_t d)
 {
-    int cx0 = npreds > 0 ? preds[0] : 0, cy0 = npreds > 0 ? preds[1] : 0;
-#define COST(X, Y) orc_satd_cost(cfg, cur, planes, lx, ly, (X), (Y), cx0, cy0)
-#define COST_FP(X, Y)                                                                          \
-    (dc->sadfp ? (int64_t)sad_any(cfg, cur, planes, lx, ly, (X), (Y)) +                \
-                     rate_cost(cfg->lr_fp, (X), (Y), cx0, cy0)                                 \
-               : COST(X, Y))
+    if (!tab) return orc_mvbits(d);
+    uint32_t a = d < 0 ? (uint32_t)(-(int64_t)d) : (uint32_t)d;
+    return tab[a < 4095u ? a : 4095u];
+}
+
+static uint32_t ms_rate(const ms_state *S, int x, int y)
+{
+    return orc_mv_rate(S->dc->mv_tab, x - S->cx0) + orc_mv_rate(S->dc->mv_tab, y - S->cy0);
+}
+
+static int64_t lam_rate(double lam, uint32_t rate) { return (int64_t)(lam * (double)rate); }
+
+static void ms_note(ms_state *S, int phase, int x, int y, int64_t c)
+{
+    if (S->trace && S->n < S->cap) {
+        int64_t *t = S->trace + 4 * (size_t)S->n;
+        t[0] = phase;
+        t[1] = x;
+        t[2] = y;
+        t[3] = c;
+    }
+    S->n++;
+}
+
+static int64_t ms_dist(const ms_state *S, int x, int y, int sad)
+{
+    if (sad) return sad_any(S->cfg, S->cur, S->planes, S->lx, S->ly, x, y);
+    return satd_dist(S->cfg, S->cur, S->planes, S->lx, S->ly, x, y);
+}
+
+/* the cost closure (mb16.rs:2540-2565) */
+static int64_t ms_cost(ms_state *S, int phase, int x, int y)
+{
+    const orc_diamond_cfg *dc = S->dc;
+    double lam_r = dc->mv_tab ? dc->lambda * 0.25 : dc->lambda;
+    int64_t c = ms_dist(S, x, y, dc->sad_only) + lam_rate(lam_r, ms_rate(S, x, y));
+    ms_note(S, phase, x, y, c);
+    return c;
+}
+
+/* cost_fp (mb16.rs:2571-2578) */
+static int64_t ms_cost_fp(ms_state *S, int phase, int x, int y)
+{
+    const orc_diamond_cfg *dc = S->dc;
+    if (!dc->sadfp) return ms_cost(S, phase, x, y);
+    int64_t c = ms_dist(S, x, y, 1) + lam_rate(dc->lambda_fp, ms_rate(S, x, y));
+    ms_note(S, phase, x, y, c);
+    return c;
+}
+
+/* a candidate priced with an explicit lambda (the x4 and grid batches) */
+static int64_t ms_cost_lam(ms_state *S, int phase, int x, int y, int sad, double lam)
+{
+    int64_t c = ms_dist(S, x, y, sad) + lam_rate(lam, ms_rate(S, x, y));
+    ms_note(S, phase, x, y, c);
+    return c;
+}
+
+int64_t orc_motion_search_ex(const orc_search_cfg *cfg, const orc_diamond_cfg *dc, const uint8_t *cur,
+                             const uint8_t *planes, int lx, int ly, const int32_t *preds, int npreds,
+                             int32_t *mv_out, int32_t *events, int64_t *trace, int trace_cap, int *trace_n)
+{
+    ms_state S = {cfg, dc, cur, planes, lx, ly, npreds > 0 ? preds[0] : 0, npreds > 0 ? preds[1] : 0,
+                  trace, trace_cap, 0};
+    const int bw = cfg->bw, bh = cfg->bh;
+    if (events) events[0] = events[1] = 0;
+    /* 1. seeds */
     int bx = 0, by = 0;
-    int64_t bc;
-    bc = COST_FP(0, 0);
+    int64_t bc = ms_cost_fp(&S, 0, 0, 0);
     for (int i = 0; i < npreds; i++) {
-        int64_t pc = COST_FP(preds[2 * i], preds[2 * i + 1]);
+        int64_t pc = ms_cost_fp(&S, 0, preds[2 * i], preds[2 * i + 1]);
         if (pc < bc) { bc = pc; bx = preds[2 * i]; by = preds[2 * i + 1]; }
     }
-    int seedx = bx, seedy = by;
+    const int seedx = bx, seedy = by;
     int64_t seed_c = bc;
+    /* 2. snap */
     if (dc->snap && ((bx & 3) != 0 || (by & 3) != 0)) {
         bx = div_euclid4(bx + 2) * 4;
         by = div_euclid4(by + 2) * 4;
-        bc = COST_FP(bx, by);
+        bc = ms_cost_fp(&S, 1, bx, by);
     }
+    /* 3. diamond */
+    const int fc = dc->fc && !dc->sad_only && ((bw == 16 && (bh == 16 || bh == 8)) || (bw == 8 && (bh == 16 || bh == 8)));
+    const double lam_fc = dc->sadfp ? dc->lambda_fp : dc->lambda;
     for (int si = 0; si < dc->nsteps; si++) {
-        int step = dc->steps[si];
+        const int step = dc->steps[si];
+        const int ring[4][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step}};
         for (;;) {
+            if (fc && (bx & 3) == 0 && (by & 3) == 0) {
+                const int st = step >> 2, ix = lx + (bx >> 2), iy = ly + (by >> 2);
+                if (ix - st >= 0 && iy - st >= 0 && ix + st + bw <= cfg->cw && iy + st + bh <= cfg->ch) {
+                    int bi = -1;
+                    int64_t bcc = bc;
+                    for (int i = 0; i < 4; i++) {
+                        const int x = bx + ring[i][0], y = by + ring[i][1];
+                        int64_t cc = bw == 16 ? ms_cost_lam(&S, 3, x, y, dc->sadfp, lam_fc) : ms_cost_fp(&S, 3, x, y);
+                        if (cc < bcc) { bcc = cc; bi = i; }
+                    }
+                    if (bi < 0) break;
+                    bc = bcc;
+                    bx += ring[bi][0];
+                    by += ring[bi][1];
+                    continue;
+                }
+            }
             int improved = 0;
-            const int ring[4][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step}};
             for (int i = 0; i < 4; i++) {
-                int x = bx + ring[i][0], y = by + ring[i][1];
-                int64_t cc = COST_FP(x, y);
+                const int x = bx + ring[i][0], y = by + ring[i][1];
+                int64_t cc = ms_cost_fp(&S, 2, x, y);
                 if (cc < bc) { bc = cc; bx = x; by = y; improved = 1; }
             }
             if (!improved) break;
         }
     }
+    /* 4. B2 reprice */
     if (dc->sadfp) {
-        bc = COST(bx, by);
-        seed_c = COST(seedx, seedy);
+        bc = ms_cost(&S, 4, bx, by);
+        seed_c = ms_cost(&S, 4, seedx, seedy);
     }
-    int pat = cfg->subpel_pat;
+    /* 5. me_wide rescue */
+    int flat = 0;
+    if (dc->rescue && !dc->sad_only && !dc->rescue_fast) {
+        uint64_t s = 0, ss = 0, n = (uint64_t)bw * bh;
+        for (int r = 0; r < bh; r++)
+            for (int i = 0; i < bw; i++) {
+                uint64_t v = cur[(size_t)(ly + r) * cfg->cw + lx + i];
+                s += v;
+                ss += v * v;
+            }
+        flat = (ss - s * s / n) / n < (uint64_t)dc->rescue_var;
+    }
+    if (dc->rescue && !dc->sad_only && (dc->rescue_fast || flat)) {
+        int64_t dist = bc - lam_rate(dc->lambda, ms_rate(&S, bx, by));
+        if (dist / (int64_t)(bw * bh) > dc->rescue_thresh) {
+            const int64_t pre_c = bc;
+            const int gx = div_euclid4(bx + 2) * 4, gy = div_euclid4(by + 2) * 4;
+            const int r = dc->rescue_range;
+            const int icx = lx + (gx >> 2), icy = ly + (gy >> 2);
+            const int batched = bw == 16 && bh == 16 && icx >= r && icx + r + 16 <= cfg->cw && icy >= r &&
+                                icy + r + 16 <= cfg->ch;
+            int gbx = bx, gby = by;
+            for (int dy = -r; dy <= r; dy += 2)
+                for (int dx = -r; dx <= r; dx += 2) {
+                    const int x = gx + 4 * dx, y = gy + 4 * dy;
+                    int64_t cc = batched ? ms_cost_lam(&S, 5, x, y, 0, dc->lambda) : ms_cost(&S, 5, x, y);
+                    if (cc < bc) { bc = cc; gbx = x; gby = y; }
+                }
+            bx = gbx;
+            by = gby;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int x = bx + 4 * dx, y = by + 4 * dy;
+                    int64_t cc = ms_cost(&S, 6, x, y);
+                    if (cc < bc) { bc = cc; bx = x; by = y; }
+                }
+            if (events) {
+                events[0] = 1;
+                events[1] = bc * 16 <= pre_c * 15;
+            }
+        }
+    }
+    /* 6. sub-pel */
+    const int pat = cfg->subpel_pat;
     if (pat == 4) {
-        qpel_window(cfg, cur, planes, lx, ly, cx0, cy0, &bx, &by, &bc);
+        const int sx = bx, sy = by;
+        for (int dy = -2; dy <= 2; dy++)
+            for (int dx = -2; dx <= 2; dx++) {
+                if (dx == 0 && dy == 0) continue;
+                int64_t cc = ms_cost(&S, 7, sx + dx, sy + dy);
+                if (cc < bc) { bc = cc; bx = sx + dx; by = sy + dy; }
+            }
     } else if (pat >= 0) {
         for (int step = 2; step >= 1; step--) {
             const int ring[8][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step},
                                     {step, step}, {-step, -step}, {step, -step}, {-step, step}};
-            int nring = (pat & 1) ? 4 : 8;
+            const int nring = (pat & 1) ? 4 : 8;
             for (;;) {
                 int improved = 0;
                 for (int i = 0; i < nring; i++) {
-                    int x = bx + ring[i][0], y = by + ring[i][1];
-                    int64_t cc = COST(x, y);
+                    const int x = bx + ring[i][0], y = by + ring[i][1];
+                    int64_t cc = ms_cost(&S, 7, x, y);
                     if (cc < bc) { bc = cc; bx = x; by = y; improved = 1; }
                 }
                 if (!improved || (pat & 2)) break;
             }
         }
     }
+    /* 7. the seed */
     if (dc->snap && seed_c < bc) { bx = seedx; by = seedy; bc = seed_c; }
-#undef COST
-#undef COST_FP
     mv_out[0] = bx;
     mv_out[1] = by;
+    if (trace_n) *trace_n = S.n;
     return bc;
+}
+
+int64_t orc_motion_search_ref(const orc_search_cfg *cfg, const orc_diamond_cfg *dc,
+                              const uint8_t *cur, const uint8_t *planes, int lx, int ly,
+                              const int32_t *preds, int npreds, int32_t *mv_out)
+{
+    return orc_motion_search_ex(cfg, dc, cur, planes, lx, ly, preds, npreds, mv_out, NULL, NULL, 0, NULL);
 }
 
 /* median3 (inter.rs:9-11) and predict_mv (inter.rs:36-73). */
@@ -875,15 +1049,31 @@ int64_t orc_mbtree_mc_satd(const uint8_t *sy, int cw, int ch, const uint8_t *ref
     return orc_satd_px(sy + (size_t)by0 * cw + bx0, cw, pred, bs, bs, bs);
 }
 
-/* inter_cost, mbtree.rs:154-197 */
-int32_t orc_inter_cost(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0, int by0,
-                       int bs, int seedx, int seedy, int max_step, int32_t *mv_out)
+static int64_t mcs_noted(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0, int by0, int bs, int x,
+                         int y, int64_t *trace, int cap, int *ne)
 {
+    int64_t v = orc_mbtree_mc_satd(sy, cw, ch, ref, bx0, by0, bs, x, y);
+    if (trace && *ne < cap) {
+        trace[3 * (size_t)*ne] = x;
+        trace[3 * (size_t)*ne + 1] = y;
+        trace[3 * (size_t)*ne + 2] = v;
+    }
+    (*ne)++;
+    return v;
+}
+
+/* inter_cost, mbtree.rs:154-197, with an evaluation trace: up to cap entries {mvx, mvy, satd} (int64),
+ * *n = the number of mc_satd evaluations. */
+int32_t orc_inter_cost_ex(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0, int by0, int bs,
+                          int seedx, int seedy, int max_step, int32_t *mv_out, int64_t *trace, int cap, int *n)
+{
+    int ne = 0;
+#define MCS(X, Y) mcs_noted(sy, cw, ch, ref, bx0, by0, bs, (X), (Y), trace, cap, &ne)
     int bmx = 0, bmy = 0;
-    int64_t best = orc_mbtree_mc_satd(sy, cw, ch, ref, bx0, by0, bs, 0, 0);
+    int64_t best = MCS(0, 0);
     if (best != 0) {
         if (seedx != 0 || seedy != 0) {
-            int64_t s = orc_mbtree_mc_satd(sy, cw, ch, ref, bx0, by0, bs, seedx, seedy);
+            int64_t s = MCS(seedx, seedy);
             if (s < best) { best = s; bmx = seedx; bmy = seedy; }
         }
         for (int step = max_step; step >= 1; step >>= 1) {
@@ -892,16 +1082,24 @@ int32_t orc_inter_cost(const uint8_t *sy, int cw, int ch, const uint8_t *ref, in
                 const int ring[4][2] = {{step, 0}, {-step, 0}, {0, step}, {0, -step}};
                 for (int i = 0; i < 4; i++) {
                     int x = bmx + ring[i][0] * 4, y = bmy + ring[i][1] * 4;
-                    int64_t s = orc_mbtree_mc_satd(sy, cw, ch, ref, bx0, by0, bs, x, y);
+                    int64_t s = MCS(x, y);
                     if (s < best) { best = s; bmx = x; bmy = y; moved = 1; }
                 }
                 if (!moved) break;
             }
         }
     }
+#undef MCS
     mv_out[0] = bmx;
     mv_out[1] = bmy;
+    if (n) *n = ne;
     return best > INT32_MAX ? INT32_MAX : (int32_t)best;
+}
+
+int32_t orc_inter_cost(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0, int by0,
+                       int bs, int seedx, int seedy, int max_step, int32_t *mv_out)
+{
+    return orc_inter_cost_ex(sy, cw, ch, ref, bx0, by0, bs, seedx, seedy, max_step, mv_out, NULL, 0, NULL);
 }
 
 /* lookahead::complexity, lookahead.rs:38-101: the rate-control score of a coded-size

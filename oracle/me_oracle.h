@@ -102,18 +102,34 @@ void orc_plain_search_frame(const orc_search_cfg *cfg, const uint8_t *cur, const
                             const int32_t *centres, const int32_t *mvps, int32_t *int_mv,
                             int64_t *int_cost, int32_t *mv, int64_t *cost, int nthreads);
 
-/* Reference-mode motion_search (mb16.rs:2433-3302) with the default knobs
- * made explicit: seeds (0,0)+predictors, integer snap, diamond rungs (qpel
- * step list, cascade walk), optional SAD full-pel phase (B2), sub-pel rings.
- * Returns the cost, writes the MV. */
+/* Reference-mode motion_search (mb16.rs:2433-3302) with every knob explicit; the
+ * steps, their order and the reference's pricing of each phase are listed above
+ * orc_motion_search_ex in me_oracle.c.  cfg supplies the frame, the block shape
+ * (bw x bh, any of the seven) and subpel_pat; the lambdas come from here (the
+ * table-free form (int64)(lambda * rate) of mb16.rs:2563). */
 typedef struct {
     const int32_t *steps; int nsteps;   /* diamond rungs in qpel units, e.g. {16,8,4} */
-    int sadfp;                           /* 1: SAD full-pel phase with lr_fp          */
+    int sadfp;                           /* 1: SAD full-pel phase (B2) priced with lambda_fp */
     int snap;                            /* integer snap of the seed (me_snap)         */
+    int fc;                              /* fixed-centre argmin-of-4 passes (me_fc)    */
+    int sad_only;                        /* the Fast preset's SAD cost closure          */
+    const uint16_t *mv_tab;              /* NULL: Exp-Golomb; else bits per |d| (4096)  */
+    double lambda, lambda_fp;            /* lambda_me and the B2 lambda                 */
+    int rescue;                          /* me_wide rescue enabled for this search      */
+    int rescue_range, rescue_thresh;     /* me_range (24), me_rescue (3)                */
+    int rescue_fast, rescue_var;         /* me_fast (1), me_wide_var (800)              */
 } orc_diamond_cfg;
+/* rate of one mvd component: Exp-Golomb length, or tab[min(|d|, 4095)] */
+uint32_t orc_mv_rate(const uint16_t *tab, int32_t d);
 int64_t orc_motion_search_ref(const orc_search_cfg *cfg, const orc_diamond_cfg *dc,
                               const uint8_t *cur, const uint8_t *planes, int lx, int ly,
                               const int32_t *preds, int npreds, int32_t *mv_out);
+/* the same search with its rescue events (events[0] = the grid ran, events[1] = it
+ * cut the cost to <= 15/16; may be NULL) and an evaluation trace: up to trace_cap
+ * entries {phase, mvx, mvy, cost} (int64), *trace_n = the number priced. */
+int64_t orc_motion_search_ex(const orc_search_cfg *cfg, const orc_diamond_cfg *dc, const uint8_t *cur,
+                             const uint8_t *planes, int lx, int ly, const int32_t *preds, int npreds,
+                             int32_t *mv_out, int32_t *events, int64_t *trace, int trace_cap, int *trace_n);
 
 /* predict_mv (inter.rs:36-73): nb = neighbours A (left), B (above), C (above-right
  * or above-left) as {available, mvx, mvy, ref_idx}; out = the predictor. */
@@ -145,6 +161,10 @@ int64_t orc_mbtree_mc_satd(const uint8_t *sy, int cw, int ch, const uint8_t *ref
                            int by0, int bs, int mvx, int mvy);
 int32_t orc_inter_cost(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0, int by0,
                        int bs, int seedx, int seedy, int max_step, int32_t *mv_out);
+/* the same with an evaluation trace: up to cap entries {mvx, mvy, satd} (int64);
+ * *n = the number of evaluations */
+int32_t orc_inter_cost_ex(const uint8_t *sy, int cw, int ch, const uint8_t *ref, int bx0, int by0, int bs,
+                          int seedx, int seedy, int max_step, int32_t *mv_out, int64_t *trace, int cap, int *n);
 /* HalfRes frame costs: half/ref_half are (mb_w*8)x(mb_h*8); ref_half NULL = IDR.
  * out: intra[mb], inter[mb], mv[2*mb] (full-res qpel). mbtree.rs:208-263 */
 void orc_frame_costs_half(const uint8_t *half, const uint8_t *ref_half, int mb_w, int mb_h,

@@ -25,7 +25,13 @@ class SearchCfg(C.Structure):
 
 
 class DiamondCfg(C.Structure):
-    _fields_ = [("steps", i32p), ("nsteps", C.c_int), ("sadfp", C.c_int), ("snap", C.c_int)]
+    _fields_ = [("steps", i32p), ("nsteps", C.c_int), ("sadfp", C.c_int), ("snap", C.c_int), ("fc", C.c_int),
+                ("sad_only", C.c_int), ("mv_tab", C.POINTER(C.c_uint16)), ("lam", C.c_double),
+                ("lam_fp", C.c_double), ("rescue", C.c_int), ("rescue_range", C.c_int),
+                ("rescue_thresh", C.c_int), ("rescue_fast", C.c_int), ("rescue_var", C.c_int)]
+
+
+u16p = C.POINTER(C.c_uint16)
 
 
 def build(fast=False):
@@ -36,13 +42,15 @@ def build(fast=False):
 _libs = {}
 
 
-def load(fast=False):
-    name = "liboracle_v3.so" if fast else "liboracle.so"
+def load(fast=False, path=None):
+    """The oracle library (path: another build of the same source, e.g. a test's)."""
+    name = path or ("liboracle_v3.so" if fast else "liboracle.so")
     if name in _libs:
         return _libs[name]
-    path = os.path.join(HERE, name)
-    if not os.path.exists(path):
-        build()
+    if path is None:
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            build()
     lib = C.CDLL(path)
     _libs[name] = lib
     return lib
@@ -59,18 +67,19 @@ def _u8(a):
 class Oracle:
     """Thin numpy-facing wrapper over the C restatement."""
 
-    def __init__(self, fast=False):
-        self.lib = load(fast)
+    def __init__(self, fast=False, path=None):
+        self.lib = load(fast, path)
         L = self.lib
         for n in ("orc_sad", "orc_satd4x4_wels", "orc_satd_wels", "orc_intra_cost"):
             getattr(L, n).restype = C.c_int32
         for n in ("orc_satd_px", "orc_full_search_block", "orc_satd_cost", "orc_subpel_refine_block",
-                  "orc_motion_search_ref", "orc_mbtree_mc_satd", "orc_satd8_px", "orc_la_block_multi",
-                  "orc_la_block_learned"):
+                  "orc_motion_search_ref", "orc_motion_search_ex", "orc_mbtree_mc_satd", "orc_satd8_px",
+                  "orc_la_block_multi", "orc_la_block_learned"):
             getattr(L, n).restype = C.c_int64
         L.orc_satd_avg.restype = C.c_uint32
         L.orc_mvbits.restype = C.c_uint32
         L.orc_inter_cost.restype = C.c_int32
+        L.orc_inter_cost_ex.restype = C.c_int32
         L.orc_rc_complexity.restype = C.c_double
 
     # ---- block metrics
@@ -137,7 +146,19 @@ class Oracle:
         lr_fp = self.lambda_table(lam * 0.5 if lam_fp is None else lam_fp)
         c = SearchCfg(cw, ch, pad, bw, bh, search_range, subpel_pat, _p(lr_fp, i64p), _p(lr, i64p), hadamard)
         c._keep = (lr, lr_fp)
+        c._lam, c._lam_fp = float(lam), float(lam * 0.5 if lam_fp is None else lam_fp)
         return c
+
+    def diamond(self, cfg, steps=(16, 8, 4), sadfp=False, snap=True, fc=False, sad_only=False, mv_tab=None,
+                rescue=False, rescue_range=24, rescue_thresh=3, rescue_fast=True, rescue_var=800):
+        """motion_search's knobs (orc_diamond_cfg); lambdas from cfg (lam, lam_fp)."""
+        st = np.ascontiguousarray(steps, np.int32)
+        tab = None if mv_tab is None else np.ascontiguousarray(mv_tab, np.uint16)
+        dc = DiamondCfg(_p(st, i32p), len(st), int(sadfp), int(snap), int(fc), int(sad_only),
+                        None if tab is None else _p(tab, u16p), cfg._lam, cfg._lam_fp, int(rescue),
+                        int(rescue_range), int(rescue_thresh), int(rescue_fast), int(rescue_var))
+        dc._keep = (st, tab)
+        return dc
 
     def full_search_block(self, cfg, cur, planes, lx, ly, cx=0, cy=0, mvpx=0, mvpy=0):
         mv = np.empty(2, np.int32)
@@ -168,14 +189,24 @@ class Oracle:
                                         _p(ic, i64p), _p(mv, i32p), _p(c, i64p), nthreads)
         return {"int_mv": imv, "int_cost": ic, "mv": mv, "cost": c}
 
-    def motion_search_ref(self, cfg, cur, planes, lx, ly, preds, steps=(16, 8, 4), sadfp=False, snap=True):
-        st = np.ascontiguousarray(steps, np.int32)
-        dc = DiamondCfg(_p(st, i32p), len(st), int(sadfp), int(snap))
+    def motion_search_ref(self, cfg, cur, planes, lx, ly, preds, steps=(16, 8, 4), sadfp=False, snap=True, **kw):
+        mv, c, _, _ = self.motion_search_trace(cfg, cur, planes, lx, ly, preds, steps=steps, sadfp=sadfp, snap=snap,
+                                               cap=0, **kw)
+        return mv, c
+
+    def motion_search_trace(self, cfg, cur, planes, lx, ly, preds, cap=4096, dc=None, **kw):
+        """(mv, cost, events (ran, paid), trace [(phase, mvx, mvy, cost)]) of one search."""
+        dc = self.diamond(cfg, **kw) if dc is None else dc
         pr = np.ascontiguousarray(preds, np.int32).reshape(-1)
         mv = np.empty(2, np.int32)
-        c = self.lib.orc_motion_search_ref(C.byref(cfg), C.byref(dc), _p(_u8(cur)), _p(_u8(planes)), lx, ly,
-                                           _p(pr, i32p), len(pr) // 2, _p(mv, i32p))
-        return (int(mv[0]), int(mv[1])), int(c)
+        ev = np.zeros(2, np.int32)
+        tr = np.zeros((max(cap, 1), 4), np.int64)
+        n = C.c_int()
+        c = self.lib.orc_motion_search_ex(C.byref(cfg), C.byref(dc), _p(_u8(cur)), _p(_u8(planes)), int(lx), int(ly),
+                                          _p(pr, i32p), len(pr) // 2, _p(mv, i32p), _p(ev, i32p), _p(tr, i64p), cap,
+                                          C.byref(n))
+        trace = [tuple(int(v) for v in row) for row in tr[:min(n.value, cap)]]
+        return (int(mv[0]), int(mv[1])), int(c), (int(ev[0]), int(ev[1])), trace
 
     def predict_mv(self, a, b, c, cur_ref):
         """predict_mv (inter.rs:36-73); a, b, c = (available, (mvx, mvy), ref_idx) or None."""
@@ -186,12 +217,11 @@ class Oracle:
         self.lib.orc_predict_mv(_p(nb, i32p), int(cur_ref), _p(out, i32p))
         return int(out[0]), int(out[1])
 
-    def motion_search_frame(self, cfg, cur, planes, lambda_ref=0.0, steps=(16, 8, 4), sadfp=False, snap=True):
+    def motion_search_frame(self, cfg, cur, planes, lambda_ref=0.0, steps=(16, 8, 4), sadfp=False, snap=True, **kw):
         """A P frame's 16x16 best_part decisions in raster order with predict_mv
         predictors; planes: one plane set (reference 0) or a list, most recent first.
         Returns (ref (nblk,) int32, mv (nblk, 2) int32, cost (nblk,) int64)."""
-        st = np.ascontiguousarray(steps, np.int32)
-        dc = DiamondCfg(_p(st, i32p), len(st), int(sadfp), int(snap))
+        dc = self.diamond(cfg, steps=steps, sadfp=sadfp, snap=snap, **kw)
         pls = [planes] if isinstance(planes, np.ndarray) else list(planes)
         pls = [_u8(x) for x in pls]
         arr = (C.c_void_p * len(pls))(*[x.ctypes.data for x in pls])
@@ -252,6 +282,15 @@ class Oracle:
         c = self.lib.orc_inter_cost(_p(_u8(sy)), cw, ch, _p(_u8(ref)), bx0, by0, bs, seed[0], seed[1], max_step,
                                     _p(mv, i32p))
         return int(c), (int(mv[0]), int(mv[1]))
+
+    def inter_cost_trace(self, sy, cw, ch, ref, bx0, by0, bs, seed=(0, 0), max_step=8, cap=512):
+        """(cost, mv, trace [(mvx, mvy, satd)]) of inter_cost."""
+        mv = np.empty(2, np.int32)
+        tr = np.zeros((cap, 3), np.int64)
+        n = C.c_int()
+        c = self.lib.orc_inter_cost_ex(_p(_u8(sy)), cw, ch, _p(_u8(ref)), bx0, by0, bs, seed[0], seed[1], max_step,
+                                       _p(mv, i32p), _p(tr, i64p), cap, C.byref(n))
+        return int(c), (int(mv[0]), int(mv[1])), [tuple(int(v) for v in r) for r in tr[:min(n.value, cap)]]
 
     def rc_complexity(self, y, ref, cw, ch):
         """lookahead::complexity of one coded-size frame (ref None = IDR)."""

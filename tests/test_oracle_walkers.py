@@ -1,0 +1,184 @@
+"""Pins the oracle's two search drivers against independent walkers.
+
+orc_motion_search_ex (motion_search, mb16.rs:2433-3302) and orc_inter_cost_ex
+(inter_cost, mbtree.rs:154-197) are compared, candidate by candidate, with the
+walkers of tests/refimpl.py, which price every candidate from the spec's sample
+equations on the unpadded frame (no plane sets, no operand table) and the matrix
+Hadamard.  The traces, not only the results, must agree: the order in which the
+candidates are priced is the search.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from tests import refimpl
+
+O = Oracle()
+
+
+def _clip(cw, ch, seed, shift):
+    """A smooth random texture and a copy moved by `shift` (dy, dx) with noise, so
+    the diamond walks several steps and the sub-pel stage finds fractional optima."""
+    rng = np.random.default_rng(seed)
+    m = 16
+    base = rng.integers(0, 256, (ch + 2 * m, cw + 2 * m)).astype(np.float64)
+    for _ in range(3):
+        base = (base + np.roll(base, 1, 0) + np.roll(base, 1, 1) + np.roll(base, (1, 1), (0, 1))) / 4
+    ref = np.clip(base[m:m + ch, m:m + cw] * 1.6 - 80, 0, 255).astype(np.uint8)
+    dy, dx = shift
+    cur = base[m - dy:m - dy + ch, m - dx:m - dx + cw] * 1.6 - 80 + rng.normal(0, 2, (ch, cw))
+    return np.clip(cur, 0, 255).astype(np.uint8), ref
+
+
+def test_spec_pred_is_mc_luma():
+    """The walkers' prediction (spec equations, clamped samples) equals mc_luma, which
+    is pinned against the per-sample spec table elsewhere: every phase, every shape,
+    vectors far outside the frame."""
+    cw, ch = 48, 32
+    _, ref = _clip(cw, ch, 1, (0, 0))
+    rng = np.random.default_rng(2)
+    for bw, bh in ((16, 16), (16, 8), (8, 16), (8, 8), (8, 4), (4, 8), (4, 4)):
+        for _ in range(40):
+            x0, y0 = int(rng.integers(0, cw - bw + 1)), int(rng.integers(0, ch - bh + 1))
+            mvx, mvy = (int(v) for v in rng.integers(-240, 241, 2))
+            assert np.array_equal(refimpl.spec_pred(ref, x0, y0, bw, bh, mvx, mvy),
+                                  O.mc_luma(ref, cw, ch, x0, y0, bw, bh, mvx, mvy)), (bw, bh, mvx, mvy)
+
+
+def test_smooth_table_is_rounding_robust():
+    """The smooth rate table (mb16.rs:507-516) rounds 4 * (2 log2(a+1) + 0.718 + [a>0]);
+    no entry lies within 1e-6 of a rounding boundary, so any log2 accurate to a few
+    ulps (the reference calls the platform's) yields the same table."""
+    import math
+    for a in range(4096):
+        v = 4.0 * (2.0 * math.log2(a + 1) + 0.718 + (1.0 if a else 0.0))
+        assert abs((v - math.floor(v)) - 0.5) > 1e-6, a
+    t = refimpl.smooth_mv_table()
+    assert t[0] == 3 and t[1] == 15 and t[4095] == 103   # 2.872 -> 3; 4 * 3.718; 4 * 25.718
+    assert (np.diff(t.astype(int)) >= 0).all()
+
+
+SMOOTH = refimpl.smooth_mv_table()
+
+# (name, shape, search knobs, sub-pel pattern)
+CASES = [
+    ("cascade", (16, 16), dict(), 0),
+    ("fc", (16, 16), dict(fc=True), 0),
+    ("fc-b2", (16, 16), dict(fc=True, sadfp=True), 0),
+    ("fc-smooth", (16, 16), dict(fc=True, mv_tab=SMOOTH), 0),
+    ("fc-b2-smooth", (16, 16), dict(fc=True, sadfp=True, mv_tab=SMOOTH), 2),
+    ("fc-16x8", (16, 8), dict(fc=True, mv_tab=SMOOTH), 0),
+    ("fc-8x16", (8, 16), dict(fc=True, sadfp=True), 1),
+    ("fc-8x8-smooth", (8, 8), dict(fc=True, mv_tab=SMOOTH), 3),
+    ("sub8-8x4", (8, 4), dict(fc=True, steps=(4,)), 2),
+    ("sub8-4x8", (4, 8), dict(steps=(4,), mv_tab=SMOOTH), 2),
+    ("sub8-4x4", (4, 4), dict(fc=True, steps=(4,)), 2),
+    ("fast", (16, 16), dict(sad_only=True, steps=(16, 4)), -1),
+    ("fast-subpel", (16, 16), dict(sad_only=True, steps=(16, 4), mv_tab=SMOOTH), 2),
+    ("window", (16, 16), dict(fc=True), 4),
+    ("no-snap", (16, 16), dict(snap=False, fc=True), 0),
+    ("wide-rungs", (16, 16), dict(fc=True, steps=(64, 32, 16, 8, 4)), 0),
+    ("rescue", (16, 16), dict(fc=True, rescue=True, rescue_range=10, rescue_thresh=0), 0),
+    ("rescue-smooth", (16, 16), dict(fc=True, rescue=True, rescue_range=8, rescue_thresh=-2, mv_tab=SMOOTH), 2),
+    ("rescue-flat", (16, 8), dict(rescue=True, rescue_range=6, rescue_thresh=0, rescue_fast=False,
+                                  rescue_var=900), 0),
+    ("rescue-b2", (16, 16), dict(fc=True, sadfp=True, rescue=True, rescue_range=8, rescue_thresh=1), 0),
+]
+
+
+@pytest.mark.parametrize("name,shape,kw,pat", CASES, ids=[c[0] for c in CASES])
+def test_motion_search_trace(name, shape, kw, pat):
+    cw, ch = 64, 48
+    bw, bh = shape
+    cur, ref = _clip(cw, ch, 7 + len(name), (3, -5))
+    pad = 32
+    pl = O.planes(ref, cw, ch, pad)
+    lam = 4.65
+    cfg = O.cfg(cw, ch, pad, bw, bh, 16, pat, lam)
+    rng = np.random.default_rng(len(name))
+    blocks = [(x, y) for y in range(0, ch, bh) for x in range(0, cw, bw)]
+    idx = rng.choice(len(blocks), min(len(blocks), 6), replace=False)
+    fired = 0
+    for k, i in enumerate(idx):
+        lx, ly = blocks[i]
+        npreds = k % 4
+        preds = [tuple(int(v) for v in rng.integers(-40, 41, 2)) for _ in range(npreds)]
+        if npreds and k % 2:
+            preds[0] = (4 * int(rng.integers(-6, 7)), 4 * int(rng.integers(-6, 7)))
+        mv, c, ev, tr = O.motion_search_trace(cfg, cur, pl, lx, ly, preds, **kw)
+        wmv, wc, wev, wtr = refimpl.motion_search_walk(cur, ref, lx, ly, bw, bh, preds, lam, lam * 0.5,
+                                                       subpel_pat=pat, **kw)
+        assert tr == wtr, (name, lx, ly, preds, next(j for j, (a, b) in enumerate(zip(tr + [None], wtr + [None]))
+                                                     if a != b))
+        assert (mv, c, ev) == (wmv, wc, wev)
+        fired += ev[0]
+    if kw.get("rescue"):
+        assert fired > 0, "the rescue grid never ran: the case pins nothing"
+
+
+def tie_corpus():
+    """Searches on a texture of period 4 in x and 8 in y at lambda 0, where distinct
+    vectors price exactly alike: (frame, ref, lx, ly, preds, knobs, pattern)."""
+    cw, ch = 64, 48
+    yy, xx = np.mgrid[0:ch, 0:cw]
+    tex = (60 + 40 * (xx % 4) + 17 * (yy % 8)).astype(np.uint8)
+    out = []
+    for kw, pat in ((dict(), 0), (dict(fc=True), 0), (dict(fc=True, sadfp=True), 2), (dict(sad_only=True), -1),
+                    (dict(fc=True, rescue=True, rescue_range=6, rescue_thresh=-1), 3)):
+        for lx, ly, preds in ((16, 16, [(16, 0)]), (32, 16, [(-16, 32), (16, 0)]), (16, 0, [(0, 32)]),
+                              (0, 16, [(4, 2), (16, 32)])):
+            out.append((tex, tex, lx, ly, preds, kw, pat))
+    return out
+
+
+def test_motion_search_ties():
+    """Ties at every phase (seeds, fixed-centre and cascade rings, grid, sub-pel) keep
+    the earlier candidate, as the reference's strict comparisons do."""
+    for cur, ref, lx, ly, preds, kw, pat in tie_corpus():
+        cw, ch = cur.shape[1], cur.shape[0]
+        cfg = O.cfg(cw, ch, 32, 16, 16, 16, pat, 0.0, lam_fp=0.0)
+        got = O.motion_search_trace(cfg, cur, O.planes(ref, cw, ch, 32), lx, ly, preds, **kw)
+        assert got == refimpl.motion_search_walk(cur, ref, lx, ly, 16, 16, preds, 0.0, 0.0, subpel_pat=pat, **kw)
+
+
+def test_motion_search_edges_and_clamped_fc():
+    """Blocks on every frame edge (the fixed-centre box test and the clamped reads),
+    predictors pointing outside the frame."""
+    cw, ch = 48, 48
+    cur, ref = _clip(cw, ch, 33, (-2, 6))
+    pl = O.planes(ref, cw, ch, 32)
+    cfg = O.cfg(cw, ch, 32, 16, 16, 16, 0, 3.0)
+    for lx, ly in ((0, 0), (32, 0), (0, 32), (32, 32), (16, 16)):
+        for preds in ([(-90, 13)], [(140, -77), (4, 4)], []):
+            kw = dict(fc=True, sadfp=True, mv_tab=SMOOTH)
+            mv, c, ev, tr = O.motion_search_trace(cfg, cur, pl, lx, ly, preds, **kw)
+            assert (mv, c, ev, tr) == refimpl.motion_search_walk(cur, ref, lx, ly, 16, 16, preds, 3.0, 1.5, **kw)
+
+
+@pytest.mark.parametrize("seed,shift", [(1, (2, 1)), (2, (-5, 7)), (3, (11, -9))])
+def test_inter_cost_trace(seed, shift):
+    cw, ch = 48, 40
+    cur, ref = _clip(cw, ch, 50 + seed, shift)
+    rng = np.random.default_rng(seed)
+    for by in range(0, ch, 8):
+        for bx in range(0, cw, 8):
+            seed_mv = tuple(4 * int(v) for v in rng.integers(-5, 6, 2)) if (bx + by) % 16 else (0, 0)
+            c, mv, tr = O.inter_cost_trace(cur, cw, ch, ref, bx, by, 8, seed_mv)
+            wc, wmv, wtr = refimpl.inter_cost_walk(cur, ref, bx, by, 8, seed_mv)
+            assert (c, mv, tr) == (wc, wmv, wtr), (bx, by)
+    # static block: the early out prices (0, 0) only
+    c, mv, tr = O.inter_cost_trace(ref, cw, ch, ref, 8, 8, 8, (12, -4))
+    assert (c, mv, len(tr)) == (0, (0, 0), 1)
+
+
+def test_frame_costs_half_walk():
+    """frame_costs (HalfRes) == the raster walk with left/above seeds halved."""
+    mb_w, mb_h = 6, 4
+    cur, ref = _clip(16 * mb_w, 16 * mb_h, 77, (4, -6))
+    hc, hr = O.downsample2x(cur, 16 * mb_w, 16 * mb_h), O.downsample2x(ref, 16 * mb_w, 16 * mb_h)
+    a, b, m = O.frame_costs_half(hc, hr, mb_w, mb_h)
+    wa, wb, wm = refimpl.frame_costs_half_walk(hc, hr, mb_w, mb_h)
+    assert np.array_equal(a.reshape(mb_h, mb_w), wa)
+    assert np.array_equal(b.reshape(mb_h, mb_w), wb)
+    assert np.array_equal(m.reshape(mb_h, mb_w, 2), wm)
+    assert (wm != 0).any()
