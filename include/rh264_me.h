@@ -206,27 +206,52 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value);
 int rh264_predict_blocks(rh264_ctx *ctx, const uint8_t *planes, int cw, int ch, int pad, int bw,
                          int bh, int n, const int32_t *xy, const int32_t *mv, uint8_t *out);
 
-/* Reference-mode motion search of every 16x16 block (motion_search, mb16.rs:
- * 2433-3302, with the default knobs explicit, as the oracle's
- * orc_motion_search_ref): the cost is cost(mv) = SATD (p->hadamard) + lr[rate] with
- * rate = mvbits(mv - preds[0]); seeds (0,0) then the block's predictors
- * preds[(frame*nblk + b)*npreds + i] (quarter pel pairs; strict <); the integer snap
- * ((v + 2) div_euclid 4) * 4 of a fractional winner; the diamond rungs dc->steps
- * (quarter pels, 4-point rings, cascade with a moving centre, each rung repeated
- * until it no longer improves); with dc->sadfp the seed and diamond phases price
- * SAD + lr_fp[rate] and the winner and seed are repriced in SATD (mb16.rs:2739-2742);
- * then the sub-pel stage p->subpel_pat; with dc->snap the seed is kept if it is
- * cheaper (mb16.rs:3297-3300).  npreds 0..8, nsteps 0..8; 16x16 only (other shapes
- * decline). */
+/* Reference-mode motion search of every p->bw x p->bh block (motion_search,
+ * mb16.rs:2433-3302, with every knob explicit; the same steps as the oracle's
+ * orc_motion_search_ex).  Per block: seeds (0,0) then the block's predictors
+ * preds[(frame*nblk + b)*npreds + i] (quarter-pel pairs; strict <), preds[0] pricing
+ * the rate; the integer snap ((v + 2) div_euclid 4) * 4 of a fractional winner;
+ * the diamond rungs dc->steps (quarter pels, each walked until it stops improving):
+ * with dc->fc, on a full-pel centre whose +-step box lies inside the frame, for
+ * 16x16/16x8/8x16/8x8 blocks, the fixed-centre argmin of the 4-point ring
+ * (mb16.rs:2646-2706), otherwise one cascade pass; with dc->sadfp the seed and
+ * diamond phases price SAD and the winner and seed are repriced (mb16.rs:2739-2742);
+ * with dc->rescue the me_wide grid (mb16.rs:2778-2889); then the sub-pel stage
+ * p->subpel_pat; with dc->snap the seed is kept if it is cheaper (mb16.rs:3297-3300).
+ * Rate and prices (mb16.rs:2453-2565): mv_cost 0 = Exp-Golomb lengths priced by
+ * p->lr (cost closure) and p->lr_fp (B2 phase); mv_cost 1 = per-|d| bits
+ * mv_tab[min(|d|, 4095)] (the smooth or measured tables of mb16.rs:505-589, in
+ * quarter bits) priced (int64)(l * rate) with l = lambda / 4 in the cost closure,
+ * lambda_fp in the B2 phase, and lambda in the reference's batched evaluations (the
+ * 16-wide fixed-centre x4 batch outside B2, mb16.rs:2680-2683; the 16x16 rescue
+ * grid inside the frame, mb16.rs:2834; the rescue's distortion estimate).
+ * npreds 0..8, nsteps 0..8; shapes 16x16, 16x8, 8x16, 8x8, 8x4, 4x8, 4x4 (the 8x8
+ * transform: 16x16 only, others decline).  events (may be NULL): per block, bit 0 =
+ * the rescue grid ran, bit 1 = it cut the cost to <= 15/16 (the payoff the
+ * reference's frame-level learning window counts, mb16.rs:2870-2886). */
 typedef struct {
-    int32_t steps[8]; /* diamond rungs in quarter pels, e.g. 16, 8, 4 */
+    int32_t steps[8];      /* diamond rungs in quarter pels, e.g. 16, 8, 4 (Fast: 16, 4) */
     int32_t nsteps;
-    int32_t sadfp;    /* 1: SAD + lr_fp for the seed and diamond phases */
-    int32_t snap;     /* 1: integer snap of a fractional seed, seed kept if cheaper */
+    int32_t sadfp;         /* 1: B2, SAD + lr_fp for the seed and diamond phases */
+    int32_t snap;          /* 1: integer snap of a fractional seed, seed kept if cheaper */
+    int32_t fc;            /* 1: fixed-centre passes (me_fc, the reference's default with asm) */
+    int32_t sad_only;      /* 1: the Fast preset's cost closure, SAD (mb16.rs:2512); no fc, no rescue */
+    int32_t mv_cost;       /* 0: Exp-Golomb with p->lr / p->lr_fp; 1: mv_tab with the lambdas */
+    int32_t rescue;        /* 1: the me_wide rescue grid */
+    int32_t rescue_range;  /* me_range, full pels (default 24), 0..64 */
+    int32_t rescue_thresh; /* me_rescue (default 3): distortion per pixel above which it runs */
+    int32_t rescue_fast;   /* me_fast (default 1): skip the flat-block test */
+    int32_t rescue_var;    /* me_wide_var (default 800): the flat test's variance bound */
+    double lambda;         /* mv_cost 1: lambda_me */
+    double lambda_fp;      /* mv_cost 1: the B2 lambda, lambda_me * RFF_ME_SADL */
+    const uint16_t *mv_tab; /* mv_cost 1: 4096 entries, device memory */
 } rh264_diamond;
 int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc, int nframes,
                         const uint8_t *cur, size_t cur_stride, const uint8_t *planes, size_t planes_stride,
-                        const int32_t *preds, int npreds, int32_t *mv, int32_t *cost);
+                        const int32_t *preds, int npreds, int32_t *mv, int32_t *cost, int32_t *events);
+/* The smooth mvd rate table (mb16.rs:507-516, mv_cost_kind 1): tab[a] = round(4 *
+ * (2 log2(a + 1) + 0.718 + (a != 0))), a = 0..4095, into host memory. */
+void rh264_mv_cost_smooth(uint16_t *tab /* 4096 */);
 
 /* The encoder's per-macroblock 16x16 decision in macroblock order, for P frames
  * whose every macroblock takes its P_L0_16x16 best_part result (mb16.rs:4921-4959
@@ -244,7 +269,10 @@ int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh26
  * until none does (the encoder-order result is the only fixed point; this
  * schedule synchronises the stream).
  * Outputs per macroblock: ref, mv (pair) and cost, int32.  16x16 only; nrefs
- * 1..4. */
+ * 1..4.  The search knobs are rh264_motion_search's, except the rescue: the
+ * reference disables it for the rest of a frame from a learning window over its
+ * first rescues in encoder order (mb16.rs:2870-2886), a sequential latch this
+ * decision does not model, so dc->rescue declines. */
 int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc,
                                   int nframes, int nrefs, double lambda_ref, const uint8_t *cur, size_t cur_stride,
                                   const uint8_t *planes, size_t planes_stride, int32_t *ref, int32_t *mv,

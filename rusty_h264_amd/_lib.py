@@ -30,7 +30,10 @@ class SearchParams(C.Structure):
 
 class Diamond(C.Structure):
     """rh264_diamond (include/rh264_me.h)."""
-    _fields_ = [("steps", C.c_int32 * 8), ("nsteps", C.c_int32), ("sadfp", C.c_int32), ("snap", C.c_int32)]
+    _fields_ = [("steps", C.c_int32 * 8), ("nsteps", C.c_int32), ("sadfp", C.c_int32), ("snap", C.c_int32),
+                ("fc", C.c_int32), ("sad_only", C.c_int32), ("mv_cost", C.c_int32), ("rescue", C.c_int32),
+                ("rescue_range", C.c_int32), ("rescue_thresh", C.c_int32), ("rescue_fast", C.c_int32),
+                ("rescue_var", C.c_int32), ("lam", C.c_double), ("lam_fp", C.c_double), ("mv_tab", C.c_void_p)]
 
 
 class Stream(C.Structure):
@@ -78,8 +81,9 @@ PROTOTYPES = {
                                                    C.c_int, C.c_int, C.c_int, C.c_int32, c_vp, c_vp, c_vp, c_vp,
                                                    c_vp]),
     "rh264_mbtree_propagate_multi_i32": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "rh264_mv_cost_smooth": (None, [C.POINTER(C.c_uint16)]),
     "rh264_motion_search": (C.c_int, [c_vp, C.POINTER(SearchParams), C.POINTER(Diamond), C.c_int, c_vp, C.c_size_t,
-                                      c_vp, C.c_size_t, c_vp, C.c_int, c_vp, c_vp]),
+                                      c_vp, C.c_size_t, c_vp, C.c_int, c_vp, c_vp, c_vp]),
     "rh264_motion_search_wavefront": (C.c_int, [c_vp, C.POINTER(SearchParams), C.POINTER(Diamond), C.c_int,
                                                 C.c_int, C.c_double, c_vp, C.c_size_t, c_vp, C.c_size_t, c_vp, c_vp,
                                                 c_vp]),

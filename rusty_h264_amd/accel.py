@@ -383,40 +383,78 @@ class MbTree:
         return r
 
 
+def mv_cost_smooth():
+    """The smooth mvd rate table (mb16.rs:507-516), 4096 uint16 entries (host)."""
+    t = np.empty(4096, np.uint16)
+    _lib.load().rh264_mv_cost_smooth(t.ctypes.data_as(C.POINTER(C.c_uint16)))
+    return t
+
+
+class DiamondCfg:
+    """rh264_diamond with its rate table in device memory (mv_tab: host uint16 table
+    of 4096 entries or None for Exp-Golomb lengths; lam / lam_fp: the lambdas that
+    price a table's rates).  Use as a context manager or call close()."""
+
+    def __init__(self, dev, steps=(16, 8, 4), sadfp=False, snap=True, fc=False, sad_only=False, mv_tab=None,
+                 lam=0.0, lam_fp=0.0, rescue=False, rescue_range=24, rescue_thresh=3, rescue_fast=True,
+                 rescue_var=800):
+        dc = _lib.Diamond()
+        for i, st in enumerate(steps):
+            dc.steps[i] = int(st)
+        dc.nsteps, dc.sadfp, dc.snap, dc.fc, dc.sad_only = len(steps), int(sadfp), int(snap), int(fc), int(sad_only)
+        dc.rescue, dc.rescue_range, dc.rescue_thresh = int(rescue), int(rescue_range), int(rescue_thresh)
+        dc.rescue_fast, dc.rescue_var = int(rescue_fast), int(rescue_var)
+        dc.lam, dc.lam_fp = float(lam), float(lam_fp)
+        self.tab = None
+        if mv_tab is not None:
+            self.tab = dev.upload(np.ascontiguousarray(mv_tab, np.uint16))
+            dc.mv_cost, dc.mv_tab = 1, self.tab.ptr
+        self.dc = dc
+
+    def close(self):
+        if self.tab is not None:
+            self.tab.free()
+            self.tab = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 def motion_search_ref(dev, params, nframes, cur, planes, preds=None, npreds=0, steps=(16, 8, 4), sadfp=False,
-                      snap=True, cur_stride=None, planes_stride=None):
+                      snap=True, cur_stride=None, planes_stride=None, events=False, **kw):
     """The reference-mode motion search (rh264_motion_search, mb16.rs:2433-3302) of every
-    16x16 block; preds: device buffer [nframes*nblk][npreds][2] (quarter pel), the first
-    predictor pricing the rate.  Returns (mv [n][2], cost [n]) as host arrays."""
-    dc = _lib.Diamond()
-    for i, st in enumerate(steps):
-        dc.steps[i] = int(st)
-    dc.nsteps, dc.sadfp, dc.snap = len(steps), int(sadfp), int(snap)
-    n = (params.cw // 16) * (params.ch // 16) * nframes
-    mv, cost = dev.alloc(8 * n), dev.alloc(4 * n)
+    params.bw x params.bh block; preds: device buffer [nframes*nblk][npreds][2] (quarter
+    pel), the first predictor pricing the rate; kw: the other DiamondCfg knobs.  Returns
+    (mv [n][2], cost [n]) as host arrays, and the rescue events [n] with events=True."""
+    n = (params.cw // params.bw) * (params.ch // params.bh) * nframes
+    mv, cost, ev = dev.alloc(8 * n), dev.alloc(4 * n), dev.alloc(4 * n)
     try:
-        _check(dev.lib.rh264_motion_search(dev.ctx, C.byref(params), C.byref(dc), nframes, int(cur),
-                                           cur_stride or params.cw * params.ch, int(planes),
-                                           planes_stride or planes_bytes(params.cw, params.ch, params.pad),
-                                           _ptr(preds), npreds, mv.ptr, cost.ptr), "rh264_motion_search")
-        return dev.download(mv, (n, 2), np.int32), dev.download(cost, (n,), np.int32)
+        with DiamondCfg(dev, steps=steps, sadfp=sadfp, snap=snap, **kw) as d:
+            _check(dev.lib.rh264_motion_search(dev.ctx, C.byref(params), C.byref(d.dc), nframes, int(cur),
+                                               cur_stride or params.cw * params.ch, int(planes),
+                                               planes_stride or planes_bytes(params.cw, params.ch, params.pad),
+                                               _ptr(preds), npreds, mv.ptr, cost.ptr, ev.ptr), "rh264_motion_search")
+            out = dev.download(mv, (n, 2), np.int32), dev.download(cost, (n,), np.int32)
+            return out + (dev.download(ev, (n,), np.int32),) if events else out
     finally:
         mv.free()
         cost.free()
+        ev.free()
 
 
 def motion_search_wavefront(dev, params, nframes, cur, planes, nrefs=1, lambda_ref=0.0, steps=(16, 8, 4),
-                            sadfp=False, snap=True, cur_stride=None, planes_stride=None, out=None):
+                            sadfp=False, snap=True, cur_stride=None, planes_stride=None, out=None, **kw):
     """Every macroblock of nframes P frames decided in macroblock order with its 16x16
     best_part result over nrefs references (rh264_motion_search_wavefront;
     mb16.rs:4921-4959, predictor predict_mv of the decided neighbours, inter.rs:36-73).
     planes: nframes + nrefs - 1 consecutive plane sets, frame f's reference r being
     set f + nrefs - 1 - r.  out: (ref, mv, cost) device buffers or None; returns them
-    (asynchronous)."""
-    dc = _lib.Diamond()
-    for i, st in enumerate(steps):
-        dc.steps[i] = int(st)
-    dc.nsteps, dc.sadfp, dc.snap = len(steps), int(sadfp), int(snap)
+    (asynchronous).  kw: the other DiamondCfg knobs."""
+    d = DiamondCfg(dev, steps=steps, sadfp=sadfp, snap=snap, **kw)
+    dc = d.dc
     n = (params.cw // 16) * (params.ch // 16) * nframes
     if out is None:
         out = (dev.alloc(4 * n), dev.alloc(8 * n), dev.alloc(4 * n))
@@ -429,6 +467,9 @@ def motion_search_wavefront(dev, params, nframes, cur, planes, nrefs=1, lambda_r
                                                  int(planes),
                                                  planes_stride or planes_bytes(params.cw, params.ch, params.pad),
                                                  ref.ptr, mv.ptr, cost.ptr), "rh264_motion_search_wavefront")
+    if d.tab is not None:
+        dev.sync()  # the table is freed once the launches that read it are done
+    d.close()
     return out
 
 

@@ -1684,15 +1684,25 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
 
 __device__ __forceinline__ int ms_div_euclid4(int v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
 
-// One 16x16 block's reference-mode search (motion_search, mb16.rs:2433-3302, the
-// default knobs explicit) by one wave: the block at (lx, ly) of the frame at
-// frame_px against plane set pl; seeds (0, 0) then pr[0..npreds) (wave-uniform
-// quarter-pel pairs), pr[0] pricing the rate; returns the vector and its cost.
+// One block's reference-mode search (motion_search, mb16.rs:2433-3302, every knob
+// explicit in rh264_diamond) by one wave.  The block (p.bw x p.bh, at (lx, ly) of the
+// frame at frame_px) occupies the top-left of the wave's 16x16 transform tile; lanes
+// outside it hold 0 for both source and prediction, so their 4x4 blocks transform to
+// zero and add nothing to the SATD or the SAD.  Seeds (0, 0) then pr[0..npreds)
+// (wave-uniform quarter-pel pairs), pr[0] pricing the rate.  Phases, in order:
+//   seeds -> snap -> diamond rungs (fixed-centre argmin-of-4 passes where the
+//   reference's x4 batch applies, cascade passes elsewhere) -> B2 reprice -> me_wide
+//   rescue (grid, then the +-1 pel walk) -> sub-pel rings or the qpel window -> seed.
+// Prices: the cost closure dist + lam_r * rate (lam_r = lambda / 4 under a rate
+// table, mb16.rs:2553); the B2 phase SAD + lambda_fp * rate; the reference's
+// batches (fixed-centre x4 of 16-wide blocks, the 16x16 rescue grid) price with
+// lambda_fp under B2 else lambda (mb16.rs:2680-2683, 2834).  oev: bit 0 the rescue
+// grid ran, bit 1 it cut the cost to <= 15/16 (mb16.rs:2878-2880).
 template <bool T8>
 __device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh264_diamond &dc,
                                           const int32_t *s_lr, const int32_t *s_lrfp, const uint8_t *frame_px,
                                           const uint8_t *pl, int lx, int ly, const int32_t *pr, int npreds, int lane,
-                                          int &obx, int &oby, int &obc)
+                                          int &obx, int &oby, int &obc, int &oev)
 {
     MxBlock B;
     B.pw = p.cw + 2 * p.pad;
@@ -1705,20 +1715,44 @@ __device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh
     B.xoff = 0;
     B.mpx = npreds > 0 ? __builtin_amdgcn_readfirstlane(pr[0]) : 0; // the rate predictor
     B.mpy = npreds > 0 ? __builtin_amdgcn_readfirstlane(pr[1]) : 0;
+    const int bw = p.bw, bh = p.bh;
     const int r = lane & 15, qd = lane >> 4;
-    const uint32_t src = *(const uint32_t *)(frame_px + (size_t)(ly + r) * p.cw + lx + 4 * qd);
+    const bool act = r < bh && 4 * qd < bw; // the lane's 4 pixels lie inside the block
+    const uint32_t src = act ? *(const uint32_t *)(frame_px + (size_t)(ly + r) * p.cw + lx + 4 * qd) : 0u;
     const MxConst K = mx_const<T8>(lane);
     const mx_f4 c2 = -mx_transform(src, K, mx_f4{0.0f, 0.0f, 0.0f, 0.0f});
-    auto rate = [&](const int32_t *lr, int mx, int my) { return lr[mvbits(mx - B.mpx) + mvbits(my - B.mpy)]; };
-    auto cost = [&](int mx, int my) -> int { // SATD phase (mc_satd_hp, mb16.rs:1930-2007)
-        const int s = mx_satd<T8>(mx_operand_far(B, lane, mx, my), K, c2);
-        return __builtin_amdgcn_readfirstlane(s + rate(s_lr, mx, my));
+    // rate of a candidate; with a table the components are tab[min(|d|, 4095)]
+    auto bits = [&](int mx, int my) -> int {
+        if (!dc.mv_cost) return mvbits(mx - B.mpx) + mvbits(my - B.mpy);
+        // the index min(|d|, 4095) from a 64-bit difference: any predictor stays in the table
+        const int64_t dx = (int64_t)mx - B.mpx, dy = (int64_t)my - B.mpy;
+        const int ax = (int)min(dx < 0 ? -dx : dx, (int64_t)4095), ay = (int)min(dy < 0 ? -dy : dy, (int64_t)4095);
+        return (int)dc.mv_tab[ax] + (int)dc.mv_tab[ay];
+    };
+    // (int64)(lambda * rate) of mb16.rs:2563; kind 0 = the cost closure's lambda
+    // (lr), 1 = the B2 lambda (lr_fp), 2 = lambda itself
+    auto price = [&](int kind, int rate) -> int {
+        if (!dc.mv_cost) return (kind == 1 ? s_lrfp : s_lr)[rate];
+        const double l = kind == 0 ? dc.lambda * 0.25 : kind == 1 ? dc.lambda_fp : dc.lambda;
+        return (int)(l * (double)rate);
+    };
+    auto operand = [&](int mx, int my) -> uint32_t { return act ? mx_operand_far(B, lane, mx, my) : 0u; };
+    auto dist = [&](int mx, int my, bool sad) -> int {
+        const uint32_t o = operand(mx, my);
+        if (sad) return mx_wave_sum((int)__builtin_amdgcn_sad_u8(src, o, 0u));
+        return mx_satd<T8>(o, K, c2);
+    };
+    auto cost = [&](int mx, int my) -> int { // the cost closure (mb16.rs:2540-2565)
+        return __builtin_amdgcn_readfirstlane(dist(mx, my, dc.sad_only) + price(0, bits(mx, my)));
     };
     auto cost_fp = [&](int mx, int my) -> int { // seed and diamond phases
         if (!dc.sadfp) return cost(mx, my);
-        const int s = mx_wave_sum((int)__builtin_amdgcn_sad_u8(src, mx_operand_far(B, lane, mx, my), 0u));
-        return __builtin_amdgcn_readfirstlane(s + rate(s_lrfp, mx, my));
+        return __builtin_amdgcn_readfirstlane(dist(mx, my, true) + price(1, bits(mx, my)));
     };
+    auto cost_lam = [&](int mx, int my, bool sad, int kind) -> int { // the reference's batches
+        return __builtin_amdgcn_readfirstlane(dist(mx, my, sad) + price(kind, bits(mx, my)));
+    };
+    oev = 0;
     // seeds: (0, 0), then the predictors (strict <)
     int bx = 0, by = 0;
     int bc = cost_fp(0, 0);
@@ -1738,10 +1772,34 @@ __device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh
         by = ms_div_euclid4(by + 2) * 4;
         bc = cost_fp(bx, by);
     }
-    // diamond rungs, each a 4-point ring with a moving centre, repeated while it improves
+    // diamond rungs, each walked until it stops improving
+    const bool fc = dc.fc && !dc.sad_only && (bw == 16 || bw == 8) && (bh == 16 || bh == 8);
+    const int fc_kind = dc.sadfp ? 1 : 2;
     for (int si = 0; si < dc.nsteps; si++) {
         const int step = dc.steps[si];
         for (;;) {
+            if (fc && ((bx | by) & 3) == 0) {
+                const int st = step >> 2, ix = lx + (bx >> 2), iy = ly + (by >> 2);
+                if (ix - st >= 0 && iy - st >= 0 && ix + st + bw <= p.cw && iy + st + bh <= p.ch) {
+                    // fixed-centre pass: the first minimum of the 4-point ring
+                    int bi = -1, bcc = bc;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int x = bx + (i == 0 ? step : i == 1 ? -step : 0);
+                        const int y = by + (i == 2 ? step : i == 3 ? -step : 0);
+                        const int cc = bw == 16 ? cost_lam(x, y, dc.sadfp, fc_kind) : cost_fp(x, y);
+                        if (cc < bcc) {
+                            bcc = cc;
+                            bi = i;
+                        }
+                    }
+                    if (bi < 0) break;
+                    bc = bcc;
+                    bx += bi == 0 ? step : bi == 1 ? -step : 0;
+                    by += bi == 2 ? step : bi == 3 ? -step : 0;
+                    continue;
+                }
+            }
             bool improved = false;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -1760,6 +1818,50 @@ __device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh
     if (dc.sadfp) {
         bc = cost(bx, by);
         seed_c = cost(seedx, seedy);
+    }
+    // me_wide rescue
+    bool gate = dc.rescue && !dc.sad_only;
+    if (gate && !dc.rescue_fast) { // the flat test: the block's variance below rescue_var
+        const int v0 = act ? (int)(src & 255) : 0, v1 = act ? (int)((src >> 8) & 255) : 0;
+        const int v2 = act ? (int)((src >> 16) & 255) : 0, v3 = act ? (int)(src >> 24) : 0;
+        const uint64_t sm = (uint64_t)mx_wave_sum(v0 + v1 + v2 + v3);
+        const uint64_t sq = (uint64_t)mx_wave_sum(v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3);
+        const uint64_t n = (uint64_t)(bw * bh);
+        gate = (sq - sm * sm / n) / n < (uint64_t)dc.rescue_var;
+    }
+    if (gate) {
+        const int dist0 = bc - price(2, bits(bx, by));
+        if (dist0 / (bw * bh) > dc.rescue_thresh) {
+            const int pre_c = bc;
+            const int gx = ms_div_euclid4(bx + 2) * 4, gy = ms_div_euclid4(by + 2) * 4;
+            const int rr = dc.rescue_range, icx = lx + (gx >> 2), icy = ly + (gy >> 2);
+            const bool batched = bw == 16 && bh == 16 && icx >= rr && icx + rr + 16 <= p.cw && icy >= rr &&
+                                 icy + rr + 16 <= p.ch;
+            int gbx = bx, gby = by;
+            for (int dy = -rr; dy <= rr; dy += 2)
+                for (int dx = -rr; dx <= rr; dx += 2) {
+                    const int x = gx + 4 * dx, y = gy + 4 * dy;
+                    const int cc = batched ? cost_lam(x, y, false, 2) : cost(x, y);
+                    if (cc < bc) {
+                        bc = cc;
+                        gbx = x;
+                        gby = y;
+                    }
+                }
+            bx = gbx;
+            by = gby;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int x = bx + 4 * dx, y = by + 4 * dy;
+                    const int cc = cost(x, y);
+                    if (cc < bc) {
+                        bc = cc;
+                        bx = x;
+                        by = y;
+                    }
+                }
+            oev = 1 | ((int64_t)bc * 16 <= (int64_t)pre_c * 15 ? 2 : 0);
+        }
     }
     // the sub-pel stage
     const int pat = p.subpel_pat;
@@ -1811,7 +1913,8 @@ __global__ __launch_bounds__(256) void k_motion_search(rh264_search_params p, rh
                                                        const uint8_t *__restrict__ cur, size_t cur_stride,
                                                        const uint8_t *__restrict__ planes, size_t planes_stride,
                                                        const int32_t *__restrict__ preds, int npreds,
-                                                       int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost)
+                                                       int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost,
+                                                       int32_t *__restrict__ out_ev)
 {
     __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
     if (threadIdx.x < RH264_LR_N) {
@@ -1819,7 +1922,7 @@ __global__ __launch_bounds__(256) void k_motion_search(rh264_search_params p, rh
         s_lrfp[threadIdx.x] = p.lr_fp[threadIdx.x];
     }
     __syncthreads();
-    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
+    const int nbx = p.cw / p.bw, nblk = nbx * (p.ch / p.bh);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     unsigned wgx, wgy;
     xcd_grid2(wgx, wgy);
@@ -1827,14 +1930,15 @@ __global__ __launch_bounds__(256) void k_motion_search(rh264_search_params p, rh
     if (b >= nblk) return;
     const int frame = (int)wgy;
     const size_t boff = (size_t)frame * nblk + b;
-    int bx, by, bc;
+    int bx, by, bc, ev;
     ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride, planes + (size_t)frame * planes_stride,
-                  (b % nbx) << 4, (b / nbx) << 4, preds + boff * (size_t)(npreds > 0 ? npreds : 1) * 2, npreds,
-                  lane, bx, by, bc);
+                  (b % nbx) * p.bw, (b / nbx) * p.bh, preds + boff * (size_t)(npreds > 0 ? npreds : 1) * 2, npreds,
+                  lane, bx, by, bc, ev);
     if (lane == 0) {
         out_mv[2 * boff] = bx;
         out_mv[2 * boff + 1] = by;
         out_cost[boff] = bc;
+        if (out_ev) out_ev[boff] = ev;
     }
 }
 
@@ -1913,10 +2017,10 @@ __global__ __launch_bounds__(256) void k_ms_wavefront(rh264_search_params p, rh2
         ms_predict_mv(out_mv + (size_t)frame * nblk * 2, out_ref + (size_t)frame * nblk, nbx, x, y, rf, px, py);
         pr[0] = __builtin_amdgcn_readfirstlane(px);
         pr[1] = __builtin_amdgcn_readfirstlane(py);
-        int bx, by, bc;
+        int bx, by, bc, ev;
         ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
                       planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, x << 4, y << 4, pr, 1, lane, bx, by,
-                      bc);
+                      bc, ev);
         bc += refc.c[rf];
         if constexpr (RW == 1) {
             if (lane == 0) {
@@ -2017,10 +2121,10 @@ __global__ __launch_bounds__(256) void k_ms_listed(rh264_search_params p, rh264_
         const int i = ok ? __builtin_amdgcn_readfirstlane(list[e]) : 0;
         const int frame = i / nblk, b = i - frame * nblk;
         if (ok && rf < nrefs) {
-            int bx, by, bc;
+            int bx, by, bc, ev;
             ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
                           planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, (b % nbx) << 4, (b / nbx) << 4,
-                          used + (size_t)i * 8 + 2 * rf, 1, lane, bx, by, bc);
+                          used + (size_t)i * 8 + 2 * rf, 1, lane, bx, by, bc, ev);
             bc += refc.c[rf];
             if constexpr (RW == 1) {
                 if (lane == 0) {
@@ -4077,30 +4181,56 @@ int rh264_mbtree_costs(rh264_ctx *ctx, const uint8_t *half, int nframes, int mb_
     return 0;
 }
 
+static int check_diamond(const rh264_diamond *dc, const char *who)
+{
+    if (!dc) return fail("%s: null diamond", who);
+    if (dc->nsteps < 0 || dc->nsteps > 8) return RH264_DECLINED;
+    for (int i = 0; i < dc->nsteps; i++)
+        if (dc->steps[i] < 1 || dc->steps[i] > 1024) return fail("%s: diamond step %d", who, dc->steps[i]);
+    const int32_t flags[] = {dc->sadfp, dc->snap, dc->fc, dc->sad_only, dc->mv_cost, dc->rescue, dc->rescue_fast};
+    for (int32_t f : flags)
+        if (f != 0 && f != 1) return fail("%s: a diamond flag is %d (0 or 1)", who, f);
+    if (dc->mv_cost) {
+        if (!dc->mv_tab) return fail("%s: mv_cost 1 without mv_tab", who);
+        if (!(dc->lambda >= 0.0 && dc->lambda <= 4096.0) || !(dc->lambda_fp >= 0.0 && dc->lambda_fp <= 4096.0))
+            return fail("%s: lambda %g / %g outside 0..4096", who, dc->lambda, dc->lambda_fp);
+    }
+    if (dc->rescue && (dc->rescue_range < 0 || dc->rescue_range > 64 || dc->rescue_var < 0))
+        return fail("%s: rescue range %d / variance bound %d", who, dc->rescue_range, dc->rescue_var);
+    return 0;
+}
+
+void rh264_mv_cost_smooth(uint16_t *tab)
+{
+    for (int a = 0; a < 4096; a++)
+        tab[a] = (uint16_t)std::lround(4.0 * (2.0 * std::log2((double)(a + 1)) + 0.718 + (a != 0 ? 1.0 : 0.0)));
+}
+
 int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc, int nframes,
                         const uint8_t *cur, size_t cur_stride, const uint8_t *planes, size_t planes_stride,
-                        const int32_t *preds, int npreds, int32_t *mv, int32_t *cost)
+                        const int32_t *preds, int npreds, int32_t *mv, int32_t *cost, int32_t *events)
 {
     int rc = check_params(p);
     if (rc) return rc;
-    if (!dc) return fail("motion_search: null diamond");
-    if (p->bw != 16 || p->bh != 16) return RH264_DECLINED;
-    if (npreds < 0 || npreds > 8 || dc->nsteps < 0 || dc->nsteps > 8) return RH264_DECLINED;
+    rc = check_diamond(dc, "motion_search");
+    if (rc) return rc;
+    if (!shape_ok(p->bw, p->bh)) return RH264_DECLINED;
+    if (p->hadamard == 8 && (p->bw != 16 || p->bh != 16)) return RH264_DECLINED;
+    if (npreds < 0 || npreds > 8) return RH264_DECLINED;
     if (npreds > 0 && !preds) return fail("motion_search: null predictors");
-    for (int i = 0; i < dc->nsteps; i++)
-        if (dc->steps[i] < 1 || dc->steps[i] > 1024) return fail("motion_search: diamond step %d", dc->steps[i]);
+    if (!mv || !cost) return fail("motion_search: null output");
     rc = check_strides(cur_stride, planes_stride);
     if (rc) return rc;
     if (nframes <= 0) return 0;
     if (nframes > 65535) return fail("nframes %d > 65535", nframes);
-    const int nblk = (p->cw / 16) * (p->ch / 16);
+    const int nblk = (p->cw / p->bw) * (p->ch / p->bh);
     const dim3 grid((unsigned)((nblk + 3) / 4), nframes);
     if (p->hadamard == 8)
         hipLaunchKernelGGL(k_motion_search<true>, grid, dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes,
-                           planes_stride, preds, npreds, mv, cost);
+                           planes_stride, preds, npreds, mv, cost, events);
     else
         hipLaunchKernelGGL(k_motion_search<false>, grid, dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes,
-                           planes_stride, preds, npreds, mv, cost);
+                           planes_stride, preds, npreds, mv, cost, events);
     LAUNCH_CHECK("k_motion_search");
     return 0;
 }
@@ -4119,11 +4249,10 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
 {
     int rc = check_params(p);
     if (rc) return rc;
-    if (!dc) return fail("motion_search_wavefront: null diamond");
-    if (p->bw != 16 || p->bh != 16) return RH264_DECLINED;
-    if (nrefs < 1 || nrefs > 4 || dc->nsteps < 0 || dc->nsteps > 8) return RH264_DECLINED;
-    for (int i = 0; i < dc->nsteps; i++)
-        if (dc->steps[i] < 1 || dc->steps[i] > 1024) return fail("motion_search: diamond step %d", dc->steps[i]);
+    rc = check_diamond(dc, "motion_search_wavefront");
+    if (rc) return rc;
+    if (p->bw != 16 || p->bh != 16 || dc->rescue) return RH264_DECLINED;
+    if (nrefs < 1 || nrefs > 4) return RH264_DECLINED;
     if (!(lambda_ref >= 0.0 && lambda_ref < 1e8)) return fail("motion_search_wavefront: lambda_ref %g", lambda_ref);
     if (!ref || !mv || !cost) return fail("motion_search_wavefront: null output");
     rc = check_strides(cur_stride, planes_stride);
