@@ -47,3 +47,43 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(RuntimeError):
         _lib.load()
+
+
+# C name of every ctypes mirror field, struct by struct (the mirrors' own names where
+# they differ from the header's)
+_LAYOUTS = {
+    "rh264_search_params": (_lib.SearchParams, {}),
+    "rh264_diamond": (_lib.Diamond, {"lam": "lambda", "lam_fp": "lambda_fp"}),
+    "rh264_stream": (_lib.Stream, {}),
+}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """gcc compiles a probe against include/rh264_me.h that prints sizeof and every
+    field's offsetof; the ctypes mirrors must agree field for field (a field added or
+    retyped on one side only would shift every later one)."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rh264_me.h"', "int main(void) {"]
+    for cname, (cls, rename) in _LAYOUTS.items():
+        lines.append(f'    printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            cf = rename.get(f, f)
+            lines.append(f'    printf("{cname} {f} %zu\\n", offsetof({cname}, {cf}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-o", str(exe),
+                    str(src)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        s, f, v = ln.split()
+        got[(s, f)] = int(v)
+    for cname, (cls, _) in _LAYOUTS.items():
+        assert got[(cname, "sizeof")] == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
