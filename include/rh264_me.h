@@ -86,6 +86,67 @@ int rh264_block_metric(rh264_ctx *ctx, int kind, int w, int h, int n, const uint
                        const int64_t *src_off, const uint8_t *ref, int rs, const int64_t *ref_off,
                        const int64_t *ref_off2, int32_t *out);
 
+/* ---- block metrics, per block (the accel crate's own call shape) ---------
+ * The functions of crates/rusty_h264-accel with its names, argument order (slices
+ * become pointer + stride) and value domains, for drop-in parity with the existing
+ * call sites (crates/rusty_h264-common/src/inter.rs, crates/rusty_h264-encoder/src/
+ * mb16.rs; the call list in accel/src/lib.rs:30-58 and x86_asm.rs:13-21).  HOST
+ * pointers; each call is synchronous: it stages its blocks on the calling thread's
+ * default context (device rh264_accel_set_device, default 0), runs one launch of
+ * the kernels above and returns the values.  Return 0 with *out written, 1 where
+ * the reference returns None / false (a shape outside satd_avg.rs:342-344's set, a
+ * plane hpel_fused leaves to the caller), < 0 on error.  A GPU serves a batch, not
+ * a 17 ns block: these are for parity and for callers that have one block; the
+ * batched entries are the fast path. */
+/* satd_16x16|16x8|8x16|8x8|4x4 (satd_sad.rs:309-340): sum_4x4 ((sum|H d| + 1) >> 1);
+ * sad_16x16|16x8|8x16 (satd_sad.rs:342-363): sum |a - b| */
+int rh264_satd_16x16(const uint8_t *a, size_t sa, const uint8_t *b, size_t sb, int32_t *out);
+int rh264_satd_16x8(const uint8_t *a, size_t sa, const uint8_t *b, size_t sb, int32_t *out);
+int rh264_satd_8x16(const uint8_t *a, size_t sa, const uint8_t *b, size_t sb, int32_t *out);
+int rh264_satd_8x8(const uint8_t *a, size_t sa, const uint8_t *b, size_t sb, int32_t *out);
+int rh264_satd_4x4(const uint8_t *a, size_t sa, const uint8_t *b, size_t sb, int32_t *out);
+int rh264_sad_16x16(const uint8_t *a, size_t sa, const uint8_t *b, size_t sb, int32_t *out);
+int rh264_sad_16x8(const uint8_t *a, size_t sa, const uint8_t *b, size_t sb, int32_t *out);
+int rh264_sad_8x16(const uint8_t *a, size_t sa, const uint8_t *b, size_t sb, int32_t *out);
+/* sad_x4 (satd_avg.rs:462-492) and satd_x4 (388-414): one w x h source block against
+ * base + o[i] (stride rs); SAD, or the unhalved sum|H d|.  Shapes 16x16/16x8/8x16/8x8. */
+int rh264_sad_x4(const uint8_t *src, size_t ss, const uint8_t *base, const size_t o[4], size_t rs, size_t w, size_t h,
+                 uint32_t out[4]);
+int rh264_satd_x4(const uint8_t *src, size_t ss, const uint8_t *base, const size_t o[4], size_t rs, size_t w,
+                  size_t h, uint32_t out[4]);
+/* satd_x4p (satd_avg.rs:350-381): four independent operands r[i] + o[i], stride rs */
+int rh264_satd_x4p(const uint8_t *src, size_t ss, const uint8_t *const r[4], const size_t o[4], size_t rs, size_t w,
+                   size_t h, uint32_t out[4]);
+/* satd_avg_x4 (satd_avg.rs:536-571): sum|H (src - (a+b+1)>>1)| for the pairs
+ * (a[i] + oa[i], b[i] + ob[i]) */
+int rh264_satd_avg_x4(const uint8_t *src, size_t ss, const uint8_t *const a[4], const size_t oa[4],
+                      const uint8_t *const b[4], const size_t ob[4], size_t rs, size_t w, size_t h, uint32_t out[4]);
+/* satd_avg (satd_avg.rs:579-602) */
+int rh264_satd_avg(const uint8_t *src, size_t src_stride, const uint8_t *a, const uint8_t *b, size_t stride, size_t w,
+                   size_t h, uint32_t *out);
+/* hpel_fused (hpel.rs:34-43): H, V, C (each pw x ph, stride pw) of the edge-padded
+ * full-pel plane f, every tap clamped into f; declines (the caller builds them its
+ * own way, as when the reference returns false) unless pw >= 24 and pw, ph are
+ * multiples of 16 (pad 32 planes of 16-multiple frames always are). */
+int rh264_hpel_fused(const uint8_t *f, size_t pw, size_t ph, uint8_t *h, uint8_t *v, uint8_t *c);
+/* MeCtx (mectx.rs:27-166): new validates the geometry once and uploads the four
+ * planes and the source block (1 = None: stride != pw, a shape outside
+ * 16x16/16x8/8x16/8x8, a short slice, a plane too small); eval prices one quarter-pel
+ * candidate, the encoder-domain SATD, 1 (None) outside the validated window
+ * (mectx.rs:114-118); eval_n prices n candidates in two launches, ok[i] = 0 where
+ * eval would return None. */
+typedef struct rh264_mectx rh264_mectx;
+int rh264_mectx_new(const uint8_t *src, size_t src_len, size_t cw, const uint8_t *f, const uint8_t *h,
+                    const uint8_t *v, const uint8_t *c, size_t plane_len, size_t stride, size_t pad, size_t pw,
+                    size_t ph, size_t lx, size_t ly, size_t w, size_t bh, rh264_mectx **out);
+int rh264_mectx_eval(rh264_mectx *m, int32_t mvx, int32_t mvy, uint32_t *out);
+int rh264_mectx_eval_n(rh264_mectx *m, int n, const int32_t *mvs, uint32_t *out, uint8_t *ok);
+int rh264_mectx_free(rh264_mectx *m);
+/* The device of the calling threads' default contexts (before their first call),
+ * and the release of the calling thread's (no reference counterpart). */
+int rh264_accel_set_device(int device);
+int rh264_accel_release(void);
+
 /* ---- half-pel plane sets -------------------------------------------------
  * Replaces build_hpel_planes (crates/rusty_h264-common/src/inter.rs:816-843:
  * pad_plane 786-814 + the fused builder crates/rusty_h264-accel/src/hpel.rs:34-203

@@ -58,6 +58,24 @@ PROTOTYPES = {
     "rh264_event_elapsed_ms": (C.c_int, [c_vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "rh264_block_metric": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.c_int, c_vp, C.c_int, c_vp,
                                      c_vp, C.c_int, c_vp, c_vp, c_vp]),
+    **{f"rh264_{k}_{s}": (C.c_int, [c_vp, C.c_size_t, c_vp, C.c_size_t, c_vp])
+       for k, shapes in (("satd", ("16x16", "16x8", "8x16", "8x8", "4x4")), ("sad", ("16x16", "16x8", "8x16")))
+       for s in shapes},
+    "rh264_sad_x4": (C.c_int, [c_vp, C.c_size_t, c_vp, c_vp, C.c_size_t, C.c_size_t, C.c_size_t, c_vp]),
+    "rh264_satd_x4": (C.c_int, [c_vp, C.c_size_t, c_vp, c_vp, C.c_size_t, C.c_size_t, C.c_size_t, c_vp]),
+    "rh264_satd_x4p": (C.c_int, [c_vp, C.c_size_t, c_vp, c_vp, C.c_size_t, C.c_size_t, C.c_size_t, c_vp]),
+    "rh264_satd_avg_x4": (C.c_int, [c_vp, C.c_size_t, c_vp, c_vp, c_vp, c_vp, C.c_size_t, C.c_size_t, C.c_size_t,
+                                    c_vp]),
+    "rh264_satd_avg": (C.c_int, [c_vp, C.c_size_t, c_vp, c_vp, C.c_size_t, C.c_size_t, C.c_size_t, c_vp]),
+    "rh264_hpel_fused": (C.c_int, [c_vp, C.c_size_t, C.c_size_t, c_vp, c_vp, c_vp]),
+    "rh264_mectx_new": (C.c_int, [c_vp, C.c_size_t, C.c_size_t, c_vp, c_vp, c_vp, c_vp, C.c_size_t, C.c_size_t,
+                                  C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                  C.POINTER(c_vp)]),
+    "rh264_mectx_eval": (C.c_int, [c_vp, C.c_int32, C.c_int32, C.POINTER(C.c_uint32)]),
+    "rh264_mectx_eval_n": (C.c_int, [c_vp, C.c_int, c_vp, c_vp, c_vp]),
+    "rh264_mectx_free": (C.c_int, [c_vp]),
+    "rh264_accel_set_device": (C.c_int, [C.c_int]),
+    "rh264_accel_release": (C.c_int, []),
     "rh264_planes_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     "rh264_build_planes": (C.c_int, [c_vp, c_vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, c_vp]),
     "rh264_lambda_table": (None, [C.c_double, c_i32p]),

@@ -19,6 +19,7 @@ FLAGS = [
 SOURCES = {
     "rh264_me.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "rh264_mvnet.hip": [],
+    "rh264_accel.hip": [],
 }
 
 
