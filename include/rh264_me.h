@@ -226,8 +226,9 @@ int rh264_plain_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes
  * crates/rusty_h264-common/src/prof.rs).  rh264_set_profiling(capacity > 0) makes
  * the next `capacity` searches record HIP events at their phase boundaries on the
  * context's stream, without any host synchronisation.  Phases: plain = integer
- * search, refinement; learned = network, candidate search (local searches,
- * selection, fall-back full search), refinement.  rh264_profile_get(i) returns
+ * search, refinement; learned = network input preparation, network, candidate
+ * search (the local searches and their selection), fall-back full search (with its
+ * merge), refinement.  rh264_profile_get(i) returns
  * search i's time before the refinement and the refinement's time;
  * rh264_profile_phases(i) writes up to `cap` phase durations (ms) and their count.
  * capacity 0 turns profiling off and resets the count. */

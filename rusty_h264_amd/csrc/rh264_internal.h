@@ -24,7 +24,7 @@ int rh264_fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
         if (e_ != hipSuccess) return fail("launch of %s failed: %s", name, hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int RH264_PROF_EV = 4; // phase boundaries per profiled search
+constexpr int RH264_PROF_EV = 6; // phase boundaries per profiled search
 
 struct rh264_ctx {
     int device;
@@ -57,3 +57,9 @@ struct rh264_ctx {
     int decide_mode;              // encoder-order decision: 0 = one launch per anti-diagonal, 1 = fixed-point
                                   // passes, 2 = the fixed point for 1-2 references, else the anti-diagonals
 };
+
+// Network inference (rh264_mvnet_forward) recording `after_prep` (if not null) on the
+// context's stream between the input preparation and the fused network kernel.
+extern "C" int mvnet_forward_ev(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref,
+                                size_t stride, int n, int w, int h, float *logits, int32_t *top8,
+                                hipEvent_t after_prep);

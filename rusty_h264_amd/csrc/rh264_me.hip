@@ -5007,11 +5007,14 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     unsigned *count = (unsigned *)(base + o_count);
     uint64_t *keys8 = (uint64_t *)(base + o_keys), *keysf = keys8 + (size_t)nb * LS_NCAND;
 
-    hipEvent_t *ev = prof_slot(ctx, 3);
+    // phases: input preparation, network, candidate search, fall-back search, refinement
+    hipEvent_t *ev = prof_slot(ctx, 5);
     if (ev) HIP_TRY(hipEventRecord(ev[0], ctx->stream));
-    rc = rh264_mvnet_forward(ctx, net, cur, ref, frame_stride, nframes, p->cw, p->ch, nullptr, top8);
+    rc = mvnet_forward_ev(ctx, net, cur, ref, frame_stride, nframes, p->cw, p->ch, nullptr, top8,
+                          ev && ctx->mvnet_fused ? ev[1] : nullptr);
     if (rc) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
+    if (ev && !ctx->mvnet_fused) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], ctx->stream));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, count, (size_t)1, 0u);
     if (local_range >= 1 && local_range <= 4 && ctx->cand_quad == 2) {
         // one lane per candidate: quad-SAD column groups plus v_sad_u8 for the rest; the
@@ -5085,6 +5088,7 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
         hipLaunchKernelGGL(k_cand_select, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, keys8,
                            centres, nb, local_range, fallback_cost, start, fallback, count, list);
     }
+    if (ev) HIP_TRY(hipEventRecord(ev[3], ctx->stream));
     // the fall-back list is short (a few % of the blocks): the v_sad_u8 variant with two
     // candidates per lane gives it twice the lanes of the quad-SAD one
     {
@@ -5105,11 +5109,11 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     hipLaunchKernelGGL(k_fallback_merge, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, list, count,
                        keysf, p->range, start);
     LAUNCH_CHECK("learned_search select");
-    if (ev) HIP_TRY(hipEventRecord(ev[2], ctx->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[4], ctx->stream));
     rc = launch_refine<16, 16>(ctx, p, nframes, cur, frame_stride, planes, planes_stride, nullptr, nullptr, start,
                                nullptr, nullptr, nullptr, mv, cost, nullptr);
     if (rc) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ev[3], ctx->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[5], ctx->stream));
     return 0;
 }
 

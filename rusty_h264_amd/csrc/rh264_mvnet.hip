@@ -843,8 +843,8 @@ static int mvnet_forward_impl(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *c
     return 0;
 }
 
-int rh264_mvnet_forward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref, size_t stride,
-                        int n, int w, int h, float *logits, int32_t *top8)
+int mvnet_forward_ev(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref, size_t stride, int n,
+                     int w, int h, float *logits, int32_t *top8, hipEvent_t after_prep)
 {
     int rc = mvnet_geom(net, n, w, h);
     if (rc) return rc;
@@ -855,6 +855,7 @@ int rh264_mvnet_forward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, co
         const int tiles_x = (w / 16 + FT - 1) / FT, tiles_y = (h / 16 + FT - 1) / FT;
         hipLaunchKernelGGL(k_prep, dim3(nblocks((long)n * h4 * w4, 256)), dim3(256), 0, ctx->stream, cur, ref, stride,
                            n, w, h, net->x0);
+        if (after_prep) HIP_TRY(hipEventRecord(after_prep, ctx->stream));
         hipLaunchKernelGGL(k_fwd_fused, dim3((unsigned)((long)n * tiles_x * tiles_y)), dim3(256), 0, ctx->stream,
                            net->x0, net->p, net->wf, h4, w4, tiles_x, tiles_y, logits, top8);
         LAUNCH_CHECK("k_fwd_fused");
@@ -869,6 +870,12 @@ int rh264_mvnet_forward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, co
         LAUNCH_CHECK("k_topk");
     }
     return 0;
+}
+
+int rh264_mvnet_forward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref, size_t stride,
+                        int n, int w, int h, float *logits, int32_t *top8)
+{
+    return mvnet_forward_ev(ctx, net, cur, ref, stride, n, w, h, logits, top8, nullptr);
 }
 
 // Forward + backward on a batch; gradients (mean cross-entropy over cells) left in

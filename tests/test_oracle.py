@@ -770,3 +770,23 @@ def test_lookahead_learned_block_brute_force_and_reductions():
     cen[0, 0] = (3, -2)  # frame 1 is frame 0 moved by (-3, +2): its block matches 3 right, 2 up
     c, mv, k, mask = O.la_block_learned(halves, cw, ch, 1, 1, 8, 1, 0, cen, 16, 16)
     assert (c, mv, k, mask) == (0, (24, -16), 1, 0)
+
+
+def test_timed_build_equals_the_checker():
+    """The CPU baseline build (-O3 -march=x86-64-v3, psadbw SAD) and the checker build
+    give identical SADs and identical plain-search results."""
+    fast = Oracle(fast=True)
+    rng = np.random.default_rng(90)
+    a = rng.integers(0, 256, (40, 45), dtype=np.uint8).reshape(-1)
+    b = rng.integers(0, 256, (40, 51), dtype=np.uint8).reshape(-1)
+    for w, h in ((16, 16), (16, 8), (8, 16), (8, 8), (8, 4), (4, 4)):
+        for oa, ob in ((0, 0), (3, 7), (45 + 1, 2 * 51 + 5)):
+            assert fast.sad(a[oa:], 45, b[ob:], 51, w, h) == O.sad(a[oa:], 45, b[ob:], 51, w, h)
+    cw, ch, pad = 64, 48, 32
+    cur, ref = _pair(cw, ch, 91)
+    for bw, bh in ((16, 16), (8, 8)):
+        r1 = O.plain_search_frame(O.cfg(cw, ch, pad, bw, bh, 6, 0, 4.65), cur, O.planes(ref, cw, ch, pad))
+        r2 = fast.plain_search_frame(fast.cfg(cw, ch, pad, bw, bh, 6, 0, 4.65), cur, fast.planes(ref, cw, ch, pad),
+                                     nthreads=2)
+        for k in r1:
+            assert np.array_equal(r1[k], r2[k]), k
