@@ -221,6 +221,20 @@ int rh264_plain_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes
                        size_t planes_stride, const int32_t *centres, const int32_t *mvps,
                        int32_t *int_mv, int32_t *int_cost, int32_t *mv, int32_t *cost);
 
+/* rh264_plain_search that allocates nothing (no counterpart in the reference: it is
+ * for callers that cannot allocate device memory, such as code under HIP graph
+ * capture or a serving loop).  Same arguments, results and profiling phases as
+ * rh264_plain_search; its argmin keys live in a static device array of
+ * RH264_NOALLOC_KEYS keys (2 MiB) that each device holds until the process ends,
+ * cleared by a fill kernel at the start of every call.  Every call on a device
+ * shares that device's array, so calls that overlap on one device (two contexts or
+ * two host threads) race.  Errors when nframes x blocks per frame exceeds
+ * RH264_NOALLOC_KEYS. */
+#define RH264_NOALLOC_KEYS (1u << 18)
+int rh264_plain_search_noalloc(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                               size_t cur_stride, const uint8_t *planes, size_t planes_stride, const int32_t *centres,
+                               const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv, int32_t *cost);
+
 /* Stage timing of rh264_plain_search and rh264_learned_search (no reference
  * counterpart; the reference's stage profiler is
  * crates/rusty_h264-common/src/prof.rs).  rh264_set_profiling(capacity > 0) makes

@@ -99,6 +99,8 @@ PROTOTYPES = {
                                                    C.c_int, C.c_int, C.c_int, C.c_int32, c_vp, c_vp, c_vp, c_vp,
                                                    c_vp]),
     "rh264_mbtree_propagate_multi_i32": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "rh264_plain_search_noalloc": (C.c_int, [c_vp, C.POINTER(SearchParams), C.c_int, c_vp, C.c_size_t, c_vp,
+                                             C.c_size_t, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "rh264_mv_cost_smooth": (None, [C.POINTER(C.c_uint16)]),
     "rh264_motion_search": (C.c_int, [c_vp, C.POINTER(SearchParams), C.POINTER(Diamond), C.c_int, c_vp, C.c_size_t,
                                       c_vp, C.c_size_t, c_vp, C.c_int, c_vp, c_vp, c_vp]),

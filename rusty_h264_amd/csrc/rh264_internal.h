@@ -31,6 +31,7 @@ struct rh264_ctx {
     hipStream_t stream;
     hipEvent_t ev[16];
     uint64_t *keys;               // per-block argmin keys of the full search
+    uint64_t *noalloc_keys;       // this device's static key array (rh264_plain_search_noalloc)
     size_t keys_cap;              // in elements
     int32_t *tmp_mv;              // integer-stage MVs when the caller does not want them
     size_t tmp_mv_cap;            // in int32 elements

@@ -49,6 +49,12 @@ int rh264_fail(const char *fmt, ...)
     return -1;
 }
 
+// The argmin keys of rh264_plain_search_noalloc: a static device array, one per device
+// the library runs on (each holds it until the process ends).  Host code never passes
+// its name to a kernel: rh264_init looks up its device address on the context's
+// device and keeps it in ctx->noalloc_keys.
+static __device__ __align__(256) uint64_t scratch[RH264_NOALLOC_KEYS];
+
 static int ensure_keys(rh264_ctx *ctx, size_t n)
 {
     if (ctx->keys_cap >= n) return 0;
@@ -3656,6 +3662,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->device = device;
     c->keys = nullptr;
     c->keys_cap = 0;
+    c->noalloc_keys = nullptr;
     c->tmp_mv = nullptr;
     c->tmp_mv_cap = 0;
     c->prof_cap = c->prof_n = 0;
@@ -3682,6 +3689,15 @@ int rh264_init(int device, rh264_ctx **out)
         return fail("hipStreamCreate: %s", hipGetErrorString(e));
     }
     for (int i = 0; i < 16; i++) HIP_TRY(hipEventCreate(&c->ev[i]));
+    // this device's copy of the no-allocation scratch (its address differs per device)
+    void *sp = nullptr;
+    e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail("hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    }
+    c->noalloc_keys = (uint64_t *)sp;
     *out = c;
     return 0;
 }
@@ -3937,27 +3953,43 @@ extern "C" {
     default: return RH264_DECLINED;                                                           \
     }
 
+// The integer search into `keys` (nframes x blocks argmin keys, cleared here by a
+// fill kernel) and the refinement from them; neither allocates.
+static int full_search_into(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                            size_t cur_stride, const uint8_t *planes, size_t planes_stride,
+                            const int32_t *centres, const int32_t *mvps, uint64_t *keys)
+{
+    const size_t nk = (size_t)(p->cw / p->bw) * (p->ch / p->bh) * (size_t)nframes;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, ctx->stream, keys, nk, ~0ull);
+    LAUNCH_CHECK("k_fill_u64");
+    SHAPE_DISPATCH(launch_full, ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps, keys);
+}
+
+static int refine_from(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                       size_t cur_stride, const uint8_t *planes, size_t planes_stride, const uint64_t *keys,
+                       const int32_t *centres, const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv,
+                       int32_t *cost)
+{
+    SHAPE_DISPATCH(launch_refine, ctx, p, nframes, cur, cur_stride, planes, planes_stride, keys, centres, nullptr,
+                   mvps, int_mv, int_cost, mv, cost, nullptr);
+}
+
 static int full_search_keys(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
                             size_t cur_stride, const uint8_t *planes, size_t planes_stride,
                             const int32_t *centres, const int32_t *mvps)
 {
-    const size_t nblk = (size_t)(p->cw / p->bw) * (p->ch / p->bh);
-    const size_t nk = nblk * (size_t)nframes;
+    const size_t nk = (size_t)(p->cw / p->bw) * (p->ch / p->bh) * (size_t)nframes;
     int rc = ensure_keys(ctx, nk);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, ctx->stream, ctx->keys, nk,
-                       ~0ull);
-    LAUNCH_CHECK("k_fill_u64");
-    SHAPE_DISPATCH(launch_full, ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps,
-                   ctx->keys);
+    return full_search_into(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps, ctx->keys);
 }
 
 static int refine_keys(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
                        size_t cur_stride, const uint8_t *planes, size_t planes_stride, const int32_t *centres,
                        const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv, int32_t *cost)
 {
-    SHAPE_DISPATCH(launch_refine, ctx, p, nframes, cur, cur_stride, planes, planes_stride, ctx->keys, centres,
-                   nullptr, mvps, int_mv, int_cost, mv, cost, nullptr);
+    return refine_from(ctx, p, nframes, cur, cur_stride, planes, planes_stride, ctx->keys, centres, mvps, int_mv,
+                       int_cost, mv, cost);
 }
 
 int rh264_full_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
@@ -4025,6 +4057,34 @@ int rh264_plain_search(rh264_ctx *ctx, const rh264_search_params *p, int nframes
     if (ev) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
     rc = refine_keys(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps, int_mv, int_cost,
                      mv, cost);
+    if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[2], ctx->stream));
+    return 0;
+}
+
+int rh264_plain_search_noalloc(rh264_ctx *ctx, const rh264_search_params *p, int nframes, const uint8_t *cur,
+                               size_t cur_stride, const uint8_t *planes, size_t planes_stride, const int32_t *centres,
+                               const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv, int32_t *cost)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (!shape_ok(p->bw, p->bh)) return RH264_DECLINED;
+    rc = check_strides(cur_stride, planes_stride);
+    if (rc) return rc;
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    if (!mv || !cost) return fail("plain_search_noalloc: null outputs");
+    const size_t nk = (size_t)(p->cw / p->bw) * (p->ch / p->bh) * (size_t)nframes;
+    if (nk > (size_t)RH264_NOALLOC_KEYS)
+        return fail("plain_search_noalloc: %zu blocks exceed the static scratch's %u", nk, (unsigned)RH264_NOALLOC_KEYS);
+    if (!ctx->noalloc_keys) return fail("plain_search_noalloc: no scratch address for this context");
+    hipEvent_t *ev = prof_slot(ctx, 2);
+    if (ev) HIP_TRY(hipEventRecord(ev[0], ctx->stream));
+    rc = full_search_into(ctx, p, nframes, cur, cur_stride, planes, planes_stride, centres, mvps, ctx->noalloc_keys);
+    if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
+    rc = refine_from(ctx, p, nframes, cur, cur_stride, planes, planes_stride, ctx->noalloc_keys, centres, mvps, int_mv,
+                     int_cost, mv, cost);
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[2], ctx->stream));
     return 0;
