@@ -376,14 +376,24 @@ int rh264_decision_predict(rh264_ctx *ctx, int cw, int ch, int pad, int nframes,
  * 1.. is below res_min > 0); offsets -strength*log2((intra + propagated)/intra)
  * centred on the GOP mean; the differentiation latch (all zero when the RMS of the
  * centred offsets / strength is below sd_min); round half away from zero, clamp to
- * +-6.  intra/inter [nframes][mb_w*mb_h] with 1 <= intra, 0 <= inter <= intra (as
+ * +-6; with grain set (the grain latch of mbtree.rs:401-420, from
+ * rh264_grain_signature below) and nframes >= 2 every offset is 0 and so are the
+ * stats.  intra/inter [nframes][mb_w*mb_h] with 1 <= intra, 0 <= inter <= intra (as
  * rh264_mbtree_costs writes them), mv pairs alike; offsets the same shape; stats
  * (may be NULL) = {residual fraction, spread, effective strength}.  The reference's
  * defaults: strength 0.9, res_min 0.03 (RFF_MBTREE_RESMIN), sd_min = the
  * configuration's mbtree_spread_min, 0.0 by default (config.rs:566). */
 int rh264_mbtree_qp_offsets(int nframes, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
-                            const int32_t *mv, double strength, double res_min, double sd_min, int32_t *offsets,
-                            double *stats);
+                            const int32_t *mv, double strength, double res_min, double sd_min, int grain,
+                            int32_t *offsets, double *stats);
+/* The grain signature that feeds `grain` above (FrameSignals::grain_signature,
+ * crates/rusty_h264-encoder/src/signals.rs:528-533, of the GOP's coded-size frames 1
+ * and 0 at mbtree.rs:404-406), on the HOST (host pointers, cw x ch luma, multiples of
+ * 16): *grain = 1 when the median macroblock variance (signals.rs:247-266) is below
+ * 200, the lower-quartile zero-motion residual per pixel (grain_floor, 311-338) is
+ * above 5 and the +-8 full-pel grid's mean SAD gain (b2_mgain, 61-129) is below
+ * 0.1.  stats (may be NULL) = those three values. */
+int rh264_grain_signature(const uint8_t *cur, const uint8_t *ref, int cw, int ch, int32_t *grain, double *stats);
 
 /* ---- batched multi-stream search ------------------------------------------
  * Many streams of uneven size, 1..4 references each, several partition shapes,

@@ -1407,16 +1407,110 @@ void orc_propagate_f64(int n, int mb_w, int mb_h, const int32_t *intra, const in
     }
 }
 
-/* gop_qp_offsets' policy (mbtree.rs:369-593) over given frame costs: f64
- * propagation, back-off latch, centred -s*log2 offsets, spread latch, round/clamp. */
+/* ---- the grain signature (signals.rs:528-533) of frame cur against ref -------
+ * median_var: frame_median_mb_var (signals.rs:247-266), per macroblock the variance
+ *   of its 64 samples at even rows and columns, (sq - sum*sum/64)/64 in int64, the
+ *   median (sorted, element n/2);
+ * floor: grain_floor (signals.rs:311-338), the zero-motion SAD per pixel of ~48
+ *   interior macroblocks (the interior walk of mbw-4 x mbh-4 from (2,2), stride
+ *   max(inner/48, 1)), the element len/4 of the sorted values; 0 below 6x6 MBs;
+ * mgain: b2_mgain (signals.rs:61-129), over ~24 interior macroblocks (stride
+ *   max(inner/24, 1)) the mean of (SAD0 - min SAD over the +-8 step-4 full-pel grid
+ *   inside the frame) / (SAD0 + 1); 0 below 6x6 MBs;
+ * grain = median_var < 200 && floor > 5 && mgain < 0.1.  stats: the three values. */
+static uint32_t sad16_at(const uint8_t *cur, const uint8_t *ref, int cw, int bx, int by, int rx, int ry)
+{
+    uint32_t s = 0;
+    for (int dy = 0; dy < 16; dy++)
+        for (int dx = 0; dx < 16; dx++)
+            s += (uint32_t)iabs((int)cur[(size_t)(by + dy) * cw + bx + dx] - (int)ref[(size_t)(ry + dy) * cw + rx + dx]);
+    return s;
+}
+
+static int cmp_i64(const void *a, const void *b)
+{
+    int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    return (x > y) - (x < y);
+}
+
+static int cmp_f64(const void *a, const void *b)
+{
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+int orc_grain_signature(const uint8_t *cur, const uint8_t *ref, int cw, int ch, double *stats)
+{
+    const int mbw = cw / 16, mbh = ch / 16;
+    int64_t *vs = (int64_t *)malloc(sizeof(int64_t) * (size_t)(mbw * mbh > 0 ? mbw * mbh : 1));
+    int nv = 0;
+    for (int my = 0; my < mbh; my++)
+        for (int mx = 0; mx < mbw; mx++) {
+            int64_t sum = 0, sq = 0;
+            for (int r = 0; r < 16; r += 2)
+                for (int c = 0; c < 16; c += 2) {
+                    int64_t v = cur[(size_t)(my * 16 + r) * cw + mx * 16 + c];
+                    sum += v;
+                    sq += v * v;
+                }
+            vs[nv++] = (sq - sum * sum / 64) / 64;
+        }
+    qsort(vs, (size_t)nv, sizeof(int64_t), cmp_i64);
+    const int64_t median_var = nv ? vs[nv / 2] : 0;
+    free(vs);
+    double floor_v = 0.0, mgain = 0.0;
+    if (mbw >= 6 && mbh >= 6) {
+        const int inner = (mbw - 4) * (mbh - 4);
+        /* grain_floor */
+        int stride = inner / 48 > 1 ? inner / 48 : 1;
+        double *fl = (double *)malloc(sizeof(double) * (size_t)(inner / stride + 2));
+        int nf = 0;
+        for (int i = 0; i < inner; i += stride) {
+            const int bx = (2 + i % (mbw - 4)) * 16, by = (2 + i / (mbw - 4)) * 16;
+            fl[nf++] = (double)sad16_at(cur, ref, cw, bx, by, bx, by) / 256.0;
+        }
+        qsort(fl, (size_t)nf, sizeof(double), cmp_f64);
+        floor_v = nf ? fl[nf / 4] : 0.0;
+        free(fl);
+        /* b2_mgain */
+        stride = inner / 24 > 1 ? inner / 24 : 1;
+        double acc = 0.0;
+        unsigned n = 0;
+        for (int i = 0; i < inner; i += stride) {
+            const int bx = (2 + i % (mbw - 4)) * 16, by = (2 + i / (mbw - 4)) * 16;
+            const uint32_t s0 = sad16_at(cur, ref, cw, bx, by, bx, by);
+            uint32_t best = s0;
+            for (int dy = -8; dy <= 8; dy += 4)
+                for (int dx = -8; dx <= 8; dx += 4) {
+                    const int rx = bx + dx, ry = by + dy;
+                    if (rx < 0 || ry < 0 || rx + 16 > cw || ry + 16 > ch) continue;
+                    const uint32_t sv = sad16_at(cur, ref, cw, bx, by, rx, ry);
+                    if (sv < best) best = sv;
+                }
+            acc += (double)(s0 - best) / (double)(s0 + 1);
+            n++;
+        }
+        mgain = n ? acc / (double)n : 0.0;
+    }
+    if (stats) {
+        stats[0] = (double)median_var;
+        stats[1] = floor_v;
+        stats[2] = mgain;
+    }
+    return median_var < 200 && floor_v > 5.0 && mgain < 0.1;
+}
+
+/* gop_qp_offsets' policy (mbtree.rs:369-593) over given frame costs: the grain latch
+ * (mbtree.rs:401-420: with grain set and n >= 2 every offset is 0), f64 propagation,
+ * back-off latch, centred -s*log2 offsets, spread latch, round/clamp. */
 void orc_mbtree_qp_offsets(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
-                           const int32_t *mv, double strength, double res_min, double sd_min, int32_t *off,
+                           const int32_t *mv, double strength, double res_min, double sd_min, int grain, int32_t *off,
                            double *stats)
 {
     size_t nm = (size_t)mb_w * mb_h, N = (size_t)n * nm;
     double resid = 0.0, sd = 0.0, eff = 0.0;
     memset(off, 0, sizeof(int32_t) * N);
-    if (strength > 0.0 && N > 0) {
+    if (strength > 0.0 && N > 0 && !(grain && n >= 2)) {
         double *prop = (double *)malloc(sizeof(double) * N), *o = (double *)malloc(sizeof(double) * N);
         orc_propagate_f64(n, mb_w, mb_h, intra, inter, mv, prop);
         double fsum = 0.0, fc = 0.0;

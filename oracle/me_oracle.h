@@ -176,10 +176,15 @@ void orc_propagate_f64(int n, int mb_w, int mb_h, const int32_t *intra, const in
 /* gop_qp_offsets' policy (mbtree.rs:369-593) over frame costs [n][mb]: the f64
  * propagation above, the back-off latch (res_min), -strength*log2(total/intra)
  * centred on the GOP mean, the spread latch (sd_min), round half away from zero,
- * clamp +-6.  stats (may be NULL): residual fraction, spread, effective strength. */
+ * clamp +-6; with grain set (the grain latch, mbtree.rs:401-420) and n >= 2 every offset is
+ * 0.  stats (may be NULL): residual fraction, spread, effective strength. */
 void orc_mbtree_qp_offsets(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
-                           const int32_t *mv, double strength, double res_min, double sd_min, int32_t *off,
+                           const int32_t *mv, double strength, double res_min, double sd_min, int grain, int32_t *off,
                            double *stats);
+/* grain_signature (signals.rs:528-533) of coded-size frame cur against ref (the GOP's
+ * frames 1 and 0, mbtree.rs:404-406): 1 = grain; stats = {median variance, residual
+ * floor, motion gain} (signals.rs:247-266, 311-338, 61-129). */
+int orc_grain_signature(const uint8_t *cur, const uint8_t *ref, int cw, int ch, double *stats);
 /* int32 fixed-point propagation (build-defined): amount_q = floor((intra+prop)
  * * (intra-inter) / intra) in int64 then per-target weights (wx*wy) >> 8. */
 void orc_propagate_i32(int n, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,

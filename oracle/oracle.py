@@ -256,7 +256,15 @@ class Oracle:
         return smv, pred, sp
 
     # ---- mb-tree
-    def mbtree_qp_offsets(self, intra, inter, mv, strength=0.9, res_min=0.03, sd_min=0.0):
+    def grain_signature(self, cur, ref):
+        """grain_signature (signals.rs:528-533): (grain bool, {median_var, floor, mgain})."""
+        cur, ref = _u8(cur), _u8(ref)
+        ch, cw = cur.shape
+        st = np.zeros(3, np.float64)
+        g = self.lib.orc_grain_signature(_p(cur), _p(ref), cw, ch, st.ctypes.data_as(C.POINTER(C.c_double)))
+        return bool(g), {"median_var": float(st[0]), "floor": float(st[1]), "mgain": float(st[2])}
+
+    def mbtree_qp_offsets(self, intra, inter, mv, strength=0.9, res_min=0.03, sd_min=0.0, grain=False):
         """gop_qp_offsets' policy on frame costs [n][mb_h][mb_w]: (offsets, stats)."""
         intra = np.ascontiguousarray(intra, np.int32)
         inter = np.ascontiguousarray(inter, np.int32)
@@ -265,8 +273,8 @@ class Oracle:
         off = np.empty_like(intra)
         st = np.zeros(3, np.float64)
         self.lib.orc_mbtree_qp_offsets(n, mb_w, mb_h, _p(intra, i32p), _p(inter, i32p), _p(mv, i32p),
-                                       C.c_double(strength), C.c_double(res_min), C.c_double(sd_min), _p(off, i32p),
-                                       st.ctypes.data_as(C.POINTER(C.c_double)))
+                                       C.c_double(strength), C.c_double(res_min), C.c_double(sd_min), int(grain),
+                                       _p(off, i32p), st.ctypes.data_as(C.POINTER(C.c_double)))
         return off, {"residual_frac": float(st[0]), "spread": float(st[1]), "eff_strength": float(st[2])}
 
     def downsample2x(self, y, cw, ch):

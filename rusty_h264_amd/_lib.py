@@ -110,7 +110,8 @@ PROTOTYPES = {
     "rh264_decision_predict": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, c_vp, C.c_size_t, c_vp,
                                          c_vp, c_vp, c_vp, c_vp]),
     "rh264_mbtree_qp_offsets": (C.c_int, [C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp, C.c_double, C.c_double,
-                                          C.c_double, c_vp, c_vp]),
+                                         C.c_double, C.c_int, c_vp, c_vp]),
+    "rh264_grain_signature": (C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.POINTER(C.c_int32), c_vp]),
     "rh264_rc_complexity": (C.c_int, [c_vp, c_vp, c_vp, C.c_size_t, C.c_int, C.c_int, C.c_int, c_vp]),
     "rh264_batch_out_len": (C.c_size_t, [C.c_int, C.c_int, c_i32p, C.c_int]),
     "rh264_batch_mvp_len": (C.c_size_t, [C.c_int, C.c_int, c_i32p, C.c_int]),

@@ -491,11 +491,25 @@ def decision_predict(dev, cw, ch, pad, nframes, nrefs, planes, ref, mv, planes_s
             x.free()
 
 
-def mbtree_qp_offsets(lib, intra, inter, mv, strength=0.9, res_min=0.03, sd_min=0.0):
+def grain_signature(lib, cur, ref):
+    """FrameSignals::grain_signature (signals.rs:528-533) of host frames cur against
+    ref, through the library's host function: (grain bool, {median_var, floor, mgain})."""
+    cur = np.ascontiguousarray(cur, np.uint8)
+    ref = np.ascontiguousarray(ref, np.uint8)
+    ch, cw = cur.shape
+    g = C.c_int32()
+    st = np.zeros(3, np.float64)
+    _check(lib.rh264_grain_signature(cur.ctypes.data, ref.ctypes.data, cw, ch, C.byref(g), st.ctypes.data),
+           "rh264_grain_signature")
+    return bool(g.value), {"median_var": float(st[0]), "floor": float(st[1]), "mgain": float(st[2])}
+
+
+def mbtree_qp_offsets(lib, intra, inter, mv, strength=0.9, res_min=0.03, sd_min=0.0, grain=False):
     """gop_qp_offsets' policy (mbtree.rs:369-593) on host arrays of frame costs
     ([n][mb_h][mb_w] intra/inter, [n][mb_h][mb_w][2] mv), through the library's host
-    function rh264_mbtree_qp_offsets.  Returns (offsets int32 like intra, stats
-    {residual_frac, spread, eff_strength})."""
+    function rh264_mbtree_qp_offsets; grain: the GOP's grain signature (the latch
+    zeroes every offset).  Returns (offsets int32 like intra, stats {residual_frac,
+    spread, eff_strength})."""
     intra = np.ascontiguousarray(intra, np.int32)
     inter = np.ascontiguousarray(inter, np.int32)
     mv = np.ascontiguousarray(mv, np.int32)
@@ -503,8 +517,8 @@ def mbtree_qp_offsets(lib, intra, inter, mv, strength=0.9, res_min=0.03, sd_min=
     off = np.empty_like(intra)
     st = np.zeros(3, np.float64)
     _check(lib.rh264_mbtree_qp_offsets(n, mb_w, mb_h, intra.ctypes.data, inter.ctypes.data, mv.ctypes.data,
-                                       float(strength), float(res_min), float(sd_min), off.ctypes.data,
-                                       st.ctypes.data), "rh264_mbtree_qp_offsets")
+                                       float(strength), float(res_min), float(sd_min), int(grain),
+                                       off.ctypes.data, st.ctypes.data), "rh264_mbtree_qp_offsets")
     return off, {"residual_frac": float(st[0]), "spread": float(st[1]), "eff_strength": float(st[2])}
 
 

@@ -4433,9 +4433,83 @@ static void mbt_propagate_to(double *prev, int mb_w, int mb_h, int mb_x, int mb_
     }
 }
 
+// The grain signature (signals.rs:528-533): three frame-level probes of the GOP's
+// frames 1 and 0, each over the interior macroblock walk the reference uses
+// (columns and rows 2 .. mb-3 in raster order, every stride-th).
+static uint32_t gs_sad16(const uint8_t *cur, const uint8_t *ref, int cw, int bx, int by, int rx, int ry)
+{
+    uint32_t s = 0;
+    for (int y = 0; y < 16; y++) {
+        const uint8_t *a = cur + (size_t)(by + y) * cw + bx, *b = ref + (size_t)(ry + y) * cw + rx;
+        for (int x = 0; x < 16; x++) s += (uint32_t)std::abs((int)a[x] - (int)b[x]);
+    }
+    return s;
+}
+
+int rh264_grain_signature(const uint8_t *cur, const uint8_t *ref, int cw, int ch, int32_t *grain, double *stats)
+{
+#pragma clang fp contract(off)
+    if (!cur || !ref || !grain) return fail("grain_signature: null pointer");
+    if (cw <= 0 || ch <= 0 || cw % 16 || ch % 16) return fail("grain_signature: %dx%d is not a macroblock grid", cw, ch);
+    const int mbw = cw / 16, mbh = ch / 16;
+    // median macroblock variance over the even rows and columns (signals.rs:247-266)
+    std::vector<int64_t> vars;
+    vars.reserve((size_t)mbw * mbh);
+    for (int my = 0; my < mbh; my++)
+        for (int mx = 0; mx < mbw; mx++) {
+            int64_t sum = 0, sq = 0;
+            for (int r = 0; r < 16; r += 2) {
+                const uint8_t *row = cur + (size_t)(my * 16 + r) * cw + mx * 16;
+                for (int c = 0; c < 16; c += 2) {
+                    sum += row[c];
+                    sq += (int64_t)row[c] * row[c];
+                }
+            }
+            vars.push_back((sq - sum * sum / 64) / 64);
+        }
+    std::sort(vars.begin(), vars.end());
+    const int64_t median_var = vars[vars.size() / 2];
+    double floor_v = 0.0, mgain = 0.0;
+    if (mbw >= 6 && mbh >= 6) {
+        const int inner = (mbw - 4) * (mbh - 4);
+        // residual floor: the lower quartile of ~48 zero-motion SADs per pixel (signals.rs:311-338)
+        std::vector<double> fl;
+        for (int i = 0, st = std::max(inner / 48, 1); i < inner; i += st) {
+            const int bx = (2 + i % (mbw - 4)) * 16, by = (2 + i / (mbw - 4)) * 16;
+            fl.push_back((double)gs_sad16(cur, ref, cw, bx, by, bx, by) / 256.0);
+        }
+        std::sort(fl.begin(), fl.end());
+        floor_v = fl[fl.size() / 4];
+        // motion gain: what a +-8 step-4 full-pel grid saves on ~24 blocks (signals.rs:61-129)
+        double acc = 0.0;
+        unsigned cnt = 0;
+        for (int i = 0, st = std::max(inner / 24, 1); i < inner; i += st) {
+            const int bx = (2 + i % (mbw - 4)) * 16, by = (2 + i / (mbw - 4)) * 16;
+            const uint32_t s0 = gs_sad16(cur, ref, cw, bx, by, bx, by);
+            uint32_t best = s0;
+            for (int dy = -8; dy <= 8; dy += 4)
+                for (int dx = -8; dx <= 8; dx += 4) {
+                    const int rx = bx + dx, ry = by + dy;
+                    if (rx >= 0 && ry >= 0 && rx + 16 <= cw && ry + 16 <= ch)
+                        best = std::min(best, gs_sad16(cur, ref, cw, bx, by, rx, ry));
+                }
+            acc += (double)(s0 - best) / (double)(s0 + 1);
+            cnt++;
+        }
+        mgain = cnt ? acc / (double)cnt : 0.0;
+    }
+    *grain = median_var < 200 && floor_v > 5.0 && mgain < 0.1;
+    if (stats) {
+        stats[0] = (double)median_var;
+        stats[1] = floor_v;
+        stats[2] = mgain;
+    }
+    return 0;
+}
+
 int rh264_mbtree_qp_offsets(int nframes, int mb_w, int mb_h, const int32_t *intra, const int32_t *inter,
-                            const int32_t *mv, double strength, double res_min, double sd_min, int32_t *offsets,
-                            double *stats)
+                            const int32_t *mv, double strength, double res_min, double sd_min, int grain,
+                            int32_t *offsets, double *stats)
 {
 #pragma clang fp contract(off)
     if (nframes < 0 || mb_w < 0 || mb_h < 0) return fail("mbtree_qp_offsets: negative size");
@@ -4452,6 +4526,8 @@ int rh264_mbtree_qp_offsets(int nframes, int mb_w, int mb_h, const int32_t *intr
         return 0;
     };
     if (strength <= 0.0) return done();
+    // the grain latch (mbtree.rs:401-420): a grainy GOP of two or more frames gets no offsets
+    if (grain && n >= 2) return done();
     // backward propagation (mbtree.rs:440-455)
     std::vector<double> prop(n * nm, 0.0);
     for (size_t f = n - 1; f >= 1; f--) {

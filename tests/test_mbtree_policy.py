@@ -88,3 +88,92 @@ def test_library_policy_rejects_invalid_costs():
     inter[1, 0, 0] = intra[1, 0, 0] + 1
     with pytest.raises(RuntimeError):
         accel.mbtree_qp_offsets(lib, intra, inter, mv)
+
+
+# ------------------------------------------------------------ the grain latch
+
+def _grain_textbook(cur, ref):
+    """grain_signature (signals.rs:528-533) by numpy: the median of the per-MB variances
+    of the even-row/column samples; the lower quartile of the zero-motion SAD / 256 of
+    the interior walk's ~48 blocks; the mean over ~24 of those blocks of (SAD0 - best of
+    the +-8 step-4 grid inside the frame) / (SAD0 + 1), summed in walk order."""
+    ch, cw = cur.shape
+    mbw, mbh = cw // 16, ch // 16
+    c = cur.astype(np.int64)
+    r = ref.astype(np.int64)
+    sub = c[0::2, 0::2].reshape(mbh, 8, mbw, 8).transpose(0, 2, 1, 3).reshape(mbh * mbw, 64)
+    s, q = sub.sum(1), (sub * sub).sum(1)
+    var = np.sort((q - s * s // 64) // 64)
+    med = int(var[len(var) // 2])
+    floor_v = mg = 0.0
+    if mbw >= 6 and mbh >= 6:
+        inner = (mbw - 4) * (mbh - 4)
+
+        def walk(target):
+            return [((2 + i % (mbw - 4)) * 16, (2 + i // (mbw - 4)) * 16) for i in range(0, inner, max(inner // target, 1))]
+
+        def sad(bx, by, rx, ry):
+            return int(np.abs(c[by:by + 16, bx:bx + 16] - r[ry:ry + 16, rx:rx + 16]).sum())
+
+        fl = sorted(sad(bx, by, bx, by) / 256.0 for bx, by in walk(48))
+        floor_v = fl[len(fl) // 4]
+        acc, n = 0.0, 0
+        for bx, by in walk(24):
+            s0 = sad(bx, by, bx, by)
+            best = min([s0] + [sad(bx, by, bx + dx, by + dy) for dy in range(-8, 9, 4) for dx in range(-8, 9, 4)
+                               if 0 <= bx + dx and bx + dx + 16 <= cw and 0 <= by + dy and by + dy + 16 <= ch])
+            acc += (s0 - best) / (s0 + 1)
+            n += 1
+        mg = acc / n
+    return med < 200 and floor_v > 5.0 and mg < 0.1, {"median_var": float(med), "floor": floor_v, "mgain": mg}
+
+
+def _grain_pairs():
+    rng = np.random.default_rng(17)
+    yy, xx = np.mgrid[0:144, 0:176]
+    flat = np.full((144, 176), 120.0)
+    tex = 128 + 60 * np.sin(xx / 3.0) * np.cos(yy / 4.0)
+    noisy = lambda b, s: np.clip(b + rng.normal(0, s, b.shape), 0, 255).astype(np.uint8)
+    return {
+        "grain": (noisy(flat, 7), noisy(flat, 7)),                      # noise only: grain
+        "static": (noisy(flat, 7),) * 2,                                # identical frames: no residual
+        "motion": (np.roll(noisy(tex, 2), 4, 1), noisy(tex, 2)),        # texture that moves: not grain
+        "texture": (noisy(tex + 40 * rng.random(tex.shape), 7), noisy(tex, 7)),  # var >= 200
+        "small": (noisy(flat[:80, :80], 7), noisy(flat[:80, :80], 7)),  # below 6x6 MBs: fails open
+        "random": tuple(rng.integers(0, 256, (96, 128), dtype=np.uint8) for _ in range(2)),
+    }
+
+
+def test_grain_signature_textbook_and_closed_forms():
+    want_flags = {"grain": True, "static": False, "motion": False, "texture": False, "small": False,
+                  "random": False}
+    for name, (cur, ref) in _grain_pairs().items():
+        g, st = O.grain_signature(cur, ref)
+        tg, tst = _grain_textbook(cur, ref)
+        assert (g, st) == (tg, tst), name
+        assert g == want_flags[name], (name, st)
+    g, st = O.grain_signature(*_grain_pairs()["static"])
+    assert st["floor"] == 0.0 and st["mgain"] == 0.0
+
+
+def test_grain_latch_zeroes_the_gop():
+    rng = np.random.default_rng(8)
+    intra, inter, mv = _two_frame_gop(rng)
+    off, st = O.mbtree_qp_offsets(intra, inter, mv, res_min=0.0, grain=True)
+    assert not off.any() and st == {"residual_frac": 0.0, "spread": 0.0, "eff_strength": 0.0}
+    assert O.mbtree_qp_offsets(intra, inter, mv, res_min=0.0)[0].any()
+    # a single-frame GOP has no pair: the latch does not apply
+    one = O.mbtree_qp_offsets(intra[:1], inter[:1], mv[:1], res_min=0.0, grain=True)
+    assert one[1]["eff_strength"] == 0.9
+
+
+def test_library_grain_equals_oracle():
+    lib = _lib.load()
+    for name, (cur, ref) in _grain_pairs().items():
+        assert accel.grain_signature(lib, cur, ref) == O.grain_signature(cur, ref), name
+    rng = np.random.default_rng(9)
+    intra, inter, mv = _two_frame_gop(rng)
+    for grain in (False, True):
+        a = accel.mbtree_qp_offsets(lib, intra, inter, mv, res_min=0.0, grain=grain)
+        b = O.mbtree_qp_offsets(intra, inter, mv, res_min=0.0, grain=grain)
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1]
