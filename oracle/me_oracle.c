@@ -994,6 +994,121 @@ void orc_motion_search_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *d
         }
 }
 
+/* The P macroblock decision with partitions (mb16.rs:8849-9010, the SATD currency:
+ * shape RD off, the all-8x8 P_8x8 arm, every macroblock inter), in raster order.
+ * Per macroblock: the neighbours A/B/C(D) read on the 4x4 motion grid of the decided
+ * macroblocks (mv_neighbors_block, mb16.rs:1871-1888, at the macroblock's top-left
+ * 4x4 with width 4: A = the left macroblock's top-right quadrant, B and C = the
+ * bottom-left quadrant of the above and above-right ones, D = the above-left one's
+ * bottom-right quadrant); best_part (mb16.rs:4921-4959) of the 16x16 with no extra
+ * seeds; when its cost exceeds split_gate, best_part of the two 16x8 halves, the two
+ * 8x16 halves and (p8x8) the four 8x8 quadrants, each with the macroblock's
+ * neighbours and the 16x16 vector as the extra seed (mb16.rs:8893-8948); the pick
+ * is the first strict minimum of c16, ct + cb, cl + cr and (int64)(lme * 4) + the
+ * quadrants' sum (mb16.rs:8897-8948).  Costs include (int64)(lme * ref_bits(r)).
+ * rec per macroblock, 14 int32: mode (0 16x16, 1 16x8, 2 8x16, 3 8x8), cost, then
+ * per quadrant (TL, TR, BL, BR) ref, mvx, mvy. */
+static void pd_quad_nb(const int32_t *rec, int nbx, int x, int y, int q, int32_t out[4])
+{
+    const int32_t *r = rec + 14 * (size_t)(y * nbx + x) + 2 + 3 * q;
+    out[0] = 1;
+    out[1] = r[1];
+    out[2] = r[2];
+    out[3] = r[0];
+}
+
+static int64_t pd_best_part(const orc_search_cfg *cfg0, const orc_diamond_cfg *dc, const uint8_t *cur,
+                            const uint8_t *const *planes, int nrefs, double lme, const int32_t nb[12], int lx, int ly,
+                            int bw, int bh, const int32_t *extra, int nextra, int32_t *ref_out, int32_t *mv_out)
+{
+    orc_search_cfg cfg = *cfg0;
+    cfg.bw = bw;
+    cfg.bh = bh;
+    int64_t bc = INT64_MAX;
+    for (int r = 0; r < nrefs; r++) {
+        int32_t seeds[8], m[2];
+        orc_predict_mv(nb, r, seeds);
+        for (int e = 0; e < nextra; e++) {
+            seeds[2 + 2 * e] = extra[2 * e];
+            seeds[3 + 2 * e] = extra[2 * e + 1];
+        }
+        int64_t c = orc_motion_search_ex(&cfg, dc, cur, planes[r], lx, ly, seeds, 1 + nextra, m, NULL, NULL, 0, NULL);
+        c += (int64_t)(lme * (double)ref_bits_orc(r, nrefs));
+        if (c < bc) { bc = c; *ref_out = r; mv_out[0] = m[0]; mv_out[1] = m[1]; }
+    }
+    return bc;
+}
+
+void orc_partition_decision_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *dc, const uint8_t *cur,
+                                  const uint8_t *const *planes, int nrefs, double lme, int64_t split_gate, int p8x8,
+                                  int32_t *rec)
+{
+    const int nbx = cfg->cw / 16, nby = cfg->ch / 16;
+    for (int y = 0; y < nby; y++)
+        for (int x = 0; x < nbx; x++) {
+            int32_t nb[12] = {0, 0, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1};
+            if (x > 0) pd_quad_nb(rec, nbx, x - 1, y, 1, nb);
+            if (y > 0) pd_quad_nb(rec, nbx, x, y - 1, 2, nb + 4);
+            if (y > 0 && x + 1 < nbx) pd_quad_nb(rec, nbx, x + 1, y - 1, 2, nb + 8);
+            else if (x > 0 && y > 0) pd_quad_nb(rec, nbx, x - 1, y - 1, 3, nb + 8);
+            const int lx = 16 * x, ly = 16 * y;
+            int32_t r16 = 0, mv16[2] = {0, 0};
+            const int64_t c16 = pd_best_part(cfg, dc, cur, planes, nrefs, lme, nb, lx, ly, 16, 16, NULL, 0, &r16, mv16);
+            int mode = 0;
+            int64_t best = c16;
+            int32_t qr[4] = {r16, r16, r16, r16}, qm[4][2];
+            for (int q = 0; q < 4; q++) { qm[q][0] = mv16[0]; qm[q][1] = mv16[1]; }
+            if (c16 > split_gate) {
+                int32_t pr[8], pm[8][2];
+                int64_t pc[8];
+                /* 16x8 top, bottom; 8x16 left, right; 8x8 TL, TR, BL, BR */
+                static const int geo[8][4] = {{0, 0, 16, 8}, {0, 8, 16, 8}, {0, 0, 8, 16}, {8, 0, 8, 16},
+                                              {0, 0, 8, 8},  {8, 0, 8, 8},  {0, 8, 8, 8},  {8, 8, 8, 8}};
+                const int nparts = p8x8 ? 8 : 4;
+                for (int k = 0; k < nparts; k++)
+                    pc[k] = pd_best_part(cfg, dc, cur, planes, nrefs, lme, nb, lx + geo[k][0], ly + geo[k][1],
+                                         geo[k][2], geo[k][3], mv16, 1, &pr[k], pm[k]);
+                if (pc[0] + pc[1] < best) {
+                    best = pc[0] + pc[1];
+                    mode = 1;
+                    const int map[4] = {0, 0, 1, 1};
+                    for (int q = 0; q < 4; q++) { qr[q] = pr[map[q]]; qm[q][0] = pm[map[q]][0]; qm[q][1] = pm[map[q]][1]; }
+                }
+                if (pc[2] + pc[3] < best) {
+                    best = pc[2] + pc[3];
+                    mode = 2;
+                    const int map[4] = {2, 3, 2, 3};
+                    for (int q = 0; q < 4; q++) { qr[q] = pr[map[q]]; qm[q][0] = pm[map[q]][0]; qm[q][1] = pm[map[q]][1]; }
+                }
+                if (p8x8) {
+                    const int64_t c8 = (int64_t)(lme * 4.0) + pc[4] + pc[5] + pc[6] + pc[7];
+                    if (c8 < best) {
+                        best = c8;
+                        mode = 3;
+                        for (int q = 0; q < 4; q++) { qr[q] = pr[4 + q]; qm[q][0] = pm[4 + q][0]; qm[q][1] = pm[4 + q][1]; }
+                    }
+                }
+            }
+            int32_t *o = rec + 14 * (size_t)(y * nbx + x);
+            o[0] = mode;
+            o[1] = (int32_t)best;
+            for (int q = 0; q < 4; q++) {
+                o[2 + 3 * q] = qr[q];
+                o[3 + 3 * q] = qm[q][0];
+                o[4 + 3 * q] = qm[q][1];
+            }
+        }
+}
+
+/* The split gate of mb16.rs:8866-8868 for a QP: ((30 * (qstep16 + 160)) >> 3) * 2,
+ * qstep16 = {10, 11, 13, 14, 16, 18}[qp % 6] << (qp / 6). */
+int64_t orc_split_gate(int qp)
+{
+    static const int64_t qs[6] = {10, 11, 13, 14, 16, 18};
+    const int64_t q = qs[qp % 6] << (qp / 6);
+    return ((30 * (q + 160)) >> 3) * 2;
+}
+
 /* skip_mv (mb16.rs:1841-1853) from the neighbours in predict_mv's layout. */
 void orc_skip_mv(const int32_t nb[12], int32_t out[2])
 {

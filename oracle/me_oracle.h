@@ -144,6 +144,17 @@ void orc_motion_search_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *d
                              const uint8_t *const *planes, int nrefs, double lambda_ref, int32_t *ref,
                              int32_t *mv, int64_t *cost);
 
+/* The P macroblock decision with partitions in raster order (mb16.rs:8849-9010 with
+ * the shape picked in the SATD currency, the all-8x8 P_8x8 arm when p8x8, every
+ * macroblock inter; steps in me_oracle.c): planes[r] = reference r's plane set; lme
+ * = the search lambda that also prices ref_bits and the P_8x8 syntax; split_gate
+ * from orc_split_gate.  rec: 14 int32 per macroblock, {mode, cost, (ref, mvx, mvy)
+ * x 4 quadrants TL, TR, BL, BR}. */
+void orc_partition_decision_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *dc, const uint8_t *cur,
+                                  const uint8_t *const *planes, int nrefs, double lme, int64_t split_gate, int p8x8,
+                                  int32_t *rec);
+int64_t orc_split_gate(int qp);
+
 /* skip_mv (mb16.rs:1841-1853): (0,0) when A or B is unavailable or a zero vector of
  * reference 0, else predict_mv(A, B, C, 0); nb as orc_predict_mv's. */
 void orc_skip_mv(const int32_t nb[12], int32_t out[2]);

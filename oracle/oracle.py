@@ -81,6 +81,7 @@ class Oracle:
         L.orc_inter_cost.restype = C.c_int32
         L.orc_inter_cost_ex.restype = C.c_int32
         L.orc_rc_complexity.restype = C.c_double
+        L.orc_split_gate.restype = C.c_int64
 
     # ---- block metrics
     def sad(self, a, sa, b, sb, w, h):
@@ -232,6 +233,22 @@ class Oracle:
         self.lib.orc_motion_search_frame(C.byref(cfg), C.byref(dc), _p(_u8(cur)), arr, len(pls),
                                          C.c_double(lambda_ref), _p(ref, i32p), _p(mv, i32p), _p(c, i64p))
         return ref, mv, c
+
+    def partition_decision_frame(self, cfg, cur, planes, lme, split_gate, p8x8=True, **kw):
+        """The P macroblock decision with partitions (SATD currency) in raster order;
+        planes: a list of plane sets, most recent first; kw: diamond knobs.  Returns
+        rec (nblk, 14) int32: mode, cost, then (ref, mvx, mvy) per quadrant."""
+        dc = self.diamond(cfg, **kw)
+        pls = [_u8(x) for x in planes]
+        arr = (C.c_void_p * len(pls))(*[x.ctypes.data for x in pls])
+        nb = (cfg.cw // 16) * (cfg.ch // 16)
+        rec = np.zeros((nb, 14), np.int32)
+        self.lib.orc_partition_decision_frame(C.byref(cfg), C.byref(dc), _p(_u8(cur)), arr, len(pls),
+                                              C.c_double(lme), C.c_int64(split_gate), int(p8x8), _p(rec, i32p))
+        return rec
+
+    def split_gate(self, qp):
+        return int(self.lib.orc_split_gate(int(qp)))
 
     def skip_mv(self, a, b, c):
         """skip_mv (mb16.rs:1841-1853); neighbours as predict_mv's."""

@@ -182,3 +182,136 @@ def test_frame_costs_half_walk():
     assert np.array_equal(b.reshape(mb_h, mb_w), wb)
     assert np.array_equal(m.reshape(mb_h, mb_w, 2), wm)
     assert (wm != 0).any()
+
+
+# ------------------------------------------------ the partition decision's frame walk
+
+def _predict_mv(a, b, c, cur_ref):
+    """predict_mv, inter.rs:36-73 (spec 8.4.1.3.1); neighbours (available, mv, ref)."""
+    res = lambda n: (n[1], n[2]) if n[0] and n[2] >= 0 else ((0, 0), -1)
+    (ma, ra), (mb, rb), (mc, rc) = res(a), res(b), res(c)
+    if not b[0] and not c[0] and a[0]:
+        mb, rb, mc, rc = ma, ra, ma, ra
+    hits = [m for m, r in ((ma, ra), (mb, rb), (mc, rc)) if r == cur_ref]
+    if len(hits) == 1:
+        return hits[0]
+    return tuple(sorted(v)[1] for v in zip(ma, mb, mc))
+
+
+def partition_decision_walk(cur, planes, cfg, lme, gate, p8x8, kw):
+    """mb16.rs:8849-9010 in the SATD currency over a 4x4 motion grid: each decided
+    macroblock writes its partitions' (mv, ref) into the 4x4 blocks they cover; the
+    next macroblocks read A = (4x-1, 4y), B = (4x, 4y-1), C = (4x+4, 4y-1) or, when
+    C is unavailable, D = (4x-1, 4y-1) (mv_neighbors_block, mb16.rs:1871-1888)."""
+    ch, cw = cur.shape
+    nbx, nby = cw // 16, ch // 16
+    W4, H4 = 4 * nbx, 4 * nby
+    grid_mv = np.zeros((H4, W4, 2), np.int64)
+    grid_ref = np.full((H4, W4), -1, np.int64)
+    coded = np.zeros((H4, W4), bool)
+
+    def get(bx, by):
+        if bx < 0 or by < 0 or bx >= W4 or by >= H4 or not coded[by, bx]:
+            return (False, (0, 0), -1)
+        return (True, tuple(int(v) for v in grid_mv[by, bx]), int(grid_ref[by, bx]))
+
+    refbits = lambda r, n: 0 if n <= 1 else 1 if n == 2 else 2 * (r + 1).bit_length() - 1
+
+    def best_part(nb, lx, ly, bw, bh, extra):
+        from oracle.oracle import Oracle  # noqa: F401  (O is module level)
+        c2 = O.cfg(cw, ch, 32, bw, bh, cfg.range, cfg.subpel_pat, cfg._lam, lam_fp=cfg._lam_fp)
+        best = None
+        for r, pl in enumerate(planes):
+            seeds = [_predict_mv(*nb, r)] + extra
+            mv, c = O.motion_search_ref(c2, cur, pl, lx, ly, seeds, **kw)
+            c += int(lme * refbits(r, len(planes)))
+            if best is None or c < best[0]:
+                best = (c, r, mv)
+        return best
+
+    rec = np.zeros((nbx * nby, 14), np.int64)
+    for y in range(nby):
+        for x in range(nbx):
+            a, b = get(4 * x - 1, 4 * y), get(4 * x, 4 * y - 1)
+            c = get(4 * x + 4, 4 * y - 1)
+            if not c[0]:
+                c = get(4 * x - 1, 4 * y - 1)
+            nb = (a, b, c)
+            c16, r16, mv16 = best_part(nb, 16 * x, 16 * y, 16, 16, [])
+            mode, best = 0, c16
+            parts = [(0, 0, 16, 16, r16, mv16)]
+            if c16 > gate:
+                t = best_part(nb, 16 * x, 16 * y, 16, 8, [mv16])
+                bo = best_part(nb, 16 * x, 16 * y + 8, 16, 8, [mv16])
+                le = best_part(nb, 16 * x, 16 * y, 8, 16, [mv16])
+                ri = best_part(nb, 16 * x + 8, 16 * y, 8, 16, [mv16])
+                if t[0] + bo[0] < best:
+                    mode, best = 1, t[0] + bo[0]
+                    parts = [(0, 0, 16, 8, t[1], t[2]), (0, 8, 16, 8, bo[1], bo[2])]
+                if le[0] + ri[0] < best:
+                    mode, best = 2, le[0] + ri[0]
+                    parts = [(0, 0, 8, 16, le[1], le[2]), (8, 0, 8, 16, ri[1], ri[2])]
+                if p8x8:
+                    qs = [best_part(nb, 16 * x + qx, 16 * y + qy, 8, 8, [mv16]) for qx, qy in
+                          ((0, 0), (8, 0), (0, 8), (8, 8))]
+                    c8 = int(lme * 4.0) + sum(q[0] for q in qs)
+                    if c8 < best:
+                        mode, best = 3, c8
+                        parts = [(qx, qy, 8, 8, q[1], q[2]) for (qx, qy), q in
+                                 zip(((0, 0), (8, 0), (0, 8), (8, 8)), qs)]
+            for px, py, pw, ph, r, mv in parts:
+                by0, bx0 = 4 * y + py // 4, 4 * x + px // 4
+                grid_mv[by0:by0 + ph // 4, bx0:bx0 + pw // 4] = mv
+                grid_ref[by0:by0 + ph // 4, bx0:bx0 + pw // 4] = r
+            coded[4 * y:4 * y + 4, 4 * x:4 * x + 4] = True
+            o = rec[y * nbx + x]
+            o[0], o[1] = mode, best
+            for q, (qx, qy) in enumerate(((0, 0), (3, 0), (0, 3), (3, 3))):
+                o[2 + 3 * q] = grid_ref[4 * y + qy, 4 * x + qx]
+                o[3 + 3 * q: 5 + 3 * q] = grid_mv[4 * y + qy, 4 * x + qx]
+    return rec
+
+
+def partition_clip(cw, ch, nrefs, seed):
+    """References of independent smooth textures and a current frame whose macroblock
+    quadrants copy them at chosen displacements: macroblock (x, y) is uniform, split
+    top/bottom, left/right or into quadrants by (x + 2y) % 4, each piece taking its
+    own reference and motion, so every partition mode and reference has a reason to
+    win."""
+    rng = np.random.default_rng(seed)
+    refs = [_clip(cw, ch, seed * 7 + k, (0, 0))[1] for k in range(nrefs)]
+    cur = np.zeros((ch, cw), np.float64)
+    yy, xx = np.mgrid[0:8, 0:8]
+    for y in range(ch // 16):
+        for x in range(cw // 16):
+            kind = (x + 2 * y) % 4
+            pieces = [(int(rng.integers(0, nrefs)), int(rng.integers(-6, 7)), int(rng.integers(-6, 7)))
+                      for _ in range(4)]
+            group = {0: (0, 0, 0, 0), 1: (0, 0, 1, 1), 2: (0, 1, 0, 1), 3: (0, 1, 2, 3)}[kind]
+            for q, (qx, qy) in enumerate(((0, 0), (8, 0), (0, 8), (8, 8))):
+                k, dx, dy = pieces[group[q]]
+                ry = np.clip(16 * y + qy + yy + dy, 0, ch - 1)
+                rx = np.clip(16 * x + qx + xx + dx, 0, cw - 1)
+                cur[16 * y + qy:16 * y + qy + 8, 16 * x + qx:16 * x + qx + 8] = refs[k][ry, rx]
+    cur = np.clip(cur + rng.normal(0, 1.5, cur.shape), 0, 255).astype(np.uint8)
+    return cur, refs
+
+
+@pytest.mark.parametrize("nrefs,qp,p8x8,kw,pat", [(1, 26, True, dict(fc=True), 0),
+                                                   (2, 20, True, dict(fc=True, mv_tab=SMOOTH), 2),
+                                                   (1, 14, False, dict(sadfp=True), 0),
+                                                   (3, 26, True, dict(fc=True, sadfp=True), 4)])
+def test_partition_decision_walk(nrefs, qp, p8x8, kw, pat):
+    """The oracle's raster-order partition decision == the walk over the 4x4 motion
+    grid (every mode occurs)."""
+    cw, ch = 80, 64
+    cur, frames = partition_clip(cw, ch, nrefs, qp + nrefs)
+    planes = [O.planes(f, cw, ch, 32) for f in frames]
+    lam = float(np.sqrt(0.85 * 2.0 ** ((qp - 12) / 3.0)))
+    cfg = O.cfg(cw, ch, 32, 16, 16, 16, pat, lam)
+    gate = O.split_gate(qp)
+    got = O.partition_decision_frame(cfg, cur, planes, lam, gate, p8x8, **kw)
+    want = partition_decision_walk(cur, planes, cfg, lam, gate, p8x8, kw)
+    assert np.array_equal(got, want), np.argwhere((got != want).any(1))[:5]
+    assert set(got[:, 0].tolist()) == ({0, 1, 2, 3} if p8x8 else {0, 1, 2}), np.bincount(got[:, 0])
+    assert set(got[:, 2::3].ravel().tolist()) == set(range(nrefs))
