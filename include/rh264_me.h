@@ -354,6 +354,31 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
                                   const uint8_t *planes, size_t planes_stride, int32_t *ref, int32_t *mv,
                                   int32_t *cost);
 
+/* The P macroblock decision with partitions, in macroblock order (mb16.rs:8849-9010
+ * with the shape picked in the SATD currency: shape RD off, the all-8x8 P_8x8 arm,
+ * every macroblock inter).  Per macroblock: the 16x16 best_part (mb16.rs:4921-4959)
+ * as rh264_motion_search_wavefront, predictors predict_mv (inter.rs:36-73) of the
+ * neighbours read on the 4x4 motion grid (mv_neighbors_block, mb16.rs:1871-1888: the
+ * left macroblock's top-right quadrant, the bottom-left quadrant of the above and
+ * above-right ones, else the above-left's bottom-right); when its cost exceeds
+ * split_gate (rh264_split_gate(qp), mb16.rs:8866-8868), best_part of the two 16x8
+ * halves, the two 8x16 halves and with p8x8 the four 8x8 quadrants, each with the
+ * macroblock's neighbours and the 16x16 vector as the extra seed (mb16.rs:
+ * 8893-8948); the first strict minimum of c16, ct + cb, cl + cr and
+ * (int64)(lambda_ref * 4) + the quadrants' sum.  lambda_ref = the search lambda
+ * (it prices ref_bits and the P_8x8 syntax).  The search knobs are
+ * rh264_motion_search's (16x16 in p; the rescue and the 8x8 transform decline).
+ * planes as rh264_motion_search_wavefront.  Decided by fixed-point passes (the
+ * stream is synchronised every 8 passes).  rec: 14 int32 per macroblock, {mode (0
+ * 16x16, 1 16x8, 2 8x16, 3 8x8), cost, then ref, mvx, mvy of each quadrant TL, TR,
+ * BL, BR}. */
+int rh264_partition_decision(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc, int nframes,
+                             int nrefs, double lambda_ref, int32_t split_gate, int p8x8, const uint8_t *cur,
+                             size_t cur_stride, const uint8_t *planes, size_t planes_stride, int32_t *rec);
+/* ((30 * (qstep16 + 160)) >> 3) * 2, qstep16 = {10, 11, 13, 14, 16, 18}[qp % 6] <<
+ * (qp / 6) (mb16.rs:8866-8868); -1 outside 0..51 */
+int32_t rh264_split_gate(int qp);
+
 /* The luma predictions of decided P frames (the step after the decision): per
  * macroblock the P_Skip vector skip_mv (skip_mv, mb16.rs:1841-1853: (0,0) when the
  * left or above neighbour is unavailable or is a zero vector of reference 0, else

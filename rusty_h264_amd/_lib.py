@@ -107,6 +107,10 @@ PROTOTYPES = {
     "rh264_motion_search_wavefront": (C.c_int, [c_vp, C.POINTER(SearchParams), C.POINTER(Diamond), C.c_int,
                                                 C.c_int, C.c_double, c_vp, C.c_size_t, c_vp, C.c_size_t, c_vp, c_vp,
                                                 c_vp]),
+    "rh264_partition_decision": (C.c_int, [c_vp, C.POINTER(SearchParams), C.POINTER(Diamond), C.c_int, C.c_int,
+                                           C.c_double, C.c_int32, C.c_int, c_vp, C.c_size_t, c_vp, C.c_size_t,
+                                           c_vp]),
+    "rh264_split_gate": (C.c_int32, [C.c_int]),
     "rh264_decision_predict": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, c_vp, C.c_size_t, c_vp,
                                          c_vp, c_vp, c_vp, c_vp]),
     "rh264_mbtree_qp_offsets": (C.c_int, [C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp, C.c_double, C.c_double,

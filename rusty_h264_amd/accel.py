@@ -473,6 +473,34 @@ def motion_search_wavefront(dev, params, nframes, cur, planes, nrefs=1, lambda_r
     return out
 
 
+def split_gate(qp):
+    """The partition split gate of mb16.rs:8866-8868 (rh264_split_gate)."""
+    return int(_lib.load().rh264_split_gate(int(qp)))
+
+
+def partition_decision(dev, params, nframes, cur, planes, nrefs=1, lambda_ref=0.0, gate=0, p8x8=True,
+                       cur_stride=None, planes_stride=None, out=None, **kw):
+    """The P macroblock decision with partitions in macroblock order
+    (rh264_partition_decision; mb16.rs:8849-9010 in the SATD currency); planes as
+    motion_search_wavefront; kw: the other DiamondCfg knobs.  Returns the device
+    buffer of 14 int32 per macroblock {mode, cost, (ref, mvx, mvy) x 4 quadrants}."""
+    n = (params.cw // 16) * (params.ch // 16) * nframes
+    if out is None:
+        out = dev.alloc(4 * 14 * n)
+    _need(out, 4 * 14 * n, "rec")
+    d = DiamondCfg(dev, **kw)
+    try:
+        _check(dev.lib.rh264_partition_decision(dev.ctx, C.byref(params), C.byref(d.dc), nframes, nrefs,
+                                                float(lambda_ref), int(gate), int(p8x8), int(cur),
+                                                cur_stride or params.cw * params.ch, int(planes),
+                                                planes_stride or planes_bytes(params.cw, params.ch, params.pad),
+                                                out.ptr), "rh264_partition_decision")
+        dev.sync()  # the rate table, if any, is freed below
+    finally:
+        d.close()
+    return out
+
+
 def decision_predict(dev, cw, ch, pad, nframes, nrefs, planes, ref, mv, planes_stride=None):
     """rh264_decision_predict over decided frames (ref, mv device buffers as
     motion_search_wavefront returns them): (skip_mv [n][2], pred [f][ch][cw],

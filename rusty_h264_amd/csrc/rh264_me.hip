@@ -1707,8 +1707,8 @@ __device__ __forceinline__ int ms_div_euclid4(int v) { return v >= 0 ? v / 4 : -
 template <bool T8>
 __device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh264_diamond &dc,
                                           const int32_t *s_lr, const int32_t *s_lrfp, const uint8_t *frame_px,
-                                          const uint8_t *pl, int lx, int ly, const int32_t *pr, int npreds, int lane,
-                                          int &obx, int &oby, int &obc, int &oev)
+                                          const uint8_t *pl, int lx, int ly, int bw, int bh, const int32_t *pr,
+                                          int npreds, int lane, int &obx, int &oby, int &obc, int &oev)
 {
     MxBlock B;
     B.pw = p.cw + 2 * p.pad;
@@ -1721,7 +1721,6 @@ __device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh
     B.xoff = 0;
     B.mpx = npreds > 0 ? __builtin_amdgcn_readfirstlane(pr[0]) : 0; // the rate predictor
     B.mpy = npreds > 0 ? __builtin_amdgcn_readfirstlane(pr[1]) : 0;
-    const int bw = p.bw, bh = p.bh;
     const int r = lane & 15, qd = lane >> 4;
     const bool act = r < bh && 4 * qd < bw; // the lane's 4 pixels lie inside the block
     const uint32_t src = act ? *(const uint32_t *)(frame_px + (size_t)(ly + r) * p.cw + lx + 4 * qd) : 0u;
@@ -1938,8 +1937,8 @@ __global__ __launch_bounds__(256) void k_motion_search(rh264_search_params p, rh
     const size_t boff = (size_t)frame * nblk + b;
     int bx, by, bc, ev;
     ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride, planes + (size_t)frame * planes_stride,
-                  (b % nbx) * p.bw, (b / nbx) * p.bh, preds + boff * (size_t)(npreds > 0 ? npreds : 1) * 2, npreds,
-                  lane, bx, by, bc, ev);
+                  (b % nbx) * p.bw, (b / nbx) * p.bh, p.bw, p.bh, preds + boff * (size_t)(npreds > 0 ? npreds : 1) * 2,
+                  npreds, lane, bx, by, bc, ev);
     if (lane == 0) {
         out_mv[2 * boff] = bx;
         out_mv[2 * boff + 1] = by;
@@ -2025,8 +2024,8 @@ __global__ __launch_bounds__(256) void k_ms_wavefront(rh264_search_params p, rh2
         pr[1] = __builtin_amdgcn_readfirstlane(py);
         int bx, by, bc, ev;
         ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
-                      planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, x << 4, y << 4, pr, 1, lane, bx, by,
-                      bc, ev);
+                      planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, x << 4, y << 4, 16, 16, pr, 1, lane,
+                      bx, by, bc, ev);
         bc += refc.c[rf];
         if constexpr (RW == 1) {
             if (lane == 0) {
@@ -2130,7 +2129,7 @@ __global__ __launch_bounds__(256) void k_ms_listed(rh264_search_params p, rh264_
             int bx, by, bc, ev;
             ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
                           planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, (b % nbx) << 4, (b / nbx) << 4,
-                          used + (size_t)i * 8 + 2 * rf, 1, lane, bx, by, bc, ev);
+                          16, 16, used + (size_t)i * 8 + 2 * rf, 1, lane, bx, by, bc, ev);
             bc += refc.c[rf];
             if constexpr (RW == 1) {
                 if (lane == 0) {
@@ -2158,6 +2157,200 @@ __global__ __launch_bounds__(256) void k_ms_listed(rh264_search_params p, rh264_
             }
             __syncthreads();
         }
+    }
+}
+
+// ---- the P macroblock decision with partitions (mb16.rs:8849-9010, SATD currency) --
+// rec per macroblock: {mode, cost, (ref, mvx, mvy) x quadrants TL, TR, BL, BR}.
+constexpr int PD_REC = 14;
+
+// predict_mv (inter.rs:36-73) against cur_ref from the neighbours read on the 4x4
+// motion grid at the macroblock's top-left block (mv_neighbors_block, mb16.rs:
+// 1871-1888): A = the left macroblock's top-right quadrant, B and C = the bottom-left
+// quadrant of the above and above-right ones, D = the above-left bottom-right.
+__device__ __forceinline__ void pd_predict_mv(const int32_t *rec, int nbx, int x, int y, int cur_ref, int &px,
+                                              int &py)
+{
+    const bool av_a = x > 0, av_b = y > 0, c_is_c = y > 0 && x + 1 < nbx;
+    const bool av_c = c_is_c || (x > 0 && y > 0);
+    const int32_t *A = rec + (size_t)PD_REC * (y * nbx + x - 1) + 2 + 3 * 1;
+    const int32_t *B = rec + (size_t)PD_REC * ((y - 1) * nbx + x) + 2 + 3 * 2;
+    const int32_t *Cq = c_is_c ? rec + (size_t)PD_REC * ((y - 1) * nbx + x + 1) + 2 + 3 * 2
+                               : rec + (size_t)PD_REC * ((y - 1) * nbx + x - 1) + 2 + 3 * 3;
+    int ax = 0, ay = 0, ra = -1, bx = 0, by = 0, rb = -1, cx = 0, cy = 0, rc = -1;
+    if (av_a) { ra = A[0]; ax = A[1]; ay = A[2]; }
+    if (av_b) { rb = B[0]; bx = B[1]; by = B[2]; }
+    if (av_c) { rc = Cq[0]; cx = Cq[1]; cy = Cq[2]; }
+    if (!av_b && !av_c && av_a) {
+        bx = cx = ax;
+        by = cy = ay;
+        rb = rc = ra;
+    }
+    const int matches = (ra == cur_ref) + (rb == cur_ref) + (rc == cur_ref);
+    if (matches == 1) {
+        px = ra == cur_ref ? ax : rb == cur_ref ? bx : cx;
+        py = ra == cur_ref ? ay : rb == cur_ref ? by : cy;
+    } else {
+        px = ms_median3(ax, bx, cx);
+        py = ms_median3(ay, by, cy);
+    }
+}
+
+// Fixed-point check, as k_ms_check, with the predictors from the quadrant grid.
+__global__ __launch_bounds__(256) void k_pd_check(int nbx, int nblk, int ntot, int nrefs, const int32_t *__restrict__ rec,
+                                                  int32_t *__restrict__ used, int32_t *__restrict__ list,
+                                                  uint32_t *__restrict__ count)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    bool dirty = false;
+    if (i < ntot) {
+        const int frame = i / nblk, b = i - frame * nblk;
+        for (int r = 0; r < nrefs; r++) {
+            int px, py;
+            pd_predict_mv(rec + (size_t)frame * nblk * PD_REC, nbx, b % nbx, b / nbx, r, px, py);
+            int32_t *u = used + (size_t)i * 8 + 2 * r;
+            if (u[0] != px || u[1] != py) {
+                u[0] = px;
+                u[1] = py;
+                dirty = true;
+            }
+        }
+    }
+    const uint64_t m = __ballot(dirty);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (dirty) list[base + below] = i;
+}
+
+// The listed macroblocks' decisions, one workgroup (4 waves) per macroblock: the 16x16
+// best_part (wave r searches reference r), then, when its cost exceeds the gate, the
+// 16x8, 8x16 and (p8x8) 8x8 best_parts with the 16x16 vector as the extra seed, their
+// (partition, reference) searches spread over the 4 waves; the first strict minimum
+// of c16, ct + cb, cl + cr, lme4 + sum c8 is written to rec.
+struct PdArgs {
+    int32_t nrefs, gate, p8x8, lme4;
+    int32_t refc[4];
+};
+template <bool T8>
+__global__ __launch_bounds__(256) void k_pd_listed(rh264_search_params p, rh264_diamond dc,
+                                                   const uint8_t *__restrict__ cur, size_t cur_stride,
+                                                   const uint8_t *__restrict__ planes, size_t planes_stride, PdArgs a,
+                                                   const int32_t *__restrict__ used, const int32_t *__restrict__ list,
+                                                   const uint32_t *__restrict__ count, uint32_t *__restrict__ next_count,
+                                                   int32_t *__restrict__ rec)
+{
+    __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
+    __shared__ int32_t s16[4][3], sp[8][4][3];
+    if (threadIdx.x < RH264_LR_N) {
+        s_lr[threadIdx.x] = p.lr[threadIdx.x];
+        s_lrfp[threadIdx.x] = p.lr_fp[threadIdx.x];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+    __syncthreads();
+    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int nrefs = a.nrefs;
+    const int n = (int)__builtin_amdgcn_readfirstlane(*count);
+    // 16x8 top, bottom; 8x16 left, right; 8x8 TL, TR, BL, BR: (x, y, w, h)
+    constexpr int geo[8][4] = {{0, 0, 16, 8}, {0, 8, 16, 8}, {0, 0, 8, 16}, {8, 0, 8, 16},
+                               {0, 0, 8, 8},  {8, 0, 8, 8},  {0, 8, 8, 8},  {8, 8, 8, 8}};
+    for (int e = (int)blockIdx.x; e < n; e += (int)gridDim.x) { // workgroup-uniform
+        const int i = __builtin_amdgcn_readfirstlane(list[e]);
+        const int frame = i / nblk, b = i - frame * nblk;
+        const int lx = (b % nbx) << 4, ly = (b / nbx) << 4;
+        const uint8_t *fpx = cur + (size_t)frame * cur_stride;
+        const int32_t *u = used + (size_t)i * 8;
+        if (wave < nrefs) {
+            int bx, by, bc, ev;
+            ms_search<T8>(p, dc, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - wave) * planes_stride, lx, ly,
+                          16, 16, u + 2 * wave, 1, lane, bx, by, bc, ev);
+            if (lane == 0) {
+                s16[wave][0] = bx;
+                s16[wave][1] = by;
+                s16[wave][2] = bc + a.refc[wave];
+            }
+        }
+        __syncthreads();
+        int r16 = 0;
+        for (int q = 1; q < nrefs; q++)
+            if (s16[q][2] < s16[r16][2]) r16 = q;
+        const int mvx16 = s16[r16][0], mvy16 = s16[r16][1], c16 = s16[r16][2];
+        const bool split = c16 > a.gate;
+        if (split) {
+            const int nparts = a.p8x8 ? 8 : 4;
+            for (int j = wave; j < nparts * nrefs; j += 4) { // wave-uniform
+                const int part = j / nrefs, rf = j - part * nrefs;
+                // seeds: the predictor against rf (pricing the rate), then the 16x16 vector
+                const int32_t prl[4] = {u[2 * rf], u[2 * rf + 1], mvx16, mvy16};
+                int bx, by, bc, ev;
+                ms_search<T8>(p, dc, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride,
+                              lx + geo[part][0], ly + geo[part][1], geo[part][2], geo[part][3], prl, 2, lane, bx, by,
+                              bc, ev);
+                if (lane == 0) {
+                    sp[part][rf][0] = bx;
+                    sp[part][rf][1] = by;
+                    sp[part][rf][2] = bc + a.refc[rf];
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int mode = 0, best = c16;
+            int qr[4] = {r16, r16, r16, r16}, qx[4] = {mvx16, mvx16, mvx16, mvx16}, qy[4] = {mvy16, mvy16, mvy16, mvy16};
+            if (split) {
+                int pr_[8];
+                int pc[8];
+                const int nparts = a.p8x8 ? 8 : 4;
+                for (int k = 0; k < nparts; k++) {
+                    int br = 0;
+                    for (int q = 1; q < nrefs; q++)
+                        if (sp[k][q][2] < sp[k][br][2]) br = q;
+                    pr_[k] = br;
+                    pc[k] = sp[k][br][2];
+                }
+                auto take = [&](const int (&map)[4]) {
+                    for (int q = 0; q < 4; q++) {
+                        qr[q] = pr_[map[q]];
+                        qx[q] = sp[map[q]][pr_[map[q]]][0];
+                        qy[q] = sp[map[q]][pr_[map[q]]][1];
+                    }
+                };
+                if (pc[0] + pc[1] < best) {
+                    best = pc[0] + pc[1];
+                    mode = 1;
+                    const int m1[4] = {0, 0, 1, 1};
+                    take(m1);
+                }
+                if (pc[2] + pc[3] < best) {
+                    best = pc[2] + pc[3];
+                    mode = 2;
+                    const int m2[4] = {2, 3, 2, 3};
+                    take(m2);
+                }
+                if (a.p8x8) {
+                    const int c8 = a.lme4 + pc[4] + pc[5] + pc[6] + pc[7];
+                    if (c8 < best) {
+                        best = c8;
+                        mode = 3;
+                        const int m3[4] = {4, 5, 6, 7};
+                        take(m3);
+                    }
+                }
+            }
+            int32_t *o = rec + (size_t)PD_REC * i;
+            o[0] = mode;
+            o[1] = best;
+            for (int q = 0; q < 4; q++) {
+                o[2 + 3 * q] = qr[q];
+                o[3 + 3 * q] = qx[q];
+                o[4 + 3 * q] = qy[q];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -4389,6 +4582,73 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
         LAUNCH_CHECK("k_ms_wavefront");
     }
     return 0;
+}
+
+int rh264_partition_decision(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc, int nframes,
+                             int nrefs, double lambda_ref, int32_t split_gate, int p8x8, const uint8_t *cur,
+                             size_t cur_stride, const uint8_t *planes, size_t planes_stride, int32_t *rec)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    rc = check_diamond(dc, "partition_decision");
+    if (rc) return rc;
+    if (p->bw != 16 || p->bh != 16 || dc->rescue || p->hadamard == 8) return RH264_DECLINED;
+    if (nrefs < 1 || nrefs > 4) return RH264_DECLINED;
+    if (!(lambda_ref >= 0.0 && lambda_ref < 1e6)) return fail("partition_decision: lambda %g", lambda_ref);
+    if (p8x8 != 0 && p8x8 != 1) return fail("partition_decision: p8x8 %d", p8x8);
+    if (!rec) return fail("partition_decision: null output");
+    rc = check_strides(cur_stride, planes_stride);
+    if (rc) return rc;
+    if (nframes <= 0) return 0;
+    if (nframes > 65535) return fail("nframes %d > 65535", nframes);
+    PdArgs a{};
+    a.nrefs = nrefs;
+    a.gate = split_gate;
+    a.p8x8 = p8x8;
+    a.lme4 = (int32_t)(int64_t)(lambda_ref * 4.0);
+    for (int r = 0; r < nrefs; r++) a.refc[r] = (int32_t)(int64_t)(lambda_ref * (double)ref_bits(r, nrefs));
+    const int nbx = p->cw / 16, nby = p->ch / 16, ndiag = (nbx - 1) + 2 * (nby - 1) + 1;
+    const size_t ntot = (size_t)nframes * nbx * nby;
+    if (ntot > (size_t)INT32_MAX / 16) return fail("partition_decision: %zu macroblocks", ntot);
+    rc = ensure_buf(ctx, &ctx->ms_buf, &ctx->ms_cap, ntot * 9 * 4 + 256);
+    if (rc) return rc;
+    int32_t *used = (int32_t *)ctx->ms_buf, *list = used + ntot * 8;
+    uint32_t *counts = (uint32_t *)(list + ntot);
+    const unsigned fg = (unsigned)((ntot * 8 + 255) / 256), rg = (unsigned)((ntot * PD_REC + 255) / 256);
+    const unsigned ng = (unsigned)((ntot + 255) / 256);
+    // S starts at (16x16, ref 0, mv 0) everywhere; no predictor matches the sentinel,
+    // so the first pass decides every macroblock
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, ctx->stream, (unsigned *)used, ntot * 8, 0x80000000u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(rg), dim3(256), 0, ctx->stream, (unsigned *)rec, ntot * PD_REC, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, counts, (size_t)2, 0u);
+    LAUNCH_CHECK("k_fill_u32");
+    const unsigned sg = (unsigned)std::min<size_t>(ntot, 256 * 16); // one macroblock per workgroup
+    const int chunk = 8;
+    for (int k = 0; k <= ndiag;) {
+        for (int c = 0; c < chunk && k <= ndiag; c++, k++) {
+            uint32_t *cnt = counts + (k & 1), *nxt = counts + ((k + 1) & 1);
+            hipLaunchKernelGGL(k_pd_check, dim3(ng), dim3(256), 0, ctx->stream, nbx, nbx * nby, (int)ntot, nrefs,
+                               (const int32_t *)rec, used, list, cnt);
+            LAUNCH_CHECK("k_pd_check");
+            hipLaunchKernelGGL(k_pd_listed<false>, dim3(sg), dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes,
+                               planes_stride, a, (const int32_t *)used, (const int32_t *)list, (const uint32_t *)cnt,
+                               nxt, rec);
+            LAUNCH_CHECK("k_pd_listed");
+        }
+        uint32_t last = 1;
+        HIP_TRY(hipMemcpyAsync(&last, counts + ((k - 1) & 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (last == 0) return 0;
+    }
+    return fail("partition_decision: no fixed point after %d passes", ndiag + 1);
+}
+
+int32_t rh264_split_gate(int qp)
+{
+    static const int32_t qs[6] = {10, 11, 13, 14, 16, 18};
+    if (qp < 0 || qp > 51) return -1;
+    const int64_t q = (int64_t)qs[qp % 6] << (qp / 6);
+    return (int32_t)(((30 * (q + 160)) >> 3) * 2);
 }
 
 int rh264_decision_predict(rh264_ctx *ctx, int cw, int ch, int pad, int nframes, int nrefs, const uint8_t *planes,
