@@ -251,8 +251,24 @@ def run_decision(dev, args, cur, planes, lam, nf):
     ms_pred = dev.elapsed_ms(0, 1) / steps
     for x in tuple(out) + (smv, pred, spred):
         x.free()
+    # the decision with partitions (16x16, 16x8, 8x16, P_8x8 in the SATD currency)
+    gate = accel.split_gate(QP)
+    rec = accel.partition_decision(dev, p, nf, cur, planes, lambda_ref=lam, gate=gate, fc=True)
+    dev.sync()
+    dev.record(0)
+    for _ in range(steps):
+        accel.partition_decision(dev, p, nf, cur, planes, lambda_ref=lam, gate=gate, fc=True, out=rec)
+    dev.record(1)
+    dev.sync()
+    ms_part = dev.elapsed_ms(0, 1) / steps
+    modes = np.bincount(dev.download(rec, (nf * (W // 16) * (H // 16), 14), np.int32)[:, 0], minlength=4)
+    rec.free()
+    part = {"mpx_s": round(W * H * nf / (ms_part / 1e3) / 1e6, 2), "ms_per_step": round(ms_part, 4),
+            "split_gate": gate, "modes": {k: int(v) for k, v in zip(("16x16", "16x8", "8x16", "8x8"), modes)},
+            "what": "encoder-order decision with partitions: 16x16 best_part, then above the split gate 16x8, "
+                    "8x16 and P_8x8 best_parts seeded with the 16x16 vector, shape by SATD + rate"}
     return {"mpx_s": round(W * H * nf / (ms / 1e3) / 1e6, 2), "ms_per_step": round(ms, 4), "steps": steps,
-            "mean_cost": float(cost.mean()), "predict_ms": round(ms_pred, 4),
+            "mean_cost": float(cost.mean()), "predict_ms": round(ms_pred, 4), "partitions": part,
             "_gpu": (gmv, cost.reshape(nf, -1)),
             "what": "encoder-order decision: 16x16, 1 ref, predict_mv predictors, diamond 16/8/4 with the "
                     "fixed-centre passes (the reference's default), ring-8 walk; predict_ms: P_Skip vectors + MC "

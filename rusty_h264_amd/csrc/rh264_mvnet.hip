@@ -404,6 +404,7 @@ __global__ void k_topk(const float *__restrict__ logits, long ncell, int32_t *__
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
+constexpr int FW = 4;                                             // waves per fused workgroup (8 measured slower)
 constexpr int FT = 8;                                             // cells per tile side
 constexpr int S4 = 2 * FT, S3 = S4 + 2, S2 = S4 + 4, S1 = S4 + 6; // a4..a1 window sides
 constexpr int SX = 2 * S1 + 1;                                    // x0 window side (1/4 res)
@@ -474,12 +475,12 @@ __device__ __forceinline__ void conv32_window(const uint8_t *in, uint8_t *out, c
     // a wave takes U 16-pixel tiles at a time (2U independent accumulators)
     constexpr int SOUT = SIN - 2, NPX = SOUT * SOUT, NT = (NPX + 15) / 16;
     const int q = lane >> 4, col = lane & 15;
-    for (int tile = wave; tile < NT; tile += 4 * U) {
+    for (int tile = wave; tile < NT; tile += FW * U) {
         int pc[U], y[U], x[U];
         bool valid[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int px = (tile + 4 * u) * 16 + col;
+            const int px = (tile + FW * u) * 16 + col;
             valid[u] = px < NPX;
             pc[u] = valid[u] ? px : NPX - 1;
             y[u] = pc[u] / SOUT;
@@ -560,7 +561,7 @@ __device__ __forceinline__ void topk_merge_lanes(uint32_t t[8], int xr)
     bitonic8_desc(t);
 }
 
-__global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ x0, const float *__restrict__ p,
+__global__ __launch_bounds__(FW * 64) void k_fwd_fused(const uint16_t *__restrict__ x0, const float *__restrict__ p,
                                                    const uint16_t *__restrict__ wf, int h4, int w4, int tiles_x,
                                                    int tiles_y, float *__restrict__ logits,
                                                    int32_t *__restrict__ top8)
@@ -589,7 +590,7 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     {
         const uint32_t *xw = (const uint32_t *)x0 + (size_t)img * h4 * w4;
         const int y0 = 2 * (oy - 3) - 1, x0c = 2 * (ox - 3) - 1;
-        for (int i = tid; i < SX * SX; i += 256) {
+        for (int i = tid; i < SX * SX; i += FW * 64) {
             const int r = i / SX, c = i - r * SX, gy = y0 + r, gx = x0c + c;
             s_x[i] = gy >= 0 && gy < h4 && gx >= 0 && gx < w4 ? xw[(size_t)gy * w4 + gx] : 0u;
         }
@@ -610,7 +611,7 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
         float b[2][4];
         load_bias(p + OFF_B1, q, b);
         constexpr int NPX = S1 * S1, NT = (NPX + 15) / 16;
-        for (int tile = wave; tile < NT; tile += 4) {
+        for (int tile = wave; tile < NT; tile += FW) {
             const int px = tile * 16 + col;
             const bool valid = px < NPX;
             const int pc = valid ? px : NPX - 1;
@@ -645,15 +646,15 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     // with its FP32 bias, so the head loop reads LDS instead of waiting on global loads
     {
         const u32x4 *src = (const u32x4 *)(wf + WF_HEAD);
-        for (int i = tid; i < K * 4; i += 256) {
+        for (int i = tid; i < K * 4; i += FW * 64) {
             const int row = i >> 2, part = i & 3;
             *(u32x4 *)(s_a + row * HSB + 16 * part) = src[i];
         }
         float *s_bh = (float *)(s_a + K * HSB);
-        for (int i = tid; i < K; i += 256) s_bh[i] = p[OFF_BH + i];
+        for (int i = tid; i < K; i += FW * 64) s_bh[i] = p[OFF_BH + i];
     }
     // 2x2 average pool -> f[cell][c] as BF16 (PSB bytes per cell); two channels per thread
-    for (int i = tid; i < FT * FT * C / 2; i += 256) {
+    for (int i = tid; i < FT * FT * C / 2; i += FW * 64) {
         const int cell = i / (C / 2), c = 2 * (i - cell * (C / 2)), cy = cell / FT, cx = cell - cy * FT;
         const uint8_t *a = s_b + ((2 * cy) * S4 + 2 * cx) * PSB + 2 * c;
         uint32_t packed = 0;
@@ -668,8 +669,12 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     }
     __syncthreads();
 
-    // head + top-8: wave w takes cells 16w .. 16w+15 of the tile (MFMA column = cell)
-    const int cell = 16 * wave + col, cy = cell / FT, cx = cell - cy * FT;
+    // head + top-8: wave w takes cells 16 (w % 4) .. + 15 of the tile (MFMA column =
+    // cell) and half w / 4 of the bin tiles; the two halves' top-8 lists are merged
+    // through LDS
+    constexpr int NHALF = FW / 4; // wave groups sharing the bin tiles of the same 64 cells
+    const int half = wave >> 2;
+    const int cell = 16 * (wave & 3) + col, cy = cell / FT, cx = cell - cy * FT;
     const int gcy = ty * FT + cy, gcx = tx * FT + cx;
     const bool cvalid = gcy < hc && gcx < wc;
     const size_t gcell = ((size_t)img * hc + (cvalid ? gcy : 0)) * wc + (cvalid ? gcx : 0);
@@ -678,9 +683,10 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     uint32_t top[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) top[j] = 0u;
-    constexpr int NBT = (K + 15) / 16;
+    constexpr int NBT = (K + 15) / 16, NBH = NHALF == 2 ? (NBT + 1) / 2 : NBT;
+    const int bt0 = half ? NBH : 0, bt1 = half ? NBT : NBH;
 #pragma unroll 2
-    for (int bt = 0; bt < NBT; bt++) {
+    for (int bt = bt0; bt < bt1; bt++) {
         const int arow = 16 * bt + col;
         const u32x4 wv = arow < K ? *(const u32x4 *)(s_a + arow * HSB + 16 * q) : u32x4{0, 0, 0, 0};
         float4_t acc = {0, 0, 0, 0};
@@ -706,6 +712,23 @@ __global__ __launch_bounds__(256) void k_fwd_fused(const uint16_t *__restrict__ 
     }
     topk_merge_lanes(top, 16);
     topk_merge_lanes(top, 32);
+    // with two wave groups the upper hands its lists to the lower (s_b is free after the
+    // pooling)
+    uint32_t *s_top = (uint32_t *)s_b;
+    if (NHALF == 2 && half && q == 0) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) s_top[cell * 8 + j] = top[j];
+    }
+    if (NHALF == 2) __syncthreads();
+    if (half) return;
+    if (NHALF == 2) {
+        uint32_t o[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) o[j] = s_top[cell * 8 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) top[j] = max(top[j], o[7 - j]);
+        bitonic8_desc(top);
+    }
     if (top8 && cvalid && q == 0) {
         int b8[8];
 #pragma unroll
@@ -856,7 +879,7 @@ int mvnet_forward_ev(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const
         hipLaunchKernelGGL(k_prep, dim3(nblocks((long)n * h4 * w4, 256)), dim3(256), 0, ctx->stream, cur, ref, stride,
                            n, w, h, net->x0);
         if (after_prep) HIP_TRY(hipEventRecord(after_prep, ctx->stream));
-        hipLaunchKernelGGL(k_fwd_fused, dim3((unsigned)((long)n * tiles_x * tiles_y)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_fwd_fused, dim3((unsigned)((long)n * tiles_x * tiles_y)), dim3(FW * 64), 0, ctx->stream,
                            net->x0, net->p, net->wf, h4, w4, tiles_x, tiles_y, logits, top8);
         LAUNCH_CHECK("k_fwd_fused");
         return 0;
