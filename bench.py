@@ -165,18 +165,21 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
     dev.sync()
     ms_step = dev.elapsed_ms(0, 1) / args.steps
     import ctypes as C
-    ph = (C.c_float * 5)()
-    nph = C.c_int()
-    phases = []
+    tt = (C.c_float * 8)()
+    nev = C.c_int()
+    times = []
     for i in range(lib.rh264_profile_count(dev.ctx)):
-        lib.rh264_profile_phases(dev.ctx, i, ph, 5, C.byref(nph))
-        phases.append([ph[k] for k in range(nph.value)])
+        assert lib.rh264_profile_times(dev.ctx, i, tt, 8, C.byref(nev)) == 0, lib.rh264_last_error()
+        times.append([tt[k] for k in range(nev.value)])
     lib.rh264_set_profiling(dev.ctx, 0)
     cost = dev.download(outs[1], (nb,), np.int32).astype(np.int64)
     fb = dev.download(outs[2], (nb,), np.int32)
     net.close()
     pc = plain_cost.astype(np.int64)
-    ph_mean = np.mean(np.array(phases), axis=0) if phases else [0.0] * 5
+    # phase end times after the search's start: prep, network, candidates, fall-back, refine
+    t = np.mean(np.array(times), axis=0)
+    ph_mean = [t[0]] + [t[k] - t[k - 1] for k in range(1, 5)]
+    search_ms = float(t[-1])
     # the learned step's phases against their rooflines (algorithmic work per step
     # over the phase's event-timed duration on the library's stream)
     nblk1 = (W // 16) * (H // 16)
@@ -188,6 +191,8 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
     # the window's unique reads per block: the source, and each of F/H/V/C over the
     # 17x17 that the 16 operands at integer offsets {-1, 0}^2 cover
     win_bytes = nb * (256 + 4 * 17 * 17)
+    fbv = int(dict(kv.split("=", 1) for kv in args.tune).get("fallback_v", 1))
+    fb_kernel = f"k_full_search_list<16,16,{ {0: 0, 1: 1, 2: 3}[fbv] }>"
     prep_bytes = nf * (2 * W * H + 2 * w4 * h4 * 2)  # both frames read, the 1/4-res BF16 pair written
     roof = []
     for name, kernel, work, ms, peak, unit, bound, wunit in (
@@ -195,9 +200,9 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
             ("network", "k_fwd_fused", net_flop, ph_mean[1], BF16_PEAK_TFLOPS, "TFLOP/s", "mfma", "FLOP"),
             ("candidates", "k_cand_search_m<2>", cand_ops, ph_mean[2], VALU_PEAK_TOPS, "TOP/s", "valu",
              "abs-diff-adds"),
-            ("fall-back", "k_full_search_list<16,16,1>", fb_ops, ph_mean[3], VALU_PEAK_TOPS, "TOP/s", "valu",
-             "abs-diff-adds"),
-            ("refine", "k_refine_wn16<false>", win_bytes, ph_mean[4], HBM_PEAK_TBS, "TB/s", "hbm", "bytes")):
+            ("fall-back", fb_kernel, fb_ops, ph_mean[3], VALU_PEAK_TOPS, "TOP/s", "valu", "abs-diff-adds"),
+            ("refine", "k_refine_wn16<false>" if pat == 4 else f"k_refine_mx16<{pat},false>", win_bytes, ph_mean[4],
+             HBM_PEAK_TBS, "TB/s", "hbm", "bytes")):
         if ms > 0:
             a = work / (ms / 1e3) / 1e12
             roof.append({"phase": name, "kernel": kernel, "bound": bound, "work_per_step": float(work),
@@ -209,7 +214,8 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
             "blocks_worse": round(float((cost > pc).mean()), 5), "blocks_better": round(float((cost < pc).mean()), 5),
             "fallback_rate": round(float(fb.mean()), 5), "local_range": mvnet.LOCAL_RANGE, "fallback_cost": fb_cost,
             "refine": "+-2 quarter-pel window (subpel_pat 4)" if pat == 4 else f"ring walk (subpel_pat {pat})",
-            "phase_ms": {r["phase"]: r["ms"] for r in roof}}
+            "phase_ms": {r["phase"]: r["ms"] for r in roof}, "search_ms": round(search_ms, 4),
+            "plane_build_ms": round(ms_step - search_ms, 4)}
 
 
 def run_decision(dev, args, cur, planes, lam, nf):

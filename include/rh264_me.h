@@ -165,7 +165,8 @@ int rh264_build_planes(rh264_ctx *ctx, const uint8_t *frames, size_t frame_strid
  * lr[b] = (int64)(lambda * b) truncated exactly as mb16.rs:2563 computes
  * `(lam_r * rate as f64) as i64` (rate = mvbits(dx) + mvbits(dy), mb16.rs:
  * 2474-2475); lr_fp prices the SAD full-pel phase (B2, lambda * RFF_ME_SADL,
- * mb16.rs:2582-2590, 756-763), lr the SATD phase.  Use rh264_lambda_table(). */
+ * mb16.rs:2582-2590, 756-763), lr the SATD phase.  Use rh264_lambda_table().
+ * Entries lie in 0..2^26 (else the searches fail with an error). */
 typedef struct {
     int32_t cw, ch, pad;   /* frame size (multiples of 16) and plane pad         */
     int32_t bw, bh;        /* partition: 16x16, 16x8, 8x16, 8x8, 8x4, 4x8, 4x4   */
@@ -242,14 +243,16 @@ int rh264_plain_search_noalloc(rh264_ctx *ctx, const rh264_search_params *p, int
  * context's stream, without any host synchronisation.  Phases: plain = integer
  * search, refinement; learned = network input preparation, network, candidate
  * search (the local searches and their selection), fall-back full search (with its
- * merge), refinement.  rh264_profile_get(i) returns
- * search i's time before the refinement and the refinement's time;
- * rh264_profile_phases(i) writes up to `cap` phase durations (ms) and their count.
- * capacity 0 turns profiling off and resets the count. */
+ * merge), refinement.  rh264_profile_get(i) returns search i's time before its last
+ * phase and that phase's time; rh264_profile_phases(i) writes up to `cap` phase
+ * durations (ms) and their count; rh264_profile_times(i) writes each phase's end
+ * time after the search's start (ms) and their count.  capacity 0 turns profiling
+ * off and resets the count. */
 int rh264_set_profiling(rh264_ctx *ctx, int capacity);
 int rh264_profile_count(rh264_ctx *ctx);
 int rh264_profile_get(rh264_ctx *ctx, int i, float *search_ms, float *refine_ms);
 int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases);
+int rh264_profile_times(rh264_ctx *ctx, int i, float *ms, int cap, int *nevents);
 
 /* Kernel-variant selection (no reference counterpart; set by the autotuner).
  * Knobs: "fs_kernel" 0 = quad-SAD (v_qsad_pk_u16_u8) accumulator ring
@@ -261,7 +264,9 @@ int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases
  * 0 = the lane-per-4x4 kernel above; "cand_quad" 2 (default) = the learned search's
  * candidate windows (local range 1..4) take one lane per candidate, quad-SADs for
  * column groups of four and v_sad_u8 for the rest, 1 = quad-SAD column groups
- * (local range <= 3), 0 = v_sad_u8 lanes;
+ * (local range <= 3), 0 = v_sad_u8 lanes; "cand_lds" 1 (default) = that
+ * lane-per-candidate kernel reads each block's source rows from LDS where it uses
+ * them, 0 = it holds them in registers;
  * "fallback_v" 1 (default) = its fall-back full search uses the v_sad_u8 x2
  * variant, 2 = v_sad_u8 with one candidate per lane, 0 = quad-SADs; "planes_rw" 1 (default) = the plane build slides a
  * register window down 256x32 tiles, 0 = the LDS-tile kernel; "mvnet_fused" 1

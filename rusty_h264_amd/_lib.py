@@ -89,6 +89,7 @@ PROTOTYPES = {
     "rh264_profile_count": (C.c_int, [c_vp]),
     "rh264_profile_get": (C.c_int, [c_vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "rh264_profile_phases": (C.c_int, [c_vp, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
+    "rh264_profile_times": (C.c_int, [c_vp, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
     "rh264_tune": (C.c_int, [c_vp, C.c_char_p, C.c_int]),
     "rh264_mbtree_downsample": (C.c_int, [c_vp, c_vp, C.c_size_t, C.c_int, C.c_int, C.c_int, c_vp]),
     "rh264_mbtree_costs": (C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp]),

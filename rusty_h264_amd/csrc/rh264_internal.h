@@ -39,6 +39,7 @@ struct rh264_ctx {
     int refine_lds;               // refinement stages the +-3 pel window in LDS (blocks of >= 8 sub-blocks)
     int refine_mfma;              // 16x16 refinement: Hadamards as matrix-core products, one wave per block
     int cand_quad;                // learned search: candidate windows with quad-SADs (local range <= 3)
+    int cand_lds;                 // learned search: the lane-per-candidate kernel reads the source rows from LDS
     int fallback_v;               // learned search: fall-back full search, 0 = quad-SAD, 1 = v_sad_u8 x2
     int planes_rw;                // plane build: 1 = register-window kernel, 0 = LDS tile kernel
     int mvnet_fused;              // network inference: 1 = one fused kernel per tile, 0 = layer by layer
@@ -60,7 +61,8 @@ struct rh264_ctx {
 };
 
 // Network inference (rh264_mvnet_forward) recording `after_prep` (if not null) on the
-// context's stream between the input preparation and the fused network kernel.
+// context's stream between the input preparation and the fused network kernel, and
+// clearing the counter `zero` (if not null) on the stream before the network's end.
 extern "C" int mvnet_forward_ev(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref,
                                 size_t stride, int n, int w, int h, float *logits, int32_t *top8,
-                                hipEvent_t after_prep);
+                                hipEvent_t after_prep, unsigned *zero);

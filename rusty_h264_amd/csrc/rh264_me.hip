@@ -164,6 +164,37 @@ __device__ __forceinline__ uint32_t load4_clamped(const uint8_t *plane, int pw, 
     return v;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear workgroup id, and
+// each XCD has its own L2.  The search kernels renumber their workgroups so that
+// XCD x runs one contiguous share of the logical ones: neighbouring blocks, whose
+// reference windows overlap, then meet in one L2 instead of being fetched by all
+// eight.
+#ifndef RH264_XCD_MAP
+#define RH264_XCD_MAP 1
+#endif
+constexpr unsigned RH264_XCDS = 8;
+__device__ __forceinline__ unsigned xcd_linear(unsigned id, unsigned n)
+{
+    if (!RH264_XCD_MAP) return id;
+    const unsigned x = id % RH264_XCDS, i = id / RH264_XCDS, q = n / RH264_XCDS, r = n % RH264_XCDS;
+    return x * q + min(x, r) + i;
+}
+// logical (x, y) of this workgroup in a 2-D grid, and (x, y, z) in a 3-D one
+__device__ __forceinline__ void xcd_grid2(unsigned &bx, unsigned &by)
+{
+    const unsigned l = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    bx = l % gridDim.x;
+    by = l / gridDim.x;
+}
+__device__ __forceinline__ void xcd_grid3(unsigned &bx, unsigned &by, unsigned &bz)
+{
+    const unsigned gxy = gridDim.x * gridDim.y;
+    const unsigned l = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x + blockIdx.z * gxy, gxy * gridDim.z);
+    bz = l / gxy;
+    by = (l - bz * gxy) / gridDim.x;
+    bx = l - bz * gxy - by * gridDim.x;
+}
+
 // ---------------------------------------------------------------------------
 // plane build: F, H, V, C over the padded plane (inter.rs:816-843 + hpel.rs /
 // build_hpel_fused inter.rs:908-987).  Every tap index is clamped, which
@@ -296,9 +327,13 @@ __global__ __launch_bounds__(256) void k_build_planes_rw(const uint8_t *__restri
     __shared__ uint32_t s_src[PB_ROWS * PB_DW];
     const int pw = cw + 2 * pad, ph = ch + 2 * pad;
     const size_t n = (size_t)pw * ph;
-    const uint8_t *fr = frames + (size_t)blockIdx.z * frame_stride;
-    uint8_t *out = planes + (size_t)blockIdx.z * planes_bytes;
-    const int x0 = blockIdx.x * PB_W, y0 = blockIdx.y * PB_H;
+    // tiles in row-major order, one contiguous share per XCD: the tiles that share
+    // halo rows and boundary cache lines read them through one L2
+    unsigned tx, ty, tz;
+    xcd_grid3(tx, ty, tz);
+    const uint8_t *fr = frames + (size_t)tz * frame_stride;
+    uint8_t *out = planes + (size_t)tz * planes_bytes;
+    const int x0 = (int)tx * PB_W, y0 = (int)ty * PB_H;
     const int tid = threadIdx.x;
     for (int i = tid; i < PB_ROWS * PB_DW; i += 256) {
         const int r = i / PB_DW, d = i - r * PB_DW;
@@ -373,7 +408,7 @@ __global__ __launch_bounds__(256) void k_build_planes_rw(const uint8_t *__restri
         *(uint32_t *)(out + 2 * n + o) = vq;
         *(uint32_t *)(out + 3 * n + o) = cq;
     }
-    if (blockIdx.x == 0 && blockIdx.y == 0 && tid < 64) *(uint32_t *)(out + 4 * n + 4 * tid) = 0u;
+    if (tx == 0 && ty == 0 && tid < 64) *(uint32_t *)(out + 4 * n + 4 * tid) = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -677,29 +712,6 @@ __device__ __forceinline__ uint64_t fs_key(const FsJob &J, const int32_t *s_lr)
     if (V == 1) return fs_sad8_key<BW, BH, 2>(J, s_lr);
     if (V == 2) return fs_sad8_key<BW, BH, 4>(J, s_lr);
     return fs_sad8_key<BW, BH, 1>(J, s_lr);
-}
-
-// Workgroups are dispatched to the 8 XCDs round-robin by linear workgroup id, and
-// each XCD has its own L2.  The search kernels renumber their workgroups so that
-// XCD x runs one contiguous share of the logical ones: neighbouring blocks, whose
-// reference windows overlap, then meet in one L2 instead of being fetched by all
-// eight.
-#ifndef RH264_XCD_MAP
-#define RH264_XCD_MAP 1
-#endif
-constexpr unsigned RH264_XCDS = 8;
-__device__ __forceinline__ unsigned xcd_linear(unsigned id, unsigned n)
-{
-    if (!RH264_XCD_MAP) return id;
-    const unsigned x = id % RH264_XCDS, i = id / RH264_XCDS, q = n / RH264_XCDS, r = n % RH264_XCDS;
-    return x * q + min(x, r) + i;
-}
-// logical (x, y) of this workgroup in a 2-D grid
-__device__ __forceinline__ void xcd_grid2(unsigned &bx, unsigned &by)
-{
-    const unsigned l = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    bx = l % gridDim.x;
-    by = l / gridDim.x;
 }
 
 // Single-stream integer search: nframes frames of one geometry (blockIdx.y = frame).
@@ -1183,6 +1195,19 @@ __device__ __forceinline__ mx_f4 mx_transform(uint32_t p, const MxConst &K, mx_f
     return __builtin_amdgcn_mfma_f32_16x16x16f16(K.hb, b, c2, 0, 0, 0);
 }
 
+// Minimum over the wave, returned as a scalar: the same DPP steps as mx_wave_sum,
+// each lane keeping its own value where a step has no source.
+__device__ __forceinline__ uint32_t mx_wave_min(uint32_t v)
+{
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false)); // row_half_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false)); // row_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x142, 0xA, 0xF, false)); // row_bcast:15
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x143, 0xC, 0xF, false)); // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // SATD of the candidate: the wave's sum |Y|, halved for the 8x8 transform (exact:
 // the sum over each 8x8 block is even)
 template <bool T8 = false> __device__ __forceinline__ int mx_satd(uint32_t p, const MxConst &K, mx_f4 c2)
@@ -1544,58 +1569,27 @@ __device__ __forceinline__ void wn_partials(float (&t)[25], const uint32_t (&o)[
     ((t[C] = partial(wn_operand<C>(o))), ...);
 }
 
+// One block's +-2 quarter-pel window refinement by one wave: the 25 positions around
+// the start (bx, by), SATD + rate against the predictor (mpx, mpy) priced with the
+// table s_lr, the first minimum (the start first, then raster order); lane 0 writes
+// out_mv / out_cost (/ out_evals) of block boff.  cur_f: the block's frame of the
+// current picture (stride p.cw); pl: its reference plane set.
 template <bool T8>
-__global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, const uint8_t *__restrict__ cur,
-                                                     size_t cur_stride, const uint8_t *__restrict__ planes,
-                                                     size_t planes_stride, const uint64_t *__restrict__ keys,
-                                                     const int32_t *__restrict__ centres,
-                                                     const int32_t *__restrict__ start_mv,
-                                                     const int32_t *__restrict__ mvps, int32_t *__restrict__ int_mv,
-                                                     int32_t *__restrict__ int_cost, int32_t *__restrict__ out_mv,
-                                                     int32_t *__restrict__ out_cost, int32_t *__restrict__ out_evals)
+__device__ __forceinline__ void wn16_block(const rh264_search_params &p, const int32_t *s_lr,
+                                           const uint8_t *__restrict__ cur_f, const uint8_t *__restrict__ pl, int lx,
+                                           int ly, int bx, int by, int mpx, int mpy, int lane, size_t boff,
+                                           int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost,
+                                           int32_t *__restrict__ out_evals)
 {
-    __shared__ int32_t s_lr[RH264_LR_N];
-    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
-    __syncthreads();
-
-    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    unsigned wgx, wgy;
-    xcd_grid2(wgx, wgy);
-    const int b = (int)wgx * 4 + wave; // wave-uniform
-    if (b >= nblk) return;
-    const int frame = (int)wgy;
-    const size_t boff = (size_t)frame * nblk + b;
-    const int lx = (b % nbx) << 4, ly = (b / nbx) << 4;
     const int r = lane & 15, qd = lane >> 4; // pixel row, column quad of this lane
-
     MxBlock B;
     B.pw = p.cw + 2 * p.pad;
     B.ph = p.ch + 2 * p.pad;
     B.pn = (size_t)B.pw * B.ph;
-    B.pl = planes + (size_t)frame * planes_stride;
-    B.mpx = __builtin_amdgcn_readfirstlane(mvps ? mvps[2 * boff] : 0);
-    B.mpy = __builtin_amdgcn_readfirstlane(mvps ? mvps[2 * boff + 1] : 0);
-    const uint32_t src = *(const uint32_t *)(cur + (size_t)frame * cur_stride + (size_t)(ly + r) * p.cw + lx + 4 * qd);
-
-    int bx, by;
-    if (keys) {
-        const uint64_t key = keys[boff];
-        const int NC = 2 * p.range + 1;
-        const uint32_t idx = __builtin_amdgcn_readfirstlane((uint32_t)key);
-        const int t = (int)(idx / (uint32_t)NC), c = (int)(idx % (uint32_t)NC);
-        const int cx = __builtin_amdgcn_readfirstlane(centres ? centres[2 * boff] : 0);
-        const int cy = __builtin_amdgcn_readfirstlane(centres ? centres[2 * boff + 1] : 0);
-        bx = cx + 4 * (c - p.range);
-        by = cy + 4 * (t - p.range);
-        if (lane == 0) {
-            if (int_mv) { int_mv[2 * boff] = bx; int_mv[2 * boff + 1] = by; }
-            if (int_cost) int_cost[boff] = (int32_t)(key >> 32);
-        }
-    } else {
-        bx = __builtin_amdgcn_readfirstlane(start_mv[2 * boff]);
-        by = __builtin_amdgcn_readfirstlane(start_mv[2 * boff + 1]);
-    }
+    B.pl = pl;
+    B.mpx = mpx;
+    B.mpy = mpy;
+    const uint32_t src = *(const uint32_t *)(cur_f + (size_t)(ly + r) * p.cw + lx + 4 * qd);
     const int fx0 = bx & 3, fy0 = by & 3;
     B.sx = bx - fx0;
     B.sy = by - fy0;
@@ -1646,17 +1640,12 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
         const int dx = pos % 5 - 2, dy = pos / 5 - 2;
         const bool valid = pos < 25;
         const int rate = s_lr[valid ? mvbits(bx + dx - B.mpx) + mvbits(by + dy - B.mpy) : 0];
-        // (cost, order): the start first, then raster order
-        uint64_t key = valid ? ((uint64_t)(uint32_t)(satd + rate) << 5) | (uint32_t)(pos == 12 ? 0 : pos + 1) : ~0ull;
-#pragma unroll
-        for (int m = 1; m < 64; m <<= 1) {
-            const uint64_t o2 = __shfl_xor(key, m);
-            key = o2 < key ? o2 : key;
-        }
-        key = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(key >> 32)) << 32) |
-              __builtin_amdgcn_readfirstlane((uint32_t)key);
-        best = (int)(key >> 5);
-        const int ord = (int)(key & 31);
+        // (cost, order): the start first, then raster order; cost < 2^22 + 2^26 (the
+        // SATD bound and the lambda tables' bound, check_params) fits 27 bits
+        const uint32_t key = valid ? ((uint32_t)(satd + rate) << 5) | (uint32_t)(pos == 12 ? 0 : pos + 1) : ~0u;
+        const uint32_t kmin = mx_wave_min(key);
+        best = (int)(kmin >> 5);
+        const int ord = (int)(kmin & 31);
         best_pos = ord == 0 ? 12 : ord - 1;
     } else {
         auto cost_at = [&](int c) -> int {
@@ -1678,6 +1667,66 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
         out_cost[boff] = best;
         if (out_evals) out_evals[boff] = 25;
     }
+}
+
+
+// The +-2 quarter-pel window refinement of every 16x16 block (one wave each), from
+// the integer search's keys or from given starts; with fflag, blocks flagged in it
+// start from their fall-back search's key in fkeys (centre 0, radius p.range)
+// instead.
+template <bool T8>
+__global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, const uint8_t *__restrict__ cur,
+                                                     size_t cur_stride, const uint8_t *__restrict__ planes,
+                                                     size_t planes_stride, const uint64_t *__restrict__ keys,
+                                                     const int32_t *__restrict__ centres,
+                                                     const int32_t *__restrict__ start_mv,
+                                                     const int32_t *__restrict__ mvps, int32_t *__restrict__ int_mv,
+                                                     int32_t *__restrict__ int_cost, int32_t *__restrict__ out_mv,
+                                                     int32_t *__restrict__ out_cost, int32_t *__restrict__ out_evals,
+                                                     const int32_t *__restrict__ fflag,
+                                                     const uint64_t *__restrict__ fkeys)
+{
+    __shared__ int32_t s_lr[RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
+    __syncthreads();
+
+    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    unsigned wgx, wgy;
+    xcd_grid2(wgx, wgy);
+    const int b = (int)wgx * 4 + wave; // wave-uniform
+    if (b >= nblk) return;
+    const int frame = (int)wgy;
+    const size_t boff = (size_t)frame * nblk + b;
+    const int lx = (b % nbx) << 4, ly = (b / nbx) << 4;
+    const int mpx = __builtin_amdgcn_readfirstlane(mvps ? mvps[2 * boff] : 0);
+    const int mpy = __builtin_amdgcn_readfirstlane(mvps ? mvps[2 * boff + 1] : 0);
+    int bx, by;
+    if (keys) {
+        const uint64_t key = keys[boff];
+        const int NC = 2 * p.range + 1;
+        const uint32_t idx = __builtin_amdgcn_readfirstlane((uint32_t)key);
+        const int t = (int)(idx / (uint32_t)NC), c = (int)(idx % (uint32_t)NC);
+        const int cx = __builtin_amdgcn_readfirstlane(centres ? centres[2 * boff] : 0);
+        const int cy = __builtin_amdgcn_readfirstlane(centres ? centres[2 * boff + 1] : 0);
+        bx = cx + 4 * (c - p.range);
+        by = cy + 4 * (t - p.range);
+        if (lane == 0) {
+            if (int_mv) { int_mv[2 * boff] = bx; int_mv[2 * boff + 1] = by; }
+            if (int_cost) int_cost[boff] = (int32_t)(key >> 32);
+        }
+    } else if (fflag && __builtin_amdgcn_readfirstlane(fflag[boff])) {
+        // a fall-back block starts at its zero-centred full search's vector
+        const uint32_t idx = __builtin_amdgcn_readfirstlane((uint32_t)fkeys[boff]);
+        const int NC = 2 * p.range + 1;
+        bx = 4 * ((int)(idx % (uint32_t)NC) - p.range);
+        by = 4 * ((int)(idx / (uint32_t)NC) - p.range);
+    } else {
+        bx = __builtin_amdgcn_readfirstlane(start_mv[2 * boff]);
+        by = __builtin_amdgcn_readfirstlane(start_mv[2 * boff + 1]);
+    }
+    wn16_block<T8>(p, s_lr, cur + (size_t)frame * cur_stride, planes + (size_t)frame * planes_stride, lx, ly, bx, by,
+                   mpx, mpy, lane, boff, out_mv, out_cost, out_evals);
 }
 
 // ---------------------------------------------------------------------------
@@ -3626,8 +3675,8 @@ void k_cand_search_q(rh264_search_params p, const uint8_t *__restrict__ cur, siz
     }
 }
 
-template <int RL, bool FAST>
-__device__ __forceinline__ void cand_rows_m(const uint32_t (&s)[16][4], const uint8_t *__restrict__ F, int pw, int ph,
+template <int RL, bool FAST, typename SRC>
+__device__ __forceinline__ void cand_rows_m(const SRC &src, const uint8_t *__restrict__ F, int pw, int ph,
                                             int x, int y0, uint64_t (&aq)[(2 * RL + 1) / 4 > 0 ? (2 * RL + 1) / 4 : 1][2 * RL + 1],
                                             uint32_t (&ar)[(2 * RL + 1) % 4 > 0 ? (2 * RL + 1) % 4 : 1][2 * RL + 1])
 {
@@ -3652,11 +3701,13 @@ __device__ __forceinline__ void cand_rows_m(const uint32_t (&s)[16][4], const ui
         for (int t = 0; t < NC; t++) {
             const int r = j - t;
             if (r < 0 || r >= 16) continue;
+            const uint4 sv = src(r); // source row r
+            const uint32_t s[4] = {sv.x, sv.y, sv.z, sv.w};
 #pragma unroll
             for (int g = 0; g < NQ; g++)
 #pragma unroll
                 for (int i = 0; i < 4; i++)
-                    aq[g][t] = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)w[g + i + 1] << 32) | w[g + i], s[r][i],
+                    aq[g][t] = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)w[g + i + 1] << 32) | w[g + i], s[i],
                                                                aq[g][t]);
 #pragma unroll
             for (int q = 0; q < REM; q++) {
@@ -3667,7 +3718,7 @@ __device__ __forceinline__ void cand_rows_m(const uint32_t (&s)[16][4], const ui
                                                          : __builtin_amdgcn_alignbyte(w[(col >> 2) + i + 1],
                                                                                       w[(col >> 2) + i],
                                                                                       (uint32_t)(col & 3));
-                    ar[q][t] = __builtin_amdgcn_sad_u8(s[r][i], ref4, ar[q][t]);
+                    ar[q][t] = __builtin_amdgcn_sad_u8(s[i], ref4, ar[q][t]);
                 }
             }
         }
@@ -3680,8 +3731,8 @@ __device__ __forceinline__ void cand_rows_m(const uint32_t (&s)[16][4], const ui
 // remaining columns with v_sad_u8 (no column beyond the window is computed), so a
 // block costs half the loads of k_cand_search_q and no wasted quad-SAD columns.
 // Keys and tie-breaks are those of k_cand_search.
-template <int RL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RL <= 2 ? 3 : 2)))
+template <int RL, bool SL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SL ? 4 : (RL <= 2 ? 3 : 2))))
 void k_cand_search_m(rh264_search_params p, const uint8_t *__restrict__ cur, size_t cur_stride,
                      const uint8_t *__restrict__ planes, size_t planes_stride, const int32_t *__restrict__ top8,
                      const int32_t *__restrict__ mvps, long nb, int32_t fb_cost, int32_t *__restrict__ start,
@@ -3722,11 +3773,19 @@ void k_cand_search_m(rh264_search_params p, const uint8_t *__restrict__ cur, siz
         const int mpx = mvps ? mvps[2 * gb] : 0, mpy = mvps ? mvps[2 * gb + 1] : 0;
         const int pw = p.cw + 2 * p.pad, ph = p.ch + 2 * p.pad;
         const uint8_t *F = planes + (size_t)frame * planes_stride;
-        uint32_t sr[16][4];
+        // the source rows: read from LDS where they are used (SL), else held in registers
+        uint32_t sr[SL ? 1 : 16][4];
+        if (!SL) {
 #pragma unroll
-        for (int r = 0; r < 16; r++)
+            for (int r = 0; r < 16; r++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) sr[r][i] = s_src[slot][4 * r + i];
+                for (int i = 0; i < 4; i++) sr[SL ? 0 : r][i] = s_src[slot][4 * r + i];
+        }
+        const uint32_t *lsrc = s_src[slot];
+        const auto src = [&](int r) -> uint4 {
+            if (SL) return *(const uint4 *)(lsrc + 4 * r);
+            return make_uint4(sr[SL ? 0 : r][0], sr[SL ? 0 : r][1], sr[SL ? 0 : r][2], sr[SL ? 0 : r][3]);
+        };
         const int x = lx + (cx >> 2) + p.pad - RL; // padded column of the window's first candidate
         const int y0 = ly + (cy >> 2) + p.pad - RL; // padded row of the window's first row
         const bool fast = x >= 0 && (x & ~3) + 4 * (NW + 1) <= pw && y0 >= 0 && y0 + NJ <= ph;
@@ -3741,9 +3800,9 @@ void k_cand_search_m(rh264_search_params p, const uint8_t *__restrict__ cur, siz
             for (int q = 0; q < REM; q++) ar[q][t] = 0;
         }
         if (fast)
-            cand_rows_m<RL, true>(sr, F, pw, ph, x, y0, aq, ar);
+            cand_rows_m<RL, true>(src, F, pw, ph, x, y0, aq, ar);
         else
-            cand_rows_m<RL, false>(sr, F, pw, ph, x, y0, aq, ar);
+            cand_rows_m<RL, false>(src, F, pw, ph, x, y0, aq, ar);
         uint64_t best = ~0ull;
 #pragma unroll
         for (int t = 0; t < NC; t++) {
@@ -3873,6 +3932,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->refine_lds = 1;
     c->refine_mfma = 1;
     c->cand_quad = 2;
+    c->cand_lds = 1;
     c->fallback_v = 1;
     c->planes_rw = 1;
     c->mvnet_fused = 1;
@@ -4037,7 +4097,7 @@ static int check_params(const rh264_search_params *p)
     if (p->hadamard != 0 && p->hadamard != 4 && p->hadamard != 8) return fail("hadamard %d: 4 or 8", p->hadamard);
     if (p->hadamard == 8 && (p->bw != 16 || p->bh != 16)) return RH264_DECLINED;
     for (int i = 0; i < RH264_LR_N; i++)
-        if (p->lr[i] < 0 || p->lr_fp[i] < 0 || p->lr[i] > (1 << 28) || p->lr_fp[i] > (1 << 28))
+        if (p->lr[i] < 0 || p->lr_fp[i] < 0 || p->lr[i] > (1 << 26) || p->lr_fp[i] > (1 << 26))
             return fail("lambda table entry %d out of range", i);
     return 0;
 }
@@ -4076,7 +4136,8 @@ static int launch_refine(rh264_ctx *ctx, const rh264_search_params *p, int nfram
                          size_t cur_stride, const uint8_t *planes, size_t planes_stride,
                          const uint64_t *keys, const int32_t *centres, const int32_t *start_mv,
                          const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv,
-                         int32_t *cost, int32_t *evals)
+                         int32_t *cost, int32_t *evals, const int32_t *fflag = nullptr,
+                         const uint64_t *fkeys = nullptr)
 {
     constexpr int GPW = RefShape<BW, BH>::GPW;
     const int nblk = (p->cw / BW) * (p->ch / BH);
@@ -4100,11 +4161,11 @@ static int launch_refine(rh264_ctx *ctx, const rh264_search_params *p, int nfram
                 if (t8)
                     hipLaunchKernelGGL(k_refine_wn16<true>, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
                                        planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv,
-                                       cost, evals);
+                                       cost, evals, fflag, fkeys);
                 else
                     hipLaunchKernelGGL(k_refine_wn16<false>, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
                                        planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv,
-                                       cost, evals);
+                                       cost, evals, fflag, fkeys);
                 LAUNCH_CHECK("k_refine_wn16");
                 return 0;
             }
@@ -4310,6 +4371,18 @@ int rh264_profile_phases(rh264_ctx *ctx, int i, float *ms, int cap, int *nphases
     return 0;
 }
 
+int rh264_profile_times(rh264_ctx *ctx, int i, float *ms, int cap, int *nevents)
+{
+    if (i < 0 || i >= ctx->prof_n) return fail("profile slot %d of %d", i, ctx->prof_n);
+    if (!ms || !nevents) return fail("rh264_profile_times: null pointers");
+    const int np = ctx->prof_np[(size_t)i];
+    hipEvent_t *ev = &ctx->prof[(size_t)RH264_PROF_EV * i];
+    HIP_TRY(hipEventSynchronize(ev[np]));
+    for (int k = 0; k < np && k < cap; k++) HIP_TRY(hipEventElapsedTime(&ms[k], ev[0], ev[k + 1]));
+    *nevents = np;
+    return 0;
+}
+
 int rh264_profile_count(rh264_ctx *ctx) { return ctx->prof_n; }
 
 int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
@@ -4336,6 +4409,11 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
     if (!strcmp(knob, "decide_mode")) {
         if (value < 0 || value > 2) return fail("decide_mode %d: 0, 1 or 2", value);
         ctx->decide_mode = value;
+        return 0;
+    }
+    if (!strcmp(knob, "cand_lds")) {
+        if (value < 0 || value > 1) return fail("cand_lds %d: 0 or 1", value);
+        ctx->cand_lds = value;
         return 0;
     }
     if (!strcmp(knob, "fallback_v")) {
@@ -5406,19 +5484,27 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
     // phases: input preparation, network, candidate search, fall-back search, refinement
     hipEvent_t *ev = prof_slot(ctx, 5);
     if (ev) HIP_TRY(hipEventRecord(ev[0], ctx->stream));
+    // the network's input preparation also clears the fall-back list's counter
     rc = mvnet_forward_ev(ctx, net, cur, ref, frame_stride, nframes, p->cw, p->ch, nullptr, top8,
-                          ev && ctx->mvnet_fused ? ev[1] : nullptr);
+                          ev && ctx->mvnet_fused ? ev[1] : nullptr, count);
     if (rc) return rc;
     if (ev && !ctx->mvnet_fused) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
     if (ev) HIP_TRY(hipEventRecord(ev[2], ctx->stream));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, count, (size_t)1, 0u);
     if (local_range >= 1 && local_range <= 4 && ctx->cand_quad == 2) {
         // one lane per candidate: quad-SAD column groups plus v_sad_u8 for the rest; the
         // kernel clears the fall-back blocks' argmin slots itself
 #define CM_LAUNCH(RL)                                                                                              \
-    hipLaunchKernelGGL(k_cand_search_m<RL>, dim3((unsigned)((nb + 256 / LS_NCAND - 1) / (256 / LS_NCAND))), dim3(256), \
-                       0, ctx->stream, *p, cur, frame_stride, planes, planes_stride, top8, (const int32_t *)nullptr,  \
-                       nb, fallback_cost, start, fallback, count, list, keysf)
+    do {                                                                                                           \
+        const dim3 grid((unsigned)((nb + 256 / LS_NCAND - 1) / (256 / LS_NCAND)));                                 \
+        if (ctx->cand_lds)                                                                                         \
+            hipLaunchKernelGGL((k_cand_search_m<RL, true>), grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride,  \
+                               planes, planes_stride, top8, (const int32_t *)nullptr, nb, fallback_cost, start,     \
+                               fallback, count, list, keysf);                                                       \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_cand_search_m<RL, false>), grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride, \
+                               planes, planes_stride, top8, (const int32_t *)nullptr, nb, fallback_cost, start,     \
+                               fallback, count, list, keysf);                                                       \
+    } while (0)
         switch (local_range) {
         case 1: CM_LAUNCH(1); break;
         case 2: CM_LAUNCH(2); break;
@@ -5502,12 +5588,18 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
             hipLaunchKernelGGL((k_full_search_list<16, 16, 0>), grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride,
                                planes, planes_stride, list, count, (const int32_t *)nullptr, keysf);
     }
-    hipLaunchKernelGGL(k_fallback_merge, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, list, count,
-                       keysf, p->range, start);
-    LAUNCH_CHECK("learned_search select");
+    // the window refinement reads the fall-back blocks' starts from their keys itself;
+    // the other refinements take them merged into `start`
+    const bool wn = p->subpel_pat == 4 && (ctx->refine_mfma || p->hadamard == 8);
+    if (!wn) {
+        hipLaunchKernelGGL(k_fallback_merge, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, list,
+                           count, keysf, p->range, start);
+        LAUNCH_CHECK("learned_search select");
+    }
     if (ev) HIP_TRY(hipEventRecord(ev[4], ctx->stream));
     rc = launch_refine<16, 16>(ctx, p, nframes, cur, frame_stride, planes, planes_stride, nullptr, nullptr, start,
-                               nullptr, nullptr, nullptr, mv, cost, nullptr);
+                               nullptr, nullptr, nullptr, mv, cost, nullptr, wn ? fallback : nullptr,
+                               wn ? keysf : nullptr);
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[5], ctx->stream));
     return 0;

@@ -55,14 +55,22 @@ __device__ __forceinline__ uint32_t f2bf2(float lo, float hi)
     return __builtin_bit_cast(uint32_t, v);
 }
 
+__global__ void k_fill_u32_nv(unsigned *p)
+{
+    if (threadIdx.x == 0) *p = 0u;
+}
+
 // ---------------------------------------------------------------------------
 // input preparation: 1/4-resolution current + reference, 2 channels
 // ---------------------------------------------------------------------------
+// zero (may be null): a counter of the caller's next kernels, cleared here to save
+// them a launch
 __global__ void k_prep(const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, size_t stride, int n, int w,
-                       int h, uint16_t *__restrict__ x0)
+                       int h, uint16_t *__restrict__ x0, unsigned *__restrict__ zero)
 {
     const int w4 = w / 4, h4 = h / 4;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (zero && i == 0) *zero = 0u;
     if (i >= (long)n * w4 * h4) return;
     const int img = (int)(i / ((long)w4 * h4));
     const int rem = (int)(i - (long)img * w4 * h4);
@@ -850,7 +858,7 @@ static int mvnet_forward_impl(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *c
     const int h4 = h / 4, w4 = w / 4, h8 = h / 8, w8 = w / 8;
     const long cells = (long)n * (h / 16) * (w / 16);
     hipLaunchKernelGGL(k_prep, dim3(nblocks((long)n * h4 * w4, 256)), dim3(256), 0, ctx->stream, cur, ref, stride, n, w,
-                       h, net->x0);
+                       h, net->x0, (unsigned *)nullptr);
     hipLaunchKernelGGL(k_conv1_fwd, dim3(nblocks((long)n * h8 * w8, 256)), dim3(256), 0, ctx->stream, net->x0, net->p,
                        n, h4, w4, net->a1);
     const long waves = (long)n * h8 * ((w8 + 15) / 16);
@@ -867,7 +875,7 @@ static int mvnet_forward_impl(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *c
 }
 
 int mvnet_forward_ev(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref, size_t stride, int n,
-                     int w, int h, float *logits, int32_t *top8, hipEvent_t after_prep)
+                     int w, int h, float *logits, int32_t *top8, hipEvent_t after_prep, unsigned *zero)
 {
     int rc = mvnet_geom(net, n, w, h);
     if (rc) return rc;
@@ -877,7 +885,7 @@ int mvnet_forward_ev(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const
         const int h4 = h / 4, w4 = w / 4;
         const int tiles_x = (w / 16 + FT - 1) / FT, tiles_y = (h / 16 + FT - 1) / FT;
         hipLaunchKernelGGL(k_prep, dim3(nblocks((long)n * h4 * w4, 256)), dim3(256), 0, ctx->stream, cur, ref, stride,
-                           n, w, h, net->x0);
+                           n, w, h, net->x0, zero);
         if (after_prep) HIP_TRY(hipEventRecord(after_prep, ctx->stream));
         hipLaunchKernelGGL(k_fwd_fused, dim3((unsigned)((long)n * tiles_x * tiles_y)), dim3(FW * 64), 0, ctx->stream,
                            net->x0, net->p, net->wf, h4, w4, tiles_x, tiles_y, logits, top8);
@@ -886,6 +894,10 @@ int mvnet_forward_ev(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const
     }
     rc = mvnet_forward_impl(ctx, net, cur, ref, stride, n, w, h);
     if (rc) return rc;
+    if (zero) {
+        hipLaunchKernelGGL(k_fill_u32_nv, dim3(1), dim3(64), 0, ctx->stream, zero);
+        LAUNCH_CHECK("k_fill_u32_nv");
+    }
     if (logits)
         HIP_TRY(hipMemcpyAsync(logits, net->logits, (size_t)cells * K * 4, hipMemcpyDeviceToDevice, ctx->stream));
     if (top8) {
@@ -898,7 +910,7 @@ int mvnet_forward_ev(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const
 int rh264_mvnet_forward(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const uint8_t *ref, size_t stride,
                         int n, int w, int h, float *logits, int32_t *top8)
 {
-    return mvnet_forward_ev(ctx, net, cur, ref, stride, n, w, h, logits, top8, nullptr);
+    return mvnet_forward_ev(ctx, net, cur, ref, stride, n, w, h, logits, top8, nullptr, nullptr);
 }
 
 // Forward + backward on a batch; gradients (mean cross-entropy over cells) left in

@@ -105,8 +105,8 @@ def test_learned_search_recosting_is_exact(dev, Rl, cand_quad, fallback_v, pat):
     and once with the threshold at the median candidate cost so both paths are
     exercised.  Local ranges up to 3 take the quad-SAD candidate kernel (cand_quad
     1) or the v_sad_u8 one, 4 the latter, larger ones the per-candidate integer
-    searches; cand_quad 2 is the lane-per-candidate kernel (ranges 1..4); both
-    fall-back search variants are covered."""
+    searches; cand_quad 2 is the lane-per-candidate kernel (ranges 1..4); every
+    fall-back search variant is covered."""
     dev.tune("cand_quad", cand_quad)
     dev.tune("fallback_v", fallback_v)
     try:
@@ -116,8 +116,30 @@ def test_learned_search_recosting_is_exact(dev, Rl, cand_quad, fallback_v, pat):
         dev.tune("fallback_v", 1)
 
 
-def _learned_recost_case(dev, Rl, pat):
-    cw, ch, pad, Rf = 176, 144, 32, 16
+def test_learned_search_source_in_registers(dev):
+    """cand_lds 0: the lane-per-candidate kernel holds the source rows in registers
+    instead of reading them from LDS; the same (oracle) results."""
+    dev.tune("cand_lds", 0)
+    try:
+        _learned_recost_case(dev, 2, 4)
+    finally:
+        dev.tune("cand_lds", 1)
+
+
+@pytest.mark.parametrize("fallback_v", [0, 1])
+def test_learned_search_windows_leave_the_planes(dev, fallback_v):
+    """pad 8 < the +-16 fall-back radius: edge blocks' fall-back windows (and the
+    network's centres up to 32 pels out) leave the padded planes and take the clamped
+    reads; the results stay the oracle's."""
+    dev.tune("fallback_v", fallback_v)
+    try:
+        _learned_recost_case(dev, 2, 4, pad=8)
+    finally:
+        dev.tune("fallback_v", 1)
+
+
+def _learned_recost_case(dev, Rl, pat, pad=32):
+    cw, ch, Rf = 176, 144, 16
     frames = synth.hd_clip(3, seed=31, w=cw, h=ch, nrects=3)
     nf = 2
     clip = dev.upload(frames)
