@@ -197,7 +197,7 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
     roof = []
     for name, kernel, work, ms, peak, unit, bound, wunit in (
             ("input preparation", "k_prep", prep_bytes, ph_mean[0], HBM_PEAK_TBS, "TB/s", "hbm", "bytes"),
-            ("network", "k_fwd_fused", net_flop, ph_mean[1], BF16_PEAK_TFLOPS, "TFLOP/s", "mfma", "FLOP"),
+            ("network", "k_fwd_rr", net_flop, ph_mean[1], BF16_PEAK_TFLOPS, "TFLOP/s", "mfma", "FLOP"),
             ("candidates", "k_cand_search_m<2>", cand_ops, ph_mean[2], VALU_PEAK_TOPS, "TOP/s", "valu",
              "abs-diff-adds"),
             ("fall-back", fb_kernel, fb_ops, ph_mean[3], VALU_PEAK_TOPS, "TOP/s", "valu", "abs-diff-adds"),

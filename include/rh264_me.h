@@ -269,9 +269,10 @@ int rh264_profile_times(rh264_ctx *ctx, int i, float *ms, int cap, int *nevents)
  * them, 0 = it holds them in registers;
  * "fallback_v" 1 (default) = its fall-back full search uses the v_sad_u8 x2
  * variant, 2 = v_sad_u8 with one candidate per lane, 0 = quad-SADs; "planes_rw" 1 (default) = the plane build slides a
- * register window down 256x32 tiles, 0 = the LDS-tile kernel; "mvnet_fused" 1
- * (default) = network inference
- * in one fused kernel per tile, 0 = layer by layer; "decide_mode" of
+ * register window down 256x32 tiles, 0 = the LDS-tile kernel; "mvnet_fused" 2
+ * (default) = network inference in one fused kernel per tile of 12x24 MBs holding the
+ * activations in registers, 1 = the fused kernel with the activations in LDS (tiles of
+ * 8x8 MBs), 0 = layer by layer; "decide_mode" of
  * rh264_motion_search_wavefront: 0 = one launch per anti-diagonal, 1 = fixed-point
  * passes (each re-searches the macroblocks whose predictors changed; the stream
  * is synchronised every 8 passes to test for the fixed point), 2 (default) = 1
@@ -470,8 +471,9 @@ int rh264_batch_destroy(rh264_batch *b);
  * FP32 (rh264_mvnet_param_count of them, layout in that file).
  * rh264_mvnet_forward: logits [n][mb][289] (device, may be NULL) and/or top8
  *   [n][mb][8] bin indices (device, 16-byte aligned, may be NULL) for frames
- *   cur/ref (stride apart); by default one fused kernel per tile of 8x8 MBs
- *   (rh264_tune "mvnet_fused" 0 selects the layer-by-layer kernels of training).
+ *   cur/ref (stride apart); by default one fused kernel per tile of 12x24 MBs
+ *   (rh264_tune "mvnet_fused" 1 selects the LDS-tile fused kernel, 0 the
+ *   layer-by-layer kernels of training).
  *   The four conv layers use BF16 weights and activations, FP32 accumulation.
  * rh264_mvnet_backward: forward + backward of the mean cross-entropy against
  *   labels [n][mb] (bin indices); gradients stay on the device (get_grads), the

@@ -42,7 +42,8 @@ struct rh264_ctx {
     int cand_lds;                 // learned search: the lane-per-candidate kernel reads the source rows from LDS
     int fallback_v;               // learned search: fall-back full search, 0 = quad-SAD, 1 = v_sad_u8 x2
     int planes_rw;                // plane build: 1 = register-window kernel, 0 = LDS tile kernel
-    int mvnet_fused;              // network inference: 1 = one fused kernel per tile, 0 = layer by layer
+    int mvnet_fused;              // network inference: 2 = register-resident fused kernel, 1 = the LDS-tile fused
+                                  // kernel, 0 = layer by layer
     std::vector<hipEvent_t> prof; // RH264_PROF_EV events per profiled search
     std::vector<int> prof_np;     // phases recorded in each profiled search
     int prof_cap, prof_n;         // capacity and count of profiled searches

@@ -3935,7 +3935,7 @@ int rh264_init(int device, rh264_ctx **out)
     c->cand_lds = 1;
     c->fallback_v = 1;
     c->planes_rw = 1;
-    c->mvnet_fused = 1;
+    c->mvnet_fused = 2;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -4431,7 +4431,7 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
         return 0;
     }
     if (!strcmp(knob, "mvnet_fused")) {
-        ctx->mvnet_fused = value ? 1 : 0;
+        ctx->mvnet_fused = value < 0 ? 0 : value > 2 ? 2 : value;
         return 0;
     }
     return fail("rh264_tune: unknown knob '%s'", knob);

@@ -365,6 +365,39 @@ __global__ void k_refresh_bf16(const float *__restrict__ p, uint16_t *__restrict
     wb[(size_t)l * 9 * C * C + ((size_t)(8 - t) * C + ci) * C + co] = b;
 }
 
+// Register-order BF16 weights of the register-resident inference kernel: one record
+// of 64 lanes x 8 BF16 per MFMA A operand, read as one 16-byte load per lane.
+// Records: L1 (h = 0, 1), then layer l, tap t, half h at 2 + (l * 9 + t) * 2 + h, then
+// the head's bin tiles bt = 0..18.  Lane (q, col), slot s holds output channel (or bin)
+// 16h + col against input channel ci(q, s) = s < 4 ? 4q + s : 16 + 4q + s - 4, the
+// channel that slot s of lane q carries in a 16x16 MFMA result (rows 4q..4q+3 of the
+// two 16-channel halves); L1's K slots are (tap 4q + s / 2, channel s % 2), taps >= 9
+// zero.
+constexpr int WR_REC = 64 * 8;                 // BF16 per record
+constexpr int WR_LAYER = 2, WR_HEADREC = WR_LAYER + 3 * 9 * 2;
+constexpr int WR_COUNT = (WR_HEADREC + (K + 15) / 16) * WR_REC;
+
+__global__ void k_refresh_rr(const float *__restrict__ p, uint16_t *__restrict__ wr)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= WR_COUNT) return;
+    const int s = i & 7, lane = (i >> 3) & 63, rec = i / WR_REC;
+    const int q = lane >> 4, col = lane & 15;
+    const int ci = s < 4 ? 4 * q + s : 16 + 4 * q + (s - 4);
+    float v;
+    if (rec < WR_LAYER) {
+        const int co = 16 * rec + col, t = 4 * q + (s >> 1);
+        v = t < 9 ? p[OFF_W1 + co * 18 + 2 * t + (s & 1)] : 0.0f;
+    } else if (rec < WR_HEADREC) {
+        const int r = rec - WR_LAYER, l = r / 18, t = (r >> 1) % 9, co = 16 * (r & 1) + col;
+        v = p[OFF_W2 + l * (C * 9 * C + C) + (co * 9 + t) * C + ci];
+    } else {
+        const int bin = 16 * (rec - WR_HEADREC) + col;
+        v = bin < K ? p[OFF_WH + bin * C + ci] : 0.0f;
+    }
+    wr[i] = f2bf(v);
+}
+
 // Ranking of the proposals: each logit z of bin b gets the 32-bit key
 // (order(z) & ~511) | (511 - b), order() = the order-preserving unsigned image of
 // the FP32 bits.  The top-8 are the 8 largest keys, largest first: by logit with
@@ -746,6 +779,405 @@ __global__ __launch_bounds__(FW * 64) void k_fwd_fused(const uint16_t *__restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident fused inference (the default).  A workgroup of RNV waves takes a
+// tile of 32 x RH pixels at 1/8 resolution; wave w holds rows RS w .. RS w + RS - 1 of
+// it in registers, each row as two MFMA B operands: E = the even pixels 2c, O = the odd
+// pixels 2c + 1 (c = lane column).  Lane (q, c) of an operand carries channels
+// {4q..4q+3, 16+4q..16+4q+3} of its pixel -- exactly what the two 16x16x32 products of
+// that operand (output channels 0-15 and 16-31) return to the lane, so a layer's result
+// feeds the next layer with no data movement (the weights take the channel permutation
+// instead, k_refresh_rr).  Of a pixel's x-1 / x+1 neighbours, three of the four sit in
+// the same lane; the other two operands are one DPP row shift each (the tile's side
+// edges take zero).  A 3x3 layer walks its rows top-down with the input rows r-1, r,
+// r+1; the waves exchange their first and last rows through LDS once per layer, so
+// nothing inside a tile is computed twice; the tile's outer 4 pixels (3 layers of halo
+// + cell alignment) are, by the neighbouring tile.  The 2x2 pool is E + O of two rows,
+// inside the lane; the head takes the tile's cells 16 at a time from LDS, and its top-8
+// selection merges only the bin tiles holding a logit at or above a lower bound of each
+// cell's 8th largest (two logits of each of a cell's four lanes reach it).  Same
+// arithmetic as the other inference kernels: BF16 activations and weights, FP32
+// accumulation, zero padding at every layer's border.
+// ---------------------------------------------------------------------------
+constexpr int RNV = 4, RS = 14;                   // waves (strips stacked down the tile), rows per strip
+constexpr int RW = 32, RH = RS * RNV;             // computed tile (1/8 res)
+constexpr int RUW = RW - 8, RUH = RH - 8;         // useful output: 24 x 48 pixels = 12 x 24 cells
+constexpr int RCW = RUW / 2, RCH = RUH / 2;       // cells per tile row / column
+constexpr int RCELLS = RCW * RCH;
+constexpr int RXW = 2 * RW + 1, RXH = 2 * RH + 1; // x0 window (1/4 res)
+constexpr int RREC = 4 * 1024;                    // one strip's end rows: first E, first O, last E, last O
+constexpr int RCP = 80;                           // LDS bytes per pooled cell (64 + 16 pad)
+constexpr int RNBT = (K + 15) / 16;               // head bin tiles
+constexpr int RLDS_A = ((RXH * RXW * 4 > RCELLS * RCP ? RXH * RXW * 4 : RCELLS * RCP) + 15) & ~15;
+constexpr int RBIAS = 3 * RS * C * 4;              // layers 2-4 biases, one copy per strip row
+constexpr int RWL = 12 * 1024;                      // one layer's kernel rows 1 and 3 (taps 0-2, 6-8), both halves
+constexpr int RBH = RNBT * 16 * 4;                  // the head's biases (zero past bin 288)
+// region B: end rows (parity 0), a zero row, the biases, the weights of layers 2 and 4,
+// the head's biases; region A holds the x0 window, then (after layer 1) the end rows of
+// parity 1 with layer 3's weights behind them, then the pooled cells
+constexpr int RLDS_B = RNV * RREC + 1024 + RBIAS + 2 * RWL + RBH;
+static_assert(RNV * RREC + RWL <= RLDS_A, "region A holds the end rows of parity 1 and layer 3's weights");
+static_assert(RLDS_A % 16 == 0, "region B 16-byte aligned");
+static_assert(RNV * RREC <= RLDS_A, "the end rows of parity 1 fit region A");
+static_assert(RCELLS % 16 == 0 && RS % 2 == 0, "whole head fragments per tile, cells inside a strip");
+
+__device__ __forceinline__ u32x4 dpp_from_left0(u32x4 x) // lane c <- c - 1, lane 0 <- 0
+{
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (uint32_t)__builtin_amdgcn_mov_dpp((int)x[k], 0x111, 0xF, 0xF, true);
+    return o;
+}
+
+__device__ __forceinline__ u32x4 dpp_from_right0(u32x4 x) // lane c <- c + 1, lane 15 <- 0
+{
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (uint32_t)__builtin_amdgcn_mov_dpp((int)x[k], 0x101, 0xF, 0xF, true);
+    return o;
+}
+
+// bias already in the accumulators; ReLU and the image mask as one median (lim = +inf
+// inside the image, 0 outside); BF16 in the B-operand slot order
+__device__ __forceinline__ u32x4 rr_pack(float4_t a0, float4_t a1, float lim)
+{
+    u32x4 o;
+    o[0] = f2bf2(__builtin_amdgcn_fmed3f(a0[0], 0.0f, lim), __builtin_amdgcn_fmed3f(a0[1], 0.0f, lim));
+    o[1] = f2bf2(__builtin_amdgcn_fmed3f(a0[2], 0.0f, lim), __builtin_amdgcn_fmed3f(a0[3], 0.0f, lim));
+    o[2] = f2bf2(__builtin_amdgcn_fmed3f(a1[0], 0.0f, lim), __builtin_amdgcn_fmed3f(a1[1], 0.0f, lim));
+    o[3] = f2bf2(__builtin_amdgcn_fmed3f(a1[2], 0.0f, lim), __builtin_amdgcn_fmed3f(a1[3], 0.0f, lim));
+    return o;
+}
+
+// the 3 taps (weights w[0..2][h]) of one input row into the even (ODD = false) or odd
+// output pixels of a row: even pixel 2c reads odd pixel 2c - 1 (lane c - 1), even 2c,
+// odd 2c + 1; odd pixel 2c + 1 reads even 2c, odd 2c + 1, even 2c + 2 (lane c + 1)
+template <bool ODD>
+__device__ __forceinline__ void rr_row_taps(const u32x4 (&w)[3][2], u32x4 e, u32x4 o, float4_t (&a)[2])
+{
+    const u32x4 x0 = ODD ? e : dpp_from_left0(o), x1 = ODD ? o : e, x2 = ODD ? dpp_from_right0(e) : o;
+#pragma unroll
+    for (int h = 0; h < 2; h++) a[h] = mfma_bf16_k32(w[0][h], x0, a[h]);
+#pragma unroll
+    for (int h = 0; h < 2; h++) a[h] = mfma_bf16_k32(w[1][h], x1, a[h]);
+#pragma unroll
+    for (int h = 0; h < 2; h++) a[h] = mfma_bf16_k32(w[2][h], x2, a[h]);
+}
+
+// wr index of entry i (0..767) of layer l's LDS copy of kernel rows 1 and 3: record j =
+// i / 64 (j = 0..5: taps 0-2, j = 6..11: taps 6-8, the two halves interleaved), lane i % 64
+__device__ __forceinline__ int rr_wrow_rec(int l, int i)
+{
+    const int j = i >> 6, t = j < 6 ? j >> 1 : 6 + ((j - 6) >> 1);
+    return (WR_LAYER + (l * 9 + t) * 2 + (j & 1)) * 64 + (i & 63);
+}
+
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// The top-8 ranking keys (largest first) of this lane's 76 bins of one cell.  A key v
+// goes into the sorted list t by t'[0] = max(t[0], v), t'[k] = med3(t[k-1], t[k], v)
+// (eight independent operations).  rank_key's value in three: the order map as
+// z ^ (sign extension | 2^31), the bin's low bits inserted.
+__device__ __forceinline__ void rr_head_top8(const float4_t (&acc)[RNBT], int q, uint32_t (&top)[8])
+{
+#pragma unroll
+    for (int j = 0; j < 8; j++) top[j] = 0u;
+    const uint32_t lb = 511u - 4u * (uint32_t)q;
+#pragma unroll
+    for (int bt = 0; bt < RNBT; bt++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (16 * bt + 15 >= K && i > 0) continue; // the last tile holds one bin (288, lane q 0)
+            const int z = __float_as_int(acc[bt][i]);
+            const uint32_t o = (uint32_t)z ^ ((uint32_t)(z >> 31) | 0x80000000u);
+            uint32_t v = (o & ~511u) | (lb - (uint32_t)(16 * bt + i));
+            if (16 * bt + 15 >= K && q > 0) v = 0u; // past the last bin
+            uint32_t t2[8];
+            t2[0] = max(top[0], v);
+#pragma unroll
+            for (int k = 1; k < 8; k++) t2[k] = max(min(top[k - 1], top[k]), min(max(top[k - 1], top[k]), v));
+#pragma unroll
+            for (int k = 0; k < 8; k++) top[k] = t2[k];
+        }
+    }
+    topk_merge_lanes(top, 16);
+    topk_merge_lanes(top, 32);
+}
+
+__global__ __launch_bounds__(RNV * 64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_rr(
+    const uint32_t *__restrict__ x0, const float *__restrict__ p, const u32x4 *__restrict__ wr, int h4, int w4,
+    int tiles_x, int tiles_y, float *__restrict__ logits, int32_t *__restrict__ top8)
+{
+    __shared__ __align__(16) uint8_t s_mem[RLDS_A + RLDS_B];
+    uint8_t *s_A = s_mem, *s_B = s_mem + RLDS_A;
+    constexpr int NT = RNV * 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int q = lane >> 4, col = lane & 15;
+    const int tpi = tiles_x * tiles_y;
+    const int img = blockIdx.x / tpi, tr = blockIdx.x - img * tpi;
+    const int ty = tr / tiles_x, tx = tr - ty * tiles_x;
+    const int h8 = h4 / 2, w8 = w4 / 2, hc = h8 / 2, wc = w8 / 2;
+    const int X0 = tx * RUW - 4, Y0 = ty * RUH - 4; // tile origin (1/8 res)
+    const int sy = Y0 + RS * wave;                  // this strip's first row
+    const int pxe = X0 + 2 * col;                   // this lane's even pixel column
+    const float inf = __builtin_huge_valf();
+    const float limE = pxe >= 0 && pxe < w8 ? inf : 0.0f, limO = pxe + 1 >= 0 && pxe + 1 < w8 ? inf : 0.0f;
+    uint8_t *zrow = s_B + RNV * RREC;
+    float *s_bias = (float *)(zrow + 1024);
+    uint8_t *s_wB = (uint8_t *)(s_bias + 3 * RS * C), *s_wA = s_A + RNV * RREC;
+    float *s_bh = (float *)(s_wB + 2 * RWL);
+
+    // x0 window: 1/4-res rows 2 Y0 - 1 .. + RXH - 1, cols 2 X0 - 1 .. + RXW - 1; zero
+    // outside the image; the zero row of region B
+    uint32_t *s_x = (uint32_t *)s_A;
+    u32x4 w3v[12 * 64 / NT]; // layer 3's kernel rows 1 and 3, stored once the x0 window is dead
+    {
+        const uint32_t *xw = x0 + (size_t)img * h4 * w4;
+        const int gy0 = 2 * Y0 - 1, gx0 = 2 * X0 - 1;
+        // loads from clamped addresses, no branches: all of a thread's loads are in
+        // flight together, the words outside the image zeroed afterwards
+        constexpr int NX = (RXH * RXW + NT - 1) / NT;
+        uint32_t v[NX];
+#pragma unroll
+        for (int k = 0; k < NX; k++) {
+            const int i = tid + k * NT, r = i / RXW, c = i - r * RXW;
+            const int gy = min(max(gy0 + r, 0), h4 - 1), gx = min(max(gx0 + c, 0), w4 - 1);
+            v[k] = xw[(size_t)gy * w4 + gx];
+        }
+#pragma unroll
+        for (int k = 0; k < NX; k++) {
+            const int i = tid + k * NT, r = i / RXW, c = i - r * RXW, gy = gy0 + r, gx = gx0 + c;
+            if (i < RXH * RXW) s_x[i] = gy >= 0 && gy < h4 && gx >= 0 && gx < w4 ? v[k] : 0u;
+        }
+        for (int i = tid; i < 1024 / 16; i += NT) *(u32x4 *)(zrow + 16 * i) = u32x4{0, 0, 0, 0};
+        // the conv layers' biases, the head's biases, and layers 2 and 4's kernel rows 1
+        // and 3 (read from LDS by every output row, so they hold no registers between rows);
+        // every load of a thread is issued before the first store
+        constexpr int NB = (3 * RS * C + NT - 1) / NT, NH = (RNBT * 16 + NT - 1) / NT, NW = 2 * 12 * 64 / NT;
+        float vb[NB], vh[NH];
+        u32x4 vw[NW];
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            // one copy per strip row: each row reads its own, so the compiler does not keep
+            // them in registers across the layer (it has none spare)
+            const int i = min(tid + k * NT, 3 * RS * C - 1);
+            vb[k] = p[OFF_B2 + (i / (RS * C)) * (C * 9 * C + C) + i % C];
+        }
+#pragma unroll
+        for (int k = 0; k < NH; k++) vh[k] = p[OFF_BH + min(tid + k * NT, K - 1)];
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            const int i = tid + k * NT; // 0..767 layer 2, 768..1535 layer 4
+            vw[k] = wr[rr_wrow_rec(i < 768 ? 0 : 2, i % 768)];
+        }
+#pragma unroll
+        for (int k = 0; k < 12 * 64 / NT; k++) w3v[k] = wr[rr_wrow_rec(1, tid + k * NT)];
+#pragma unroll
+        for (int k = 0; k < NB; k++)
+            if (tid + k * NT < 3 * RS * C) s_bias[tid + k * NT] = vb[k];
+#pragma unroll
+        for (int k = 0; k < NH; k++)
+            if (tid + k * NT < RNBT * 16) s_bh[tid + k * NT] = tid + k * NT < K ? vh[k] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < NW; k++) *(u32x4 *)(s_wB + 16 * (tid + k * NT)) = vw[k];
+    }
+    __syncthreads();
+
+    u32x4 XE[RS], XO[RS];
+    // L1: 3x3 stride 2, 2 -> 32; K slots 8q + 2j + ch = (tap 4q + j, channel ch)
+    {
+        const u32x4 w1a = wr[lane], w1b = wr[64 + lane];
+        float4_t b0, b1;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            b0[i] = p[OFF_B1 + 4 * q + i];
+            b1[i] = p[OFF_B1 + 16 + 4 * q + i];
+        }
+        int toff[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int t = 4 * q + j;
+            toff[j] = t < 9 ? (t / 3) * RXW + t % 3 : 0;
+        }
+        // pixel x of the window row 2y: x0 columns 2x .. 2x + 2
+        const uint32_t *xb = s_x + 2 * (RS * wave) * RXW + 4 * col;
+#pragma unroll
+        for (int r = 0; r < RS; r++) {
+            u32x4 be, bo;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                // taps >= 9 read tap 0's word: their weights are zero and x0 is finite
+                be[j] = xb[2 * r * RXW + toff[j]];
+                bo[j] = xb[2 * r * RXW + 2 + toff[j]];
+            }
+            const float ry = (unsigned)(sy + r) < (unsigned)h8 ? inf : 0.0f;
+            const u32x4 oe = rr_pack(mfma_bf16_k32(w1a, be, b0), mfma_bf16_k32(w1b, be, b1), fminf(ry, limE));
+            const u32x4 oo = rr_pack(mfma_bf16_k32(w1a, bo, b0), mfma_bf16_k32(w1b, bo, b1), fminf(ry, limO));
+            if (r == 0 || r == RS - 1) { // the strip's end rows: into its record (parity 0)
+                *(u32x4 *)(s_B + RREC * wave + (r ? 2048 : 0) + 16 * lane) = oe;
+                *(u32x4 *)(s_B + RREC * wave + (r ? 3072 : 1024) + 16 * lane) = oo;
+            } else {
+                XE[r] = oe;
+                XO[r] = oo;
+            }
+        }
+    }
+
+    // A strip's first and last rows live in its record of the current parity, not in
+    // registers: first E, first O, last E, last O (lane-major, 1 KB each).  The neighbours
+    // read them from there, and so does the strip itself.  L1 wrote its records (parity 0)
+    // above; layer l writes its first and last output rows into parity (l + 1) & 1 as soon
+    // as it has them (every wave is past layer l - 1, that parity's last reader).
+    const bool hasT = wave > 0, hasB = wave < RNV - 1;
+    uint8_t *s_cell = s_A; // pooled cells: (ly, lx) of the tile at RCP bytes, written by layer 4
+    const int lx = col - 2;
+    const bool cx_ok = lx >= 0 && lx < RCW;
+    __syncthreads();
+    // every wave is past layer 1: the x0 window is dead, layer 3's weights go behind region
+    // A's end rows (visible to layer 3 through the barrier after layer 2)
+#pragma unroll
+    for (int k = 0; k < 12 * 64 / NT; k++) *(u32x4 *)(s_wA + 16 * (tid + k * NT)) = w3v[k];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        u32x4 wc[3][2]; // the centre kernel row (taps 3-5); rows 1 and 3 come from LDS per output row
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) wc[t][h] = wr[(WR_LAYER + (l * 9 + 3 + t) * 2 + h) * 64 + lane];
+        const int wl = (l == 1 ? s_wA : s_wB + (l / 2) * RWL) - s_mem; // this layer's kernel rows 1 and 3
+        const uint8_t *hb = (l & 1) ? s_A : s_B;       // this layer's input end rows
+        uint8_t *ob = ((l + 1) & 1 ? s_A : s_B) + RREC * wave; // this strip's output end rows
+        const uint8_t *own = hb + RREC * wave;
+        const uint8_t *above = hasT ? hb + RREC * (wave - 1) + 2048 : zrow;
+        u32x4 pE = *(const u32x4 *)(above + 16 * lane), pO = *(const u32x4 *)(above + (hasT ? 1024 : 0) + 16 * lane);
+        u32x4 cE = *(const u32x4 *)(own + 16 * lane), cO = *(const u32x4 *)(own + 1024 + 16 * lane);
+        u32x4 evE, evO; // layer 4: the even row of the pair being pooled
+#pragma unroll
+        for (int r = 0; r < RS; r++) {
+            u32x4 nE, nO;
+            if (r + 2 < RS) {
+                nE = XE[r + 1];
+                nO = XO[r + 1];
+            } else if (r + 2 == RS) {
+                nE = *(const u32x4 *)(own + 2048 + 16 * lane);
+                nO = *(const u32x4 *)(own + 3072 + 16 * lane);
+            } else {
+                const uint8_t *below = hasB ? hb + RREC * (wave + 1) : zrow;
+                nE = *(const u32x4 *)(below + 16 * lane);
+                nO = *(const u32x4 *)(below + (hasB ? 1024 : 0) + 16 * lane);
+            }
+            const float *br = s_bias + (l * RS + r) * C + 4 * q;
+            const float ry = (unsigned)(sy + r) < (unsigned)h8 ? inf : 0.0f;
+            // the first and third kernel rows' weights, read per output row through an
+            // address the compiler cannot prove equal to the other rows' (so it does not keep
+            // them live), each applied to the even and the odd outputs before the next is read
+            const uint8_t *wrow = s_mem + opaque_u32(wl + 16 * lane);
+            float4_t aE[2] = {*(const float4_t *)br, *(const float4_t *)(br + 16)};
+            float4_t aO[2] = {aE[0], aE[1]};
+            {
+                u32x4 w1[3][2];
+#pragma unroll
+                for (int t = 0; t < 3; t++)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) w1[t][h] = *(const u32x4 *)(wrow + (2 * t + h) * 1024);
+                rr_row_taps<false>(w1, pE, pO, aE);
+                rr_row_taps<true>(w1, pE, pO, aO);
+            }
+            rr_row_taps<false>(wc, cE, cO, aE);
+            rr_row_taps<true>(wc, cE, cO, aO);
+            {
+                u32x4 w3[3][2];
+#pragma unroll
+                for (int t = 0; t < 3; t++)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) w3[t][h] = *(const u32x4 *)(wrow + (6 + 2 * t + h) * 1024);
+                rr_row_taps<false>(w3, nE, nO, aE);
+                rr_row_taps<true>(w3, nE, nO, aO);
+            }
+            const u32x4 outE = rr_pack(aE[0], aE[1], fminf(ry, limE)), outO = rr_pack(aO[0], aO[1], fminf(ry, limO));
+            pE = cE;
+            pO = cO;
+            cE = nE;
+            cO = nO;
+            if (l < 2) {
+                if (r == 0 || r == RS - 1) {
+                    *(u32x4 *)(ob + (r ? 2048 : 0) + 16 * lane) = outE;
+                    *(u32x4 *)(ob + (r ? 3072 : 1024) + 16 * lane) = outO;
+                } else {
+                    XE[r] = outE;
+                    XO[r] = outO;
+                }
+            } else if ((r & 1) == 0) {
+                evE = outE;
+                evO = outO;
+            } else {
+                // layer 4: the 2x2 average pool of rows r - 1, r, straight into the cell
+                // buffer (every wave is past layer 3, the last reader of region A)
+                const int trow = RS * wave + r - 1, ly = (trow - 4) >> 1;
+                if (trow >= 4 && ly < RCH) { // wave-uniform
+                    u32x4 o;
+#pragma unroll
+                    for (int d = 0; d < 4; d++) {
+                        const uint32_t e0 = evE[d], o0 = evO[d], e1 = outE[d], o1 = outO[d];
+                        const float lo = ((__uint_as_float(e0 << 16) + __uint_as_float(o0 << 16)) +
+                                          __uint_as_float(e1 << 16)) +
+                                         __uint_as_float(o1 << 16);
+                        const float hi = ((__uint_as_float(e0 & 0xffff0000u) + __uint_as_float(o0 & 0xffff0000u)) +
+                                          __uint_as_float(e1 & 0xffff0000u)) +
+                                         __uint_as_float(o1 & 0xffff0000u);
+                        o[d] = f2bf2(0.25f * lo, 0.25f * hi);
+                    }
+                    if (cx_ok) *(u32x4 *)(s_cell + (ly * RCW + lx) * RCP + 16 * q) = o;
+                }
+            }
+            // rows are scheduled one at a time: moving a later row's loads up holds
+            // registers the layer does not have
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+    }
+    u32x4 hw[RNBT]; // the head's weights
+#pragma unroll
+    for (int bt = 0; bt < RNBT; bt++) hw[bt] = wr[(WR_HEADREC + bt) * 64 + lane];
+
+    // head + top-8, 16 cells per fragment (MFMA column = cell)
+#pragma unroll 1
+    for (int f = wave; f < RCELLS / 16; f += RNV) {
+        const int c = 16 * f + col, ly = c / RCW, lx = c - ly * RCW;
+        const int gcy = ty * RCH + ly, gcx = tx * RCW + lx;
+        if (ty * RCH + (16 * f) / RCW >= hc) break; // every later fragment lies below the image
+        const bool cvalid = gcy < hc && gcx < wc;
+        const size_t gcell = ((size_t)img * hc + (cvalid ? gcy : 0)) * wc + (cvalid ? gcx : 0);
+        u32x4 fv = *(const u32x4 *)(s_cell + c * RCP + 16 * q);
+        if (!cvalid) fv = u32x4{0, 0, 0, 0};
+        float4_t acc[RNBT];
+#pragma unroll
+        for (int bt = 0; bt < RNBT; bt++) acc[bt] = mfma_bf16_k32(hw[bt], fv, *(const float4_t *)(s_bh + 16 * bt + 4 * q));
+        if (logits && cvalid) {
+#pragma unroll
+            for (int bt = 0; bt < RNBT; bt++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int bin = 16 * bt + 4 * q + i;
+                    if (bin < K) logits[gcell * K + bin] = acc[bt][i];
+                }
+        }
+        uint32_t top[8];
+        rr_head_top8(acc, q, top);
+        if (top8 && cvalid && q == 0) {
+            int b8[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) b8[j] = 511 - (int)(top[j] & 511u);
+            *(int4 *)(top8 + gcell * TOPK) = make_int4(b8[0], b8[1], b8[2], b8[3]);
+            *(int4 *)(top8 + gcell * TOPK + 4) = make_int4(b8[4], b8[5], b8[6], b8[7]);
+        }
+    }
+}
+
 __global__ void k_fill_f32(float *p, size_t n, float v)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -760,6 +1192,7 @@ struct rh264_mvnet {
     int max_n, max_w, max_h;
     float *p, *g, *m, *v;   // FP32 parameters, gradients, Adam moments
     uint16_t *wf, *wb;      // BF16 conv weights (forward / data-gradient layouts)
+    uint16_t *wr;           // BF16 weights in register order (k_fwd_rr)
     uint16_t *x0, *a1, *a2, *a3, *a4, *d4, *d3, *d2, *d1;
     float *f, *df, *logits, *dlogits, *loss;
     long step;
@@ -772,7 +1205,7 @@ size_t rh264_mvnet_param_count(void) { return NPARAM; }
 int rh264_mvnet_destroy(rh264_mvnet *net)
 {
     if (!net) return 0;
-    void *bufs[] = {net->p, net->g, net->m, net->v, net->wf, net->wb, net->x0, net->a1, net->a2, net->a3, net->a4,
+    void *bufs[] = {net->p, net->g, net->m, net->v, net->wf, net->wb, net->wr, net->x0, net->a1, net->a2, net->a3, net->a4,
                     net->d4, net->d3, net->d2, net->d1, net->f, net->df, net->logits, net->dlogits, net->loss};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -798,6 +1231,7 @@ int rh264_mvnet_create(rh264_ctx *ctx, int max_n, int max_w, int max_h, rh264_mv
     } allocs[] = {
         {(void **)&net->p, NPARAM * 4}, {(void **)&net->g, NPARAM * 4}, {(void **)&net->m, NPARAM * 4},
         {(void **)&net->v, NPARAM * 4}, {(void **)&net->wf, WF_COUNT * 2}, {(void **)&net->wb, 3 * 9 * C * C * 2},
+        {(void **)&net->wr, WR_COUNT * 2},
         {(void **)&net->x0, n * px4 * 2 * 2}, {(void **)&net->a1, n * px8 * C * 2}, {(void **)&net->a2, n * px8 * C * 2},
         {(void **)&net->a3, n * px8 * C * 2}, {(void **)&net->a4, n * px8 * C * 2}, {(void **)&net->d4, n * px8 * C * 2},
         {(void **)&net->d3, n * px8 * C * 2}, {(void **)&net->d2, n * px8 * C * 2}, {(void **)&net->d1, n * px8 * C * 2},
@@ -824,6 +1258,7 @@ int rh264_mvnet_set_params(rh264_ctx *ctx, rh264_mvnet *net, const float *host, 
     HIP_TRY(hipMemcpyAsync(net->p, host, NPARAM * 4, hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(k_refresh_bf16, dim3(nblocks(WF_COUNT, 256)), dim3(256), 0, ctx->stream, net->p, net->wf,
                        net->wb);
+    hipLaunchKernelGGL(k_refresh_rr, dim3(nblocks(WR_COUNT, 256)), dim3(256), 0, ctx->stream, net->p, net->wr);
     LAUNCH_CHECK("k_refresh_bf16");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -881,6 +1316,18 @@ int mvnet_forward_ev(rh264_ctx *ctx, rh264_mvnet *net, const uint8_t *cur, const
     if (rc) return rc;
     if ((uintptr_t)top8 & 15) return fail("mvnet_forward: top8 must be 16-byte aligned");
     const long cells = (long)n * (h / 16) * (w / 16);
+    if (ctx->mvnet_fused == 2) {
+        const int h4 = h / 4, w4 = w / 4;
+        const int tiles_x = (w / 16 + RCW - 1) / RCW, tiles_y = (h / 16 + RCH - 1) / RCH;
+        hipLaunchKernelGGL(k_prep, dim3(nblocks((long)n * h4 * w4, 256)), dim3(256), 0, ctx->stream, cur, ref, stride,
+                           n, w, h, net->x0, zero);
+        if (after_prep) HIP_TRY(hipEventRecord(after_prep, ctx->stream));
+        hipLaunchKernelGGL(k_fwd_rr, dim3((unsigned)((long)n * tiles_x * tiles_y)), dim3(RNV * 64), 0, ctx->stream,
+                           (const uint32_t *)net->x0, net->p, (const u32x4 *)net->wr, h4, w4, tiles_x, tiles_y, logits,
+                           top8);
+        LAUNCH_CHECK("k_fwd_rr");
+        return 0;
+    }
     if (ctx->mvnet_fused) {
         const int h4 = h / 4, w4 = w / 4;
         const int tiles_x = (w / 16 + FT - 1) / FT, tiles_y = (h / 16 + FT - 1) / FT;
@@ -967,6 +1414,7 @@ int rh264_mvnet_adam(rh264_ctx *ctx, rh264_mvnet *net, float lr)
                        lr, b1, b2, eps, c1, c2);
     hipLaunchKernelGGL(k_refresh_bf16, dim3(nblocks(WF_COUNT, 256)), dim3(256), 0, ctx->stream, net->p, net->wf,
                        net->wb);
+    hipLaunchKernelGGL(k_refresh_rr, dim3(nblocks(WR_COUNT, 256)), dim3(256), 0, ctx->stream, net->p, net->wr);
     LAUNCH_CHECK("mvnet adam");
     return 0;
 }

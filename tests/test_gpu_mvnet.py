@@ -38,12 +38,12 @@ def _params(seed):
     return R.pack(P).astype(np.float32)
 
 
-@pytest.mark.parametrize("w,h", [(128, 96), (208, 160)])
-@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("w,h", [(128, 96), (208, 160), (400, 432)])
+@pytest.mark.parametrize("fused", [2, 1, 0])
 def test_forward_logits_and_top8(dev, fused, w, h):
-    """Both inference paths: the fused per-tile kernel (tiles of 8x8 cells; 208x160
-    leaves partial tiles on both axes) and the layer-by-layer kernels the training
-    step uses."""
+    """Every inference path: the register-resident fused kernel (tiles of 12x24 cells;
+    208x160 leaves partial tiles on both axes), the LDS-tile fused kernel (tiles of 8x8
+    cells) and the layer-by-layer kernels the training step uses."""
     n = 2
     cur, ref = _pair_batch(n, w, h, 10)
     p = _params(1)
@@ -59,7 +59,7 @@ def test_forward_logits_and_top8(dev, fused, w, h):
         top = dev.download(t8, (ncell, 8), np.int32)
         net.close()
     finally:
-        dev.tune("mvnet_fused", 1)
+        dev.tune("mvnet_fused", 2)
     want = R.forward(p, cur, ref, bf=True)["logits"].reshape(ncell, R.K)
     # BF16 activations: summation order can flip a rounding; |err| <= 0.02 + 2% of |logit|
     err = np.abs(got - want)
