@@ -335,15 +335,24 @@ __global__ __launch_bounds__(256) void k_build_planes_rw(const uint8_t *__restri
     uint8_t *out = planes + (size_t)tz * planes_bytes;
     const int x0 = (int)tx * PB_W, y0 = (int)ty * PB_H;
     const int tid = threadIdx.x;
-    for (int i = tid; i < PB_ROWS * PB_DW; i += 256) {
-        const int r = i / PB_DW, d = i - r * PB_DW;
-        const uint8_t *row = fr + (size_t)clampi(y0 - 2 + r - pad, 0, ch - 1) * cw;
-        const int sx = x0 - 4 + 4 * d - pad;
-        uint32_t v;
-        if (sx < 0) v = 0x01010101u * row[0];
-        else if (sx >= cw) v = 0x01010101u * row[cw - 1];
-        else v = *(const uint32_t *)(row + sx);
-        s_src[i] = v;
+    {
+        // every load of a thread is issued before its first store: a dword left of the
+        // frame is the first column's byte replicated (byte 0 of the dword at column 0),
+        // right of it the last column's (byte 3 of the dword at cw - 4)
+        constexpr int NS = (PB_ROWS * PB_DW + 255) / 256;
+        uint32_t v[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const int i = min(tid + 256 * k, PB_ROWS * PB_DW - 1), r = i / PB_DW, d = i - r * PB_DW;
+            const uint8_t *row = fr + (size_t)clampi(y0 - 2 + r - pad, 0, ch - 1) * cw;
+            v[k] = *(const uint32_t *)(row + clampi(x0 - 4 + 4 * d - pad, 0, cw - 4));
+        }
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const int i = tid + 256 * k, r = i / PB_DW, d = i - r * PB_DW, sx = x0 - 4 + 4 * d - pad;
+            const uint32_t w = sx < 0 ? 0x01010101u * (v[k] & 0xffu) : sx >= cw ? 0x01010101u * (v[k] >> 24) : v[k];
+            if (i < PB_ROWS * PB_DW) s_src[i] = w;
+        }
     }
     __syncthreads();
     const int c = tid & 63, g = tid >> 6;
