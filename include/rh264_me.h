@@ -583,6 +583,38 @@ int rh264_mbtree_propagate_multi_i32(rh264_ctx *ctx, int nframes, int mb_w, int 
 int rh264_rc_complexity(rh264_ctx *ctx, const uint8_t *cur, const uint8_t *ref, size_t frame_stride, int nframes,
                         int cw, int ch, double *out);
 
+/* ---- The bitstream of decided frames (host code, no GPU) ----
+ * H.264 Annex-B bytes in the reference's CAVLC configuration (Constrained Baseline,
+ * profile 66 with constraint_set1, CAVLC, one slice per picture, deblocking signalled
+ * on; params.rs:31-172, slice.rs:49-107, nal.rs:78-117).  Each entry writes into out
+ * (cap bytes), sets *len to the bytes the unit needs and returns 0, or returns < 0
+ * when they do not fit (and for malformed arguments).
+ * rh264_write_headers: SPS + PPS (Sps/Pps::write_rbsp, params.rs:56-101, 148-172):
+ *   mb grid ceil(width/16) x ceil(height/16) with cropping, max_num_ref_frames and the
+ *   PPS default active count num_ref_frames, pic_init_qp qp.
+ * rh264_write_idr_pcm: the IDR picture (write_idr_slice_header, slice.rs:87-107) of
+ *   I_PCM macroblocks: y is (16 mb_w) x (16 mb_h) bytes, u and v (8 mb_w) x (8 mb_h).
+ * rh264_write_p_slice: a P picture (write_p_slice_header, slice.rs:51-85;
+ *   frame_num and poc_lsb are written mod 16; nrefs active references, signalled when
+ *   num_ref_frames > 1) from rh264_partition_decision's records (14 int32 per
+ *   macroblock: mode 0..3, cost, ref/mvx/mvy of the TL, TR, BL, BR quadrants; the
+ *   16x16 decision's (ref, mv) is mode 0 with four equal quadrants).  No residual is
+ *   coded: a 16x16 reference-0 macroblock at its P_Skip vector (8.4.1.1) is skipped,
+ *   every other is P_L0_16x16 / P_L0_L0_16x8 / P_L0_L0_8x16 / P_8x8 (four P_L0_8x8)
+ *   with ref_idx (mb16.rs:5957-5964) and mvd against the clause 8.4.1.3 predictors,
+ *   coded_block_pattern 0 (mb16.rs:4416-4435; cavlc.rs:136-138). */
+typedef struct rh264_seq {
+    int32_t width, height;  /* luma size; the macroblock grid rounds up */
+    int32_t qp;             /* pic_init_qp; every slice has slice_qp_delta 0 */
+    int32_t num_ref_frames; /* 1..16 */
+    int32_t level_idc;      /* 30 is the reference's default (config.rs:463) */
+} rh264_seq;
+int rh264_write_headers(const rh264_seq *s, uint8_t *out, size_t cap, size_t *len);
+int rh264_write_idr_pcm(const rh264_seq *s, const uint8_t *y, const uint8_t *u, const uint8_t *v, uint8_t *out,
+                        size_t cap, size_t *len);
+int rh264_write_p_slice(const rh264_seq *s, int frame_num, int poc_lsb, int nrefs, const int32_t *rec, uint8_t *out,
+                        size_t cap, size_t *len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,8 +42,18 @@ class Stream(C.Structure):
                 ("ch", C.c_int32), ("nrefs", C.c_int32), ("out", C.c_void_p)]
 
 
+class Seq(C.Structure):
+    """rh264_seq (include/rh264_me.h)."""
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("qp", C.c_int32), ("num_ref_frames", C.c_int32),
+                ("level_idc", C.c_int32)]
+
+
 # name -> (restype, argtypes); mirrors include/rh264_me.h one to one
 PROTOTYPES = {
+    "rh264_write_headers": (C.c_int, [C.POINTER(Seq), c_vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "rh264_write_idr_pcm": (C.c_int, [C.POINTER(Seq), c_vp, c_vp, c_vp, c_vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "rh264_write_p_slice": (C.c_int, [C.POINTER(Seq), C.c_int, C.c_int, C.c_int, c_vp, c_vp, C.c_size_t,
+                                      C.POINTER(C.c_size_t)]),
     "rh264_init": (C.c_int, [C.c_int, C.POINTER(c_vp)]),
     "rh264_destroy": (C.c_int, [c_vp]),
     "rh264_last_error": (C.c_char_p, []),

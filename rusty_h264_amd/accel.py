@@ -550,6 +550,55 @@ def mbtree_qp_offsets(lib, intra, inter, mv, strength=0.9, res_min=0.03, sd_min=
     return off, {"residual_frac": float(st[0]), "spread": float(st[1]), "eff_strength": float(st[2])}
 
 
+def seq(width, height, qp=26, num_ref_frames=1, level_idc=30):
+    """rh264_seq: the sequence the bitstream entries describe (the reference's CAVLC
+    configuration, params.rs:31-172)."""
+    return _lib.Seq(width, height, qp, num_ref_frames, level_idc)
+
+
+def _write(fn, what, *args):
+    n = C.c_size_t()
+    fn(*args, None, 0, C.byref(n))  # size query (fails with the needed size set)
+    buf = np.empty(max(int(n.value), 1), np.uint8)
+    _check(fn(*args, buf.ctypes.data, buf.size, C.byref(n)), what)
+    return bytes(buf[:n.value])
+
+
+def write_headers(lib, s):
+    """SPS + PPS as Annex-B bytes (rh264_write_headers; Sps/Pps::write_rbsp,
+    params.rs:56-101, 148-172)."""
+    return _write(lib.rh264_write_headers, "rh264_write_headers", C.byref(s))
+
+
+def write_idr_pcm(lib, s, y, u, v):
+    """The IDR picture of I_PCM macroblocks from host planes: y (16 mb_h) x (16 mb_w),
+    u and v half size (rh264_write_idr_pcm)."""
+    y, u, v = (np.ascontiguousarray(a, np.uint8) for a in (y, u, v))
+    return _write(lib.rh264_write_idr_pcm, "rh264_write_idr_pcm", C.byref(s), y.ctypes.data, u.ctypes.data,
+                  v.ctypes.data)
+
+
+def write_p_slice(lib, s, frame_num, poc_lsb, nrefs, rec):
+    """A P picture from decision records ([mb] x 14 int32, rh264_partition_decision's
+    layout) as Annex-B bytes (rh264_write_p_slice)."""
+    rec = np.ascontiguousarray(rec, np.int32)
+    return _write(lib.rh264_write_p_slice, "rh264_write_p_slice", C.byref(s), int(frame_num), int(poc_lsb),
+                  int(nrefs), rec.ctypes.data)
+
+
+def records_16x16(ref, mv):
+    """The 16x16 decision's (ref [mb], mv [mb][2]) as partition-decision records (mode
+    0, cost 0, four equal quadrants)."""
+    ref = np.asarray(ref, np.int32).reshape(-1)
+    mv = np.asarray(mv, np.int32).reshape(-1, 2)
+    rec = np.zeros((ref.size, 14), np.int32)
+    for q in range(4):
+        rec[:, 2 + 3 * q] = ref
+        rec[:, 3 + 3 * q] = mv[:, 0]
+        rec[:, 4 + 3 * q] = mv[:, 1]
+    return rec
+
+
 def rc_complexity(dev, cur, ref, nframes, cw, ch, frame_stride=None, out=None):
     """lookahead::complexity of nframes device-resident frames (ref None = IDR); returns
     the nframes scores as float64 (lookahead.rs:38-101)."""

@@ -20,6 +20,7 @@ SOURCES = {
     "rh264_me.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "rh264_mvnet.hip": [],
     "rh264_accel.hip": [],
+    "rh264_bitstream.hip": [],
 }
 
 

@@ -55,6 +55,7 @@ _LAYOUTS = {
     "rh264_search_params": (_lib.SearchParams, {}),
     "rh264_diamond": (_lib.Diamond, {"lam": "lambda", "lam_fp": "lambda_fp"}),
     "rh264_stream": (_lib.Stream, {}),
+    "rh264_seq": (_lib.Seq, {}),
 }
 
 
