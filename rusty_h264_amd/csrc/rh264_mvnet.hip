@@ -837,15 +837,26 @@ __device__ __forceinline__ u32x4 dpp_from_right0(u32x4 x) // lane c <- c + 1, la
     return o;
 }
 
+// a value the compiler can neither see through nor move across the other empty asm
+// statements, barriers and scheduling barriers
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // bias already in the accumulators; ReLU and the image mask as one median (lim = +inf
-// inside the image, 0 outside); BF16 in the B-operand slot order
+// inside the image, 0 outside); BF16 in the B-operand slot order.  The packed words are
+// made opaque where they are produced: otherwise the compiler sinks the packing into the
+// next layer (the only user of most rows), past the barrier, and holds two FP32 values
+// per word through the layer, which spills.
 __device__ __forceinline__ u32x4 rr_pack(float4_t a0, float4_t a1, float lim)
 {
     u32x4 o;
-    o[0] = f2bf2(__builtin_amdgcn_fmed3f(a0[0], 0.0f, lim), __builtin_amdgcn_fmed3f(a0[1], 0.0f, lim));
-    o[1] = f2bf2(__builtin_amdgcn_fmed3f(a0[2], 0.0f, lim), __builtin_amdgcn_fmed3f(a0[3], 0.0f, lim));
-    o[2] = f2bf2(__builtin_amdgcn_fmed3f(a1[0], 0.0f, lim), __builtin_amdgcn_fmed3f(a1[1], 0.0f, lim));
-    o[3] = f2bf2(__builtin_amdgcn_fmed3f(a1[2], 0.0f, lim), __builtin_amdgcn_fmed3f(a1[3], 0.0f, lim));
+    o[0] = opaque_u32(f2bf2(__builtin_amdgcn_fmed3f(a0[0], 0.0f, lim), __builtin_amdgcn_fmed3f(a0[1], 0.0f, lim)));
+    o[1] = opaque_u32(f2bf2(__builtin_amdgcn_fmed3f(a0[2], 0.0f, lim), __builtin_amdgcn_fmed3f(a0[3], 0.0f, lim)));
+    o[2] = opaque_u32(f2bf2(__builtin_amdgcn_fmed3f(a1[0], 0.0f, lim), __builtin_amdgcn_fmed3f(a1[1], 0.0f, lim)));
+    o[3] = opaque_u32(f2bf2(__builtin_amdgcn_fmed3f(a1[2], 0.0f, lim), __builtin_amdgcn_fmed3f(a1[3], 0.0f, lim)));
     return o;
 }
 
@@ -872,11 +883,6 @@ __device__ __forceinline__ int rr_wrow_rec(int l, int i)
     return (WR_LAYER + (l * 9 + t) * 2 + (j & 1)) * 64 + (i & 63);
 }
 
-__device__ __forceinline__ uint32_t opaque_u32(uint32_t v)
-{
-    asm volatile("" : "+v"(v));
-    return v;
-}
 
 // The top-8 ranking keys (largest first) of this lane's 76 bins of one cell.  A key v
 // goes into the sorted list t by t'[0] = max(t[0], v), t'[k] = med3(t[k-1], t[k], v)
