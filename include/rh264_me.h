@@ -268,7 +268,14 @@ int rh264_profile_times(rh264_ctx *ctx, int i, float *ms, int cap, int *nevents)
  * lane-per-candidate kernel reads each block's source rows from LDS where it uses
  * them, 0 = it holds them in registers;
  * "fallback_v" 1 (default) = its fall-back full search uses the v_sad_u8 x2
- * variant, 2 = v_sad_u8 with one candidate per lane, 0 = quad-SADs; "planes_rw" 1 (default) = the plane build slides a
+ * variant, 2 = v_sad_u8 with one candidate per lane, 0 = quad-SADs; "fallback_wgs"
+ * (1..65536, default 2048) = that search's grid cap in workgroups (they loop over a
+ * longer list); "fallback_overlap" 1 = with the +-2 quarter-pel window refinement,
+ * the fall-back search runs on a second stream of the context beside the refinement
+ * of the other blocks and its blocks are refined after it (the context's stream
+ * waits for it), 0 (default) = it runs on the context's stream before one
+ * refinement of every block (both kernels are throughput-bound: side by side they
+ * take longer than in turn); "planes_rw" 1 (default) = the plane build slides a
  * register window down 256x32 tiles, 0 = the LDS-tile kernel; "mvnet_fused" 2
  * (default) = network inference in one fused kernel per tile of 12x24 MBs holding the
  * activations in registers, 1 = the fused kernel with the activations in LDS (tiles of

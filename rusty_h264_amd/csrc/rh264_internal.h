@@ -29,6 +29,8 @@ constexpr int RH264_PROF_EV = 6; // phase boundaries per profiled search
 struct rh264_ctx {
     int device;
     hipStream_t stream;
+    hipStream_t side;             // second stream (default priority): the learned search's fall-back
+    hipEvent_t fork_ev, join_ev;  // stream -> side after the candidates, side -> stream after the fall-back
     hipEvent_t ev[16];
     uint64_t *keys;               // per-block argmin keys of the full search
     uint64_t *noalloc_keys;       // this device's static key array (rh264_plain_search_noalloc)
@@ -41,6 +43,8 @@ struct rh264_ctx {
     int cand_quad;                // learned search: candidate windows with quad-SADs (local range <= 3)
     int cand_lds;                 // learned search: the lane-per-candidate kernel reads the source rows from LDS
     int fallback_v;               // learned search: fall-back full search, 0 = quad-SAD, 1 = v_sad_u8 x2
+    int fallback_wgs;             // learned search: the fall-back full search's grid cap (workgroups)
+    int fallback_overlap;         // learned search: the fall-back search runs beside the other blocks' refinement
     int planes_rw;                // plane build: 1 = register-window kernel, 0 = LDS tile kernel
     int mvnet_fused;              // network inference: 2 = register-resident fused kernel, 1 = the LDS-tile fused
                                   // kernel, 0 = layer by layer

@@ -1682,7 +1682,7 @@ __device__ __forceinline__ void wn16_block(const rh264_search_params &p, const i
 // The +-2 quarter-pel window refinement of every 16x16 block (one wave each), from
 // the integer search's keys or from given starts; with fflag, blocks flagged in it
 // start from their fall-back search's key in fkeys (centre 0, radius p.range)
-// instead.
+// instead, or (fskip) are left to k_refine_wn16_list.
 template <bool T8>
 __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, const uint8_t *__restrict__ cur,
                                                      size_t cur_stride, const uint8_t *__restrict__ planes,
@@ -1693,7 +1693,7 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
                                                      int32_t *__restrict__ int_cost, int32_t *__restrict__ out_mv,
                                                      int32_t *__restrict__ out_cost, int32_t *__restrict__ out_evals,
                                                      const int32_t *__restrict__ fflag,
-                                                     const uint64_t *__restrict__ fkeys)
+                                                     const uint64_t *__restrict__ fkeys, int fskip)
 {
     __shared__ int32_t s_lr[RH264_LR_N];
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
@@ -1725,6 +1725,7 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
             if (int_cost) int_cost[boff] = (int32_t)(key >> 32);
         }
     } else if (fflag && __builtin_amdgcn_readfirstlane(fflag[boff])) {
+        if (fskip) return; // wave-uniform, no barrier follows
         // a fall-back block starts at its zero-centred full search's vector
         const uint32_t idx = __builtin_amdgcn_readfirstlane((uint32_t)fkeys[boff]);
         const int NC = 2 * p.range + 1;
@@ -1736,6 +1737,34 @@ __global__ __launch_bounds__(256) void k_refine_wn16(rh264_search_params p, cons
     }
     wn16_block<T8>(p, s_lr, cur + (size_t)frame * cur_stride, planes + (size_t)frame * planes_stride, lx, ly, bx, by,
                    mpx, mpy, lane, boff, out_mv, out_cost, out_evals);
+}
+
+// The same refinement of the listed blocks only (frame-major global block indices, no
+// rate predictor), each from its fall-back search's key in fkeys; the waves of a capped
+// grid loop over the list, whose length is read here.
+template <bool T8>
+__global__ __launch_bounds__(256) void k_refine_wn16_list(rh264_search_params p, const uint8_t *__restrict__ cur,
+                                                          size_t cur_stride, const uint8_t *__restrict__ planes,
+                                                          size_t planes_stride, const int32_t *__restrict__ list,
+                                                          const unsigned *__restrict__ count,
+                                                          const uint64_t *__restrict__ fkeys,
+                                                          int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost)
+{
+    __shared__ int32_t s_lr[RH264_LR_N];
+    const long n = (long)*count;
+    if ((long)blockIdx.x * 4 >= n) return;
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
+    __syncthreads();
+    const int nbx = p.cw >> 4, nblk = nbx * (p.ch >> 4), NC = 2 * p.range + 1;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    for (long e = (long)blockIdx.x * 4 + wave; e < n; e += (long)gridDim.x * 4) {
+        const int gb = __builtin_amdgcn_readfirstlane(list[e]);
+        const int frame = gb / nblk, b = gb - frame * nblk;
+        const uint32_t idx = __builtin_amdgcn_readfirstlane((uint32_t)fkeys[gb]);
+        const int bx = 4 * ((int)(idx % (uint32_t)NC) - p.range), by = 4 * ((int)(idx / (uint32_t)NC) - p.range);
+        wn16_block<T8>(p, s_lr, cur + (size_t)frame * cur_stride, planes + (size_t)frame * planes_stride,
+                       (b % nbx) << 4, (b / nbx) << 4, bx, by, 0, 0, lane, (size_t)gb, out_mv, out_cost, nullptr);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3856,15 +3885,16 @@ void k_full_search_list(rh264_search_params p, const uint8_t *__restrict__ cur, 
                         uint64_t *__restrict__ keys)
 {
     __shared__ int32_t s_lr[RH264_LR_N];
-    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
-    __syncthreads();
     const int NG = fs_groups<V>(2 * p.range + 1);
     const long n = (long)*count;
+    // the list's length is known only here: workgroups with no chunk leave at once, the
+    // others take chunks blockIdx.x, + gridDim.x, ... (the grid is capped, not sized
+    // for a list of every block)
+    if ((long)blockIdx.x * 256 >= n * NG) return;
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr_fp[threadIdx.x];
+    __syncthreads();
     const int nbx = p.cw / BW, nblk = nbx * (p.ch / BH);
-    // the grid covers every block; workgroups past the list's end (known only here) leave
-    {
-        const long chunk = blockIdx.x;
-        if (chunk * 256 >= n * NG) return;
+    for (long chunk = blockIdx.x; chunk * 256 < n * NG; chunk += gridDim.x) {
         const long gi = chunk * 256 + threadIdx.x;
         const long e = gi / NG;
         const bool active = e < n;
@@ -3943,6 +3973,8 @@ int rh264_init(int device, rh264_ctx **out)
     c->cand_quad = 2;
     c->cand_lds = 1;
     c->fallback_v = 1;
+    c->fallback_wgs = 256 * 8;
+    c->fallback_overlap = 0;
     c->planes_rw = 1;
     c->mvnet_fused = 2;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -3950,12 +3982,21 @@ int rh264_init(int device, rh264_ctx **out)
         delete c;
         return fail("hipStreamCreate: %s", hipGetErrorString(e));
     }
+    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail("hipStreamCreate: %s", hipGetErrorString(e));
+    }
     for (int i = 0; i < 16; i++) HIP_TRY(hipEventCreate(&c->ev[i]));
+    HIP_TRY(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
     // this device's copy of the no-allocation scratch (its address differs per device)
     void *sp = nullptr;
     e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
     if (e != hipSuccess || !sp) {
         (void)hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->side);
         delete c;
         return fail("hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
     }
@@ -3969,6 +4010,7 @@ int rh264_destroy(rh264_ctx *ctx)
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->side);
     if (ctx->keys) (void)hipFree(ctx->keys);
     if (ctx->tmp_mv) (void)hipFree(ctx->tmp_mv);
     if (ctx->ls_buf) (void)hipFree(ctx->ls_buf);
@@ -3977,7 +4019,10 @@ int rh264_destroy(rh264_ctx *ctx)
     if (ctx->ms_buf) (void)hipFree(ctx->ms_buf);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
     for (hipEvent_t e : ctx->prof) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(ctx->fork_ev);
+    (void)hipEventDestroy(ctx->join_ev);
     (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->side);
     delete ctx;
     return 0;
 }
@@ -4146,7 +4191,7 @@ static int launch_refine(rh264_ctx *ctx, const rh264_search_params *p, int nfram
                          const uint64_t *keys, const int32_t *centres, const int32_t *start_mv,
                          const int32_t *mvps, int32_t *int_mv, int32_t *int_cost, int32_t *mv,
                          int32_t *cost, int32_t *evals, const int32_t *fflag = nullptr,
-                         const uint64_t *fkeys = nullptr)
+                         const uint64_t *fkeys = nullptr, int fskip = 0)
 {
     constexpr int GPW = RefShape<BW, BH>::GPW;
     const int nblk = (p->cw / BW) * (p->ch / BH);
@@ -4170,11 +4215,11 @@ static int launch_refine(rh264_ctx *ctx, const rh264_search_params *p, int nfram
                 if (t8)
                     hipLaunchKernelGGL(k_refine_wn16<true>, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
                                        planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv,
-                                       cost, evals, fflag, fkeys);
+                                       cost, evals, fflag, fkeys, fskip);
                 else
                     hipLaunchKernelGGL(k_refine_wn16<false>, grid, dim3(256), 0, ctx->stream, *p, cur, cur_stride,
                                        planes, planes_stride, keys, centres, start_mv, mvps, int_mv, int_cost, mv,
-                                       cost, evals, fflag, fkeys);
+                                       cost, evals, fflag, fkeys, fskip);
                 LAUNCH_CHECK("k_refine_wn16");
                 return 0;
             }
@@ -4428,6 +4473,16 @@ int rh264_tune(rh264_ctx *ctx, const char *knob, int value)
     if (!strcmp(knob, "fallback_v")) {
         if (value < 0 || value > 2) return fail("fallback_v %d: 0, 1 or 2", value);
         ctx->fallback_v = value;
+        return 0;
+    }
+    if (!strcmp(knob, "fallback_wgs")) {
+        if (value < 1 || value > 65536) return fail("fallback_wgs %d: 1..65536", value);
+        ctx->fallback_wgs = value;
+        return 0;
+    }
+    if (!strcmp(knob, "fallback_overlap")) {
+        if (value < 0 || value > 1) return fail("fallback_overlap %d: 0 or 1", value);
+        ctx->fallback_overlap = value;
         return 0;
     }
     if (!strcmp(knob, "cand_quad")) {
@@ -5580,26 +5635,56 @@ int rh264_learned_search(rh264_ctx *ctx, const rh264_search_params *p, rh264_mvn
                            centres, nb, local_range, fallback_cost, start, fallback, count, list);
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], ctx->stream));
+    // the window refinement reads the fall-back blocks' starts from their keys itself;
+    // the other refinements take them merged into `start`
+    const bool wn = p->subpel_pat == 4 && (ctx->refine_mfma || p->hadamard == 8);
+    // with the window refinement, the fall-back search runs on the side stream beside
+    // the refinement of the other blocks; its own blocks are refined after the join
+    const bool ovl = wn && ctx->fallback_overlap;
+    hipStream_t fs = ctx->stream;
+    if (ovl) {
+        HIP_TRY(hipEventRecord(ctx->fork_ev, ctx->stream));
+        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+        fs = ctx->side;
+    }
     // the fall-back list is short (a few % of the blocks): the v_sad_u8 variant with two
     // candidates per lane gives it twice the lanes of the quad-SAD one
     {
         const int V = ctx->fallback_v == 2 ? 3 : ctx->fallback_v;
         const long lanes = nb * (V == 3 ? fs_groups<3>(2 * p->range + 1)
                                         : V == 1 ? fs_groups<1>(2 * p->range + 1) : fs_groups<0>(2 * p->range + 1));
-        const dim3 grid((unsigned)((lanes + 255) / 256));
+        // at most fallback_wgs workgroups (8 per CU by default), which loop when the list
+        // is longer than they cover
+        const dim3 grid((unsigned)std::min<long>((lanes + 255) / 256, ctx->fallback_wgs));
         if (V == 3)
-            hipLaunchKernelGGL((k_full_search_list<16, 16, 3>), grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride,
+            hipLaunchKernelGGL((k_full_search_list<16, 16, 3>), grid, dim3(256), 0, fs, *p, cur, frame_stride,
                                planes, planes_stride, list, count, (const int32_t *)nullptr, keysf);
         else if (V == 1)
-            hipLaunchKernelGGL((k_full_search_list<16, 16, 1>), grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride,
+            hipLaunchKernelGGL((k_full_search_list<16, 16, 1>), grid, dim3(256), 0, fs, *p, cur, frame_stride,
                                planes, planes_stride, list, count, (const int32_t *)nullptr, keysf);
         else
-            hipLaunchKernelGGL((k_full_search_list<16, 16, 0>), grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride,
+            hipLaunchKernelGGL((k_full_search_list<16, 16, 0>), grid, dim3(256), 0, fs, *p, cur, frame_stride,
                                planes, planes_stride, list, count, (const int32_t *)nullptr, keysf);
     }
-    // the window refinement reads the fall-back blocks' starts from their keys itself;
-    // the other refinements take them merged into `start`
-    const bool wn = p->subpel_pat == 4 && (ctx->refine_mfma || p->hadamard == 8);
+    LAUNCH_CHECK("k_full_search_list");
+    if (ovl) {
+        if (ev) HIP_TRY(hipEventRecord(ev[4], ctx->side));
+        HIP_TRY(hipEventRecord(ctx->join_ev, ctx->side));
+        rc = launch_refine<16, 16>(ctx, p, nframes, cur, frame_stride, planes, planes_stride, nullptr, nullptr, start,
+                                   nullptr, nullptr, nullptr, mv, cost, nullptr, fallback, keysf, 1);
+        if (rc) return rc;
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
+        const dim3 grid((unsigned)std::min<long>((nb + 3) / 4, ctx->fallback_wgs));
+        if (p->hadamard == 8)
+            hipLaunchKernelGGL(k_refine_wn16_list<true>, grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride, planes,
+                               planes_stride, list, count, keysf, mv, cost);
+        else
+            hipLaunchKernelGGL(k_refine_wn16_list<false>, grid, dim3(256), 0, ctx->stream, *p, cur, frame_stride,
+                               planes, planes_stride, list, count, keysf, mv, cost);
+        LAUNCH_CHECK("k_refine_wn16_list");
+        if (ev) HIP_TRY(hipEventRecord(ev[5], ctx->stream));
+        return 0;
+    }
     if (!wn) {
         hipLaunchKernelGGL(k_fallback_merge, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, list,
                            count, keysf, p->range, start);

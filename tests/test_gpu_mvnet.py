@@ -116,6 +116,32 @@ def test_learned_search_recosting_is_exact(dev, Rl, cand_quad, fallback_v, pat):
         dev.tune("fallback_v", 1)
 
 
+@pytest.mark.parametrize("fallback_v,wgs,Rf", [(1, 1, 16), (2, 3, 16), (1, 2, 32), (0, 2, 16)])
+def test_learned_search_fallback_grid_loops(dev, fallback_v, wgs, Rf):
+    """fallback_wgs caps the fall-back search's grid: with 1-3 workgroups each one
+    loops over several parts of the list (half the blocks at the median threshold);
+    the results stay the oracle's."""
+    dev.tune("fallback_v", fallback_v)
+    dev.tune("fallback_wgs", wgs)
+    try:
+        _learned_recost_case(dev, 2, 4, Rf=Rf)
+    finally:
+        dev.tune("fallback_v", 1)
+        dev.tune("fallback_wgs", 2048)
+
+
+def test_learned_search_fallback_beside_refinement(dev):
+    """fallback_overlap 1: the fall-back search runs on the context's second stream
+    beside the refinement of the other blocks, and its own blocks are refined after
+    the join (by default it runs on the context's stream before one refinement of
+    every block); the same (oracle) results."""
+    dev.tune("fallback_overlap", 1)
+    try:
+        _learned_recost_case(dev, 2, 4)
+    finally:
+        dev.tune("fallback_overlap", 0)
+
+
 def test_learned_search_source_in_registers(dev):
     """cand_lds 0: the lane-per-candidate kernel holds the source rows in registers
     instead of reading them from LDS; the same (oracle) results."""
@@ -138,8 +164,8 @@ def test_learned_search_windows_leave_the_planes(dev, fallback_v):
         dev.tune("fallback_v", 1)
 
 
-def _learned_recost_case(dev, Rl, pat, pad=32):
-    cw, ch, Rf = 176, 144, 16
+def _learned_recost_case(dev, Rl, pat, pad=32, Rf=16):
+    cw, ch = 176, 144
     frames = synth.hd_clip(3, seed=31, w=cw, h=ch, nrects=3)
     nf = 2
     clip = dev.upload(frames)
