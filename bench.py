@@ -41,7 +41,14 @@ C_NET, K_NET = 32, 289     # the proposal network's channels and MV bins
 # recorded in profiles/motion_search_1080p_pmc.md
 TRAFFIC_SOURCE = "profiles/motion_search_1080p_pmc.md"
 KERNEL_TRAFFIC = {
-    "k_full_search<16,16>": (33046 * 2 + 1912) * 1024,
+    "k_full_search<16,16>": (33054 * 2 + 1912) * 1024,
+    "k_prep": (30624 * 2 + 7650) * 1024,
+    "k_fwd_rr": (8864 * 2 + 3825) * 1024,
+    "k_cand_search_m<2,true>": (34148 * 2 + 1531) * 1024,
+    "k_full_search_list<16,16,1>": (12630 * 2 + 5090) * 1024,
+    "k_refine_wn16<false>": (80170 * 2 + 1434) * 1024,
+    "k_refine_mx16<0,false>": (80386 * 2 + 2152) * 1024,
+    "k_build_planes_rw": (15438 * 2 + 134003) * 1024,
 }
 HBM_PEAK_TBS = HBM_PEAK_GBS / 1000.0
 # learned-seeded 4K lookahead: +-2 half-res pels around each proposal; fall-back SAD
@@ -198,7 +205,7 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
     for name, kernel, work, ms, peak, unit, bound, wunit in (
             ("input preparation", "k_prep", prep_bytes, ph_mean[0], HBM_PEAK_TBS, "TB/s", "hbm", "bytes"),
             ("network", "k_fwd_rr", net_flop, ph_mean[1], BF16_PEAK_TFLOPS, "TFLOP/s", "mfma", "FLOP"),
-            ("candidates", "k_cand_search_m<2>", cand_ops, ph_mean[2], VALU_PEAK_TOPS, "TOP/s", "valu",
+            ("candidates", "k_cand_search_m<2,true>", cand_ops, ph_mean[2], VALU_PEAK_TOPS, "TOP/s", "valu",
              "abs-diff-adds"),
             ("fall-back", fb_kernel, fb_ops, ph_mean[3], VALU_PEAK_TOPS, "TOP/s", "valu", "abs-diff-adds"),
             ("refine", "k_refine_wn16<false>" if pat == 4 else f"k_refine_mx16<{pat},false>", win_bytes, ph_mean[4],
@@ -207,7 +214,7 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
             a = work / (ms / 1e3) / 1e12
             roof.append({"phase": name, "kernel": kernel, "bound": bound, "work_per_step": float(work),
                          "work_unit": wunit, "ms": round(float(ms), 4), "achieved": round(a, 3), "peak": peak,
-                         "unit": unit, "frac": round(a / peak, 4)})
+                         "unit": unit, "frac": round(a / peak, 4), "traffic": KERNEL_TRAFFIC.get(kernel)})
     return {"mpx_s": round(W * H * nf / (ms_step / 1e3) / 1e6, 2), "ms_per_step": round(ms_step, 4),
             "roofline": roof,
             "regret": round(float(cost.sum() / pc.sum() - 1.0), 6),
@@ -284,8 +291,10 @@ def run_decision(dev, args, cur, planes, lam, nf):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 1000 for 1080p, about a second of GPU work per path; "
+                         "20 for batched; 100 for lookahead4k)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed warm-up steps (default 10, 3 for batched)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal path")
     ap.add_argument("--no-decision", action="store_true", help="skip the encoder-order decision path")
@@ -296,6 +305,10 @@ def main():
     ap.add_argument("--workload", default="1080p", choices=["1080p", "batched", "lookahead4k"],
                     help="1080p (the headline metric) or a secondary workload")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"1080p": 1000, "batched": 20, "lookahead4k": 100}[args.workload]
+    if args.warmup is None:
+        args.warmup = 3 if args.workload == "batched" else 10
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the motion search on a single MI355X "
               "(streams and GOPs shard across GPUs with no exchange between them). Use --gpus 1.",
