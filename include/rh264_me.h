@@ -621,6 +621,21 @@ int rh264_write_idr_pcm(const rh264_seq *s, const uint8_t *y, const uint8_t *u, 
                         size_t cap, size_t *len);
 int rh264_write_p_slice(const rh264_seq *s, int frame_num, int poc_lsb, int nrefs, const int32_t *rec, uint8_t *out,
                         size_t cap, size_t *len);
+/* rh264_write_p_slice with the luma residual: src is the current picture's luma and
+ * pred its inter prediction at the records' vectors (both (16 mb_w) x (16 mb_h) host
+ * bytes, stride 16 mb_w).  Each 4x4 block of src - pred goes through the forward core
+ * transform and the inter quantiser at s->qp ((|W| MF + 2^qbits / 6) >> qbits,
+ * transform.rs:92, 121, 172-174); a macroblock's coded_block_pattern has the bits of its
+ * 8x8 quadrants with a nonzero level (inter me(v) mapping, cavlc.rs:94-98, 136-138; no
+ * chroma residual), mb_qp_delta 0, and each coded block is residual_block_cavlc
+ * (cavlc.rs: coeff_token with the nC context of its left and upper blocks, trailing-one
+ * signs, levels with the adaptive suffix length, total_zeros, run_before; H.264
+ * Tables 9-5, 9-7, 9-8, 9-10).  A 16x16 reference-0 macroblock at its P_Skip vector
+ * with no nonzero level is skipped.  recon (same layout) receives the decoder's luma:
+ * pred + the scaled, inverse-transformed residual (8.5.12), clipped to 0..255. */
+int rh264_write_p_slice_residual(const rh264_seq *s, int frame_num, int poc_lsb, int nrefs, const int32_t *rec,
+                                 const uint8_t *src, const uint8_t *pred, uint8_t *recon, uint8_t *out, size_t cap,
+                                 size_t *len);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,12 @@ decode side of the check):
 Motion vector prediction is restated from ITU-T H.264 8.4.1.3 (with 6.4.11.7 for the
 neighbouring partitions) on a 4x4-block motion grid, and P_Skip from 8.4.1.1.
 
+The luma residual (forward core transform, inter quantiser, CAVLC residual blocks and
+the 8.5.12 reconstruction) is restated from the same sources and H.264 clause 9.2;
+its pins are in tests/test_bitstream_residual.py (two textbook CAVLC known answers,
+prefix-free tables, write/read round trips in every nC context, inverse-transform
+closed forms, the quantiser's reconstruction error at every QP, planted mistakes).
+
 Pins (tests/test_bitstream.py): the reference's own known answers (Table 9-2/9-3
 exp-Golomb strings, bit_writer.rs:187-228; emulation prevention and the NAL header
 byte, nal.rs:143-160; the SPS/PPS field values of params.rs:226-276), the predict_mv
@@ -231,9 +237,14 @@ def _quad_of(shape, part):
     return (y4 // 2) * 2 + x4 // 2
 
 
-def p_slice(width, height, num_ref_frames, frame_num, poc_lsb, nrefs, rec):
-    """The P picture of decision records rec [mb][14] (mode, cost, TL/TR/BL/BR ref mvx mvy)."""
+def p_slice(width, height, num_ref_frames, frame_num, poc_lsb, nrefs, rec, src=None, pred=None, qp=26):
+    """The P picture of decision records rec [mb][14] (mode, cost, TL/TR/BL/BR ref mvx mvy).
+    With src and pred (coded-size luma planes) the residual src - pred is coded at qp and
+    (bytes, reconstructed luma) is returned; without them no residual (bytes only)."""
     mbw, mbh = (width + 15) // 16, (height + 15) // 16
+    res = src is not None
+    recon = np.zeros((mbh * 16, mbw * 16), np.uint8) if res else None
+    tc = np.zeros((mbh * 4, mbw * 4), np.int64)
     rec = np.asarray(rec).reshape(mbh * mbw, 14)
     w = BitWriter()
     for val in (0, 5, 0):
@@ -259,9 +270,12 @@ def p_slice(width, height, num_ref_frames, frame_num, poc_lsb, nrefs, rec):
             r = [int(v) for v in rec[my * mbw + mx]]
             mode = r[0]
             q = lambda k: r[2 + 3 * k:5 + 3 * k]
-            if mode == 0 and q(0)[0] == 0 and tuple(q(0)[1:]) == skip_mv(grid, mx, my):
+            lv, cbp = mb_residual(src, pred, mx, my, qp) if res else (None, 0)
+            if mode == 0 and q(0)[0] == 0 and cbp == 0 and tuple(q(0)[1:]) == skip_mv(grid, mx, my):
                 run += 1
                 grid.put(mx * 4, my * 4, 4, 4, (0, q(0)[1], q(0)[2]))
+                if res:
+                    recon[my * 16:my * 16 + 16, mx * 16:mx * 16 + 16] = pred[my * 16:my * 16 + 16, mx * 16:mx * 16 + 16]
                 continue
             w.ue(run)
             run = 0
@@ -284,11 +298,289 @@ def p_slice(width, height, num_ref_frames, frame_num, poc_lsb, nrefs, rec):
                 w.se(y - py)
                 x4, y4, w4, h4 = SHAPES[mode][k]
                 grid.put(mx * 4 + x4, my * 4 + y4, w4, h4, (ref, x, y))
-            w.ue(0)
+            if res:
+                write_residual(w, lv, cbp, mx, my, tc)
+                reconstruct(pred, lv, mx, my, qp, recon)
+            else:
+                w.ue(0)
     if run:
         w.ue(run)
     w.trailing()
-    return nal(3, 1, w.bytes())
+    return (nal(3, 1, w.bytes()), recon) if res else nal(3, 1, w.bytes())
+
+
+# ---------------------------------------------------------------------------- residual
+# The luma residual of a P macroblock (7.3.5.3, 8.5.12, 9.2): the reference's CAVLC
+# residual (crates/rusty_h264-common/src/cavlc.rs) and its inter quantiser
+# (crates/rusty_h264-common/src/transform.rs:92, 121, 172-174), restated.
+
+# H.264 Table 9-5 (coeff_token), [table][TrailingOnes][TotalCoeff] for 0<=nC<2, 2<=nC<4, 4<=nC<8
+COEFF_TOKEN = (
+    (
+        ('1', '000101', '00000111', '000000111', '0000000111', '00000000111', '0000000001111', '0000000001011', '0000000001000', '00000000001111', '00000000001011', '000000000001111', '000000000001011', '0000000000001111', '0000000000001011', '0000000000000111', '0000000000000100'),
+        ('', '01', '000100', '00000110', '000000110', '0000000110', '00000000110', '0000000001110', '0000000001010', '00000000001110', '00000000001010', '000000000001110', '000000000001010', '000000000000001', '0000000000001110', '0000000000001010', '0000000000000110'),
+        ('', '', '001', '0000101', '00000101', '000000101', '0000000101', '00000000101', '0000000001101', '0000000001001', '00000000001101', '00000000001001', '000000000001101', '000000000001001', '0000000000001101', '0000000000001001', '0000000000000101'),
+        ('', '', '', '00011', '000011', '0000100', '00000100', '000000100', '0000000100', '00000000100', '0000000001100', '00000000001100', '00000000001000', '000000000001100', '000000000001000', '0000000000001100', '0000000000001000'),
+    ),
+    (
+        ('11', '001011', '000111', '0000111', '00000111', '00000100', '000000111', '00000001111', '00000001011', '000000001111', '000000001011', '000000001000', '0000000001111', '0000000001011', '0000000000111', '00000000001001', '00000000000111'),
+        ('', '10', '00111', '001010', '000110', '0000110', '00000110', '000000110', '00000001110', '00000001010', '000000001110', '000000001010', '0000000001110', '0000000001010', '00000000001011', '00000000001000', '00000000000110'),
+        ('', '', '011', '001001', '000101', '0000101', '00000101', '000000101', '00000001101', '00000001001', '000000001101', '000000001001', '0000000001101', '0000000001001', '0000000000110', '00000000001010', '00000000000101'),
+        ('', '', '', '0101', '0100', '00110', '001000', '000100', '0000100', '000000100', '00000001100', '00000001000', '000000001100', '0000000001100', '0000000001000', '0000000000001', '00000000000100'),
+    ),
+    (
+        ('1111', '001111', '001011', '001000', '0001111', '0001011', '0001001', '0001000', '00001111', '00001011', '000001111', '000001011', '000001000', '0000001101', '0000001001', '0000000101', '0000000001'),
+        ('', '1110', '01111', '01100', '01010', '01000', '001110', '001010', '0001110', '00001110', '00001010', '000001110', '000001010', '000000111', '0000001100', '0000001000', '0000000100'),
+        ('', '', '1101', '01110', '01011', '01001', '001101', '001001', '0001101', '0001010', '00001101', '00001001', '000001101', '000001001', '0000001011', '0000000111', '0000000011'),
+        ('', '', '', '1100', '1011', '1010', '1001', '1000', '01101', '001100', '0001100', '00001100', '00001000', '000001100', '0000001010', '0000000110', '0000000010'),
+    ),
+)
+# Tables 9-7 / 9-8 (total_zeros of a 4x4 block), [TotalCoeff - 1][total_zeros]
+TOTAL_ZEROS = (
+    ('1', '011', '010', '0011', '0010', '00011', '00010', '000011', '000010', '0000011', '0000010', '00000011', '00000010', '000000011', '000000010', '000000001'),
+    ('111', '110', '101', '100', '011', '0101', '0100', '0011', '0010', '00011', '00010', '000011', '000010', '000001', '000000'),
+    ('0101', '111', '110', '101', '0100', '0011', '100', '011', '0010', '00011', '00010', '000001', '00001', '000000'),
+    ('00011', '111', '0101', '0100', '110', '101', '100', '0011', '011', '0010', '00010', '00001', '00000'),
+    ('0101', '0100', '0011', '111', '110', '101', '100', '011', '0010', '00001', '0001', '00000'),
+    ('000001', '00001', '111', '110', '101', '100', '011', '010', '0001', '001', '000000'),
+    ('000001', '00001', '101', '100', '011', '11', '010', '0001', '001', '000000'),
+    ('000001', '0001', '00001', '011', '11', '10', '010', '001', '000000'),
+    ('000001', '000000', '0001', '11', '10', '001', '01', '00001'),
+    ('00001', '00000', '001', '11', '10', '01', '0001'),
+    ('0000', '0001', '001', '010', '1', '011'),
+    ('0000', '0001', '01', '1', '001'),
+    ('000', '001', '1', '01'),
+    ('00', '01', '1'),
+    ('0', '1'),
+)
+# Table 9-10 (run_before), [min(zerosLeft, 7) - 1][run_before]
+RUN_BEFORE = (
+    ('1', '0',),
+    ('1', '01', '00',),
+    ('11', '10', '01', '00',),
+    ('11', '10', '01', '001', '000',),
+    ('11', '10', '011', '010', '001', '000',),
+    ('11', '000', '001', '011', '010', '101', '100',),
+    ('111', '110', '101', '100', '011', '010', '001', '0001', '00001', '000001', '0000001', '00000001', '000000001', '0000000001', '00000000001',),
+)
+
+ZIGZAG = (0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15)  # Table 8-13: scan -> raster
+# Table 9-4, inter column: codeNum -> coded_block_pattern (chroma_format_idc 1)
+CBP_INTER = (0, 16, 1, 2, 4, 8, 32, 3, 5, 10, 12, 15, 47, 7, 11, 13, 14, 6, 9, 31, 35, 37, 42, 44, 33, 34, 36, 40,
+             39, 43, 45, 46, 17, 18, 20, 24, 19, 21, 26, 28, 23, 27, 29, 30, 22, 25, 38, 41)
+CF = np.array([[1, 1, 1, 1], [2, 1, -1, -2], [1, -1, -1, 1], [1, -2, 2, -1]], np.int64)
+MF = ((13107, 5243, 8066), (11916, 4660, 7490), (10082, 4194, 6554), (9362, 3647, 5825), (8192, 3355, 5243),
+      (7282, 2893, 4559))
+V = ((10, 16, 13), (11, 18, 14), (13, 20, 16), (14, 23, 18), (16, 25, 20), (18, 29, 23))
+
+
+def _cls(i, j):
+    """Position class of coefficient (i, j): 0 both even, 1 both odd, 2 mixed."""
+    return 0 if i % 2 == 0 and j % 2 == 0 else 1 if i % 2 and j % 2 else 2
+
+
+def forward4x4(x):
+    """W = Cf X Cf^T (the textbook matrix form of the core transform)."""
+    return CF @ np.asarray(x, np.int64).reshape(4, 4) @ CF.T
+
+
+def quant4x4(w, qp):
+    """Inter dead zone: |level| = (|W| MF + 2^qbits / 6) >> qbits, the sign of W."""
+    qbits = 15 + qp // 6
+    f = (1 << qbits) // 6
+    z = np.zeros((4, 4), np.int64)
+    for i in range(4):
+        for j in range(4):
+            a = abs(int(w[i][j]))
+            lvl = (a * MF[qp % 6][_cls(i, j)] + f) >> qbits
+            z[i][j] = -lvl if w[i][j] < 0 else lvl
+    return z
+
+
+def dequant4x4(z, qp):
+    """8.5.12.1 with flat weights: d = c * LevelScale4x4 << (qp/6 - 4) for qp >= 24, else
+    (c * LevelScale4x4 + 2^(3 - qp/6)) >> (4 - qp/6), LevelScale4x4 = 16 v."""
+    d = np.zeros((4, 4), np.int64)
+    for i in range(4):
+        for j in range(4):
+            ls = 16 * V[qp % 6][_cls(i, j)]
+            c = int(z[i][j])
+            d[i][j] = (c * ls) << (qp // 6 - 4) if qp >= 24 else (c * ls + (1 << (3 - qp // 6))) >> (4 - qp // 6)
+    return d
+
+
+def inverse4x4(d):
+    """8.5.12.2: the row and column butterflies (with the >> 1 of the odd terms), then
+    (x + 32) >> 6."""
+    d = np.asarray(d, np.int64)
+    f = np.zeros((4, 4), np.int64)
+    for i in range(4):
+        e0, e1 = d[i][0] + d[i][2], d[i][0] - d[i][2]
+        e2, e3 = (d[i][1] >> 1) - d[i][3], d[i][1] + (d[i][3] >> 1)
+        f[i] = (e0 + e3, e1 + e2, e1 - e2, e0 - e3)
+    h = np.zeros((4, 4), np.int64)
+    for j in range(4):
+        g0, g1 = f[0][j] + f[2][j], f[0][j] - f[2][j]
+        g2, g3 = (f[1][j] >> 1) - f[3][j], f[1][j] + (f[3][j] >> 1)
+        h[0][j], h[1][j], h[2][j], h[3][j] = g0 + g3, g1 + g2, g1 - g2, g0 - g3
+    return (h + 32) >> 6
+
+
+def write_block(w, lv, nc):
+    """residual_block_cavlc of 16 levels in scan order with context nc; returns TotalCoeff."""
+    nz = [i for i in range(16) if lv[i]]
+    n = len(nz)
+    t1 = 0
+    for k in range(n - 1, -1, -1):
+        if t1 == 3 or abs(lv[nz[k]]) != 1:
+            break
+        t1 += 1
+    if nc >= 8:
+        w.u(3 if n == 0 else ((n - 1) << 2) | t1, 6)
+    else:
+        w.bits += [int(b) for b in COEFF_TOKEN[0 if nc < 2 else 1 if nc < 4 else 2][t1][n]]
+    if n == 0:
+        return 0
+    for k in range(n - 1, n - 1 - t1, -1):
+        w.u(1 if lv[nz[k]] < 0 else 0, 1)
+    sl = 1 if n > 10 and t1 < 3 else 0
+    for k in range(n - 1 - t1, -1, -1):
+        v = lv[nz[k]]
+        code = 2 * v - 2 if v > 0 else -2 * v - 1
+        if k == n - 1 - t1 and t1 < 3:
+            code -= 2
+        if sl == 0 and code < 14:
+            prefix, size, suffix = code, 0, 0
+        elif sl == 0 and code < 30:
+            prefix, size, suffix = 14, 4, code - 14
+        elif sl == 0:
+            prefix, size, suffix = 15, 12, code - 30
+        elif code < (15 << sl):
+            prefix, size, suffix = code >> sl, sl, code & ((1 << sl) - 1)
+        else:
+            prefix, size, suffix = 15, 12, code - (15 << sl)
+        assert suffix < 4096, "level beyond the escape range"
+        w.u(1, prefix + 1)
+        if size:
+            w.u(suffix, size)
+        sl = max(sl, 1)
+        if abs(v) > (3 << (sl - 1)) and sl < 6:
+            sl += 1
+    tz = nz[-1] + 1 - n
+    if n < 16:
+        w.bits += [int(b) for b in TOTAL_ZEROS[n - 1][tz]]
+    left = tz
+    for k in range(n - 1, 0, -1):
+        if left <= 0:
+            break
+        run = nz[k] - nz[k - 1] - 1
+        w.bits += [int(b) for b in RUN_BEFORE[min(left, 7) - 1][run]]
+        left -= run
+    return n
+
+
+def _read_vlc(r, codes):
+    """Index of the code word of `codes` (a prefix-free sequence, "" = unused) next in r."""
+    cur = ""
+    while True:
+        cur += str(r.u(1))
+        if cur in codes:
+            return codes.index(cur)
+        assert len(cur) <= 16, "no code word"
+
+
+def read_block(r, nc):
+    """The 16 levels (scan order) of residual_block_cavlc with context nc, and TotalCoeff."""
+    if nc >= 8:
+        v = r.u(6)
+        n, t1 = (0, 0) if v == 3 else ((v >> 2) + 1, v & 3)
+    else:
+        tab = COEFF_TOKEN[0 if nc < 2 else 1 if nc < 4 else 2]
+        flat = [tab[t][k] for k in range(17) for t in range(4)]
+        i = _read_vlc(r, flat)
+        n, t1 = i // 4, i % 4
+    lv = [0] * 16
+    if n == 0:
+        return lv, 0
+    levels = []
+    for _ in range(t1):
+        levels.append(-1 if r.u(1) else 1)
+    sl = 1 if n > 10 and t1 < 3 else 0
+    for i in range(t1, n):
+        prefix = 0
+        while r.u(1) == 0:
+            prefix += 1
+        size = 4 if (prefix == 14 and sl == 0) else 12 if prefix >= 15 else sl
+        suffix = r.u(size) if size else 0
+        code = (min(15, prefix) << sl) + suffix
+        if prefix >= 15 and sl == 0:
+            code += 15
+        assert prefix <= 15, "escape beyond Baseline"
+        if i == t1 and t1 < 3:
+            code += 2
+        v = (code + 2) >> 1 if code % 2 == 0 else (-code - 1) >> 1
+        levels.append(v)
+        sl = max(sl, 1)
+        if abs(v) > (3 << (sl - 1)) and sl < 6:
+            sl += 1
+    tz = _read_vlc(r, list(TOTAL_ZEROS[n - 1])) if n < 16 else 0
+    runs, left = [], tz
+    for _ in range(n - 1):
+        run = _read_vlc(r, list(RUN_BEFORE[min(left, 7) - 1])) if left > 0 else 0
+        runs.append(run)
+        left -= run
+    runs.append(left)
+    pos = tz + n - 1  # scan position of the highest-frequency level
+    for k in range(n):
+        lv[pos] = levels[k]
+        pos -= 1 + runs[k]
+    return lv, n
+
+
+def _nc(tc, gx, gy):
+    """nC of the 4x4 block at (gx, gy) of the picture's 4x4 grid (9.2.1, one slice)."""
+    ha, hb = gx > 0, gy > 0
+    na = int(tc[gy, gx - 1]) if ha else 0
+    nb = int(tc[gy - 1, gx]) if hb else 0
+    return (na + nb + 1) >> 1 if ha and hb else na if ha else nb
+
+
+def mb_residual(src, pred, mx, my, qp):
+    """The quantised levels [block raster 0..15][4x4 raster] of macroblock (mx, my) and
+    its luma coded_block_pattern."""
+    lv = np.zeros((16, 4, 4), np.int64)
+    cbp = 0
+    for b in range(16):
+        y0, x0 = my * 16 + (b // 4) * 4, mx * 16 + (b % 4) * 4
+        d = src[y0:y0 + 4, x0:x0 + 4].astype(np.int64) - pred[y0:y0 + 4, x0:x0 + 4].astype(np.int64)
+        lv[b] = quant4x4(forward4x4(d), qp)
+        if lv[b].any():
+            cbp |= 1 << ((b // 8) * 2 + (b % 4) // 2)
+    return lv, cbp
+
+
+def reconstruct(pred, lv, mx, my, qp, out):
+    """out[macroblock] = clip(pred + the decoded residual of lv)."""
+    for b in range(16):
+        y0, x0 = my * 16 + (b // 4) * 4, mx * 16 + (b % 4) * 4
+        r = inverse4x4(dequant4x4(lv[b], qp)) if lv[b].any() else 0
+        out[y0:y0 + 4, x0:x0 + 4] = np.clip(pred[y0:y0 + 4, x0:x0 + 4].astype(np.int64) + r, 0, 255)
+
+
+def write_residual(w, lv, cbp, mx, my, tc):
+    """coded_block_pattern, mb_qp_delta and residual_luma of a coded P macroblock."""
+    w.ue(CBP_INTER.index(cbp))
+    if cbp:
+        w.se(0)
+    for q8 in range(4):
+        for q4 in range(4):
+            x4, y4 = (q8 % 2) * 2 + q4 % 2, (q8 // 2) * 2 + q4 // 2
+            gx, gy = mx * 4 + x4, my * 4 + y4
+            n = 0
+            if cbp & (1 << q8):
+                blk = lv[y4 * 4 + x4].reshape(16)
+                n = write_block(w, [int(blk[ZIGZAG[i]]) for i in range(16)], _nc(tc, gx, gy))
+            tc[gy, gx] = n
 
 
 # ---------------------------------------------------------------------------- parser
@@ -389,6 +681,8 @@ def parse_slice(rbsp, nal_type, sps, pps):
         r.trailing()
         return h, mbs
     grid = MotionGrid(mbw, mbh)
+    tc = np.zeros((mbh * 4, mbw * 4), np.int64)
+    h["levels"] = [None] * (mbw * mbh)  # per macroblock: [16 blocks][4][4] levels, or None
     n, skip = 0, 0
     while n < mbw * mbh:
         if skip == 0:
@@ -429,7 +723,20 @@ def parse_slice(rbsp, nal_type, sps, pps):
                 qx, qy = (qq % 2) * 2, (qq // 2) * 2
                 if x4 <= qx < x4 + w4 and y4 <= qy < y4 + h4:
                     quads[qq] = (refs[k], x, y)
-        assert r.ue() == 0, "coded_block_pattern 0"
+        cbp = CBP_INTER[r.ue()]
+        assert cbp < 16, "chroma residual"
+        if cbp:
+            h.setdefault("mb_qp_delta", []).append(r.se())
+            lv = np.zeros((16, 4, 4), np.int64)
+            for q8 in range(4):
+                for q4 in range(4):
+                    x4, y4 = (q8 % 2) * 2 + q4 % 2, (q8 // 2) * 2 + q4 // 2
+                    gx, gy = mx * 4 + x4, my * 4 + y4
+                    if cbp & (1 << q8):
+                        scan, tc[gy, gx] = read_block(r, _nc(tc, gx, gy))
+                        for i in range(16):
+                            lv[y4 * 4 + x4].reshape(16)[ZIGZAG[i]] = scan[i]
+            h["levels"][n] = lv
         mbs.append((mode, quads, False))
         n += 1
         skip = 0

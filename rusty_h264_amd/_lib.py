@@ -54,6 +54,8 @@ PROTOTYPES = {
     "rh264_write_idr_pcm": (C.c_int, [C.POINTER(Seq), c_vp, c_vp, c_vp, c_vp, C.c_size_t, C.POINTER(C.c_size_t)]),
     "rh264_write_p_slice": (C.c_int, [C.POINTER(Seq), C.c_int, C.c_int, C.c_int, c_vp, c_vp, C.c_size_t,
                                       C.POINTER(C.c_size_t)]),
+    "rh264_write_p_slice_residual": (C.c_int, [C.POINTER(Seq), C.c_int, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp,
+                                               c_vp, C.c_size_t, C.POINTER(C.c_size_t)]),
     "rh264_init": (C.c_int, [C.c_int, C.POINTER(c_vp)]),
     "rh264_destroy": (C.c_int, [c_vp]),
     "rh264_last_error": (C.c_char_p, []),

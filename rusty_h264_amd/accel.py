@@ -586,6 +586,22 @@ def write_p_slice(lib, s, frame_num, poc_lsb, nrefs, rec):
                   int(nrefs), rec.ctypes.data)
 
 
+def write_p_slice_residual(lib, s, frame_num, poc_lsb, nrefs, rec, src, pred):
+    """A P picture with its luma residual (rh264_write_p_slice_residual): src and pred
+    are the current picture and its inter prediction at the records' vectors, both
+    (16 mb_h) x (16 mb_w) host luma planes; the residual src - pred is coded at s.qp.
+    Returns (Annex-B bytes, the reconstructed luma plane)."""
+    rec = np.ascontiguousarray(rec, np.int32)
+    src = np.ascontiguousarray(src, np.uint8)
+    pred = np.ascontiguousarray(pred, np.uint8)
+    if src.shape != pred.shape or src.shape != (16 * ((s.height + 15) // 16), 16 * ((s.width + 15) // 16)):
+        raise ValueError("write_p_slice_residual: src and pred must be the coded-size luma plane")
+    recon = np.empty_like(src)
+    data = _write(lib.rh264_write_p_slice_residual, "rh264_write_p_slice_residual", C.byref(s), int(frame_num),
+                  int(poc_lsb), int(nrefs), rec.ctypes.data, src.ctypes.data, pred.ctypes.data, recon.ctypes.data)
+    return data, recon
+
+
 def records_16x16(ref, mv):
     """The 16x16 decision's (ref [mb], mv [mb][2]) as partition-decision records (mode
     0, cost 0, four equal quadrants)."""
