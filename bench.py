@@ -225,6 +225,68 @@ def run_learned(dev, args, cur, refs, planes, p, plain_cost, pat=None):
             "plane_build_ms": round(ms_step - search_ms, 4)}
 
 
+BD_QPS = (22, 27, 32, 37)
+
+
+def bjontegaard_rate(r1, p1, r2, p2):
+    """BD-rate (%) of the curve (r2, p2) against (r1, p1): cubic fits of log10(rate)
+    against PSNR, their mean difference over the common PSNR interval (Bjontegaard,
+    VCEG-M33)."""
+    c1, c2 = np.polyfit(p1, np.log10(r1), 3), np.polyfit(p2, np.log10(r2), 3)
+    lo, hi = max(min(p1), min(p2)), min(max(p1), max(p2))
+    i1, i2 = np.polyint(c1), np.polyint(c2)
+    avg = ((np.polyval(i2, hi) - np.polyval(i2, lo)) - (np.polyval(i1, hi) - np.polyval(i1, lo))) / (hi - lo)
+    return float((10 ** avg - 1) * 100)
+
+
+def run_bd_rate(dev, frames, cur, refs, planes):
+    """BD-rate of the learned path's vectors against the plain path's: the clip coded
+    as an I_PCM IDR + 15 P pictures (16x16, one reference, luma residual, CAVLC) at
+    QP 22/27/32/37, each path searching with that QP's lambda on the source frames and
+    the pictures predicted from the previous reconstruction (accel.encode_p_frames).
+    Rate = the P pictures' bytes, distortion = their mean luma PSNR."""
+    from rusty_h264_amd import accel, mvnet
+    if not os.path.exists(mvnet.WEIGHTS):
+        return None
+    nf = NFRAMES - 1
+    nb = (W // 16) * (H // 16) * nf
+    pbytes = accel.planes_bytes(W, H, PAD)
+    accel.build_planes(dev, refs, nf, W, H, PAD, out=planes)
+    net = mvnet.MVNet(dev, nf, W, H, mvnet.load_weights())
+    outs = (dev.alloc(8 * nb), dev.alloc(4 * nb), dev.alloc(4 * nb))
+    curves = {"plain": ([], []), "learned": ([], [])}
+    per_qp = []
+    t0 = time.perf_counter()
+    for qp in BD_QPS:
+        lam = accel.lambda_me(qp)
+        ms = accel.MotionSearch(dev, accel.search_params(W, H, PAD, 16, 16, R, 0, lam))
+        ms.run_plain(cur, planes, nf)
+        mvs = {"plain": ms.results(nf)["mv"]}
+        pl = accel.search_params(W, H, PAD, 16, 16, R, mvnet.REFINE_PAT, lam)
+        mvnet.learned_search(dev, net, pl, nf, cur, refs, W * H, planes, pbytes, mvnet.LOCAL_RANGE,
+                             mvnet.FALLBACK_COST, out=outs)
+        mvs["learned"] = dev.download(outs[0], (nb, 2), np.int32)
+        row = {"qp": qp}
+        for name, mv in mvs.items():
+            _, sizes, recon = accel.encode_p_frames(dev, frames, mv, qp, pad=PAD)
+            psnr = [10 * np.log10(255.0 ** 2 / max(np.mean((recon[f].astype(np.float64) - frames[f]) ** 2), 1e-9))
+                    for f in range(1, NFRAMES)]
+            bits = 8 * sum(sizes[1:])
+            curves[name][0].append(bits)
+            curves[name][1].append(float(np.mean(psnr)))
+            row[name] = {"p_bytes": bits // 8, "psnr_y": round(float(np.mean(psnr)), 4)}
+        per_qp.append(row)
+    net.close()
+    for x in outs:
+        x.free()
+    bd = bjontegaard_rate(curves["plain"][0], curves["plain"][1], curves["learned"][0], curves["learned"][1])
+    return {"bd_rate_pct": round(bd, 4), "qps": per_qp, "seconds": round(time.perf_counter() - t0, 1),
+            "what": "learned vs plain vectors, each path at the QP's lambda; I_PCM IDR + 15 P pictures, 16x16 "
+                    "P_L0 / P_Skip, 1 ref, luma residual (forward core transform, inter dead zone, CAVLC), "
+                    "predicted from the previous reconstruction; rate = P-picture bytes, PSNR = mean luma PSNR "
+                    "of the P pictures; negative = the learned vectors need fewer bits"}
+
+
 def run_decision(dev, args, cur, planes, lam, nf):
     """The encoder-order 16x16 decision of the 15 P-frames (rh264_motion_search_wavefront:
     one reference, each macroblock's predictor predict_mv of its decided neighbours,
@@ -298,6 +360,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal path")
     ap.add_argument("--no-decision", action="store_true", help="skip the encoder-order decision path")
+    ap.add_argument("--no-bdrate", action="store_true",
+                    help="skip the BD-rate of the learned path's vectors against the plain path's")
     ap.add_argument("--fallback-cost", type=int, default=None, help="learned path: fall-back threshold")
     ap.add_argument("--la-fb", type=int, default=LA_FALLBACK_SAD,
                     help="lookahead4k: 8x8 SAD above which a learned-seeded block falls back to the full search")
@@ -379,6 +443,9 @@ def main():
             # the same proposal refined by the plain search's ring walk, for comparison
             learned_ring = run_learned(dev, args, cur, refs, planes, p, res["cost"], pat=0)
         decision = None if args.no_decision else run_decision(dev, args, cur, planes, lam, nf)
+        bd = None if (args.no_learned or args.no_bdrate) else run_bd_rate(dev, frames, cur, refs, planes)
+        if learned is not None and bd is not None:
+            learned["bd_rate"] = bd
 
     ms_per_step = ev_ms / args.steps
     mpx = W * H * nf / (ms_per_step / 1e3) / 1e6

@@ -556,8 +556,12 @@ def seq(width, height, qp=26, num_ref_frames=1, level_idc=30):
     return _lib.Seq(width, height, qp, num_ref_frames, level_idc)
 
 
-def _write(fn, what, *args):
+def _write(fn, what, *args, guess=0):
     n = C.c_size_t()
+    if guess:  # one call when the unit fits the guess
+        buf = np.empty(guess, np.uint8)
+        if fn(*args, buf.ctypes.data, buf.size, C.byref(n)) == 0:
+            return bytes(buf[:n.value])
     fn(*args, None, 0, C.byref(n))  # size query (fails with the needed size set)
     buf = np.empty(max(int(n.value), 1), np.uint8)
     _check(fn(*args, buf.ctypes.data, buf.size, C.byref(n)), what)
@@ -598,8 +602,45 @@ def write_p_slice_residual(lib, s, frame_num, poc_lsb, nrefs, rec, src, pred):
         raise ValueError("write_p_slice_residual: src and pred must be the coded-size luma plane")
     recon = np.empty_like(src)
     data = _write(lib.rh264_write_p_slice_residual, "rh264_write_p_slice_residual", C.byref(s), int(frame_num),
-                  int(poc_lsb), int(nrefs), rec.ctypes.data, src.ctypes.data, pred.ctypes.data, recon.ctypes.data)
+                  int(poc_lsb), int(nrefs), rec.ctypes.data, src.ctypes.data, pred.ctypes.data, recon.ctypes.data,
+                  guess=src.size + 65536)
     return data, recon
+
+
+def encode_p_frames(dev, frames, mv, qp, pad=HPEL_PAD_DEFAULT):
+    """The bitstream of a clip decided at 16x16, one reference: frames[0] (host luma,
+    [n][ch][cw]) as the I_PCM IDR, each later frame f as a P picture whose macroblocks
+    take the vectors mv[f - 1] ([nblk][2] quarter pel) against the previous
+    reconstruction, with the luma residual at qp.  Per picture the reference's plane
+    set is built from the previous reconstruction on the GPU, rh264_decision_predict
+    predicts the picture and rh264_write_p_slice_residual codes it (host).  Returns
+    (Annex-B bytes, bytes per picture, reconstructed luma pictures)."""
+    frames = np.ascontiguousarray(frames, np.uint8)
+    nf, ch, cw = frames.shape
+    nb = (cw // 16) * (ch // 16)
+    mv = np.ascontiguousarray(mv, np.int32).reshape(nf - 1, nb, 2)
+    lib = dev.lib
+    s = seq(cw, ch, qp=qp, num_ref_frames=1)
+    chroma = np.full((ch // 2, cw // 2), 128, np.uint8)
+    head, idr = write_headers(lib, s), write_idr_pcm(lib, s, frames[0], chroma, chroma)
+    rbuf, pl = dev.alloc(cw * ch), dev.alloc(planes_bytes(cw, ch, pad))
+    refb, mvb = dev.upload(np.zeros(nb, np.int32)), dev.alloc(8 * nb)
+    zeros = np.zeros(nb, np.int32)
+    out, sizes, recon = [head, idr], [len(idr)], [frames[0]]
+    try:
+        for f in range(1, nf):
+            _check(lib.rh264_h2d(dev.ctx, rbuf.ptr, recon[-1].ctypes.data, cw * ch), "rh264_h2d")
+            _check(lib.rh264_h2d(dev.ctx, mvb.ptr, mv[f - 1].ctypes.data, 8 * nb), "rh264_h2d")
+            build_planes(dev, rbuf, 1, cw, ch, pad, out=pl)
+            _, pred, _ = decision_predict(dev, cw, ch, pad, 1, 1, pl, refb, mvb)
+            data, rc = write_p_slice_residual(lib, s, f, 2 * f, 1, records_16x16(zeros, mv[f - 1]), frames[f], pred[0])
+            out.append(data)
+            sizes.append(len(data))
+            recon.append(rc)
+    finally:
+        for x in (rbuf, pl, refb, mvb):
+            x.free()
+    return b"".join(out), sizes, recon
 
 
 def records_16x16(ref, mv):

@@ -101,3 +101,54 @@ def test_decided_clip_decodes_to_the_prediction(dev):
     assert skipped.sum() > 0
     for x in (cur, refs, pl, ref_d, mv_d, cost_d):
         x.free()
+
+
+@pytest.mark.parametrize("qp", [22, 32])
+def test_residual_coded_clip_decodes(dev, qp):
+    """A CIF clip searched on the GPU (plain path, 16x16) and coded with the luma
+    residual against the previous reconstruction (accel.encode_p_frames: GPU plane
+    build and prediction, host residual writer): the GPU's prediction of every picture
+    equals the oracle's mc_luma on the reconstruction, the library's bytes equal the
+    oracle writer's, and the oracle decodes the stream alone to the reconstructions."""
+    cw, ch, nf = 352, 288, 4
+    lib = _lib.load()
+    clip = synth.cif_clip(nf + 1, seed=93 + qp, w=cw, h=ch)
+    lam = accel.lambda_me(qp)
+    cur, refs = dev.upload(clip[1:]), dev.upload(clip[:-1])
+    pl = accel.build_planes(dev, refs, nf, cw, ch, PAD)
+    ms = accel.MotionSearch(dev, accel.search_params(cw, ch, PAD, 16, 16, 32, 0, lam))
+    print("residual clip", cw, ch, nf, "qp", qp, flush=True)
+    ms.run_plain(cur, pl, nf)
+    mv = ms.results(nf)["mv"]
+    stream, sizes, recon = accel.encode_p_frames(dev, clip, mv, qp, pad=PAD)
+    nb, mbw = (cw // 16) * (ch // 16), cw // 16
+    units = B.parse_stream(stream)
+    assert [u[0] for u in units] == ["sps", "pps"] + ["slice"] * (nf + 1)
+    decoded = [np.block([[units[2][4][my * mbw + mx][1] for mx in range(mbw)] for my in range(ch // 16)])]
+    assert np.array_equal(decoded[0], clip[0])
+    s = accel.seq(cw, ch, qp=qp)
+    coded = 0
+    for f in range(1, nf + 1):
+        hdr, mbs = units[2 + f][3], units[2 + f][4]
+        fmv = mv[(f - 1) * nb:f * nb]
+        # the GPU's prediction on the reconstruction, through the oracle's mc_luma
+        pred = np.zeros((ch, cw), np.uint8)
+        for m in range(nb):
+            by, bx = (m // mbw) * 16, (m % mbw) * 16
+            pred[by:by + 16, bx:bx + 16] = O.mc_luma(decoded[-1], cw, ch, bx, by, 16, 16, int(fmv[m, 0]), int(fmv[m, 1]))
+        rec = accel.records_16x16(np.zeros(nb, np.int32), fmv)
+        want, wrecon = B.p_slice(cw, ch, 1, f, 2 * f, 1, rec, src=clip[f], pred=pred, qp=qp)
+        off = len(stream) - sum(sizes[f:])  # the library's bytes of picture f
+        assert stream[off:off + sizes[f]] == want, f
+        assert np.array_equal(wrecon, recon[f]), f
+        out = pred.copy()
+        for m, (mode, quads, _) in enumerate(mbs):
+            assert quads[0][1:] == (int(fmv[m, 0]), int(fmv[m, 1])), (f, m)
+            if hdr["levels"][m] is not None:
+                B.reconstruct(pred, hdr["levels"][m], m % mbw, m // mbw, qp, out)
+                coded += 1
+        assert np.array_equal(out, recon[f]), f
+        decoded.append(out)
+    assert coded > 0
+    for x in (cur, refs, pl):
+        x.free()
