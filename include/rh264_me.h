@@ -367,8 +367,9 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
                                   const uint8_t *planes, size_t planes_stride, int32_t *ref, int32_t *mv,
                                   int32_t *cost);
 
-/* The P macroblock decision with partitions, in macroblock order (mb16.rs:8849-9010
- * with the shape picked in the SATD currency: shape RD off, the all-8x8 P_8x8 arm,
+/* The P macroblock decision with partitions, in macroblock order (the default CAVLC
+ * path, mb16.rs:5520-5567; mb16.rs:8849-9010 on the CABAC path without its sub-8x8
+ * arm; the shape picked in the SATD currency: shape RD off, the all-8x8 P_8x8 arm,
  * every macroblock inter).  Per macroblock: the 16x16 best_part (mb16.rs:4921-4959)
  * as rh264_motion_search_wavefront, predictors predict_mv (inter.rs:36-73) of the
  * neighbours read on the 4x4 motion grid (mv_neighbors_block, mb16.rs:1871-1888: the
@@ -632,7 +633,10 @@ int rh264_write_p_slice(const rh264_seq *s, int frame_num, int poc_lsb, int nref
  * signs, levels with the adaptive suffix length, total_zeros, run_before; H.264
  * Tables 9-5, 9-7, 9-8, 9-10).  A 16x16 reference-0 macroblock at its P_Skip vector
  * with no nonzero level is skipped.  recon (same layout) receives the decoder's luma:
- * pred + the scaled, inverse-transformed residual (8.5.12), clipped to 0..255. */
+ * pred + the scaled, inverse-transformed residual (8.5.12), clipped to 0..255; the
+ * slice header signals disable_deblocking_filter_idc 1 (the reference's writes 0 and
+ * deblocks its reconstruction; this writer does not deblock), so recon is exactly a
+ * decoder's output picture. */
 int rh264_write_p_slice_residual(const rh264_seq *s, int frame_num, int poc_lsb, int nrefs, const int32_t *rec,
                                  const uint8_t *src, const uint8_t *pred, uint8_t *recon, uint8_t *out, size_t cap,
                                  size_t *len);

@@ -259,9 +259,12 @@ def p_slice(width, height, num_ref_frames, frame_num, poc_lsb, nrefs, rec, src=N
     w.u(0, 1)
     w.u(0, 1)
     w.se(0)
-    w.ue(0)
-    w.se(0)
-    w.se(0)
+    if res:
+        w.ue(1)  # disable_deblocking_filter_idc 1: the reconstruction is the decoder's output
+    else:
+        w.ue(0)
+        w.se(0)
+        w.se(0)
     nact = nrefs if num_ref_frames > 1 else 1
     grid = MotionGrid(mbw, mbh)
     run = 0
@@ -667,7 +670,8 @@ def parse_slice(rbsp, nal_type, sps, pps):
         h["adaptive_ref_pic_marking"] = r.u(1)
     h["slice_qp_delta"] = r.se()
     h["disable_deblocking_filter_idc"] = r.ue()
-    h["alpha"], h["beta"] = r.se(), r.se()
+    if h["disable_deblocking_filter_idc"] != 1:
+        h["alpha"], h["beta"] = r.se(), r.se()
     h["num_ref_idx_active"] = nact
     mbs = []
     if h["slice_type"] % 5 == 2:

@@ -545,9 +545,15 @@ int write_p_slice(const rh264_seq *s, int frame_num, int poc_lsb, int nrefs, con
     w.bit(0);                            // ref_pic_list_modification_flag_l0
     w.bit(0);                            // adaptive_ref_pic_marking_mode_flag
     w.se(0);                             // slice_qp_delta
-    w.ue(0);                             // disable_deblocking_filter_idc
-    w.se(0);
-    w.se(0);
+    if (res) {
+        // disable_deblocking_filter_idc 1: the reconstruction returned (and predicted
+        // from) is then exactly what a decoder outputs
+        w.ue(1);
+    } else {
+        w.ue(0);                         // the reference's header: filter on, offsets 0
+        w.se(0);
+        w.se(0);
+    }
     const int nactive = s->num_ref_frames > 1 ? nrefs : 1;
     Grid g(mbw, mbh);
     uint32_t skip_run = 0;

@@ -196,6 +196,7 @@ def test_clip_decodes_to_the_reconstruction(lib):
                 B.reconstruct(pred, lv, m % mbw, m // mbw, qp + hdr["slice_qp_delta"], out)
                 coded += 1
         assert all(d == 0 for d in hdr.get("mb_qp_delta", []))
+        assert hdr["disable_deblocking_filter_idc"] == 1 and "alpha" not in hdr
         decoded.append(out)
         assert np.array_equal(out, recons[f]), f
     assert coded > 0
