@@ -966,6 +966,10 @@ void orc_motion_search_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *d
                              int32_t *mv, int64_t *cost)
 {
     const int nbx = cfg->cw / 16, nby = cfg->ch / 16;
+    /* the frame's rescue learning window (mb16.rs:2868-2886): searches whose grid ran,
+     * those that paid, and the latch that turns the rescue off for the frame */
+    int resc_n = 0, resc_big = 0, resc_off = 0;
+    orc_diamond_cfg d = *dc;
     /* raster macroblock order, as the encoder's frame loop; each macroblock
      * takes its 16x16 best_part result (mb16.rs:4921-4959) */
     for (int y = 0; y < nby; y++)
@@ -981,9 +985,17 @@ void orc_motion_search_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *d
             int64_t bc = INT64_MAX;
             int br = 0, bmx = 0, bmy = 0;
             for (int r = 0; r < nrefs; r++) {
-                int32_t pr[2], m[2];
+                int32_t pr[2], m[2], ev[2] = {0, 0};
                 orc_predict_mv(nb, r, pr);
-                int64_t c = orc_motion_search_ref(cfg, dc, cur, planes[r], 16 * x, 16 * y, pr, 1, m);
+                d.rescue = dc->rescue && !resc_off;
+                int64_t c = orc_motion_search_ex(cfg, &d, cur, planes[r], 16 * x, 16 * y, pr, 1, m, ev, NULL, 0,
+                                                 NULL);
+                if (ev[0] && resc_n < dc->rescue_learn) {
+                    resc_n++;
+                    if (ev[1]) resc_big++;
+                    if (resc_n == dc->rescue_learn && resc_big * 100 < dc->rescue_learn * dc->rescue_payoff)
+                        resc_off = 1;
+                }
                 c += (int64_t)(lambda_ref * (double)ref_bits_orc(r, nrefs));
                 if (c < bc) { bc = c; br = r; bmx = m[0]; bmy = m[1]; }
             }

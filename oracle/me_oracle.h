@@ -118,6 +118,8 @@ typedef struct {
     int rescue;                          /* me_wide rescue enabled for this search      */
     int rescue_range, rescue_thresh;     /* me_range (24), me_rescue (3)                */
     int rescue_fast, rescue_var;         /* me_fast (1), me_wide_var (800)              */
+    int rescue_learn, rescue_payoff;     /* the frame's learning window: me_learn (40),
+                                            me_payoff_pct (15); orc_motion_search_frame */
 } orc_diamond_cfg;
 /* rate of one mvd component: Exp-Golomb length, or tab[min(|d|, 4095)] */
 uint32_t orc_mv_rate(const uint16_t *tab, int32_t d);
@@ -139,7 +141,12 @@ void orc_predict_mv(const int32_t nb[12], int cur_ref, int32_t out[2]);
  * reference r (planes[r], r = 0 the most recent) orc_motion_search_ref with the
  * single predictor predict_mv(.., r) of the decided neighbours (mv_neighbors,
  * mb16.rs:1813-1836), cost + (int64)(lambda_ref * ref_bits(r, nrefs)), the first
- * minimum in reference order.  ref, cost: one per macroblock; mv: two. */
+ * minimum in reference order.  ref, cost: one per macroblock; mv: two.  With
+ * dc->rescue, the frame's me_wide learning window (mb16.rs:2868-2886): the first
+ * rescue_learn searches (in macroblock, then reference order) whose rescue grid ran
+ * are tallied, those that cut their cost to <= 15/16 counted; when the window fills
+ * with fewer than rescue_payoff % of them, the rescue is off for every later search
+ * of the frame. */
 void orc_motion_search_frame(const orc_search_cfg *cfg, const orc_diamond_cfg *dc, const uint8_t *cur,
                              const uint8_t *const *planes, int nrefs, double lambda_ref, int32_t *ref,
                              int32_t *mv, int64_t *cost);

@@ -28,7 +28,8 @@ class DiamondCfg(C.Structure):
     _fields_ = [("steps", i32p), ("nsteps", C.c_int), ("sadfp", C.c_int), ("snap", C.c_int), ("fc", C.c_int),
                 ("sad_only", C.c_int), ("mv_tab", C.POINTER(C.c_uint16)), ("lam", C.c_double),
                 ("lam_fp", C.c_double), ("rescue", C.c_int), ("rescue_range", C.c_int),
-                ("rescue_thresh", C.c_int), ("rescue_fast", C.c_int), ("rescue_var", C.c_int)]
+                ("rescue_thresh", C.c_int), ("rescue_fast", C.c_int), ("rescue_var", C.c_int),
+                ("rescue_learn", C.c_int), ("rescue_payoff", C.c_int)]
 
 
 u16p = C.POINTER(C.c_uint16)
@@ -151,13 +152,16 @@ class Oracle:
         return c
 
     def diamond(self, cfg, steps=(16, 8, 4), sadfp=False, snap=True, fc=False, sad_only=False, mv_tab=None,
-                rescue=False, rescue_range=24, rescue_thresh=3, rescue_fast=True, rescue_var=800):
-        """motion_search's knobs (orc_diamond_cfg); lambdas from cfg (lam, lam_fp)."""
+                rescue=False, rescue_range=24, rescue_thresh=3, rescue_fast=True, rescue_var=800, rescue_learn=40,
+                rescue_payoff=15):
+        """motion_search's knobs (orc_diamond_cfg); lambdas from cfg (lam, lam_fp); the
+        learning window (rescue_learn, rescue_payoff) applies to the frame decision."""
         st = np.ascontiguousarray(steps, np.int32)
         tab = None if mv_tab is None else np.ascontiguousarray(mv_tab, np.uint16)
         dc = DiamondCfg(_p(st, i32p), len(st), int(sadfp), int(snap), int(fc), int(sad_only),
                         None if tab is None else _p(tab, u16p), cfg._lam, cfg._lam_fp, int(rescue),
-                        int(rescue_range), int(rescue_thresh), int(rescue_fast), int(rescue_var))
+                        int(rescue_range), int(rescue_thresh), int(rescue_fast), int(rescue_var), int(rescue_learn),
+                        int(rescue_payoff))
         dc._keep = (st, tab)
         return dc
 

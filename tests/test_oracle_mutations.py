@@ -5,7 +5,8 @@ into a temporary directory, on the CPU), then runs the trace comparisons of
 test_oracle_walkers.py against the mutated build: at least one must disagree.
 The planted mistakes are the ones a restatement of the two search drivers could
 plausibly make (ring order, a dropped repeat loop, a truncating snap, a tie rule,
-the fixed-centre pricing, the rescue walk's centre).
+the fixed-centre pricing, the rescue walk's centre, the rescue learning window's
+count and latch).
 """
 import os
 import shutil
@@ -16,7 +17,7 @@ import pytest
 
 from oracle.oracle import Oracle
 from tests import refimpl
-from tests.test_oracle_walkers import CASES, _clip, tie_corpus
+from tests.test_oracle_walkers import CASES, _clip, decision_walk, latch_clip, tie_corpus
 
 SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
 
@@ -38,6 +39,15 @@ MUTATIONS = [
     ("rescue walk keeps its centre", "if (cc < bc) { bc = cc; bx = x; by = y; }\n                }\n            if (events)",
      "if (cc < bc) { bc = cc; gbx = x; gby = y; }\n                }\n            bx = gbx; by = gby;\n"
      "            if (events)", 0, "ms"),
+    ("window counts searches that did not rescue", "if (ev[0] && resc_n < dc->rescue_learn) {",
+     "if (resc_n < dc->rescue_learn) {", 0, "fr"),
+    ("window closes on payoff <=", "resc_big * 100 < dc->rescue_learn * dc->rescue_payoff",
+     "resc_big * 100 <= dc->rescue_learn * dc->rescue_payoff", 0, "fr"),
+    ("window tallies the closing search's payoff after its check",
+     "                    if (ev[1]) resc_big++;\n                    if (resc_n == dc->rescue_learn && resc_big * 100 < "
+     "dc->rescue_learn * dc->rescue_payoff)\n                        resc_off = 1;",
+     "                    if (resc_n == dc->rescue_learn && resc_big * 100 < dc->rescue_learn * dc->rescue_payoff)\n"
+     "                        resc_off = 1;\n                    if (ev[1]) resc_big++;", 0, "fr"),
     ("inter_cost ring order", RING, RING_SWAPPED, 1, "ic"),
     ("inter_cost rung walked once", "                if (!moved) break;", "                break;", 0, "ic"),
     ("inter_cost without the early out", "    if (best != 0) {\n        if (seedx", "    if (1) {\n        if (seedx", 0,
@@ -104,14 +114,33 @@ def _ic_agrees(O):
     return True
 
 
+def _fr_agrees(O):
+    """The frame decision's learning window against decision_walk (latch_clip)."""
+    cw, ch, lam = 96, 64, 4.0
+    cur, ref = latch_clip(cw, ch, 1, (10, -13))
+    kw = dict(fc=True, rescue=True, rescue_range=12, rescue_thresh=0)
+    cfg = O.cfg(cw, ch, 32, 16, 16, 32, 0, lam)
+    pl = [O.planes(ref, cw, ch, 32)]
+    # (10, 30): 3 of 10 paid, exactly the threshold (stays open); (9, 20): the 9th
+    # search's payoff keeps it open
+    for learn, payoff in ((4, 50), (14, 50), (13, 10), (12, 0), (16, 7), (10, 30), (9, 20)):
+        dec, _ = decision_walk(cur, [ref], lam, 0.0, learn, payoff, kw)
+        _, mv, cost = O.motion_search_frame(cfg, cur, pl, rescue_learn=learn, rescue_payoff=payoff, **kw)
+        for (x, y), (_, m, c) in dec.items():
+            i = y * (cw // 16) + x
+            if (int(mv[i, 0]), int(mv[i, 1])) != tuple(m) or int(cost[i]) != c:
+                return False
+    return True
+
+
 def test_unmutated_copy_agrees(tmp_path):
     """The copy-and-compile route itself changes nothing."""
     O = _mutated(tmp_path, "/* 1. seeds */", "/* 1. seeds */", 0)
-    assert _ms_agrees(O) and _ic_agrees(O)
+    assert _ms_agrees(O) and _ic_agrees(O) and _fr_agrees(O)
 
 
 @pytest.mark.parametrize("name,text,repl,which,driver", MUTATIONS, ids=[m[0] for m in MUTATIONS])
 def test_planted_mistake_is_caught(tmp_path, name, text, repl, which, driver):
     O = _mutated(tmp_path, text, repl, which)
-    agrees = _ms_agrees(O) if driver == "ms" else _ic_agrees(O)
+    agrees = {"ms": _ms_agrees, "ic": _ic_agrees, "fr": _fr_agrees}[driver](O)
     assert not agrees, f"the walkers did not notice: {name}"

@@ -315,3 +315,95 @@ def test_partition_decision_walk(nrefs, qp, p8x8, kw, pat):
     assert np.array_equal(got, want), np.argwhere((got != want).any(1))[:5]
     assert set(got[:, 0].tolist()) == ({0, 1, 2, 3} if p8x8 else {0, 1, 2}), np.bincount(got[:, 0])
     assert set(got[:, 2::3].ravel().tolist()) == set(range(nrefs))
+
+
+# ---- the frame decision's me_wide learning window (mb16.rs:2868-2886) ----
+def decision_walk(cur, refs, lam, lambda_ref, learn, payoff, kw):
+    """The 16x16 frame decision in raster order, each macroblock's searches in
+    reference order, composed from the independent search walker and predict_mv; the
+    rescue learning window restated: the first `learn` searches whose grid ran are
+    tallied (those that paid counted), and a window that fills with fewer than
+    `payoff` % paid turns the rescue off for the rest of the frame."""
+    ch, cw = cur.shape
+    nbx, nby = cw // 16, ch // 16
+    dec = {}
+    n = big = 0
+    off = False
+    for y in range(nby):
+        for x in range(nbx):
+            def nb(xx, yy):
+                if 0 <= xx < nbx and 0 <= yy < nby and (xx, yy) in dec:
+                    r_, mv_ = dec[(xx, yy)][0], dec[(xx, yy)][1]
+                    return (True, mv_, r_)
+                return (False, (0, 0), -1)
+            a, b = nb(x - 1, y), nb(x, y - 1)
+            c = nb(x + 1, y - 1) if (y > 0 and x + 1 < nbx) else nb(x - 1, y - 1)
+            best = None
+            for r, ref in enumerate(refs):
+                pr = _predict_mv(a, b, c, r)
+                mv, cost, (ran, paid), _ = refimpl.motion_search_walk(cur, ref, 16 * x, 16 * y, 16, 16, [pr], lam,
+                                                                      rescue=kw["rescue"] and not off, **{
+                                                                          k: v for k, v in kw.items() if k != "rescue"})
+                if ran and n < learn:
+                    n += 1
+                    big += bool(paid)
+                    if n == learn and big * 100 < learn * payoff:
+                        off = True
+                cost += int(lambda_ref * float(2 * int(np.floor(np.log2(r + 1))) + 1)) if len(refs) > 2 else \
+                    int(lambda_ref * float(1 if len(refs) == 2 else 0))
+                if best is None or cost < best[2]:
+                    best = (r, mv, cost)
+            dec[(x, y)] = best
+    return dec, (n, big, off)
+
+
+def latch_clip(cw, ch, seed, shift):
+    """Top half: the reference, plus heavy noise in the left half (every rescue there
+    runs and none pays; the clean right half runs none); bottom half: a textured field
+    moved by `shift`, more than the diamond finds from (0, 0), so the rescue grid pays
+    there -- the learning window decides which of the bottom macroblocks still get it."""
+    rng = np.random.default_rng(seed)
+    m = 24
+    base = rng.integers(0, 256, (ch + 2 * m, cw + 2 * m)).astype(np.float64)
+    for _ in range(2):
+        base = (base + np.roll(base, 1, 0) + np.roll(base, 1, 1) + np.roll(base, (1, 1), (0, 1))) / 4
+    base = base * 2.0 - 128
+    ref = np.clip(base[m:m + ch, m:m + cw], 0, 255)
+    dy, dx = shift
+    cur = base[m - dy:m - dy + ch, m - dx:m - dx + cw].copy()
+    top = ch // 2
+    cur[:top] = ref[:top]
+    cur[:top, :cw // 2] += rng.normal(0, 10, (top, cw // 2))
+    return np.clip(cur, 0, 255).astype(np.uint8), ref.astype(np.uint8)
+
+
+@pytest.mark.parametrize("learn,payoff,nrefs", [(4, 50, 1), (14, 50, 1), (13, 10, 1), (4, 0, 1), (10, 50, 2),
+                                                (1000, 15, 1)])
+def test_decision_rescue_window(learn, payoff, nrefs):
+    """orc_motion_search_frame with the rescue == the walk above, on a clip where the
+    window matters: it closes on the top half's unpaid rescues (learn 4), after the
+    first paid ones of the bottom half (learn 14 and 13: 1 paid, under 50 % and 10 %),
+    never closes (payoff 0) or never fills (learn 1000); with two references both
+    searches of every macroblock count."""
+    rng = np.random.default_rng(learn + payoff)
+    cw, ch = 96, 64
+    cur, ref0 = latch_clip(cw, ch, 1, (10, -13))
+    refs = [ref0] + [np.clip(ref0.astype(int) + rng.integers(-6, 7, ref0.shape), 0, 255).astype(np.uint8)
+                     for _ in range(nrefs - 1)]
+    lam = 4.0
+    kw = dict(fc=True, rescue=True, rescue_range=12, rescue_thresh=0)
+    dec, (n, big, off) = decision_walk(cur, refs, lam, 0.0, learn, payoff, kw)
+    cfg = O.cfg(cw, ch, 32, 16, 16, 32, 0, lam)
+    pls = [O.planes(r, cw, ch, 32) for r in refs]
+    ref, mv, cost = O.motion_search_frame(cfg, cur, pls, lambda_ref=0.0, rescue_learn=learn, rescue_payoff=payoff, **kw)
+    nbx = cw // 16
+    for (x, y), (r, m, c) in dec.items():
+        i = y * nbx + x
+        assert (int(ref[i]), (int(mv[i, 0]), int(mv[i, 1])), int(cost[i])) == (r, tuple(m), c), (x, y)
+    # the window's outcome, and that it shows in the vectors
+    _, mv_open, _ = O.motion_search_frame(cfg, cur, pls, lambda_ref=0.0, rescue_learn=1 << 30, **kw)
+    print("window", learn, payoff, nrefs, "ran", n, "paid", big, "off", off, flush=True)
+    if off:
+        assert (mv != mv_open).any(), "a closed window must change the decision on this clip"
+    else:
+        assert (mv == mv_open).all()
