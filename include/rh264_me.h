@@ -334,6 +334,8 @@ typedef struct {
     double lambda;         /* mv_cost 1: lambda_me */
     double lambda_fp;      /* mv_cost 1: the B2 lambda, lambda_me * RFF_ME_SADL */
     const uint16_t *mv_tab; /* mv_cost 1: 4096 entries, device memory */
+    int32_t rescue_learn;  /* me_learn (default 40): rh264_motion_search_wavefront's learning window */
+    int32_t rescue_payoff; /* me_payoff_pct (default 15), percent */
 } rh264_diamond;
 int rh264_motion_search(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc, int nframes,
                         const uint8_t *cur, size_t cur_stride, const uint8_t *planes, size_t planes_stride,
@@ -358,10 +360,14 @@ void rh264_mv_cost_smooth(uint16_t *tab /* 4096 */);
  * until none does (the encoder-order result is the only fixed point; this
  * schedule synchronises the stream).
  * Outputs per macroblock: ref, mv (pair) and cost, int32.  16x16 only; nrefs
- * 1..4.  The search knobs are rh264_motion_search's, except the rescue: the
- * reference disables it for the rest of a frame from a learning window over its
- * first rescues in encoder order (mb16.rs:2870-2886), a sequential latch this
- * decision does not model, so dc->rescue declines. */
+ * 1..4.  The search knobs are rh264_motion_search's.  With dc->rescue the frame's
+ * learning window applies (mb16.rs:2868-2886): the first dc->rescue_learn searches
+ * of the frame (macroblock, then reference order) whose rescue grid ran are
+ * tallied, with those that cut their cost to <= 15/16; a window that fills under
+ * dc->rescue_payoff % of them turns the rescue off for every later search of the
+ * frame.  A search's switch depends only on earlier searches' events, so the
+ * fixed-point schedule (always used with the rescue) also re-searches the
+ * macroblocks whose switch changed, from a per-frame scan of the events each pass. */
 int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, const rh264_diamond *dc,
                                   int nframes, int nrefs, double lambda_ref, const uint8_t *cur, size_t cur_stride,
                                   const uint8_t *planes, size_t planes_stride, int32_t *ref, int32_t *mv,

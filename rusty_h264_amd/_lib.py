@@ -33,7 +33,8 @@ class Diamond(C.Structure):
     _fields_ = [("steps", C.c_int32 * 8), ("nsteps", C.c_int32), ("sadfp", C.c_int32), ("snap", C.c_int32),
                 ("fc", C.c_int32), ("sad_only", C.c_int32), ("mv_cost", C.c_int32), ("rescue", C.c_int32),
                 ("rescue_range", C.c_int32), ("rescue_thresh", C.c_int32), ("rescue_fast", C.c_int32),
-                ("rescue_var", C.c_int32), ("lam", C.c_double), ("lam_fp", C.c_double), ("mv_tab", C.c_void_p)]
+                ("rescue_var", C.c_int32), ("lam", C.c_double), ("lam_fp", C.c_double), ("mv_tab", C.c_void_p),
+                ("rescue_learn", C.c_int32), ("rescue_payoff", C.c_int32)]
 
 
 class Stream(C.Structure):

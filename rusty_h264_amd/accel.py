@@ -397,13 +397,14 @@ class DiamondCfg:
 
     def __init__(self, dev, steps=(16, 8, 4), sadfp=False, snap=True, fc=False, sad_only=False, mv_tab=None,
                  lam=0.0, lam_fp=0.0, rescue=False, rescue_range=24, rescue_thresh=3, rescue_fast=True,
-                 rescue_var=800):
+                 rescue_var=800, rescue_learn=40, rescue_payoff=15):
         dc = _lib.Diamond()
         for i, st in enumerate(steps):
             dc.steps[i] = int(st)
         dc.nsteps, dc.sadfp, dc.snap, dc.fc, dc.sad_only = len(steps), int(sadfp), int(snap), int(fc), int(sad_only)
         dc.rescue, dc.rescue_range, dc.rescue_thresh = int(rescue), int(rescue_range), int(rescue_thresh)
         dc.rescue_fast, dc.rescue_var = int(rescue_fast), int(rescue_var)
+        dc.rescue_learn, dc.rescue_payoff = int(rescue_learn), int(rescue_payoff)
         dc.lam, dc.lam_fp = float(lam), float(lam_fp)
         self.tab = None
         if mv_tab is not None:

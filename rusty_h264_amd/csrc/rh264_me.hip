@@ -2152,10 +2152,14 @@ __global__ __launch_bounds__(256) void k_ms_wavefront(rh264_search_params p, rh2
 // Check: one thread per macroblock of every frame, predictors from S for each
 // reference; a changed macroblock stores its new predictors in used[i][r] and
 // joins the work list (one atomic per wave).
+// With the rescue (usedf non-null) a search's switch joins its inputs: on unless the
+// frame's learning window closed before it (fstate[f] = {order index of the search
+// that filled the window, or INT32_MAX; closed}); usedf[i][r] keeps the switch used.
 __global__ __launch_bounds__(256) void k_ms_check(int nbx, int nblk, int ntot, int nrefs,
                                                   const int32_t *__restrict__ ref, const int32_t *__restrict__ mv,
                                                   int32_t *__restrict__ used, int32_t *__restrict__ list,
-                                                  uint32_t *__restrict__ count)
+                                                  uint32_t *__restrict__ count, int32_t *__restrict__ usedf,
+                                                  const int32_t *__restrict__ fstate)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     bool dirty = false;
@@ -2169,6 +2173,13 @@ __global__ __launch_bounds__(256) void k_ms_check(int nbx, int nblk, int ntot, i
                 u[0] = px;
                 u[1] = py;
                 dirty = true;
+            }
+            if (usedf) {
+                const int on = !(fstate[2 * frame + 1] && b * nrefs + r > fstate[2 * frame]);
+                if (usedf[(size_t)i * 4 + r] != on) {
+                    usedf[(size_t)i * 4 + r] = on;
+                    dirty = true;
+                }
             }
         }
     }
@@ -2193,7 +2204,8 @@ __global__ __launch_bounds__(256) void k_ms_listed(rh264_search_params p, rh264_
                                                    int nrefs, MsRefCost refc, const int32_t *__restrict__ used,
                                                    const int32_t *__restrict__ list, const uint32_t *__restrict__ count,
                                                    uint32_t *__restrict__ next_count, int32_t *__restrict__ out_ref,
-                                                   int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost)
+                                                   int32_t *__restrict__ out_mv, int32_t *__restrict__ out_cost,
+                                                   const int32_t *__restrict__ usedf, int32_t *__restrict__ events)
 {
     __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
     __shared__ int32_t s_res[4][3];
@@ -2214,9 +2226,12 @@ __global__ __launch_bounds__(256) void k_ms_listed(rh264_search_params p, rh264_
         const int frame = i / nblk, b = i - frame * nblk;
         if (ok && rf < nrefs) {
             int bx, by, bc, ev;
-            ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
+            rh264_diamond d = dc;
+            d.rescue = usedf ? __builtin_amdgcn_readfirstlane(usedf[(size_t)i * 4 + rf]) : 0;
+            ms_search<T8>(p, d, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
                           planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, (b % nbx) << 4, (b / nbx) << 4,
                           16, 16, used + (size_t)i * 8 + 2 * rf, 1, lane, bx, by, bc, ev);
+            if (events && lane == 0) events[(size_t)i * 4 + rf] = ev;
             bc += refc.c[rf];
             if constexpr (RW == 1) {
                 if (lane == 0) {
@@ -2244,6 +2259,54 @@ __global__ __launch_bounds__(256) void k_ms_listed(rh264_search_params p, rh264_
             }
             __syncthreads();
         }
+    }
+}
+
+// The frame's rescue learning window from the searches' events (mb16.rs:2868-2886),
+// one wave per frame over the searches in order (macroblock, then reference): the
+// search whose grid ran for the learn-th time fills the window, which closes when
+// fewer than payoff % of those learn searches cut their cost to <= 15/16.  fstate[f]
+// = {order index of that search, or INT32_MAX when the window does not fill; closed}.
+__global__ void k_ms_latch_init(int nframes, int32_t *__restrict__ fstate)
+{
+    const int f = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (f < nframes) {
+        fstate[2 * f] = INT32_MAX;
+        fstate[2 * f + 1] = 0;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_ms_latch(int nblk, int nrefs, int learn, int payoff,
+                                                 const int32_t *__restrict__ events, int32_t *__restrict__ fstate)
+{
+    const int f = blockIdx.x, lane = threadIdx.x;
+    const int total = nblk * nrefs;
+    int n = 0, big = 0, pstar = INT32_MAX, closed = 0;
+    if (learn > 0)
+        for (int base = 0; base < total; base += 64) { // wave-uniform
+            const int idx = base + lane;
+            const int e = idx < total ? events[((size_t)f * nblk + idx / nrefs) * 4 + idx % nrefs] : 0;
+            const bool ran = e & 1, paid = ran && (e & 2);
+            const uint64_t mr = __ballot(ran), mp = __ballot(paid);
+            const int cnt = __popcll(mr);
+            if (n + cnt >= learn) {
+                // the (learn - n)-th search of this chunk whose grid ran
+                const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mr >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)mr, 0u));
+                const uint64_t hit = __ballot(ran && below == learn - n - 1);
+                const int p = __ffsll((long long)hit) - 1;
+                const uint64_t upto = p == 63 ? ~0ull : ((1ull << (p + 1)) - 1);
+                big += __popcll(mp & upto);
+                pstar = base + p;
+                closed = big * 100 < learn * payoff;
+                break;
+            }
+            n += cnt;
+            big += __popcll(mp);
+        }
+    if (lane == 0) {
+        fstate[2 * f] = pstar;
+        fstate[2 * f + 1] = closed;
     }
 }
 
@@ -4646,9 +4709,12 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
     if (rc) return rc;
     rc = check_diamond(dc, "motion_search_wavefront");
     if (rc) return rc;
-    if (p->bw != 16 || p->bh != 16 || dc->rescue) return RH264_DECLINED;
+    if (p->bw != 16 || p->bh != 16) return RH264_DECLINED;
     if (nrefs < 1 || nrefs > 4) return RH264_DECLINED;
     if (!(lambda_ref >= 0.0 && lambda_ref < 1e8)) return fail("motion_search_wavefront: lambda_ref %g", lambda_ref);
+    const bool resc = dc->rescue && !dc->sad_only;
+    if (resc && (dc->rescue_learn < 0 || dc->rescue_payoff < 0 || dc->rescue_payoff > 100))
+        return fail("motion_search_wavefront: rescue window %d / payoff %d", dc->rescue_learn, dc->rescue_payoff);
     if (!ref || !mv || !cost) return fail("motion_search_wavefront: null output");
     rc = check_strides(cur_stride, planes_stride);
     if (rc) return rc;
@@ -4661,16 +4727,31 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
     const bool t8 = p->hadamard == 8;
     // mode 2 picks by measurement at 1080p: the fixed point for one or two
     // references, where few passes settle it, the anti-diagonals for more
-    if (ctx->decide_mode == 1 || (ctx->decide_mode == 2 && nrefs <= 2)) {
+    if (resc || ctx->decide_mode == 1 || (ctx->decide_mode == 2 && nrefs <= 2)) {
         // fixed-point passes: S starts at (ref 0, mv 0) everywhere and no predictor
         // matches the sentinel, so the first pass searches every macroblock
         const size_t ntot = (size_t)nframes * nbx * nby;
         if (ntot > (size_t)INT32_MAX / 8) return fail("motion_search_wavefront: %zu macroblocks", ntot);
-        rc = ensure_buf(ctx, &ctx->ms_buf, &ctx->ms_cap, ntot * 9 * 4 + 256);
+        // used [ntot][8] | list [ntot] | counts [2] (+pad) | with the rescue: usedf
+        // [ntot][4] | events [ntot][4] | fstate [nframes][2]
+        const size_t base_words = ntot * 9 + 64;
+        const size_t words = base_words + (resc ? ntot * 8 + (size_t)nframes * 2 : 0);
+        rc = ensure_buf(ctx, &ctx->ms_buf, &ctx->ms_cap, words * 4);
         if (rc) return rc;
         int32_t *used = (int32_t *)ctx->ms_buf, *list = used + ntot * 8;
         uint32_t *counts = (uint32_t *)(list + ntot); // [2], alternating between passes
+        int32_t *usedf = resc ? (int32_t *)ctx->ms_buf + base_words : nullptr;
+        int32_t *events = resc ? usedf + ntot * 4 : nullptr, *fstate = resc ? events + ntot * 4 : nullptr;
         const unsigned fg = (unsigned)((ntot * 8 + 255) / 256), ng = (unsigned)((ntot + 255) / 256);
+        if (resc) {
+            // switches unknown (a sentinel: every first check differs), no events, every
+            // frame's window open
+            hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, ctx->stream, (unsigned *)usedf, ntot * 4,
+                               0x7fffffffu);
+            hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, ctx->stream, (unsigned *)events, ntot * 4, 0u);
+            hipLaunchKernelGGL(k_ms_latch_init, dim3((unsigned)((nframes + 255) / 256)), dim3(256), 0, ctx->stream,
+                               nframes, fstate);
+        }
         hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, ctx->stream, (unsigned *)used, ntot * 8, 0x80000000u);
         hipLaunchKernelGGL(k_fill_u32, dim3(ng), dim3(256), 0, ctx->stream, (unsigned *)ref, ntot, 0u);
         hipLaunchKernelGGL(k_fill_u32, dim3(2 * ng), dim3(256), 0, ctx->stream, (unsigned *)mv, ntot * 2, 0u);
@@ -4679,16 +4760,20 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
         // enough workgroups to fill the chip on the first (full) pass
         const unsigned sg = (unsigned)std::min<size_t>((ntot + per_wg - 1) / per_wg, 256 * 16);
         const int chunk = 8;
-        for (int k = 0; k <= ndiag;) {
-            for (int c = 0; c < chunk && k <= ndiag; c++, k++) {
+        // the anti-diagonals bound the passes; with the rescue a switch depends on every
+        // earlier search of its frame, which bounds them by the searches in a frame
+        const long kmax = ndiag + (resc ? (long)nbx * nby * nrefs : 0);
+        for (long k = 0; k <= kmax;) {
+            for (int c = 0; c < chunk && k <= kmax; c++, k++) {
                 uint32_t *cnt = counts + (k & 1), *nxt = counts + ((k + 1) & 1);
                 hipLaunchKernelGGL(k_ms_check, dim3(ng), dim3(256), 0, ctx->stream, nbx, nbx * nby, (int)ntot, nrefs,
-                                   (const int32_t *)ref, (const int32_t *)mv, used, list, cnt);
+                                   (const int32_t *)ref, (const int32_t *)mv, used, list, cnt, usedf,
+                                   (const int32_t *)fstate);
                 LAUNCH_CHECK("k_ms_check");
 #define MS_FP_LAUNCH(T8, RW)                                                                                          \
     hipLaunchKernelGGL((k_ms_listed<T8, RW>), dim3(sg), dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes,   \
                        planes_stride, nrefs, refc, (const int32_t *)used, (const int32_t *)list,                        \
-                       (const uint32_t *)cnt, nxt, ref, mv, cost)
+                       (const uint32_t *)cnt, nxt, ref, mv, cost, (const int32_t *)usedf, events)
                 if (rw == 1) {
                     if (t8) MS_FP_LAUNCH(true, 1); else MS_FP_LAUNCH(false, 1);
                 } else if (rw == 2) {
@@ -4698,6 +4783,11 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
                 }
 #undef MS_FP_LAUNCH
                 LAUNCH_CHECK("k_ms_listed");
+                if (resc) {
+                    hipLaunchKernelGGL(k_ms_latch, dim3((unsigned)nframes), dim3(64), 0, ctx->stream, nbx * nby, nrefs,
+                                       dc->rescue_learn, dc->rescue_payoff, (const int32_t *)events, fstate);
+                    LAUNCH_CHECK("k_ms_latch");
+                }
             }
             // the last check's count (the search clears only the other counter)
             uint32_t last = 1;
@@ -4705,7 +4795,7 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             if (last == 0) return 0;
         }
-        return fail("motion_search_wavefront: no fixed point after %d passes", ndiag + 1);
+        return fail("motion_search_wavefront: no fixed point after %ld passes", kmax + 1);
     }
     // anti-diagonals x + 2y = t: each macroblock's left, above and above-right
     // (or above-left) neighbours lie on earlier ones

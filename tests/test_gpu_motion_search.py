@@ -110,10 +110,10 @@ def test_motion_search_declines_and_rejects(dev):
                                 mv_tab=_smooth(), lam=-1.0)
     with pytest.raises(RuntimeError):
         accel.motion_search_ref(dev, accel.search_params(cw, ch, pad, 16, 16, 16, 0, LAM26), 1, cur, pl, fc=2)
-    # the decision does not model the rescue's frame-level latch
-    with pytest.raises(accel.Declined):
+    # a learning window's payoff is a percentage
+    with pytest.raises(RuntimeError):
         accel.motion_search_wavefront(dev, accel.search_params(cw, ch, pad, 16, 16, 16, 0, LAM26), 1, cur, pl,
-                                      rescue=True)
+                                      rescue=True, rescue_payoff=101)
     for x in (cur, ref, pl):
         x.free()
 
@@ -151,6 +151,87 @@ def test_decision_default_knobs(dev, nrefs, kw, pat, mode):
     for f in range(nf):
         pls = [opl[f + nrefs - 1 - r] for r in range(nrefs)]
         wr, wm, wc = O.motion_search_frame(cfg, clip[nrefs + f], pls, lambda_ref=lam_ref, **kw)
+        assert np.array_equal(ref[f], wr) and np.array_equal(mv[f], wm) and np.array_equal(cost[f], wc), f
+    for x in tuple(out) + (cur, refs, pl):
+        x.free()
+
+
+@pytest.mark.parametrize("learn,payoff,nrefs", [(4, 50, 1), (14, 50, 1), (9, 20, 1), (10, 30, 1), (1000, 15, 1),
+                                                (10, 50, 2), (6, 15, 3)])
+def test_decision_rescue_window(dev, learn, payoff, nrefs):
+    """The encoder-order decision with the me_wide rescue and its learning window ==
+    the oracle's raster decision, on two frames of the designed clip (a noisy static
+    top half whose rescues do not pay, a far-moving bottom half where they do) in one
+    call, each frame with its own window; from 1 to 3 references."""
+    from tests.test_oracle_walkers import latch_clip
+    cw, ch, pad, nf = 96, 64, 32, 2
+    kw = dict(fc=True, rescue=True, rescue_range=12, rescue_thresh=0, rescue_learn=learn, rescue_payoff=payoff)
+    rng = np.random.default_rng(nrefs)
+    curs, frames = [], []
+    for f in range(nf):
+        cur, ref0 = latch_clip(cw, ch, 1 + f, (10, -13) if f == 0 else (-11, 9))
+        older = [np.clip(ref0.astype(int) + rng.integers(-6, 7, ref0.shape), 0, 255).astype(np.uint8)
+                 for _ in range(nrefs - 1)]
+        curs.append(cur)
+        frames.append(older[::-1] + [ref0])  # oldest first: reference r = frames[f][nrefs - 1 - r]
+    # frame f's plane sets f .. f + nrefs - 1 must be its references: one plane set per
+    # (frame, reference) slot, laid out so that set f + nrefs - 1 - r is reference r
+    print("rescue window", learn, payoff, nrefs, flush=True)
+    lam = 4.0
+    p = accel.search_params(cw, ch, pad, 16, 16, 32, 0, lam)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 32, 0, lam)
+    nb = cw // 16 * (ch // 16)
+    for f in range(nf):  # one frame per call keeps the plane layout simple; then both at once below
+        planes = np.stack(frames[f])
+        pl_in = dev.upload(planes)
+        pl = accel.build_planes(dev, pl_in, nrefs, cw, ch, pad)
+        cur = dev.upload(curs[f])
+        out = accel.motion_search_wavefront(dev, p, 1, cur, pl, nrefs=nrefs, lambda_ref=0.0, **kw)
+        ref = dev.download(out[0], (nb,), np.int32)
+        mv = dev.download(out[1], (nb, 2), np.int32)
+        cost = dev.download(out[2], (nb,), np.int32)
+        pls = [O.planes(frames[f][nrefs - 1 - r], cw, ch, pad) for r in range(nrefs)]
+        wr, wm, wc = O.motion_search_frame(cfg, curs[f], pls, lambda_ref=0.0, **kw)
+        assert np.array_equal(ref, wr) and np.array_equal(mv, wm) and np.array_equal(cost, wc), f
+        for x in tuple(out) + (cur, pl, pl_in):
+            x.free()
+    # both frames in one call (one reference): a consecutive clip whose frame 1 is
+    # frame 0's current picture, so each frame keeps its own window
+    if nrefs == 1:
+        clip = np.stack([frames[0][0], curs[0], curs[1]])
+        cur = dev.upload(clip[1:])
+        refs = dev.upload(clip[:-1])
+        pl = accel.build_planes(dev, refs, 2, cw, ch, pad)
+        out = accel.motion_search_wavefront(dev, p, 2, cur, pl, nrefs=1, lambda_ref=0.0, **kw)
+        mv = dev.download(out[1], (2, nb, 2), np.int32)
+        cost = dev.download(out[2], (2, nb), np.int32)
+        for f in range(2):
+            _, wm, wc = O.motion_search_frame(cfg, clip[f + 1], [O.planes(clip[f], cw, ch, pad)], **kw)
+            assert np.array_equal(mv[f], wm) and np.array_equal(cost[f], wc), ("batch", f)
+        for x in tuple(out) + (cur, refs, pl):
+            x.free()
+
+
+def test_decision_rescue_cif(dev):
+    """The decision with the reference's Quality-preset rescue knobs (me_range 24,
+    me_rescue 3, me_fast, a 40-search window at 15 %) on a CIF clip, two references."""
+    cw, ch, pad, nf, nrefs = 352, 288, 32, 2, 2
+    clip = synth.cif_clip(nf + nrefs, seed=77, w=cw, h=ch)
+    cur, refs = dev.upload(clip[nrefs:]), dev.upload(clip[:-1])
+    pl = accel.build_planes(dev, refs, nf + nrefs - 1, cw, ch, pad)
+    p = accel.search_params(cw, ch, pad, 16, 16, 16, 0, LAM26)
+    kw = dict(fc=True, rescue=True, lam=LAM26, lam_fp=LAM26 * 0.5)
+    print("rescue cif", flush=True)
+    out = accel.motion_search_wavefront(dev, p, nf, cur, pl, nrefs=nrefs, lambda_ref=3.2, **kw)
+    nb = (cw // 16) * (ch // 16)
+    ref = dev.download(out[0], (nf, nb), np.int32)
+    mv = dev.download(out[1], (nf, nb, 2), np.int32)
+    cost = dev.download(out[2], (nf, nb), np.int32)
+    cfg = O.cfg(cw, ch, pad, 16, 16, 16, 0, LAM26)
+    opl = [O.planes(clip[i], cw, ch, pad) for i in range(nf + nrefs - 1)]
+    for f in range(nf):
+        pls = [opl[f + nrefs - 1 - r] for r in range(nrefs)]
+        wr, wm, wc = O.motion_search_frame(cfg, clip[nrefs + f], pls, lambda_ref=3.2, fc=True, rescue=True)
         assert np.array_equal(ref[f], wr) and np.array_equal(mv[f], wm) and np.array_equal(cost[f], wc), f
     for x in tuple(out) + (cur, refs, pl):
         x.free()
