@@ -387,7 +387,10 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
  * 8893-8948); the first strict minimum of c16, ct + cb, cl + cr and
  * (int64)(lambda_ref * 4) + the quadrants' sum.  lambda_ref = the search lambda
  * (it prices ref_bits and the P_8x8 syntax).  The search knobs are
- * rh264_motion_search's (16x16 in p; the rescue and the 8x8 transform decline).
+ * rh264_motion_search's (16x16 in p; the 8x8 transform declines); with dc->rescue
+ * the frame's learning window of rh264_motion_search_wavefront counts every search in
+ * call order: the 16x16 references, then each partition's (16x8 top, bottom, 8x16
+ * left, right, 8x8 TL, TR, BL, BR) references.
  * planes as rh264_motion_search_wavefront.  Decided by fixed-point passes (the
  * stream is synchronised every 8 passes).  rec: 14 int32 per macroblock, {mode (0
  * 16x16, 1 16x8, 2 8x16, 3 8x8), cost, then ref, mvx, mvy of each quadrant TL, TR,

@@ -1029,22 +1029,40 @@ static void pd_quad_nb(const int32_t *rec, int nbx, int x, int y, int q, int32_t
     out[3] = r[0];
 }
 
+/* the frame's rescue learning window (mb16.rs:2868-2886) over its searches in call order */
+typedef struct {
+    int n, big, off;
+} resc_window;
+
+static void window_tally(resc_window *w, const orc_diamond_cfg *dc, const int32_t ev[2])
+{
+    if (ev[0] && w->n < dc->rescue_learn) {
+        w->n++;
+        if (ev[1]) w->big++;
+        if (w->n == dc->rescue_learn && w->big * 100 < dc->rescue_learn * dc->rescue_payoff) w->off = 1;
+    }
+}
+
 static int64_t pd_best_part(const orc_search_cfg *cfg0, const orc_diamond_cfg *dc, const uint8_t *cur,
                             const uint8_t *const *planes, int nrefs, double lme, const int32_t nb[12], int lx, int ly,
-                            int bw, int bh, const int32_t *extra, int nextra, int32_t *ref_out, int32_t *mv_out)
+                            int bw, int bh, const int32_t *extra, int nextra, int32_t *ref_out, int32_t *mv_out,
+                            resc_window *w)
 {
     orc_search_cfg cfg = *cfg0;
     cfg.bw = bw;
     cfg.bh = bh;
+    orc_diamond_cfg d = *dc;
     int64_t bc = INT64_MAX;
     for (int r = 0; r < nrefs; r++) {
-        int32_t seeds[8], m[2];
+        int32_t seeds[8], m[2], ev[2] = {0, 0};
         orc_predict_mv(nb, r, seeds);
         for (int e = 0; e < nextra; e++) {
             seeds[2 + 2 * e] = extra[2 * e];
             seeds[3 + 2 * e] = extra[2 * e + 1];
         }
-        int64_t c = orc_motion_search_ex(&cfg, dc, cur, planes[r], lx, ly, seeds, 1 + nextra, m, NULL, NULL, 0, NULL);
+        d.rescue = dc->rescue && !w->off;
+        int64_t c = orc_motion_search_ex(&cfg, &d, cur, planes[r], lx, ly, seeds, 1 + nextra, m, ev, NULL, 0, NULL);
+        window_tally(w, dc, ev);
         c += (int64_t)(lme * (double)ref_bits_orc(r, nrefs));
         if (c < bc) { bc = c; *ref_out = r; mv_out[0] = m[0]; mv_out[1] = m[1]; }
     }
@@ -1056,6 +1074,7 @@ void orc_partition_decision_frame(const orc_search_cfg *cfg, const orc_diamond_c
                                   int32_t *rec)
 {
     const int nbx = cfg->cw / 16, nby = cfg->ch / 16;
+    resc_window w = {0, 0, 0};
     for (int y = 0; y < nby; y++)
         for (int x = 0; x < nbx; x++) {
             int32_t nb[12] = {0, 0, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1};
@@ -1065,7 +1084,8 @@ void orc_partition_decision_frame(const orc_search_cfg *cfg, const orc_diamond_c
             else if (x > 0 && y > 0) pd_quad_nb(rec, nbx, x - 1, y - 1, 3, nb + 8);
             const int lx = 16 * x, ly = 16 * y;
             int32_t r16 = 0, mv16[2] = {0, 0};
-            const int64_t c16 = pd_best_part(cfg, dc, cur, planes, nrefs, lme, nb, lx, ly, 16, 16, NULL, 0, &r16, mv16);
+            const int64_t c16 =
+                pd_best_part(cfg, dc, cur, planes, nrefs, lme, nb, lx, ly, 16, 16, NULL, 0, &r16, mv16, &w);
             int mode = 0;
             int64_t best = c16;
             int32_t qr[4] = {r16, r16, r16, r16}, qm[4][2];
@@ -1079,7 +1099,7 @@ void orc_partition_decision_frame(const orc_search_cfg *cfg, const orc_diamond_c
                 const int nparts = p8x8 ? 8 : 4;
                 for (int k = 0; k < nparts; k++)
                     pc[k] = pd_best_part(cfg, dc, cur, planes, nrefs, lme, nb, lx + geo[k][0], ly + geo[k][1],
-                                         geo[k][2], geo[k][3], mv16, 1, &pr[k], pm[k]);
+                                         geo[k][2], geo[k][3], mv16, 1, &pr[k], pm[k], &w);
                 if (pc[0] + pc[1] < best) {
                     best = pc[0] + pc[1];
                     mode = 1;

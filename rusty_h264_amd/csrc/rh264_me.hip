@@ -2276,16 +2276,18 @@ __global__ void k_ms_latch_init(int nframes, int32_t *__restrict__ fstate)
     }
 }
 
-__global__ __launch_bounds__(64) void k_ms_latch(int nblk, int nrefs, int learn, int payoff,
+// slots: searches per macroblock (their order inside it), stride: event words per
+// macroblock.
+__global__ __launch_bounds__(64) void k_ms_latch(int nblk, int slots, int stride, int learn, int payoff,
                                                  const int32_t *__restrict__ events, int32_t *__restrict__ fstate)
 {
     const int f = blockIdx.x, lane = threadIdx.x;
-    const int total = nblk * nrefs;
+    const int total = nblk * slots;
     int n = 0, big = 0, pstar = INT32_MAX, closed = 0;
     if (learn > 0)
         for (int base = 0; base < total; base += 64) { // wave-uniform
             const int idx = base + lane;
-            const int e = idx < total ? events[((size_t)f * nblk + idx / nrefs) * 4 + idx % nrefs] : 0;
+            const int e = idx < total ? events[((size_t)f * nblk + idx / slots) * stride + idx % slots] : 0;
             const bool ran = e & 1, paid = ran && (e & 2);
             const uint64_t mr = __ballot(ran), mp = __ballot(paid);
             const int cnt = __popcll(mr);
@@ -2347,9 +2349,14 @@ __device__ __forceinline__ void pd_predict_mv(const int32_t *rec, int nbx, int x
 }
 
 // Fixed-point check, as k_ms_check, with the predictors from the quadrant grid.
+// With the rescue (usedf non-null) every search slot of the macroblock (PD_SLOTS words
+// each: the 16x16 references, then each partition's references) keeps its switch, as in
+// k_ms_check; slots = the slots in use.
+constexpr int PD_SLOTS = 36; // (1 + 8 partitions) x 4 references
 __global__ __launch_bounds__(256) void k_pd_check(int nbx, int nblk, int ntot, int nrefs, const int32_t *__restrict__ rec,
                                                   int32_t *__restrict__ used, int32_t *__restrict__ list,
-                                                  uint32_t *__restrict__ count)
+                                                  uint32_t *__restrict__ count, int slots, int32_t *__restrict__ usedf,
+                                                  const int32_t *__restrict__ fstate)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     bool dirty = false;
@@ -2365,6 +2372,14 @@ __global__ __launch_bounds__(256) void k_pd_check(int nbx, int nblk, int ntot, i
                 dirty = true;
             }
         }
+        if (usedf)
+            for (int sl = 0; sl < slots; sl++) {
+                const int on = !(fstate[2 * frame + 1] && b * slots + sl > fstate[2 * frame]);
+                if (usedf[(size_t)i * PD_SLOTS + sl] != on) {
+                    usedf[(size_t)i * PD_SLOTS + sl] = on;
+                    dirty = true;
+                }
+            }
     }
     const uint64_t m = __ballot(dirty);
     if (m == 0) return;
@@ -2391,7 +2406,8 @@ __global__ __launch_bounds__(256) void k_pd_listed(rh264_search_params p, rh264_
                                                    const uint8_t *__restrict__ planes, size_t planes_stride, PdArgs a,
                                                    const int32_t *__restrict__ used, const int32_t *__restrict__ list,
                                                    const uint32_t *__restrict__ count, uint32_t *__restrict__ next_count,
-                                                   int32_t *__restrict__ rec)
+                                                   int32_t *__restrict__ rec, const int32_t *__restrict__ usedf,
+                                                   int32_t *__restrict__ events)
 {
     __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
     __shared__ int32_t s16[4][3], sp[8][4][3];
@@ -2416,8 +2432,11 @@ __global__ __launch_bounds__(256) void k_pd_listed(rh264_search_params p, rh264_
         const int32_t *u = used + (size_t)i * 8;
         if (wave < nrefs) {
             int bx, by, bc, ev;
-            ms_search<T8>(p, dc, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - wave) * planes_stride, lx, ly,
+            rh264_diamond d = dc;
+            d.rescue = usedf ? __builtin_amdgcn_readfirstlane(usedf[(size_t)i * PD_SLOTS + wave]) : 0;
+            ms_search<T8>(p, d, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - wave) * planes_stride, lx, ly,
                           16, 16, u + 2 * wave, 1, lane, bx, by, bc, ev);
+            if (events && lane == 0) events[(size_t)i * PD_SLOTS + wave] = ev;
             if (lane == 0) {
                 s16[wave][0] = bx;
                 s16[wave][1] = by;
@@ -2437,15 +2456,20 @@ __global__ __launch_bounds__(256) void k_pd_listed(rh264_search_params p, rh264_
                 // seeds: the predictor against rf (pricing the rate), then the 16x16 vector
                 const int32_t prl[4] = {u[2 * rf], u[2 * rf + 1], mvx16, mvy16};
                 int bx, by, bc, ev;
-                ms_search<T8>(p, dc, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride,
+                rh264_diamond d = dc;
+                d.rescue = usedf ? __builtin_amdgcn_readfirstlane(usedf[(size_t)i * PD_SLOTS + nrefs + j]) : 0;
+                ms_search<T8>(p, d, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride,
                               lx + geo[part][0], ly + geo[part][1], geo[part][2], geo[part][3], prl, 2, lane, bx, by,
                               bc, ev);
+                if (events && lane == 0) events[(size_t)i * PD_SLOTS + nrefs + j] = ev;
                 if (lane == 0) {
                     sp[part][rf][0] = bx;
                     sp[part][rf][1] = by;
                     sp[part][rf][2] = bc + a.refc[rf];
                 }
             }
+        } else if (events && (int)threadIdx.x < (a.p8x8 ? 8 : 4) * nrefs) {
+            events[(size_t)i * PD_SLOTS + nrefs + threadIdx.x] = 0; // no partition search ran this pass
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -4785,7 +4809,7 @@ int rh264_motion_search_wavefront(rh264_ctx *ctx, const rh264_search_params *p, 
                 LAUNCH_CHECK("k_ms_listed");
                 if (resc) {
                     hipLaunchKernelGGL(k_ms_latch, dim3((unsigned)nframes), dim3(64), 0, ctx->stream, nbx * nby, nrefs,
-                                       dc->rescue_learn, dc->rescue_payoff, (const int32_t *)events, fstate);
+                                       4, dc->rescue_learn, dc->rescue_payoff, (const int32_t *)events, fstate);
                     LAUNCH_CHECK("k_ms_latch");
                 }
             }
@@ -4824,9 +4848,12 @@ int rh264_partition_decision(rh264_ctx *ctx, const rh264_search_params *p, const
     if (rc) return rc;
     rc = check_diamond(dc, "partition_decision");
     if (rc) return rc;
-    if (p->bw != 16 || p->bh != 16 || dc->rescue || p->hadamard == 8) return RH264_DECLINED;
+    if (p->bw != 16 || p->bh != 16 || p->hadamard == 8) return RH264_DECLINED;
     if (nrefs < 1 || nrefs > 4) return RH264_DECLINED;
     if (!(lambda_ref >= 0.0 && lambda_ref < 1e6)) return fail("partition_decision: lambda %g", lambda_ref);
+    const bool resc = dc->rescue && !dc->sad_only;
+    if (resc && (dc->rescue_learn < 0 || dc->rescue_payoff < 0 || dc->rescue_payoff > 100))
+        return fail("partition_decision: rescue window %d / payoff %d", dc->rescue_learn, dc->rescue_payoff);
     if (p8x8 != 0 && p8x8 != 1) return fail("partition_decision: p8x8 %d", p8x8);
     if (!rec) return fail("partition_decision: null output");
     rc = check_strides(cur_stride, planes_stride);
@@ -4841,11 +4868,26 @@ int rh264_partition_decision(rh264_ctx *ctx, const rh264_search_params *p, const
     for (int r = 0; r < nrefs; r++) a.refc[r] = (int32_t)(int64_t)(lambda_ref * (double)ref_bits(r, nrefs));
     const int nbx = p->cw / 16, nby = p->ch / 16, ndiag = (nbx - 1) + 2 * (nby - 1) + 1;
     const size_t ntot = (size_t)nframes * nbx * nby;
-    if (ntot > (size_t)INT32_MAX / 16) return fail("partition_decision: %zu macroblocks", ntot);
-    rc = ensure_buf(ctx, &ctx->ms_buf, &ctx->ms_cap, ntot * 9 * 4 + 256);
+    if (ntot > (size_t)INT32_MAX / PD_SLOTS) return fail("partition_decision: %zu macroblocks", ntot);
+    // used [ntot][8] | list [ntot] | counts [2] (+pad) | with the rescue: usedf and
+    // events [ntot][PD_SLOTS] | fstate [nframes][2]
+    const size_t base_words = ntot * 9 + 64;
+    const size_t words = base_words + (resc ? ntot * 2 * PD_SLOTS + (size_t)nframes * 2 : 0);
+    rc = ensure_buf(ctx, &ctx->ms_buf, &ctx->ms_cap, words * 4);
     if (rc) return rc;
     int32_t *used = (int32_t *)ctx->ms_buf, *list = used + ntot * 8;
     uint32_t *counts = (uint32_t *)(list + ntot);
+    int32_t *usedf = resc ? (int32_t *)ctx->ms_buf + base_words : nullptr;
+    int32_t *events = resc ? usedf + ntot * PD_SLOTS : nullptr, *fstate = resc ? events + ntot * PD_SLOTS : nullptr;
+    const int slots = (1 + (p8x8 ? 8 : 4)) * nrefs;
+    if (resc) {
+        const unsigned sg2 = (unsigned)((ntot * PD_SLOTS + 255) / 256);
+        hipLaunchKernelGGL(k_fill_u32, dim3(sg2), dim3(256), 0, ctx->stream, (unsigned *)usedf, ntot * PD_SLOTS,
+                           0x7fffffffu);
+        hipLaunchKernelGGL(k_fill_u32, dim3(sg2), dim3(256), 0, ctx->stream, (unsigned *)events, ntot * PD_SLOTS, 0u);
+        hipLaunchKernelGGL(k_ms_latch_init, dim3((unsigned)((nframes + 255) / 256)), dim3(256), 0, ctx->stream,
+                           nframes, fstate);
+    }
     const unsigned fg = (unsigned)((ntot * 8 + 255) / 256), rg = (unsigned)((ntot * PD_REC + 255) / 256);
     const unsigned ng = (unsigned)((ntot + 255) / 256);
     // S starts at (16x16, ref 0, mv 0) everywhere; no predictor matches the sentinel,
@@ -4856,23 +4898,29 @@ int rh264_partition_decision(rh264_ctx *ctx, const rh264_search_params *p, const
     LAUNCH_CHECK("k_fill_u32");
     const unsigned sg = (unsigned)std::min<size_t>(ntot, 256 * 16); // one macroblock per workgroup
     const int chunk = 8;
-    for (int k = 0; k <= ndiag;) {
-        for (int c = 0; c < chunk && k <= ndiag; c++, k++) {
+    const long kmax = ndiag + (resc ? (long)nbx * nby * slots : 0);
+    for (long k = 0; k <= kmax;) {
+        for (int c = 0; c < chunk && k <= kmax; c++, k++) {
             uint32_t *cnt = counts + (k & 1), *nxt = counts + ((k + 1) & 1);
             hipLaunchKernelGGL(k_pd_check, dim3(ng), dim3(256), 0, ctx->stream, nbx, nbx * nby, (int)ntot, nrefs,
-                               (const int32_t *)rec, used, list, cnt);
+                               (const int32_t *)rec, used, list, cnt, slots, usedf, (const int32_t *)fstate);
             LAUNCH_CHECK("k_pd_check");
             hipLaunchKernelGGL(k_pd_listed<false>, dim3(sg), dim3(256), 0, ctx->stream, *p, *dc, cur, cur_stride, planes,
                                planes_stride, a, (const int32_t *)used, (const int32_t *)list, (const uint32_t *)cnt,
-                               nxt, rec);
+                               nxt, rec, (const int32_t *)usedf, events);
             LAUNCH_CHECK("k_pd_listed");
+            if (resc) {
+                hipLaunchKernelGGL(k_ms_latch, dim3((unsigned)nframes), dim3(64), 0, ctx->stream, nbx * nby, slots,
+                                   PD_SLOTS, dc->rescue_learn, dc->rescue_payoff, (const int32_t *)events, fstate);
+                LAUNCH_CHECK("k_ms_latch");
+            }
         }
         uint32_t last = 1;
         HIP_TRY(hipMemcpyAsync(&last, counts + ((k - 1) & 1), 4, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (last == 0) return 0;
     }
-    return fail("partition_decision: no fixed point after %d passes", ndiag + 1);
+    return fail("partition_decision: no fixed point after %ld passes", kmax + 1);
 }
 
 int32_t rh264_split_gate(int qp)

@@ -80,3 +80,31 @@ def test_partition_decision_batch_of_frames(dev):
     print("modes", np.bincount(got[..., 0].ravel(), minlength=4), flush=True)
     for x in (dcur, drefs, pl, out):
         x.free()
+
+
+@pytest.mark.parametrize("learn,payoff,nrefs", [(6, 50, 1), (20, 40, 1), (8, 60, 2), (1000, 15, 2)])
+def test_partition_decision_rescue_window(dev, learn, payoff, nrefs):
+    """The decision with partitions and the me_wide rescue under the frame's learning
+    window (every search in call order) == the oracle, on the designed clip where the
+    window changes the decisions."""
+    from tests.test_oracle_walkers import latch_clip
+    cw, ch, qp = 96, 64, 20
+    cur, ref0 = latch_clip(cw, ch, 3, (10, -13))
+    rng = np.random.default_rng(nrefs)
+    refs = [ref0] + [np.clip(ref0.astype(int) + rng.integers(-6, 7, ref0.shape), 0, 255).astype(np.uint8)
+                     for _ in range(nrefs - 1)]
+    dref = dev.upload(np.stack(refs[::-1]))
+    pl = accel.build_planes(dev, dref, nrefs, cw, ch, PAD)
+    dcur = dev.upload(cur)
+    lam, gate = 4.0, accel.split_gate(qp)
+    p = accel.search_params(cw, ch, PAD, 16, 16, 16, 0, lam)
+    kw = dict(fc=True, rescue=True, rescue_range=12, rescue_thresh=0, rescue_learn=learn, rescue_payoff=payoff)
+    print("partition rescue window", learn, payoff, nrefs, flush=True)
+    out = accel.partition_decision(dev, p, 1, dcur, pl, nrefs=nrefs, lambda_ref=lam, gate=gate, p8x8=True, **kw)
+    got = dev.download(out, ((cw // 16) * (ch // 16), 14), np.int32)
+    cfg = O.cfg(cw, ch, PAD, 16, 16, 16, 0, lam)
+    want = O.partition_decision_frame(cfg, cur, [O.planes(r, cw, ch, PAD) for r in refs], lam, gate, True, **kw)
+    bad = np.argwhere((got != want).any(1)).ravel()
+    assert bad.size == 0, (bad[:5], got[bad[:2]], want[bad[:2]])
+    for x in (dref, pl, dcur, out):
+        x.free()

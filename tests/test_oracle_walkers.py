@@ -216,14 +216,24 @@ def partition_decision_walk(cur, planes, cfg, lme, gate, p8x8, kw):
         return (True, tuple(int(v) for v in grid_mv[by, bx]), int(grid_ref[by, bx]))
 
     refbits = lambda r, n: 0 if n <= 1 else 1 if n == 2 else 2 * (r + 1).bit_length() - 1
+    # the rescue's learning window over the frame's searches in call order
+    kw = dict(kw)
+    learn, payoff = kw.pop("rescue_learn", 40), kw.pop("rescue_payoff", 15)
+    rescue = kw.pop("rescue", False)
+    win = {"n": 0, "big": 0, "off": False}
 
     def best_part(nb, lx, ly, bw, bh, extra):
-        from oracle.oracle import Oracle  # noqa: F401  (O is module level)
         c2 = O.cfg(cw, ch, 32, bw, bh, cfg.range, cfg.subpel_pat, cfg._lam, lam_fp=cfg._lam_fp)
         best = None
         for r, pl in enumerate(planes):
             seeds = [_predict_mv(*nb, r)] + extra
-            mv, c = O.motion_search_ref(c2, cur, pl, lx, ly, seeds, **kw)
+            mv, c, (ran, paid), _ = O.motion_search_trace(c2, cur, pl, lx, ly, seeds, cap=0,
+                                                          rescue=rescue and not win["off"], **kw)
+            if ran and win["n"] < learn:
+                win["n"] += 1
+                win["big"] += bool(paid)
+                if win["n"] == learn and win["big"] * 100 < learn * payoff:
+                    win["off"] = True
             c += int(lme * refbits(r, len(planes)))
             if best is None or c < best[0]:
                 best = (c, r, mv)
@@ -315,6 +325,29 @@ def test_partition_decision_walk(nrefs, qp, p8x8, kw, pat):
     assert np.array_equal(got, want), np.argwhere((got != want).any(1))[:5]
     assert set(got[:, 0].tolist()) == ({0, 1, 2, 3} if p8x8 else {0, 1, 2}), np.bincount(got[:, 0])
     assert set(got[:, 2::3].ravel().tolist()) == set(range(nrefs))
+
+
+@pytest.mark.parametrize("learn,payoff,nrefs", [(6, 50, 1), (20, 40, 1), (1000, 15, 2), (8, 60, 2)])
+def test_partition_decision_rescue_window(learn, payoff, nrefs):
+    """The partition decision with the rescue: the window counts every search in call
+    order (the 16x16 references, then 16x8, 8x16 and the 8x8 quadrants) == the walk."""
+    cw, ch, qp = 96, 64, 20
+    cur, ref0 = latch_clip(cw, ch, 3, (10, -13))
+    rng = np.random.default_rng(nrefs)
+    frames = [ref0] + [np.clip(ref0.astype(int) + rng.integers(-6, 7, ref0.shape), 0, 255).astype(np.uint8)
+                       for _ in range(nrefs - 1)]
+    planes = [O.planes(f, cw, ch, 32) for f in frames]
+    lam = 4.0
+    cfg = O.cfg(cw, ch, 32, 16, 16, 16, 0, lam)
+    gate = O.split_gate(qp)
+    kw = dict(fc=True, rescue=True, rescue_range=12, rescue_thresh=0, rescue_learn=learn, rescue_payoff=payoff)
+    got = O.partition_decision_frame(cfg, cur, planes, lam, gate, True, **kw)
+    want = partition_decision_walk(cur, planes, cfg, lam, gate, True, kw)
+    assert np.array_equal(got, want), np.argwhere((got != want).any(1))[:5]
+    assert (got[:, 0] > 0).any()  # some macroblocks split: their searches are in the window
+    if learn < 1000:  # the window closes on this clip and changes decisions
+        assert (got != O.partition_decision_frame(cfg, cur, planes, lam, gate, True,
+                                                  **dict(kw, rescue_learn=1 << 30))).any()
 
 
 # ---- the frame decision's me_wide learning window (mb16.rs:2868-2886) ----
