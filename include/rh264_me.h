@@ -454,6 +454,13 @@ int rh264_grain_signature(const uint8_t *cur, const uint8_t *ref, int cw, int ch
  * centres).  A batch is prepared once (tables uploaded, scratch allocated), then
  * run any number of times, directly or as a captured HIP graph.
  *
+ * When the batch holds the 4x4 shape and at least one other of 16x16, 16x8, 8x16,
+ * 8x8, every stream is whole macroblocks (cw, ch multiples of 16) and every lr_fp
+ * entry is at most 2^21, those shapes share one integer-search launch instead: it
+ * computes each 4x4 block's SADs once per candidate and adds them into the larger
+ * partitions' SADs, pricing every partition with its own predictor.  The records are
+ * the same as with one search per shape.
+ *
  * Per stream: planes = nrefs plane sets back to back (reference 0 first); mvps =
  * per-shape block predictors in shape order (rh264_batch_mvp_len int32 pairs
  * total) or NULL; out = per-shape records {ref, mvx, mvy, cost} int32, shapes in
@@ -472,8 +479,12 @@ int rh264_batch_create(rh264_ctx *ctx, const rh264_stream *streams, int nstreams
                        int nshapes, int pad, int range, int subpel_pat, const int32_t *lr_fp,
                        const int32_t *lr, double lambda_ref, rh264_batch **out);
 int rh264_batch_run(rh264_ctx *ctx, rh264_batch *b);
+/* The integer searches of one run alone (no refinement, no reference choice; the
+ * records are not written): for timing that phase.  No reference counterpart. */
+int rh264_batch_search(rh264_ctx *ctx, rh264_batch *b);
 /* Times every kernel variant x scheduling choice of each shape's integer search
- * (reps runs each) and keeps the fastest; results are identical for all. */
+ * (reps runs each) and keeps the fastest; results are identical for all.  Shapes
+ * that search fused are left out: rh264_batch_choice reports variant 3 for them. */
 int rh264_batch_autotune(rh264_ctx *ctx, rh264_batch *b, int reps);
 int rh264_batch_choice(rh264_batch *b, int shape_index, int *variant, int *persist);
 /* Capture one rh264_batch_run into a HIP graph; replay launches it. */

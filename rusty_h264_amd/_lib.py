@@ -136,6 +136,7 @@ PROTOTYPES = {
     "rh264_batch_create": (C.c_int, [c_vp, C.POINTER(Stream), C.c_int, c_i32p, C.c_int, C.c_int, C.c_int, C.c_int,
                                      c_i32p, c_i32p, C.c_double, C.POINTER(c_vp)]),
     "rh264_batch_run": (C.c_int, [c_vp, c_vp]),
+    "rh264_batch_search": (C.c_int, [c_vp, c_vp]),
     "rh264_batch_autotune": (C.c_int, [c_vp, c_vp, C.c_int]),
     "rh264_batch_choice": (C.c_int, [c_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "rh264_batch_capture": (C.c_int, [c_vp, c_vp]),

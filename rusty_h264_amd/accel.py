@@ -719,6 +719,10 @@ class Batch:
     def run(self):
         _check(self.dev.lib.rh264_batch_run(self.dev.ctx, self.h), "rh264_batch_run")
 
+    def search(self):
+        """The integer searches alone (for timing that phase)."""
+        _check(self.dev.lib.rh264_batch_search(self.dev.ctx, self.h), "rh264_batch_search")
+
     def autotune(self, reps=2):
         _check(self.dev.lib.rh264_batch_autotune(self.dev.ctx, self.h, reps), "rh264_batch_autotune")
         out = []

@@ -2777,6 +2777,249 @@ void k_fs_multi(const SegDesc *__restrict__ segs, int nseg, long total_lanes, in
     }
 }
 
+// ---------------------------------------------------------------------------
+// fused integer search of the macroblock shapes
+// ---------------------------------------------------------------------------
+//
+// The 16x16, 16x8, 8x16, 8x8 and 4x4 searches of a batch scan the same +-R window of
+// the same macroblocks, so one pass serves all of them.  A row of 16 lanes owns one
+// unit (macroblock, column group g); lane sb of the row owns the 4x4 sub-block whose
+// position bits are sb = x0 | y0 << 1 | x1 << 2 | y1 << 3 (x0, y0: the 4x4 inside its
+// 8x8 quadrant; x1, y1: the quadrant).  Each lane computes its sub-block's quad-SADs
+// for candidate columns 4g..4g+3 row by row, as the 4x4 search does.  When a
+// candidate row completes, DPP adds inside the 16-lane row form the larger
+// partitions' SADs: the xor-1 and xor-2 quad permutations sum a quadrant (8x8), the
+// half-row mirror pairs quadrants 0-1 and 2-3 (16x8), rotation by 8 pairs 0-2 and
+// 1-3 (8x16), and both together give the 16x16.  The four u16 SADs of a lane sit two
+// to a dword and no partial sum exceeds 256 * 255 < 2^16, so plain dword adds never
+// carry from one into the other.  Every lane prices its 4x4 block; nine lanes also
+// price one larger partition (sb 0: the 16x16; sb 1, 5, 9, 13: the 8x8 of their
+// quadrant; sb 2 / 10: the top / bottom 16x8; sb 3 / 7: the left / right 8x16).
+// Costs, predictors and the tie order are those of each shape's own search (SAD +
+// lr_fp[bits(x) + bits(y)], first minimum in raster candidate order), so every
+// shape's keys equal what its own k_fs_multi launch writes.  A lane keeps its two
+// running minima as 32-bit keys cost << 10 | t << 2 | k (t: candidate row, k: the
+// lane's column), which needs cost < 2^22: a batch fuses only when every lr_fp entry
+// is at most 2^21.  Columns past 2R (the last group's lanes) price against the
+// sentinel entries 128.. of the LDS table (2^21 + 2^16, above every real cost).
+
+constexpr int FU_NS = 5;               // fused shapes, in this order: 16x16, 16x8, 8x16, 8x8, 4x4
+constexpr int FU_LR_MAX = 1 << 21;     // largest lr_fp entry a fused batch accepts
+constexpr int FU_SENTINEL = (1 << 21) + (1 << 16);
+
+struct FuSeg {                         // one (stream, reference) of a fused batch
+    const uint8_t *cur;
+    const uint8_t *planes;
+    const int32_t *mvps[FU_NS];        // the stream's block predictors per shape, or null
+    int64_t unit0;                     // first unit of the segment
+    int64_t blk0[FU_NS];               // first block of the segment in each shape's keys; -1: not searched
+    int32_t cw, ch, mbw, pad;
+};
+
+struct FuKeys {
+    uint64_t *k[FU_NS];
+};
+
+struct FuLane {
+    uint32_t src[4];
+    uint64_t acc[4];
+    uint32_t bestA, bestB;             // 4x4 block; the lane's larger partition
+    int rxA[4], rxB[4];                // 4 x column bits per candidate column (4 x 128: past 2R)
+    int mpyA, mpyB;
+    uint32_t mh, mv, md;               // all-ones where the lane's larger partition adds the quadrant beside, below, diagonal
+    const uint8_t *row;                // next reference row (the aligned dword left of the lane's window)
+};
+
+template <bool MVP>
+__device__ __forceinline__ void fu_price(FuLane &L, uint64_t a4, int t, int R, const int32_t *s_lr)
+{
+    const uint32_t lo = (uint32_t)a4, hi = (uint32_t)(a4 >> 32);
+    uint32_t l8 = lo + (uint32_t)__builtin_amdgcn_mov_dpp((int)lo, 0xB1, 0xF, 0xF, true);
+    uint32_t h8 = hi + (uint32_t)__builtin_amdgcn_mov_dpp((int)hi, 0xB1, 0xF, 0xF, true);
+    l8 += (uint32_t)__builtin_amdgcn_mov_dpp((int)l8, 0x4E, 0xF, 0xF, true); // the lane's quadrant (8x8)
+    h8 += (uint32_t)__builtin_amdgcn_mov_dpp((int)h8, 0x4E, 0xF, 0xF, true);
+    // the quadrant beside (half-row mirror), below / above (rotate by 8), diagonal
+    const uint32_t lh = (uint32_t)__builtin_amdgcn_mov_dpp((int)l8, 0x141, 0xF, 0xF, true);
+    const uint32_t hh = (uint32_t)__builtin_amdgcn_mov_dpp((int)h8, 0x141, 0xF, 0xF, true);
+    const uint32_t lv = (uint32_t)__builtin_amdgcn_mov_dpp((int)l8, 0x128, 0xF, 0xF, true);
+    const uint32_t hv = (uint32_t)__builtin_amdgcn_mov_dpp((int)h8, 0x128, 0xF, 0xF, true);
+    const uint32_t ld = (uint32_t)__builtin_amdgcn_mov_dpp((int)lh, 0x128, 0xF, 0xF, true);
+    const uint32_t hd = (uint32_t)__builtin_amdgcn_mov_dpp((int)hh, 0x128, 0xF, 0xF, true);
+    const uint32_t lB = l8 + (lh & L.mh) + (lv & L.mv) + (ld & L.md);
+    const uint32_t hB = h8 + (hh & L.mh) + (hv & L.mv) + (hd & L.md);
+    const int ryA = 4 * mvbits(4 * (t - R) - L.mpyA);
+    const int ryB = MVP ? 4 * mvbits(4 * (t - R) - L.mpyB) : ryA;
+    const char *lr = (const char *)s_lr; // byte offsets: 4 x (bits(x) + bits(y))
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t tk = ((uint32_t)t << 2) | (uint32_t)k;
+        const uint32_t sA = ((k < 2 ? lo : hi) >> (16 * (k & 1))) & 0xffffu;
+        const uint32_t sB = ((k < 2 ? lB : hB) >> (16 * (k & 1))) & 0xffffu;
+        const uint32_t lrA = *(const uint32_t *)(lr + L.rxA[k] + ryA);
+        const uint32_t lrB = MVP ? *(const uint32_t *)(lr + L.rxB[k] + ryB) : lrA;
+        L.bestA = min(L.bestA, ((sA + lrA) << 10) | tk);
+        L.bestB = min(L.bestB, ((sB + lrB) << 10) | tk);
+    }
+}
+
+// Rows jb .. jb+3 of the lane's window (MODE as in fs_rows: 1 = first ring block,
+// 2 = last, 0 = any).  FAST: every lane of the wave has its whole window inside the
+// padded plane and walks L.row down it; otherwise every lane reads clamped.
+template <bool MVP, int MODE, bool FAST>
+__device__ __forceinline__ void fu_rows(FuLane &L, const uint8_t *__restrict__ F, int pw, int ph, int px0, int py0,
+                                        int shift, int jb, int NJ, int R, const int32_t *s_lr)
+{
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const int j = jb + jj;
+        if (MODE == 0 && j >= NJ) break;
+        uint32_t w0, w1;
+        const int ry = py0 + j;
+        if (FAST) {
+            const uint32_t *rp = (const uint32_t *)L.row;
+            const uint32_t a0 = rp[0], a1 = rp[1], a2 = rp[2];
+            L.row += pw;
+            w0 = __builtin_amdgcn_alignbyte(a1, a0, shift);
+            w1 = __builtin_amdgcn_alignbyte(a2, a1, shift);
+        } else {
+            w0 = load4_clamped(F, pw, ph, px0, ry);
+            w1 = load4_clamped(F, pw, ph, px0 + 4, ry);
+        }
+        const uint64_t ref8 = ((uint64_t)w1 << 32) | w0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (MODE == 1 && k > jj) continue;
+            if (MODE == 2 && k < jj) continue;
+            const int slot = (jj - k + 4) % 4;
+            L.acc[slot] = __builtin_amdgcn_qsad_pk_u16_u8(ref8, L.src[k], L.acc[slot]);
+        }
+        const int t = j - 3;
+        const int sd = (jj + 1) % 4;
+        if (t >= 0) fu_price<MVP>(L, L.acc[sd], t, R, s_lr);
+        L.acc[sd] = 0;
+    }
+}
+
+template <bool MVP>
+__global__ __launch_bounds__(256) void k_fs_fused(const FuSeg *__restrict__ segs, int nseg, long total_units, int R,
+                                                  LrTab lr_fp, FuKeys keys)
+{
+    __shared__ int32_t s_lr[2 * RH264_LR_N];
+    if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = lr_fp.v[threadIdx.x];
+    else s_lr[threadIdx.x] = FU_SENTINEL; // 256 threads: entries 128..255
+    __syncthreads();
+    const int NC = 2 * R + 1, NG = (NC + 3) / 4;
+    const int lane = threadIdx.x & 63, sb = threadIdx.x & 15;
+    const long u0 = ((long)blockIdx.x * 256 + threadIdx.x) >> 4;
+    const bool inrange = u0 < total_units;
+    const long unit = inrange ? u0 : total_units - 1;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].unit0 <= unit) lo = mid; else hi = mid - 1;
+    }
+    const FuSeg &S = segs[lo];
+    const long local = unit - S.unit0;
+    const int mb = (int)(local / NG), g = (int)(local - (long)mb * NG);
+    const int mbw = S.mbw, mbx = mb % mbw, mby = mb / mbw;
+    const int x4 = (sb & 1) | ((sb >> 1) & 2), y4 = ((sb >> 1) & 1) | ((sb >> 2) & 2);
+    const int lx = 16 * mbx + 4 * x4, ly = 16 * mby + 4 * y4;
+    const int cw = S.cw, pad = S.pad, pw = cw + 2 * pad, ph = S.ch + 2 * pad;
+    // the lane's larger partition (shape index and block index in that shape's grid)
+    int shB = -1, bB = 0;
+    if (sb == 0) {
+        shB = 0;
+        bB = mb;
+    } else if ((sb & 3) == 1) {
+        shB = 3;
+        bB = (2 * mby + (sb >> 3)) * (2 * mbw) + 2 * mbx + ((sb >> 2) & 1);
+    } else if (sb == 2 || sb == 10) {
+        shB = 1;
+        bB = (2 * mby + (sb >> 3)) * mbw + mbx;
+    } else if (sb == 3 || sb == 7) {
+        shB = 2;
+        bB = mby * (2 * mbw) + 2 * mbx + (sb >> 2);
+    }
+    const int bA = (4 * mby + y4) * (4 * mbw) + 4 * mbx + x4;
+    FuLane L;
+    L.mh = (shB == 0 || shB == 1) ? ~0u : 0u;
+    L.mv = (shB == 0 || shB == 2) ? ~0u : 0u;
+    L.md = shB == 0 ? ~0u : 0u;
+    int mpxA = 0, mpxB = 0;
+    L.mpyA = 0;
+    L.mpyB = 0;
+    if (MVP) {
+        const int32_t *pa = S.mvps[FU_NS - 1];
+        if (pa) {
+            mpxA = pa[2 * bA];
+            L.mpyA = pa[2 * bA + 1];
+        }
+        const int32_t *pb = shB >= 0 ? S.mvps[shB] : nullptr;
+        if (pb) {
+            mpxB = pb[2 * bB];
+            L.mpyB = pb[2 * bB + 1];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) L.src[k] = *(const uint32_t *)(S.cur + (size_t)(ly + k) * cw + lx);
+    const int dx0 = -R + 4 * g;
+    const int px0 = lx + pad + dx0, py0 = ly + pad - R;
+    const int wx_lo = lx + pad - R, wx_hi = wx_lo + 2 * R + 3;
+    const bool fast = wx_lo >= 0 && wx_hi <= pw - 1 && py0 >= 0 && py0 + 2 * R + 3 <= ph - 1;
+    const int shift = px0 & 3;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool valid = 4 * g + k < NC;
+        L.rxA[k] = 4 * (valid ? mvbits(4 * (dx0 + k) - mpxA) : RH264_LR_N);
+        L.rxB[k] = 4 * (valid ? mvbits(4 * (dx0 + k) - mpxB) : RH264_LR_N);
+        L.acc[k] = 0;
+    }
+    L.bestA = 0xffffffffu;
+    L.bestB = 0xffffffffu;
+    const uint8_t *F = S.planes;
+    L.row = F + (size_t)py0 * pw + (px0 & ~3);
+    const int NJ = 2 * R + 4;
+    // one branch for the wave: the DPP sums need every lane on the same path
+    if (__all(fast)) {
+        if ((2 * R) % 4 == 0) {
+            fu_rows<MVP, 1, true>(L, F, pw, ph, px0, py0, shift, 0, NJ, R, s_lr);
+            for (int jb = 4; jb < NJ - 4; jb += 4) fu_rows<MVP, 0, true>(L, F, pw, ph, px0, py0, shift, jb, NJ, R, s_lr);
+            fu_rows<MVP, 2, true>(L, F, pw, ph, px0, py0, shift, NJ - 4, NJ, R, s_lr);
+        } else {
+            for (int jb = 0; jb < NJ; jb += 4) fu_rows<MVP, 0, true>(L, F, pw, ph, px0, py0, shift, jb, NJ, R, s_lr);
+        }
+    } else {
+        if ((2 * R) % 4 == 0) {
+            fu_rows<MVP, 1, false>(L, F, pw, ph, px0, py0, shift, 0, NJ, R, s_lr);
+            for (int jb = 4; jb < NJ - 4; jb += 4)
+                fu_rows<MVP, 0, false>(L, F, pw, ph, px0, py0, shift, jb, NJ, R, s_lr);
+            fu_rows<MVP, 2, false>(L, F, pw, ph, px0, py0, shift, NJ - 4, NJ, R, s_lr);
+        } else {
+            for (int jb = 0; jb < NJ; jb += 4) fu_rows<MVP, 0, false>(L, F, pw, ph, px0, py0, shift, jb, NJ, R, s_lr);
+        }
+    }
+    // the keys the separate searches write: cost << 32 | (t * NC + c)
+    uint64_t kA = ((uint64_t)(L.bestA >> 10) << 32) | (uint32_t)(((L.bestA >> 2) & 255u) * NC + 4 * g + (L.bestA & 3u));
+    uint64_t kB = ((uint64_t)(L.bestB >> 10) << 32) | (uint32_t)(((L.bestB >> 2) & 255u) * NC + 4 * g + (L.bestB & 3u));
+    // minimum over the wave's units of the same macroblock (segments start on whole
+    // macroblocks, so unit / NG numbers the macroblocks of the batch), then one
+    // atomic per macroblock run and partition
+    const int gmb = (int)(unit / NG);
+#pragma unroll
+    for (int off = 16; off < 64; off <<= 1) {
+        const uint64_t oA = __shfl_down(kA, off, 64), oB = __shfl_down(kB, off, 64);
+        const int og = __shfl_down(gmb, off, 64);
+        if (lane + off < 64 && og == gmb) {
+            kA = oA < kA ? oA : kA;
+            kB = oB < kB ? oB : kB;
+        }
+    }
+    const int up = __shfl_up(gmb, 16, 64);
+    if (!inrange || (lane >= 16 && up == gmb)) return;
+    if (S.blk0[FU_NS - 1] >= 0) atomicMin((unsigned long long *)(keys.k[FU_NS - 1] + S.blk0[FU_NS - 1] + bA), kA);
+    if (shB >= 0 && S.blk0[shB] >= 0) atomicMin((unsigned long long *)(keys.k[shB] + S.blk0[shB] + bB), kB);
+}
+
 // Refinement of every block of every segment from its integer result; writes
 // {mvx, mvy, cost} per block into res.
 template <int BW, int BH, bool USE_LDS>
@@ -5328,6 +5571,8 @@ struct BatchShape {
     SegDesc *d_segs;
     SelDesc *d_sels;
     int variant, persist; // tuned kernel choice: variant 0..2, persist 0 = one tile per block, k = k*4 blocks per CU
+    int fused;            // index among the fused shapes (16x16, 16x8, 8x16, 8x8, 4x4), -1: searched on its own
+    size_t koff;          // first key of the shape in the fused search's key array
 };
 
 struct rh264_batch {
@@ -5337,6 +5582,10 @@ struct rh264_batch {
     uint64_t *keys;
     int32_t *res;
     unsigned *counter;
+    FuSeg *d_fu;          // fused search: one descriptor per (stream, reference), or null
+    int fu_nseg, fu_mvp;
+    long fu_units;
+    size_t fu_keys;
     hipGraph_t graph;
     hipGraphExec_t exec;
     bool has_graph;
@@ -5354,6 +5603,7 @@ int rh264_batch_destroy(rh264_batch *b)
         if (S.d_sels) (void)hipFree(S.d_sels);
     }
     if (b->keys) (void)hipFree(b->keys);
+    if (b->d_fu) (void)hipFree(b->d_fu);
     if (b->res) (void)hipFree(b->res);
     if (b->counter) (void)hipFree(b->counter);
     delete b;
@@ -5386,6 +5636,11 @@ int rh264_batch_create(rh264_ctx *ctx, const rh264_stream *streams, int nstreams
     b->keys = nullptr;
     b->res = nullptr;
     b->counter = nullptr;
+    b->d_fu = nullptr;
+    b->fu_nseg = 0;
+    b->fu_mvp = 0;
+    b->fu_units = 0;
+    b->fu_keys = 0;
     b->has_graph = false;
     memcpy(b->lr_fp.v, lr_fp, sizeof(b->lr_fp.v));
     memcpy(b->lr.v, lr, sizeof(b->lr.v));
@@ -5406,6 +5661,8 @@ int rh264_batch_create(rh264_ctx *ctx, const rh264_stream *streams, int nstreams
         S.d_sels = nullptr;
         S.variant = 0;
         S.persist = 0;
+        S.fused = -1;
+        S.koff = 0;
         std::vector<SegDesc> segs;
         std::vector<SelDesc> sels;
         long lane = 0, blk = 0, sel = 0;
@@ -5480,6 +5737,77 @@ int rh264_batch_create(rh264_ctx *ctx, const rh264_stream *streams, int nstreams
         }
         max_blocks = blk > max_blocks ? blk : max_blocks;
     }
+    // The macroblock shapes search in one fused pass when the 4x4 and at least one
+    // other of them are in the batch, every frame is whole macroblocks and the rate
+    // table keeps the fused keys' costs below 2^22 (k_fs_fused).
+    const int fu_w[FU_NS] = {16, 16, 8, 8, 4}, fu_h[FU_NS] = {16, 8, 16, 8, 4};
+    int fu_idx[FU_NS], nfu = 0;
+    for (int f = 0; f < FU_NS; f++) {
+        fu_idx[f] = -1;
+        for (int s = 0; s < nshapes && fu_idx[f] < 0; s++)
+            if (shapes[2 * s] == fu_w[f] && shapes[2 * s + 1] == fu_h[f]) fu_idx[f] = s;
+        nfu += fu_idx[f] >= 0;
+    }
+    bool fuse = fu_idx[FU_NS - 1] >= 0 && nfu >= 2;
+    for (int i = 0; i < nstreams && fuse; i++) fuse = streams[i].cw % 16 == 0 && streams[i].ch % 16 == 0;
+    for (int i = 0; i < RH264_LR_N && fuse; i++) fuse = lr_fp[i] <= FU_LR_MAX;
+    size_t fu_total = 0;
+    if (fuse) {
+        for (int f = 0; f < FU_NS; f++)
+            if (fu_idx[f] >= 0) {
+                BatchShape &S = b->shapes[fu_idx[f]];
+                S.fused = f;
+                S.koff = fu_total;
+                fu_total += (size_t)S.blocks;
+            }
+        std::vector<FuSeg> fu;
+        long unit = 0;
+        int64_t blk[FU_NS] = {0, 0, 0, 0, 0};
+        const int ng = (NC + 3) / 4;
+        for (int i = 0; i < nstreams; i++) {
+            const rh264_stream &st = streams[i];
+            const size_t pbytes = rh264_planes_bytes(st.cw, st.ch, pad);
+            const int32_t *mv[FU_NS];
+            int nb[FU_NS];
+            for (int f = 0; f < FU_NS; f++) {
+                nb[f] = (st.cw / fu_w[f]) * (st.ch / fu_h[f]);
+                mv[f] = nullptr;
+                if (fu_idx[f] < 0 || !st.mvps) continue;
+                size_t m_off = 0;
+                for (int t = 0; t < fu_idx[f]; t++) m_off += 2 * (size_t)(st.cw / shapes[2 * t]) * (st.ch / shapes[2 * t + 1]);
+                mv[f] = st.mvps + m_off;
+                b->fu_mvp = 1;
+            }
+            for (int r = 0; r < st.nrefs; r++) {
+                FuSeg g;
+                memset(&g, 0, sizeof(g));
+                g.cur = st.cur;
+                g.planes = st.planes + (size_t)r * pbytes;
+                g.unit0 = unit;
+                g.cw = st.cw;
+                g.ch = st.ch;
+                g.mbw = st.cw / 16;
+                g.pad = pad;
+                for (int f = 0; f < FU_NS; f++) {
+                    g.mvps[f] = mv[f];
+                    g.blk0[f] = fu_idx[f] >= 0 ? blk[f] : -1;
+                    blk[f] += nb[f];
+                }
+                fu.push_back(g);
+                unit += (long)(st.cw / 16) * (st.ch / 16) * ng;
+            }
+        }
+        b->fu_nseg = (int)fu.size();
+        b->fu_units = unit;
+        b->fu_keys = fu_total;
+        hipError_t e = hipMalloc(&b->d_fu, fu.size() * sizeof(FuSeg));
+        if (e == hipSuccess) e = hipMemcpy(b->d_fu, fu.data(), fu.size() * sizeof(FuSeg), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            rh264_batch_destroy(b);
+            return fail("batch_create: %s", hipGetErrorString(e));
+        }
+        if ((long)fu_total > max_blocks) max_blocks = (long)fu_total;
+    }
     hipError_t e = hipMalloc(&b->keys, (size_t)max_blocks * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&b->res, (size_t)max_blocks * 3 * sizeof(int32_t));
     if (e == hipSuccess) e = hipMalloc(&b->counter, 64);
@@ -5506,25 +5834,30 @@ static void launch_fs_multi_v(rh264_ctx *ctx, rh264_batch *b, const BatchShape &
                        S.nseg, S.lanes[V], b->range, b->pad, b->lr_fp, b->keys, b->counter, S.persist);
 }
 
+// mode 0: integer search, refinement and reference choice; 1: the integer search only;
+// 2: refinement and reference choice from the fused search's keys.
 template <int BW, int BH>
-static int batch_shape_run(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S, bool search_only)
+static int batch_shape_run(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S, int mode)
 {
-    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((S.blocks + 255) / 256)), dim3(256), 0, ctx->stream, b->keys,
-                       (size_t)S.blocks, ~0ull);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, b->counter, (size_t)16, 0u);
-    if (S.variant == 1) launch_fs_multi_v<BW, BH, 1>(ctx, b, S);
-    else if (S.variant == 2) launch_fs_multi_v<BW, BH, 2>(ctx, b, S);
-    else launch_fs_multi_v<BW, BH, 0>(ctx, b, S);
-    LAUNCH_CHECK("k_fs_multi");
-    if (search_only) return 0;
+    const uint64_t *keys = mode == 2 ? b->keys + S.koff : b->keys;
+    if (mode != 2) {
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((S.blocks + 255) / 256)), dim3(256), 0, ctx->stream, b->keys,
+                           (size_t)S.blocks, ~0ull);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, b->counter, (size_t)16, 0u);
+        if (S.variant == 1) launch_fs_multi_v<BW, BH, 1>(ctx, b, S);
+        else if (S.variant == 2) launch_fs_multi_v<BW, BH, 2>(ctx, b, S);
+        else launch_fs_multi_v<BW, BH, 0>(ctx, b, S);
+        LAUNCH_CHECK("k_fs_multi");
+    }
+    if (mode == 1) return 0;
     constexpr int GPW = RefShape<BW, BH>::GPW;
     if (RefShape<BW, BH>::NSB >= 8 && ctx->refine_lds)
         hipLaunchKernelGGL((k_refine_multi<BW, BH, RefShape<BW, BH>::NSB >= 8>),
                            dim3((unsigned)((S.blocks + GPW - 1) / GPW)), dim3(256), 0, ctx->stream, S.d_segs, S.nseg,
-                           S.blocks, b->range, b->pad, b->pat, b->lr, b->keys, b->res);
+                           S.blocks, b->range, b->pad, b->pat, b->lr, keys, b->res);
     else
         hipLaunchKernelGGL((k_refine_multi<BW, BH, false>), dim3((unsigned)((S.blocks + GPW - 1) / GPW)), dim3(256), 0,
-                           ctx->stream, S.d_segs, S.nseg, S.blocks, b->range, b->pad, b->pat, b->lr, b->keys, b->res);
+                           ctx->stream, S.d_segs, S.nseg, S.blocks, b->range, b->pad, b->pat, b->lr, keys, b->res);
     LAUNCH_CHECK("k_refine_multi");
     hipLaunchKernelGGL(k_select_ref, dim3((unsigned)((S.sel_blocks + 255) / 256)), dim3(256), 0, ctx->stream, S.d_sels,
                        S.nsel, S.sel_blocks, b->res);
@@ -5532,18 +5865,37 @@ static int batch_shape_run(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S, 
     return 0;
 }
 
-static int batch_shape_dispatch(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S, bool search_only)
+static int batch_shape_dispatch(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S, int mode)
 {
     switch (S.bw * 100 + S.bh) {
-    case 1616: return batch_shape_run<16, 16>(ctx, b, S, search_only);
-    case 1608: return batch_shape_run<16, 8>(ctx, b, S, search_only);
-    case 816: return batch_shape_run<8, 16>(ctx, b, S, search_only);
-    case 808: return batch_shape_run<8, 8>(ctx, b, S, search_only);
-    case 804: return batch_shape_run<8, 4>(ctx, b, S, search_only);
-    case 408: return batch_shape_run<4, 8>(ctx, b, S, search_only);
-    case 404: return batch_shape_run<4, 4>(ctx, b, S, search_only);
+    case 1616: return batch_shape_run<16, 16>(ctx, b, S, mode);
+    case 1608: return batch_shape_run<16, 8>(ctx, b, S, mode);
+    case 816: return batch_shape_run<8, 16>(ctx, b, S, mode);
+    case 808: return batch_shape_run<8, 8>(ctx, b, S, mode);
+    case 804: return batch_shape_run<8, 4>(ctx, b, S, mode);
+    case 408: return batch_shape_run<4, 8>(ctx, b, S, mode);
+    case 404: return batch_shape_run<4, 4>(ctx, b, S, mode);
     default: return RH264_DECLINED;
     }
+}
+
+static int batch_fused_search(rh264_ctx *ctx, rh264_batch *b)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((b->fu_keys + 255) / 256)), dim3(256), 0, ctx->stream, b->keys,
+                       b->fu_keys, ~0ull);
+    FuKeys k;
+    for (int f = 0; f < FU_NS; f++) k.k[f] = nullptr;
+    for (auto &S : b->shapes)
+        if (S.fused >= 0) k.k[S.fused] = b->keys + S.koff;
+    const unsigned grid = (unsigned)((b->fu_units + 15) / 16); // 16 units of 16 lanes per workgroup
+    if (b->fu_mvp)
+        hipLaunchKernelGGL(k_fs_fused<true>, dim3(grid), dim3(256), 0, ctx->stream, b->d_fu, b->fu_nseg, b->fu_units,
+                           b->range, b->lr_fp, k);
+    else
+        hipLaunchKernelGGL(k_fs_fused<false>, dim3(grid), dim3(256), 0, ctx->stream, b->d_fu, b->fu_nseg, b->fu_units,
+                           b->range, b->lr_fp, k);
+    LAUNCH_CHECK("k_fs_fused");
+    return 0;
 }
 
 extern "C" {
@@ -5551,8 +5903,30 @@ extern "C" {
 int rh264_batch_run(rh264_ctx *ctx, rh264_batch *b)
 {
     if (!b) return fail("batch_run: null batch");
+    if (b->d_fu) {
+        int rc = batch_fused_search(ctx, b);
+        if (rc) return rc;
+        for (auto &S : b->shapes)
+            if (S.fused >= 0 && (rc = batch_shape_dispatch(ctx, b, S, 2))) return rc;
+    }
     for (auto &S : b->shapes) {
-        int rc = batch_shape_dispatch(ctx, b, S, false);
+        if (S.fused >= 0) continue;
+        int rc = batch_shape_dispatch(ctx, b, S, 0);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int rh264_batch_search(rh264_ctx *ctx, rh264_batch *b)
+{
+    if (!b) return fail("batch_search: null batch");
+    if (b->d_fu) {
+        int rc = batch_fused_search(ctx, b);
+        if (rc) return rc;
+    }
+    for (auto &S : b->shapes) {
+        if (S.fused >= 0) continue;
+        int rc = batch_shape_dispatch(ctx, b, S, 1);
         if (rc) return rc;
     }
     return 0;
@@ -5565,17 +5939,18 @@ int rh264_batch_autotune(rh264_ctx *ctx, rh264_batch *b, int reps)
     if (reps < 1) reps = 1;
     const int persists[4] = {0, 1, 2, 4};
     for (auto &S : b->shapes) {
+        if (S.fused >= 0) continue; // one fused launch, nothing to choose
         float best_ms = 1e30f;
         int bv = 0, bp = 0;
         for (int v = 0; v < 3; v++)
             for (int pi = 0; pi < 4; pi++) {
                 S.variant = v;
                 S.persist = persists[pi];
-                int rc = batch_shape_dispatch(ctx, b, S, true); // warm
+                int rc = batch_shape_dispatch(ctx, b, S, 1); // warm
                 if (rc) return rc;
                 HIP_TRY(hipEventRecord(ctx->ev[14], ctx->stream));
                 for (int k = 0; k < reps; k++) {
-                    rc = batch_shape_dispatch(ctx, b, S, true);
+                    rc = batch_shape_dispatch(ctx, b, S, 1);
                     if (rc) return rc;
                 }
                 HIP_TRY(hipEventRecord(ctx->ev[15], ctx->stream));
@@ -5597,8 +5972,9 @@ int rh264_batch_autotune(rh264_ctx *ctx, rh264_batch *b, int reps)
 int rh264_batch_choice(rh264_batch *b, int shape_index, int *variant, int *persist)
 {
     if (!b || shape_index < 0 || shape_index >= (int)b->shapes.size()) return fail("batch_choice: bad index");
-    *variant = b->shapes[shape_index].variant;
-    *persist = b->shapes[shape_index].persist;
+    const BatchShape &S = b->shapes[shape_index];
+    *variant = S.fused >= 0 ? 3 : S.variant;
+    *persist = S.fused >= 0 ? 0 : S.persist;
     return 0;
 }
 
