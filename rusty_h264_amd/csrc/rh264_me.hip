@@ -2826,7 +2826,9 @@ struct FuLane {
     uint32_t bestA, bestB;             // 4x4 block; the lane's larger partition
     int rxA[4], rxB[4];                // 4 x column bits per candidate column (4 x 128: past 2R)
     int mpyA, mpyB;
-    uint32_t mh, mv, md;               // all-ones where the lane's larger partition adds the quadrant beside, below, diagonal
+    uint32_t mhs, mvs;                 // all-ones where the lane's mirror / row-rotation partner adds this lane's sum
+    int ryc;                           // without predictors: the y bits lrc was read for (wave-uniform)
+    uint32_t lrc[4];
     const uint8_t *row;                // next reference row (the aligned dword left of the lane's window)
 };
 
@@ -2838,24 +2840,34 @@ __device__ __forceinline__ void fu_price(FuLane &L, uint64_t a4, int t, int R, c
     uint32_t h8 = hi + (uint32_t)__builtin_amdgcn_mov_dpp((int)hi, 0xB1, 0xF, 0xF, true);
     l8 += (uint32_t)__builtin_amdgcn_mov_dpp((int)l8, 0x4E, 0xF, 0xF, true); // the lane's quadrant (8x8)
     h8 += (uint32_t)__builtin_amdgcn_mov_dpp((int)h8, 0x4E, 0xF, 0xF, true);
-    // the quadrant beside (half-row mirror), below / above (rotate by 8), diagonal
-    const uint32_t lh = (uint32_t)__builtin_amdgcn_mov_dpp((int)l8, 0x141, 0xF, 0xF, true);
-    const uint32_t hh = (uint32_t)__builtin_amdgcn_mov_dpp((int)h8, 0x141, 0xF, 0xF, true);
-    const uint32_t lv = (uint32_t)__builtin_amdgcn_mov_dpp((int)l8, 0x128, 0xF, 0xF, true);
-    const uint32_t hv = (uint32_t)__builtin_amdgcn_mov_dpp((int)h8, 0x128, 0xF, 0xF, true);
-    const uint32_t ld = (uint32_t)__builtin_amdgcn_mov_dpp((int)lh, 0x128, 0xF, 0xF, true);
-    const uint32_t hd = (uint32_t)__builtin_amdgcn_mov_dpp((int)hh, 0x128, 0xF, 0xF, true);
-    const uint32_t lB = l8 + (lh & L.mh) + (lv & L.mv) + (ld & L.md);
-    const uint32_t hB = h8 + (hh & L.mh) + (hv & L.mv) + (hd & L.md);
-    const int ryA = 4 * mvbits(4 * (t - R) - L.mpyA);
+    // Each lane sends its sum to its half-row mirror partner (the quadrant beside)
+    // only where that partner pairs horizontally, then the pair sums to the lane 8
+    // apart (the quadrant below / above) only where that one pairs vertically: sb 0
+    // gets all four quadrants, 2 / 10 the top / bottom pair, 3 / 7 the left / right
+    // pair, the others their own quadrant.
+    const uint32_t lp = l8 + (uint32_t)__builtin_amdgcn_mov_dpp((int)(l8 & L.mhs), 0x141, 0xF, 0xF, true);
+    const uint32_t hp = h8 + (uint32_t)__builtin_amdgcn_mov_dpp((int)(h8 & L.mhs), 0x141, 0xF, 0xF, true);
+    const uint32_t lB = lp + (uint32_t)__builtin_amdgcn_mov_dpp((int)(lp & L.mvs), 0x128, 0xF, 0xF, true);
+    const uint32_t hB = hp + (uint32_t)__builtin_amdgcn_mov_dpp((int)(hp & L.mvs), 0x128, 0xF, 0xF, true);
+    // without predictors |4 (t - R)| <= 512, so mvbits' clamp is not needed
+    const int dyA = 4 * (t - R) - L.mpyA;
+    const int ryA = MVP ? 4 * mvbits(dyA)
+                        : 4 * (1 + 2 * (31 - __clz((int)((dyA > 0 ? 2u * dyA - 1u : -2u * dyA) + 1u))));
     const int ryB = MVP ? 4 * mvbits(4 * (t - R) - L.mpyB) : ryA;
     const char *lr = (const char *)s_lr; // byte offsets: 4 x (bits(x) + bits(y))
+    if (!MVP && ryA != L.ryc) {
+        // no predictors: the y bits depend on t alone, so the wave re-reads its four
+        // rates only when they change
+#pragma unroll
+        for (int k = 0; k < 4; k++) L.lrc[k] = *(const uint32_t *)(lr + L.rxA[k] + ryA);
+        L.ryc = ryA;
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const uint32_t tk = ((uint32_t)t << 2) | (uint32_t)k;
         const uint32_t sA = ((k < 2 ? lo : hi) >> (16 * (k & 1))) & 0xffffu;
         const uint32_t sB = ((k < 2 ? lB : hB) >> (16 * (k & 1))) & 0xffffu;
-        const uint32_t lrA = *(const uint32_t *)(lr + L.rxA[k] + ryA);
+        const uint32_t lrA = MVP ? *(const uint32_t *)(lr + L.rxA[k] + ryA) : L.lrc[k];
         const uint32_t lrB = MVP ? *(const uint32_t *)(lr + L.rxB[k] + ryB) : lrA;
         L.bestA = min(L.bestA, ((sA + lrA) << 10) | tk);
         L.bestB = min(L.bestB, ((sB + lrB) << 10) | tk);
@@ -2900,9 +2912,12 @@ __device__ __forceinline__ void fu_rows(FuLane &L, const uint8_t *__restrict__ F
     }
 }
 
+#ifndef RH264_FU_WAVES
+#define RH264_FU_WAVES 8 // 8 waves per SIMD (64 VGPRs): 16.2 ms against 19.6 ms at the compiler's 101
+#endif
 template <bool MVP>
-__global__ __launch_bounds__(256) void k_fs_fused(const FuSeg *__restrict__ segs, int nseg, long total_units, int R,
-                                                  LrTab lr_fp, FuKeys keys)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH264_FU_WAVES)))
+void k_fs_fused(const FuSeg *__restrict__ segs, int nseg, long total_units, int R, LrTab lr_fp, FuKeys keys)
 {
     __shared__ int32_t s_lr[2 * RH264_LR_N];
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = lr_fp.v[threadIdx.x];
@@ -2942,9 +2957,16 @@ __global__ __launch_bounds__(256) void k_fs_fused(const FuSeg *__restrict__ segs
     }
     const int bA = (4 * mby + y4) * (4 * mbw) + 4 * mbx + x4;
     FuLane L;
-    L.mh = (shB == 0 || shB == 1) ? ~0u : 0u;
-    L.mv = (shB == 0 || shB == 2) ? ~0u : 0u;
-    L.md = shB == 0 ? ~0u : 0u;
+    {
+        // partners: half-row mirror (sb & 8) | (7 - (sb & 7)), row rotation sb ^ 8;
+        // horizontal pairs: sb 0, 2, 10 and 8 (which carries the bottom pair to sb 0);
+        // vertical pairs: sb 0, 3, 7
+        const int mir = (sb & 8) | (7 - (sb & 7)), rot = sb ^ 8;
+        L.mhs = (mir == 0 || mir == 2 || mir == 8 || mir == 10) ? ~0u : 0u;
+        L.mvs = (rot == 0 || rot == 3 || rot == 7) ? ~0u : 0u;
+    }
+    L.ryc = -1;
+    for (int k = 0; k < 4; k++) L.lrc[k] = 0;
     int mpxA = 0, mpxB = 0;
     L.mpyA = 0;
     L.mpyB = 0;
