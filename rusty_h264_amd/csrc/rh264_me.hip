@@ -1792,7 +1792,10 @@ __device__ __forceinline__ int ms_div_euclid4(int v) { return v >= 0 ? v / 4 : -
 // lambda_fp under B2 else lambda (mb16.rs:2680-2683, 2834).  oev: bit 0 the rescue
 // grid ran, bit 1 it cut the cost to <= 15/16 (mb16.rs:2878-2880).
 template <bool T8>
-__device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh264_diamond &dc,
+// rescue: this search's me_wide switch (dc.rescue, or the decision's per-search
+// switch under the frame's learning window); passed apart from dc so that dc stays
+// the read-only kernel argument.
+__device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh264_diamond &dc, int rescue,
                                           const int32_t *s_lr, const int32_t *s_lrfp, const uint8_t *frame_px,
                                           const uint8_t *pl, int lx, int ly, int bw, int bh, const int32_t *pr,
                                           int npreds, int lane, int &obx, int &oby, int &obc, int &oev)
@@ -1912,7 +1915,7 @@ __device__ __forceinline__ void ms_search(const rh264_search_params &p, const rh
         seed_c = cost(seedx, seedy);
     }
     // me_wide rescue
-    bool gate = dc.rescue && !dc.sad_only;
+    bool gate = rescue && !dc.sad_only;
     if (gate && !dc.rescue_fast) { // the flat test: the block's variance below rescue_var
         const int v0 = act ? (int)(src & 255) : 0, v1 = act ? (int)((src >> 8) & 255) : 0;
         const int v2 = act ? (int)((src >> 16) & 255) : 0, v3 = act ? (int)(src >> 24) : 0;
@@ -2023,7 +2026,7 @@ __global__ __launch_bounds__(256) void k_motion_search(rh264_search_params p, rh
     const int frame = (int)wgy;
     const size_t boff = (size_t)frame * nblk + b;
     int bx, by, bc, ev;
-    ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride, planes + (size_t)frame * planes_stride,
+    ms_search<T8>(p, dc, dc.rescue, s_lr, s_lrfp, cur + (size_t)frame * cur_stride, planes + (size_t)frame * planes_stride,
                   (b % nbx) * p.bw, (b / nbx) * p.bh, p.bw, p.bh, preds + boff * (size_t)(npreds > 0 ? npreds : 1) * 2,
                   npreds, lane, bx, by, bc, ev);
     if (lane == 0) {
@@ -2110,7 +2113,7 @@ __global__ __launch_bounds__(256) void k_ms_wavefront(rh264_search_params p, rh2
         pr[0] = __builtin_amdgcn_readfirstlane(px);
         pr[1] = __builtin_amdgcn_readfirstlane(py);
         int bx, by, bc, ev;
-        ms_search<T8>(p, dc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
+        ms_search<T8>(p, dc, dc.rescue, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
                       planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, x << 4, y << 4, 16, 16, pr, 1, lane,
                       bx, by, bc, ev);
         bc += refc.c[rf];
@@ -2226,9 +2229,8 @@ __global__ __launch_bounds__(256) void k_ms_listed(rh264_search_params p, rh264_
         const int frame = i / nblk, b = i - frame * nblk;
         if (ok && rf < nrefs) {
             int bx, by, bc, ev;
-            rh264_diamond d = dc;
-            d.rescue = usedf ? __builtin_amdgcn_readfirstlane(usedf[(size_t)i * 4 + rf]) : 0;
-            ms_search<T8>(p, d, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
+            const int resc = usedf ? __builtin_amdgcn_readfirstlane(usedf[(size_t)i * 4 + rf]) : 0;
+            ms_search<T8>(p, dc, resc, s_lr, s_lrfp, cur + (size_t)frame * cur_stride,
                           planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride, (b % nbx) << 4, (b / nbx) << 4,
                           16, 16, used + (size_t)i * 8 + 2 * rf, 1, lane, bx, by, bc, ev);
             if (events && lane == 0) events[(size_t)i * 4 + rf] = ev;
@@ -2400,14 +2402,14 @@ struct PdArgs {
     int32_t nrefs, gate, p8x8, lme4;
     int32_t refc[4];
 };
+// 6 waves per SIMD: 80 VGPRs without spills (the compiler's own 84 allow 5)
 template <bool T8>
-__global__ __launch_bounds__(256) void k_pd_listed(rh264_search_params p, rh264_diamond dc,
-                                                   const uint8_t *__restrict__ cur, size_t cur_stride,
-                                                   const uint8_t *__restrict__ planes, size_t planes_stride, PdArgs a,
-                                                   const int32_t *__restrict__ used, const int32_t *__restrict__ list,
-                                                   const uint32_t *__restrict__ count, uint32_t *__restrict__ next_count,
-                                                   int32_t *__restrict__ rec, const int32_t *__restrict__ usedf,
-                                                   int32_t *__restrict__ events)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+void k_pd_listed(rh264_search_params p, rh264_diamond dc, const uint8_t *__restrict__ cur, size_t cur_stride,
+                 const uint8_t *__restrict__ planes, size_t planes_stride, PdArgs a, const int32_t *__restrict__ used,
+                 const int32_t *__restrict__ list, const uint32_t *__restrict__ count,
+                 uint32_t *__restrict__ next_count, int32_t *__restrict__ rec, const int32_t *__restrict__ usedf,
+                 int32_t *__restrict__ events)
 {
     __shared__ int32_t s_lr[RH264_LR_N], s_lrfp[RH264_LR_N];
     __shared__ int32_t s16[4][3], sp[8][4][3];
@@ -2432,9 +2434,8 @@ __global__ __launch_bounds__(256) void k_pd_listed(rh264_search_params p, rh264_
         const int32_t *u = used + (size_t)i * 8;
         if (wave < nrefs) {
             int bx, by, bc, ev;
-            rh264_diamond d = dc;
-            d.rescue = usedf ? __builtin_amdgcn_readfirstlane(usedf[(size_t)i * PD_SLOTS + wave]) : 0;
-            ms_search<T8>(p, d, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - wave) * planes_stride, lx, ly,
+            const int resc = usedf ? __builtin_amdgcn_readfirstlane(usedf[(size_t)i * PD_SLOTS + wave]) : 0;
+            ms_search<T8>(p, dc, resc, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - wave) * planes_stride, lx, ly,
                           16, 16, u + 2 * wave, 1, lane, bx, by, bc, ev);
             if (events && lane == 0) events[(size_t)i * PD_SLOTS + wave] = ev;
             if (lane == 0) {
@@ -2456,9 +2457,8 @@ __global__ __launch_bounds__(256) void k_pd_listed(rh264_search_params p, rh264_
                 // seeds: the predictor against rf (pricing the rate), then the 16x16 vector
                 const int32_t prl[4] = {u[2 * rf], u[2 * rf + 1], mvx16, mvy16};
                 int bx, by, bc, ev;
-                rh264_diamond d = dc;
-                d.rescue = usedf ? __builtin_amdgcn_readfirstlane(usedf[(size_t)i * PD_SLOTS + nrefs + j]) : 0;
-                ms_search<T8>(p, d, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride,
+                const int resc = usedf ? __builtin_amdgcn_readfirstlane(usedf[(size_t)i * PD_SLOTS + nrefs + j]) : 0;
+                ms_search<T8>(p, dc, resc, s_lr, s_lrfp, fpx, planes + (size_t)(frame + nrefs - 1 - rf) * planes_stride,
                               lx + geo[part][0], ly + geo[part][1], geo[part][2], geo[part][3], prl, 2, lane, bx, by,
                               bc, ev);
                 if (events && lane == 0) events[(size_t)i * PD_SLOTS + nrefs + j] = ev;
