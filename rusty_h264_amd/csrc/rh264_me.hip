@@ -854,6 +854,9 @@ __device__ __forceinline__ uint32_t lds_load4(const uint8_t *rowp, int x)
 // does the whole 4x4 Hadamard in registers; a block's NSB = (BW/4)*(BH/4) lanes
 // then add their sums with DPP (no cross-lane butterflies).  Blocks of at least
 // 8 sub-blocks can stage their +-3 pel neighbourhood in LDS (USE_LDS).
+#ifndef RH264_LDS_MIN_NSB
+#define RH264_LDS_MIN_NSB 8 // the batch's refinements stage windows in LDS from this many lanes per block
+#endif
 template <int BW, int BH> struct RefShape {
     static constexpr int NSB = (BW / 4) * (BH / 4); // lanes per block
     static constexpr int GPW = 256 / NSB;           // blocks per 256-thread workgroup
@@ -5873,8 +5876,8 @@ static int batch_shape_run(rh264_ctx *ctx, rh264_batch *b, const BatchShape &S, 
     }
     if (mode == 1) return 0;
     constexpr int GPW = RefShape<BW, BH>::GPW;
-    if (RefShape<BW, BH>::NSB >= 8 && ctx->refine_lds)
-        hipLaunchKernelGGL((k_refine_multi<BW, BH, RefShape<BW, BH>::NSB >= 8>),
+    if (RefShape<BW, BH>::NSB >= RH264_LDS_MIN_NSB && ctx->refine_lds)
+        hipLaunchKernelGGL((k_refine_multi<BW, BH, RefShape<BW, BH>::NSB >= RH264_LDS_MIN_NSB>),
                            dim3((unsigned)((S.blocks + GPW - 1) / GPW)), dim3(256), 0, ctx->stream, S.d_segs, S.nseg,
                            S.blocks, b->range, b->pad, b->pat, b->lr, keys, b->res);
     else
