@@ -3058,7 +3058,9 @@ __global__ __launch_bounds__(256) void k_refine_multi(const SegDesc *__restrict_
     __shared__ __align__(16) uint8_t s_win[USE_LDS ? GPW * W::BYTES : 16];
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = lr.v[threadIdx.x];
     const int grp = threadIdx.x / NSB, li = threadIdx.x % NSB;
-    const long gb0 = (long)blockIdx.x * GPW + grp;
+    // workgroups renumbered per XCD: consecutive block strips, which read the same
+    // plane rows, share one L2
+    const long gb0 = (long)xcd_linear(blockIdx.x, gridDim.x) * GPW + grp;
     const bool active = gb0 < total_blocks;
     const long gb = active ? gb0 : total_blocks - 1;
     const int si = seg_find_blk(segs, nseg, gb);
