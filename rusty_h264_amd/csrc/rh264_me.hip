@@ -1052,9 +1052,12 @@ __global__ __launch_bounds__(256) void k_subpel_refine(rh264_search_params p,
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = p.lr[threadIdx.x];
 
     const int nbx = p.cw / BW, nby = p.ch / BH, nblk = nbx * nby;
-    const int frame = blockIdx.y;
+    // workgroups renumbered per XCD: consecutive block strips share one L2
+    unsigned wgx, wgy;
+    xcd_grid2(wgx, wgy);
+    const int frame = (int)wgy;
     const int grp = threadIdx.x / NSB, li = threadIdx.x % NSB;
-    const int b0 = blockIdx.x * GPW + grp;
+    const int b0 = (int)wgx * GPW + grp;
     const bool active = b0 < nblk;
     const int b = active ? b0 : nblk - 1;
     const size_t boff = (size_t)frame * nblk + b;
@@ -2749,7 +2752,7 @@ void k_fs_multi(const SegDesc *__restrict__ segs, int nseg, long total_lanes, in
         } else {
             __syncthreads();
             if (it > 0) break;
-            tile = blockIdx.x;
+            tile = xcd_linear(blockIdx.x, gridDim.x); // consecutive tiles share one L2
         }
         if ((long)tile * 256 >= total_lanes) break;
         const long gi = (long)tile * 256 + threadIdx.x;
