@@ -3061,9 +3061,10 @@ __global__ __launch_bounds__(256) void k_refine_multi(const SegDesc *__restrict_
     __shared__ __align__(16) uint8_t s_win[USE_LDS ? GPW * W::BYTES : 16];
     if (threadIdx.x < RH264_LR_N) s_lr[threadIdx.x] = lr.v[threadIdx.x];
     const int grp = threadIdx.x / NSB, li = threadIdx.x % NSB;
-    // workgroups renumbered per XCD: consecutive block strips, which read the same
-    // plane rows, share one L2
-    const long gb0 = (long)xcd_linear(blockIdx.x, gridDim.x) * GPW + grp;
+    // without LDS windows, workgroups are renumbered per XCD: consecutive block strips,
+    // which read the same plane rows, share one L2 (4x4: 44.5 -> 0.9 GB fetched past L2
+    // per launch); the staged forms run 3-4 % faster in dispatch order
+    const long gb0 = (long)(USE_LDS ? blockIdx.x : xcd_linear(blockIdx.x, gridDim.x)) * GPW + grp;
     const bool active = gb0 < total_blocks;
     const long gb = active ? gb0 : total_blocks - 1;
     const int si = seg_find_blk(segs, nseg, gb);
